@@ -1,0 +1,284 @@
+/*
+ * pict_hip.h -- C ABI of libpict_hip, the MI355X (gfx950) implementation of
+ * the PICT PISO time step.
+ *
+ * Every entry point takes plain pointers, sizes and POD descriptors; no torch
+ * or HIP types appear in a signature (streams are passed as `void*`, a
+ * hipStream_t, NULL = default stream).  All entry points return a
+ * pict_status; on failure pict_last_error() holds a message for the calling
+ * thread.
+ *
+ * Each entry point names the reference interface it replaces
+ * (tum-pbs/PICT, paths relative to the repository root).
+ */
+#ifndef PICT_HIP_H
+#define PICT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* status, enums                                                             */
+/* ------------------------------------------------------------------------ */
+
+typedef int32_t pict_status;
+#define PICT_OK               0
+#define PICT_ERR_INVALID      1  /* invalid argument or descriptor          */
+#define PICT_ERR_HIP          2  /* HIP runtime error                       */
+#define PICT_ERR_UNSUPPORTED  3  /* option not implemented by this library  */
+#define PICT_ERR_DEVICE_SYNC  4  /* device-side barrier spin limit reached  */
+
+#define PICT_DTYPE_F32 0
+#define PICT_DTYPE_F64 1
+
+/* Boundary types: values of BoundaryType in extensions/domain_structs_gpu.h:98-114 */
+#define PICT_BOUNDARY_CONNECTED 20
+#define PICT_BOUNDARY_PERIODIC  21
+#define PICT_BOUNDARY_FIXED     30
+
+/* BoundaryConditionType, extensions/domain_structs_gpu.h:140-148 */
+#define PICT_BC_DIRICHLET 0
+#define PICT_BC_NEUMANN   1
+
+/* non-orthogonal flags, extensions/PISO_multiblock_cuda.h:9-13 */
+#define PICT_NON_ORTHO_DIRECT_MATRIX   1
+#define PICT_NON_ORTHO_DIRECT_RHS      2
+#define PICT_NON_ORTHO_DIAGONAL_MATRIX 4
+#define PICT_NON_ORTHO_DIAGONAL_RHS    8
+#define PICT_NON_ORTHO_CENTER_MATRIX  16
+
+/* ConvergenceCriterion, extensions/bicgstab_solver.h:11-18 */
+#define PICT_CONV_NORM2            0
+#define PICT_CONV_NORM2_NORMALIZED 1
+#define PICT_CONV_ABS_SUM          2
+#define PICT_CONV_ABS_MEAN         3
+#define PICT_CONV_ABS_MAX          4
+
+/* ------------------------------------------------------------------------ */
+/* domain descriptor                                                         */
+/* ------------------------------------------------------------------------ */
+/*
+ * Layouts (identical to the reference, extensions/domain_structs.cpp):
+ *  - block fields NCDHW [1, C, (Z), Y, X]; cell flat index x + y*X + z*X*Y
+ *  - global vectors component-major: v[d*N + globalOffset + flat]
+ *  - transform [1, (Z), Y, X, 2*D*D+1] = {M row-major, Minv row-major, det}
+ *  - face transform [1, D, (Z+1), Y+1, X+1, 2*D*D+1]
+ *  - CSR: row int32[N+1], index int32[nnz], value real[nnz]; rows sorted by
+ *    ascending global column.
+ * `real` is float (dtype F32) or double (dtype F64).  Pointers are device
+ * pointers for the HIP library.
+ */
+
+typedef struct pict_boundary_desc {
+    int32_t type;             /* PICT_BOUNDARY_*                                   */
+    int32_t connected_block;  /* CONNECTED: index of the other block               */
+    int32_t axes[4];          /* CONNECTED: axes[0] = face of the other block,
+                                 axes[1..2] = (axis<<1)|inverted of the other block
+                                 for axes (face/2+1)%D and (face/2+2)%D              */
+    int32_t velocity_static;  /* FIXED: velocity is [1,D] (else [1,D,(z),y,x])    */
+    int32_t velocity_bc;      /* FIXED: PICT_BC_*                                  */
+    int32_t has_transform;    /* FIXED: transform slice present                    */
+    int32_t reserved;
+    int32_t size[4];          /* FIXED: x,y,z (face axis 1), w = D                 */
+    int32_t stride[4];        /* FIXED: 1, x, x*y, x*y*z                           */
+    void   *velocity;         /* FIXED */
+    void   *velocity_grad;    /* FIXED, may be NULL */
+    void   *transform;        /* FIXED, [1,(z),y,x,2D^2+1] or NULL */
+} pict_boundary_desc;
+
+typedef struct pict_block_desc {
+    int32_t global_offset;    /* first global cell index of the block              */
+    int32_t csr_offset;       /* first CSR entry of the block                      */
+    int32_t size[4];          /* x,y,z, w = D                                      */
+    int32_t stride[4];        /* 1, x, x*y, x*y*z                                  */
+    int32_t viscosity_static; /* viscosity tensor is [1] (else [1,1,(z),y,x])      */
+    int32_t velocity_source_static; /* velocity source is [1,D] (else NCDHW)       */
+    int32_t has_transform;
+    int32_t has_face_transform;
+    void *velocity;           /* [1,D,(z),y,x]                                     */
+    void *pressure;           /* [1,1,(z),y,x]                                     */
+    void *viscosity;          /* NULL -> domain viscosity                          */
+    void *velocity_source;    /* NULL -> none                                      */
+    void *transform;          /* NULL if !has_transform                            */
+    void *face_transform;     /* NULL if !has_face_transform                       */
+    void *velocity_grad;
+    void *pressure_grad;
+    void *viscosity_grad;
+    void *velocity_source_grad;
+    pict_boundary_desc bounds[6]; /* -x,+x,-y,+y,-z,+z                             */
+} pict_block_desc;
+
+typedef struct pict_domain_desc {
+    int32_t num_dims;
+    int32_t num_blocks;
+    int32_t num_cells;        /* N                                                 */
+    int32_t nnz;              /* CSR entries of C and P                            */
+    int32_t dtype;            /* PICT_DTYPE_*                                      */
+    int32_t reserved;
+    double  viscosity;        /* domain viscosity (a CPU scalar in the reference)  */
+    const pict_block_desc *blocks; /* host array of num_blocks descriptors         */
+    int32_t *csr_row;         /* pattern shared by C, P and their gradients        */
+    int32_t *csr_index;
+    void *C_value;
+    void *P_value;
+    void *A;                  /* [N] diagonal of C                                 */
+    void *velocity_rhs;       /* [D*N] */
+    void *velocity_result;    /* [D*N] */
+    void *pressure_rhs;       /* [D*N] */
+    void *pressure_rhs_div;   /* [N]   */
+    void *pressure_result;    /* [N]   */
+    void *viscosity_grad;     /* [1] or NULL */
+    void *C_grad;             /* [nnz] or NULL */
+    void *P_grad;
+    void *A_grad;
+    void *velocity_rhs_grad;
+    void *velocity_result_grad;
+    void *pressure_rhs_grad;
+    void *pressure_rhs_div_grad;
+    void *pressure_result_grad;
+} pict_domain_desc;
+
+/* Device-resident copy of a domain descriptor plus its launch tables.
+   Replaces the reference's DomainAtlasSet / CopyDomainToGPU
+   (extensions/domain_structs_gpu.h:376-384, PISO_multiblock_cuda_kernel.cu:54-57)
+   and its per-call CopyBlockIndices (PISO_multiblock_cuda_kernel.cu:86-93). */
+typedef struct pict_atlas pict_atlas;
+
+/* ------------------------------------------------------------------------ */
+/* library / atlas                                                           */
+/* ------------------------------------------------------------------------ */
+
+const char *pict_last_error(void);
+int32_t     pict_abi_version(void);
+
+/* Domain::PrepareSolve -> SetupDomainGPU (extensions/domain_structs.cpp:2554-2676, :2992-3029) */
+pict_status pict_atlas_create(const pict_domain_desc *desc, pict_atlas **out, void *stream);
+/* Domain::UpdateDomainData -> UpdateDomainGPU (extensions/domain_structs.cpp:3031-3266).
+   The block table is copied on `stream` (stream ordered: kernels queued
+   earlier on the same stream still see the previous table).  The structure
+   (block count and sizes) must be unchanged since pict_atlas_create. */
+pict_status pict_atlas_update(pict_atlas *atlas, const pict_domain_desc *desc, void *stream);
+pict_status pict_atlas_destroy(pict_atlas *atlas);
+
+/* Writes the CSR pattern (row, index) of C/P into desc->csr_row/csr_index.
+   The reference writes it inside PISO_build_matrix / PISO_build_pressure_matrix
+   (extensions/PISO_multiblock_cuda_kernel.cu:3634-3637, :3846-3860); the pattern
+   is step invariant, so it is written once per structure here. */
+pict_status pict_build_csr_pattern(const pict_atlas *atlas, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* forward stages                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* SetupAdvectionMatrix (extensions/PISOtorch.cpp:503-505; kernel
+   PISO_multiblock_cuda_kernel.cu:3601-3865): writes C values and A. */
+pict_status pict_setup_advection_matrix(const pict_atlas *atlas, double time_step,
+                                        int32_t non_ortho_flags, void *stream);
+/* SetupAdvectionVelocity (PISOtorch.cpp:509-511; kernel :4281-4385): velocity_rhs. */
+pict_status pict_setup_advection_velocity(const pict_atlas *atlas, double time_step,
+                                          int32_t non_ortho_flags,
+                                          int32_t apply_pressure_gradient, void *stream);
+/* SetupPressureMatrix (PISOtorch.cpp:516-518; kernel :4797-4963): P values. */
+pict_status pict_setup_pressure_matrix(const pict_atlas *atlas, double time_step,
+                                       int32_t non_ortho_flags, int32_t use_face_transform,
+                                       void *stream);
+/* SetupPressureRHS (PISOtorch.cpp:519-521; kernels :5121-5240 and :5374-5419):
+   pressure_rhs and pressure_rhs_div. */
+pict_status pict_setup_pressure_rhs(const pict_atlas *atlas, double time_step,
+                                    int32_t non_ortho_flags, int32_t use_face_transform,
+                                    int32_t time_step_norm, void *stream);
+/* SetupPressureRHSdiv (PISOtorch.cpp:522-524; kernel :5374-5419): pressure_rhs_div. */
+pict_status pict_setup_pressure_rhs_div(const pict_atlas *atlas, double time_step,
+                                        int32_t non_ortho_flags, int32_t use_face_transform,
+                                        int32_t time_step_norm, void *stream);
+/* SetupPressureCorrection (PISOtorch.cpp:512-514; host :5498-5563):
+   pressure matrix + pressure RHS + divergence. */
+pict_status pict_setup_pressure_correction(const pict_atlas *atlas, double time_step,
+                                           int32_t non_ortho_flags, int32_t use_face_transform,
+                                           int32_t time_step_norm, void *stream);
+/* CorrectVelocity version 1 (PISOtorch.cpp:525-527; kernel :5947-5980). */
+pict_status pict_correct_velocity(const pict_atlas *atlas, double time_step, int32_t version,
+                                  int32_t time_step_norm, void *stream);
+/* ComputeVelocityDivergence (PISOtorch.cpp:528; kernel :6333-6354): out [N]. */
+pict_status pict_compute_velocity_divergence(const pict_atlas *atlas, void *out, void *stream);
+
+/* Copy{Velocity,Pressure}Result{To,From}Blocks (PISOtorch.cpp:533-538; :6541-6733). */
+pict_status pict_copy_velocity_result_to_blocks(const pict_atlas *atlas, void *stream);
+pict_status pict_copy_velocity_result_from_blocks(const pict_atlas *atlas, void *stream);
+pict_status pict_copy_pressure_result_to_blocks(const pict_atlas *atlas, void *stream);
+pict_status pict_copy_pressure_result_from_blocks(const pict_atlas *atlas, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* linear solvers                                                            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct pict_csr_desc {
+    int32_t n;                /* rows (= columns)                                  */
+    int32_t nnz;
+    int32_t dtype;            /* PICT_DTYPE_*                                      */
+    int32_t reserved;
+    const int32_t *row;       /* [n+1] */
+    const int32_t *index;     /* [nnz] */
+    const void    *value;     /* [nnz] */
+} pict_csr_desc;
+
+typedef struct pict_solve_opts {
+    int32_t max_iterations;
+    int32_t convergence_criterion;   /* PICT_CONV_* */
+    int32_t use_bicg;
+    int32_t matrix_rank_deficient;   /* must be 0 */
+    int32_t residual_reset_steps;    /* CG: recompute r = b - A x every k its; 0 = off */
+    int32_t transpose_a;             /* solve A^T x = b */
+    int32_t print_residual;
+    int32_t return_best_result;      /* CG only */
+    int32_t bicg_with_preconditioner;/* ILU(0): not implemented -> UNSUPPORTED */
+    int32_t zero_rhs_shortcut;       /* if every right-hand side is exactly 0: x = 0,
+                                        no iterations (PISOtorch_diff.py:251, :292-293) */
+    double  tolerance;
+} pict_solve_opts;
+
+typedef struct pict_solve_info {     /* LinearSolverResultInfo, bicgstab_solver.h:20-25 */
+    double  final_residual;
+    int32_t used_iterations;
+    int32_t converged;
+    int32_t is_finite_residual;
+    int32_t reserved;
+} pict_solve_info;
+
+/* SolveLinear (PISOtorch.cpp:559-563; PISO_multiblock_cuda_kernel.cu:7021-7103,
+   cg_solver_kernel.cu:114-456, bicgstab_solver_kernel.cu:47-395).
+   rhs and x hold nrhs consecutive vectors of length n; x is the initial guess
+   and receives the solution.  infos_out receives nrhs entries (host memory).
+   Every right-hand side keeps its own iteration state and stopping test, as
+   when the reference solves them one after the other. */
+pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, int32_t nrhs,
+                              const pict_solve_opts *opts, pict_solve_info *infos_out,
+                              void *stream);
+
+/* SparseOuterProduct (PISOtorch.cpp:564; cg_solver_kernel.cu:869-908):
+   out_value[k] = a[i] * b[j] for every pattern entry k = (i, j). */
+pict_status pict_sparse_outer_product(const void *a, const void *b, const pict_csr_desc *pattern,
+                                      void *out_value, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* metrics                                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* CoordsToTransforms (grid_gen.cu:284-381): coords [1,D,(Z+1),Y+1,X+1] ->
+   out [1,(Z),Y,X,2D^2+1].  vertex_size = {X+1, Y+1, Z+1} (unused axes 1). */
+pict_status pict_coords_to_transforms(const void *coords, int32_t num_dims,
+                                      const int32_t vertex_size[3], int32_t dtype,
+                                      void *out, void *stream);
+/* CoordsToFaceTransforms (grid_gen.cu:383-514): out [1,D,(Z+1),Y+1,X+1,2D^2+1]. */
+pict_status pict_coords_to_face_transforms(const void *coords, int32_t num_dims,
+                                           const int32_t vertex_size[3], int32_t dtype,
+                                           void *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PICT_HIP_H */

@@ -1,0 +1,296 @@
+"""Python harness of the CPU oracle (pict_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+
+The oracle reads the same pict_domain_desc as libpict_hip.  A CPU "twin" of a
+PISOtorch.Domain (float64, host tensors) is packed with the product's own
+descriptor packer and handed to the C restatement of the reference stages.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+
+_libs = {}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(bits=64):
+    if bits not in _libs:
+        path = os.path.join(BUILD, "liboracle_f%d.so" % bits)
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.oracle_last_error.restype = ctypes.c_char_p
+        for n in ("oracle_setup_advection_matrix",):
+            getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.oracle_setup_advection_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        L.oracle_setup_pressure_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_setup_pressure_rhs_only.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.oracle_setup_pressure_rhs_div.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.oracle_correct_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.oracle_compute_velocity_divergence.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_copy_velocity_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_copy_pressure_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_build_csr_pattern.argtypes = [ctypes.c_void_p]
+        L.oracle_csr_row_end_closed_form.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.POINTER(ctypes.c_int)]
+        L.oracle_solve_linear.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_sparse_outer_product.argtypes = [ctypes.c_void_p] * 4
+        L.oracle_coords_to_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_coords_to_face_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                       ctypes.c_void_p]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        _libs[bits] = L
+    return _libs[bits]
+
+
+def _bits(domain):
+    return 64 if domain.getDtype() == torch.float64 else 32
+
+
+def _chk(L, st, what):
+    if st != 0:
+        raise RuntimeError("oracle %s: %s" % (what, L.oracle_last_error().decode()))
+
+
+def _desc(domain):
+    d, arr = domain._pack()
+    domain._oracle_keep = (d, arr)
+    return ctypes.byref(d)
+
+
+# ---------------------------------------------------------------------------
+# CPU twin of a domain
+# ---------------------------------------------------------------------------
+
+def cpu_twin(domain, dtype=torch.float64):
+    """A host-memory copy of `domain` (same structure) in `dtype`, prepared and with the CSR pattern built by the
+    oracle.  Transforms are copied from the source domain."""
+    from pict_amd import PISOtorch as P
+
+    def cv(t):
+        return None if t is None else t.detach().to(device="cpu", dtype=dtype).contiguous()
+
+    d = P.Domain(domain.getSpatialDims(), domain.viscosity.to(dtype), domain.name + "_cpu", dtype, "cpu",
+                 passiveScalarChannels=0, _allow_cpu=True)
+    mapping = {}
+    for b in domain.blocks:
+        nb = P.Block(d, b.name, velocity=cv(b.velocity), pressure=cv(b.pressure))
+        if b.hasTransform():
+            nb.setTransform(cv(b.transform), cv(b.faceTransform))
+        if b.viscosity is not None:
+            nb.setViscosity(cv(b.viscosity))
+        if b.velocitySource is not None:
+            nb.setVelocitySource(cv(b.velocitySource))
+        d.AddBlock(nb)
+        mapping[id(b)] = nb
+    for b in domain.blocks:
+        nb = mapping[id(b)]
+        for f, bd in enumerate(b.boundaries):
+            if bd.type == P.BoundaryType.FIXED:
+                fb = P.FixedBoundary(cv(bd.velocity), bd.velocityType, None, None, cv(bd.transform), d)
+                fb._setSize(list(bd.getSizes()))
+                nb.boundaries[f] = fb
+            elif bd.type == P.BoundaryType.CONNECTED:
+                nb.boundaries[f] = P.ConnectedBoundary(mapping[id(bd.getConnectedBlock())], list(bd.axes), d)
+            else:
+                nb.boundaries[f] = P.PeriodicBoundary(d)
+    d.PrepareSolve()
+    build_csr_pattern(d)
+    return d
+
+
+# ---------------------------------------------------------------------------
+# stages (same names and arguments as PISOtorch)
+# ---------------------------------------------------------------------------
+
+def build_csr_pattern(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_build_csr_pattern(_desc(domain)), "pattern")
+
+
+def csr_row_end_closed_form(domain, block, flat):
+    L = lib(_bits(domain))
+    rs = ctypes.c_int()
+    re = L.oracle_csr_row_end_closed_form(_desc(domain), block, flat, ctypes.byref(rs))
+    return re, rs.value
+
+
+def _dt(t):
+    return float(t[0]) if isinstance(t, torch.Tensor) else float(t)
+
+
+def SetupAdvectionMatrix(domain, timeStep, nonOrthoFlags=0):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_advection_matrix(_desc(domain), _dt(timeStep), int(nonOrthoFlags)), "advection matrix")
+
+
+def SetupAdvectionVelocity(domain, timeStep, nonOrthoFlags=0, applyPressureGradient=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_advection_velocity(_desc(domain), _dt(timeStep), int(nonOrthoFlags),
+                                              int(applyPressureGradient)), "advection rhs")
+
+
+def SetupPressureMatrix(domain, timeStep, nonOrthoMode=0, useFaceTransform=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_pressure_matrix(_desc(domain), int(nonOrthoMode)), "pressure matrix")
+
+
+def SetupPressureRHSdiv(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_pressure_rhs_div(_desc(domain), _dt(timeStep), int(timeStepNorm)), "pressure div")
+
+
+def SetupPressureRHS(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_pressure_rhs_only(_desc(domain), _dt(timeStep)), "pressure rhs")
+    SetupPressureRHSdiv(domain, timeStep, nonOrthoMode, useFaceTransform, timeStepNorm)
+
+
+def SetupPressureCorrection(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    SetupPressureMatrix(domain, timeStep, nonOrthoMode)
+    SetupPressureRHS(domain, timeStep, nonOrthoMode, useFaceTransform, timeStepNorm)
+
+
+def CorrectVelocity(domain, timeStep, version=0, timeStepNorm=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_correct_velocity(_desc(domain), _dt(timeStep), int(timeStepNorm)), "correct velocity")
+
+
+def ComputeVelocityDivergence(domain):
+    L = lib(_bits(domain))
+    out = torch.zeros(domain.getTotalSize(), dtype=domain.getDtype())
+    _chk(L, L.oracle_compute_velocity_divergence(_desc(domain), ctypes.c_void_p(out.data_ptr())), "divergence")
+    return out
+
+
+def CopyVelocityResultToBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_velocity_result(_desc(domain), 1), "copy")
+
+
+def CopyVelocityResultFromBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_velocity_result(_desc(domain), 0), "copy")
+
+
+def CopyPressureResultToBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_pressure_result(_desc(domain), 1), "copy")
+
+
+def CopyPressureResultFromBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_pressure_result(_desc(domain), 0), "copy")
+
+
+class SolveInfo:
+    def __init__(self, s):
+        self.finalResidual = s.final_residual
+        self.usedIterations = s.used_iterations
+        self.converged = bool(s.converged)
+        self.isFiniteResidual = bool(s.is_finite_residual)
+
+    def __repr__(self):
+        return "SolveInfo(res=%g, it=%d, conv=%d, finite=%d)" % (self.finalResidual, self.usedIterations,
+                                                                  self.converged, self.isFiniteResidual)
+
+
+def SolveLinear(A, RHS, x, maxIterations=1000, tolerance=1e-8, convergenceCriterion=1, useBiCG=False,
+                residualResetSteps=0, transposeA=False, returnBestResult=False):
+    """CPU restatement of SolveLinear on host tensors (A: PISOtorch.CSRmatrix on CPU)."""
+    from pict_amd import _abi
+    bits = 64 if A.value.dtype == torch.float64 else 32
+    L = lib(bits)
+    desc = A._desc()
+    o = _abi.SolveOpts()
+    o.max_iterations = int(maxIterations)
+    o.convergence_criterion = int(convergenceCriterion)
+    o.use_bicg = int(useBiCG)
+    o.residual_reset_steps = int(residualResetSteps)
+    o.transpose_a = int(transposeA)
+    o.return_best_result = int(returnBestResult)
+    o.bicg_with_preconditioner = 0
+    o.tolerance = float(tolerance)
+    nb = RHS.size(0) // A.getRows()
+    infos = (_abi.SolveInfo * nb)()
+    _chk(L, L.oracle_solve_linear(ctypes.byref(desc), ctypes.c_void_p(RHS.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                  nb, ctypes.byref(o), infos), "solve")
+    return [SolveInfo(i) for i in infos]
+
+
+def SparseOuterProduct(a, b, pattern):
+    bits = 64 if a.dtype == torch.float64 else 32
+    L = lib(bits)
+    desc = pattern._desc()
+    _chk(L, L.oracle_sparse_outer_product(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+                                          ctypes.byref(desc), ctypes.c_void_p(pattern.value.data_ptr())), "outer")
+
+
+def CoordsToTransforms(coords):
+    dims = coords.dim() - 2
+    vs = [coords.size(dims + 1 - d) for d in range(dims)] + [1] * (3 - dims)
+    out = torch.zeros([1] + [vs[d] - 1 for d in reversed(range(dims))] + [2 * dims * dims + 1], dtype=coords.dtype)
+    L = lib(64 if coords.dtype == torch.float64 else 32)
+    c = coords.contiguous()
+    _chk(L, L.oracle_coords_to_transforms(ctypes.c_void_p(c.data_ptr()), dims, (ctypes.c_int32 * 3)(*vs),
+                                          ctypes.c_void_p(out.data_ptr())), "coords")
+    return out
+
+
+def CoordsToFaceTransforms(coords):
+    dims = coords.dim() - 2
+    vs = [coords.size(dims + 1 - d) for d in range(dims)] + [1] * (3 - dims)
+    out = torch.zeros([1, dims] + [vs[d] for d in reversed(range(dims))] + [2 * dims * dims + 1], dtype=coords.dtype)
+    L = lib(64 if coords.dtype == torch.float64 else 32)
+    c = coords.contiguous()
+    _chk(L, L.oracle_coords_to_face_transforms(ctypes.c_void_p(c.data_ptr()), dims, (ctypes.c_int32 * 3)(*vs),
+                                               ctypes.c_void_p(out.data_ptr())), "face coords")
+    return out
+
+
+def csr_to_scipy(A):
+    import scipy.sparse as sp
+    n = A.getRows()
+    return sp.csr_matrix((A.value.cpu().double().numpy(), A.index.cpu().numpy(), A.row.cpu().numpy()), shape=(n, n))
+
+
+# ---------------------------------------------------------------------------
+# one PISO step (PISOtorch_simulation.py:1054-1327, orthogonal branch)
+# ---------------------------------------------------------------------------
+
+def piso_step(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, max_iter=5000,
+              normalize_pressure=True, infos=None):
+    """One PISO step of the orthogonal driver branch on a CPU twin, with the oracle's solvers."""
+    dt = torch.tensor([time_step], dtype=domain.getDtype())
+    SetupAdvectionMatrix(domain, dt, 0)
+    SetupAdvectionVelocity(domain, dt, 0, False)
+    x = torch.zeros_like(domain.velocityRHS)
+    inf = SolveLinear(domain.C, domain.velocityRHS, x, max_iter, advection_tol, 1, useBiCG=True)
+    if infos is not None:
+        infos.append(("adv", inf))
+    domain.setVelocityResult(x)
+    for _ in range(corrector_steps):
+        SetupPressureCorrection(domain, dt, 0)
+        p = torch.zeros_like(domain.pressureRHSdiv)
+        inf = SolveLinear(domain.P, domain.pressureRHSdiv, p, max_iter, pressure_tol, 1, useBiCG=False)
+        if infos is not None:
+            infos.append(("p", inf))
+        if normalize_pressure:
+            p = p - torch.mean(p)
+        domain.setPressureResult(p.contiguous())
+        CopyPressureResultToBlocks(domain)
+        CorrectVelocity(domain, dt, 1)
+    CopyVelocityResultToBlocks(domain)
+    domain._set_changed(False)

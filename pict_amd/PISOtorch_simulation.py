@@ -1,0 +1,523 @@
+"""PISOtorch_simulation -- the PISO step driver over PISOtorch / PISOtorch_diff.
+
+Mirrors `Simulation` of the reference (tum-pbs/PICT PISOtorch_simulation.py:388-1518):
+the same constructor arguments and defaults, the same call order in
+`_PISO_split_step` (:1054-1327), the same solver defaults (BiCGStab for
+advection, CG for pressure, tolerance None -> 1e-5 FP32 / 1e-8 FP64, maxit 5000,
+pressure mean subtraction) and the same `prep_fn` hook points.
+
+Image/VTK output, adaptive CFL stepping and the scipy solve path are host
+conveniences outside the PISO step and are not reproduced: the corresponding
+arguments are accepted and must stay at their defaults.
+"""
+import logging
+import numbers
+from contextlib import nullcontext
+
+import numpy as np
+import torch
+
+from . import PISOtorch
+from . import PISOtorch_diff
+
+_LOG = logging.getLogger("PISOsim")
+cpu_device = torch.device("cpu")
+
+
+def tensor_as_np(tensor):
+    return tensor.detach().cpu().numpy()
+
+
+def ntonp(t):
+    return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+
+
+class StopHandler:
+    """PISOtorch_simulation.py:27-49 (without signal registration side effects unless asked)."""
+
+    def __init__(self, log=None):
+        self._log = log
+        self.reset()
+
+    def stop(self):
+        self._stop = True
+
+    def __call__(self):
+        return self._stop
+
+    def reset(self):
+        self._stop = False
+
+
+class Simulation:
+    # must match PISO_multiblock_cuda.h:9-13
+    NON_ORTHO_DIRECT_MATRIX = 1
+    NON_ORTHO_DIRECT_RHS = 2
+    NON_ORTHO_DIAGONAL_MATRIX = 4
+    NON_ORTHO_DIAGONAL_RHS = 8
+    NON_ORTHO_CENTER_MATRIX = 16
+    _NON_ORTHO_MODE = NON_ORTHO_CENTER_MATRIX | NON_ORTHO_DIRECT_MATRIX | NON_ORTHO_DIAGONAL_RHS
+
+    def __init__(self, domain=None, *, time_step=1.0, substeps=1, corrector_steps=2, density_viscosity=None,
+                 adaptive_CFL=0.8, prep_fn=None, advection_use_BiCG=True, pressure_use_BiCG=False,
+                 scipy_solve_advection=False, scipy_solve_pressure=False, preconditionBiCG=False,
+                 BiCG_precondition_fallback=True, advection_tol=None, pressure_tol=None, convergence_tol=None,
+                 solver_double_fallback=False, advect_non_ortho_steps=1, pressure_non_ortho_steps=1,
+                 advect_reuse_result=False, pressure_reuse_result=False, normalize_pressure_result=True,
+                 pressure_return_best_result=False, advect_passive_scalar=True, pressure_time_step_normalized=False,
+                 velocity_corrector="FD", non_orthogonal=True, differentiable=False,
+                 exclude_advection_solve_gradients=False, exclude_pressure_solve_gradients=False, log_dir=None,
+                 log_interval=0, log_images=True, log_vtk=False, norm_vel=False, block_layout=None,
+                 output_mode3D="slice", log_fn=None, output_resampling_coords=None, output_resampling_shape=None,
+                 output_resampling_fill_max_steps=0, save_domain_name=None, stop_fn=lambda: False):
+        if density_viscosity is not None:
+            raise NotImplementedError("Simulation.density_viscosity is superseded by Domain.scalarViscosity.")
+        if scipy_solve_advection or scipy_solve_pressure:
+            raise NotImplementedError("the scipy solve path is not part of this build")
+        self.domain = domain
+        self.prep_fn = prep_fn
+        self.non_orthogonal = non_orthogonal
+        self.time_step = time_step
+        self.substeps = substeps
+        self.corrector_steps = corrector_steps
+        self.convergence_tol = convergence_tol
+        self.adaptive_CFL = adaptive_CFL
+        self.scipy_solve_advection = False
+        self.advection_use_BiCG = advection_use_BiCG
+        self.advect_non_ortho_steps = advect_non_ortho_steps
+        self._advection_tol = advection_tol
+        self.advect_passive_scalar = advect_passive_scalar
+        self.advect_reuse_result = advect_reuse_result
+        self.scipy_solve_pressure = False
+        self.pressure_use_BiCG = pressure_use_BiCG
+        self.pressure_non_ortho_steps = pressure_non_ortho_steps
+        self._pressure_tol = pressure_tol
+        self.normalize_pressure_result = normalize_pressure_result
+        self.pressure_return_best_result = pressure_return_best_result
+        self.pressure_time_step_normalized = pressure_time_step_normalized
+        self.pressure_reuse_result = pressure_reuse_result
+        self.solver_double_fallback = solver_double_fallback
+        self.linear_solve_max_iterations = 5000
+        self.preconditionBiCG = preconditionBiCG
+        self.BiCG_precondition_fallback = BiCG_precondition_fallback
+        self._velocity_corrector_versions = {"FD": 1, "FVM_CENTER": 5, "FVM_FACE": 6}
+        self.velocity_corrector = velocity_corrector
+        self.log_dir = log_dir
+        self.log_interval = log_interval
+        self.log_images = log_images
+        self.log_vtk = log_vtk
+        self.norm_vel = norm_vel
+        self.block_layout = block_layout
+        self.log_fn = log_fn
+        self.save_domain_name = save_domain_name
+        self.differentiable = differentiable
+        self.exclude_advection_solve_gradients = exclude_advection_solve_gradients
+        self.exclude_pressure_solve_gradients = exclude_pressure_solve_gradients
+        self.stop_fn = stop_fn
+        self.solver_infos = []  # (kind, [LinearSolverResultInfo]) of the last step, for iteration-count logging
+        self.record_solver_infos = False
+        self.reset_step_counters()
+
+    # --- properties (validation as in :552-835) ---
+    @property
+    def domain(self):
+        return self._domain
+
+    @domain.setter
+    def domain(self, domain):
+        if domain is not None:
+            if not isinstance(domain, PISOtorch.Domain):
+                raise TypeError("domain must be a PISOtorch.Domain object or None.")
+            if not domain.IsInitialized():
+                raise RuntimeError("domain must be initilized. Call domain.PrepareSolve() before assignment.")
+        self._domain = domain
+
+    def _check_domain(self):
+        if self._domain is None:
+            raise ValueError("no domain set.")
+
+    def _get_dtype(self):
+        self._check_domain()
+        if self.domain.getNumBlocks() == 0:
+            raise RuntimeError("Block required to infer dtype")
+        return self.domain.getBlock(0).velocity.dtype
+
+    @property
+    def differentiable(self):
+        return self._differentiable
+
+    @differentiable.setter
+    def differentiable(self, d):
+        if not isinstance(d, bool):
+            raise TypeError("differentiable must be bool.")
+        self._differentiable = d
+        self._backend = PISOtorch_diff if d else PISOtorch
+
+    @property
+    def non_orthogonal(self):
+        return self._non_orthogonal
+
+    @non_orthogonal.setter
+    def non_orthogonal(self, v):
+        if not isinstance(v, bool):
+            raise TypeError("non_orthogonal must be bool.")
+        self._non_orthogonal = v
+        self._non_ortho_flags = Simulation._NON_ORTHO_MODE if v else 0
+
+    @property
+    def time_step(self):
+        return self._time_step
+
+    @time_step.setter
+    def time_step(self, t):
+        if not isinstance(t, numbers.Real):
+            raise TypeError("time_step must be float.")
+        self._time_step = t
+
+    def _get_time_step_torch(self):
+        return torch.tensor([self._time_step], device=cpu_device, dtype=self._get_dtype())
+
+    @property
+    def substeps(self):
+        return self._substeps
+
+    @substeps.setter
+    def substeps(self, s):
+        if isinstance(s, str) and s.upper() == "ADAPTIVE":
+            raise NotImplementedError("adaptive substeps are not part of this build")
+        if not isinstance(s, numbers.Integral):
+            raise TypeError("substeps must be integral type or 'ADAPTIVE'.")
+        if not s > 0:
+            raise ValueError("substeps must be positive.")
+        self._substeps = s
+
+    @property
+    def corrector_steps(self):
+        return self._corrector_steps
+
+    @corrector_steps.setter
+    def corrector_steps(self, c):
+        if not isinstance(c, numbers.Integral):
+            raise TypeError("corrector_steps must be integral type.")
+        if c < 0:
+            raise ValueError("corrector_steps must not be negative.")
+        self._corrector_steps = c
+
+    @property
+    def advection_tol(self):
+        if self._advection_tol is None:
+            return PISOtorch_diff._get_solver_tolerance(dtype=self._get_dtype())
+        return self._advection_tol
+
+    @advection_tol.setter
+    def advection_tol(self, t):
+        if t is not None and not (isinstance(t, numbers.Real) and t > 0):
+            raise ValueError("advection_tol must be positive float or None.")
+        self._advection_tol = t
+
+    @property
+    def pressure_tol(self):
+        if self._pressure_tol is None:
+            return PISOtorch_diff._get_solver_tolerance(dtype=self._get_dtype())
+        return self._pressure_tol
+
+    @pressure_tol.setter
+    def pressure_tol(self, t):
+        if t is not None and not (isinstance(t, numbers.Real) and t > 0):
+            raise ValueError("pressure_tol must be positive float or None.")
+        self._pressure_tol = t
+
+    @property
+    def velocity_corrector(self):
+        return self._velocity_corrector
+
+    @velocity_corrector.setter
+    def velocity_corrector(self, v):
+        if v not in self._velocity_corrector_versions:
+            raise ValueError("velocity_corrector must be one of %s" % list(self._velocity_corrector_versions))
+        self._velocity_corrector = v
+        self._velocity_corrector_version = self._velocity_corrector_versions[v]
+
+    # --- hooks ---
+    @property
+    def prep_fn(self):
+        return self._prep_fn
+
+    @prep_fn.setter
+    def prep_fn(self, fn):
+        if fn is not None and not isinstance(fn, dict):
+            raise TypeError("prep_fn must be a dict {hook name: callable or list of callables}")
+        self._prep_fn = fn
+
+    def _run_prep_fn(self, name, **kwargs):
+        if self._prep_fn is not None and name in self._prep_fn:
+            fns = self._prep_fn[name]
+            if not isinstance(fns, (list, tuple)):
+                fns = [fns]
+            for fn in fns:
+                if fn is not None:
+                    fn(**kwargs)
+
+    def _check_stop(self):
+        return bool(self.stop_fn())
+
+    def reset_total_step(self):
+        self.total_step = 0
+
+    def reset_total_time(self):
+        self.total_time = 0
+
+    def end_step(self, time_step=0):
+        self.total_step += 1
+        self.total_time += float(ntonp(time_step)[0])
+
+    def reset_step_counters(self):
+        self.reset_total_step()
+        self.reset_total_time()
+
+    # --- linear solves (:861-903) ---
+    def linear_solve_GPU(self, csrMat, rhs, x=None, use_BiCG=False, tol=None, max_iter=5000,
+                         matrix_rank_deficient=False, residual_reset_step=0, return_best_result=False,
+                         double_fallback=False, BiCG_with_preconditioner=True, BiCG_precondition_fallback=False):
+        if self._backend is PISOtorch:
+            maxit_torch = torch.IntTensor([max_iter])
+            tol_torch = PISOtorch_diff._get_solver_tolerance_torch(tol, rhs.dtype)
+            if x is None:
+                x = torch.zeros_like(rhs)
+            infos = PISOtorch_diff._linear_solve_wrapper(
+                csrMat, rhs, x, maxit_torch, tol_torch, PISOtorch.ConvergenceCriterion.NORM2_NORMALIZED, use_BiCG,
+                matrix_rank_deficient, residual_reset_step, False, False, return_best_result,
+                double_fallback=double_fallback, debug_out=False, BiCG_with_preconditioner=BiCG_with_preconditioner,
+                BiCG_precondition_fallback=BiCG_precondition_fallback)
+            if self.record_solver_infos:
+                self.solver_infos.append(("BiCG" if use_BiCG else "CG", infos))
+            return x, True
+        return self._backend.linear_solve_GPU(csrMat, rhs, transpose=False, use_BiCG=use_BiCG, tol=tol,
+                                              max_iter=max_iter, return_best_result=return_best_result,
+                                              double_fallback=double_fallback,
+                                              BiCG_with_preconditioner=BiCG_with_preconditioner,
+                                              BiCG_precondition_fallback=BiCG_precondition_fallback), True
+
+    def linear_solve(self, csrMat, rhs, x=None, use_BiCG=False, tol=None, max_iter=None, matrix_rank_deficient=False,
+                     residual_reset_step=0, use_scipy=False, return_best_result=False):
+        if use_scipy:
+            raise NotImplementedError("the scipy solve path is not part of this build")
+        if max_iter is None:
+            max_iter = self.linear_solve_max_iterations
+        return self.linear_solve_GPU(csrMat, rhs, x, use_BiCG, tol, max_iter, matrix_rank_deficient,
+                                     residual_reset_step, return_best_result,
+                                     double_fallback=self.solver_double_fallback,
+                                     BiCG_with_preconditioner=self.preconditionBiCG,
+                                     BiCG_precondition_fallback=self.BiCG_precondition_fallback)
+
+    # --- the PISO step (:1054-1327) ---
+    def _PISO_split_step(self, iterations, time_step=None):
+        self._check_domain()
+        solve_ok = True
+        if time_step is None:
+            time_step = self._get_time_step_torch()
+        advect_use_prev_result = self.advect_reuse_result and not self.differentiable
+        advect_non_ortho_reuse_result = self.advect_reuse_result and not self.differentiable
+        pressure_reuse_result = self.pressure_reuse_result and not self.differentiable
+        pressure_use_face_transform = False
+        vcv = self._velocity_corrector_version
+        if isinstance(iterations, torch.Tensor):
+            iterations = iterations.numpy()[0]
+        assert iterations > 0
+        domain = self.domain
+        non_ortho_flags = self._non_ortho_flags
+        B = self._backend
+        self.solver_infos = []
+
+        for step in range(iterations):
+            if self.convergence_tol is not None:
+                B.CopyVelocityResultFromBlocks(domain)
+                last_vel = domain.velocityResult.clone().detach()
+            self._run_prep_fn("PRE", domain=domain, local_step=step, time_step=time_step, total_step=self.total_step,
+                              total_time=self.total_time)
+            domain.UpdateDomainData()
+
+            if self.advect_passive_scalar and domain.hasPassiveScalar():
+                raise PISOtorch.PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip; "
+                                                     "use advect_passive_scalar=False or passiveScalarChannels=0")
+
+            apply_pressure_gradient = False
+            self._run_prep_fn("PRE_VELOCITY_SETUP", domain=domain, local_step=step, time_step=time_step,
+                              total_step=self.total_step, total_time=self.total_time)
+            B.SetupAdvectionMatrix(domain, time_step, non_ortho_flags)
+            if not self.non_orthogonal:
+                B.SetupAdvectionVelocity(domain, time_step, 0, apply_pressure_gradient)
+                self._run_prep_fn("POST_VELOCITY_SETUP", domain=domain, no_step=0, local_step=step,
+                                  time_step=time_step, total_step=self.total_step, total_time=self.total_time)
+                with torch.no_grad() if self.exclude_advection_solve_gradients else nullcontext():
+                    x = None if (not advect_use_prev_result) else domain.velocityResult
+                    velocityResult, solve_ok = self.linear_solve(domain.C, domain.velocityRHS, x=x,
+                                                                 use_BiCG=self.advection_use_BiCG,
+                                                                 tol=self.advection_tol)
+                domain.setVelocityResult(velocityResult)
+                domain.UpdateDomainData()
+            else:
+                B.CopyVelocityResultFromBlocks(domain)
+                for no_step in range(self.advect_non_ortho_steps):
+                    B.SetupAdvectionVelocity(domain, time_step, non_ortho_flags, apply_pressure_gradient)
+                    self._run_prep_fn("POST_VELOCITY_SETUP", domain=domain, no_step=no_step, local_step=step,
+                                      time_step=time_step, total_step=self.total_step, total_time=self.total_time)
+                    with torch.no_grad() if self.exclude_advection_solve_gradients else nullcontext():
+                        x = None if (no_step == 0 or not advect_non_ortho_reuse_result) else domain.velocityResult
+                        velocityResult, solve_ok = self.linear_solve(domain.C, domain.velocityRHS, x=x,
+                                                                     use_BiCG=self.advection_use_BiCG,
+                                                                     tol=self.advection_tol)
+                    domain.setVelocityResult(velocityResult)
+                    domain.UpdateDomainData()
+                    if not solve_ok or self._check_stop():
+                        return solve_ok
+            if not solve_ok:
+                return solve_ok
+
+            self._run_prep_fn("POST_PREDICTION", domain=domain, local_step=step, time_step=time_step,
+                              total_step=self.total_step, total_time=self.total_time)
+            if self._check_stop():
+                break
+
+            for cstep in range(self.corrector_steps):
+                if not self.non_orthogonal:
+                    B.SetupPressureCorrection(domain, time_step, 0, pressure_use_face_transform,
+                                              timeStepNorm=self.pressure_time_step_normalized)
+                    self._run_prep_fn("POST_PRESSURE_SETUP", domain=domain, local_step=step, time_step=time_step,
+                                      total_step=self.total_step, total_time=self.total_time)
+                    with torch.no_grad() if self.exclude_pressure_solve_gradients else nullcontext():
+                        pressureResult, solve_ok = self.linear_solve(domain.P, domain.pressureRHSdiv, x=None,
+                                                                     use_BiCG=self.pressure_use_BiCG,
+                                                                     tol=self.pressure_tol,
+                                                                     return_best_result=self.pressure_return_best_result)
+                    if not solve_ok:
+                        return solve_ok
+                    if self.normalize_pressure_result:
+                        pressureResult = pressureResult - torch.mean(pressureResult)
+                    domain.setPressureResult(pressureResult)
+                    domain.UpdateDomainData()
+                    self._run_prep_fn("POST_PRESSURE_RESULT", domain=domain, local_step=step, time_step=time_step,
+                                      total_step=self.total_step, total_time=self.total_time)
+                else:
+                    B.SetupPressureMatrix(domain, time_step, non_ortho_flags, pressure_use_face_transform)
+                    for pstep in range(self.pressure_non_ortho_steps):
+                        if pstep == 0:
+                            B.SetupPressureRHS(domain, time_step, non_ortho_flags, pressure_use_face_transform,
+                                               timeStepNorm=self.pressure_time_step_normalized)
+                        else:
+                            B.SetupPressureRHSdiv(domain, time_step, non_ortho_flags, pressure_use_face_transform,
+                                                  timeStepNorm=self.pressure_time_step_normalized)
+                        self._run_prep_fn("POST_PRESSURE_SETUP", domain=domain, local_step=step, time_step=time_step,
+                                          total_step=self.total_step, total_time=self.total_time)
+                        with torch.no_grad() if self.exclude_pressure_solve_gradients else nullcontext():
+                            x = None if (pstep == 0 or not pressure_reuse_result) else domain.pressureResult
+                            pressureResult, solve_ok = self.linear_solve(
+                                domain.P, domain.pressureRHSdiv, x=x, matrix_rank_deficient=False,
+                                residual_reset_step=100, use_BiCG=self.pressure_use_BiCG, tol=self.pressure_tol,
+                                return_best_result=self.pressure_return_best_result)
+                        if self.normalize_pressure_result:
+                            pressureResult = pressureResult - torch.mean(pressureResult)
+                        domain.setPressureResult(pressureResult)
+                        domain.UpdateDomainData()
+                        self._run_prep_fn("POST_PRESSURE_RESULT", domain=domain, local_step=step,
+                                          time_step=time_step, total_step=self.total_step, total_time=self.total_time)
+                        if not solve_ok:
+                            return solve_ok
+                        if self._check_stop():
+                            break
+                self._run_prep_fn("POST_PRESSURE_NON_ORTHO", domain=domain, local_step=step, time_step=time_step,
+                                  total_step=self.total_step, total_time=self.total_time)
+                B.CopyPressureResultToBlocks(domain)
+                B.CorrectVelocity(domain, time_step, version=vcv, timeStepNorm=self.pressure_time_step_normalized)
+                self._run_prep_fn("POST_VELOCITY_CORRECTION", domain=domain, local_step=step, time_step=time_step,
+                                  total_step=self.total_step, total_time=self.total_time)
+                if self._check_stop():
+                    break
+
+            B.CopyVelocityResultToBlocks(domain)
+            self._run_prep_fn("POST", domain=domain, local_step=step, time_step=time_step, total_step=self.total_step,
+                              total_time=self.total_time)
+            if self.convergence_tol is not None:
+                step_max_diff = torch.max(torch.abs(last_vel - domain.velocityResult)).cpu().numpy()
+                if step_max_diff < self.convergence_tol:
+                    solve_ok = False
+            if not solve_ok or self._check_stop():
+                break
+            self.end_step(time_step)
+        return solve_ok
+
+    def run(self, iterations, static=False, log_domain=True):
+        """Simulation.run (:1374-1518) without image/VTK output."""
+        self._check_domain()
+        if static:
+            raise NotImplementedError("static (passive-scalar only) runs are not part of this build")
+        domain = self.domain
+        flux = domain.GetBoundaryFluxBalance()
+        if float(ntonp(torch.abs(flux))) > float(self.pressure_tol):
+            _LOG.warning("Domain boundary is not divergence free (flux balance: %.03e). This can prevent pressure "
+                         "solve convergence.", float(ntonp(flux)))
+            return False
+        sim_ok = True
+        for it in range(1, iterations + 1):
+            try:
+                sim_ok = self._PISO_split_step(iterations=self.substeps)
+            except PISOtorch_diff.LinsolveError:
+                _LOG.exception("Simulation failed in major step %d (total step %d):", it, self.total_step)
+                break
+            max_mag = tensor_as_np(domain.getMaxVelocityMagnitude(False))
+            if np.isnan(max_mag).any():
+                _LOG.warning("NaN encountered in velocity, stopping")
+                sim_ok = False
+                break
+            if self.log_fn is not None:
+                self.log_fn(domain=domain, out_dir=self.log_dir, it=it, out_it=0, total_step=self.total_step)
+            if self._check_stop() or not sim_ok:
+                break
+        return sim_ok
+
+    def make_divergence_free(self, iterations=1, max_iter=1000):
+        """Projection of the block velocities onto divergence-free fields with A = 1 and time step 1
+        (PISOtorch_simulation.py:985-1052)."""
+        self._check_domain()
+        pressure_reuse_result = self.pressure_reuse_result and not self.differentiable
+        vcv = self._velocity_corrector_version
+        if isinstance(iterations, torch.Tensor):
+            iterations = iterations.numpy()[0]
+        domain = self.domain
+        B = self._backend
+        non_ortho_flags = self._non_ortho_flags
+        time_step = torch.tensor([1], device=cpu_device, dtype=domain.A.dtype)
+        domain.setA(torch.ones_like(domain.A))
+        solve_ok = True
+        for step in range(iterations):
+            self._run_prep_fn("PRE", domain=domain, local_step=step, time_step=time_step, total_step=self.total_step,
+                              total_time=self.total_time)
+            domain.UpdateDomainData()
+            B.CopyVelocityResultFromBlocks(domain)
+            domain.setPressureRHS(domain.velocityResult)
+            domain.UpdateDomainData()
+            B.SetupPressureMatrix(domain, time_step, non_ortho_flags, False)
+            for pstep in range(self.pressure_non_ortho_steps):
+                B.SetupPressureRHSdiv(domain, time_step, non_ortho_flags, False,
+                                      timeStepNorm=self.pressure_time_step_normalized)
+                with torch.no_grad() if self.exclude_pressure_solve_gradients else nullcontext():
+                    x = None if (pstep == 0 or not pressure_reuse_result) else domain.pressureResult
+                    pressureResult, solve_ok = self.linear_solve(domain.P, domain.pressureRHSdiv, x=x,
+                                                                 use_BiCG=self.pressure_use_BiCG, tol=self.pressure_tol,
+                                                                 return_best_result=self.pressure_return_best_result,
+                                                                 max_iter=max_iter)
+                if self.normalize_pressure_result:
+                    pressureResult = pressureResult - torch.mean(pressureResult)
+                domain.setPressureResult(pressureResult)
+                domain.UpdateDomainData()
+                if not solve_ok:
+                    return solve_ok
+                if self._check_stop():
+                    break
+            B.CopyPressureResultToBlocks(domain)
+            B.CorrectVelocity(domain, time_step, version=vcv, timeStepNorm=self.pressure_time_step_normalized)
+            B.CopyVelocityResultToBlocks(domain)
+            if not solve_ok or self._check_stop():
+                return solve_ok
+            self.end_step(time_step)
+        return solve_ok

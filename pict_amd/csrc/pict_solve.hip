@@ -1,0 +1,850 @@
+// pict_solve.hip -- CG and BiCGStab for the PISO linear systems on gfx950.
+//
+// One cooperative (persistent) launch per solve.  All right-hand sides of a
+// call are iterated together, each with its own scalars and stopping test, so
+// every right-hand side follows exactly the iteration it would follow alone
+// (the reference solves them one after the other, cg_solver_kernel.cu:208-433,
+// bicgstab_solver_kernel.cu:219-365).  Dot products, norms and the Krylov
+// scalars are accumulated in FP64 whatever the vector precision.
+//
+// Grid-wide synchronisation is a "reduce barrier": every workgroup writes its
+// FP64 partial sums, arrives on a device-scope counter, and the last workgroup
+// to arrive reduces all partials in a fixed order (deterministic), updates the
+// per-RHS state and releases the others.  The textbook iterations are
+// re-associated so that a CG iteration needs 2 and a BiCGStab iteration 3 such
+// barriers: the search direction update p = r + beta*p (BiCGStab: with the
+// omega*v term) is folded into the following SpMV by evaluating it on the fly
+// for the neighbouring entries, with double-buffered p (and v), using an
+// explicit fma so that the on-the-fly value is bitwise the owner's value.
+//
+// Every wait is bounded: a barrier that spins longer than kSpinLimit sets an
+// error flag, after which all workgroups leave the kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pict_host.hpp"
+
+namespace pict {
+
+constexpr int kSolveThreads = 256;
+constexpr int kSlots = 8;              // FP64 partial slots per right-hand side
+constexpr unsigned kSpinLimit = 1u << 23;
+constexpr int kMaxRhs = 4;
+
+struct Ctl {
+    unsigned count;
+    unsigned gen;
+    int err;
+    int zero_all;
+};
+
+struct RhsState {
+    double rho, rho_prev, alpha, omega, beta;
+    double best_crit, last_crit;
+    int active, converged, finite, used;
+    int best_it, rising, save_best, restore_best;
+    double final_res;
+    double pad;
+};
+
+template <typename R>
+struct SolveArgs {
+    const int32_t *row, *col, *perm;  // perm != nullptr: use A^T (value of (j,i) for entry (i,j))
+    const R *val;
+    const R *b;
+    R *x;
+    R *r, *rhat, *p0, *p1, *v0, *v1, *s, *t, *best;  // each [nrhs*n]
+    double *partials;                              // [nwg * nrhs * kSlots]
+    Ctl *ctl;
+    RhsState *st;                                  // [nrhs]
+    int n, nrhs, maxit, conv, reset_steps, return_best, zero_shortcut;
+    double tol;
+    unsigned nwg;
+};
+
+// --------------------------------------------------------------------------
+// workgroup reductions (wave64 shuffles + LDS), results broadcast to all lanes
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+// Reduce K values per thread over the workgroup; out[k] valid in all threads.
+template <int K>
+__device__ __forceinline__ void block_reduce(double (&v)[K], const bool (&is_max)[K]) {
+    __shared__ double lds[kSolveThreads / 64][K];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double w = is_max[k] ? wave_max(v[k]) : wave_sum(v[k]);
+        if (lane == 0) lds[wave][k] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double a = lds[0][k];
+#pragma unroll
+        for (int w = 1; w < kSolveThreads / 64; ++w) a = is_max[k] ? fmax(a, lds[w][k]) : a + lds[w][k];
+        v[k] = a;
+    }
+    __syncthreads();
+}
+
+// --------------------------------------------------------------------------
+// grid reduce-barrier
+// --------------------------------------------------------------------------
+__device__ __forceinline__ bool arrive(Ctl *ctl, unsigned nwg) {
+    __shared__ int s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(&ctl->count, 1u);
+        s_last = prev == nwg - 1;
+        __threadfence();
+    }
+    __syncthreads();
+    return s_last != 0;
+}
+__device__ __forceinline__ void release(Ctl *ctl) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+}
+__device__ __forceinline__ void wait_gen(Ctl *ctl, unsigned my_gen) {
+    if (threadIdx.x == 0) {
+        unsigned spins = 0;
+        while (__hip_atomic_load(&ctl->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > kSpinLimit) {
+                atomicExch(&ctl->err, 1);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+__device__ __forceinline__ bool failed(Ctl *ctl) {
+    return __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// slot layout of the partial sums (per workgroup and right-hand side)
+enum { S_RR = 0, S_ASUM = 1, S_AMAX = 2, S_D0 = 3, S_D1 = 4, S_BMAX = 5 };
+
+// Flush NS accumulators per RHS of this thread: workgroup-reduce and store
+// this workgroup's partials for the slots `ids`.
+template <typename R, int NS>
+__device__ __forceinline__ void flush(const SolveArgs<R> &a, double (&acc)[kMaxRhs][NS], const int (&ids)[NS],
+                                      const bool (&ismax)[NS]) {
+#pragma unroll
+    for (int k = 0; k < kMaxRhs; ++k) {
+        if (k >= a.nrhs) break;
+        block_reduce<NS>(acc[k], ismax);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) a.partials[((size_t)blockIdx.x * a.nrhs + k) * kSlots + ids[s]] = acc[k][s];
+    }
+}
+
+// Master step: reduce every workgroup's partials (fixed order) for the slots
+// `ids`; red[k*kSlots + id] receives the totals (LDS, valid in all threads).
+template <int NS>
+__device__ __forceinline__ void master_reduce(const double *partials, unsigned nwg, int nrhs, const int (&ids)[NS],
+                                              const bool (&ismax)[NS], double *red) {
+    for (int k = 0; k < nrhs; ++k) {
+        double v[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double acc = 0.0;
+            for (unsigned w = threadIdx.x; w < nwg; w += kSolveThreads) {
+                const double pv = partials[((size_t)w * nrhs + k) * kSlots + ids[s]];
+                acc = ismax[s] ? fmax(acc, pv) : acc + pv;
+            }
+            v[s] = acc;
+        }
+        block_reduce<NS>(v, ismax);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) red[k * kSlots + ids[s]] = v[s];
+    }
+    __syncthreads();
+}
+
+// criterion from (sum r^2, sum |r|, max |r|), ComputeConvergenceCriterion cg_solver_kernel.cu:77-110
+__device__ __forceinline__ double criterion(int conv, double rr, double asum, double amax, int n) {
+    switch (conv) {
+        case PICT_CONV_NORM2: return sqrt(rr);
+        case PICT_CONV_NORM2_NORMALIZED: return sqrt(rr) / sqrt((double)n);
+        case PICT_CONV_ABS_SUM: return asum;
+        case PICT_CONV_ABS_MEAN: return asum / n;
+        default: return amax;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ R mat_val(const SolveArgs<R> &a, int k) {
+    return a.perm ? a.val[a.perm[k]] : a.val[k];
+}
+
+template <int NS>
+__device__ __forceinline__ void zero_acc(double (&acc)[kMaxRhs][NS]) {
+#pragma unroll
+    for (int k = 0; k < kMaxRhs; ++k)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[k][s] = 0.0;
+}
+// residual accumulators r^2, |r|, max|r| in slots 0..2 of acc[k]
+template <typename R, int NS>
+__device__ __forceinline__ void acc_res(double (&acc)[kMaxRhs][NS], int k, R rv) {
+    const double d = (double)rv;
+    acc[k][0] += d * d;
+    acc[k][1] += fabs(d);
+    acc[k][2] = fmax(acc[k][2], fabs(d));
+}
+
+// Run one reduce-barrier: flush, arrive, master (reduce + `update`), release/wait.
+#define PICT_REDUCE_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                  \
+    do {                                                                                  \
+        flush<R, NS>(a, ACC, IDS, ISMAX);                                                 \
+        if (arrive(a.ctl, a.nwg)) {                                                       \
+            master_reduce<NS>(a.partials, a.nwg, a.nrhs, IDS, ISMAX, red);                \
+            if (threadIdx.x == 0) { UPDATE; }                                             \
+            release(a.ctl);                                                               \
+        } else {                                                                          \
+            wait_gen(a.ctl, gen);                                                         \
+        }                                                                                 \
+        ++gen;                                                                            \
+        if (failed(a.ctl)) return;                                                        \
+    } while (0)
+
+// ==========================================================================
+// CG  (cgSolveGPU, cg_solver_kernel.cu:114-456)
+// ==========================================================================
+template <typename R>
+__global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
+    __shared__ double red[kMaxRhs * kSlots];
+    const int n = a.n;
+    const size_t stride = (size_t)a.nwg * kSolveThreads;
+    const size_t i0 = (size_t)blockIdx.x * kSolveThreads + threadIdx.x;
+    unsigned gen = 0;
+    constexpr int IDS_R[4] = {S_RR, S_ASUM, S_AMAX, S_BMAX};
+    constexpr bool MAX_R[4] = {false, false, true, true};
+    constexpr int IDS_RES[3] = {S_RR, S_ASUM, S_AMAX};
+    constexpr bool MAX_RES[3] = {false, false, true};
+    constexpr int IDS_D[1] = {S_D0};
+    constexpr bool MAX_D[1] = {false};
+
+    // ---- phase 0: r = b - A x, p_old = 0; rho = r.r ----
+    {
+        double acc[kMaxRhs][4];
+        zero_acc(acc);
+        for (size_t i = i0; i < (size_t)n; i += stride) {
+            const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                if (k >= a.nrhs) break;
+                const size_t o = (size_t)k * n;
+                R s = 0;
+                for (int e = rs; e < re; ++e) s += mat_val(a, e) * a.x[o + a.col[e]];
+                const R bv = a.b[o + i];
+                const R rv = bv - s;
+                a.r[o + i] = rv;
+                a.p0[o + i] = 0;
+                acc_res(acc, k, rv);
+                acc[k][3] = fmax(acc[k][3], fabs((double)bv));
+            }
+        }
+        PICT_REDUCE_BARRIER(4, acc, IDS_R, MAX_R, {
+            bool all_zero = true;
+            for (int k = 0; k < a.nrhs; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
+            const int zero_all = (a.zero_shortcut && all_zero) ? 1 : 0;
+            a.ctl->zero_all = zero_all;
+            for (int k = 0; k < a.nrhs; ++k) {
+                RhsState &st = a.st[k];
+                st.rho = red[k * kSlots + S_RR];
+                st.beta = 0;
+                st.active = zero_all ? 0 : 1;
+                st.converged = zero_all;
+                st.finite = 1;
+                st.used = -1;
+                st.final_res = 0;
+                st.best_crit = 0;
+                st.last_crit = 0;
+                st.best_it = -1;
+                st.rising = 0;
+                st.save_best = 0;
+                st.restore_best = 0;
+            }
+        });
+    }
+    if (a.ctl->zero_all) {
+        for (size_t i = i0; i < (size_t)n; i += stride)
+            for (int k = 0; k < a.nrhs; ++k) a.x[(size_t)k * n + i] = 0;
+        return;
+    }
+
+    for (int it = 0; it < a.maxit; ++it) {
+        // ---- optional residual reset r = b - A x, p = r (cg_solver_kernel.cu:266-287) ----
+        if (a.reset_steps > 0 && (it + 1) % a.reset_steps == 0) {
+            bool act[kMaxRhs];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) act[k] = k < a.nrhs && a.st[k].active;
+            double acc[kMaxRhs][3];
+            zero_acc(acc);
+            for (size_t i = i0; i < (size_t)n; i += stride) {
+                const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) {
+                    if (!act[k]) continue;
+                    const size_t o = (size_t)k * n;
+                    R s = 0;
+                    for (int e = rs; e < re; ++e) s += mat_val(a, e) * a.x[o + a.col[e]];
+                    const R rv = a.b[o + i] - s;
+                    a.r[o + i] = rv;
+                    acc_res(acc, k, rv);
+                }
+            }
+            PICT_REDUCE_BARRIER(3, acc, IDS_RES, MAX_RES, {
+                for (int k = 0; k < a.nrhs; ++k)
+                    if (a.st[k].active) {
+                        a.st[k].rho = red[k * kSlots + S_RR];
+                        a.st[k].beta = 0;
+                    }
+            });
+        }
+        R *pold = (it & 1) ? a.p1 : a.p0;
+        R *pnew = (it & 1) ? a.p0 : a.p1;
+        // ---- phase 1: p = r + beta p (evaluated on the fly), Ap = A p; p.Ap ----
+        {
+            bool act[kMaxRhs], save[kMaxRhs];
+            R beta[kMaxRhs];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                act[k] = k < a.nrhs && a.st[k].active;
+                save[k] = act[k] && a.st[k].save_best;
+                beta[k] = act[k] ? (R)a.st[k].beta : R(0);
+            }
+            double acc[kMaxRhs][1];
+            zero_acc(acc);
+            for (size_t i = i0; i < (size_t)n; i += stride) {
+                const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) {
+                    if (!act[k]) continue;
+                    const size_t o = (size_t)k * n;
+                    if (save[k]) a.best[o + i] = a.x[o + i];
+                    const R pn = fma(beta[k], pold[o + i], a.r[o + i]);
+                    pnew[o + i] = pn;
+                    R s = 0;
+                    for (int e = rs; e < re; ++e) {
+                        const size_t c = o + a.col[e];
+                        s += mat_val(a, e) * fma(beta[k], pold[c], a.r[c]);
+                    }
+                    a.v0[o + i] = s;  // Ap
+                    acc[k][0] += (double)pn * (double)s;
+                }
+            }
+            PICT_REDUCE_BARRIER(1, acc, IDS_D, MAX_D, {
+                for (int k = 0; k < a.nrhs; ++k)
+                    if (a.st[k].active) {
+                        a.st[k].alpha = a.st[k].rho / red[k * kSlots + S_D0];
+                        a.st[k].save_best = 0;
+                    }
+            });
+        }
+        // ---- phase 2: x += alpha p, r -= alpha Ap; r.r, convergence ----
+        {
+            bool act[kMaxRhs];
+            R alpha[kMaxRhs];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                act[k] = k < a.nrhs && a.st[k].active;
+                alpha[k] = act[k] ? (R)a.st[k].alpha : R(0);
+            }
+            double acc[kMaxRhs][3];
+            zero_acc(acc);
+            for (size_t i = i0; i < (size_t)n; i += stride) {
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) {
+                    if (!act[k]) continue;
+                    const size_t o = (size_t)k * n;
+                    a.x[o + i] += alpha[k] * pnew[o + i];
+                    const R rv = a.r[o + i] - alpha[k] * a.v0[o + i];
+                    a.r[o + i] = rv;
+                    acc_res(acc, k, rv);
+                }
+            }
+            PICT_REDUCE_BARRIER(3, acc, IDS_RES, MAX_RES, {
+                for (int k = 0; k < a.nrhs; ++k) {
+                    RhsState &st = a.st[k];
+                    if (!st.active) continue;
+                    const double rr = red[k * kSlots + S_RR];
+                    const double crit = criterion(a.conv, rr, red[k * kSlots + S_ASUM], red[k * kSlots + S_AMAX], n);
+                    if (!isfinite(crit)) {
+                        st.converged = 0; st.used = it; st.final_res = crit; st.finite = 0; st.active = 0;
+                        continue;
+                    }
+                    if (a.return_best) {
+                        if (it == 0 || crit < st.best_crit) { st.best_crit = crit; st.best_it = it; st.save_best = 1; }
+                        if (it > 0 && crit >= st.last_crit) ++st.rising; else st.rising = 0;
+                        st.last_crit = crit;
+                    }
+                    st.used = it;
+                    st.final_res = crit;
+                    if (crit < a.tol) { st.converged = 1; st.finite = 1; st.active = 0; continue; }
+                    if (a.return_best && (it == a.maxit - 1 || st.rising >= 100)) {
+                        st.restore_best = st.best_it != it;
+                        st.converged = 0; st.used = st.best_it; st.final_res = st.best_crit; st.finite = 1;
+                        st.active = 0;
+                        continue;
+                    }
+                    st.rho_prev = st.rho;
+                    st.rho = rr;
+                    st.beta = st.rho / st.rho_prev;
+                }
+            });
+        }
+        bool any = false;
+        for (int k = 0; k < a.nrhs; ++k) any = any || a.st[k].active;
+        if (!any) break;
+    }
+    // restore best iterates (cg_solver_kernel.cu:399-412)
+    for (int k = 0; k < a.nrhs; ++k) {
+        if (!a.st[k].restore_best) continue;
+        const size_t o = (size_t)k * n;
+        for (size_t i = i0; i < (size_t)n; i += stride) a.x[o + i] = a.best[o + i];
+    }
+}
+
+// ==========================================================================
+// BiCGStab without preconditioner (bicgstabSolveGPU, bicgstab_solver_kernel.cu:222-365)
+// ==========================================================================
+template <typename R>
+__global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
+    __shared__ double red[kMaxRhs * kSlots];
+    const int n = a.n;
+    const size_t stride = (size_t)a.nwg * kSolveThreads;
+    const size_t i0 = (size_t)blockIdx.x * kSolveThreads + threadIdx.x;
+    unsigned gen = 0;
+    constexpr int IDS_R[4] = {S_RR, S_ASUM, S_AMAX, S_BMAX};
+    constexpr bool MAX_R[4] = {false, false, true, true};
+    constexpr int IDS_D[1] = {S_D0};
+    constexpr bool MAX_D[1] = {false};
+    constexpr int IDS_B[5] = {S_RR, S_ASUM, S_AMAX, S_D0, S_D1};
+    constexpr bool MAX_B[5] = {false, false, true, false, false};
+    constexpr int IDS_C[4] = {S_RR, S_ASUM, S_AMAX, S_D0};
+    constexpr bool MAX_C[4] = {false, false, true, false};
+
+    // ---- phase 0: r = b - A x, rhat = r, p = v = 0 ----
+    {
+        double acc[kMaxRhs][4];
+        zero_acc(acc);
+        for (size_t i = i0; i < (size_t)n; i += stride) {
+            const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                if (k >= a.nrhs) break;
+                const size_t o = (size_t)k * n;
+                R s = 0;
+                for (int e = rs; e < re; ++e) s += mat_val(a, e) * a.x[o + a.col[e]];
+                const R bv = a.b[o + i];
+                const R rv = bv - s;
+                a.r[o + i] = rv;
+                a.rhat[o + i] = rv;
+                a.p0[o + i] = 0;
+                a.v0[o + i] = 0;
+                acc_res(acc, k, rv);
+                acc[k][3] = fmax(acc[k][3], fabs((double)bv));
+            }
+        }
+        PICT_REDUCE_BARRIER(4, acc, IDS_R, MAX_R, {
+            bool all_zero = true;
+            for (int k = 0; k < a.nrhs; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
+            const int zero_all = (a.zero_shortcut && all_zero) ? 1 : 0;
+            a.ctl->zero_all = zero_all;
+            for (int k = 0; k < a.nrhs; ++k) {
+                RhsState &st = a.st[k];
+                const double nr0 = criterion(a.conv, red[k * kSlots + S_RR], red[k * kSlots + S_ASUM],
+                                             red[k * kSlots + S_AMAX], n);
+                st.finite = 1;
+                st.restore_best = 0;
+                st.save_best = 0;
+                st.rho = red[k * kSlots + S_RR];  // (rhat, r) with rhat = r
+                st.beta = 0;
+                st.alpha = 0;
+                st.omega = 0;
+                st.used = -1;
+                if (zero_all) {
+                    st.active = 0; st.converged = 1; st.final_res = 0;
+                } else if (nr0 < a.tol) {  // bicgstab_solver_kernel.cu:259, :351-356
+                    st.active = 0; st.converged = 1; st.final_res = nr0;
+                } else {
+                    st.active = 1; st.converged = 0; st.final_res = 0;
+                }
+            }
+        });
+    }
+    if (a.ctl->zero_all) {
+        for (size_t i = i0; i < (size_t)n; i += stride)
+            for (int k = 0; k < a.nrhs; ++k) a.x[(size_t)k * n + i] = 0;
+        return;
+    }
+
+    for (int it = 0; it < a.maxit; ++it) {
+        {
+            bool any = false;
+            for (int k = 0; k < a.nrhs; ++k) any = any || a.st[k].active;
+            if (!any) break;
+        }
+        R *pold = (it & 1) ? a.p1 : a.p0;
+        R *pnew = (it & 1) ? a.p0 : a.p1;
+        R *vold = (it & 1) ? a.v1 : a.v0;
+        R *vnew = (it & 1) ? a.v0 : a.v1;
+        // ---- phase A: p = r + beta (p - omega v) on the fly; v = A p; (rhat, v) ----
+        {
+            bool act[kMaxRhs];
+            R beta[kMaxRhs], mom[kMaxRhs];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                act[k] = k < a.nrhs && a.st[k].active;
+                beta[k] = act[k] ? (R)a.st[k].beta : R(0);
+                mom[k] = act[k] ? (R)(-a.st[k].omega) : R(0);
+            }
+            double acc[kMaxRhs][1];
+            zero_acc(acc);
+            for (size_t i = i0; i < (size_t)n; i += stride) {
+                const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) {
+                    if (!act[k]) continue;
+                    const size_t o = (size_t)k * n;
+                    const R pn = fma(beta[k], fma(mom[k], vold[o + i], pold[o + i]), a.r[o + i]);
+                    pnew[o + i] = pn;
+                    R s = 0;
+                    for (int e = rs; e < re; ++e) {
+                        const size_t c = o + a.col[e];
+                        s += mat_val(a, e) * fma(beta[k], fma(mom[k], vold[c], pold[c]), a.r[c]);
+                    }
+                    vnew[o + i] = s;
+                    acc[k][0] += (double)a.rhat[o + i] * (double)s;
+                }
+            }
+            PICT_REDUCE_BARRIER(1, acc, IDS_D, MAX_D, {
+                for (int k = 0; k < a.nrhs; ++k)
+                    if (a.st[k].active) a.st[k].alpha = a.st[k].rho / red[k * kSlots + S_D0];
+            });
+        }
+        // ---- phase B: s = r - alpha v; t = A s (on the fly); x += alpha p ----
+        {
+            bool act[kMaxRhs];
+            R alpha[kMaxRhs];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                act[k] = k < a.nrhs && a.st[k].active;
+                alpha[k] = act[k] ? (R)a.st[k].alpha : R(0);
+            }
+            double acc[kMaxRhs][5];
+            zero_acc(acc);
+            for (size_t i = i0; i < (size_t)n; i += stride) {
+                const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) {
+                    if (!act[k]) continue;
+                    const size_t o = (size_t)k * n;
+                    const R mal = -alpha[k];
+                    const R sv = fma(mal, vnew[o + i], a.r[o + i]);
+                    a.s[o + i] = sv;
+                    R tv = 0;
+                    for (int e = rs; e < re; ++e) {
+                        const size_t c = o + a.col[e];
+                        tv += mat_val(a, e) * fma(mal, vnew[c], a.r[c]);
+                    }
+                    a.t[o + i] = tv;
+                    a.x[o + i] += alpha[k] * pnew[o + i];
+                    acc_res(acc, k, sv);
+                    acc[k][3] += (double)tv * (double)sv;
+                    acc[k][4] += (double)tv * (double)tv;
+                }
+            }
+            PICT_REDUCE_BARRIER(5, acc, IDS_B, MAX_B, {
+                for (int k = 0; k < a.nrhs; ++k) {
+                    RhsState &st = a.st[k];
+                    if (!st.active) continue;
+                    const double nr = criterion(a.conv, red[k * kSlots + S_RR], red[k * kSlots + S_ASUM],
+                                                red[k * kSlots + S_AMAX], n);
+                    if (!isfinite(nr)) {
+                        st.converged = 0; st.used = it; st.final_res = nr; st.finite = 0; st.active = 0;
+                        continue;
+                    }
+                    st.used = it;
+                    st.final_res = nr;
+                    if (nr < a.tol) { st.converged = 1; st.active = 0; continue; }
+                    st.omega = red[k * kSlots + S_D0] / red[k * kSlots + S_D1];
+                }
+            });
+        }
+        // ---- phase C: x += omega s; r = s - omega t; (rhat, r), convergence ----
+        {
+            bool act[kMaxRhs];
+            R omega[kMaxRhs];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                act[k] = k < a.nrhs && a.st[k].active;
+                omega[k] = act[k] ? (R)a.st[k].omega : R(0);
+            }
+            double acc[kMaxRhs][4];
+            zero_acc(acc);
+            for (size_t i = i0; i < (size_t)n; i += stride) {
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) {
+                    if (!act[k]) continue;
+                    const size_t o = (size_t)k * n;
+                    const R sv = a.s[o + i];
+                    a.x[o + i] += omega[k] * sv;
+                    const R rv = sv - omega[k] * a.t[o + i];
+                    a.r[o + i] = rv;
+                    acc_res(acc, k, rv);
+                    acc[k][3] += (double)a.rhat[o + i] * (double)rv;
+                }
+            }
+            PICT_REDUCE_BARRIER(4, acc, IDS_C, MAX_C, {
+                for (int k = 0; k < a.nrhs; ++k) {
+                    RhsState &st = a.st[k];
+                    if (!st.active) continue;
+                    const double nr = criterion(a.conv, red[k * kSlots + S_RR], red[k * kSlots + S_ASUM],
+                                                red[k * kSlots + S_AMAX], n);
+                    st.final_res = nr;
+                    if (nr < a.tol) { st.converged = 1; st.used = it + 1; st.active = 0; continue; }
+                    st.rho_prev = st.rho;
+                    st.rho = red[k * kSlots + S_D0];
+                    st.beta = (st.rho / st.rho_prev) * (st.alpha / st.omega);
+                }
+            });
+        }
+    }
+}
+
+// A^T permutation for a structurally symmetric pattern: perm[e] = index of (j,i) for e = (i,j).
+__global__ void k_transpose_perm(const int32_t *row, const int32_t *col, int n, int32_t *perm, int *missing) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int e = row[i]; e < row[i + 1]; ++e) {
+        const int j = col[e];
+        int lo = row[j], hi = row[j + 1] - 1, found = -1;
+        while (lo <= hi) {  // rows are sorted by column
+            const int mid = (lo + hi) >> 1;
+            const int c = col[mid];
+            if (c == i) { found = mid; break; }
+            if (c < i) lo = mid + 1; else hi = mid - 1;
+        }
+        perm[e] = found < 0 ? e : found;
+        if (found < 0) atomicExch(missing, 1);
+    }
+}
+
+template <typename R>
+__global__ void k_outer(const R *a, const R *b, const int32_t *row, const int32_t *col, int n, R *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R ai = a[i];
+    for (int e = row[i]; e < row[i + 1]; ++e) out[e] = ai * b[col[e]];
+}
+
+}  // namespace pict
+
+using namespace pict;
+
+namespace {
+
+struct DeviceLimits {
+    int cus = 0;
+    int occ_cg_f = 0, occ_cg_d = 0, occ_bi_f = 0, occ_bi_d = 0;
+};
+
+pict_status device_limits(DeviceLimits &L) {
+    static thread_local int cached_dev = -1;
+    static thread_local DeviceLimits cached;
+    int dev;
+    PICT_HIP_TRY(hipGetDevice(&dev));
+    if (dev == cached_dev) {
+        L = cached;
+        return PICT_OK;
+    }
+    PICT_HIP_TRY(hipDeviceGetAttribute(&L.cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg_f, k_cg<float>, kSolveThreads, 0));
+    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg_d, k_cg<double>, kSolveThreads, 0));
+    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_bi_f, k_bicgstab<float>, kSolveThreads, 0));
+    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_bi_d, k_bicgstab<double>, kSolveThreads, 0));
+    cached = L;
+    cached_dev = dev;
+    return PICT_OK;
+}
+
+template <typename R>
+pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
+                        pict_solve_info *infos, hipStream_t stream) {
+    const int n = A->n;
+    DeviceLimits L;
+    pict_status st = device_limits(L);
+    if (st) return st;
+    const bool f32 = sizeof(R) == 4;
+    const int occ = o->use_bicg ? (f32 ? L.occ_bi_f : L.occ_bi_d) : (f32 ? L.occ_cg_f : L.occ_cg_d);
+    if (occ < 1) {
+        set_error("solver kernel cannot be resident (occupancy 0)");
+        return PICT_ERR_HIP;
+    }
+    const long long want = ((long long)n + kSolveThreads - 1) / kSolveThreads;
+    long long nwg = want;
+    const long long cap = (long long)occ * L.cus;
+    if (nwg > cap) nwg = cap;
+    if (nwg > 1024) nwg = 1024;
+    if (nwg < 1) nwg = 1;
+
+    const size_t vec = (size_t)n * nrhs;
+    const int nvec = o->use_bicg ? 8 : (o->return_best_result ? 5 : 4);
+    const size_t bytes_vec = sizeof(R) * vec * nvec;
+    const size_t bytes_part = sizeof(double) * (size_t)nwg * nrhs * kSlots;
+    const size_t bytes_perm = o->transpose_a ? sizeof(int32_t) * (size_t)A->nnz : 0;
+    const size_t off_part = (bytes_vec + 255) & ~size_t(255);
+    const size_t off_ctl = off_part + ((bytes_part + 255) & ~size_t(255));
+    const size_t off_st = off_ctl + 256;
+    const size_t off_perm = off_st + ((sizeof(RhsState) * nrhs + 255) & ~size_t(255));
+    const size_t total = off_perm + bytes_perm + 256;
+
+    char *ws = nullptr;
+    PICT_HIP_TRY(hipMallocAsync((void **)&ws, total, stream));
+    struct Guard {
+        char *p; hipStream_t s;
+        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+    } guard{ws, stream};
+
+    SolveArgs<R> a;
+    memset(&a, 0, sizeof a);
+    a.row = A->row;
+    a.col = A->index;
+    a.val = (const R *)A->value;
+    a.b = (const R *)rhs;
+    a.x = (R *)x;
+    R *v = (R *)ws;
+    a.r = v; v += vec;
+    a.p0 = v; v += vec;
+    a.p1 = v; v += vec;
+    a.v0 = v; v += vec;
+    if (o->use_bicg) {
+        a.rhat = v; v += vec;
+        a.v1 = v; v += vec;
+        a.s = v; v += vec;
+        a.t = v; v += vec;
+    } else if (o->return_best_result) {
+        a.best = v; v += vec;
+    }
+    a.partials = (double *)(ws + off_part);
+    a.ctl = (Ctl *)(ws + off_ctl);
+    a.st = (RhsState *)(ws + off_st);
+    a.n = n;
+    a.nrhs = nrhs;
+    a.maxit = o->max_iterations;
+    a.conv = o->convergence_criterion;
+    a.reset_steps = o->residual_reset_steps;
+    a.return_best = o->return_best_result;
+    a.zero_shortcut = o->zero_rhs_shortcut;
+    a.tol = o->tolerance;
+    a.nwg = (unsigned)nwg;
+    PICT_HIP_TRY(hipMemsetAsync(ws + off_ctl, 0, 256, stream));
+    if (o->transpose_a) {
+        int32_t *perm = (int32_t *)(ws + off_perm);
+        int *missing = (int *)(ws + off_ctl + 128);
+        hipLaunchKernelGGL(k_transpose_perm, dim3((n + 255) / 256), dim3(256), 0, stream, A->row, A->index, n, perm,
+                           missing);
+        PICT_HIP_TRY(hipGetLastError());
+        int h_missing = 0;
+        PICT_HIP_TRY(hipMemcpyAsync(&h_missing, missing, sizeof(int), hipMemcpyDeviceToHost, stream));
+        PICT_HIP_TRY(hipStreamSynchronize(stream));
+        if (h_missing) {
+            set_error("transposed solve requires a structurally symmetric CSR pattern");
+            return PICT_ERR_UNSUPPORTED;
+        }
+        a.perm = perm;
+    }
+    void *args[] = {&a};
+    if (o->use_bicg) {
+        PICT_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bicgstab<R>, dim3((unsigned)nwg), dim3(kSolveThreads),
+                                                args, 0, stream));
+    } else {
+        PICT_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_cg<R>, dim3((unsigned)nwg), dim3(kSolveThreads), args,
+                                                0, stream));
+    }
+    std::vector<RhsState> hs(nrhs);
+    Ctl hctl;
+    PICT_HIP_TRY(hipMemcpyAsync(hs.data(), a.st, sizeof(RhsState) * nrhs, hipMemcpyDeviceToHost, stream));
+    PICT_HIP_TRY(hipMemcpyAsync(&hctl, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    PICT_HIP_TRY(hipStreamSynchronize(stream));
+    if (hctl.err) {
+        set_error("solver grid barrier exceeded its spin limit");
+        return PICT_ERR_DEVICE_SYNC;
+    }
+    for (int k = 0; k < nrhs; ++k) {
+        infos[k].final_residual = hs[k].final_res;
+        infos[k].used_iterations = hs[k].used;
+        infos[k].converged = hs[k].converged;
+        infos[k].is_finite_residual = hs[k].finite;
+        infos[k].reserved = 0;
+    }
+    return PICT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, int32_t nrhs,
+                              const pict_solve_opts *opts, pict_solve_info *infos_out, void *stream) {
+    PICT_CHECK_ARG(A && rhs && x && opts && infos_out, "NULL argument");
+    PICT_CHECK_ARG(A->n > 0 && A->row && A->index && A->value, "invalid matrix");
+    PICT_CHECK_ARG(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs must be 1..%d (got %d)", kMaxRhs, nrhs);
+    PICT_CHECK_ARG(opts->max_iterations >= 0, "max_iterations must be >= 0");
+    PICT_CHECK_ARG(opts->convergence_criterion >= 0 && opts->convergence_criterion <= 4, "unknown criterion");
+    if (opts->matrix_rank_deficient) {
+        set_error("matrixRankDeficient is not implemented by libpict_hip");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    if (opts->use_bicg && opts->bicg_with_preconditioner) {
+        set_error("ILU(0)-preconditioned BiCGStab is not implemented by libpict_hip");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    if (A->dtype == PICT_DTYPE_F32)
+        return solve_typed<float>(A, rhs, x, nrhs, opts, infos_out, (hipStream_t)stream);
+    if (A->dtype == PICT_DTYPE_F64)
+        return solve_typed<double>(A, rhs, x, nrhs, opts, infos_out, (hipStream_t)stream);
+    set_error("unknown dtype %d", A->dtype);
+    return PICT_ERR_INVALID;
+}
+
+pict_status pict_sparse_outer_product(const void *a, const void *b, const pict_csr_desc *pat, void *out,
+                                      void *stream) {
+    PICT_CHECK_ARG(a && b && pat && out && pat->row && pat->index, "NULL argument");
+    const int n = pat->n;
+    if (pat->dtype == PICT_DTYPE_F32)
+        hipLaunchKernelGGL(k_outer<float>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, (const float *)a,
+                           (const float *)b, pat->row, pat->index, n, (float *)out);
+    else
+        hipLaunchKernelGGL(k_outer<double>, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                           (const double *)a, (const double *)b, pat->row, pat->index, n, (double *)out);
+    PICT_HIP_TRY(hipGetLastError());
+    return PICT_OK;
+}
+
+}  // extern "C"

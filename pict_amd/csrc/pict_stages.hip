@@ -1,0 +1,601 @@
+// pict_stages.hip -- domain atlas and the per-cell PISO stage kernels.
+//
+// One workgroup = 256 consecutive cells of one block, one thread per cell.
+// The workgroup->(block, first cell) tables are built once per domain
+// structure and live in device memory next to the block table (the reference
+// rebuilds and uploads them on every call, PISO_multiblock_cuda_kernel.cu:59-93).
+// The block table is uploaded stream-ordered, so no host synchronisation is
+// needed when tensors are swapped between stages.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "pict_host.hpp"
+
+namespace pict {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+// ---------------------------------------------------------------------------
+// CSR pattern: row pointer from the closed form of
+// getCSRMatrixRowEndOffsetFromBlockBoundaries3D (:247-312), columns of
+// diag + non-prescribed neighbours sorted ascending (:3663-3772, :3846-3860).
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ int row_end_closed_form(const pict_block_desc *b, int flatPos, int &rs) {
+    const I4 p = unflat(flatPos, b);
+    rs = 2 * D + 1;
+    int re = (flatPos + 1) * rs;
+    if (prescribed(0, b)) { re -= flatPos / b->size[0] + 1; if (p.a[0] == 0) --rs; }
+    if (prescribed(1, b)) { re -= (flatPos + 1) / b->size[0]; if (p.a[0] == b->size[0] - 1) --rs; }
+    if (D > 1) {
+        if (prescribed(2, b)) { re -= b->size[0] * p.a[2] + (p.a[1] == 0 ? p.a[0] + 1 : b->size[0]); if (p.a[1] == 0) --rs; }
+        if (prescribed(3, b)) { re -= b->size[0] * p.a[2] + (p.a[1] == b->size[1] - 1 ? p.a[0] + 1 : 0); if (p.a[1] == b->size[1] - 1) --rs; }
+    }
+    if (D > 2) {
+        if (prescribed(4, b)) { re -= (p.a[2] == 0 ? flatPos + 1 : b->stride[2]); if (p.a[2] == 0) --rs; }
+        if (prescribed(5, b)) { re -= (p.a[2] == b->size[2] - 1 ? flatPos + 1 - p.a[2] * b->stride[2] : 0); if (p.a[2] == b->size[2] - 1) --rs; }
+    }
+    return re + b->csr_offset;
+}
+
+template <typename R, int D>
+__device__ __forceinline__ void row_slots(const DomainK<R> &dom, const pict_block_desc *b, int bi, const I4 &pos,
+                                          int *idx) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) idx[k] = -1;
+    idx[0] = global_index(pos, b);
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+        int nb;
+        const I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
+        idx[face + 1] = global_index(t, dom.blocks + nb);
+    }
+}
+
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_csr_pattern(DomainK<R> dom) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    int rs;
+    const int re = row_end_closed_form<D>(b, f, rs);
+    const int g = b->global_offset + f;
+    dom.csr_row[g + 1] = re;
+    if (g == 0) dom.csr_row[0] = 0;
+    int idx[7];
+    row_slots<R, D>(dom, b, bi, pos, idx);
+    const int start = re - rs;
+    for (int i = 0; i < rs; ++i) {
+        const int s = lowest_slot(idx);
+        dom.csr_index[start + i] = s < 0 ? -1 : idx[s];
+        if (s >= 0) idx[s] = -1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PISO_build_matrix :3601-3865 (velocity matrix C and its diagonal A),
+// orthogonal terms.  Values only; the pattern is written by k_csr_pattern.
+// ---------------------------------------------------------------------------
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix(DomainK<R> dom, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    R fl[2 * D];
+    fluxes_nd<R, D>(dom, b, pos, fl, nullptr);
+    int idx[7];
+    R rv[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { idx[k] = -1; rv[k] = 0; }
+    const R det = cell_det<R, D>(b, pos);
+    R diag = det / dt;
+    idx[0] = f + b->global_offset;
+    const R visc_p = viscosity(dom, b, pos);
+    R alpha_p[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        alpha_p[d] = laplace_orth<R, D>(b, t);
+    }
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        const int dim = face >> 1;
+        const R fs = (face & 1) ? R(1) : R(-1);
+        if (!(at_bound(pos, face, b) && prescribed(face, b))) {
+            const R ff = fs * R(0.5) * fl[face];
+            diag += ff;
+            rv[face + 1] += ff;
+            int nb;
+            I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
+            const pict_block_desc *nbp = dom.blocks + nb;
+            const R alpha = laplace_orth<R, D>(nbp, t);
+            t.a[3] = 0;
+            const R visc_n = viscosity(dom, nbp, t);
+            idx[face + 1] = global_index(t, nbp);
+            const R vc = (alpha_p[dim] * visc_p + alpha * visc_n) * R(0.5);
+            diag += vc;
+            rv[face + 1] -= vc;
+        } else {
+            const int slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
+            diag += R((1 - slip) * 2) * visc_p * alpha_p[dim];
+        }
+    }
+    rv[0] = diag;
+    const int g = f + b->global_offset;
+    const int start = dom.csr_row[g], end = dom.csr_row[g + 1];
+    for (int i = 0; i < end - start; ++i) {
+        const int s = lowest_slot(idx);
+        if (s < 0) break;
+        rv[s] /= det;
+        dom.C[start + i] = rv[s];
+        idx[s] = -1;
+    }
+    dom.A[g] = rv[0];
+}
+
+// kPISO_build_advection_RHS :4281-4385 (orthogonal)
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs(DomainK<R> dom, R dt, int apply_pg) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const R det = cell_det<R, D>(b, pos);
+    R pg[D];
+    if (apply_pg) pressure_gradient<R, D>(dom, b, pos, pg);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        R r = det * reinterpret_cast<const R *>(b->velocity)[flat4(t, b->stride)] / dt;
+        r += fixed_boundary_source<R, D>(dom, b, pos, d);
+        r /= det;
+        r += velocity_source<R>(b, t);
+        if (apply_pg) r -= pg[d] * dt;
+        dom.vrhs[(size_t)d * dom.N + b->global_offset + f] = r;
+    }
+}
+
+// PISO_build_pressure_matrix :4797-4963 (orthogonal, cell metrics)
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix(DomainK<R> dom) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    int idx[7];
+    R rv[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { idx[k] = -1; rv[k] = 0; }
+    R alphaP[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        alphaP[d] = laplace_orth<R, D>(b, t);
+    }
+    const R raP = R(1.0) / dom.A[g];
+    idx[0] = g;
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+        const int dim = face >> 1;
+        int nb;
+        I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
+        const pict_block_desc *nbp = dom.blocks + nb;
+        const R alphaN = laplace_orth<R, D>(nbp, t);
+        t.a[3] = 0;
+        const int gn = global_index(t, nbp);
+        const R raN = R(1.0) / dom.A[gn];
+        const R coef = R(0.5) * (alphaP[dim] * raP + alphaN * raN);
+        rv[0] -= coef;
+        rv[face + 1] += coef;
+        idx[face + 1] = gn;
+    }
+    const int start = dom.csr_row[g], end = dom.csr_row[g + 1];
+    for (int i = 0; i < end - start; ++i) {
+        const int s = lowest_slot(idx);
+        if (s < 0) break;
+        dom.P[start + i] = rv[s];
+        idx[s] = -1;
+    }
+}
+
+// PISO_build_pressure_rhs :5121-5240
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_rhs(DomainK<R> dom, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R rDiag = R(1) / dom.A[g];
+    const R det = cell_det<R, D>(b, pos);
+    const int cs = dom.csr_row[g], ce = dom.csr_row[g + 1];
+    int col[7];
+    R val[7];
+    const int rs = ce - cs < 7 ? ce - cs : 7;
+    for (int k = 0; k < rs; ++k) {
+        col[k] = dom.csr_index[cs + k];
+        val[k] = dom.C[cs + k];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        const R velOld = reinterpret_cast<const R *>(b->velocity)[flat4(t, b->stride)] / dt;
+        R H = 0;
+        for (int k = 0; k < rs; ++k)
+            if (col[k] != g) H += val[k] * dom.vres[col[k] + (size_t)dom.N * d];
+        R S = fixed_boundary_source<R, D>(dom, b, pos, d);
+        S /= det;
+        S += velocity_source<R>(b, t);
+        dom.prhs[g + (size_t)dom.N * d] = rDiag * (velOld - H + S);
+    }
+}
+
+// k_computePressureRHSdivergenceFromFlux :5374-5419 (vg = pressure RHS) and
+// k_computeVelocityDivergenceFromFlux :6333-6354 (vg = nullptr, block velocity).
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_divergence(DomainK<R> dom, const R *vg, R *out, R dt, int tsn) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    R fl[2 * D];
+    fluxes_nd<R, D>(dom, b, pos, fl, vg);
+    R div = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) div += fl[2 * d + 1] - fl[2 * d];
+    if (tsn) div /= dt;
+    out[f + b->global_offset] = div;
+}
+
+// PISO_update_velocity :5947-5980
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_correct_velocity(DomainK<R> dom, R dt, int tsn) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R rDiag = R(1) / dom.A[g];
+    R pg[D];
+    pressure_gradient<R, D>(dom, b, pos, pg);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        R u = -rDiag * pg[d];
+        if (tsn) u *= dt;
+        u += dom.prhs[g + (size_t)dom.N * d];
+        dom.vres[g + (size_t)dom.N * d] = u;
+    }
+}
+
+// Copy*Result*Blocks :6541-6733 -- one launch instead of a memcpy per block and component.
+template <typename R>
+__global__ __launch_bounds__(kCellsPerTile) void k_copy_result(DomainK<R> dom, int velocity, int to_blocks) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const int g = b->global_offset + f;
+    if (velocity) {
+        R *bv = reinterpret_cast<R *>(b->velocity);
+        for (int d = 0; d < dom.D; ++d) {
+            if (to_blocks) bv[(size_t)d * b->stride[3] + f] = dom.vres[(size_t)d * dom.N + g];
+            else dom.vres[(size_t)d * dom.N + g] = bv[(size_t)d * b->stride[3] + f];
+        }
+    } else {
+        R *bp = reinterpret_cast<R *>(b->pressure);
+        if (to_blocks) bp[f] = dom.pres[g];
+        else dom.pres[g] = bp[f];
+    }
+}
+
+}  // namespace pict
+
+using namespace pict;
+
+// ---------------------------------------------------------------------------
+// dispatch helpers
+// ---------------------------------------------------------------------------
+#define PICT_DISPATCH(atlas, BODY)                                                   \
+    do {                                                                             \
+        const int _D = (atlas)->desc.num_dims;                                       \
+        if ((atlas)->desc.dtype == PICT_DTYPE_F32) {                                 \
+            using R = float;                                                         \
+            if (_D == 1) { constexpr int D = 1; BODY; }                              \
+            else if (_D == 2) { constexpr int D = 2; BODY; }                         \
+            else { constexpr int D = 3; BODY; }                                      \
+        } else {                                                                     \
+            using R = double;                                                        \
+            if (_D == 1) { constexpr int D = 1; BODY; }                              \
+            else if (_D == 2) { constexpr int D = 2; BODY; }                         \
+            else { constexpr int D = 3; BODY; }                                      \
+        }                                                                            \
+    } while (0)
+
+#define PICT_LAUNCH_CELLS(atlas, stream, KERNEL, ...)                                           \
+    hipLaunchKernelGGL(KERNEL, dim3((atlas)->num_tiles), dim3(kCellsPerTile), 0,                \
+                       (hipStream_t)(stream), __VA_ARGS__)
+
+static pict_status check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("kernel launch failed: %s", hipGetErrorString(e));
+        return PICT_ERR_HIP;
+    }
+    return PICT_OK;
+}
+
+static pict_status validate_desc(const pict_domain_desc *d) {
+    PICT_CHECK_ARG(d != nullptr, "descriptor is NULL");
+    PICT_CHECK_ARG(d->num_dims >= 1 && d->num_dims <= 3, "num_dims must be 1..3 (got %d)", d->num_dims);
+    PICT_CHECK_ARG(d->num_blocks >= 1 && d->blocks != nullptr, "domain has no blocks");
+    PICT_CHECK_ARG(d->dtype == PICT_DTYPE_F32 || d->dtype == PICT_DTYPE_F64, "unknown dtype %d", d->dtype);
+    long long cells = 0;
+    for (int i = 0; i < d->num_blocks; ++i) {
+        const pict_block_desc &b = d->blocks[i];
+        PICT_CHECK_ARG(b.global_offset == cells, "block %d: global_offset %d, expected %lld", i, b.global_offset, cells);
+        for (int a = 0; a < 3; ++a) {
+            PICT_CHECK_ARG(b.size[a] >= 1, "block %d: invalid size", i);
+            if (a < d->num_dims) PICT_CHECK_ARG(b.size[a] > 2, "block %d: spatial sizes must be at least 3", i);
+        }
+        PICT_CHECK_ARG(b.stride[3] == b.size[0] * b.size[1] * b.size[2], "block %d: stride.w mismatch", i);
+        PICT_CHECK_ARG(b.velocity && b.pressure, "block %d: velocity/pressure missing", i);
+        PICT_CHECK_ARG(!b.has_transform || b.transform, "block %d: transform flag without data", i);
+        for (int f = 0; f < 2 * d->num_dims; ++f) {
+            const pict_boundary_desc &bd = b.bounds[f];
+            if (bd.type == PICT_BOUNDARY_CONNECTED) {
+                PICT_CHECK_ARG(bd.connected_block >= 0 && bd.connected_block < d->num_blocks,
+                               "block %d face %d: connected block out of range", i, f);
+            } else if (bd.type == PICT_BOUNDARY_FIXED) {
+                PICT_CHECK_ARG(bd.velocity != nullptr, "block %d face %d: fixed boundary without velocity", i, f);
+                PICT_CHECK_ARG(!bd.has_transform || bd.transform, "block %d face %d: transform flag without data", i, f);
+            } else if (bd.type != PICT_BOUNDARY_PERIODIC) {
+                set_error("block %d face %d: unsupported boundary type %d", i, f, bd.type);
+                return PICT_ERR_UNSUPPORTED;
+            }
+        }
+        cells += b.stride[3];
+    }
+    PICT_CHECK_ARG(cells == d->num_cells, "num_cells %d does not match blocks (%lld)", d->num_cells, cells);
+    return PICT_OK;
+}
+
+extern "C" {
+
+const char *pict_last_error(void) { return g_err.c_str(); }
+int32_t pict_abi_version(void) { return 1; }
+
+pict_status pict_atlas_create(const pict_domain_desc *desc, pict_atlas **out, void *stream) {
+    PICT_CHECK_ARG(out != nullptr, "out is NULL");
+    pict_status st = validate_desc(desc);
+    if (st) return st;
+    pict_atlas *a = new pict_atlas();
+    a->desc = *desc;
+    a->host_blocks.assign(desc->blocks, desc->blocks + desc->num_blocks);
+    a->desc.blocks = a->host_blocks.data();
+    std::vector<int32_t> tb, ts;
+    for (int i = 0; i < desc->num_blocks; ++i) {
+        const int cells = desc->blocks[i].stride[3];
+        for (int s = 0; s < cells; s += kCellsPerTile) {
+            tb.push_back(i);
+            ts.push_back(s);
+        }
+    }
+    a->num_tiles = (int)tb.size();
+    const size_t bytes_blocks = sizeof(pict_block_desc) * desc->num_blocks;
+    const size_t bytes_tiles = sizeof(int32_t) * tb.size();
+    a->dev_bytes = bytes_blocks + 2 * bytes_tiles;
+    hipError_t e = hipGetDevice(&a->device);
+    if (e == hipSuccess) e = hipMalloc(&a->dev, a->dev_bytes);
+    if (e != hipSuccess) {
+        set_error("atlas allocation failed: %s", hipGetErrorString(e));
+        delete a;
+        return PICT_ERR_HIP;
+    }
+    a->dev_blocks = (pict_block_desc *)a->dev;
+    a->dev_tile_block = (int32_t *)((char *)a->dev + bytes_blocks);
+    a->dev_tile_start = a->dev_tile_block + tb.size();
+    std::vector<char> staging(a->dev_bytes);
+    memcpy(staging.data(), desc->blocks, bytes_blocks);
+    memcpy(staging.data() + bytes_blocks, tb.data(), bytes_tiles);
+    memcpy(staging.data() + bytes_blocks + bytes_tiles, ts.data(), bytes_tiles);
+    e = hipMemcpyAsync(a->dev, staging.data(), a->dev_bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error("atlas upload failed: %s", hipGetErrorString(e));
+        (void)hipFree(a->dev);
+        delete a;
+        return PICT_ERR_HIP;
+    }
+    *out = a;
+    return PICT_OK;
+}
+
+static void atlas_free(pict_atlas *a) {
+    for (int i = 0; i < pict_atlas::kRing; ++i) {
+        if (a->ev[i]) (void)hipEventDestroy(a->ev[i]);
+        if (a->pinned[i]) (void)hipHostFree(a->pinned[i]);
+    }
+    if (a->dev) (void)hipFree(a->dev);
+    delete a;
+}
+
+pict_status pict_atlas_update(pict_atlas *a, const pict_domain_desc *desc, void *stream) {
+    PICT_CHECK_ARG(a != nullptr, "atlas is NULL");
+    pict_status st = validate_desc(desc);
+    if (st) return st;
+    PICT_CHECK_ARG(desc->num_blocks == a->desc.num_blocks && desc->num_cells == a->desc.num_cells &&
+                       desc->num_dims == a->desc.num_dims && desc->dtype == a->desc.dtype,
+                   "domain structure changed; create a new atlas");
+    for (int i = 0; i < desc->num_blocks; ++i)
+        for (int k = 0; k < 4; ++k)
+            PICT_CHECK_ARG(desc->blocks[i].size[k] == a->host_blocks[i].size[k], "block %d size changed", i);
+    const size_t bytes = sizeof(pict_block_desc) * desc->num_blocks;
+    const bool same_blocks = memcmp(desc->blocks, a->host_blocks.data(), bytes) == 0;
+    a->desc = *desc;
+    if (!same_blocks) {
+        a->host_blocks.assign(desc->blocks, desc->blocks + desc->num_blocks);
+        // Stream-ordered upload from a pinned ring slot: kernels queued earlier on
+        // `stream` still read the previous table; the slot is reused only after
+        // its previous copy has completed.
+        const int slot = a->ring_pos;
+        a->ring_pos = (a->ring_pos + 1) % pict_atlas::kRing;
+        if (!a->pinned[slot]) {
+            PICT_HIP_TRY(hipHostMalloc(&a->pinned[slot], bytes, hipHostMallocDefault));
+            PICT_HIP_TRY(hipEventCreateWithFlags(&a->ev[slot], hipEventDisableTiming));
+        } else {
+            PICT_HIP_TRY(hipEventSynchronize(a->ev[slot]));
+        }
+        memcpy(a->pinned[slot], a->host_blocks.data(), bytes);
+        PICT_HIP_TRY(hipMemcpyAsync(a->dev_blocks, a->pinned[slot], bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+        PICT_HIP_TRY(hipEventRecord(a->ev[slot], (hipStream_t)stream));
+    }
+    a->desc.blocks = a->host_blocks.data();
+    return PICT_OK;
+}
+
+pict_status pict_atlas_destroy(pict_atlas *a) {
+    if (!a) return PICT_OK;
+    // make sure no queued upload still reads a pinned slot
+    for (int i = 0; i < pict_atlas::kRing; ++i)
+        if (a->ev[i]) (void)hipEventSynchronize(a->ev[i]);
+    atlas_free(a);
+    return PICT_OK;
+}
+
+pict_status pict_build_csr_pattern(const pict_atlas *a, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.csr_row && a->desc.csr_index, "pattern buffers missing");
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_csr_pattern<R, D>), a->view<R>()));
+    return check_launch();
+}
+
+static pict_status check_non_ortho(int flags) {
+    if (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX | PICT_NON_ORTHO_DIRECT_RHS |
+                 PICT_NON_ORTHO_DIAGONAL_RHS | PICT_NON_ORTHO_DIAGONAL_MATRIX)) {
+        set_error("non-orthogonal correction terms (flags %d) are not implemented by libpict_hip", flags);
+        return PICT_ERR_UNSUPPORTED;
+    }
+    return PICT_OK;
+}
+
+pict_status pict_setup_advection_matrix(const pict_atlas *a, double dt, int32_t flags, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.C_value && a->desc.A, "C/A missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix<R, D>), a->view<R>(), (R)dt));
+    return check_launch();
+}
+
+pict_status pict_setup_advection_velocity(const pict_atlas *a, double dt, int32_t flags, int32_t apply_pg,
+                                          void *stream) {
+    PICT_CHECK_ARG(a && a->desc.velocity_rhs, "velocity_rhs missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_rhs<R, D>), a->view<R>(), (R)dt, (int)apply_pg));
+    return check_launch();
+}
+
+pict_status pict_setup_pressure_matrix(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
+                                       void *stream) {
+    (void)dt;
+    PICT_CHECK_ARG(a && a->desc.P_value && a->desc.A, "P/A missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    if (use_face_transform) {
+        set_error("useFaceTransform is not implemented by libpict_hip");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix<R, D>), a->view<R>()));
+    return check_launch();
+}
+
+pict_status pict_setup_pressure_rhs_div(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
+                                        int32_t tsn, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.pressure_rhs && a->desc.pressure_rhs_div, "pressure rhs missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    if (use_face_transform) {
+        set_error("useFaceTransform is not implemented by libpict_hip");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_divergence<R, D>), a->view<R>(),
+                                       (const R *)a->desc.pressure_rhs, (R *)a->desc.pressure_rhs_div, (R)dt,
+                                       (int)tsn));
+    return check_launch();
+}
+
+pict_status pict_setup_pressure_rhs(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
+                                    int32_t tsn, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.pressure_rhs && a->desc.C_value && a->desc.velocity_result, "pressure rhs inputs missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    if (use_face_transform) {
+        set_error("useFaceTransform is not implemented by libpict_hip");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs<R, D>), a->view<R>(), (R)dt));
+    st = check_launch();
+    if (st) return st;
+    return pict_setup_pressure_rhs_div(a, dt, flags, use_face_transform, tsn, stream);
+}
+
+pict_status pict_setup_pressure_correction(const pict_atlas *a, double dt, int32_t flags,
+                                           int32_t use_face_transform, int32_t tsn, void *stream) {
+    pict_status st = pict_setup_pressure_matrix(a, dt, flags, use_face_transform, stream);
+    if (st) return st;
+    return pict_setup_pressure_rhs(a, dt, flags, use_face_transform, tsn, stream);
+}
+
+pict_status pict_correct_velocity(const pict_atlas *a, double dt, int32_t version, int32_t tsn, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.velocity_result && a->desc.pressure_rhs && a->desc.A, "inputs missing");
+    if (!(version == 0 || version == 1)) {
+        set_error("CorrectVelocity version %d is not implemented by libpict_hip (versions 0/1: finite differences)",
+                  version);
+        return PICT_ERR_UNSUPPORTED;
+    }
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_correct_velocity<R, D>), a->view<R>(), (R)dt, (int)tsn));
+    return check_launch();
+}
+
+pict_status pict_compute_velocity_divergence(const pict_atlas *a, void *out, void *stream) {
+    PICT_CHECK_ARG(a && out, "output missing");
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_divergence<R, D>), a->view<R>(), (const R *)nullptr, (R *)out,
+                                       R(1), 0));
+    return check_launch();
+}
+
+static pict_status copy_result(const pict_atlas *a, int velocity, int to_blocks, void *stream) {
+    PICT_CHECK_ARG(a && (velocity ? a->desc.velocity_result : a->desc.pressure_result), "result missing");
+    if (a->desc.dtype == PICT_DTYPE_F32)
+        PICT_LAUNCH_CELLS(a, stream, (k_copy_result<float>), a->view<float>(), velocity, to_blocks);
+    else
+        PICT_LAUNCH_CELLS(a, stream, (k_copy_result<double>), a->view<double>(), velocity, to_blocks);
+    return check_launch();
+}
+pict_status pict_copy_velocity_result_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 1, 1, s); }
+pict_status pict_copy_velocity_result_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 1, 0, s); }
+pict_status pict_copy_pressure_result_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 0, 1, s); }
+pict_status pict_copy_pressure_result_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 0, 0, s); }
+
+}  // extern "C"

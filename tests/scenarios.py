@@ -1,0 +1,147 @@
+"""Small seeded domains exercising every boundary/connection/transform path.
+
+Restates the kinds of setups in the reference's tests/test_setups.py:30-496
+(1/2/4-block, periodic/closed, uniform/stretched grids, 2D and 3D, connections
+with permuted and inverted axes) at sizes the oracle finishes in milliseconds.
+"""
+import math
+
+import torch
+
+from pict_amd import PISOtorch as P
+from pict_amd.configs import uniform_coords
+
+
+def _domain(dims, visc, dtype, device):
+    return P.Domain(dims, torch.tensor([visc], dtype=dtype), "test", dtype, device, passiveScalarChannels=0,
+                    _allow_cpu=(str(device) == "cpu"))
+
+
+def _coords_block(d, coords, name="Block"):
+    """Block from vertex coordinates; on a host (oracle) domain the metrics come from the oracle."""
+    if str(d.getDevice()) == "cpu":
+        from oracle import oracle as O
+        c64 = coords.double()
+        b = d.CreateBlock(velocity=torch.zeros([1, coords.size(1)] + [s - 1 for s in coords.shape[2:]],
+                                               dtype=coords.dtype), name=name)
+        b.setTransform(O.CoordsToTransforms(c64).to(coords.dtype),
+                       O.CoordsToFaceTransforms(c64).to(coords.dtype) if coords.size(1) > 1 else None)
+        return b
+    return d.CreateBlock(vertexCoordinates=coords, name=name)
+
+
+def _rand(shape, gen, dtype, device, scale=1.0):
+    return (torch.rand(shape, generator=gen, dtype=torch.float64) * 2 - 1).mul(scale).to(dtype=dtype,
+                                                                                     device=device).contiguous()
+
+
+def stretched_coords(nx, ny, dtype, device, ratio=1.1, x0=0.0, y0=0.0):
+    """Orthogonal, non-uniform 2D grid: geometric spacing in x and y."""
+    def axis(n, r, o):
+        w = torch.tensor([r ** i for i in range(n)], dtype=torch.float64)
+        return torch.cat([torch.zeros(1, dtype=torch.float64), torch.cumsum(w / w.sum(), 0)]) + o
+    xs, ys = axis(nx, ratio, x0), axis(ny, 1.0 / ratio, y0)
+    gy, gx = torch.meshgrid(ys, xs, indexing="ij")
+    return torch.stack([gx, gy])[None].to(dtype=dtype, device=device).contiguous()
+
+
+def cavity(dtype=torch.float64, device="cuda", n=8, coords=True, seed=0, random_init=True):
+    d = _domain(2, 0.05, dtype, device)
+    if coords:
+        b = _coords_block(d, uniform_coords(n, n, dtype=dtype, device=device))
+    else:
+        b = d.CreateBlockWithSize(P.Int4(n, n, 1))
+    b.CloseAllBoundaries()
+    b.CloseBoundary("+y", velocity=torch.tensor([[1.0, 0.0]], dtype=dtype, device=device))
+    if random_init:
+        g = torch.Generator().manual_seed(seed)
+        b.setVelocity(_rand(b.velocity.shape, g, dtype, device, 0.3))
+    d.PrepareSolve()
+    return d
+
+
+def periodic_two_block(dtype=torch.float64, device="cuda", nx=10, ny=6, coords=False, seed=1):
+    """2 blocks connected on both y faces, periodic in x (the 2-block periodic box)."""
+    d = _domain(2, 0.02, dtype, device)
+    g = torch.Generator().manual_seed(seed)
+    bs = []
+    for i in range(2):
+        if coords:
+            b = _coords_block(d, uniform_coords(nx, ny, lx=1.0, ly=ny / nx, y0=i * ny / nx, dtype=dtype,
+                                                device=device))
+        else:
+            b = d.CreateBlockWithSize(P.Int4(nx, ny, 1))
+        b.MakePeriodic("x")
+        b.setVelocity(_rand(b.velocity.shape, g, dtype, device, 0.5))
+        bs.append(b)
+    bs[0].ConnectBlock("+y", bs[1], "-y", "-x")
+    bs[1].ConnectBlock("+y", bs[0], "-y", "-x")
+    d.PrepareSolve()
+    return d
+
+
+def rotated_connection(dtype=torch.float64, device="cuda", seed=2):
+    """Two blocks of different sizes joined with permuted and inverted axes: block0 +x <-> block1 -y, with block0's y
+    axis running against block1's x axis; stretched orthogonal metrics; a moving wall, a varying wall velocity, a
+    per-cell viscosity and a velocity source."""
+    d = _domain(2, 0.03, dtype, device)
+    g = torch.Generator().manual_seed(seed)
+    b0 = _coords_block(d, stretched_coords(7, 5, dtype, device, 1.15))
+    b1 = _coords_block(d, stretched_coords(5, 6, dtype, device, 0.9, x0=2.0))
+    b0.CloseAllBoundaries()
+    b1.CloseAllBoundaries()
+    b0.ConnectBlock("+x", b1, "-y", "+x")  # inverted axis
+    b0.CloseBoundary("-y", velocity=torch.tensor([[0.7, 0.0]], dtype=dtype, device=device))
+    vvar = _rand([1, 2, 1, 5], g, dtype, device, 0.5)
+    vvar[:, 1] = 0  # tangential wall motion: zero net boundary flux (compatible pressure system)
+    b1.CloseBoundary("+y", velocity=vvar)
+    for b in (b0, b1):
+        b.setVelocity(_rand(b.velocity.shape, g, dtype, device, 0.4))
+        b.setPressure(_rand(b.pressure.shape, g, dtype, device, 0.2))
+    b0.setViscosity((torch.rand([1, 1, 5, 7], generator=g, dtype=torch.float64) * 0.05 + 0.01).to(dtype=dtype,
+                                                                                                   device=device))
+    b1.setVelocitySource(_rand([1, 2], g, dtype, device, 0.3))
+    d.PrepareSolve()
+    return d
+
+
+def box3d(dtype=torch.float64, device="cuda", seed=3, size=(5, 4, 6)):
+    """3D block: periodic in z, closed in x and y with a moving +y wall, a varying velocity source, transforms."""
+    d = _domain(3, 0.04, dtype, device)
+    g = torch.Generator().manual_seed(seed)
+    nx, ny, nz = size
+    b = _coords_block(d, uniform_coords(nx, ny, nz, lx=1.0, ly=0.8, lz=1.2, dtype=dtype, device=device))
+    b.CloseAllBoundaries()
+    b.MakePeriodic("z")
+    b.CloseBoundary("+y", velocity=torch.tensor([[0.5, 0.0, 0.2]], dtype=dtype, device=device))
+    b.setVelocity(_rand(b.velocity.shape, g, dtype, device, 0.3))
+    b.setVelocitySource(_rand(b.velocity.shape, g, dtype, device, 0.1))
+    d.PrepareSolve()
+    return d
+
+
+def three_d_connected(dtype=torch.float64, device="cuda", seed=4):
+    """Two 3D blocks joined x-face to z-face with swapped tangential axes."""
+    d = _domain(3, 0.05, dtype, device)
+    g = torch.Generator().manual_seed(seed)
+    b0 = d.CreateBlockWithSize(P.Int4(4, 3, 5))
+    b1 = d.CreateBlockWithSize(P.Int4(5, 3, 4))
+    b0.CloseAllBoundaries()
+    b1.CloseAllBoundaries()
+    # b0 +x face (y:3, z:5) <-> b1 -z face (x:5, y:3); b0's y aligns to b1's y, b0's z to b1's x (inverted)
+    b0.ConnectBlock("+x", b1, "-z", "-y", "+x")
+    for b in (b0, b1):
+        b.setVelocity(_rand(b.velocity.shape, g, dtype, device, 0.3))
+    d.PrepareSolve()
+    return d
+
+
+ALL = {
+    "cavity": cavity,
+    "cavity_nocoords": lambda **kw: cavity(coords=False, **kw),
+    "periodic_two_block": periodic_two_block,
+    "periodic_two_block_coords": lambda **kw: periodic_two_block(coords=True, **kw),
+    "rotated_connection": rotated_connection,
+    "box3d": box3d,
+    "three_d_connected": three_d_connected,
+}
