@@ -258,6 +258,12 @@ pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, 
                               const pict_solve_opts *opts, pict_solve_info *infos_out,
                               void *stream);
 
+/* Diagnostics (no counterpart in the reference): when enabled, the solver
+   records HIP events around its kernel launch on the launch stream; the
+   elapsed time of the last solve's kernel is then available (milliseconds). */
+pict_status pict_set_kernel_timing(int32_t enable);
+double      pict_last_solve_kernel_ms(void);
+
 /* SparseOuterProduct (PISOtorch.cpp:564; cg_solver_kernel.cu:869-908):
    out_value[k] = a[i] * b[j] for every pattern entry k = (i, j). */
 pict_status pict_sparse_outer_product(const void *a, const void *b, const pict_csr_desc *pattern,

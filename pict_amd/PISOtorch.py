@@ -1906,6 +1906,18 @@ def _solve_opts(maxIterations, tolerance, convergenceCriterion, useBiCG, matrixR
     return o
 
 
+# Solver diagnostics: when enabled, every solve appends
+# (kind, n, nnz, nrhs, dtype bytes, [LinearSolverResultInfo], kernel ms) -- kernel time from HIP events
+# recorded by the library around its launch on the launch stream.
+solve_log = None
+
+
+def enable_solve_log(enable=True):
+    global solve_log
+    solve_log = [] if enable else None
+    _abi.check(_abi.load_library().pict_set_kernel_timing(1 if enable else 0))
+
+
 def _solve(A, RHS, x, opts):
     n = A.getRows()
     _check(RHS.is_cuda and RHS.is_contiguous(), "RHS must be a contiguous CUDA tensor")
@@ -1923,8 +1935,12 @@ def _solve(A, RHS, x, opts):
     _abi.check(lib.pict_solve_linear(ctypes.byref(desc), ctypes.c_void_p(RHS.data_ptr()),
                                      ctypes.c_void_p(x.data_ptr()), nb, ctypes.byref(opts), infos,
                                      _stream(RHS.device)), "SolveLinear")
-    return [LinearSolverResultInfo(i.final_residual, i.used_iterations, i.converged, i.is_finite_residual)
-            for i in infos]
+    res = [LinearSolverResultInfo(i.final_residual, i.used_iterations, i.converged, i.is_finite_residual)
+           for i in infos]
+    if solve_log is not None:
+        solve_log.append(("BiCG" if opts.use_bicg else "CG", n, A.getSize(), nb, RHS.element_size(), res,
+                          lib.pict_last_solve_kernel_ms()))
+    return res
 
 
 def SolveLinear(A, RHS, x, maxIterations=1000, tolerance=1e-8, convergenceCriterion=ConvergenceCriterion.NORM2_NORMALIZED,

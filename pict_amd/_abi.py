@@ -159,6 +159,8 @@ SIGNATURES = {
     "pict_copy_pressure_result_from_blocks": (c_int32, [_P, _P]),
     "pict_solve_linear": (c_int32, [POINTER(CsrDesc), _P, _P, c_int32, POINTER(SolveOpts), POINTER(SolveInfo), _P]),
     "pict_sparse_outer_product": (c_int32, [_P, _P, POINTER(CsrDesc), _P, _P]),
+    "pict_set_kernel_timing": (c_int32, [c_int32]),
+    "pict_last_solve_kernel_ms": (c_double, []),
     "pict_coords_to_transforms": (c_int32, [_P, c_int32, POINTER(c_int32), c_int32, _P, _P]),
     "pict_coords_to_face_transforms": (c_int32, [_P, c_int32, POINTER(c_int32), c_int32, _P, _P]),
 }

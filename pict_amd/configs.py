@@ -59,3 +59,38 @@ def periodic_box_2block(nx=256, ny_block=128, length=2 * math.pi, viscosity=1e-3
     blocks[1].ConnectBlock("+y", blocks[0], "-y", "-x")
     dom.PrepareSolve()
     return dom
+
+
+def spectral_velocity_2d(n=256, kmin=1, kmax=32, slope=-5.0 / 3.0, rms=1.0, seed=0):
+    """Divergence-free random 2D velocity on an n x n periodic grid from a streamfunction with random phases and
+    energy spectrum E(k) ~ k^slope for kmin <= |k| <= kmax, scaled to the given RMS speed.
+    Returns u [2, n, n] (float64, [component, y, x])."""
+    g = torch.Generator().manual_seed(seed)
+    k = torch.fft.fftfreq(n, d=1.0 / n, dtype=torch.float64)
+    ky, kx = torch.meshgrid(k, k, indexing="ij")
+    kk = torch.sqrt(kx ** 2 + ky ** 2)
+    band = (kk >= kmin) & (kk <= kmax)
+    amp = torch.zeros_like(kk)
+    # |u_k|^2 ~ E(k)/k (shell density), |psi_k| = |u_k| / k
+    amp[band] = torch.sqrt(kk[band] ** slope / kk[band]) / kk[band]
+    phase = torch.rand((n, n), generator=g, dtype=torch.float64) * 2 * math.pi
+    psi_h = amp * torch.exp(1j * phase)
+    u = torch.fft.ifft2(1j * ky * psi_h).real
+    v = torch.fft.ifft2(-1j * kx * psi_h).real
+    vel = torch.stack([u, v])
+    vel *= rms / torch.sqrt((vel ** 2).sum(0).mean())
+    return vel
+
+
+def periodic_box_c3(n=256, viscosity=1e-3, dtype=torch.float32, device="cuda", seed=0):
+    """Differentiation configuration: 2-block periodic box [0,2pi]^2 of n x n/2 blocks, spectral initial velocity
+    (k^-5/3, k in 1..32, RMS 1).  Returns (domain, time_step) with dt = 0.25 h / max|u0|."""
+    dom = periodic_box_2block(n, n // 2, 2 * math.pi, viscosity, dtype, device, coords=True, name="box_c3")
+    vel = spectral_velocity_2d(n, seed=seed)
+    half = n // 2
+    for i, b in enumerate(dom.blocks):
+        b.setVelocity(vel[None, :, i * half:(i + 1) * half, :].to(dtype=dtype, device=device).contiguous())
+    dom.UpdateDomainData()
+    h = 2 * math.pi / n
+    dt = 0.25 * h / torch.linalg.vector_norm(vel, dim=0).max().item()
+    return dom, dt

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -34,12 +35,18 @@ constexpr int kSlots = 8;              // FP64 partial slots per right-hand side
 constexpr unsigned kSpinLimit = 1u << 23;
 constexpr int kMaxRhs = 4;
 
+// Control block, one 128-byte line per hot word (device memory, zeroed before launch).
 struct Ctl {
-    unsigned count;
-    unsigned gen;
-    int err;
-    int zero_all;
+    unsigned count;              // top-level arrivals (one per group)
+    unsigned pad0[31];
+    unsigned gen;                // barrier generation
+    unsigned pad1[31];
+    int err;                     // a wait exceeded its spin limit
+    int zero_all;                // every right-hand side is zero
+    unsigned pad2[30];
+    unsigned group_count[8][32]; // per-group arrivals
 };
+constexpr int kGroups = 8;       // arrival tree fan-in (one group per XCD when blocks are dealt round-robin)
 
 struct RhsState {
     double rho, rho_prev, alpha, omega, beta;
@@ -102,43 +109,64 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], const bool (&is_max
 // --------------------------------------------------------------------------
 // grid reduce-barrier
 // --------------------------------------------------------------------------
-__device__ __forceinline__ bool arrive(Ctl *ctl, unsigned nwg) {
+// Two-level arrival tree: workgroups of one group (blockIdx % G) count on their
+// group's line; the last of each group arrives at the top counter; the last top
+// arrival is the master.  Release fences publish this workgroup's writes before
+// it arrives; the master and the waiters take one acquire fence after the wait
+// (never inside the spin, which only does relaxed loads).
+__device__ __forceinline__ bool arrive(Ctl *ctl, unsigned nwg, int &s_err) {
     __shared__ int s_last;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(&ctl->count, 1u);
-        s_last = prev == nwg - 1;
-        __threadfence();
+        const unsigned G = nwg < kGroups ? nwg : kGroups;
+        const unsigned g = blockIdx.x % G;
+        const unsigned gsize = (nwg - g + G - 1) / G;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned prev = __hip_atomic_fetch_add(&ctl->group_count[g][0], 1u, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        int last = 0;
+        if (prev == gsize - 1) {
+            __hip_atomic_store(&ctl->group_count[g][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            const unsigned top = __hip_atomic_fetch_add(&ctl->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (top == G - 1) {
+                __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                last = 1;
+            }
+        }
+        s_last = last;
+        s_err = 0;
     }
     __syncthreads();
     return s_last != 0;
 }
+// master: publish the new state and open the next generation
 __device__ __forceinline__ void release(Ctl *ctl) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
 }
-__device__ __forceinline__ void wait_gen(Ctl *ctl, unsigned my_gen) {
+__device__ __forceinline__ void wait_gen(Ctl *ctl, unsigned my_gen, int &s_err) {
     if (threadIdx.x == 0) {
         unsigned spins = 0;
-        while (__hip_atomic_load(&ctl->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
-            __builtin_amdgcn_s_sleep(2);
+        int err = 0;
+        while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
+            __builtin_amdgcn_s_sleep(1);
             if (++spins > kSpinLimit) {
-                atomicExch(&ctl->err, 1);
+                __hip_atomic_store(&ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                err = 1;
                 break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (!err) err = __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_err = err;
     }
     __syncthreads();
-}
-__device__ __forceinline__ bool failed(Ctl *ctl) {
-    return __hip_atomic_load(&ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
 // slot layout of the partial sums (per workgroup and right-hand side)
@@ -219,15 +247,15 @@ __device__ __forceinline__ void acc_res(double (&acc)[kMaxRhs][NS], int k, R rv)
 #define PICT_REDUCE_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                  \
     do {                                                                                  \
         flush<R, NS>(a, ACC, IDS, ISMAX);                                                 \
-        if (arrive(a.ctl, a.nwg)) {                                                       \
+        if (arrive(a.ctl, a.nwg, s_err)) {                                                \
             master_reduce<NS>(a.partials, a.nwg, a.nrhs, IDS, ISMAX, red);                \
             if (threadIdx.x == 0) { UPDATE; }                                             \
             release(a.ctl);                                                               \
         } else {                                                                          \
-            wait_gen(a.ctl, gen);                                                         \
+            wait_gen(a.ctl, gen, s_err);                                                  \
         }                                                                                 \
         ++gen;                                                                            \
-        if (failed(a.ctl)) return;                                                        \
+        if (s_err) return;                                                                \
     } while (0)
 
 // ==========================================================================
@@ -236,6 +264,7 @@ __device__ __forceinline__ void acc_res(double (&acc)[kMaxRhs][NS], int k, R rv)
 template <typename R>
 __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
     __shared__ double red[kMaxRhs * kSlots];
+    __shared__ int s_err;
     const int n = a.n;
     const size_t stride = (size_t)a.nwg * kSolveThreads;
     const size_t i0 = (size_t)blockIdx.x * kSolveThreads + threadIdx.x;
@@ -430,11 +459,261 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
 }
 
 // ==========================================================================
+// CG, single-reduction form (Chronopoulos & Gear 1989): the same Krylov
+// iterates as cgSolveGPU (cg_solver_kernel.cu:114-456) in exact arithmetic,
+// with (r,r) and (Ar,r) reduced together, so an iteration costs one
+// reduce-barrier.  Per iteration i, for every own row m:
+//   p = r + beta p;  s = w + beta s;  x += alpha p;  r' = r - alpha s;
+//   w' = A r'  (r'_j of the neighbours evaluated on the fly from r, w, s of the
+//   previous iteration, bitwise equal to the owner's value through explicit fma);
+//   gamma' = (r', r'), delta' = (w', r');
+//   beta' = gamma'/gamma, alpha' = gamma' / (delta' - beta' gamma' / alpha).
+// r, w, s are double-buffered (neighbours read the previous values); p and x
+// are only touched by their owner.
+// ==========================================================================
+template <typename R>
+__global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
+    __shared__ double red[kMaxRhs * kSlots];
+    __shared__ int s_err;
+    const int n = a.n;
+    const size_t stride = (size_t)a.nwg * kSolveThreads;
+    const size_t i0 = (size_t)blockIdx.x * kSolveThreads + threadIdx.x;
+    unsigned gen = 0;
+    R *rb[2] = {a.r, a.rhat}, *wb[2] = {a.v0, a.v1}, *sb[2] = {a.s, a.t};
+    R *p = a.p0;
+    constexpr int IDS_R[4] = {S_RR, S_ASUM, S_AMAX, S_BMAX};
+    constexpr bool MAX_R[4] = {false, false, true, true};
+    constexpr int IDS_G[5] = {S_RR, S_ASUM, S_AMAX, S_D0, S_D1};
+    constexpr bool MAX_G[5] = {false, false, true, false, false};
+    int cur = 0;  // buffer holding r_i, w_i, s_{i-1}
+
+    // phase I0: r0 = b - A x0
+    {
+        double acc[kMaxRhs][4];
+        zero_acc(acc);
+        for (size_t i = i0; i < (size_t)n; i += stride) {
+            const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                if (k >= a.nrhs) break;
+                const size_t o = (size_t)k * n;
+                R s = 0;
+                for (int e = rs; e < re; ++e) s += mat_val(a, e) * a.x[o + a.col[e]];
+                const R bv = a.b[o + i];
+                const R rv = bv - s;
+                rb[0][o + i] = rv;
+                p[o + i] = 0;
+                sb[0][o + i] = 0;
+                acc_res(acc, k, rv);
+                acc[k][3] = fmax(acc[k][3], fabs((double)bv));
+            }
+        }
+        PICT_REDUCE_BARRIER(4, acc, IDS_R, MAX_R, {
+            bool all_zero = true;
+            for (int k = 0; k < a.nrhs; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
+            const int zero_all = (a.zero_shortcut && all_zero) ? 1 : 0;
+            a.ctl->zero_all = zero_all;
+            for (int k = 0; k < a.nrhs; ++k) {
+                RhsState &st = a.st[k];
+                st.rho = red[k * kSlots + S_RR];
+                st.beta = 0;
+                st.active = zero_all ? 0 : 1;
+                st.converged = zero_all;
+                st.finite = 1;
+                st.used = -1;
+                st.final_res = 0;
+                st.best_crit = 0;
+                st.last_crit = 0;
+                st.best_it = -1;
+                st.rising = 0;
+                st.save_best = 0;
+                st.restore_best = 0;
+            }
+        });
+    }
+    if (a.ctl->zero_all) {
+        for (size_t i = i0; i < (size_t)n; i += stride)
+            for (int k = 0; k < a.nrhs; ++k) a.x[(size_t)k * n + i] = 0;
+        return;
+    }
+    // phase I1: w0 = A r0; gamma0 = (r0,r0), delta0 = (w0,r0); alpha0 = gamma0/delta0
+    {
+        double acc[kMaxRhs][5];
+        zero_acc(acc);
+        bool act[kMaxRhs];
+#pragma unroll
+        for (int k = 0; k < kMaxRhs; ++k) act[k] = k < a.nrhs && a.st[k].active;
+        for (size_t i = i0; i < (size_t)n; i += stride) {
+            const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                if (!act[k]) continue;
+                const size_t o = (size_t)k * n;
+                R w = 0;
+                for (int e = rs; e < re; ++e) w += mat_val(a, e) * rb[0][o + a.col[e]];
+                wb[0][o + i] = w;
+                const R rv = rb[0][o + i];
+                acc_res(acc, k, rv);
+                acc[k][3] += (double)w * (double)rv;
+            }
+        }
+        PICT_REDUCE_BARRIER(5, acc, IDS_G, MAX_G, {
+            for (int k = 0; k < a.nrhs; ++k) {
+                RhsState &st = a.st[k];
+                if (!st.active) continue;
+                st.rho = red[k * kSlots + S_RR];
+                st.alpha = st.rho / red[k * kSlots + S_D0];
+                st.beta = 0;
+            }
+        });
+    }
+
+    for (int it = 0; it < a.maxit; ++it) {
+        if (a.reset_steps > 0 && (it + 1) % a.reset_steps == 0) {
+            // ---- residual reset (cg_solver_kernel.cu:266-287): r = b - A x, w = A r, restart ----
+            {
+                bool act[kMaxRhs];
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) act[k] = k < a.nrhs && a.st[k].active;
+                double acc[kMaxRhs][4];
+                zero_acc(acc);
+                for (size_t i = i0; i < (size_t)n; i += stride) {
+                    const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+                    for (int k = 0; k < kMaxRhs; ++k) {
+                        if (!act[k]) continue;
+                        const size_t o = (size_t)k * n;
+                        R s = 0;
+                        for (int e = rs; e < re; ++e) s += mat_val(a, e) * a.x[o + a.col[e]];
+                        rb[cur][o + i] = a.b[o + i] - s;
+                        p[o + i] = 0;
+                        sb[cur][o + i] = 0;
+                    }
+                }
+                PICT_REDUCE_BARRIER(4, acc, IDS_R, MAX_R, {});
+            }
+            {
+                bool act[kMaxRhs];
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) act[k] = k < a.nrhs && a.st[k].active;
+                double acc[kMaxRhs][5];
+                zero_acc(acc);
+                for (size_t i = i0; i < (size_t)n; i += stride) {
+                    const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+                    for (int k = 0; k < kMaxRhs; ++k) {
+                        if (!act[k]) continue;
+                        const size_t o = (size_t)k * n;
+                        R w = 0;
+                        for (int e = rs; e < re; ++e) w += mat_val(a, e) * rb[cur][o + a.col[e]];
+                        wb[cur][o + i] = w;
+                        const R rv = rb[cur][o + i];
+                        acc_res(acc, k, rv);
+                        acc[k][3] += (double)w * (double)rv;
+                    }
+                }
+                PICT_REDUCE_BARRIER(5, acc, IDS_G, MAX_G, {
+                    for (int k = 0; k < a.nrhs; ++k) {
+                        RhsState &st = a.st[k];
+                        if (!st.active) continue;
+                        st.rho = red[k * kSlots + S_RR];
+                        st.alpha = st.rho / red[k * kSlots + S_D0];
+                        st.beta = 0;
+                    }
+                });
+            }
+        }
+        const int nxt = cur ^ 1;
+        // ---- the iteration: one pass, one reduce-barrier ----
+        {
+            bool act[kMaxRhs], save[kMaxRhs];
+            R alpha[kMaxRhs], beta[kMaxRhs];
+#pragma unroll
+            for (int k = 0; k < kMaxRhs; ++k) {
+                act[k] = k < a.nrhs && a.st[k].active;
+                save[k] = act[k] && a.st[k].save_best;
+                alpha[k] = act[k] ? (R)a.st[k].alpha : R(0);
+                beta[k] = act[k] ? (R)a.st[k].beta : R(0);
+            }
+            const R *r = rb[cur], *w = wb[cur], *so = sb[cur];
+            R *rn = rb[nxt], *wn = wb[nxt], *sn = sb[nxt];
+            double acc[kMaxRhs][5];
+            zero_acc(acc);
+            for (size_t i = i0; i < (size_t)n; i += stride) {
+                const int rs = a.row[i], re = a.row[i + 1];
+#pragma unroll
+                for (int k = 0; k < kMaxRhs; ++k) {
+                    if (!act[k]) continue;
+                    const size_t o = (size_t)k * n;
+                    if (save[k]) a.best[o + i] = a.x[o + i];
+                    const R pm = fma(beta[k], p[o + i], r[o + i]);
+                    const R sm = fma(beta[k], so[o + i], w[o + i]);
+                    p[o + i] = pm;
+                    sn[o + i] = sm;
+                    a.x[o + i] += alpha[k] * pm;
+                    const R rm = fma(-alpha[k], sm, r[o + i]);
+                    rn[o + i] = rm;
+                    R wv = 0;
+                    for (int e = rs; e < re; ++e) {
+                        const size_t c = o + a.col[e];
+                        wv += mat_val(a, e) * fma(-alpha[k], fma(beta[k], so[c], w[c]), r[c]);
+                    }
+                    wn[o + i] = wv;
+                    acc_res(acc, k, rm);
+                    acc[k][3] += (double)wv * (double)rm;
+                }
+            }
+            PICT_REDUCE_BARRIER(5, acc, IDS_G, MAX_G, {
+                for (int k = 0; k < a.nrhs; ++k) {
+                    RhsState &st = a.st[k];
+                    if (!st.active) continue;
+                    st.save_best = 0;
+                    const double gam = red[k * kSlots + S_RR];
+                    const double crit = criterion(a.conv, gam, red[k * kSlots + S_ASUM], red[k * kSlots + S_AMAX], n);
+                    if (!isfinite(crit)) {
+                        st.converged = 0; st.used = it; st.final_res = crit; st.finite = 0; st.active = 0;
+                        continue;
+                    }
+                    if (a.return_best) {
+                        if (it == 0 || crit < st.best_crit) { st.best_crit = crit; st.best_it = it; st.save_best = 1; }
+                        if (it > 0 && crit >= st.last_crit) ++st.rising; else st.rising = 0;
+                        st.last_crit = crit;
+                    }
+                    st.used = it;
+                    st.final_res = crit;
+                    if (crit < a.tol) { st.converged = 1; st.finite = 1; st.active = 0; continue; }
+                    if (a.return_best && (it == a.maxit - 1 || st.rising >= 100)) {
+                        st.restore_best = st.best_it != it;
+                        st.converged = 0; st.used = st.best_it; st.final_res = st.best_crit; st.finite = 1;
+                        st.active = 0;
+                        continue;
+                    }
+                    const double beta_n = gam / st.rho;
+                    st.alpha = gam / (red[k * kSlots + S_D0] - beta_n * gam / st.alpha);
+                    st.beta = beta_n;
+                    st.rho = gam;
+                }
+            });
+        }
+        cur = nxt;
+        bool any = false;
+        for (int k = 0; k < a.nrhs; ++k) any = any || a.st[k].active;
+        if (!any) break;
+    }
+    for (int k = 0; k < a.nrhs; ++k) {
+        if (!a.st[k].restore_best) continue;
+        const size_t o = (size_t)k * n;
+        for (size_t i = i0; i < (size_t)n; i += stride) a.x[o + i] = a.best[o + i];
+    }
+}
+
+// ==========================================================================
 // BiCGStab without preconditioner (bicgstabSolveGPU, bicgstab_solver_kernel.cu:222-365)
 // ==========================================================================
 template <typename R>
 __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
     __shared__ double red[kMaxRhs * kSlots];
+    __shared__ int s_err;
     const int n = a.n;
     const size_t stride = (size_t)a.nwg * kSolveThreads;
     const size_t i0 = (size_t)blockIdx.x * kSolveThreads + threadIdx.x;
@@ -669,9 +948,13 @@ using namespace pict;
 
 namespace {
 
+thread_local bool g_timing = false;
+thread_local double g_last_ms = 0.0;
+thread_local hipEvent_t g_ev[2] = {nullptr, nullptr};
+
 struct DeviceLimits {
     int cus = 0;
-    int occ_cg_f = 0, occ_cg_d = 0, occ_bi_f = 0, occ_bi_d = 0;
+    int occ_cg_f = 0, occ_cg_d = 0, occ_bi_f = 0, occ_bi_d = 0, occ_cg1_f = 0, occ_cg1_d = 0;
 };
 
 pict_status device_limits(DeviceLimits &L) {
@@ -688,6 +971,8 @@ pict_status device_limits(DeviceLimits &L) {
     PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg_d, k_cg<double>, kSolveThreads, 0));
     PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_bi_f, k_bicgstab<float>, kSolveThreads, 0));
     PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_bi_d, k_bicgstab<double>, kSolveThreads, 0));
+    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg1_f, k_cg1<float>, kSolveThreads, 0));
+    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg1_d, k_cg1<double>, kSolveThreads, 0));
     cached = L;
     cached_dev = dev;
     return PICT_OK;
@@ -701,26 +986,38 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
     pict_status st = device_limits(L);
     if (st) return st;
     const bool f32 = sizeof(R) == 4;
-    const int occ = o->use_bicg ? (f32 ? L.occ_bi_f : L.occ_bi_d) : (f32 ? L.occ_cg_f : L.occ_cg_d);
+    const char *tb_env0 = getenv("PICT_CG_TEXTBOOK");
+    const bool textbook0 = tb_env0 && atoi(tb_env0) != 0;
+    const int occ = o->use_bicg ? (f32 ? L.occ_bi_f : L.occ_bi_d)
+                                : (textbook0 ? (f32 ? L.occ_cg_f : L.occ_cg_d) : (f32 ? L.occ_cg1_f : L.occ_cg1_d));
     if (occ < 1) {
         set_error("solver kernel cannot be resident (occupancy 0)");
         return PICT_ERR_HIP;
     }
-    const long long want = ((long long)n + kSolveThreads - 1) / kSolveThreads;
-    long long nwg = want;
+    // rows per thread: more rows per workgroup means fewer barrier participants
+    static const int rows_per_thread = [] {
+        const char *e = getenv("PICT_SOLVER_ROWS_PER_THREAD");
+        const int v = e ? atoi(e) : 2;
+        return v < 1 ? 1 : v;
+    }();
+    const long long per_wg = (long long)kSolveThreads * rows_per_thread;
+    long long nwg = ((long long)n + per_wg - 1) / per_wg;
     const long long cap = (long long)occ * L.cus;
     if (nwg > cap) nwg = cap;
     if (nwg > 1024) nwg = 1024;
     if (nwg < 1) nwg = 1;
 
     const size_t vec = (size_t)n * nrhs;
-    const int nvec = o->use_bicg ? 8 : (o->return_best_result ? 5 : 4);
+    // PICT_CG_TEXTBOOK=1 selects the two-barrier textbook CG kernel (diagnostics / parity checks)
+    const char *tb_env = getenv("PICT_CG_TEXTBOOK");
+    const bool textbook_cg = tb_env && atoi(tb_env) != 0;
+    const int nvec = (o->use_bicg || !textbook_cg) ? 9 : (o->return_best_result ? 5 : 4);
     const size_t bytes_vec = sizeof(R) * vec * nvec;
     const size_t bytes_part = sizeof(double) * (size_t)nwg * nrhs * kSlots;
     const size_t bytes_perm = o->transpose_a ? sizeof(int32_t) * (size_t)A->nnz : 0;
     const size_t off_part = (bytes_vec + 255) & ~size_t(255);
     const size_t off_ctl = off_part + ((bytes_part + 255) & ~size_t(255));
-    const size_t off_st = off_ctl + 256;
+    const size_t off_st = off_ctl + ((sizeof(Ctl) + 255) & ~size_t(255));
     const size_t off_perm = off_st + ((sizeof(RhsState) * nrhs + 255) & ~size_t(255));
     const size_t total = off_perm + bytes_perm + 256;
 
@@ -743,11 +1040,12 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
     a.p0 = v; v += vec;
     a.p1 = v; v += vec;
     a.v0 = v; v += vec;
-    if (o->use_bicg) {
+    if (o->use_bicg || !textbook_cg) {
         a.rhat = v; v += vec;
         a.v1 = v; v += vec;
         a.s = v; v += vec;
         a.t = v; v += vec;
+        a.best = v; v += vec;
     } else if (o->return_best_result) {
         a.best = v; v += vec;
     }
@@ -763,10 +1061,10 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
     a.zero_shortcut = o->zero_rhs_shortcut;
     a.tol = o->tolerance;
     a.nwg = (unsigned)nwg;
-    PICT_HIP_TRY(hipMemsetAsync(ws + off_ctl, 0, 256, stream));
+    PICT_HIP_TRY(hipMemsetAsync(ws + off_ctl, 0, sizeof(Ctl), stream));
     if (o->transpose_a) {
         int32_t *perm = (int32_t *)(ws + off_perm);
-        int *missing = (int *)(ws + off_ctl + 128);
+        int *missing = &a.ctl->zero_all;
         hipLaunchKernelGGL(k_transpose_perm, dim3((n + 255) / 256), dim3(256), 0, stream, A->row, A->index, n, perm,
                            missing);
         PICT_HIP_TRY(hipGetLastError());
@@ -780,18 +1078,34 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
         a.perm = perm;
     }
     void *args[] = {&a};
+    if (g_timing) {
+        if (!g_ev[0]) {
+            PICT_HIP_TRY(hipEventCreate(&g_ev[0]));
+            PICT_HIP_TRY(hipEventCreate(&g_ev[1]));
+        }
+        PICT_HIP_TRY(hipEventRecord(g_ev[0], stream));
+    }
     if (o->use_bicg) {
         PICT_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bicgstab<R>, dim3((unsigned)nwg), dim3(kSolveThreads),
                                                 args, 0, stream));
-    } else {
+    } else if (textbook_cg) {
         PICT_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_cg<R>, dim3((unsigned)nwg), dim3(kSolveThreads), args,
                                                 0, stream));
+    } else {
+        PICT_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_cg1<R>, dim3((unsigned)nwg), dim3(kSolveThreads), args,
+                                                0, stream));
     }
+    if (g_timing) PICT_HIP_TRY(hipEventRecord(g_ev[1], stream));
     std::vector<RhsState> hs(nrhs);
     Ctl hctl;
     PICT_HIP_TRY(hipMemcpyAsync(hs.data(), a.st, sizeof(RhsState) * nrhs, hipMemcpyDeviceToHost, stream));
     PICT_HIP_TRY(hipMemcpyAsync(&hctl, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
     PICT_HIP_TRY(hipStreamSynchronize(stream));
+    if (g_timing) {
+        float ms = 0.f;
+        PICT_HIP_TRY(hipEventElapsedTime(&ms, g_ev[0], g_ev[1]));
+        g_last_ms = ms;
+    }
     if (hctl.err) {
         set_error("solver grid barrier exceeded its spin limit");
         return PICT_ERR_DEVICE_SYNC;
@@ -832,6 +1146,12 @@ pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, 
     set_error("unknown dtype %d", A->dtype);
     return PICT_ERR_INVALID;
 }
+
+pict_status pict_set_kernel_timing(int32_t enable) {
+    g_timing = enable != 0;
+    return PICT_OK;
+}
+double pict_last_solve_kernel_ms(void) { return g_last_ms; }
 
 pict_status pict_sparse_outer_product(const void *a, const void *b, const pict_csr_desc *pat, void *out,
                                       void *stream) {

@@ -66,10 +66,13 @@ def test_stage_parity(name, dtype):
 
 
 @pytest.mark.parametrize("name", ["cavity", "periodic_two_block", "rotated_connection", "box3d"])
-@pytest.mark.parametrize("use_bicg", [False, True])
-def test_solver_parity(name, use_bicg):
-    """Same matrix, same RHS: HIP solver vs oracle solver (FP64), iteration counts within 1 and solutions close."""
-    print("case", name, use_bicg, flush=True)
+@pytest.mark.parametrize("variant", ["bicg", "cg", "cg_textbook"])
+def test_solver_parity(name, variant, monkeypatch):
+    """Same matrix, same RHS: HIP solver vs oracle solver (FP64).  The textbook kernels follow the oracle's
+    iteration counts within 1; the single-reduction CG (same iterates in exact arithmetic) within 3%+2."""
+    print("case", name, variant, flush=True)
+    use_bicg = variant == "bicg"
+    monkeypatch.setenv("PICT_CG_TEXTBOOK", "1" if variant == "cg_textbook" else "0")
     g = scenarios.ALL[name](dtype=torch.float64, device="cuda")
     dt = torch.tensor([0.05], dtype=torch.float64)
     P.SetupAdvectionMatrix(g, dt, 0)
@@ -87,11 +90,12 @@ def test_solver_parity(name, use_bicg):
                            False, 0, transpose, False, False, False)
         xc = torch.zeros(b.shape, dtype=torch.float64)
         ic = O.SolveLinear(Acpu, b.cpu().contiguous(), xc, 2000, 1e-11, 1, use_bicg, 0, transpose)
-        print(name, "bicg" if use_bicg else "cg", "T" if transpose else "", [i.usedIterations for i in ig],
+        print(name, variant, "T" if transpose else "", [i.usedIterations for i in ig],
               [i.usedIterations for i in ic], rel_err(xg, xc), flush=True)
         for a_, c_ in zip(ig, ic):
             assert a_.converged and c_.converged
-            assert abs(a_.usedIterations - c_.usedIterations) <= 1
+            slack = 1 if variant != "cg" else 2 + 0.03 * c_.usedIterations
+            assert abs(a_.usedIterations - c_.usedIterations) <= slack
         assert_close(xg, xc, 1e-8, "solution")
 
 

@@ -1,0 +1,280 @@
+"""Pins the CPU oracle (oracle/pict_oracle.c) without the reference: closed forms, invariants, library routines
+(scipy) and brute force on tiny grids.  Each check is chosen so that a plausible restatement mistake (a dropped
+term, a wrong sign, a transposed index, a wrong neighbour across a connection) fails it.
+
+The reference ships no golden vectors for the PISO step (SURVEY: data/ holds only a placeholder); its own tests
+check invariants (tests/pisosteps.py:206-250) which are restated here as well.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+import torch
+
+import scenarios
+from oracle import oracle as O
+from pict_amd import PISOtorch as P
+from pict_amd.configs import uniform_coords
+
+CPU = dict(dtype=torch.float64, device="cpu")
+
+
+def _all_cpu():
+    return {name: fn(**CPU) for name, fn in scenarios.ALL.items()}
+
+
+@pytest.fixture(scope="module")
+def domains():
+    ds = _all_cpu()
+    for d in ds.values():
+        O.build_csr_pattern(d)
+    return ds
+
+
+def test_csr_pattern_closed_form(domains):
+    """Row pointers of the enumeration equal the reference's closed form (PISO_multiblock_cuda_kernel.cu:247-312)
+    for every cell of every block; columns are sorted and unique; nnz = sum of Block.ComputeCSRSize."""
+    for name, d in domains.items():
+        assert d._nnz == sum(b.ComputeCSRSize() for b in d.blocks)
+        for bi, b in enumerate(d.blocks):
+            for f in range(b.getStrides().w):
+                re, rs = O.csr_row_end_closed_form(d, bi, f)
+                g = b.globalOffset + f
+                assert re == int(d.C.row[g + 1]), (name, bi, f)
+                assert rs == int(d.C.row[g + 1] - d.C.row[g]), (name, bi, f)
+        row, idx = d.C.row.numpy(), d.C.index.numpy()
+        for i in range(d.getTotalSize()):
+            cols = idx[row[i]:row[i + 1]]
+            assert np.all(np.diff(cols) > 0), (name, i, cols)
+            assert i in cols
+
+
+def test_csr_pattern_brute_force_neighbors():
+    """On a 3x4 periodic/closed block every row holds exactly its stencil neighbours computed by hand."""
+    d = P.Domain(2, torch.tensor([0.1], dtype=torch.float64), "t", torch.float64, "cpu", 0, _allow_cpu=True)
+    b = d.CreateBlockWithSize(P.Int4(3, 4, 1))
+    b.MakePeriodic("x")
+    b.CloseBoundary("-y")
+    b.CloseBoundary("+y")
+    d.PrepareSolve()
+    O.build_csr_pattern(d)
+    for y in range(4):
+        for x in range(3):
+            i = x + 3 * y
+            expect = {i, (x + 1) % 3 + 3 * y, (x - 1) % 3 + 3 * y}
+            if y > 0:
+                expect.add(x + 3 * (y - 1))
+            if y < 3:
+                expect.add(x + 3 * (y + 1))
+            cols = set(d.C.index[d.C.row[i]:d.C.row[i + 1]].tolist())
+            assert cols == expect, (x, y, cols, expect)
+
+
+def _ones_consistent_A(d):
+    O.SetupAdvectionMatrix(d, 0.05)
+
+
+def test_pressure_matrix_null_space_and_symmetry(domains):
+    """P has zero row sums (constant null vector: Neumann walls, periodic, connections) and is symmetric,
+    since P_ij = 1/2 (alpha_i/A_i + alpha_j/A_j) from both sides (:4862-4869)."""
+    for name, d in domains.items():
+        O.SetupAdvectionMatrix(d, 0.05)
+        O.SetupPressureMatrix(d, 0)
+        Pm = O.csr_to_scipy(d.P)
+        scale = abs(Pm).max()
+        assert np.abs(Pm @ np.ones(Pm.shape[0])).max() < 1e-12 * scale, name
+        assert abs(Pm - Pm.T).max() < 1e-12 * scale, name
+        assert np.all(Pm.diagonal() < 0), name
+
+
+def test_advection_matrix_row_sum_divergence_free():
+    """For a uniform flow on a fully periodic grid (divergence free, no walls) every row of C sums to 1/dt:
+    the advective fluxes cancel and the diffusion rows sum to zero (:3678-3734, :3849)."""
+    d = P.Domain(2, torch.tensor([0.07], dtype=torch.float64), "t", torch.float64, "cpu", 0, _allow_cpu=True)
+    c = uniform_coords(6, 5, lx=1.2, ly=1.0, dtype=torch.float64, device="cpu")
+    b = d.CreateBlock(velocity=torch.zeros(1, 2, 5, 6, dtype=torch.float64))
+    b.setTransform(O.CoordsToTransforms(c), O.CoordsToFaceTransforms(c))
+    b.MakeAllPeriodic()
+    v = torch.zeros(1, 2, 5, 6, dtype=torch.float64)
+    v[:, 0] = 0.8
+    v[:, 1] = -0.3
+    b.setVelocity(v)
+    d.PrepareSolve()
+    O.build_csr_pattern(d)
+    dt = 0.05
+    O.SetupAdvectionMatrix(d, dt)
+    C = O.csr_to_scipy(d.C)
+    np.testing.assert_allclose(C @ np.ones(C.shape[0]), 1 / dt, rtol=1e-12)
+    # symmetric part: the diffusion operator -nu * alpha / det with alpha = det |Minv_d|^2, det = hx*hy
+    hx, hy = 1.2 / 6, 1.0 / 5
+    S = 0.5 * (C + C.T)
+    i = 2 + 6 * 2
+    np.testing.assert_allclose(S[i, i + 1], -0.07 / hx ** 2, rtol=1e-12)
+    np.testing.assert_allclose(S[i, i + 6], -0.07 / hy ** 2, rtol=1e-12)
+    # antisymmetric part: central advection 1/2 F_face / det with F = det * Minv u
+    K = 0.5 * (C - C.T)
+    np.testing.assert_allclose(K[i, i + 1], 0.5 * 0.8 / hx, rtol=1e-12)
+    np.testing.assert_allclose(K[i, i + 6], 0.5 * -0.3 / hy, rtol=1e-12)
+    np.testing.assert_allclose(d.A.numpy(), C.diagonal(), rtol=0, atol=0)
+
+
+def test_advection_matrix_wall_and_rhs_terms():
+    """A closed 1-block cavity at rest with a moving lid: the lid row gets +2 nu alpha_P on the diagonal
+    (:3831) and the RHS gets + 2 nu alpha_b u_lid (:4322); no advective boundary flux (tangential lid)."""
+    nu, dt, n = 0.1, 0.02, 4
+    d = P.Domain(2, torch.tensor([nu], dtype=torch.float64), "t", torch.float64, "cpu", 0, _allow_cpu=True)
+    b = d.CreateBlockWithSize(P.Int4(n, n, 1))
+    b.CloseAllBoundaries()
+    b.CloseBoundary("+y", velocity=torch.tensor([[1.5, 0.0]], dtype=torch.float64))
+    d.PrepareSolve()
+    O.build_csr_pattern(d)
+    O.SetupAdvectionMatrix(d, dt)
+    O.SetupAdvectionVelocity(d, dt)
+    C = O.csr_to_scipy(d.C)
+    interior = 1 + n * 1
+    corner_top = 1 + n * (n - 1)  # +y wall only
+    np.testing.assert_allclose(C[interior, interior], 1 / dt + 4 * nu, rtol=1e-13)
+    np.testing.assert_allclose(C[corner_top, corner_top], 1 / dt + 3 * nu + 2 * nu, rtol=1e-13)
+    rhs = d.velocityRHS.numpy()
+    np.testing.assert_allclose(rhs[corner_top], 2 * nu * 1.5, rtol=1e-13)  # u component
+    np.testing.assert_allclose(rhs[d.getTotalSize() + corner_top], 0.0, atol=1e-15)
+    np.testing.assert_allclose(rhs[interior], 0.0, atol=1e-15)
+
+
+def test_pressure_gradient_linear_exact():
+    """CorrectVelocity with A = 1 and pressureRHS = 0 gives u = -grad p; for linear p the central / one-sided
+    differences of getPressureGradient (:801-834) are exact everywhere, including walls and the metric."""
+    d = P.Domain(2, torch.tensor([0.1], dtype=torch.float64), "t", torch.float64, "cpu", 0, _allow_cpu=True)
+    c = uniform_coords(5, 4, lx=2.0, ly=1.0, dtype=torch.float64, device="cpu")
+    b = d.CreateBlock(velocity=torch.zeros(1, 2, 4, 5, dtype=torch.float64))
+    b.setTransform(O.CoordsToTransforms(c), O.CoordsToFaceTransforms(c))
+    b.CloseAllBoundaries()
+    d.PrepareSolve()
+    O.build_csr_pattern(d)
+    xc = (torch.arange(5, dtype=torch.float64) + 0.5) * 0.4
+    yc = (torch.arange(4, dtype=torch.float64) + 0.5) * 0.25
+    gy, gx = torch.meshgrid(yc, xc, indexing="ij")
+    b.setPressure((0.7 * gx - 1.3 * gy)[None, None].contiguous())
+    d.setA(torch.ones_like(d.A))
+    O.CorrectVelocity(d, 1.0)
+    u = d.velocityResult.numpy()
+    np.testing.assert_allclose(u[:20], -0.7, rtol=1e-12)
+    np.testing.assert_allclose(u[20:], 1.3, rtol=1e-12)
+
+
+def test_divergence_of_uniform_flow_is_zero(domains):
+    """The flux divergence (:1552-1630, :6333-6354) of a uniform flow through periodic/connected faces vanishes."""
+    d = domains["periodic_two_block"]
+    for b in d.blocks:
+        v = torch.zeros_like(b.velocity)
+        v[:, 0] = 0.6
+        v[:, 1] = -0.2
+        b.setVelocity(v)
+    div = O.ComputeVelocityDivergence(d)
+    assert div.abs().max().item() < 1e-14
+
+
+def test_transforms_affine_grid():
+    """CoordsToTransforms (grid_gen.cu:284-339) of an affine grid x = J xi: M = J h, det = det(M), Minv = M^-1;
+    the face metrics of an affine grid equal the cell metrics."""
+    J = torch.tensor([[1.3, 0.4], [-0.2, 0.9]], dtype=torch.float64)
+    xi = uniform_coords(4, 3, lx=4.0, ly=3.0, dtype=torch.float64, device="cpu")  # unit index spacing
+    c = torch.einsum("ij,bjyx->biyx", J, xi).contiguous()
+    T = O.CoordsToTransforms(c)
+    M = T[..., :4].reshape(-1, 2, 2)
+    Minv = T[..., 4:8].reshape(-1, 2, 2)
+    det = T[..., 8].reshape(-1)
+    torch.testing.assert_close(M, J.expand_as(M), rtol=1e-13, atol=1e-13)
+    torch.testing.assert_close(Minv, torch.linalg.inv(J).expand_as(Minv), rtol=1e-13, atol=1e-13)
+    torch.testing.assert_close(det, torch.linalg.det(J).expand_as(det), rtol=1e-13, atol=1e-13)
+    F = O.CoordsToFaceTransforms(c)
+    Fm = F[0, :, :-1, :-1, :4].reshape(-1, 2, 2)
+    torch.testing.assert_close(Fm, J.expand_as(Fm), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("use_bicg", [False, True])
+def test_solvers_against_scipy(domains, use_bicg):
+    """Oracle CG / BiCGStab (cg_solver_kernel.cu:114-456, bicgstab_solver_kernel.cu:47-395) reach the direct
+    solution (scipy spsolve; lsqr min-norm for the singular pressure matrix) to the requested residual, stop at the
+    first iteration whose ||r||/sqrt(n) < tol, and solve A^T with transposeA."""
+    for name in ("cavity", "periodic_two_block", "rotated_connection", "box3d"):
+        d = domains[name]
+        O.SetupAdvectionMatrix(d, 0.05)
+        O.SetupAdvectionVelocity(d, 0.05)
+        if use_bicg:
+            A = d.C
+            b = d.velocityRHS[:d.getTotalSize()].clone()
+        else:
+            O.SetupPressureMatrix(d, 0)
+            A = d.P
+            b = torch.randn(d.getTotalSize(), dtype=torch.float64, generator=torch.Generator().manual_seed(3))
+            b -= b.mean()
+        S = O.csr_to_scipy(A)
+        for transpose in (False, True):
+            x = torch.zeros_like(b)
+            info = O.SolveLinear(A, b, x, 5000, 1e-12, 1, use_bicg, transposeA=transpose)[0]
+            assert info.converged, (name, info)
+            M = S.T if transpose else S
+            r = b.numpy() - M @ x.numpy()
+            assert np.sqrt(np.mean(r ** 2)) < 1e-10, name
+            if use_bicg:
+                xd = spla.spsolve(M.tocsc(), b.numpy())
+                np.testing.assert_allclose(x.numpy(), xd, rtol=1e-8, atol=1e-10 * np.abs(xd).max())
+            else:
+                xd = spla.lsqr(M, b.numpy(), atol=1e-14, btol=1e-14)[0]
+                np.testing.assert_allclose(x.numpy() - x.numpy().mean(), xd - xd.mean(), atol=1e-8)
+            # stopping rule: with one iteration fewer the solve must not report convergence
+            it = info.usedIterations
+            if it > 0:
+                x2 = torch.zeros_like(b)
+                i2 = O.SolveLinear(A, b, x2, it, 1e-12, 1, use_bicg, transposeA=transpose)[0]
+                if not use_bicg:
+                    assert not i2.converged, name
+
+
+def test_solver_flags_and_criteria():
+    """Zero RHS converges immediately (BiCGStab initial check, bicgstab_solver_kernel.cu:259); maxit reached
+    reports not converged; the other norms follow ComputeConvergenceCriterion (cg_solver_kernel.cu:77-110)."""
+    d = scenarios.cavity(**CPU)
+    O.build_csr_pattern(d)
+    O.SetupAdvectionMatrix(d, 0.05)
+    b = torch.zeros(d.getTotalSize(), dtype=torch.float64)
+    x = torch.zeros_like(b)
+    i = O.SolveLinear(d.C, b, x, 100, 1e-8, 1, True)[0]
+    assert i.converged and i.usedIterations == -1
+    b = torch.linspace(-1, 1, d.getTotalSize(), dtype=torch.float64)
+    for crit in range(5):
+        x = torch.zeros_like(b)
+        i = O.SolveLinear(d.C, b, x, 500, 1e-9, crit, True)[0]
+        r = (b - torch.tensor(O.csr_to_scipy(d.C) @ x.numpy())).abs()
+        val = [r.norm(), r.norm() / math.sqrt(len(r)), r.sum(), r.mean(), r.max()][crit].item()
+        assert i.converged and val < 1e-8, (crit, val)
+    x = torch.zeros_like(b)
+    i = O.SolveLinear(d.C, b, x, 1, 1e-14, 1, True)[0]
+    assert not i.converged
+
+
+def test_poiseuille_steady_state():
+    """Plane Poiseuille flow (tests/validations.py:488-489): periodic channel, no-slip walls, constant forcing G
+    via the velocity source; the steady state of repeated PISO steps is u(y) = G/(2 nu) y (L - y) up to the
+    second-order cell-centred discretisation error."""
+    nu, G, ny = 0.5, 1.0, 12
+    d = P.Domain(2, torch.tensor([nu], dtype=torch.float64), "t", torch.float64, "cpu", 0, _allow_cpu=True)
+    c = uniform_coords(4, ny, lx=4.0 / ny, ly=1.0, dtype=torch.float64, device="cpu")
+    b = d.CreateBlock(velocity=torch.zeros(1, 2, ny, 4, dtype=torch.float64))
+    b.setTransform(O.CoordsToTransforms(c), O.CoordsToFaceTransforms(c))
+    b.MakePeriodic("x")
+    b.CloseBoundary("-y")
+    b.CloseBoundary("+y")
+    b.setVelocitySource(torch.tensor([[G, 0.0]], dtype=torch.float64))
+    d.PrepareSolve()
+    O.build_csr_pattern(d)
+    for _ in range(300):
+        O.piso_step(d, 0.02, advection_tol=1e-12, pressure_tol=1e-12)
+    y = (torch.arange(ny, dtype=torch.float64) + 0.5) / ny
+    exact = G / (2 * nu) * y * (1 - y)
+    u = b.velocity[0, 0, :, 0]
+    err = (u - exact).abs().max().item() / exact.max().item()
+    assert err < 2e-2, err
+    assert b.velocity[0, 1].abs().max().item() < 1e-10
