@@ -174,12 +174,11 @@ enum { S_RR = 0, S_ASUM = 1, S_AMAX = 2, S_D0 = 3, S_D1 = 4, S_BMAX = 5 };
 
 // Flush NS accumulators per RHS of this thread: workgroup-reduce and store
 // this workgroup's partials for the slots `ids`.
-template <typename R, int NS>
-__device__ __forceinline__ void flush(const SolveArgs<R> &a, double (&acc)[kMaxRhs][NS], const int (&ids)[NS],
+template <typename R, int NR, int NS>
+__device__ __forceinline__ void flush(const SolveArgs<R> &a, double (&acc)[NR][NS], const int (&ids)[NS],
                                       const bool (&ismax)[NS]) {
 #pragma unroll
-    for (int k = 0; k < kMaxRhs; ++k) {
-        if (k >= a.nrhs) break;
+    for (int k = 0; k < NR; ++k) {
         block_reduce<NS>(acc[k], ismax);
         if (threadIdx.x == 0)
 #pragma unroll
@@ -227,16 +226,16 @@ __device__ __forceinline__ R mat_val(const SolveArgs<R> &a, int k) {
     return a.perm ? a.val[a.perm[k]] : a.val[k];
 }
 
-template <int NS>
-__device__ __forceinline__ void zero_acc(double (&acc)[kMaxRhs][NS]) {
+template <int NR, int NS>
+__device__ __forceinline__ void zero_acc(double (&acc)[NR][NS]) {
 #pragma unroll
-    for (int k = 0; k < kMaxRhs; ++k)
+    for (int k = 0; k < NR; ++k)
 #pragma unroll
         for (int s = 0; s < NS; ++s) acc[k][s] = 0.0;
 }
 // residual accumulators r^2, |r|, max|r| in slots 0..2 of acc[k]
-template <typename R, int NS>
-__device__ __forceinline__ void acc_res(double (&acc)[kMaxRhs][NS], int k, R rv) {
+template <typename R, int NR, int NS>
+__device__ __forceinline__ void acc_res(double (&acc)[NR][NS], int k, R rv) {
     const double d = (double)rv;
     acc[k][0] += d * d;
     acc[k][1] += fabs(d);
@@ -246,7 +245,7 @@ __device__ __forceinline__ void acc_res(double (&acc)[kMaxRhs][NS], int k, R rv)
 // Run one reduce-barrier: flush, arrive, master (reduce + `update`), release/wait.
 #define PICT_REDUCE_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                  \
     do {                                                                                  \
-        flush<R, NS>(a, ACC, IDS, ISMAX);                                                 \
+        flush<R, NRHS, NS>(a, ACC, IDS, ISMAX);                                           \
         if (arrive(a.ctl, a.nwg, s_err)) {                                                \
             master_reduce<NS>(a.partials, a.nwg, a.nrhs, IDS, ISMAX, red);                \
             if (threadIdx.x == 0) { UPDATE; }                                             \
@@ -258,12 +257,97 @@ __device__ __forceinline__ void acc_res(double (&acc)[kMaxRhs][NS], int k, R rv)
         if (s_err) return;                                                                \
     } while (0)
 
+// --------------------------------------------------------------------------
+// Flat reduce-barrier with replicated reduction (single-reduction CG and
+// BiCGStab): every workgroup stores its block-reduced FP64 partials (double
+// buffered by generation parity), publishes them with a release fence and one
+// relaxed atomic increment; the last arrival opens the next generation.
+// After the wait every workgroup reduces all partials itself in a fixed order,
+// so all workgroups hold bitwise identical totals and run the same scalar
+// update on their own LDS copy of the solver state.
+// --------------------------------------------------------------------------
+constexpr int kPartStride = kMaxRhs * kSlots;
+
+template <int NR, int NS>
+__device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg, unsigned gen,
+                                            double (&acc)[NR][NS], const int (&ids)[NS],
+                                            const bool (&ismax)[NS], double *red, int &s_err) {
+    constexpr int nrhs = NR;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) block_reduce<NS>(acc[k], ismax);
+    double *base = part + (size_t)(gen & 1) * nwg * kPartStride;
+    if (threadIdx.x == 0) {
+        double *mine = base + (size_t)blockIdx.x * kPartStride;
+        for (int k = 0; k < nrhs; ++k)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) mine[k * kSlots + ids[s]] = acc[k][s];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned prev = __hip_atomic_fetch_add(&ctl->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int err = 0;
+        if (prev == nwg - 1) {
+            __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit) {
+                    __hip_atomic_store(&ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    err = 1;
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        s_err = err;
+    }
+    __syncthreads();
+    for (int k = 0; k < nrhs; ++k) {
+        double v[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double sum = 0.0;
+            for (unsigned w = threadIdx.x; w < nwg; w += kSolveThreads) {
+                const double pv = base[(size_t)w * kPartStride + k * kSlots + ids[s]];
+                sum = ismax[s] ? fmax(sum, pv) : sum + pv;
+            }
+            v[s] = sum;
+        }
+        block_reduce<NS>(v, ismax);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) red[k * kSlots + ids[s]] = v[s];
+    }
+    __syncthreads();
+}
+
+#define PICT_FLAT_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                            \
+    do {                                                                                          \
+        flat_reduce<NRHS, NS>(a.partials, a.ctl, a.nwg, gen, ACC, IDS, ISMAX, red, s_err);       \
+        ++gen;                                                                                    \
+        if (s_err) return;                                                                        \
+        if (threadIdx.x == 0) { UPDATE; }                                                         \
+        __syncthreads();                                                                          \
+    } while (0)
+
+// Matrix rows of one thread kept in registers across iterations: RPT rows,
+// row q = i0 + q * stride (the grid-stride assignment), at most 7 entries.
+constexpr int kMaxNz = 7;
+constexpr int kRowsCached = 2;
+template <typename R, int RPT>
+struct RowCache {
+    int len[RPT];
+    int col[RPT][kMaxNz];
+    R val[RPT][kMaxNz];
+};
+
 // ==========================================================================
 // CG  (cgSolveGPU, cg_solver_kernel.cu:114-456)
 // ==========================================================================
-template <typename R>
+template <typename R, int NRHS>
 __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
-    __shared__ double red[kMaxRhs * kSlots];
+    __shared__ double red[NRHS * kSlots];
     __shared__ int s_err;
     const int n = a.n;
     const size_t stride = (size_t)a.nwg * kSolveThreads;
@@ -278,13 +362,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
 
     // ---- phase 0: r = b - A x, p_old = 0; rho = r.r ----
     {
-        double acc[kMaxRhs][4];
+        double acc[NRHS][4];
         zero_acc(acc);
         for (size_t i = i0; i < (size_t)n; i += stride) {
             const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                if (k >= a.nrhs) break;
+            for (int k = 0; k < NRHS; ++k) {
+                
                 const size_t o = (size_t)k * n;
                 R s = 0;
                 for (int e = rs; e < re; ++e) s += mat_val(a, e) * a.x[o + a.col[e]];
@@ -298,10 +382,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
         }
         PICT_REDUCE_BARRIER(4, acc, IDS_R, MAX_R, {
             bool all_zero = true;
-            for (int k = 0; k < a.nrhs; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
+            for (int k = 0; k < NRHS; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
             const int zero_all = (a.zero_shortcut && all_zero) ? 1 : 0;
             a.ctl->zero_all = zero_all;
-            for (int k = 0; k < a.nrhs; ++k) {
+            for (int k = 0; k < NRHS; ++k) {
                 RhsState &st = a.st[k];
                 st.rho = red[k * kSlots + S_RR];
                 st.beta = 0;
@@ -321,22 +405,22 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
     }
     if (a.ctl->zero_all) {
         for (size_t i = i0; i < (size_t)n; i += stride)
-            for (int k = 0; k < a.nrhs; ++k) a.x[(size_t)k * n + i] = 0;
+            for (int k = 0; k < NRHS; ++k) a.x[(size_t)k * n + i] = 0;
         return;
     }
 
     for (int it = 0; it < a.maxit; ++it) {
         // ---- optional residual reset r = b - A x, p = r (cg_solver_kernel.cu:266-287) ----
         if (a.reset_steps > 0 && (it + 1) % a.reset_steps == 0) {
-            bool act[kMaxRhs];
+            bool act[NRHS];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) act[k] = k < a.nrhs && a.st[k].active;
-            double acc[kMaxRhs][3];
+            for (int k = 0; k < NRHS; ++k) act[k] = k < NRHS && a.st[k].active;
+            double acc[NRHS][3];
             zero_acc(acc);
             for (size_t i = i0; i < (size_t)n; i += stride) {
                 const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) {
+                for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     R s = 0;
@@ -347,7 +431,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
                 }
             }
             PICT_REDUCE_BARRIER(3, acc, IDS_RES, MAX_RES, {
-                for (int k = 0; k < a.nrhs; ++k)
+                for (int k = 0; k < NRHS; ++k)
                     if (a.st[k].active) {
                         a.st[k].rho = red[k * kSlots + S_RR];
                         a.st[k].beta = 0;
@@ -358,20 +442,20 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
         R *pnew = (it & 1) ? a.p0 : a.p1;
         // ---- phase 1: p = r + beta p (evaluated on the fly), Ap = A p; p.Ap ----
         {
-            bool act[kMaxRhs], save[kMaxRhs];
-            R beta[kMaxRhs];
+            bool act[NRHS], save[NRHS];
+            R beta[NRHS];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                act[k] = k < a.nrhs && a.st[k].active;
+            for (int k = 0; k < NRHS; ++k) {
+                act[k] = k < NRHS && a.st[k].active;
                 save[k] = act[k] && a.st[k].save_best;
                 beta[k] = act[k] ? (R)a.st[k].beta : R(0);
             }
-            double acc[kMaxRhs][1];
+            double acc[NRHS][1];
             zero_acc(acc);
             for (size_t i = i0; i < (size_t)n; i += stride) {
                 const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) {
+                for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     if (save[k]) a.best[o + i] = a.x[o + i];
@@ -387,7 +471,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
                 }
             }
             PICT_REDUCE_BARRIER(1, acc, IDS_D, MAX_D, {
-                for (int k = 0; k < a.nrhs; ++k)
+                for (int k = 0; k < NRHS; ++k)
                     if (a.st[k].active) {
                         a.st[k].alpha = a.st[k].rho / red[k * kSlots + S_D0];
                         a.st[k].save_best = 0;
@@ -396,18 +480,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
         }
         // ---- phase 2: x += alpha p, r -= alpha Ap; r.r, convergence ----
         {
-            bool act[kMaxRhs];
-            R alpha[kMaxRhs];
+            bool act[NRHS];
+            R alpha[NRHS];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                act[k] = k < a.nrhs && a.st[k].active;
+            for (int k = 0; k < NRHS; ++k) {
+                act[k] = k < NRHS && a.st[k].active;
                 alpha[k] = act[k] ? (R)a.st[k].alpha : R(0);
             }
-            double acc[kMaxRhs][3];
+            double acc[NRHS][3];
             zero_acc(acc);
             for (size_t i = i0; i < (size_t)n; i += stride) {
 #pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) {
+                for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     a.x[o + i] += alpha[k] * pnew[o + i];
@@ -417,7 +501,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
                 }
             }
             PICT_REDUCE_BARRIER(3, acc, IDS_RES, MAX_RES, {
-                for (int k = 0; k < a.nrhs; ++k) {
+                for (int k = 0; k < NRHS; ++k) {
                     RhsState &st = a.st[k];
                     if (!st.active) continue;
                     const double rr = red[k * kSlots + S_RR];
@@ -447,11 +531,11 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
             });
         }
         bool any = false;
-        for (int k = 0; k < a.nrhs; ++k) any = any || a.st[k].active;
+        for (int k = 0; k < NRHS; ++k) any = any || a.st[k].active;
         if (!any) break;
     }
     // restore best iterates (cg_solver_kernel.cu:399-412)
-    for (int k = 0; k < a.nrhs; ++k) {
+    for (int k = 0; k < NRHS; ++k) {
         if (!a.st[k].restore_best) continue;
         const size_t o = (size_t)k * n;
         for (size_t i = i0; i < (size_t)n; i += stride) a.x[o + i] = a.best[o + i];
@@ -471,9 +555,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
 // r, w, s are double-buffered (neighbours read the previous values); p and x
 // are only touched by their owner.
 // ==========================================================================
-template <typename R>
+template <typename R, int NRHS>
 __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
-    __shared__ double red[kMaxRhs * kSlots];
+    __shared__ double red[NRHS * kSlots];
+    __shared__ RhsState sst[NRHS];
     __shared__ int s_err;
     const int n = a.n;
     const size_t stride = (size_t)a.nwg * kSolveThreads;
@@ -486,16 +571,17 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
     constexpr int IDS_G[5] = {S_RR, S_ASUM, S_AMAX, S_D0, S_D1};
     constexpr bool MAX_G[5] = {false, false, true, false, false};
     int cur = 0;  // buffer holding r_i, w_i, s_{i-1}
+    int zero_all = 0;
 
-    // phase I0: r0 = b - A x0
+    // ---- phase I0: r0 = b - A x0; p = s = 0 ----
     {
-        double acc[kMaxRhs][4];
+        double acc[NRHS][4];
         zero_acc(acc);
         for (size_t i = i0; i < (size_t)n; i += stride) {
             const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                if (k >= a.nrhs) break;
+            for (int k = 0; k < NRHS; ++k) {
+                
                 const size_t o = (size_t)k * n;
                 R s = 0;
                 for (int e = rs; e < re; ++e) s += mat_val(a, e) * a.x[o + a.col[e]];
@@ -508,17 +594,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                 acc[k][3] = fmax(acc[k][3], fabs((double)bv));
             }
         }
-        PICT_REDUCE_BARRIER(4, acc, IDS_R, MAX_R, {
+        PICT_FLAT_BARRIER(4, acc, IDS_R, MAX_R, {
             bool all_zero = true;
-            for (int k = 0; k < a.nrhs; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
-            const int zero_all = (a.zero_shortcut && all_zero) ? 1 : 0;
-            a.ctl->zero_all = zero_all;
-            for (int k = 0; k < a.nrhs; ++k) {
-                RhsState &st = a.st[k];
+            for (int k = 0; k < NRHS; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
+            const int za = (a.zero_shortcut && all_zero) ? 1 : 0;
+            for (int k = 0; k < NRHS; ++k) {
+                RhsState &st = sst[k];
                 st.rho = red[k * kSlots + S_RR];
+                st.alpha = 0;
                 st.beta = 0;
-                st.active = zero_all ? 0 : 1;
-                st.converged = zero_all;
+                st.omega = za;  // zero-RHS flag travels in omega (unused by CG)
+                st.active = za ? 0 : 1;
+                st.converged = za;
                 st.finite = 1;
                 st.used = -1;
                 st.final_res = 0;
@@ -530,57 +617,83 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                 st.restore_best = 0;
             }
         });
+        zero_all = sst[0].omega != 0.0;
     }
-    if (a.ctl->zero_all) {
+    if (zero_all) {
         for (size_t i = i0; i < (size_t)n; i += stride)
-            for (int k = 0; k < a.nrhs; ++k) a.x[(size_t)k * n + i] = 0;
+            for (int k = 0; k < NRHS; ++k) a.x[(size_t)k * n + i] = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (int k = 0; k < NRHS; ++k) a.st[k] = sst[k];
         return;
     }
-    // phase I1: w0 = A r0; gamma0 = (r0,r0), delta0 = (w0,r0); alpha0 = gamma0/delta0
-    {
-        double acc[kMaxRhs][5];
-        zero_acc(acc);
-        bool act[kMaxRhs];
+
+    // w = A r (buffer `buf`); gamma = (r,r), delta = (w,r); alpha = gamma/delta, beta = 0
+#define PICT_CG1_W_PHASE(buf)                                                                     \
+    {                                                                                             \
+        bool act[NRHS];                                                                        \
+        _Pragma("unroll") for (int k = 0; k < NRHS; ++k) act[k] = k < NRHS && sst[k].active; \
+        double acc[NRHS][5];                                                                   \
+        zero_acc(acc);                                                                            \
+        for (size_t i = i0; i < (size_t)n; i += stride) {                                         \
+            const int rs = a.row[i], re = a.row[i + 1];                                           \
+            _Pragma("unroll") for (int k = 0; k < NRHS; ++k) {                                 \
+                if (!act[k]) continue;                                                            \
+                const size_t o = (size_t)k * n;                                                   \
+                R w = 0;                                                                          \
+                for (int e = rs; e < re; ++e) w += mat_val(a, e) * rb[buf][o + a.col[e]];         \
+                wb[buf][o + i] = w;                                                               \
+                const R rv = rb[buf][o + i];                                                      \
+                acc_res(acc, k, rv);                                                              \
+                acc[k][3] += (double)w * (double)rv;                                              \
+            }                                                                                     \
+        }                                                                                         \
+        PICT_FLAT_BARRIER(5, acc, IDS_G, MAX_G, {                                                 \
+            for (int k = 0; k < NRHS; ++k) {                                                    \
+                RhsState &st = sst[k];                                                            \
+                if (!st.active) continue;                                                         \
+                st.rho = red[k * kSlots + S_RR];                                                  \
+                st.alpha = st.rho / red[k * kSlots + S_D0];                                       \
+                st.beta = 0;                                                                      \
+            }                                                                                     \
+        });                                                                                       \
+    }
+
+    PICT_CG1_W_PHASE(0)
+
+    // matrix rows of this thread in registers when every thread owns at most kRowsCached rows
+    RowCache<R, kRowsCached> rc;
+    // rc.len[q]: -1 no row, -2 row longer than kMaxNz (read from memory), else cached length
+    const bool cached = stride * kRowsCached >= (size_t)n;
+    if (cached) {
 #pragma unroll
-        for (int k = 0; k < kMaxRhs; ++k) act[k] = k < a.nrhs && a.st[k].active;
-        for (size_t i = i0; i < (size_t)n; i += stride) {
-            const int rs = a.row[i], re = a.row[i + 1];
+        for (int q = 0; q < kRowsCached; ++q) {
+            const size_t i = i0 + (size_t)q * stride;
+            rc.len[q] = -1;
+            if (i < (size_t)n) {
+                const int rs = a.row[i], re = a.row[i + 1];
+                rc.len[q] = re - rs <= kMaxNz ? re - rs : -2;
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                if (!act[k]) continue;
-                const size_t o = (size_t)k * n;
-                R w = 0;
-                for (int e = rs; e < re; ++e) w += mat_val(a, e) * rb[0][o + a.col[e]];
-                wb[0][o + i] = w;
-                const R rv = rb[0][o + i];
-                acc_res(acc, k, rv);
-                acc[k][3] += (double)w * (double)rv;
+                for (int e = 0; e < kMaxNz; ++e) {
+                    rc.col[q][e] = e < re - rs ? a.col[rs + e] : 0;
+                    rc.val[q][e] = e < re - rs ? mat_val(a, rs + e) : R(0);
+                }
             }
         }
-        PICT_REDUCE_BARRIER(5, acc, IDS_G, MAX_G, {
-            for (int k = 0; k < a.nrhs; ++k) {
-                RhsState &st = a.st[k];
-                if (!st.active) continue;
-                st.rho = red[k * kSlots + S_RR];
-                st.alpha = st.rho / red[k * kSlots + S_D0];
-                st.beta = 0;
-            }
-        });
     }
 
     for (int it = 0; it < a.maxit; ++it) {
         if (a.reset_steps > 0 && (it + 1) % a.reset_steps == 0) {
-            // ---- residual reset (cg_solver_kernel.cu:266-287): r = b - A x, w = A r, restart ----
+            // residual reset (cg_solver_kernel.cu:266-287): r = b - A x, restart with p = r
             {
-                bool act[kMaxRhs];
+                bool act[NRHS];
 #pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) act[k] = k < a.nrhs && a.st[k].active;
-                double acc[kMaxRhs][4];
+                for (int k = 0; k < NRHS; ++k) act[k] = k < NRHS && sst[k].active;
+                double acc[NRHS][4];
                 zero_acc(acc);
                 for (size_t i = i0; i < (size_t)n; i += stride) {
                     const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
-                    for (int k = 0; k < kMaxRhs; ++k) {
+                    for (int k = 0; k < NRHS; ++k) {
                         if (!act[k]) continue;
                         const size_t o = (size_t)k * n;
                         R s = 0;
@@ -590,59 +703,31 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                         sb[cur][o + i] = 0;
                     }
                 }
-                PICT_REDUCE_BARRIER(4, acc, IDS_R, MAX_R, {});
+                PICT_FLAT_BARRIER(4, acc, IDS_R, MAX_R, {});
             }
-            {
-                bool act[kMaxRhs];
-#pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) act[k] = k < a.nrhs && a.st[k].active;
-                double acc[kMaxRhs][5];
-                zero_acc(acc);
-                for (size_t i = i0; i < (size_t)n; i += stride) {
-                    const int rs = a.row[i], re = a.row[i + 1];
-#pragma unroll
-                    for (int k = 0; k < kMaxRhs; ++k) {
-                        if (!act[k]) continue;
-                        const size_t o = (size_t)k * n;
-                        R w = 0;
-                        for (int e = rs; e < re; ++e) w += mat_val(a, e) * rb[cur][o + a.col[e]];
-                        wb[cur][o + i] = w;
-                        const R rv = rb[cur][o + i];
-                        acc_res(acc, k, rv);
-                        acc[k][3] += (double)w * (double)rv;
-                    }
-                }
-                PICT_REDUCE_BARRIER(5, acc, IDS_G, MAX_G, {
-                    for (int k = 0; k < a.nrhs; ++k) {
-                        RhsState &st = a.st[k];
-                        if (!st.active) continue;
-                        st.rho = red[k * kSlots + S_RR];
-                        st.alpha = st.rho / red[k * kSlots + S_D0];
-                        st.beta = 0;
-                    }
-                });
-            }
+            PICT_CG1_W_PHASE(cur)
         }
         const int nxt = cur ^ 1;
         // ---- the iteration: one pass, one reduce-barrier ----
         {
-            bool act[kMaxRhs], save[kMaxRhs];
-            R alpha[kMaxRhs], beta[kMaxRhs];
+            bool act[NRHS], save[NRHS];
+            R alpha[NRHS], beta[NRHS];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                act[k] = k < a.nrhs && a.st[k].active;
-                save[k] = act[k] && a.st[k].save_best;
-                alpha[k] = act[k] ? (R)a.st[k].alpha : R(0);
-                beta[k] = act[k] ? (R)a.st[k].beta : R(0);
+            for (int k = 0; k < NRHS; ++k) {
+                act[k] = k < NRHS && sst[k].active;
+                save[k] = act[k] && sst[k].save_best;
+                alpha[k] = act[k] ? (R)sst[k].alpha : R(0);
+                beta[k] = act[k] ? (R)sst[k].beta : R(0);
             }
             const R *r = rb[cur], *w = wb[cur], *so = sb[cur];
             R *rn = rb[nxt], *wn = wb[nxt], *sn = sb[nxt];
-            double acc[kMaxRhs][5];
+            double acc[NRHS][5];
             zero_acc(acc);
-            for (size_t i = i0; i < (size_t)n; i += stride) {
-                const int rs = a.row[i], re = a.row[i + 1];
+            // one row: update p, s, x, r and form w' = A r' with r' of the neighbours on the fly;
+            // `ent(e, c, v)` yields entry e of the row (column c, value v), nz entries
+            auto row_update = [&](size_t i, int nz, auto ent) {
 #pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) {
+                for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     if (save[k]) a.best[o + i] = a.x[o + i];
@@ -654,18 +739,41 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                     const R rm = fma(-alpha[k], sm, r[o + i]);
                     rn[o + i] = rm;
                     R wv = 0;
-                    for (int e = rs; e < re; ++e) {
-                        const size_t c = o + a.col[e];
-                        wv += mat_val(a, e) * fma(-alpha[k], fma(beta[k], so[c], w[c]), r[c]);
-                    }
+                    ent(nz, [&](int cidx, R v) {
+                        const size_t c = o + cidx;
+                        wv += v * fma(-alpha[k], fma(beta[k], so[c], w[c]), r[c]);
+                    });
                     wn[o + i] = wv;
                     acc_res(acc, k, rm);
                     acc[k][3] += (double)wv * (double)rm;
                 }
+            };
+            auto mem_row = [&](size_t i) {
+                const int rs = a.row[i], re = a.row[i + 1];
+                row_update(i, re - rs, [&](int, auto f) {
+                    for (int e = rs; e < re; ++e) f(a.col[e], mat_val(a, e));
+                });
+            };
+            if (cached) {
+#pragma unroll
+                for (int q = 0; q < kRowsCached; ++q) {
+                    const size_t i = i0 + (size_t)q * stride;
+                    if (rc.len[q] >= 0) {
+                        row_update(i, rc.len[q], [&](int nz, auto f) {
+#pragma unroll
+                            for (int e = 0; e < kMaxNz; ++e)
+                                if (e < nz) f(rc.col[q][e], rc.val[q][e]);
+                        });
+                    } else if (rc.len[q] == -2) {
+                        mem_row(i);
+                    }
+                }
+            } else {
+                for (size_t i = i0; i < (size_t)n; i += stride) mem_row(i);
             }
-            PICT_REDUCE_BARRIER(5, acc, IDS_G, MAX_G, {
-                for (int k = 0; k < a.nrhs; ++k) {
-                    RhsState &st = a.st[k];
+            PICT_FLAT_BARRIER(5, acc, IDS_G, MAX_G, {
+                for (int k = 0; k < NRHS; ++k) {
+                    RhsState &st = sst[k];
                     if (!st.active) continue;
                     st.save_best = 0;
                     const double gam = red[k * kSlots + S_RR];
@@ -697,22 +805,29 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
         }
         cur = nxt;
         bool any = false;
-        for (int k = 0; k < a.nrhs; ++k) any = any || a.st[k].active;
+        for (int k = 0; k < NRHS; ++k) any = any || sst[k].active;
         if (!any) break;
     }
-    for (int k = 0; k < a.nrhs; ++k) {
-        if (!a.st[k].restore_best) continue;
+#undef PICT_CG1_W_PHASE
+    for (int k = 0; k < NRHS; ++k) {
+        if (!sst[k].restore_best) continue;
         const size_t o = (size_t)k * n;
         for (size_t i = i0; i < (size_t)n; i += stride) a.x[o + i] = a.best[o + i];
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int k = 0; k < NRHS; ++k) a.st[k] = sst[k];
 }
 
 // ==========================================================================
 // BiCGStab without preconditioner (bicgstabSolveGPU, bicgstab_solver_kernel.cu:222-365)
+// Three reduce-barriers per iteration: (rhat,v) | half-step norm, (t,s), (t,t) |
+// norm, (rhat,r).  p = r + beta (p - omega v) and s = r - alpha v are folded
+// into the following SpMVs, evaluated on the fly for the neighbours.
 // ==========================================================================
-template <typename R>
+template <typename R, int NRHS>
 __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
-    __shared__ double red[kMaxRhs * kSlots];
+    __shared__ double red[NRHS * kSlots];
+    __shared__ RhsState sst[NRHS];
     __shared__ int s_err;
     const int n = a.n;
     const size_t stride = (size_t)a.nwg * kSolveThreads;
@@ -726,16 +841,17 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
     constexpr bool MAX_B[5] = {false, false, true, false, false};
     constexpr int IDS_C[4] = {S_RR, S_ASUM, S_AMAX, S_D0};
     constexpr bool MAX_C[4] = {false, false, true, false};
+    int zero_all = 0;
 
     // ---- phase 0: r = b - A x, rhat = r, p = v = 0 ----
     {
-        double acc[kMaxRhs][4];
+        double acc[NRHS][4];
         zero_acc(acc);
         for (size_t i = i0; i < (size_t)n; i += stride) {
             const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                if (k >= a.nrhs) break;
+            for (int k = 0; k < NRHS; ++k) {
+                
                 const size_t o = (size_t)k * n;
                 R s = 0;
                 for (int e = rs; e < re; ++e) s += mat_val(a, e) * a.x[o + a.col[e]];
@@ -749,13 +865,12 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
                 acc[k][3] = fmax(acc[k][3], fabs((double)bv));
             }
         }
-        PICT_REDUCE_BARRIER(4, acc, IDS_R, MAX_R, {
+        PICT_FLAT_BARRIER(4, acc, IDS_R, MAX_R, {
             bool all_zero = true;
-            for (int k = 0; k < a.nrhs; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
-            const int zero_all = (a.zero_shortcut && all_zero) ? 1 : 0;
-            a.ctl->zero_all = zero_all;
-            for (int k = 0; k < a.nrhs; ++k) {
-                RhsState &st = a.st[k];
+            for (int k = 0; k < NRHS; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
+            const int za = (a.zero_shortcut && all_zero) ? 1 : 0;
+            for (int k = 0; k < NRHS; ++k) {
+                RhsState &st = sst[k];
                 const double nr0 = criterion(a.conv, red[k * kSlots + S_RR], red[k * kSlots + S_ASUM],
                                              red[k * kSlots + S_AMAX], n);
                 st.finite = 1;
@@ -766,7 +881,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
                 st.alpha = 0;
                 st.omega = 0;
                 st.used = -1;
-                if (zero_all) {
+                st.best_crit = za;  // zero-RHS flag travels in best_crit (unused by BiCGStab)
+                if (za) {
                     st.active = 0; st.converged = 1; st.final_res = 0;
                 } else if (nr0 < a.tol) {  // bicgstab_solver_kernel.cu:259, :351-356
                     st.active = 0; st.converged = 1; st.final_res = nr0;
@@ -775,17 +891,17 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
                 }
             }
         });
+        zero_all = sst[0].best_crit != 0.0;
     }
-    if (a.ctl->zero_all) {
+    if (zero_all) {
         for (size_t i = i0; i < (size_t)n; i += stride)
-            for (int k = 0; k < a.nrhs; ++k) a.x[(size_t)k * n + i] = 0;
-        return;
+            for (int k = 0; k < NRHS; ++k) a.x[(size_t)k * n + i] = 0;
     }
 
-    for (int it = 0; it < a.maxit; ++it) {
+    for (int it = 0; it < a.maxit && !zero_all; ++it) {
         {
             bool any = false;
-            for (int k = 0; k < a.nrhs; ++k) any = any || a.st[k].active;
+            for (int k = 0; k < NRHS; ++k) any = any || sst[k].active;
             if (!any) break;
         }
         R *pold = (it & 1) ? a.p1 : a.p0;
@@ -794,20 +910,20 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
         R *vnew = (it & 1) ? a.v0 : a.v1;
         // ---- phase A: p = r + beta (p - omega v) on the fly; v = A p; (rhat, v) ----
         {
-            bool act[kMaxRhs];
-            R beta[kMaxRhs], mom[kMaxRhs];
+            bool act[NRHS];
+            R beta[NRHS], mom[NRHS];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                act[k] = k < a.nrhs && a.st[k].active;
-                beta[k] = act[k] ? (R)a.st[k].beta : R(0);
-                mom[k] = act[k] ? (R)(-a.st[k].omega) : R(0);
+            for (int k = 0; k < NRHS; ++k) {
+                act[k] = k < NRHS && sst[k].active;
+                beta[k] = act[k] ? (R)sst[k].beta : R(0);
+                mom[k] = act[k] ? (R)(-sst[k].omega) : R(0);
             }
-            double acc[kMaxRhs][1];
+            double acc[NRHS][1];
             zero_acc(acc);
             for (size_t i = i0; i < (size_t)n; i += stride) {
                 const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) {
+                for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     const R pn = fma(beta[k], fma(mom[k], vold[o + i], pold[o + i]), a.r[o + i]);
@@ -821,26 +937,26 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
                     acc[k][0] += (double)a.rhat[o + i] * (double)s;
                 }
             }
-            PICT_REDUCE_BARRIER(1, acc, IDS_D, MAX_D, {
-                for (int k = 0; k < a.nrhs; ++k)
-                    if (a.st[k].active) a.st[k].alpha = a.st[k].rho / red[k * kSlots + S_D0];
+            PICT_FLAT_BARRIER(1, acc, IDS_D, MAX_D, {
+                for (int k = 0; k < NRHS; ++k)
+                    if (sst[k].active) sst[k].alpha = sst[k].rho / red[k * kSlots + S_D0];
             });
         }
         // ---- phase B: s = r - alpha v; t = A s (on the fly); x += alpha p ----
         {
-            bool act[kMaxRhs];
-            R alpha[kMaxRhs];
+            bool act[NRHS];
+            R alpha[NRHS];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                act[k] = k < a.nrhs && a.st[k].active;
-                alpha[k] = act[k] ? (R)a.st[k].alpha : R(0);
+            for (int k = 0; k < NRHS; ++k) {
+                act[k] = k < NRHS && sst[k].active;
+                alpha[k] = act[k] ? (R)sst[k].alpha : R(0);
             }
-            double acc[kMaxRhs][5];
+            double acc[NRHS][5];
             zero_acc(acc);
             for (size_t i = i0; i < (size_t)n; i += stride) {
                 const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) {
+                for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     const R mal = -alpha[k];
@@ -858,9 +974,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
                     acc[k][4] += (double)tv * (double)tv;
                 }
             }
-            PICT_REDUCE_BARRIER(5, acc, IDS_B, MAX_B, {
-                for (int k = 0; k < a.nrhs; ++k) {
-                    RhsState &st = a.st[k];
+            PICT_FLAT_BARRIER(5, acc, IDS_B, MAX_B, {
+                for (int k = 0; k < NRHS; ++k) {
+                    RhsState &st = sst[k];
                     if (!st.active) continue;
                     const double nr = criterion(a.conv, red[k * kSlots + S_RR], red[k * kSlots + S_ASUM],
                                                 red[k * kSlots + S_AMAX], n);
@@ -877,18 +993,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
         }
         // ---- phase C: x += omega s; r = s - omega t; (rhat, r), convergence ----
         {
-            bool act[kMaxRhs];
-            R omega[kMaxRhs];
+            bool act[NRHS];
+            R omega[NRHS];
 #pragma unroll
-            for (int k = 0; k < kMaxRhs; ++k) {
-                act[k] = k < a.nrhs && a.st[k].active;
-                omega[k] = act[k] ? (R)a.st[k].omega : R(0);
+            for (int k = 0; k < NRHS; ++k) {
+                act[k] = k < NRHS && sst[k].active;
+                omega[k] = act[k] ? (R)sst[k].omega : R(0);
             }
-            double acc[kMaxRhs][4];
+            double acc[NRHS][4];
             zero_acc(acc);
             for (size_t i = i0; i < (size_t)n; i += stride) {
 #pragma unroll
-                for (int k = 0; k < kMaxRhs; ++k) {
+                for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     const R sv = a.s[o + i];
@@ -899,9 +1015,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
                     acc[k][3] += (double)a.rhat[o + i] * (double)rv;
                 }
             }
-            PICT_REDUCE_BARRIER(4, acc, IDS_C, MAX_C, {
-                for (int k = 0; k < a.nrhs; ++k) {
-                    RhsState &st = a.st[k];
+            PICT_FLAT_BARRIER(4, acc, IDS_C, MAX_C, {
+                for (int k = 0; k < NRHS; ++k) {
+                    RhsState &st = sst[k];
                     if (!st.active) continue;
                     const double nr = criterion(a.conv, red[k * kSlots + S_RR], red[k * kSlots + S_ASUM],
                                                 red[k * kSlots + S_AMAX], n);
@@ -914,6 +1030,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
             });
         }
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int k = 0; k < NRHS; ++k) a.st[k] = sst[k];
 }
 
 // A^T permutation for a structurally symmetric pattern: perm[e] = index of (j,i) for e = (i,j).
@@ -952,44 +1070,56 @@ thread_local bool g_timing = false;
 thread_local double g_last_ms = 0.0;
 thread_local hipEvent_t g_ev[2] = {nullptr, nullptr};
 
-struct DeviceLimits {
-    int cus = 0;
-    int occ_cg_f = 0, occ_cg_d = 0, occ_bi_f = 0, occ_bi_d = 0, occ_cg1_f = 0, occ_cg1_d = 0;
-};
-
-pict_status device_limits(DeviceLimits &L) {
-    static thread_local int cached_dev = -1;
-    static thread_local DeviceLimits cached;
+// Occupancy (resident workgroups per CU) of a solver kernel on the current device, cached.
+pict_status occupancy(const void *kernel, int &occ, int &cus) {
+    struct Entry { int dev; const void *k; int occ; int cus; };
+    static thread_local std::vector<Entry> cache;
     int dev;
     PICT_HIP_TRY(hipGetDevice(&dev));
-    if (dev == cached_dev) {
-        L = cached;
-        return PICT_OK;
-    }
-    PICT_HIP_TRY(hipDeviceGetAttribute(&L.cus, hipDeviceAttributeMultiprocessorCount, dev));
-    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg_f, k_cg<float>, kSolveThreads, 0));
-    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg_d, k_cg<double>, kSolveThreads, 0));
-    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_bi_f, k_bicgstab<float>, kSolveThreads, 0));
-    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_bi_d, k_bicgstab<double>, kSolveThreads, 0));
-    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg1_f, k_cg1<float>, kSolveThreads, 0));
-    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&L.occ_cg1_d, k_cg1<double>, kSolveThreads, 0));
-    cached = L;
-    cached_dev = dev;
+    for (const Entry &e : cache)
+        if (e.dev == dev && e.k == kernel) {
+            occ = e.occ;
+            cus = e.cus;
+            return PICT_OK;
+        }
+    PICT_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kSolveThreads, 0));
+    cache.push_back({dev, kernel, occ, cus});
     return PICT_OK;
+}
+
+enum class Kind { kCG1, kCGTextbook, kBiCG };
+
+template <typename R, int NR>
+const void *kernel_for(Kind kind) {
+    switch (kind) {
+        case Kind::kBiCG: return (const void *)k_bicgstab<R, NR>;
+        case Kind::kCGTextbook: return (const void *)k_cg<R, NR>;
+        default: return (const void *)k_cg1<R, NR>;
+    }
+}
+template <typename R>
+const void *kernel_for_nrhs(Kind kind, int nrhs) {
+    switch (nrhs) {
+        case 1: return kernel_for<R, 1>(kind);
+        case 2: return kernel_for<R, 2>(kind);
+        case 3: return kernel_for<R, 3>(kind);
+        default: return kernel_for<R, 4>(kind);
+    }
 }
 
 template <typename R>
 pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
                         pict_solve_info *infos, hipStream_t stream) {
     const int n = A->n;
-    DeviceLimits L;
-    pict_status st = device_limits(L);
+    // PICT_CG_TEXTBOOK=1 selects the two-barrier textbook CG kernel (diagnostics / parity checks)
+    const char *tb_env = getenv("PICT_CG_TEXTBOOK");
+    const bool textbook_cg = tb_env && atoi(tb_env) != 0;
+    const Kind kind = o->use_bicg ? Kind::kBiCG : (textbook_cg ? Kind::kCGTextbook : Kind::kCG1);
+    const void *kernel = kernel_for_nrhs<R>(kind, nrhs);
+    int occ = 0, cus = 0;
+    pict_status st = occupancy(kernel, occ, cus);
     if (st) return st;
-    const bool f32 = sizeof(R) == 4;
-    const char *tb_env0 = getenv("PICT_CG_TEXTBOOK");
-    const bool textbook0 = tb_env0 && atoi(tb_env0) != 0;
-    const int occ = o->use_bicg ? (f32 ? L.occ_bi_f : L.occ_bi_d)
-                                : (textbook0 ? (f32 ? L.occ_cg_f : L.occ_cg_d) : (f32 ? L.occ_cg1_f : L.occ_cg1_d));
     if (occ < 1) {
         set_error("solver kernel cannot be resident (occupancy 0)");
         return PICT_ERR_HIP;
@@ -1002,18 +1132,15 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
     }();
     const long long per_wg = (long long)kSolveThreads * rows_per_thread;
     long long nwg = ((long long)n + per_wg - 1) / per_wg;
-    const long long cap = (long long)occ * L.cus;
+    const long long cap = (long long)occ * cus;
     if (nwg > cap) nwg = cap;
-    if (nwg > 1024) nwg = 1024;
+    if (nwg > 2048) nwg = 2048;
     if (nwg < 1) nwg = 1;
 
     const size_t vec = (size_t)n * nrhs;
-    // PICT_CG_TEXTBOOK=1 selects the two-barrier textbook CG kernel (diagnostics / parity checks)
-    const char *tb_env = getenv("PICT_CG_TEXTBOOK");
-    const bool textbook_cg = tb_env && atoi(tb_env) != 0;
-    const int nvec = (o->use_bicg || !textbook_cg) ? 9 : (o->return_best_result ? 5 : 4);
+    const int nvec = kind == Kind::kCGTextbook ? (o->return_best_result ? 5 : 4) : 9;
     const size_t bytes_vec = sizeof(R) * vec * nvec;
-    const size_t bytes_part = sizeof(double) * (size_t)nwg * nrhs * kSlots;
+    const size_t bytes_part = sizeof(double) * (size_t)2 * nwg * kPartStride;
     const size_t bytes_perm = o->transpose_a ? sizeof(int32_t) * (size_t)A->nnz : 0;
     const size_t off_part = (bytes_vec + 255) & ~size_t(255);
     const size_t off_ctl = off_part + ((bytes_part + 255) & ~size_t(255));
@@ -1040,7 +1167,7 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
     a.p0 = v; v += vec;
     a.p1 = v; v += vec;
     a.v0 = v; v += vec;
-    if (o->use_bicg || !textbook_cg) {
+    if (kind != Kind::kCGTextbook) {
         a.rhat = v; v += vec;
         a.v1 = v; v += vec;
         a.s = v; v += vec;
@@ -1085,16 +1212,7 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
         }
         PICT_HIP_TRY(hipEventRecord(g_ev[0], stream));
     }
-    if (o->use_bicg) {
-        PICT_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bicgstab<R>, dim3((unsigned)nwg), dim3(kSolveThreads),
-                                                args, 0, stream));
-    } else if (textbook_cg) {
-        PICT_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_cg<R>, dim3((unsigned)nwg), dim3(kSolveThreads), args,
-                                                0, stream));
-    } else {
-        PICT_HIP_TRY(hipLaunchCooperativeKernel((const void *)k_cg1<R>, dim3((unsigned)nwg), dim3(kSolveThreads), args,
-                                                0, stream));
-    }
+    PICT_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3((unsigned)nwg), dim3(kSolveThreads), args, 0, stream));
     if (g_timing) PICT_HIP_TRY(hipEventRecord(g_ev[1], stream));
     std::vector<RhsState> hs(nrhs);
     Ctl hctl;
