@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--config", default="c3", choices=["c3", "c1"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
+                    help="after the timed region, run 3 plain device copies of MIB MiB (a known-size kernel that "
+                         "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PISO step of one domain runs on a single MI355X "
@@ -100,6 +103,14 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     ms_per_step = ms_total / args.steps
     value = N * args.steps / (ms_total * 1e-3)
+
+    if args.copy_check > 0:
+        src = torch.ones(args.copy_check * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:
