@@ -212,6 +212,54 @@ pict_status pict_copy_pressure_result_to_blocks(const pict_atlas *atlas, void *s
 pict_status pict_copy_pressure_result_from_blocks(const pict_atlas *atlas, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* backward stages (vector-Jacobian products)                                */
+/* ------------------------------------------------------------------------ */
+/* Inputs are the *_grad tensors of the stage outputs; outputs are the
+   *_grad tensors of its inputs, which the caller zero-initialises (the
+   reference's Create*Grad).  Tensors a stage accumulates into are marked +=,
+   tensors it overwrites are marked =.  Orthogonal terms only. */
+
+/* SetupAdvectionMatrixGrad (PISOtorch.cpp:597-599; kernel PISO_build_matrix_GRAD
+   :3869-4075): C_grad, A_grad -> block velocity_grad (+=), viscosity grads (+=). */
+pict_status pict_setup_advection_matrix_grad(const pict_atlas *atlas, double time_step,
+                                             int32_t non_ortho_flags, void *stream);
+/* SetupAdvectionVelocityGrad (PISOtorch.cpp:603-605; kernel :4388-4476):
+   velocity_rhs_grad -> block velocity_grad (=), velocity_source_grad (+=),
+   fixed-boundary velocity_grad (+=), viscosity grads (+=). */
+pict_status pict_setup_advection_velocity_grad(const pict_atlas *atlas, double time_step,
+                                               int32_t non_ortho_flags,
+                                               int32_t apply_pressure_gradient, void *stream);
+/* SetupPressureMatrixGrad (PISOtorch.cpp:609-610; kernel :4967-5115): P_grad -> A_grad (+=). */
+pict_status pict_setup_pressure_matrix_grad(const pict_atlas *atlas, double time_step,
+                                            int32_t non_ortho_flags, int32_t use_face_transform,
+                                            void *stream);
+/* SetupPressureRHSdivGrad (PISOtorch.cpp:613-614; kernel :5423-5452):
+   pressure_rhs_div_grad -> pressure_rhs_grad (+=), fixed-boundary velocity_grad (+=). */
+pict_status pict_setup_pressure_rhs_div_grad(const pict_atlas *atlas, double time_step,
+                                             int32_t non_ortho_flags, int32_t use_face_transform,
+                                             int32_t time_step_norm, void *stream);
+/* SetupPressureRHSGrad (PISOtorch.cpp:611-612; host :5808-5844, kernel :5243-5369):
+   the divergence VJP above, then pressure_rhs_grad -> block velocity_grad (=),
+   velocity_result_grad (+=), C_grad (= on every row), A_grad (+=), velocity
+   source, fixed-boundary velocity and viscosity grads (+=). */
+pict_status pict_setup_pressure_rhs_grad(const pict_atlas *atlas, double time_step,
+                                         int32_t non_ortho_flags, int32_t use_face_transform,
+                                         int32_t time_step_norm, void *stream);
+/* SetupPressureCorrectionGrad (PISOtorch.cpp:607-608; host :5707-5752):
+   pressure RHS grad, then pressure matrix grad. */
+pict_status pict_setup_pressure_correction_grad(const pict_atlas *atlas, double time_step,
+                                                int32_t non_ortho_flags, int32_t use_face_transform,
+                                                int32_t time_step_norm, void *stream);
+/* CorrectVelocityGrad (PISOtorch.cpp:615; kernel :6250-6293): velocity_result_grad ->
+   pressure_rhs_grad (=), block pressure_grad (+=), A_grad (=). */
+pict_status pict_correct_velocity_grad(const pict_atlas *atlas, double time_step,
+                                       int32_t time_step_norm, void *stream);
+/* Copy{Velocity,Pressure}ResultGrad{To,From}Blocks (PISOtorch.cpp:619-621; :6819-6915). */
+pict_status pict_copy_velocity_result_grad_to_blocks(const pict_atlas *atlas, void *stream);
+pict_status pict_copy_velocity_result_grad_from_blocks(const pict_atlas *atlas, void *stream);
+pict_status pict_copy_pressure_result_grad_from_blocks(const pict_atlas *atlas, void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* linear solvers                                                            */
 /* ------------------------------------------------------------------------ */
 

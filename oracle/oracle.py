@@ -51,6 +51,13 @@ def lib(bits=64):
         L.oracle_coords_to_face_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                        ctypes.c_void_p]
         L.oracle_set_threads.argtypes = [ctypes.c_int]
+        for n in ("oracle_setup_advection_matrix_grad", "oracle_setup_advection_velocity_grad",
+                  "oracle_setup_pressure_rhs_div_grad", "oracle_correct_velocity_grad"):
+            getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.oracle_setup_pressure_matrix_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_setup_pressure_rhs_only_grad.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.oracle_copy_velocity_result_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.oracle_copy_pressure_result_grad_from_blocks.argtypes = [ctypes.c_void_p]
         _libs[bits] = L
     return _libs[bits]
 
@@ -193,6 +200,59 @@ def CopyPressureResultToBlocks(domain):
 def CopyPressureResultFromBlocks(domain):
     L = lib(_bits(domain))
     _chk(L, L.oracle_copy_pressure_result(_desc(domain), 0), "copy")
+
+
+# backward stages (same names and arguments as PISOtorch's *Grad functions)
+
+def SetupAdvectionMatrixGrad(domain, timeStep, nonOrthoFlags=0):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_advection_matrix_grad(_desc(domain), _dt(timeStep), int(nonOrthoFlags)), "adv matrix grad")
+
+
+def SetupAdvectionVelocityGrad(domain, timeStep, nonOrthoFlags=0, applyPressureGradient=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_advection_velocity_grad(_desc(domain), _dt(timeStep), int(nonOrthoFlags)), "adv rhs grad")
+
+
+def SetupPressureMatrixGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_pressure_matrix_grad(_desc(domain), int(nonOrthoMode)), "pressure matrix grad")
+
+
+def SetupPressureRHSdivGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_pressure_rhs_div_grad(_desc(domain), _dt(timeStep), int(timeStepNorm)), "div grad")
+
+
+def SetupPressureRHSGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    SetupPressureRHSdivGrad(domain, timeStep, nonOrthoMode, useFaceTransform, timeStepNorm)
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_pressure_rhs_only_grad(_desc(domain), _dt(timeStep)), "pressure rhs grad")
+
+
+def SetupPressureCorrectionGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    SetupPressureRHSGrad(domain, timeStep, nonOrthoMode, useFaceTransform, timeStepNorm)
+    SetupPressureMatrixGrad(domain, timeStep, nonOrthoMode, useFaceTransform)
+
+
+def CorrectVelocityGrad(domain, timeStep, timeStepNorm=False):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_correct_velocity_grad(_desc(domain), _dt(timeStep), int(timeStepNorm)), "correct velocity grad")
+
+
+def CopyVelocityResultGradToBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_velocity_result_grad(_desc(domain), 1), "copy grad")
+
+
+def CopyVelocityResultGradFromBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_velocity_result_grad(_desc(domain), 0), "copy grad")
+
+
+def CopyPressureResultGradFromBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_pressure_result_grad_from_blocks(_desc(domain)), "copy grad")
 
 
 class SolveInfo:

@@ -657,6 +657,378 @@ int oracle_copy_pressure_result(const pict_domain_desc *d, int to_blocks) {
 }
 
 /* ------------------------------------------------------------------ */
+/* backward stages (vector-Jacobian products), serial                   */
+/* ------------------------------------------------------------------ */
+
+/* Slot -> CSR position of the cell's sorted row (LoadCSRrowNeighborSorted :3503-3563). */
+static void slot_positions(const Ctx *c, int bi, I4 pos, int g, int *at) {
+    int idx[7];
+    row_slots(c, bi, pos, idx);
+    for (int k = 0; k < 7; ++k) {
+        if (idx[k] < 0) { at[k] = -1; continue; }
+        int r = 0;
+        for (int j = 0; j < 7; ++j) r += (idx[j] >= 0 && idx[j] < idx[k]);
+        at[k] = c->d->csr_row[g] + r;
+    }
+}
+
+/* scatterViscosityBlock_GRAD :1852-1873 */
+static void scatter_viscosity(const Ctx *c, const pict_block_desc *b, I4 p, real g) {
+    real *dst = NULL;
+    if (!b->viscosity) dst = (real *)c->d->viscosity_grad;
+    else if (b->viscosity_static) dst = (real *)b->viscosity_grad;
+    else if (b->viscosity_grad) { p.a[3] = 0; dst = (real *)b->viscosity_grad + flat4(p, b->stride); }
+    if (dst) *dst += g;
+}
+/* scatterBlockVelocitySource_GRAD :2255-2264 */
+static void scatter_velocity_source(const pict_block_desc *b, I4 p, real g) {
+    if (!b->velocity_source_grad) return;
+    real *dst = (real *)b->velocity_source_grad;
+    if (b->velocity_source_static) dst[p.a[3]] += g; else dst[flat4(p, b->stride)] += g;
+}
+/* scatterFixedBoundaryData (VELOCITY_GRAD) :2451-2476 */
+static void scatter_fixed_velocity(const pict_boundary_desc *fb, I4 p, real g) {
+    if (!fb->velocity_grad) return;
+    real *dst = (real *)fb->velocity_grad;
+    if (fb->velocity_static) dst[p.a[3]] += g; else dst[flat4(p, fb->stride)] += g;
+}
+/* scatterContravariantComponentGradBoundaryFixed :700-716, :545-560 */
+static void scatter_fixed_contravariant(const Ctx *c, const pict_boundary_desc *fb, I4 p, real g) {
+    if (!fb->velocity_grad) return;
+    int comp = p.a[3];
+    if (fb->has_transform) {
+        const real *T = bound_T(c, fb, p);
+        for (int j = 0; j < c->D; ++j) { p.a[3] = j; scatter_fixed_velocity(fb, p, T_MINV(c, T, comp, j) * T_DET(c, T) * g); }
+    } else {
+        scatter_fixed_velocity(fb, p, g);
+    }
+}
+/* scatterContravariantComponentGrad :684-698, :527-543 */
+static void scatter_contravariant(const Ctx *c, const pict_block_desc *b, I4 p, real g, real *vgg) {
+    int comp = p.a[3];
+    if (b->has_transform) {
+        const real *T = cell_T(c, b, p);
+        for (int j = 0; j < c->D; ++j) {
+            p.a[3] = j;
+            real v = T_MINV(c, T, comp, j) * (T_DET(c, T) * g);
+            if (vgg) vgg[(size_t)j * c->N + global_index(p, b)] += v;
+            else ((real *)b->velocity_grad)[flat4(p, b->stride)] += v;
+        }
+    } else {
+        if (vgg) vgg[(size_t)comp * c->N + global_index(p, b)] += g;
+        else ((real *)b->velocity_grad)[flat4(p, b->stride)] += g;
+    }
+}
+/* ScatterFluxesGradNDLoop :1635-1699 */
+static void scatter_fluxes(const Ctx *c, int bi, I4 pos, const real *fg, real *vgg) {
+    const pict_block_desc *b = blk(c, bi);
+    for (int dim = 0; dim < c->D; ++dim) {
+        I4 t = pos; t.a[3] = dim;
+        real own = 0;
+        for (int up = 0; up < 2; ++up) {
+            int face = 2 * dim + up;
+            real g = fg[face];
+            t.a[dim] = pos.a[dim];
+            if (at_bound(pos, face, b)) {
+                const pict_boundary_desc *bd = &b->bounds[face];
+                if (bd->type == PICT_BOUNDARY_FIXED) {
+                    t.a[dim] = 0;
+                    scatter_fixed_contravariant(c, bd, t, g);
+                } else if (bd->type == PICT_BOUNDARY_CONNECTED) {
+                    real of = ((bd->axes[0] & 1) == up) ? (real)-0.5 : (real)0.5;
+                    I4 o = connected_pos(c, t, dim, bd, 0);
+                    o.a[3] = bd->axes[0] >> 1;
+                    scatter_contravariant(c, blk(c, bd->connected_block), o, of * g, vgg);
+                    own += (real)0.5 * g;
+                } else if (bd->type == PICT_BOUNDARY_PERIODIC) {
+                    t.a[dim] = up ? 0 : b->size[dim] - 1;
+                    scatter_contravariant(c, b, t, (real)0.5 * g, vgg);
+                    own += (real)0.5 * g;
+                }
+            } else {
+                t.a[dim] = pos.a[dim] + (up ? 1 : -1);
+                scatter_contravariant(c, b, t, (real)0.5 * g, vgg);
+                own += (real)0.5 * g;
+            }
+        }
+        t.a[dim] = pos.a[dim];
+        scatter_contravariant(c, b, t, own, vgg);
+    }
+}
+/* VJP of fixed_boundary_source: kPISO_build_advection_RHS_GRAD :4415-4465,
+   PISO_build_pressure_rhs_GRAD :5302-5355 (G = gradient of the source term / det) */
+static void fixed_boundary_source_grad(const Ctx *c, const pict_block_desc *b, I4 pos, int d, real G) {
+    for (int face = 0; face < 2 * c->D; ++face) {
+        if (!at_bound(pos, face, b)) continue;
+        const pict_boundary_desc *fb = &b->bounds[face];
+        if (fb->type != PICT_BOUNDARY_FIXED) continue;
+        int ax = face >> 1;
+        real normal = (face & 1) ? 1 : -1;
+        I4 t = pos; t.a[3] = d; t.a[ax] = 0;
+        real vel = fixed_vel(c, fb, t, d);
+        t.a[3] = ax;
+        real alpha = laplace_orth_fixed(c, fb, t);
+        real flux = fixed_contravariant(c, fb, t) * normal;
+        real slip = fb->velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
+        real visc = viscosity(c, b, pos);
+        scatter_fixed_contravariant(c, fb, t, (-vel * G) * normal);
+        t.a[3] = d;
+        scatter_fixed_velocity(fb, t, -flux * G + (1 - slip) * visc * 2 * alpha * G);
+        scatter_viscosity(c, b, pos, vel * (1 - slip) * 2 * alpha * G);
+    }
+}
+
+/* PISO_build_matrix_GRAD :3869-4075 */
+int oracle_setup_advection_matrix_grad(const pict_domain_desc *d, double time_step, int flags) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    if ((st = check_ortho_flags(flags))) return st;
+    (void)time_step;
+    const real *Cg = (const real *)d->C_grad, *Ag = (const real *)d->A_grad;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            int g = b->global_offset + f;
+            real rdet = 1 / cell_det(&c, b, pos);
+            int at[7]; slot_positions(&c, bi, pos, g, at);
+            real cg[7];
+            for (int k = 0; k < 7; ++k) cg[k] = at[k] >= 0 ? Cg[at[k]] * rdet : 0;
+            real diag_grad = Ag[g] * rdet + cg[0];
+            real alpha_p[3] = {1, 1, 1};
+            for (int dd = 0; dd < c.D; ++dd) { I4 t = pos; t.a[3] = dd; alpha_p[dd] = laplace_orth(&c, b, t); }
+            real fg[6] = {0, 0, 0, 0, 0, 0}, visc_grad = 0;
+            for (int face = 0; face < 2 * c.D; ++face) {
+                int dim = face >> 1;
+                real fs = (face & 1) ? 1 : -1;
+                if (!(at_bound(pos, face, b) && prescribed(face, b))) {
+                    int nb;
+                    I4 t = neighbor_cell(&c, b, pos, face, &nb, bi);
+                    real alpha = laplace_orth(&c, blk(&c, nb), t);
+                    fg[face] = fs * (real)0.5 * (diag_grad + cg[face + 1]);
+                    real vcg = diag_grad - cg[face + 1];
+                    visc_grad += (real)0.5 * alpha_p[dim] * vcg;
+                    t.a[3] = 0;
+                    scatter_viscosity(&c, blk(&c, nb), t, (real)0.5 * alpha * vcg);
+                } else {
+                    real slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
+                    visc_grad += (1 - slip) * 2 * alpha_p[dim] * diag_grad;
+                }
+            }
+            if (b->velocity_grad) scatter_fluxes(&c, bi, pos, fg, NULL);
+            scatter_viscosity(&c, b, pos, visc_grad);
+        }
+    }
+    return PICT_OK;
+}
+
+/* kPISO_build_advection_RHS_GRAD :4388-4476 */
+int oracle_setup_advection_velocity_grad(const pict_domain_desc *d, double time_step, int flags) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    if ((st = check_ortho_flags(flags))) return st;
+    const real dt = (real)time_step;
+    const real *rg = (const real *)d->velocity_rhs_grad;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            real det = cell_det(&c, b, pos);
+            for (int dd = 0; dd < c.D; ++dd) {
+                I4 t = pos; t.a[3] = dd;
+                real G = rg[(size_t)dd * c.N + b->global_offset + f];
+                scatter_velocity_source(b, t, G);
+                G /= det;
+                fixed_boundary_source_grad(&c, b, pos, dd, G);
+                G *= det;
+                if (b->velocity_grad) ((real *)b->velocity_grad)[flat4(t, b->stride)] = G / dt;
+            }
+        }
+    }
+    return PICT_OK;
+}
+
+/* PISO_build_pressure_matrix_GRAD :4967-5115 */
+int oracle_setup_pressure_matrix_grad(const pict_domain_desc *d, int flags) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    if ((st = check_ortho_flags(flags))) return st;
+    const real *Pg = (const real *)d->P_grad, *A = (const real *)d->A;
+    real *Ag = (real *)d->A_grad;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            int g = b->global_offset + f;
+            int at[7]; slot_positions(&c, bi, pos, g, at);
+            real raP_grad = 0;
+            for (int face = 0; face < 2 * c.D; ++face) {
+                if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+                int dim = face >> 1, nb;
+                I4 t = neighbor_cell(&c, b, pos, face, &nb, bi);
+                real alphaN = laplace_orth(&c, blk(&c, nb), t);
+                I4 tp = pos; tp.a[3] = dim;
+                real alphaP = laplace_orth(&c, b, tp);
+                t.a[3] = 0;
+                int gn = global_index(t, blk(&c, nb));
+                real raN = 1 / A[gn];
+                real cg = Pg[at[face + 1]] - Pg[at[0]];
+                raP_grad += (real)0.5 * alphaP * cg;
+                Ag[gn] += -((real)0.5 * alphaN * cg) * raN * raN;
+            }
+            real raP = 1 / A[g];
+            Ag[g] += -raP_grad * raP * raP;
+        }
+    }
+    return PICT_OK;
+}
+
+/* k_computePressureRHSdivergenceFromFlux_GRAD :5423-5452 */
+int oracle_setup_pressure_rhs_div_grad(const pict_domain_desc *d, double time_step, int time_step_norm) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    const real dt = (real)time_step;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            real dg = ((const real *)d->pressure_rhs_div_grad)[b->global_offset + f];
+            if (time_step_norm) dg /= dt;
+            real fg[6];
+            for (int dd = 0; dd < c.D; ++dd) { fg[2 * dd + 1] = dg; fg[2 * dd] = -dg; }
+            scatter_fluxes(&c, bi, unflat(f, b), fg, (real *)d->pressure_rhs_grad);
+        }
+    }
+    return PICT_OK;
+}
+
+/* PISO_build_pressure_rhs_GRAD :5243-5369 */
+int oracle_setup_pressure_rhs_only_grad(const pict_domain_desc *d, double time_step) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    const real dt = (real)time_step;
+    const real *A = (const real *)d->A, *Cv = (const real *)d->C_value, *vr = (const real *)d->velocity_result;
+    const real *prhs = (const real *)d->pressure_rhs, *pg = (const real *)d->pressure_rhs_grad;
+    real *vrg = (real *)d->velocity_result_grad, *Cg = (real *)d->C_grad, *Ag = (real *)d->A_grad;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            int g = b->global_offset + f;
+            real rDiag = 1 / A[g], det = cell_det(&c, b, pos), rDiag_grad = 0;
+            int cs = d->csr_row[g], ce = d->csr_row[g + 1];
+            real cvg[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int dd = 0; dd < c.D; ++dd) {
+                I4 t = pos; t.a[3] = dd;
+                size_t o = (size_t)c.N * dd;
+                real pGrad = pg[g + o];
+                rDiag_grad += prhs[g + o] / rDiag * pGrad;
+                if (b->velocity_grad) ((real *)b->velocity_grad)[flat4(t, b->stride)] = rDiag / dt * pGrad;
+                real w = rDiag * pGrad;
+                for (int k = cs; k < ce; ++k) {
+                    int col = d->csr_index[k];
+                    if (col == g) continue;
+                    if (vrg) vrg[col + o] += -Cv[k] * w;
+                    cvg[k - cs] -= vr[col + o] * w;
+                }
+                real S_grad = w;
+                scatter_velocity_source(b, t, S_grad);
+                S_grad /= det;
+                fixed_boundary_source_grad(&c, b, pos, dd, S_grad);
+            }
+            if (Cg) for (int k = cs; k < ce; ++k) Cg[k] = cvg[k - cs];
+            if (Ag) Ag[g] += rDiag_grad * (-rDiag * rDiag);
+        }
+    }
+    return PICT_OK;
+}
+
+/* scatterPressureGradToWithBounds :3411-3477 */
+static void scatter_pressure_to(const Ctx *c, const pict_block_desc *b, I4 pos, real g) {
+    I4 t = pos; t.a[3] = 0;
+    for (int dim = 0; dim < c->D; ++dim) {
+        if (pos.a[dim] < 0 || pos.a[dim] >= b->size[dim]) {
+            int lower = pos.a[dim] < 0;
+            const pict_boundary_desc *bd = &b->bounds[2 * dim + (lower ? 0 : 1)];
+            if (bd->type == PICT_BOUNDARY_FIXED) {
+                t.a[dim] = lower ? 0 : b->size[dim] - 1;
+                ((real *)b->pressure_grad)[flat4(t, b->stride)] += g;
+            } else if (bd->type == PICT_BOUNDARY_CONNECTED) {
+                const pict_block_desc *ob = blk(c, bd->connected_block);
+                I4 o = connected_pos(c, t, dim, bd, lower ? -(pos.a[dim] + 1) : pos.a[dim] - b->size[dim]);
+                ((real *)ob->pressure_grad)[flat4(o, ob->stride)] += g;
+            } else if (bd->type == PICT_BOUNDARY_PERIODIC) {
+                t.a[dim] += lower ? b->size[dim] : -b->size[dim];
+                ((real *)b->pressure_grad)[flat4(t, b->stride)] += g;
+            }
+            return;
+        }
+    }
+    ((real *)b->pressure_grad)[flat4(t, b->stride)] += g;
+}
+
+/* PISO_update_velocity_GRAD :6250-6293 with scatterPressureGradientGrad :838-870 */
+int oracle_correct_velocity_grad(const pict_domain_desc *d, double time_step, int time_step_norm) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    const real dt = (real)time_step;
+    const real *A = (const real *)d->A, *vrg = (const real *)d->velocity_result_grad;
+    real *prg = (real *)d->pressure_rhs_grad, *Ag = (real *)d->A_grad;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            int g = b->global_offset + f;
+            real rDiag = 1 / A[g], pg[3], pgg[3] = {0, 0, 0}, rDiag_grad = 0;
+            pressure_gradient(&c, b, pos, pg);
+            for (int dd = 0; dd < c.D; ++dd) {
+                size_t o = (size_t)c.N * dd;
+                real vug = vrg[g + o];
+                prg[g + o] = vug;
+                if (time_step_norm) vug *= dt;
+                pgg[dd] = -rDiag * vug;
+                rDiag_grad -= pg[dd] * vug;
+            }
+            if (b->pressure_grad) {
+                real v[3];
+                if (b->has_transform) {
+                    const real *T = cell_T(&c, b, pos);
+                    for (int i = 0; i < c.D; ++i) { real s = 0; for (int j = 0; j < c.D; ++j) s += T_MINV(&c, T, i, j) * pgg[j]; v[i] = s; }
+                } else {
+                    for (int i = 0; i < c.D; ++i) v[i] = pgg[i];
+                }
+                for (int dim = 0; dim < c.D; ++dim) {
+                    I4 pP = pos, pN = pos;
+                    real fac = (real)0.5;
+                    if (!(pos.a[dim] == 0 && prescribed(2 * dim, b))) pN.a[dim] = pos.a[dim] - 1; else fac = 1;
+                    if (!(pos.a[dim] == b->size[dim] - 1 && prescribed(2 * dim + 1, b))) pP.a[dim] = pos.a[dim] + 1; else fac = 1;
+                    scatter_pressure_to(&c, b, pN, v[dim] * -fac);
+                    scatter_pressure_to(&c, b, pP, v[dim] * fac);
+                }
+            }
+            Ag[g] = -rDiag_grad * rDiag * rDiag;
+        }
+    }
+    return PICT_OK;
+}
+
+/* Copy{Velocity,Pressure}ResultGrad{To,From}Blocks :6819-6915 */
+int oracle_copy_velocity_result_grad(const pict_domain_desc *d, int to_blocks) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int dd = 0; dd < c.D; ++dd) {
+            real *bv = (real *)b->velocity_grad + (size_t)dd * b->stride[3];
+            real *gv = (real *)d->velocity_result_grad + (size_t)dd * c.N + b->global_offset;
+            if (to_blocks) memcpy(bv, gv, sizeof(real) * b->stride[3]);
+            else memcpy(gv, bv, sizeof(real) * b->stride[3]);
+        }
+    }
+    return PICT_OK;
+}
+int oracle_copy_pressure_result_grad_from_blocks(const pict_domain_desc *d) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        memcpy((real *)d->pressure_result_grad + b->global_offset, b->pressure_grad, sizeof(real) * b->stride[3]);
+    }
+    return PICT_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* Krylov solvers                                                       */
 /* ------------------------------------------------------------------ */
 

@@ -829,8 +829,11 @@ class Block:
         self._changed = True
 
     def CreateViscosityGrad(self):
+        """Block::CreateViscosityGrad (domain_structs.cpp:1510-1517): cleared when the block has no viscosity."""
         if self._viscosity is not None:
             self.setViscosityGrad(torch.zeros_like(self._viscosity))
+        else:
+            self.clearViscosityGrad()
 
     def hasViscosityGrad(self):
         return self._viscosity_grad is not None
@@ -1358,7 +1361,11 @@ class Domain:
         self._changed = True
 
     def CreateViscosityGrad(self):
+        """Domain::CreateViscosityGrad (domain_structs.cpp:2875-2886): the domain's [1] gradient (a device tensor,
+        while the viscosity itself is a host tensor) and the blocks' viscosity gradients."""
         self.setViscosityGrad(torch.zeros(1, dtype=self._dtype, device=self._device))
+        for b in self.blocks:
+            b.CreateViscosityGrad()
 
     # --- blocks ---
     def CreateBlock(self, velocity=None, pressure=None, passiveScalar=None, vertexCoordinates=None, name="Block"):
@@ -1883,6 +1890,80 @@ def CopyScalarResultToBlocks(domain):
 
 
 CopyScalarResultFromBlocks = CopyScalarResultToBlocks
+
+
+# ----------------------------------------------------------------------------
+# backward stages (PISOtorch.cpp:596-621): read the *_grad tensors of a stage's outputs, write the *_grad tensors
+# of its inputs; the caller creates those (zero-initialised) first, as PISOtorch_diff does
+# ----------------------------------------------------------------------------
+
+def SetupAdvectionMatrixGrad(domain, timeStep, nonOrthoFlags, forPassiveScalar=False, passiveScalarChannel=0):
+    dt = _check_domain(domain, timeStep)
+    if forPassiveScalar:
+        raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+    _call(domain, "pict_setup_advection_matrix_grad", dt, domain._effective_non_ortho_flags(nonOrthoFlags))
+
+
+def SetupAdvectionScalarGrad(domain, timeStep, nonOrthoFlags):
+    _check_domain(domain, timeStep)
+    raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+
+
+def SetupAdvectionVelocityGrad(domain, timeStep, nonOrthoFlags, applyPressureGradient=False):
+    dt = _check_domain(domain, timeStep)
+    _call(domain, "pict_setup_advection_velocity_grad", dt, domain._effective_non_ortho_flags(nonOrthoFlags),
+          int(bool(applyPressureGradient)))
+
+
+def SetupPressureCorrectionGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    dt = _check_domain(domain, timeStep)
+    _call(domain, "pict_setup_pressure_correction_grad", dt, domain._effective_non_ortho_flags(nonOrthoMode),
+          int(bool(useFaceTransform)), int(bool(timeStepNorm)))
+
+
+def SetupPressureMatrixGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False):
+    dt = _check_domain(domain, timeStep)
+    _call(domain, "pict_setup_pressure_matrix_grad", dt, domain._effective_non_ortho_flags(nonOrthoMode),
+          int(bool(useFaceTransform)))
+
+
+def SetupPressureRHSGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    dt = _check_domain(domain, timeStep)
+    _call(domain, "pict_setup_pressure_rhs_grad", dt, domain._effective_non_ortho_flags(nonOrthoMode),
+          int(bool(useFaceTransform)), int(bool(timeStepNorm)))
+
+
+def SetupPressureRHSdivGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    dt = _check_domain(domain, timeStep)
+    _call(domain, "pict_setup_pressure_rhs_div_grad", dt, domain._effective_non_ortho_flags(nonOrthoMode),
+          int(bool(useFaceTransform)), int(bool(timeStepNorm)))
+
+
+def CorrectVelocityGrad(domain, timeStep, timeStepNorm=False):
+    dt = _check_domain(domain, timeStep)
+    _call(domain, "pict_correct_velocity_grad", dt, int(bool(timeStepNorm)))
+
+
+def CopyVelocityResultGradToBlocks(domain):
+    _check_domain(domain)
+    _call(domain, "pict_copy_velocity_result_grad_to_blocks")
+
+
+def CopyVelocityResultGradFromBlocks(domain):
+    _check_domain(domain)
+    _call(domain, "pict_copy_velocity_result_grad_from_blocks")
+
+
+def CopyPressureResultGradFromBlocks(domain):
+    _check_domain(domain)
+    _call(domain, "pict_copy_pressure_result_grad_from_blocks")
+
+
+def CopyScalarResultGradFromBlocks(domain):
+    raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+
+
+CopyScalarResultGradToBlocks = CopyScalarResultGradFromBlocks
 
 
 # ----------------------------------------------------------------------------

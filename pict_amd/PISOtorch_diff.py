@@ -138,8 +138,478 @@ def linear_solve_GPU(csrMat, rhs, transpose=False, use_BiCG=False, tol=None, max
     return LinearSolveFunction.apply(csrMat.value, rhs)
 
 
-# Forward-only passthroughs (the differentiable stage wrappers are defined in PISOtorch_diff_stages)
-from .PISOtorch import (  # noqa: E402,F401
-    CopyPressureResultFromBlocks, CopyPressureResultToBlocks, CopyVelocityResultFromBlocks, CopyVelocityResultToBlocks,
-    CorrectVelocity, SetupAdvectionMatrix, SetupAdvectionVelocity, SetupPressureCorrection, SetupPressureMatrix,
-    SetupPressureRHS, SetupPressureRHSdiv)
+# ----------------------------------------------------------------------------
+# gradient bookkeeping (PISOtorch_diff.py:13-181)
+# ----------------------------------------------------------------------------
+
+_EXCLUDED_GRADIENTS = {}
+
+
+def exclude_gradient(function_name, grad_name, is_gradient_input):
+    """Zero a gradient input (is_gradient_input) or drop a gradient output of a stage (PISOtorch_diff.py:14-20)."""
+    if function_name not in _EXCLUDED_GRADIENTS:
+        raise ValueError("Function '%s' does not exist" % (function_name,))
+    if not grad_name.endswith("_GRAD"):
+        raise ValueError("Only gradients can be excluded (name should end with '_GRAD').")
+    _EXCLUDED_GRADIENTS[function_name][0 if is_gradient_input else 1].add(grad_name)
+
+
+def clear_excluded_gradients():
+    for inputs, outputs in _EXCLUDED_GRADIENTS.values():
+        inputs.clear()
+        outputs.clear()
+
+
+def _set_gradient_input(var_grad, grad_setter, function_name, grad_name):
+    if grad_name in _EXCLUDED_GRADIENTS[function_name][0]:
+        grad_setter(torch.zeros_like(var_grad))
+    else:
+        # the backward kernels may accumulate into their gradient inputs: never hand them autograd's own buffer
+        grad_setter(var_grad.contiguous().clone())
+
+
+def check_non_ortho_rhs(non_ortho_flags):
+    """NON_ORTHO_DIRECT_RHS (2) or NON_ORTHO_DIAGONAL_RHS (8) set (PISOtorch_diff.py:34-42)."""
+    return (non_ortho_flags & 2) > 0 or (non_ortho_flags & 8) > 0
+
+
+def _domain_entries(domain):
+    """(section, name, getter, setter, clear, on_cpu) for every tensor flatten_domain knows, in its order
+    (PISOtorch_diff.py:44-118)."""
+    d = domain
+    yield ("D", "C", lambda: d.C.value, d.C.setValue, None, False)
+    yield ("D", "C_GRAD", lambda: d.CGrad.value, d.CGrad.setValue, None, False)
+    yield ("D", "A", lambda: d.A, d.setA, None, False)
+    yield ("D", "A_GRAD", lambda: d.AGrad, d.setAGrad, None, False)
+    yield ("D", "P", lambda: d.P.value, d.P.setValue, None, False)
+    yield ("D", "P_GRAD", lambda: d.PGrad.value, d.PGrad.setValue, None, False)
+    yield ("D", "VISCOSITY", lambda: d.viscosity, d.setViscosity, None, False)
+    yield ("D", "VISCOSITY_GRAD", lambda: d.viscosityGrad, None, None, True)
+    yield ("D", "VELOCITY_RESULT", lambda: d.velocityResult, d.setVelocityResult, None, False)
+    yield ("D", "VELOCITY_RESULT_GRAD", lambda: d.velocityResultGrad, d.setVelocityResultGrad, None, False)
+    yield ("D", "PRESSURE_RESULT", lambda: d.pressureResult, d.setPressureResult, None, False)
+    yield ("D", "PRESSURE_RESULT_GRAD", lambda: d.pressureResultGrad, d.setPressureResultGrad, None, False)
+    yield ("D", "VELOCITY_RHS", lambda: d.velocityRHS, d.setVelocityRHS, None, False)
+    yield ("D", "VELOCITY_RHS_GRAD", lambda: d.velocityRHSGrad, d.setVelocityRHSGrad, None, False)
+    yield ("D", "PRESSURE_RHS", lambda: d.pressureRHS, d.setPressureRHS, None, False)
+    yield ("D", "PRESSURE_RHS_GRAD", lambda: d.pressureRHSGrad, d.setPressureRHSGrad, None, False)
+    yield ("D", "PRESSURE_RHS_DIV", lambda: d.pressureRHSdiv, d.setPressureRHSdiv, None, False)
+    yield ("D", "PRESSURE_RHS_DIV_GRAD", lambda: d.pressureRHSdivGrad, d.setPressureRHSdivGrad, None, False)
+    for bi, b in enumerate(d.getBlocks()):
+        sec = ("B", bi)
+        yield (sec, "VISCOSITY_BLOCK", lambda b=b: b.viscosity, b.setViscosity, b.clearViscosity, False)
+        yield (sec, "VISCOSITY_BLOCK_GRAD", lambda b=b: b.viscosityGrad, b.setViscosityGrad, None, False)
+        yield (sec, "VELOCITY", lambda b=b: b.velocity, b.setVelocity, None, False)
+        yield (sec, "VELOCITY_GRAD", lambda b=b: b.velocityGrad, b.setVelocityGrad, None, False)
+        yield (sec, "VELOCITY_SOURCE", lambda b=b: b.velocitySource, b.setVelocitySource, b.clearVelocitySource,
+               False)
+        yield (sec, "VELOCITY_SOURCE_GRAD", lambda b=b: b.velocitySourceGrad, b.setVelocitySourceGrad, None, False)
+        yield (sec, "PRESSURE", lambda b=b: b.pressure, b.setPressure, None, False)
+        yield (sec, "PRESSURE_GRAD", lambda b=b: b.pressureGrad, b.setPressureGrad, None, False)
+        for fi, fb in b.getFixedBoundaries():
+            bsec = ("F", bi, fi)
+            yield (bsec, "BOUNDARY_VELOCITY", lambda fb=fb: fb.velocity, fb.setVelocity, None, False)
+            yield (bsec, "BOUNDARY_VELOCITY_GRAD", lambda fb=fb: fb.velocityGrad, fb.setVelocityGrad, None, False)
+
+
+def flatten_domain(domain, tensor_filter, clone=False, empty=False, exclusion_list=()):
+    """The domain's tensors whose names pass `tensor_filter` as a flat list, plus the index map needed to put them
+    back (PISOtorch_diff.py:44-118).  Absent optional tensors are None; `empty` or `exclusion_list` entries too."""
+    index, tensors = {}, []
+    for sec, name, get, _, _, on_cpu in _domain_entries(domain):
+        if tensor_filter is not None and name not in tensor_filter:
+            continue
+        t = None if (empty or name in exclusion_list) else get()
+        if t is not None and clone:
+            t = t.clone()
+        if t is not None and on_cpu:
+            t = t.cpu()
+        index[(sec, name)] = len(tensors)
+        tensors.append(t)
+    return index, tensors
+
+
+def set_domain_tensors_from_flat(domain, domain_dict, tensor_list, tensor_filter=None, clone=False):
+    """Inverse of flatten_domain (PISOtorch_diff.py:120-181): set the listed tensors back on the domain (clearing
+    optional ones that were None)."""
+    for sec, name, _, setter, clear, _ in _domain_entries(domain):
+        if (sec, name) not in domain_dict or setter is None:
+            continue
+        if tensor_filter is not None and name not in tensor_filter:
+            continue
+        t = tensor_list[domain_dict[(sec, name)]]
+        if t is not None:
+            setter(t.clone() if clone else t)
+        elif clear is not None:
+            clear()
+
+
+def _stage(fn_name, tracked, saved, forward, backward, outputs):
+    """Builds a stage's autograd.Function: `tracked` inputs (flatten_domain names) are its differentiable inputs,
+    `saved` tensors are restored on the domain before the backward kernel runs, `outputs(domain)` are returned."""
+    grad_names = [t + "_GRAD" for t in tracked]
+
+    class StageFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, domain, time_step, *tracked_tensors):
+            forward(domain)
+            ctx.domain = domain
+            if saved:
+                ctx.saved_dict, tensors = flatten_domain(domain, saved)
+            else:
+                ctx.saved_dict, tensors = None, []
+            ctx.saved_none = [t is None for t in tensors]
+            ctx.save_for_backward(*[t for t in tensors if t is not None], time_step)
+            outs = outputs(domain)
+            return outs if isinstance(outs, tuple) else (outs,)
+
+        @staticmethod
+        @torch.autograd.function.once_differentiable
+        def backward(ctx, *out_grads):
+            domain = ctx.domain
+            if any(ctx.needs_input_grad[2:]):
+                st = list(ctx.saved_tensors)
+                time_step = st.pop()
+                if ctx.saved_dict is not None:
+                    it = iter(st)
+                    tensors = [None if none else next(it) for none in ctx.saved_none]
+                    set_domain_tensors_from_flat(domain, ctx.saved_dict, tensors)
+                backward(domain, time_step, out_grads)
+                _, grads = flatten_domain(domain, grad_names, exclusion_list=_EXCLUDED_GRADIENTS[fn_name][1])
+            else:
+                _, grads = flatten_domain(domain, grad_names, empty=True)
+            return (None, None, *grads)
+
+    return StageFunction
+
+
+def _apply(fn, domain, time_step, tracked):
+    _, tensors = flatten_domain(domain, tracked)
+    outs = fn.apply(domain, time_step, *tensors)
+    return outs
+
+
+_EXCLUDED_GRADIENTS["SetupAdvectionMatrix"] = (set(), set())
+
+
+def SetupAdvectionMatrix(domain, time_step, non_ortho_flags, forPassiveScalar=False, passiveScalarChannel=0):
+    """PISOtorch_diff.py:377-441: (A, C.value) from block velocity, boundary velocity and viscosities."""
+    if forPassiveScalar:
+        raise PISOtorch.PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+    tracked = ["VELOCITY", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK"]
+    name = "SetupAdvectionMatrix"
+
+    def fwd(d):
+        d.CreateA()
+        d.C.CreateValue()
+        d.UpdateDomainData()
+        PISOtorch.SetupAdvectionMatrix(d, time_step, non_ortho_flags)
+
+    def bwd(d, dt, grads):
+        _set_gradient_input(grads[0], d.setAGrad, name, "A_GRAD")
+        _set_gradient_input(grads[1], d.CGrad.setValue, name, "C_GRAD")
+        d.CreateViscosityGrad()
+        d.CreateVelocityGradOnBlocks()
+        d.CreateVelocityGradOnBoundaries()
+        d.UpdateDomainData()
+        PISOtorch.SetupAdvectionMatrixGrad(d, dt, non_ortho_flags)
+
+    fn = _stage(name, tracked, [], fwd, bwd, lambda d: (d.A, d.C.value))
+    return _apply(fn, domain, time_step, tracked)
+
+
+_EXCLUDED_GRADIENTS["SetupAdvectionVelocity"] = (set(), set())
+
+
+def SetupAdvectionVelocity(domain, time_step, non_ortho_flags, apply_pressure_gradient=False):
+    """PISOtorch_diff.py:531-614: velocityRHS from block velocity, velocity source, boundary velocity, viscosities."""
+    if check_non_ortho_rhs(non_ortho_flags):
+        raise PISOtorch.PictUnsupportedError("non-orthogonal RHS terms are not implemented by libpict_hip")
+    tracked = ["VELOCITY", "VELOCITY_SOURCE", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK"]
+    saved = ["BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK", "VELOCITY_SOURCE"]
+    name = "SetupAdvectionVelocity"
+
+    def fwd(d):
+        d.CreateVelocityRHS()
+        d.UpdateDomainData()
+        PISOtorch.SetupAdvectionVelocity(d, time_step, non_ortho_flags, apply_pressure_gradient)
+
+    def bwd(d, dt, grads):
+        _set_gradient_input(grads[0], d.setVelocityRHSGrad, name, "VELOCITY_RHS_GRAD")
+        d.CreateViscosityGrad()
+        d.CreateVelocityGradOnBlocks()
+        d.CreateVelocityGradOnBoundaries()
+        d.CreateVelocitySourceGradOnBlocks()
+        d.UpdateDomainData()
+        PISOtorch.SetupAdvectionVelocityGrad(d, dt, non_ortho_flags, apply_pressure_gradient)
+
+    fn = _stage(name, tracked, saved, fwd, bwd, lambda d: d.velocityRHS)
+    return _apply(fn, domain, time_step, tracked)[0]
+
+
+def _copy_stage(forward, backward, tensors_in, outputs):
+    class CopyFunction(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, domain, *inputs):
+            ctx.domain = domain
+            forward(domain)
+            outs = outputs(domain)
+            return outs if isinstance(outs, tuple) else (outs,)
+
+        @staticmethod
+        @torch.autograd.function.once_differentiable
+        def backward(ctx, *grads):
+            if any(ctx.needs_input_grad[1:]):
+                res = backward(ctx.domain, grads)
+            else:
+                res = [None] * (len(ctx.needs_input_grad) - 1)
+            return (None, *res)
+
+    return CopyFunction
+
+
+def CopyVelocityResultToBlocks(domain):
+    """PISOtorch_diff.py:697-729: block velocities from velocityResult."""
+    def fwd(d):
+        d.CreateVelocityOnBlocks()
+        d.UpdateDomainData()
+        PISOtorch.CopyVelocityResultToBlocks(d)
+
+    def bwd(d, grads):
+        for b, g in zip(d.getBlocks(), grads):
+            b.setVelocityGrad(g.contiguous())
+        d.CreateVelocityResultGrad()
+        d.UpdateDomainData()
+        PISOtorch.CopyVelocityResultGradFromBlocks(d)
+        return [d.velocityResultGrad]
+
+    fn = _copy_stage(fwd, bwd, None, lambda d: tuple(b.velocity for b in d.getBlocks()))
+    fn.apply(domain, domain.velocityResult)
+
+
+def CopyVelocityResultFromBlocks(domain):
+    """PISOtorch_diff.py:731-763: velocityResult from the block velocities."""
+    def fwd(d):
+        d.CreateVelocityResult()
+        d.UpdateDomainData()
+        PISOtorch.CopyVelocityResultFromBlocks(d)
+
+    def bwd(d, grads):
+        d.setVelocityResultGrad(grads[0].contiguous())
+        d.CreateVelocityGradOnBlocks()
+        d.UpdateDomainData()
+        PISOtorch.CopyVelocityResultGradToBlocks(d)
+        return [b.velocityGrad for b in d.getBlocks()]
+
+    fn = _copy_stage(fwd, bwd, None, lambda d: d.velocityResult)
+    fn.apply(domain, *[b.velocity for b in domain.getBlocks()])
+
+
+def CopyPressureResultToBlocks(domain):
+    """PISOtorch_diff.py:765-799: block pressures from pressureResult."""
+    def fwd(d):
+        d.CreatePressureOnBlocks()
+        d.UpdateDomainData()
+        PISOtorch.CopyPressureResultToBlocks(d)
+
+    def bwd(d, grads):
+        for b, g in zip(d.getBlocks(), grads):
+            b.setPressureGrad(g.contiguous())
+        d.CreatePressureResultGrad()
+        d.UpdateDomainData()
+        PISOtorch.CopyPressureResultGradFromBlocks(d)
+        return [d.pressureResultGrad]
+
+    fn = _copy_stage(fwd, bwd, None, lambda d: tuple(b.pressure for b in d.getBlocks()))
+    fn.apply(domain, domain.pressureResult)
+
+
+def CopyScalarResultToBlocks(domain):
+    raise PISOtorch.PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+
+
+CopyScalarResultFromBlocks = CopyScalarResultToBlocks
+
+
+def SetupAdvectionScalar(domain, time_step, non_ortho_flags):
+    raise PISOtorch.PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+
+
+_PRESSURE_TRACKED = ["VELOCITY", "VELOCITY_SOURCE", "VELOCITY_RESULT", "BOUNDARY_VELOCITY", "VISCOSITY",
+                     "VISCOSITY_BLOCK", "A", "C"]
+_PRESSURE_SAVED = ["BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK", "A", "C", "VELOCITY_RESULT", "PRESSURE_RHS",
+                   "VELOCITY_SOURCE"]
+
+
+def _pressure_rhs_grad_setup(d, name, prhs_grad, prhs_div_grad):
+    _set_gradient_input(prhs_grad, d.setPressureRHSGrad, name, "PRESSURE_RHS_GRAD")
+    _set_gradient_input(prhs_div_grad, d.setPressureRHSdivGrad, name, "PRESSURE_RHS_DIV_GRAD")
+    d.CreateAGrad()
+    d.CGrad.CreateValue()
+    d.CreateViscosityGrad()
+    d.CreateVelocityResultGrad()
+    d.CreateVelocityGradOnBlocks()
+    d.CreateVelocitySourceGradOnBlocks()
+    d.CreateVelocityGradOnBoundaries()
+
+
+_EXCLUDED_GRADIENTS["SetupPressureCorrection"] = (set(), set())
+
+
+def SetupPressureCorrection(domain, time_step, non_ortho_flags, use_face_transform=False, timeStepNorm=False):
+    """PISOtorch_diff.py:802-894: (P.value, pressureRHS, pressureRHSdiv)."""
+    if check_non_ortho_rhs(non_ortho_flags):
+        raise PISOtorch.PictUnsupportedError("non-orthogonal RHS terms are not implemented by libpict_hip")
+    name = "SetupPressureCorrection"
+
+    def fwd(d):
+        d.CreatePressureRHS()
+        d.CreatePressureRHSdiv()
+        d.P.CreateValue()
+        d.UpdateDomainData()
+        PISOtorch.SetupPressureCorrection(d, time_step, non_ortho_flags, use_face_transform, timeStepNorm=timeStepNorm)
+
+    def bwd(d, dt, grads):
+        if use_face_transform:
+            raise NotImplementedError("SetupPressureCorrection: Gradient does not support face transformations.")
+        _set_gradient_input(grads[0], d.PGrad.setValue, name, "P_GRAD")
+        _pressure_rhs_grad_setup(d, name, grads[1], grads[2])
+        d.UpdateDomainData()
+        PISOtorch.SetupPressureCorrectionGrad(d, dt, non_ortho_flags, use_face_transform, timeStepNorm)
+
+    fn = _stage(name, _PRESSURE_TRACKED, _PRESSURE_SAVED, fwd, bwd, lambda d: (d.P.value, d.pressureRHS,
+                                                                               d.pressureRHSdiv))
+    _apply(fn, domain, time_step, _PRESSURE_TRACKED)
+
+
+_EXCLUDED_GRADIENTS["SetupPressureMatrix"] = (set(), set())
+
+
+def SetupPressureMatrix(domain, time_step, non_ortho_flags, use_face_transform=False):
+    """PISOtorch_diff.py:897-964: P.value from A."""
+    name = "SetupPressureMatrix"
+
+    def fwd(d):
+        d.P.CreateValue()
+        d.UpdateDomainData()
+        PISOtorch.SetupPressureMatrix(d, time_step, non_ortho_flags, use_face_transform)
+
+    def bwd(d, dt, grads):
+        if use_face_transform:
+            raise NotImplementedError("SetupPressureMatrix: Gradient does not support face transformations.")
+        _set_gradient_input(grads[0], d.PGrad.setValue, name, "P_GRAD")
+        d.CreateAGrad()
+        d.UpdateDomainData()
+        PISOtorch.SetupPressureMatrixGrad(d, dt, non_ortho_flags, use_face_transform)
+
+    fn = _stage(name, ["A"], ["A"], fwd, bwd, lambda d: d.P.value)
+    _apply(fn, domain, time_step, ["A"])
+
+
+_EXCLUDED_GRADIENTS["SetupPressureRHS"] = (set(), set())
+
+
+def SetupPressureRHS(domain, time_step, non_ortho_flags, use_face_transform=False, timeStepNorm=False):
+    """PISOtorch_diff.py:967-1064: (pressureRHS, pressureRHSdiv)."""
+    if check_non_ortho_rhs(non_ortho_flags):
+        raise PISOtorch.PictUnsupportedError("non-orthogonal RHS terms are not implemented by libpict_hip")
+    name = "SetupPressureRHS"
+
+    def fwd(d):
+        d.CreatePressureRHS()
+        d.CreatePressureRHSdiv()
+        d.UpdateDomainData()
+        PISOtorch.SetupPressureRHS(d, time_step, non_ortho_flags, use_face_transform, timeStepNorm=timeStepNorm)
+
+    def bwd(d, dt, grads):
+        if use_face_transform:
+            raise NotImplementedError("SetupPressureRHS: Gradient does not support face transformations.")
+        _pressure_rhs_grad_setup(d, name, grads[0], grads[1])
+        d.UpdateDomainData()
+        PISOtorch.SetupPressureRHSGrad(d, dt, non_ortho_flags, use_face_transform, timeStepNorm)
+
+    fn = _stage(name, _PRESSURE_TRACKED, _PRESSURE_SAVED, fwd, bwd, lambda d: (d.pressureRHS, d.pressureRHSdiv))
+    _apply(fn, domain, time_step, _PRESSURE_TRACKED)
+
+
+_EXCLUDED_GRADIENTS["SetupPressureRHSdiv"] = (set(), set())
+
+
+def SetupPressureRHSdiv(domain, time_step, non_ortho_flags, use_face_transform=False, timeStepNorm=False):
+    """PISOtorch_diff.py:1067-1157: pressureRHSdiv from pressureRHS (and the boundary velocities)."""
+    if check_non_ortho_rhs(non_ortho_flags):
+        raise PISOtorch.PictUnsupportedError("non-orthogonal RHS terms are not implemented by libpict_hip")
+    name = "SetupPressureRHSdiv"
+    tracked = ["PRESSURE_RHS", "BOUNDARY_VELOCITY"]
+
+    def fwd(d):
+        d.CreatePressureRHSdiv()
+        d.UpdateDomainData()
+        PISOtorch.SetupPressureRHSdiv(d, time_step, non_ortho_flags, use_face_transform, timeStepNorm=timeStepNorm)
+
+    def bwd(d, dt, grads):
+        if use_face_transform:
+            raise NotImplementedError("SetupPressureRHSdiv: Gradient does not support face transformations.")
+        _set_gradient_input(grads[0], d.setPressureRHSdivGrad, name, "PRESSURE_RHS_DIV_GRAD")
+        d.CreatePressureRHSGrad()
+        d.CreateVelocityGradOnBoundaries()
+        d.UpdateDomainData()
+        PISOtorch.SetupPressureRHSdivGrad(d, dt, non_ortho_flags, use_face_transform, timeStepNorm)
+
+    fn = _stage(name, tracked, [], fwd, bwd, lambda d: d.pressureRHSdiv)
+    _apply(fn, domain, time_step, tracked)
+
+
+_EXCLUDED_GRADIENTS["CorrectVelocity"] = (set(), set())
+
+
+def CorrectVelocity(domain, time_step, version, timeStepNorm=False):
+    """PISOtorch_diff.py:1160-1234: velocityResult from A, block pressures and pressureRHS."""
+    name = "CorrectVelocity"
+    tracked = ["A", "PRESSURE", "PRESSURE_RHS"]
+
+    def fwd(d):
+        d.CreateVelocityResult()
+        d.UpdateDomainData()
+        PISOtorch.CorrectVelocity(d, time_step, version, timeStepNorm=timeStepNorm)
+
+    def bwd(d, dt, grads):
+        if version not in (0, 1):
+            raise NotImplementedError("CorrectVelocity: Gradient only implemented for Finite Difference pressure "
+                                      "gradients (version 0 or 1).")
+        _set_gradient_input(grads[0], d.setVelocityResultGrad, name, "VELOCITY_RESULT_GRAD")
+        d.CreateAGrad()
+        d.CreatePressureGradOnBlocks()
+        d.CreatePressureRHSGrad()
+        d.UpdateDomainData()
+        PISOtorch.CorrectVelocityGrad(d, dt, timeStepNorm)
+
+    fn = _stage(name, tracked, ["A", "PRESSURE"], fwd, bwd, lambda d: d.velocityResult)
+    _apply(fn, domain, time_step, tracked)
+
+
+def reset_domain(domain):
+    """PISOtorch_diff.py:1238-1242: fresh forward tensors, so that autograd graphs held by the domain are freed."""
+    domain.CreateVelocityOnBlocks()
+    domain.CreatePressureOnBlocks()
+    domain.PrepareSolve()
+
+
+def detach_domain_fwd(domain):
+    domain.DetachFwd()
+
+
+def detach_domain_grad(domain):
+    domain.DetachGrad()
+
+
+def detach_domain(domain):
+    domain.Detach()
+
+
+def is_tensor_empty(tensor):
+    return tensor.dim() == 1 and tensor.size(0) == 0
+
+
+# Forward-only passthroughs (no gradient in the reference either)
+from .PISOtorch import CopyPressureResultFromBlocks  # noqa: E402,F401

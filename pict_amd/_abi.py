@@ -157,6 +157,16 @@ SIGNATURES = {
     "pict_copy_velocity_result_from_blocks": (c_int32, [_P, _P]),
     "pict_copy_pressure_result_to_blocks": (c_int32, [_P, _P]),
     "pict_copy_pressure_result_from_blocks": (c_int32, [_P, _P]),
+    "pict_setup_advection_matrix_grad": (c_int32, [_P, c_double, c_int32, _P]),
+    "pict_setup_advection_velocity_grad": (c_int32, [_P, c_double, c_int32, c_int32, _P]),
+    "pict_setup_pressure_matrix_grad": (c_int32, [_P, c_double, c_int32, c_int32, _P]),
+    "pict_setup_pressure_rhs_div_grad": (c_int32, [_P, c_double, c_int32, c_int32, c_int32, _P]),
+    "pict_setup_pressure_rhs_grad": (c_int32, [_P, c_double, c_int32, c_int32, c_int32, _P]),
+    "pict_setup_pressure_correction_grad": (c_int32, [_P, c_double, c_int32, c_int32, c_int32, _P]),
+    "pict_correct_velocity_grad": (c_int32, [_P, c_double, c_int32, _P]),
+    "pict_copy_velocity_result_grad_to_blocks": (c_int32, [_P, _P]),
+    "pict_copy_velocity_result_grad_from_blocks": (c_int32, [_P, _P]),
+    "pict_copy_pressure_result_grad_from_blocks": (c_int32, [_P, _P]),
     "pict_solve_linear": (c_int32, [POINTER(CsrDesc), _P, _P, c_int32, POINTER(SolveOpts), POINTER(SolveInfo), _P]),
     "pict_sparse_outer_product": (c_int32, [_P, _P, POINTER(CsrDesc), _P, _P]),
     "pict_set_kernel_timing": (c_int32, [c_int32]),

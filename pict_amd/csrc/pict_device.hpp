@@ -353,6 +353,23 @@ __device__ __forceinline__ void pressure_gradient(const DomainK<R> &dom, const p
     }
 }
 
+// Unsorted row entries of a cell: slot 0 = diagonal, slot face+1 = neighbour
+// across `face` (-1 for prescribed faces); PISO_build_matrix :3663-3772.
+template <typename R, int D>
+__device__ __forceinline__ void row_slots(const DomainK<R> &dom, const pict_block_desc *b, int bi, const I4 &pos,
+                                          int *idx) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) idx[k] = -1;
+    idx[0] = global_index(pos, b);
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+        int nb;
+        const I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
+        idx[face + 1] = global_index(t, dom.blocks + nb);
+    }
+}
+
 // findLowestColumnIndex :3479-3489
 __device__ __forceinline__ int lowest_slot(const int *idx) {
     int v = 0x7fffffff, k = -1;

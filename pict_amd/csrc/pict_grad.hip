@@ -1,0 +1,525 @@
+// pict_grad.hip -- backward (vector-Jacobian) kernels of the PISO stages.
+//
+// Each kernel is the exact VJP of the forward kernel of the same stage in
+// pict_stages.hip, with the reference's output contract: which gradient
+// tensors are written, and whether a kernel assigns (=) or accumulates (+=).
+// Scatters to cells other than the thread's own use device-scope float
+// atomics (the gradient tensors are zero-initialised by the caller, as in
+// the reference's Create*Grad functions).  Orthogonal terms only, like the
+// forward path.
+#include <hip/hip_runtime.h>
+
+#include "pict_host.hpp"
+
+namespace pict {
+
+template <typename R>
+__device__ __forceinline__ void add_to(R *p, R v) {
+    unsafeAtomicAdd(p, v);
+}
+
+// Position of each row slot (diag, -x, +x, ...) in the cell's sorted CSR row
+// (LoadCSRrowNeighborSorted :3503-3563): rank of the slot's column among the
+// row's columns.  -1 for prescribed faces.
+template <typename R, int D>
+__device__ __forceinline__ void slot_positions(const DomainK<R> &dom, const pict_block_desc *b, int bi, const I4 &pos,
+                                               int g, int *at) {
+    int idx[7];
+    row_slots<R, D>(dom, b, bi, pos, idx);
+    const int start = dom.csr_row[g];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        if (idx[k] < 0) {
+            at[k] = -1;
+            continue;
+        }
+        int r = 0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) r += (idx[j] >= 0 && idx[j] < idx[k]) ? 1 : 0;
+        at[k] = start + r;
+    }
+}
+
+// scatterViscosityBlock_GRAD :1852-1873 (+ scatterViscosity_GRAD :1834-1849)
+template <typename R>
+__device__ __forceinline__ void scatter_viscosity(const DomainK<R> &dom, const pict_block_desc *b, I4 p, R g) {
+    R *dst;
+    if (!b->viscosity) {
+        dst = dom.visc_grad;
+    } else if (b->viscosity_static) {
+        dst = reinterpret_cast<R *>(b->viscosity_grad);
+    } else {
+        p.a[3] = 0;
+        dst = b->viscosity_grad ? reinterpret_cast<R *>(b->viscosity_grad) + flat4(p, b->stride) : nullptr;
+    }
+    if (dst) add_to(dst, g);
+}
+
+// scatterBlockVelocitySource_GRAD :2255-2264 (p.w = component)
+template <typename R>
+__device__ __forceinline__ void scatter_velocity_source(const pict_block_desc *b, const I4 &p, R g) {
+    if (!b->velocity_source_grad) return;
+    R *dst = reinterpret_cast<R *>(b->velocity_source_grad);
+    add_to(b->velocity_source_static ? dst + p.a[3] : dst + flat4(p, b->stride), g);
+}
+
+// scatterFixedBoundaryData (VELOCITY_GRAD) :2451-2476, p on the boundary grid, w = component
+template <typename R>
+__device__ __forceinline__ void scatter_fixed_velocity(const pict_boundary_desc *fb, const I4 &p, R g) {
+    if (!fb->velocity_grad) return;
+    R *dst = reinterpret_cast<R *>(fb->velocity_grad);
+    add_to(fb->velocity_static ? dst + p.a[3] : dst + flat4(p, fb->stride), g);
+}
+
+// scatterContravariantComponentGradBoundaryFixed :700-716 (+ :545-560).  Like the
+// reference, the transformed case scatters to all components whatever the
+// boundary condition type.
+template <typename R, int D>
+__device__ __forceinline__ void scatter_fixed_contravariant(const pict_boundary_desc *fb, I4 p, R g) {
+    if (!fb->velocity_grad) return;
+    const int comp = p.a[3];
+    if (fb->has_transform) {
+        const R *T = bound_T<R, D>(fb, p);
+        const R s = Tdet<R, D>(T) * g;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            p.a[3] = j;
+            scatter_fixed_velocity<R>(fb, p, Tminv<R, D>(T, comp, j) * s);
+        }
+    } else {
+        scatter_fixed_velocity<R>(fb, p, g);
+    }
+}
+
+// scatterContravariantComponentGrad :684-698 (+ VelocityGradFromContravariantComponentGrad :527-543):
+// into the block velocity gradient, or into a global component-major vector `vgg`.
+template <typename R, int D>
+__device__ __forceinline__ void scatter_contravariant(const DomainK<R> &dom, const pict_block_desc *b, I4 p, R g,
+                                                      R *vgg) {
+    if (!vgg && !b->velocity_grad) return;
+    const int comp = p.a[3];
+    if (b->has_transform) {
+        const R *T = cell_T<R, D>(b, p);
+        const R s = Tdet<R, D>(T) * g;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            p.a[3] = j;
+            const R v = Tminv<R, D>(T, comp, j) * s;
+            if (vgg) add_to(vgg + (size_t)j * dom.N + global_index(p, b), v);
+            else add_to(reinterpret_cast<R *>(b->velocity_grad) + flat4(p, b->stride), v);
+        }
+    } else {
+        if (vgg) add_to(vgg + (size_t)comp * dom.N + global_index(p, b), g);
+        else add_to(reinterpret_cast<R *>(b->velocity_grad) + flat4(p, b->stride), g);
+    }
+}
+
+// ScatterFluxesGradNDLoop :1635-1699: transpose of fluxes_nd.
+template <typename R, int D>
+__device__ __forceinline__ void scatter_fluxes(const DomainK<R> &dom, const pict_block_desc *b, const I4 &pos,
+                                               const R *fg, R *vgg) {
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        I4 t = pos;
+        t.a[3] = dim;
+        R own = 0;
+#pragma unroll
+        for (int up = 0; up < 2; ++up) {
+            const int face = 2 * dim + up;
+            const R g = fg[face];
+            if (g == R(0)) continue;
+            t.a[dim] = pos.a[dim];
+            if (at_bound(pos, face, b)) {
+                const pict_boundary_desc *bd = &b->bounds[face];
+                if (bd->type == PICT_BOUNDARY_FIXED) {
+                    t.a[dim] = 0;
+                    scatter_fixed_contravariant<R, D>(bd, t, g);
+                } else if (bd->type == PICT_BOUNDARY_CONNECTED) {
+                    const R of = ((bd->axes[0] & 1) == up) ? R(-0.5) : R(0.5);
+                    const pict_block_desc *ob = dom.blocks + bd->connected_block;
+                    I4 o = connected_pos<D>(dom.blocks, t, dim, bd, 0);
+                    o.a[3] = bd->axes[0] >> 1;
+                    scatter_contravariant<R, D>(dom, ob, o, of * g, vgg);
+                    own += R(0.5) * g;
+                } else if (bd->type == PICT_BOUNDARY_PERIODIC) {
+                    t.a[dim] = up ? 0 : b->size[dim] - 1;
+                    scatter_contravariant<R, D>(dom, b, t, R(0.5) * g, vgg);
+                    own += R(0.5) * g;
+                }
+            } else {
+                t.a[dim] = pos.a[dim] + (up ? 1 : -1);
+                scatter_contravariant<R, D>(dom, b, t, R(0.5) * g, vgg);
+                own += R(0.5) * g;
+            }
+        }
+        t.a[dim] = pos.a[dim];
+        if (own != R(0)) scatter_contravariant<R, D>(dom, b, t, own, vgg);
+    }
+}
+
+// VJP of fixed_boundary_source (boundary terms of the advection / pressure RHS,
+// kPISO_build_advection_RHS_GRAD :4415-4465, PISO_build_pressure_rhs_GRAD :5302-5355);
+// G = gradient of the source term of component d (already divided by det).
+template <typename R, int D>
+__device__ __forceinline__ void fixed_boundary_source_grad(const DomainK<R> &dom, const pict_block_desc *b,
+                                                           const I4 &pos, int d, R G) {
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        if (!at_bound(pos, face, b)) continue;
+        const pict_boundary_desc *fb = &b->bounds[face];
+        if (fb->type != PICT_BOUNDARY_FIXED) continue;
+        const int ax = face >> 1;
+        const R normal = (face & 1) ? R(1) : R(-1);
+        I4 t = pos;
+        t.a[3] = d;
+        t.a[ax] = 0;
+        const R vel = fixed_vel<R>(fb, t, d);
+        t.a[3] = ax;
+        const R alpha = laplace_orth_fixed<R, D>(fb, t);
+        const R flux = fixed_contravariant<R, D>(fb, t) * normal;
+        const R slip = fb->velocity_bc == PICT_BC_DIRICHLET ? R(0) : R(1);
+        const R visc = viscosity(dom, b, pos);
+        const R flux_grad = -vel * G;
+        const R vel_grad = -flux * G + (R(1) - slip) * visc * R(2) * alpha * G;
+        const R visc_grad = vel * (R(1) - slip) * R(2) * alpha * G;
+        scatter_fixed_contravariant<R, D>(fb, t, flux_grad * normal);
+        t.a[3] = d;
+        scatter_fixed_velocity<R>(fb, t, vel_grad);
+        scatter_viscosity<R>(dom, b, pos, visc_grad);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PISO_build_matrix_GRAD :3869-4075 (orthogonal): C_grad, A_grad ->
+// block velocity grad, viscosity grads.
+// ---------------------------------------------------------------------------
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_grad(DomainK<R> dom, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    (void)dt;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R rdet = R(1) / cell_det<R, D>(b, pos);
+    int at[7];
+    slot_positions<R, D>(dom, b, bi, pos, g, at);
+    R cg[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) cg[k] = (at[k] >= 0 && dom.C_grad) ? dom.C_grad[at[k]] * rdet : R(0);
+    const R diag_grad = (dom.A_grad ? dom.A_grad[g] * rdet : R(0)) + cg[0];
+    R alpha_p[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        alpha_p[d] = laplace_orth<R, D>(b, t);
+    }
+    R fg[2 * D];
+    R visc_grad = 0;
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        const int dim = face >> 1;
+        const R fs = (face & 1) ? R(1) : R(-1);
+        if (!(at_bound(pos, face, b) && prescribed(face, b))) {
+            int nb;
+            I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
+            const pict_block_desc *nbp = dom.blocks + nb;
+            const R alpha = laplace_orth<R, D>(nbp, t);
+            fg[face] = fs * R(0.5) * (diag_grad + cg[face + 1]);
+            const R vc_grad = diag_grad - cg[face + 1];
+            visc_grad += R(0.5) * alpha_p[dim] * vc_grad;
+            t.a[3] = 0;
+            scatter_viscosity<R>(dom, nbp, t, R(0.5) * alpha * vc_grad);
+        } else {
+            fg[face] = 0;
+            const R slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? R(0) : R(1);
+            visc_grad += (R(1) - slip) * R(2) * alpha_p[dim] * diag_grad;
+        }
+    }
+    if (b->velocity_grad) scatter_fluxes<R, D>(dom, b, pos, fg, nullptr);
+    scatter_viscosity<R>(dom, b, pos, visc_grad);
+}
+
+// kPISO_build_advection_RHS_GRAD :4388-4476: velocity_rhs_grad -> velocity grad (=),
+// velocity source grad, fixed boundary velocity and viscosity grads.
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs_grad(DomainK<R> dom, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R det = cell_det<R, D>(b, pos);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        R G = dom.vrhs_grad[(size_t)d * dom.N + g];
+        scatter_velocity_source<R>(b, t, G);
+        G /= det;
+        fixed_boundary_source_grad<R, D>(dom, b, pos, d, G);
+        G *= det;
+        if (b->velocity_grad) reinterpret_cast<R *>(b->velocity_grad)[flat4(t, b->stride)] = G / dt;
+    }
+}
+
+// PISO_build_pressure_matrix_GRAD :4967-5115 (orthogonal): P_grad -> A_grad (+=).
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_grad(DomainK<R> dom) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    int at[7];
+    slot_positions<R, D>(dom, b, bi, pos, g, at);
+    const R pg0 = dom.P_grad[at[0]];
+    R raP_grad = 0;
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+        const int dim = face >> 1;
+        int nb;
+        I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
+        const pict_block_desc *nbp = dom.blocks + nb;
+        const R alphaN = laplace_orth<R, D>(nbp, t);
+        I4 tp = pos;
+        tp.a[3] = dim;
+        const R alphaP = laplace_orth<R, D>(b, tp);
+        t.a[3] = 0;
+        const int gn = global_index(t, nbp);
+        const R raN = R(1) / dom.A[gn];
+        const R coef_grad = dom.P_grad[at[face + 1]] - pg0;
+        raP_grad += R(0.5) * alphaP * coef_grad;
+        add_to(dom.A_grad + gn, -(R(0.5) * alphaN * coef_grad) * raN * raN);
+    }
+    const R raP = R(1) / dom.A[g];
+    add_to(dom.A_grad + g, -raP_grad * raP * raP);
+}
+
+// k_computePressureRHSdivergenceFromFlux_GRAD :5423-5452: pressure_rhs_div_grad ->
+// pressure_rhs_grad (+=), fixed boundary velocity grads.
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_div_grad(DomainK<R> dom, R dt, int tsn) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    R dg = dom.prhs_div_grad[b->global_offset + f];
+    if (tsn) dg /= dt;
+    R fg[2 * D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        fg[2 * d + 1] = dg;
+        fg[2 * d] = -dg;
+    }
+    scatter_fluxes<R, D>(dom, b, pos, fg, dom.prhs_grad);
+}
+
+// PISO_build_pressure_rhs_GRAD :5243-5369: pressure_rhs_grad -> velocity grad (=),
+// velocity_result grad (+=), C_grad (= over the row), A_grad (+=), source and
+// boundary grads.
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_rhs_grad(DomainK<R> dom, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R rDiag = R(1) / dom.A[g];
+    const R det = cell_det<R, D>(b, pos);
+    const int cs = dom.csr_row[g], ce = dom.csr_row[g + 1];
+    const int rs = ce - cs < 7 ? ce - cs : 7;
+    int col[7];
+    R val[7], cvg[7];
+    for (int k = 0; k < rs; ++k) {
+        col[k] = dom.csr_index[cs + k];
+        val[k] = dom.C[cs + k];
+        cvg[k] = 0;
+    }
+    R rDiag_grad = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        const size_t o = (size_t)dom.N * d;
+        const R pGrad = dom.prhs_grad[g + o];
+        rDiag_grad += dom.prhs[g + o] / rDiag * pGrad;
+        if (b->velocity_grad) reinterpret_cast<R *>(b->velocity_grad)[flat4(t, b->stride)] = rDiag / dt * pGrad;
+        const R w = rDiag * pGrad;
+        for (int k = 0; k < rs; ++k) {
+            if (col[k] == g) continue;
+            if (dom.vres_grad) add_to(dom.vres_grad + col[k] + o, -val[k] * w);
+            cvg[k] -= dom.vres[col[k] + o] * w;
+        }
+        R S_grad = w;
+        scatter_velocity_source<R>(b, t, S_grad);
+        S_grad /= det;
+        fixed_boundary_source_grad<R, D>(dom, b, pos, d, S_grad);
+    }
+    if (dom.C_grad)
+        for (int k = 0; k < rs; ++k) dom.C_grad[cs + k] = cvg[k];
+    if (dom.A_grad) dom.A_grad[g] += rDiag_grad * (-rDiag * rDiag);
+}
+
+// scatterPressureGradToWithBounds :3411-3477 (transpose of pressure_at)
+template <typename R, int D>
+__device__ __forceinline__ void scatter_pressure_to(const DomainK<R> &dom, const pict_block_desc *b, const I4 &pos,
+                                                    R g) {
+    I4 t = pos;
+    t.a[3] = 0;
+    R *own = reinterpret_cast<R *>(b->pressure_grad);
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        if (pos.a[dim] < 0 || pos.a[dim] >= b->size[dim]) {
+            const bool lower = pos.a[dim] < 0;
+            const pict_boundary_desc *bd = &b->bounds[2 * dim + (lower ? 0 : 1)];
+            if (bd->type == PICT_BOUNDARY_FIXED) {
+                t.a[dim] = lower ? 0 : b->size[dim] - 1;
+                if (own) add_to(own + flat4(t, b->stride), g);
+            } else if (bd->type == PICT_BOUNDARY_CONNECTED) {
+                const pict_block_desc *ob = dom.blocks + bd->connected_block;
+                const int border = lower ? -(pos.a[dim] + 1) : pos.a[dim] - b->size[dim];
+                I4 o = connected_pos<D>(dom.blocks, t, dim, bd, border);
+                if (ob->pressure_grad) add_to(reinterpret_cast<R *>(ob->pressure_grad) + flat4(o, ob->stride), g);
+            } else if (bd->type == PICT_BOUNDARY_PERIODIC) {
+                t.a[dim] += lower ? b->size[dim] : -b->size[dim];
+                if (own) add_to(own + flat4(t, b->stride), g);
+            }
+            return;
+        }
+    }
+    if (own) add_to(own + flat4(t, b->stride), g);
+}
+
+// PISO_update_velocity_GRAD :6250-6293 (+ scatterPressureGradientGrad :838-870):
+// velocity_result_grad -> pressure_rhs_grad (=), block pressure grads (+=), A_grad (=).
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_correct_velocity_grad(DomainK<R> dom, R dt, int tsn) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R rDiag = R(1) / dom.A[g];
+    R pg[D], pgg[D];
+    pressure_gradient<R, D>(dom, b, pos, pg);
+    R rDiag_grad = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const size_t o = (size_t)dom.N * d;
+        R vug = dom.vres_grad[g + o];
+        dom.prhs_grad[g + o] = vug;
+        if (tsn) vug *= dt;
+        pgg[d] = -rDiag * vug;
+        rDiag_grad -= pg[d] * vug;
+    }
+    {
+        R v[D];
+        if (b->has_transform) {
+            const R *T = cell_T<R, D>(b, pos);
+#pragma unroll
+            for (int i = 0; i < D; ++i) {
+                R s = 0;
+#pragma unroll
+                for (int j = 0; j < D; ++j) s += Tminv<R, D>(T, i, j) * pgg[j];
+                v[i] = s;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < D; ++i) v[i] = pgg[i];
+        }
+#pragma unroll
+        for (int dim = 0; dim < D; ++dim) {
+            I4 pP = pos, pN = pos;
+            pP.a[3] = pN.a[3] = 0;
+            R fac = R(0.5);
+            if (!(pos.a[dim] == 0 && prescribed(2 * dim, b))) pN.a[dim] = pos.a[dim] - 1;
+            else fac = R(1);
+            if (!(pos.a[dim] == b->size[dim] - 1 && prescribed(2 * dim + 1, b))) pP.a[dim] = pos.a[dim] + 1;
+            else fac = R(1);
+            scatter_pressure_to<R, D>(dom, b, pN, v[dim] * -fac);
+            scatter_pressure_to<R, D>(dom, b, pP, v[dim] * fac);
+        }
+    }
+    dom.A_grad[g] = -rDiag_grad * rDiag * rDiag;
+}
+
+}  // namespace pict
+
+using namespace pict;
+
+extern "C" {
+
+pict_status pict_setup_advection_matrix_grad(const pict_atlas *a, double dt, int32_t flags, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.C_grad && a->desc.A_grad, "C_grad/A_grad missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_grad<R, D>), a->view<R>(), (R)dt));
+    return check_launch();
+}
+
+pict_status pict_setup_advection_velocity_grad(const pict_atlas *a, double dt, int32_t flags, int32_t apply_pg,
+                                               void *stream) {
+    (void)apply_pg;  // the reference's VJP omits the pressure-gradient term (:4388-4476)
+    PICT_CHECK_ARG(a && a->desc.velocity_rhs_grad, "velocity_rhs_grad missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_rhs_grad<R, D>), a->view<R>(), (R)dt));
+    return check_launch();
+}
+
+pict_status pict_setup_pressure_matrix_grad(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
+                                            void *stream) {
+    (void)dt;
+    PICT_CHECK_ARG(a && a->desc.P_grad && a->desc.A_grad && a->desc.A, "P_grad/A_grad missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    if (use_face_transform) {
+        set_error("useFaceTransform gradients are not implemented (nor in the reference)");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_grad<R, D>), a->view<R>()));
+    return check_launch();
+}
+
+pict_status pict_setup_pressure_rhs_div_grad(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
+                                             int32_t tsn, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.pressure_rhs_div_grad && a->desc.pressure_rhs_grad, "pressure rhs grads missing");
+    pict_status st = check_non_ortho(flags);
+    if (st) return st;
+    if (use_face_transform) {
+        set_error("useFaceTransform gradients are not implemented (nor in the reference)");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_div_grad<R, D>), a->view<R>(), (R)dt, (int)tsn));
+    return check_launch();
+}
+
+pict_status pict_setup_pressure_rhs_grad(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
+                                         int32_t tsn, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.pressure_rhs_grad && a->desc.pressure_rhs && a->desc.C_value &&
+                       a->desc.velocity_result && a->desc.A,
+                   "pressure rhs grad inputs missing");
+    pict_status st = pict_setup_pressure_rhs_div_grad(a, dt, flags, use_face_transform, tsn, stream);
+    if (st) return st;
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs_grad<R, D>), a->view<R>(), (R)dt));
+    return check_launch();
+}
+
+pict_status pict_setup_pressure_correction_grad(const pict_atlas *a, double dt, int32_t flags,
+                                                int32_t use_face_transform, int32_t tsn, void *stream) {
+    pict_status st = pict_setup_pressure_rhs_grad(a, dt, flags, use_face_transform, tsn, stream);
+    if (st) return st;
+    return pict_setup_pressure_matrix_grad(a, dt, flags, use_face_transform, stream);
+}
+
+pict_status pict_correct_velocity_grad(const pict_atlas *a, double dt, int32_t tsn, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.velocity_result_grad && a->desc.pressure_rhs_grad && a->desc.A_grad && a->desc.A,
+                   "correct velocity grads missing");
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_correct_velocity_grad<R, D>), a->view<R>(), (R)dt, (int)tsn));
+    return check_launch();
+}
+
+}  // extern "C"

@@ -33,7 +33,48 @@ void set_error(const char *fmt, ...);
         }                                    \
     } while (0)
 
+inline pict_status check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("kernel launch failed: %s", hipGetErrorString(e));
+        return PICT_ERR_HIP;
+    }
+    return PICT_OK;
+}
+
+// Non-orthogonal correction terms (PISO_multiblock_cuda.h flags) are not implemented.
+inline pict_status check_non_ortho(int flags) {
+    if (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX | PICT_NON_ORTHO_DIRECT_RHS |
+                 PICT_NON_ORTHO_DIAGONAL_RHS | PICT_NON_ORTHO_DIAGONAL_MATRIX)) {
+        set_error("non-orthogonal correction terms (flags %d) are not implemented by libpict_hip", flags);
+        return PICT_ERR_UNSUPPORTED;
+    }
+    return PICT_OK;
+}
+
 }  // namespace pict
+
+// dtype x spatial-dimension dispatch of a kernel template <R, D>
+#define PICT_DISPATCH(atlas, BODY)                                                   \
+    do {                                                                             \
+        const int _D = (atlas)->desc.num_dims;                                       \
+        if ((atlas)->desc.dtype == PICT_DTYPE_F32) {                                 \
+            using R = float;                                                         \
+            if (_D == 1) { constexpr int D = 1; BODY; }                              \
+            else if (_D == 2) { constexpr int D = 2; BODY; }                         \
+            else { constexpr int D = 3; BODY; }                                      \
+        } else {                                                                     \
+            using R = double;                                                        \
+            if (_D == 1) { constexpr int D = 1; BODY; }                              \
+            else if (_D == 2) { constexpr int D = 2; BODY; }                         \
+            else { constexpr int D = 3; BODY; }                                      \
+        }                                                                            \
+    } while (0)
+
+// one workgroup per 256-cell tile of a block
+#define PICT_LAUNCH_CELLS(atlas, stream, KERNEL, ...)                                           \
+    hipLaunchKernelGGL(KERNEL, dim3((atlas)->num_tiles), dim3(kCellsPerTile), 0,                \
+                       (hipStream_t)(stream), __VA_ARGS__)
 
 // Device-resident domain: descriptor copy + launch tables (one allocation).
 struct pict_atlas {

@@ -53,21 +53,6 @@ __device__ int row_end_closed_form(const pict_block_desc *b, int flatPos, int &r
 }
 
 template <typename R, int D>
-__device__ __forceinline__ void row_slots(const DomainK<R> &dom, const pict_block_desc *b, int bi, const I4 &pos,
-                                          int *idx) {
-#pragma unroll
-    for (int k = 0; k < 7; ++k) idx[k] = -1;
-    idx[0] = global_index(pos, b);
-#pragma unroll
-    for (int face = 0; face < 2 * D; ++face) {
-        if (at_bound(pos, face, b) && prescribed(face, b)) continue;
-        int nb;
-        const I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
-        idx[face + 1] = global_index(t, dom.blocks + nb);
-    }
-}
-
-template <typename R, int D>
 __global__ __launch_bounds__(kCellsPerTile) void k_csr_pattern(DomainK<R> dom) {
     const pict_block_desc *b;
     int bi, f;
@@ -290,61 +275,28 @@ __global__ __launch_bounds__(kCellsPerTile) void k_correct_velocity(DomainK<R> d
     }
 }
 
-// Copy*Result*Blocks :6541-6733 -- one launch instead of a memcpy per block and component.
+// Copy*Result*Blocks :6541-6733 and Copy*ResultGrad*Blocks :6819-6915 -- one
+// launch instead of a memcpy per block and component.  `field` selects the
+// block tensor: 0 velocity, 1 pressure, 2 velocity_grad, 3 pressure_grad;
+// `gv` is the matching global vector (component-major for velocities).
 template <typename R>
-__global__ __launch_bounds__(kCellsPerTile) void k_copy_result(DomainK<R> dom, int velocity, int to_blocks) {
+__global__ __launch_bounds__(kCellsPerTile) void k_copy_result(DomainK<R> dom, R *gv, int field, int to_blocks) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, b, bi, f)) return;
     const int g = b->global_offset + f;
-    if (velocity) {
-        R *bv = reinterpret_cast<R *>(b->velocity);
-        for (int d = 0; d < dom.D; ++d) {
-            if (to_blocks) bv[(size_t)d * b->stride[3] + f] = dom.vres[(size_t)d * dom.N + g];
-            else dom.vres[(size_t)d * dom.N + g] = bv[(size_t)d * b->stride[3] + f];
-        }
-    } else {
-        R *bp = reinterpret_cast<R *>(b->pressure);
-        if (to_blocks) bp[f] = dom.pres[g];
-        else dom.pres[g] = bp[f];
+    void *bp = field == 0 ? b->velocity : field == 1 ? b->pressure : field == 2 ? b->velocity_grad : b->pressure_grad;
+    R *bv = reinterpret_cast<R *>(bp);
+    const int comps = (field & 1) ? 1 : dom.D;
+    for (int d = 0; d < comps; ++d) {
+        if (to_blocks) bv[(size_t)d * b->stride[3] + f] = gv[(size_t)d * dom.N + g];
+        else gv[(size_t)d * dom.N + g] = bv[(size_t)d * b->stride[3] + f];
     }
 }
 
 }  // namespace pict
 
 using namespace pict;
-
-// ---------------------------------------------------------------------------
-// dispatch helpers
-// ---------------------------------------------------------------------------
-#define PICT_DISPATCH(atlas, BODY)                                                   \
-    do {                                                                             \
-        const int _D = (atlas)->desc.num_dims;                                       \
-        if ((atlas)->desc.dtype == PICT_DTYPE_F32) {                                 \
-            using R = float;                                                         \
-            if (_D == 1) { constexpr int D = 1; BODY; }                              \
-            else if (_D == 2) { constexpr int D = 2; BODY; }                         \
-            else { constexpr int D = 3; BODY; }                                      \
-        } else {                                                                     \
-            using R = double;                                                        \
-            if (_D == 1) { constexpr int D = 1; BODY; }                              \
-            else if (_D == 2) { constexpr int D = 2; BODY; }                         \
-            else { constexpr int D = 3; BODY; }                                      \
-        }                                                                            \
-    } while (0)
-
-#define PICT_LAUNCH_CELLS(atlas, stream, KERNEL, ...)                                           \
-    hipLaunchKernelGGL(KERNEL, dim3((atlas)->num_tiles), dim3(kCellsPerTile), 0,                \
-                       (hipStream_t)(stream), __VA_ARGS__)
-
-static pict_status check_launch() {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        set_error("kernel launch failed: %s", hipGetErrorString(e));
-        return PICT_ERR_HIP;
-    }
-    return PICT_OK;
-}
 
 static pict_status validate_desc(const pict_domain_desc *d) {
     PICT_CHECK_ARG(d != nullptr, "descriptor is NULL");
@@ -490,15 +442,6 @@ pict_status pict_build_csr_pattern(const pict_atlas *a, void *stream) {
     return check_launch();
 }
 
-static pict_status check_non_ortho(int flags) {
-    if (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX | PICT_NON_ORTHO_DIRECT_RHS |
-                 PICT_NON_ORTHO_DIAGONAL_RHS | PICT_NON_ORTHO_DIAGONAL_MATRIX)) {
-        set_error("non-orthogonal correction terms (flags %d) are not implemented by libpict_hip", flags);
-        return PICT_ERR_UNSUPPORTED;
-    }
-    return PICT_OK;
-}
-
 pict_status pict_setup_advection_matrix(const pict_atlas *a, double dt, int32_t flags, void *stream) {
     PICT_CHECK_ARG(a && a->desc.C_value && a->desc.A, "C/A missing");
     pict_status st = check_non_ortho(flags);
@@ -585,17 +528,28 @@ pict_status pict_compute_velocity_divergence(const pict_atlas *a, void *out, voi
     return check_launch();
 }
 
-static pict_status copy_result(const pict_atlas *a, int velocity, int to_blocks, void *stream) {
-    PICT_CHECK_ARG(a && (velocity ? a->desc.velocity_result : a->desc.pressure_result), "result missing");
+static pict_status copy_result(const pict_atlas *a, int field, int to_blocks, void *stream) {
+    PICT_CHECK_ARG(a != nullptr, "atlas is NULL");
+    void *gv = field == 0 ? a->desc.velocity_result : field == 1 ? a->desc.pressure_result
+             : field == 2 ? a->desc.velocity_result_grad : a->desc.pressure_result_grad;
+    PICT_CHECK_ARG(gv != nullptr, "global vector missing");
+    for (int i = 0; i < a->desc.num_blocks; ++i) {
+        const pict_block_desc &b = a->host_blocks[i];
+        const void *bp = field == 0 ? b.velocity : field == 1 ? b.pressure : field == 2 ? b.velocity_grad : b.pressure_grad;
+        PICT_CHECK_ARG(bp != nullptr, "block %d: tensor missing", i);
+    }
     if (a->desc.dtype == PICT_DTYPE_F32)
-        PICT_LAUNCH_CELLS(a, stream, (k_copy_result<float>), a->view<float>(), velocity, to_blocks);
+        PICT_LAUNCH_CELLS(a, stream, (k_copy_result<float>), a->view<float>(), (float *)gv, field, to_blocks);
     else
-        PICT_LAUNCH_CELLS(a, stream, (k_copy_result<double>), a->view<double>(), velocity, to_blocks);
+        PICT_LAUNCH_CELLS(a, stream, (k_copy_result<double>), a->view<double>(), (double *)gv, field, to_blocks);
     return check_launch();
 }
-pict_status pict_copy_velocity_result_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 1, 1, s); }
-pict_status pict_copy_velocity_result_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 1, 0, s); }
-pict_status pict_copy_pressure_result_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 0, 1, s); }
-pict_status pict_copy_pressure_result_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 0, 0, s); }
+pict_status pict_copy_velocity_result_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 0, 1, s); }
+pict_status pict_copy_velocity_result_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 0, 0, s); }
+pict_status pict_copy_pressure_result_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 1, 1, s); }
+pict_status pict_copy_pressure_result_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 1, 0, s); }
+pict_status pict_copy_velocity_result_grad_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 2, 1, s); }
+pict_status pict_copy_velocity_result_grad_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 2, 0, s); }
+pict_status pict_copy_pressure_result_grad_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 3, 0, s); }
 
 }  // extern "C"

@@ -136,6 +136,23 @@ def three_d_connected(dtype=torch.float64, device="cuda", seed=4):
     return d
 
 
+def cavity_static_fields(dtype=torch.float64, device="cuda", n=6, seed=5):
+    """Cavity with a static block viscosity, a static velocity source and a varying lid velocity."""
+    d = _domain(2, 0.05, dtype, device)
+    b = _coords_block(d, uniform_coords(n, n + 1, dtype=dtype, device=device))
+    g = torch.Generator().manual_seed(seed)
+    b.CloseAllBoundaries()
+    lid = _rand([1, 2, 1, n], g, dtype, device, 0.5)
+    lid[:, 1] = 0
+    b.CloseBoundary("+y", velocity=lid)
+    b.setVelocity(_rand(b.velocity.shape, g, dtype, device, 0.3))
+    b.setPressure(_rand(b.pressure.shape, g, dtype, device, 0.2))
+    b.setViscosity(torch.tensor([[0.07]], dtype=dtype, device=device))
+    b.setVelocitySource(_rand([1, 2], g, dtype, device, 0.2))
+    d.PrepareSolve()
+    return d
+
+
 ALL = {
     "cavity": cavity,
     "cavity_nocoords": lambda **kw: cavity(coords=False, **kw),
@@ -144,4 +161,5 @@ ALL = {
     "rotated_connection": rotated_connection,
     "box3d": box3d,
     "three_d_connected": three_d_connected,
+    "cavity_static_fields": cavity_static_fields,
 }
