@@ -15,9 +15,9 @@ pytestmark = pytest.mark.gpu
 
 DT = 0.05
 GPU = {
-    "SetupAdvectionMatrix": (P.SetupAdvectionMatrixGrad, O.SetupAdvectionMatrixGrad,
+    "SetupAdvectionMatrix": (lambda d, dt: P.SetupAdvectionMatrixGrad(d, dt, 0), O.SetupAdvectionMatrixGrad,
                              ["VELOCITY", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK"], ["A", "C"]),
-    "SetupAdvectionVelocity": (P.SetupAdvectionVelocityGrad, O.SetupAdvectionVelocityGrad,
+    "SetupAdvectionVelocity": (lambda d, dt: P.SetupAdvectionVelocityGrad(d, dt, 0), O.SetupAdvectionVelocityGrad,
                                ["VELOCITY", "VELOCITY_SOURCE", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK"],
                                ["VELOCITY_RHS"]),
     "SetupPressureMatrix": (P.SetupPressureMatrixGrad, O.SetupPressureMatrixGrad, ["A"], ["P"]),
