@@ -1,17 +1,18 @@
 """Benchmark: cells*PISO-steps/s of the PISO time step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--config c3|c1] [--no-cpu-baseline]
+    python bench.py [--mode fwdbwd|fwd] [--steps K] [--warmup W] [--config c3|c1] [--no-cpu-baseline]
 
-Default workload (c3): the 2-block 256x256 periodic box (2 blocks of 256x128,
-[0,2pi]^2, nu=1e-3, spectral k^-5/3 initial velocity, RMS 1, dt = 0.25 h/max|u0|),
-FP32, BiCGStab advection / CG pressure at tol 1e-6, 2 correctors.  A step is one
-PISO step over the whole domain; inputs are resident in HBM.
+Default workload (differentiation configuration): the 2-block 256x256 periodic box (2 blocks of 256x128,
+[0,2pi]^2, nu=1e-3, spectral k^-5/3 initial velocity with RMS 1, dt = 0.25 h/max|u0|), FP32, BiCGStab advection
+and CG pressure at tol 1e-6, 2 correctors; a learnable per-cell forcing (block velocity sources, zero-initialised);
+a K-step rollout (default 16) with per-step checkpointing, MSE loss to a seeded target state, and backward.
+A bench step is one PISO step forward + its backward (recomputation included); the timed region is one whole
+rollout plus backward, inputs resident in HBM.  `--mode fwd` times K forward steps only.
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the one with the
-largest total time among the solver kernels, timed with HIP events on its launch
-stream): algorithmic bytes per launch / average launch duration vs the 8 TB/s HBM
-peak.  `cpu_baseline` is the FP32 C restatement of the same step (oracle/pict_oracle.c,
-OpenMP) on this host's cores, over a bounded sample of steps of the same workload.
+Prints one JSON line.  `roofline` is for the dominant kernel (the solver kernel with the largest total time in the
+timed region, timed with HIP events on its launch stream): algorithmic bytes per launch / average launch duration
+vs the 8 TB/s HBM peak.  `cpu_baseline` is the FP32 C restatement (oracle/pict_oracle.c, OpenMP) of the same
+step -- forward, or forward plus the reverse pass -- on this host's cores, over a bounded sample of steps.
 """
 import argparse
 import json
@@ -30,20 +31,74 @@ def solver_bytes(kind, n, nnz, nrhs, s, iters):
     SpMV  = nnz*(s+4) + (n+1)*4 + 2*n*s
     CG iteration       = SpMV + 9*n*s            (x, r read/write; p, Ap read; r, p read/write)
     BiCGStab iteration = 2*SpMV + 17*n*s         (per right-hand side)
-    plus the initial residual r0 = b - A x0 (SpMV + n*s for b) once per right-hand side."""
+    plus the initial residual r0 = b - A x0 (SpMV + n*s for b) once per right-hand side.  Adjoint (transposed)
+    solves are counted like forward ones."""
     spmv = nnz * (s + 4) + (n + 1) * 4 + 2 * n * s
     per_it = spmv + 9 * n * s if kind == "CG" else 2 * spmv + 17 * n * s
     return sum(max(it, 0) * per_it + spmv + n * s for it in iters)
 
 
+def make_workload(args, torch):
+    from pict_amd import PISOtorch_simulation as S
+    from pict_amd import configs
+    dtype = torch.float32
+    if args.config == "c3":
+        dom, dt = configs.periodic_box_c3(256, dtype=dtype, device="cuda")
+        workload = ("periodic box 256x256 (2 blocks of 256x128), FP32, spectral k^-5/3 init, learnable per-cell "
+                    "forcing, %s" % ("%d-step rollout with per-step checkpointing, MSE loss, forward+backward"
+                                     % args.steps if args.mode == "fwdbwd" else "forward PISO steps"))
+    else:
+        dom = configs.lid_cavity(32, viscosity=0.01, dtype=dtype, device="cuda")
+        dt = 0.01
+        workload = "lid-driven cavity 32x32, Re=100, FP32, " + ("forward+backward" if args.mode == "fwdbwd"
+                                                                else "forward PISO steps")
+    for b in dom.blocks:
+        b.setVelocitySource(torch.zeros_like(b.velocity))
+    dom.UpdateDomainData()
+    sim = S.Simulation(dom, time_step=dt, corrector_steps=2, non_orthogonal=False, advection_tol=1e-6,
+                       pressure_tol=1e-6, advect_passive_scalar=False)
+    return dom, sim, dt, workload
+
+
+def target_state(dom, args, torch):
+    from pict_amd import configs
+    if args.config == "c3":
+        vel = configs.spectral_velocity_2d(256, seed=1)
+        half = 128
+        return [vel[None, :, i * half:(i + 1) * half, :].to(dtype=torch.float32, device="cuda").contiguous()
+                for i in range(2)]
+    g = torch.Generator().manual_seed(1)
+    return [(torch.rand(b.velocity.shape, generator=g) * 0.2).cuda() for b in dom.blocks]
+
+
+def rollout_fwdbwd(dom, sim, steps, target, torch):
+    from pict_amd import checkpoint as CK
+    u0 = [b.velocity.detach().clone() for b in dom.blocks]
+    srcs = [torch.zeros_like(b.velocity).requires_grad_(True) for b in dom.blocks]
+    for b, s in zip(dom.blocks, srcs):
+        b.setVelocitySource(s)
+    dom.UpdateDomainData()
+    out = CK.rollout(sim, steps, params=srcs, checkpoint=True)
+    loss = sum(((o - t) ** 2).mean() for o, t in zip(out, target))
+    loss.backward()
+    # back to the initial state for the next rollout
+    for b, u in zip(dom.blocks, u0):
+        b.setVelocity(u)
+        b.setVelocitySource(torch.zeros_like(u))
+    dom.UpdateDomainData()
+    return [s.grad for s in srcs]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="fwdbwd", choices=["fwdbwd", "fwd"])
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=["c3", "c1"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline sample: whole steps until this much time has passed (at most 64 steps)")
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="after the timed region, run 3 plain device copies of MIB MiB (a known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -55,49 +110,48 @@ def main():
 
     import torch
     from pict_amd import PISOtorch as P
-    from pict_amd import PISOtorch_simulation as S
-    from pict_amd import configs
 
-    dtype = torch.float32
-    if args.config == "c3":
-        dom, dt = configs.periodic_box_c3(256, dtype=dtype, device="cuda")
-        workload = "periodic box 256x256, 2 blocks of 256x128, FP32, spectral k^-5/3 init, forward PISO step"
-    else:
-        dom = configs.lid_cavity(32, viscosity=0.01, dtype=dtype, device="cuda")
-        dt = 0.01
-        workload = "lid-driven cavity 32x32, Re=100, FP32, forward PISO step"
+    dom, sim, dt, workload = make_workload(args, torch)
     N = dom.getTotalSize()
-    sim = S.Simulation(dom, time_step=dt, corrector_steps=2, non_orthogonal=False, advection_tol=1e-6,
-                       pressure_tol=1e-6, advect_passive_scalar=False)
+    target = target_state(dom, args, torch)
+
+    def run(k):
+        if args.mode == "fwdbwd":
+            return rollout_fwdbwd(dom, sim, k, target, torch)
+        assert sim._PISO_split_step(k)
+        return None
 
     for _ in range(args.warmup):
-        assert sim._PISO_split_step(1)
+        run(2)
     torch.cuda.synchronize()
     P.enable_solve_log(True)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0.record()
-    for _ in range(args.steps):
-        assert sim._PISO_split_step(1)
+    grads = run(args.steps)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_time(e1)
     log = P.solve_log
     P.enable_solve_log(False)
+    if grads is not None:
+        assert all(torch.isfinite(g).all() for g in grads)
 
     # per-kernel roofline from the solver log
     agg = {}
-    for kind, n, nnz, nrhs, s, infos, kms in log:
-        a = agg.setdefault(kind, {"ms": 0.0, "bytes": 0, "launches": 0, "iters": 0})
+    for kind, n, nnz, nrhs, s, infos, kms, transposed in log:
+        name = "k_cg1<float,%d>" % nrhs if kind == "CG" else "k_bicgstab<float,%d>" % nrhs
+        a = agg.setdefault(name, {"ms": 0.0, "bytes": 0, "launches": 0, "iters": 0, "adjoint_launches": 0})
         its = [i.usedIterations + 1 for i in infos]
         a["ms"] += kms
         a["bytes"] += solver_bytes(kind, n, nnz, nrhs, s, its)
         a["launches"] += 1
+        a["adjoint_launches"] += int(transposed)
         a["iters"] += sum(its)
-    dom_kind = max(agg, key=lambda k: agg[k]["ms"])
-    d = agg[dom_kind]
+    dom_kernel = max(agg, key=lambda k: agg[k]["ms"])
+    d = agg[dom_kernel]
     avg_ms = d["ms"] / d["launches"]
     bytes_per_launch = d["bytes"] / d["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
@@ -114,10 +168,10 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(dom, dt, args)
+        cpu = cpu_baseline(dom, dt, args, target)
 
     out = {
-        "metric": "cells*PISO-steps/s (forward)",
+        "metric": "cells*PISO-steps/s (%s)" % ("forward+backward" if args.mode == "fwdbwd" else "forward"),
         "value": value,
         "unit": "cells*steps/s",
         "n_gpus": 1,
@@ -129,33 +183,48 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (FP64 Krylov reductions)",
         "data": "synthetic (seeded)",
-        "config": {"workload": workload, "cells": N, "config": args.config, "dt": dt,
+        "config": {"workload": workload, "cells": N, "config": args.config, "mode": args.mode, "dt": dt,
+                   "solver_launches": {k: v["launches"] for k, v in agg.items()},
+                   "solver_adjoint_launches": {k: v["adjoint_launches"] for k, v in agg.items()},
                    "solver_iterations_per_step": {k: v["iters"] / args.steps for k, v in agg.items()},
                    "solver_ms_per_step": {k: v["ms"] / args.steps for k, v in agg.items()},
                    "wall_s": wall},
-        "roofline": {"bound": "hbm", "kernel": "k_cg<float>" if dom_kind == "CG" else "k_bicgstab<float>",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
-def cpu_baseline(dom, dt, args):
-    """FP32 C restatement of the same step (oracle/pict_oracle.c built with -fopenmp), on a sample of steps."""
+def cpu_baseline(dom, dt, args, target):
+    """FP32 C restatement of the same step (oracle/pict_oracle.c built with -fopenmp) on a sample of steps: the
+    forward step, and for fwdbwd also the recomputed forward and the reverse pass of a checkpointed step."""
     import torch
     from oracle import oracle as O
     cores = min(16, os.cpu_count() or 1)
     O.lib(32).oracle_set_threads(cores)
     twin = O.cpu_twin(dom, dtype=torch.float32)
+    g = torch.Generator().manual_seed(0)
     t0 = time.perf_counter()
-    for _ in range(args.cpu_steps):
+    steps = 0
+    while steps < 64 and (steps == 0 or time.perf_counter() - t0 < args.cpu_seconds):
+        steps += 1
+        u_in = [b.velocity.clone() for b in twin.blocks]
         O.piso_step(twin, dt, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6)
+        if args.mode == "fwdbwd":
+            gout = [torch.rand(b.velocity.shape, generator=g) * 1e-3 for b in twin.blocks]
+            out = [b.velocity.clone() for b in twin.blocks]
+            for b, u in zip(twin.blocks, u_in):
+                b.setVelocity(u)
+            O.piso_step_vjp(twin, dt, gout, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6)
+            for b, u in zip(twin.blocks, out):
+                b.setVelocity(u)
     t = time.perf_counter() - t0
-    return {"value": dom.getTotalSize() * args.cpu_steps / t, "unit": "cells*steps/s", "cores": cores,
-            "kind": "port", "sample": "%d PISO steps of the same workload (FP32, OpenMP %d threads)"
-            % (args.cpu_steps, cores)}
+    what = "forward step + recomputed forward + reverse pass" if args.mode == "fwdbwd" else "forward step"
+    return {"value": dom.getTotalSize() * steps / t, "unit": "cells*steps/s", "cores": cores,
+            "kind": "port", "sample": "%d PISO step(s) of the same workload in %.1f s, %s (FP32, OpenMP %d threads)"
+            % (steps, t, what, cores)}
 
 
 if __name__ == "__main__":

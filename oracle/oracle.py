@@ -354,3 +354,143 @@ def piso_step(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure
         CorrectVelocity(domain, dt, 1)
     CopyVelocityResultToBlocks(domain)
     domain._set_changed(False)
+
+
+# ---------------------------------------------------------------------------
+# adjoint of one PISO step (reverse of piso_step, with the stage VJPs and adjoint solves)
+# ---------------------------------------------------------------------------
+
+def _zero_grads(domain):
+    domain.CreateAGrad()
+    domain.CGrad.CreateValue()
+    domain.PGrad.CreateValue()
+    domain.CreateViscosityGrad()
+    domain.CreateVelocityGradOnBlocks()
+    domain.CreateVelocitySourceGradOnBlocks()
+    domain.CreateVelocityGradOnBoundaries()
+    domain.CreatePressureGradOnBlocks()
+    domain.CreateVelocityRHSGrad()
+    domain.CreateVelocityResultGrad()
+    domain.CreatePressureRHSGrad()
+    domain.CreatePressureRHSdivGrad()
+    domain.CreatePressureResultGrad()
+
+
+def piso_step_vjp(domain, time_step, velocity_grad_out, corrector_steps=2, advection_tol=1e-12, pressure_tol=1e-12,
+                  max_iter=5000):
+    """One forward PISO step (orthogonal branch, as piso_step) followed by its reverse pass.
+
+    velocity_grad_out: per-block gradients of a loss w.r.t. the step's output block velocities.
+    Returns (velocity_grad_in per block, velocity_source_grad per block or None, viscosity_grad float): the gradient
+    w.r.t. the step's input block velocities, the blocks' velocity sources and the domain viscosity, accumulated
+    over every stage in the order PISOtorch_diff's autograd graph visits them (PISOtorch_diff.py:295-1234)."""
+    dt = torch.tensor([time_step], dtype=domain.getDtype())
+    dtf = float(time_step)
+    N = domain.getTotalSize()
+    D = domain.getSpatialDims()
+    blocks = domain.blocks
+    u_in = [b.velocity.clone() for b in blocks]
+    # ---- forward, recording what the reverse pass needs ----
+    SetupAdvectionMatrix(domain, dt, 0)
+    SetupAdvectionVelocity(domain, dt, 0, False)
+    C_adv, A_adv = domain.C.value.clone(), domain.A.clone()
+    ustar = torch.zeros_like(domain.velocityRHS)
+    SolveLinear(domain.C, domain.velocityRHS, ustar, max_iter, advection_tol, 1, useBiCG=True)
+    domain.setVelocityResult(ustar.clone())  # CorrectVelocity overwrites the velocity result in place
+    rec = []
+    vr = ustar
+    for _ in range(corrector_steps):
+        SetupPressureCorrection(domain, dt, 0)
+        prhs = domain.pressureRHS.clone()
+        Pv = domain.P.value.clone()
+        p = torch.zeros_like(domain.pressureRHSdiv)
+        SolveLinear(domain.P, domain.pressureRHSdiv, p, max_iter, pressure_tol, 1, useBiCG=False)
+        ph = (p - torch.mean(p)).contiguous()
+        domain.setPressureResult(ph)
+        CopyPressureResultToBlocks(domain)
+        rec.append((vr.clone(), prhs, Pv, p.clone(), [b.pressure.clone() for b in blocks]))
+        CorrectVelocity(domain, dt, 1)
+        vr = domain.velocityResult.clone()
+    CopyVelocityResultToBlocks(domain)
+    u_out = [b.velocity.clone() for b in blocks]
+
+    # ---- reverse ----
+    g_u = [torch.zeros_like(u) for u in u_in]
+    g_src = [torch.zeros_like(b.velocitySource) if b.velocitySource is not None else None for b in blocks]
+    g_visc = 0.0
+    g_A = torch.zeros(N, dtype=domain.getDtype())
+    g_C = torch.zeros_like(C_adv)
+
+    def collect(d):
+        nonlocal g_visc
+        for i, b in enumerate(d.blocks):
+            g_u[i] += b.velocityGrad
+            if g_src[i] is not None:
+                g_src[i] += b.velocitySourceGrad
+        g_visc += float(d.viscosityGrad[0])
+
+    # CopyVelocityResultToBlocks: velocity result gradient = output block gradients
+    g_vr = torch.cat([torch.cat([gv[0, c].reshape(-1) for gv in velocity_grad_out]) for c in range(D)]).clone()
+    for vr_prev, prhs, Pv, p, pblocks in reversed(rec):
+        # CorrectVelocity (inputs: A, block pressure, pressure RHS)
+        domain.setPressureRHS(prhs)
+        for b, pb in zip(blocks, pblocks):
+            b.setPressure(pb)
+        domain.setA(A_adv.clone())
+        _zero_grads(domain)
+        domain.setVelocityResultGrad(g_vr.clone())
+        CorrectVelocityGrad(domain, dt)
+        g_A += domain.AGrad
+        g_prhs = domain.pressureRHSGrad.clone()
+        # CopyPressureResultToBlocks, mean subtraction, pressure solve
+        CopyPressureResultGradFromBlocks(domain)
+        g_ph = domain.pressureResultGrad.clone()
+        g_p = g_ph - torch.mean(g_ph)
+        lam = torch.zeros_like(g_p)
+        domain.P.setValue(Pv.clone())
+        if bool((g_p != 0).any()):
+            SolveLinear(domain.P, g_p.contiguous(), lam, max_iter, pressure_tol, 1, useBiCG=False, transposeA=True)
+        outer = domain.P.WithZeroValue()
+        SparseOuterProduct(lam, p, outer)
+        gP = -outer.value
+        # SetupPressureCorrection (inputs: velocity, source, velocity result, boundaries, viscosity, A, C)
+        domain.setVelocityResult(vr_prev)
+        domain.setPressureRHS(prhs)
+        domain.C.setValue(C_adv.clone())
+        domain.setA(A_adv.clone())
+        for b, u in zip(blocks, u_in):
+            b.setVelocity(u.clone())
+        _zero_grads(domain)
+        domain.PGrad.setValue(gP.contiguous())
+        domain.setPressureRHSGrad(g_prhs)
+        domain.setPressureRHSdivGrad(lam.clone())
+        SetupPressureCorrectionGrad(domain, dt)
+        collect(domain)
+        g_A += domain.AGrad
+        g_C += domain.CGrad.value
+        g_vr = domain.velocityResultGrad.clone()
+    # advection solve: C u* = rhs, one right-hand side per component
+    lam = torch.zeros_like(g_vr)
+    if bool((g_vr != 0).any()):
+        domain.C.setValue(C_adv.clone())
+        SolveLinear(domain.C, g_vr.contiguous(), lam, max_iter, advection_tol, 1, useBiCG=True, transposeA=True)
+    for c in range(D):
+        outer = domain.C.WithZeroValue()
+        SparseOuterProduct(lam[c * N:(c + 1) * N].contiguous(), ustar[c * N:(c + 1) * N].contiguous(), outer)
+        g_C -= outer.value
+    # SetupAdvectionVelocity (rhs) and SetupAdvectionMatrix (A, C)
+    for b, u in zip(blocks, u_in):
+        b.setVelocity(u.clone())
+    _zero_grads(domain)
+    domain.setVelocityRHSGrad(lam.clone())
+    SetupAdvectionVelocityGrad(domain, dt)
+    collect(domain)
+    _zero_grads(domain)
+    domain.setAGrad(g_A.clone())
+    domain.CGrad.setValue(g_C.clone())
+    SetupAdvectionMatrixGrad(domain, dt)
+    collect(domain)
+    # leave the domain at the step's output state
+    for b, u in zip(blocks, u_out):
+        b.setVelocity(u)
+    return g_u, g_src, g_visc

@@ -1615,7 +1615,7 @@ class Domain:
         d.num_cells = self._totalSize
         d.nnz = self._nnz
         d.dtype = _dtype_code(self._dtype)
-        d.viscosity = float(self.viscosity[0])
+        d.viscosity = float(self.viscosity.detach()[0])
         d.blocks = ctypes.cast(arr, ctypes.POINTER(_abi.BlockDesc))
         d.csr_row = _ptr(self.C.row)
         d.csr_index = _ptr(self.C.index)
@@ -2020,7 +2020,7 @@ def _solve(A, RHS, x, opts):
            for i in infos]
     if solve_log is not None:
         solve_log.append(("BiCG" if opts.use_bicg else "CG", n, A.getSize(), nb, RHS.element_size(), res,
-                          lib.pict_last_solve_kernel_ms()))
+                          lib.pict_last_solve_kernel_ms(), bool(opts.transpose_a)))
     return res
 
 

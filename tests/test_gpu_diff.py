@@ -83,8 +83,8 @@ def test_rollout_gradients_match_finite_differences(name):
             lp = _forward_loss(name, v0, s0, target, steps, visc=visc.detach() + eps)
             lm = _forward_loss(name, v0, s0, target, steps, visc=visc.detach() - eps)
         else:
-            dirs = [((torch.rand(v.shape, generator=g, dtype=torch.float64) * 2 - 1)).cuda() for v in v0]
             base = v0 if kind == "velocity" else s0
+            dirs = [((torch.rand(v.shape, generator=g, dtype=torch.float64) * 2 - 1)).cuda() for v in base]
             grads = grads_v if kind == "velocity" else grads_s
             an = sum(float((gr * dv).sum()) for gr, dv in zip(grads, dirs))
             plus = [x + eps * dv for x, dv in zip(base, dirs)]
@@ -100,3 +100,74 @@ def test_rollout_gradients_match_finite_differences(name):
         print("  %-9s fd %.10e  adjoint %.10e  rel %.2e" % (kind, fd, an, rel), flush=True)
         checks.append((kind, rel))
     assert checks and all(rel <= 1e-4 for _, rel in checks), checks
+
+
+def test_checkpointed_rollout_matches_full_graph():
+    """Per-step checkpointing (forward on the plain backend, recomputation in backward) gives the same gradients as
+    keeping the whole rollout graph, and leaves the domain at the rollout's final state."""
+    from pict_amd import checkpoint as CK
+    steps = 3
+    grads = {}
+    finals = {}
+    for ck in (False, True):
+        d = _setup("periodic_two_block")
+        v0, s0 = _state(d)
+        g = torch.Generator().manual_seed(5)
+        target = [((torch.rand(v.shape, generator=g, dtype=torch.float64) * 2 - 1) * 0.3).cuda() for v in v0]
+        srcs = [s.clone().requires_grad_(True) for s in s0]
+        vin = [v.clone().requires_grad_(True) for v in v0]
+        for b, s, v in zip(d.blocks, srcs, vin):
+            b.setVelocitySource(s)
+            b.setVelocity(v)
+        d.UpdateDomainData()
+        sim = _sim(d, False)
+        out = CK.rollout(sim, steps, params=srcs, checkpoint=ck)
+        L = sum(((o - t) ** 2).sum() for o, t in zip(out, target))
+        L.backward()
+        grads[ck] = [x.grad.detach().clone() for x in srcs + vin]
+        finals[ck] = [b.velocity.detach().clone() for b in d.blocks]
+        assert sim.total_step == steps
+    for a, b in zip(grads[True], grads[False]):
+        err = float((a - b).abs().max() / b.abs().max())
+        print("  checkpointed vs full graph: %.3e" % err, flush=True)
+        assert err < 1e-10
+    for a, b in zip(finals[True], finals[False]):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("name", ["periodic_two_block_coords", "rotated_connection", "box3d"])
+def test_step_gradient_matches_oracle_adjoint(name):
+    """Gradient of one differentiable GPU step (PISOtorch_diff graph: stage VJP kernels, adjoint solves, SDDMM)
+    against the oracle's reverse pass of the same step (oracle.piso_step_vjp), FP64, w.r.t. the input velocities,
+    the velocity sources and the domain viscosity."""
+    from oracle import oracle as O
+    print("case", name, flush=True)
+    d = _setup(name)
+    c = O.cpu_twin(d)
+    v0, s0 = _state(d)
+    g = torch.Generator().manual_seed(31)
+    gout = [((torch.rand(v.shape, generator=g, dtype=torch.float64) * 2 - 1)).cuda() for v in v0]
+    vs = [v.clone().requires_grad_(True) for v in v0]
+    ss = [s.clone().requires_grad_(True) for s in s0]
+    visc = d.viscosity.clone().requires_grad_(True)
+    for b, v, s in zip(d.blocks, vs, ss):
+        b.setVelocity(v)
+        b.setVelocitySource(s)
+    d.setViscosity(visc)
+    d.UpdateDomainData()
+    assert _sim(d, True)._PISO_split_step(1)
+    L = sum((b.velocity * go).sum() for b, go in zip(d.blocks, gout))
+    L.backward()
+    gu, gs, gv = O.piso_step_vjp(c, DT, [x.cpu() for x in gout], corrector_steps=2)
+    for a, b in zip(vs, gu):
+        e = float((a.grad.cpu() - b).abs().max() / b.abs().max())
+        print("  velocity grad rel err %.3e" % e, flush=True)
+        assert e < 1e-8
+    for a, b in zip(ss, gs):
+        e = float((a.grad.cpu() - b).abs().max() / b.abs().max())
+        print("  source grad rel err %.3e" % e, flush=True)
+        assert e < 1e-8
+    if not all(b.viscosity is not None for b in d.blocks):
+        e = abs(float(visc.grad[0]) - gv) / abs(gv)
+        print("  viscosity grad rel err %.3e" % e, flush=True)
+        assert e < 1e-8

@@ -122,3 +122,29 @@ def test_fd_check_detects_a_planted_sign_error(monkeypatch):
                                                             *STAGES["SetupAdvectionVelocity"][2:]))
     with pytest.raises(AssertionError):
         run_check(d, "SetupAdvectionVelocity")
+
+
+@pytest.mark.parametrize("name", ["periodic_two_block", "cavity", "rotated_connection", "box3d"])
+@pytest.mark.parametrize("correctors", [0, 1, 2])
+def test_oracle_step_adjoint_matches_finite_differences(name, correctors):
+    """oracle.piso_step_vjp (the reverse of a whole PISO step: stage VJPs, adjoint solves on A^T, -lambda x^T on the
+    pattern, the pressure mean subtraction) against central differences of oracle.piso_step."""
+    d = scenarios.ALL[name](**CPU)
+    O.build_csr_pattern(d)
+    u0 = [b.velocity.clone() for b in d.blocks]
+    g = _gen(3)
+    gout = [_rand_like(u, g) for u in u0]
+    gu, _, _ = O.piso_step_vjp(d, DT, gout, corrector_steps=correctors)
+
+    def loss(us):
+        dd = scenarios.ALL[name](**CPU)
+        O.build_csr_pattern(dd)
+        for b, u in zip(dd.blocks, us):
+            b.setVelocity(u.clone())
+        O.piso_step(dd, DT, correctors, 1e-13, 1e-13)
+        return sum(float((b.velocity * go).sum()) for b, go in zip(dd.blocks, gout))
+
+    V = [_rand_like(u, g) for u in u0]
+    fd = (loss([u + EPS * v for u, v in zip(u0, V)]) - loss([u - EPS * v for u, v in zip(u0, V)])) / (2 * EPS)
+    an = sum(float((a * v).sum()) for a, v in zip(gu, V))
+    assert abs(fd - an) <= 1e-5 * abs(fd) + 1e-9, (fd, an)
