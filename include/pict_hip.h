@@ -271,6 +271,8 @@ typedef struct pict_csr_desc {
     const int32_t *row;       /* [n+1] */
     const int32_t *index;     /* [nnz] */
     const void    *value;     /* [nnz] */
+    const int32_t *transpose_perm; /* optional [nnz]: position of (j,i) for entry (i,j), from
+                                      pict_csr_transpose_permutation; NULL = built per transposed solve */
 } pict_csr_desc;
 
 typedef struct pict_solve_opts {
@@ -305,6 +307,41 @@ typedef struct pict_solve_info {     /* LinearSolverResultInfo, bicgstab_solver.
 pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, int32_t nrhs,
                               const pict_solve_opts *opts, pict_solve_info *infos_out,
                               void *stream);
+
+/* Transpose map of a structurally symmetric pattern (no counterpart in the reference, which
+   transposes inside cuSPARSE): perm_out[e] = position of entry (j,i) for entry e = (i,j).
+   Returns PICT_ERR_UNSUPPORTED if some (j,i) is missing.  Synchronises `stream`. */
+pict_status pict_csr_transpose_permutation(const pict_csr_desc *pattern, int32_t *perm_out, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* whole step                                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct pict_step_opts {
+    double  time_step;
+    int32_t corrector_steps;
+    int32_t velocity_corrector;   /* CorrectVelocity version: 0 or 1                    */
+    int32_t time_step_norm;
+    int32_t normalize_pressure;   /* subtract the mean of every pressure result         */
+    pict_solve_opts advection;    /* the velocity solve (num_dims right-hand sides)     */
+    pict_solve_opts pressure;     /* each pressure solve                                */
+} pict_step_opts;
+
+/* One PISO step, orthogonal branch of the reference driver without hooks or passive
+   scalars (PISOtorch_simulation.py:1054-1327): the stages and solves of
+   PISOtorch_simulation._PISO_split_step queued back to back on `stream`, with one host
+   wait at the end.  Block velocities are read and overwritten; block pressures and the
+   domain's C, A, P, RHS and result vectors are written.  infos_out receives num_dims
+   advection entries followed by one entry per corrector.  No counterpart in the
+   reference, whose driver calls each stage from Python. */
+pict_status pict_piso_step(const pict_atlas *atlas, const pict_step_opts *opts, pict_solve_info *infos_out,
+                           void *stream);
+/* v -= mean(v) in place, the mean accumulated in FP64 in a fixed order: the pressure
+   normalisation of the driver (PISOtorch_simulation.py:1243-1244, a torch expression there). */
+pict_status pict_subtract_mean(void *v, int32_t n, int32_t dtype, void *stream);
+/* Diagnostics: HIP-event kernel times of the solves of the last pict_piso_step. */
+pict_status pict_set_step_timing(int32_t enable);
+int32_t     pict_last_step_solve_ms(double *ms_out, int32_t max_count);
 
 /* Diagnostics (no counterpart in the reference): when enabled, the solver
    records HIP events around its kernel launch on the launch stream; the

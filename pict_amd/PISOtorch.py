@@ -242,7 +242,18 @@ class CSRmatrix:
         d.row = self._row.data_ptr()
         d.index = self._index.data_ptr()
         d.value = self._value.data_ptr()
+        perm = getattr(self._index, "_pict_transpose_perm", None)
+        d.transpose_perm = perm.data_ptr() if perm is not None else 0
         return d
+
+    def _ensure_transpose_perm(self):
+        """The A^T map of the pattern, built once and kept with the (immutable, shared) index tensor."""
+        if getattr(self._index, "_pict_transpose_perm", None) is None and self._index.is_cuda:
+            perm = torch.empty_like(self._index)
+            d = self._desc()
+            _abi.check(_abi.load_library().pict_csr_transpose_permutation(
+                ctypes.byref(d), ctypes.c_void_p(perm.data_ptr()), _stream(self._index.device)), "transpose")
+            self._index._pict_transpose_perm = perm
 
 
 # ----------------------------------------------------------------------------
@@ -1189,7 +1200,34 @@ class Block:
             " ".join("%s=%s" % (n, b) for n, b in zip(["-x", "+x", "-y", "+y", "-z", "+z"], self.boundaries)))
 
     # --- descriptor packing ---
+    def _pack_signature(self, block_index_of):
+        """Everything the block descriptor is built from: tensor addresses and ranks, offsets and the boundary
+        objects.  An unchanged signature means an identical descriptor."""
+        def t(x):
+            return None if x is None else (x.data_ptr(), x.dim())
+        sig = [self.globalOffset, self.csrOffset, t(self.velocity), t(self.pressure), t(self._viscosity),
+               t(self.velocitySource), t(self._transform), t(self._faceTransform), t(self._velocity_grad),
+               t(self._pressure_grad), t(self._viscosity_grad), t(self.velocitySource_grad)]
+        for bd in self.boundaries:
+            if bd.type == BoundaryType.CONNECTED:
+                sig.append((id(bd), block_index_of(bd.getConnectedBlock()), tuple(bd.axes)))
+            elif bd.type == BoundaryType.FIXED:
+                sig.append((id(bd), t(bd.velocity), t(bd._velocity_grad), t(bd.transform), int(bd.velocityType),
+                            None if bd._size is None else tuple(bd._size)))
+            else:
+                sig.append(int(bd.type))
+        return tuple(sig)
+
     def _pack(self, block_index_of):
+        sig = self._pack_signature(block_index_of)
+        cached = getattr(self, "_packed", None)
+        if cached is not None and cached[0] == sig:
+            return cached[1]
+        b = self._pack_full(block_index_of)
+        self._packed = (sig, b)
+        return b
+
+    def _pack_full(self, block_index_of):
         b = _abi.BlockDesc()
         s, st = self.getSizes(), self.getStrides()
         b.global_offset = self.globalOffset
@@ -1605,38 +1643,38 @@ class Domain:
             _abi.check(lib.pict_build_csr_pattern(self._atlas, _stream(self._device)), "PrepareSolve")
 
     def _pack(self):
+        """Descriptor of the domain (pict_domain_desc) and its block array; rebuilt only when a tensor address,
+        an offset or a boundary object changed since the last call."""
+        g = lambda t: 0 if (t is None or t.numel() == 0) else t.data_ptr()  # noqa: E731
+        blocks = [b._pack(self._block_index) for b in self.blocks]
+        ptrs = (self._totalSize, self._nnz, float(self.viscosity.detach()[0]), _ptr(self.C.row), _ptr(self.C.index),
+                _ptr(self.C.value), _ptr(self.P.value), _ptr(self.A), _ptr(self.velocityRHS),
+                _ptr(self.velocityResult), _ptr(self.pressureRHS), _ptr(self.pressureRHSdiv),
+                _ptr(self.pressureResult), g(self.viscosity_grad),
+                g(self.C_grad.value if self.C_grad is not None else None),
+                g(self.P_grad.value if self.P_grad is not None else None), g(self.A_grad), g(self.velocityRHS_grad),
+                g(self.velocityResult_grad), g(self.pressureRHS_grad), g(self.pressureRHSdiv_grad),
+                g(self.pressureResult_grad))
+        sig = (ptrs, tuple(id(b) for b in blocks))
+        cached = getattr(self, "_packed", None)
+        if cached is not None and cached[0] == sig:
+            return cached[1], cached[2]
         n = len(self.blocks)
         arr = (_abi.BlockDesc * n)()
-        for i, b in enumerate(self.blocks):
-            arr[i] = b._pack(self._block_index)
+        for i, b in enumerate(blocks):
+            arr[i] = b
         d = _abi.DomainDesc()
         d.num_dims = self._dims
         d.num_blocks = n
         d.num_cells = self._totalSize
         d.nnz = self._nnz
         d.dtype = _dtype_code(self._dtype)
-        d.viscosity = float(self.viscosity.detach()[0])
+        d.viscosity = ptrs[2]
         d.blocks = ctypes.cast(arr, ctypes.POINTER(_abi.BlockDesc))
-        d.csr_row = _ptr(self.C.row)
-        d.csr_index = _ptr(self.C.index)
-        d.C_value = _ptr(self.C.value)
-        d.P_value = _ptr(self.P.value)
-        d.A = _ptr(self.A)
-        d.velocity_rhs = _ptr(self.velocityRHS)
-        d.velocity_result = _ptr(self.velocityResult)
-        d.pressure_rhs = _ptr(self.pressureRHS)
-        d.pressure_rhs_div = _ptr(self.pressureRHSdiv)
-        d.pressure_result = _ptr(self.pressureResult)
-        g = lambda t: 0 if (t is None or t.numel() == 0) else t.data_ptr()  # noqa: E731
-        d.viscosity_grad = g(self.viscosity_grad)
-        d.C_grad = g(self.C_grad.value if self.C_grad is not None else None)
-        d.P_grad = g(self.P_grad.value if self.P_grad is not None else None)
-        d.A_grad = g(self.A_grad)
-        d.velocity_rhs_grad = g(self.velocityRHS_grad)
-        d.velocity_result_grad = g(self.velocityResult_grad)
-        d.pressure_rhs_grad = g(self.pressureRHS_grad)
-        d.pressure_rhs_div_grad = g(self.pressureRHSdiv_grad)
-        d.pressure_result_grad = g(self.pressureResult_grad)
+        (d.csr_row, d.csr_index, d.C_value, d.P_value, d.A, d.velocity_rhs, d.velocity_result, d.pressure_rhs,
+         d.pressure_rhs_div, d.pressure_result, d.viscosity_grad, d.C_grad, d.P_grad, d.A_grad, d.velocity_rhs_grad,
+         d.velocity_result_grad, d.pressure_rhs_grad, d.pressure_rhs_div_grad, d.pressure_result_grad) = ptrs[3:]
+        self._packed = (sig, d, arr)
         return d, arr
 
     def UpdateDomainData(self):
@@ -1959,6 +1997,17 @@ def CopyPressureResultGradFromBlocks(domain):
     _call(domain, "pict_copy_pressure_result_grad_from_blocks")
 
 
+def _subtract_mean(t):
+    """t - mean(t) as a new tensor; the mean is accumulated in FP64 in a fixed order by the library (the pressure
+    normalisation of the driver, PISOtorch_simulation.py:1243-1244)."""
+    _check(t.is_cuda and t.dim() == 1, "a 1D device tensor is required")
+    out = t.detach().clone().contiguous()
+    if out.numel() > 0:
+        _abi.check(_abi.load_library().pict_subtract_mean(ctypes.c_void_p(out.data_ptr()), out.numel(),
+                                                          _dtype_code(out.dtype), _stream(out.device)), "mean")
+    return out
+
+
 def CopyScalarResultGradFromBlocks(domain):
     raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
 
@@ -2011,6 +2060,8 @@ def _solve(A, RHS, x, opts):
     _check(x.dtype == A.getDtype(), "Data type of x must match A.")
     nb = RHS.size(0) // n
     infos = (_abi.SolveInfo * nb)()
+    if opts.transpose_a:
+        A._ensure_transpose_perm()
     desc = A._desc()
     lib = _abi.load_library()
     _abi.check(lib.pict_solve_linear(ctypes.byref(desc), ctypes.c_void_p(RHS.data_ptr()),

@@ -86,56 +86,58 @@ def _linear_solve_wrapper(csrMat, rhs, result, maxit_torch, tol_torch, convergen
     return infos
 
 
+class _LinearSolveFunction(torch.autograd.Function):
+    """x = A^-1 b (PISOtorch_diff.py:316-372); `cfg` = (A detached, maxit, tol, criterion, use_BiCG, transpose,
+    return_best_result, double_fallback, BiCG_with_preconditioner, BiCG_precondition_fallback)."""
+
+    @staticmethod
+    def forward(ctx, cfg, A_val, b):
+        A, maxit_torch, tol_torch, conv, use_BiCG, transpose, best, dfb, pre, pfb = cfg
+        x = torch.zeros_like(b)
+        _linear_solve_wrapper(A, b.contiguous(), x, maxit_torch, tol_torch, conv, use_BiCG, False, 0, transpose,
+                              False, best, is_FWD=True, double_fallback=dfb, BiCG_with_preconditioner=pre,
+                              BiCG_precondition_fallback=pfb)
+        ctx.cfg = cfg
+        ctx.save_for_backward(A_val, x)
+        return x
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, grad_x):
+        A, maxit_torch, tol_torch, conv, use_BiCG, transpose, best, dfb, pre, pfb = ctx.cfg
+        grad_b = None
+        grad_A_val = None
+        if ctx.needs_input_grad[2] or ctx.needs_input_grad[1]:
+            _, x = ctx.saved_tensors
+            grad_b = torch.zeros_like(grad_x)
+            _linear_solve_wrapper(A, grad_x.contiguous(), grad_b, maxit_torch, tol_torch, conv, use_BiCG, False, 0,
+                                  not transpose, False, best, is_FWD=False, double_fallback=dfb,
+                                  BiCG_with_preconditioner=pre, BiCG_precondition_fallback=pfb)
+            if ctx.needs_input_grad[1]:
+                grad_A = A.WithZeroValue()
+                grad_A_val = grad_A.value
+                n = grad_A.getRows()
+                if n == grad_b.size(0):
+                    PISOtorch.SparseOuterProduct(grad_b, x, grad_A)
+                else:
+                    for gb, xs in zip(torch.split(grad_b, n), torch.split(x, n)):
+                        part = grad_A.WithZeroValue()
+                        PISOtorch.SparseOuterProduct(gb.contiguous(), xs.contiguous(), part)
+                        grad_A_val += part.value
+                grad_A_val *= -1
+        return None, grad_A_val, grad_b
+
+
 def linear_solve_GPU(csrMat, rhs, transpose=False, use_BiCG=False, tol=None, max_iter=5000, return_best_result=False,
                      double_fallback=False, BiCG_with_preconditioner=True, BiCG_precondition_fallback=False):
     """PISOtorch_diff.py:295-374: x = A^-1 b with gradients for b (adjoint solve on A^T) and A (-lambda x^T on
     the pattern, summed over batched right-hand sides)."""
-    A = csrMat.clone()
-    A.detach()
-    conv = PISOtorch.ConvergenceCriterion.NORM2_NORMALIZED
-    dtype = rhs.dtype
-    tol_torch = _get_solver_tolerance_torch(tol, dtype)
-    maxit_torch = torch.IntTensor([max_iter])
-
-    class LinearSolveFunction(torch.autograd.Function):
-        @staticmethod
-        def forward(ctx, A_val, b):
-            x = torch.zeros_like(b)
-            _linear_solve_wrapper(A, b.contiguous(), x, maxit_torch, tol_torch, conv, use_BiCG, False, 0, transpose,
-                                  False, return_best_result, is_FWD=True, double_fallback=double_fallback,
-                                  BiCG_with_preconditioner=BiCG_with_preconditioner,
-                                  BiCG_precondition_fallback=BiCG_precondition_fallback)
-            ctx.save_for_backward(A_val, x)
-            return x
-
-        @staticmethod
-        @torch.autograd.function.once_differentiable
-        def backward(ctx, grad_x):
-            grad_b = None
-            grad_A_val = None
-            if ctx.needs_input_grad[1] or ctx.needs_input_grad[0]:
-                _, x = ctx.saved_tensors
-                grad_b = torch.zeros_like(grad_x)
-                _linear_solve_wrapper(A, grad_x.contiguous(), grad_b, maxit_torch, tol_torch, conv, use_BiCG, False,
-                                      0, not transpose, False, return_best_result, is_FWD=False,
-                                      double_fallback=double_fallback,
-                                      BiCG_with_preconditioner=BiCG_with_preconditioner,
-                                      BiCG_precondition_fallback=BiCG_precondition_fallback)
-                if ctx.needs_input_grad[0]:
-                    grad_A = A.WithZeroValue()
-                    grad_A_val = grad_A.value
-                    n = grad_A.getRows()
-                    if n == grad_b.size(0):
-                        PISOtorch.SparseOuterProduct(grad_b, x, grad_A)
-                    else:
-                        for gb, xs in zip(torch.split(grad_b, n), torch.split(x, n)):
-                            part = grad_A.WithZeroValue()
-                            PISOtorch.SparseOuterProduct(gb.contiguous(), xs.contiguous(), part)
-                            grad_A_val += part.value
-                    grad_A_val *= -1
-            return grad_A_val, grad_b
-
-    return LinearSolveFunction.apply(csrMat.value, rhs)
+    # a detached copy of the values (later stages may overwrite csrMat.value in place); the pattern is immutable
+    A = PISOtorch.CSRmatrix(csrMat.value.detach().clone(), csrMat.index, csrMat.row)
+    cfg = (A, torch.IntTensor([max_iter]), _get_solver_tolerance_torch(tol, rhs.dtype),
+           PISOtorch.ConvergenceCriterion.NORM2_NORMALIZED, use_BiCG, transpose, return_best_result, double_fallback,
+           BiCG_with_preconditioner, BiCG_precondition_fallback)
+    return _LinearSolveFunction.apply(cfg, csrMat.value, rhs)
 
 
 # ----------------------------------------------------------------------------
@@ -212,11 +214,21 @@ def _domain_entries(domain):
             yield (bsec, "BOUNDARY_VELOCITY_GRAD", lambda fb=fb: fb.velocityGrad, fb.setVelocityGrad, None, False)
 
 
+def _entries(domain):
+    """_domain_entries as a list, cached on the domain for its current block and boundary objects."""
+    key = tuple((id(b), tuple(id(bd) for bd in b.boundaries)) for b in domain.getBlocks())
+    cached = getattr(domain, "_diff_entries", None)
+    if cached is None or cached[0] != key:
+        cached = (key, list(_domain_entries(domain)))
+        domain._diff_entries = cached
+    return cached[1]
+
+
 def flatten_domain(domain, tensor_filter, clone=False, empty=False, exclusion_list=()):
     """The domain's tensors whose names pass `tensor_filter` as a flat list, plus the index map needed to put them
     back (PISOtorch_diff.py:44-118).  Absent optional tensors are None; `empty` or `exclusion_list` entries too."""
     index, tensors = {}, []
-    for sec, name, get, _, _, on_cpu in _domain_entries(domain):
+    for sec, name, get, _, _, on_cpu in _entries(domain):
         if tensor_filter is not None and name not in tensor_filter:
             continue
         t = None if (empty or name in exclusion_list) else get()
@@ -232,7 +244,7 @@ def flatten_domain(domain, tensor_filter, clone=False, empty=False, exclusion_li
 def set_domain_tensors_from_flat(domain, domain_dict, tensor_list, tensor_filter=None, clone=False):
     """Inverse of flatten_domain (PISOtorch_diff.py:120-181): set the listed tensors back on the domain (clearing
     optional ones that were None)."""
-    for sec, name, _, setter, clear, _ in _domain_entries(domain):
+    for sec, name, _, setter, clear, _ in _entries(domain):
         if (sec, name) not in domain_dict or setter is None:
             continue
         if tensor_filter is not None and name not in tensor_filter:
@@ -244,49 +256,53 @@ def set_domain_tensors_from_flat(domain, domain_dict, tensor_list, tensor_filter
             clear()
 
 
+class _StageFunction(torch.autograd.Function):
+    """autograd.Function of one stage; the stage is described by `spec` = (name, tracked, saved, forward,
+    backward, outputs): `tracked` inputs (flatten_domain names) are its differentiable inputs, `saved` tensors are
+    restored on the domain before the backward kernel runs, `outputs(domain)` are returned."""
+
+    @staticmethod
+    def forward(ctx, spec, domain, time_step, *tracked_tensors):
+        name, tracked, saved, forward, backward, outputs = spec
+        forward(domain)
+        ctx.spec = spec
+        ctx.domain = domain
+        if saved:
+            ctx.saved_dict, tensors = flatten_domain(domain, saved)
+        else:
+            ctx.saved_dict, tensors = None, []
+        ctx.saved_none = [t is None for t in tensors]
+        ctx.save_for_backward(*[t for t in tensors if t is not None], time_step)
+        outs = outputs(domain)
+        return outs if isinstance(outs, tuple) else (outs,)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, *out_grads):
+        name, tracked, saved, forward, backward, outputs = ctx.spec
+        domain = ctx.domain
+        grad_names = [t + "_GRAD" for t in tracked]
+        if any(ctx.needs_input_grad[3:]):
+            st = list(ctx.saved_tensors)
+            time_step = st.pop()
+            if ctx.saved_dict is not None:
+                it = iter(st)
+                tensors = [None if none else next(it) for none in ctx.saved_none]
+                set_domain_tensors_from_flat(domain, ctx.saved_dict, tensors)
+            backward(domain, time_step, out_grads)
+            _, grads = flatten_domain(domain, grad_names, exclusion_list=_EXCLUDED_GRADIENTS[name][1])
+        else:
+            _, grads = flatten_domain(domain, grad_names, empty=True)
+        return (None, None, None, *grads)
+
+
 def _stage(fn_name, tracked, saved, forward, backward, outputs):
-    """Builds a stage's autograd.Function: `tracked` inputs (flatten_domain names) are its differentiable inputs,
-    `saved` tensors are restored on the domain before the backward kernel runs, `outputs(domain)` are returned."""
-    grad_names = [t + "_GRAD" for t in tracked]
-
-    class StageFunction(torch.autograd.Function):
-        @staticmethod
-        def forward(ctx, domain, time_step, *tracked_tensors):
-            forward(domain)
-            ctx.domain = domain
-            if saved:
-                ctx.saved_dict, tensors = flatten_domain(domain, saved)
-            else:
-                ctx.saved_dict, tensors = None, []
-            ctx.saved_none = [t is None for t in tensors]
-            ctx.save_for_backward(*[t for t in tensors if t is not None], time_step)
-            outs = outputs(domain)
-            return outs if isinstance(outs, tuple) else (outs,)
-
-        @staticmethod
-        @torch.autograd.function.once_differentiable
-        def backward(ctx, *out_grads):
-            domain = ctx.domain
-            if any(ctx.needs_input_grad[2:]):
-                st = list(ctx.saved_tensors)
-                time_step = st.pop()
-                if ctx.saved_dict is not None:
-                    it = iter(st)
-                    tensors = [None if none else next(it) for none in ctx.saved_none]
-                    set_domain_tensors_from_flat(domain, ctx.saved_dict, tensors)
-                backward(domain, time_step, out_grads)
-                _, grads = flatten_domain(domain, grad_names, exclusion_list=_EXCLUDED_GRADIENTS[fn_name][1])
-            else:
-                _, grads = flatten_domain(domain, grad_names, empty=True)
-            return (None, None, *grads)
-
-    return StageFunction
+    return (fn_name, tracked, saved, forward, backward, outputs)
 
 
-def _apply(fn, domain, time_step, tracked):
+def _apply(spec, domain, time_step, tracked):
     _, tensors = flatten_domain(domain, tracked)
-    outs = fn.apply(domain, time_step, *tensors)
-    return outs
+    return _StageFunction.apply(spec, domain, time_step, *tensors)
 
 
 _EXCLUDED_GRADIENTS["SetupAdvectionMatrix"] = (set(), set())
@@ -347,25 +363,30 @@ def SetupAdvectionVelocity(domain, time_step, non_ortho_flags, apply_pressure_gr
     return _apply(fn, domain, time_step, tracked)[0]
 
 
+class _CopyFunction(torch.autograd.Function):
+    """autograd.Function of a Copy*Result* stage; `spec` = (forward, backward, outputs)."""
+
+    @staticmethod
+    def forward(ctx, spec, domain, *inputs):
+        forward, backward, outputs = spec
+        ctx.spec = spec
+        ctx.domain = domain
+        forward(domain)
+        outs = outputs(domain)
+        return outs if isinstance(outs, tuple) else (outs,)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, *grads):
+        if any(ctx.needs_input_grad[2:]):
+            res = ctx.spec[1](ctx.domain, grads)
+        else:
+            res = [None] * (len(ctx.needs_input_grad) - 2)
+        return (None, None, *res)
+
+
 def _copy_stage(forward, backward, tensors_in, outputs):
-    class CopyFunction(torch.autograd.Function):
-        @staticmethod
-        def forward(ctx, domain, *inputs):
-            ctx.domain = domain
-            forward(domain)
-            outs = outputs(domain)
-            return outs if isinstance(outs, tuple) else (outs,)
-
-        @staticmethod
-        @torch.autograd.function.once_differentiable
-        def backward(ctx, *grads):
-            if any(ctx.needs_input_grad[1:]):
-                res = backward(ctx.domain, grads)
-            else:
-                res = [None] * (len(ctx.needs_input_grad) - 1)
-            return (None, *res)
-
-    return CopyFunction
+    return (forward, backward, outputs)
 
 
 def CopyVelocityResultToBlocks(domain):
@@ -384,7 +405,7 @@ def CopyVelocityResultToBlocks(domain):
         return [d.velocityResultGrad]
 
     fn = _copy_stage(fwd, bwd, None, lambda d: tuple(b.velocity for b in d.getBlocks()))
-    fn.apply(domain, domain.velocityResult)
+    _CopyFunction.apply(fn, domain, domain.velocityResult)
 
 
 def CopyVelocityResultFromBlocks(domain):
@@ -402,7 +423,7 @@ def CopyVelocityResultFromBlocks(domain):
         return [b.velocityGrad for b in d.getBlocks()]
 
     fn = _copy_stage(fwd, bwd, None, lambda d: d.velocityResult)
-    fn.apply(domain, *[b.velocity for b in domain.getBlocks()])
+    _CopyFunction.apply(fn, domain, *[b.velocity for b in domain.getBlocks()])
 
 
 def CopyPressureResultToBlocks(domain):
@@ -421,7 +442,7 @@ def CopyPressureResultToBlocks(domain):
         return [d.pressureResultGrad]
 
     fn = _copy_stage(fwd, bwd, None, lambda d: tuple(b.pressure for b in d.getBlocks()))
-    fn.apply(domain, domain.pressureResult)
+    _CopyFunction.apply(fn, domain, domain.pressureResult)
 
 
 def CopyScalarResultToBlocks(domain):
@@ -586,6 +607,23 @@ def CorrectVelocity(domain, time_step, version, timeStepNorm=False):
 
     fn = _stage(name, tracked, ["A", "PRESSURE"], fwd, bwd, lambda d: d.velocityResult)
     _apply(fn, domain, time_step, tracked)
+
+
+class _SubtractMeanFunction(torch.autograd.Function):
+    """x - mean(x); its VJP is g - mean(g)."""
+
+    @staticmethod
+    def forward(ctx, x):
+        return PISOtorch._subtract_mean(x)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g):
+        return PISOtorch._subtract_mean(g.contiguous())
+
+
+def _subtract_mean(x):
+    return _SubtractMeanFunction.apply(x)
 
 
 def reset_domain(domain):

@@ -10,6 +10,7 @@ Image/VTK output, adaptive CFL stepping and the scipy solve path are host
 conveniences outside the PISO step and are not reproduced: the corresponding
 arguments are accepted and must stay at their defaults.
 """
+import ctypes
 import logging
 import numbers
 from contextlib import nullcontext
@@ -49,6 +50,10 @@ class StopHandler:
         self._stop = False
 
 
+def _never_stop():
+    return False
+
+
 class Simulation:
     # must match PISO_multiblock_cuda.h:9-13
     NON_ORTHO_DIRECT_MATRIX = 1
@@ -69,7 +74,7 @@ class Simulation:
                  exclude_advection_solve_gradients=False, exclude_pressure_solve_gradients=False, log_dir=None,
                  log_interval=0, log_images=True, log_vtk=False, norm_vel=False, block_layout=None,
                  output_mode3D="slice", log_fn=None, output_resampling_coords=None, output_resampling_shape=None,
-                 output_resampling_fill_max_steps=0, save_domain_name=None, stop_fn=lambda: False):
+                 output_resampling_fill_max_steps=0, save_domain_name=None, stop_fn=_never_stop, native_step=True):
         if density_viscosity is not None:
             raise NotImplementedError("Simulation.density_viscosity is superseded by Domain.scalarViscosity.")
         if scipy_solve_advection or scipy_solve_pressure:
@@ -114,6 +119,8 @@ class Simulation:
         self.exclude_advection_solve_gradients = exclude_advection_solve_gradients
         self.exclude_pressure_solve_gradients = exclude_pressure_solve_gradients
         self.stop_fn = stop_fn
+        # run eligible steps as one native call (pict_piso_step); not a reference option
+        self.native_step = native_step
         self.solver_infos = []  # (kind, [LinearSolverResultInfo]) of the last step, for iteration-count logging
         self.record_solver_infos = False
         self.reset_step_counters()
@@ -328,6 +335,8 @@ class Simulation:
         non_ortho_flags = self._non_ortho_flags
         B = self._backend
         self.solver_infos = []
+        if self._native_step_eligible():
+            return self._native_steps(iterations, time_step)
 
         for step in range(iterations):
             if self.convergence_tol is not None:
@@ -393,7 +402,7 @@ class Simulation:
                     if not solve_ok:
                         return solve_ok
                     if self.normalize_pressure_result:
-                        pressureResult = pressureResult - torch.mean(pressureResult)
+                        pressureResult = B._subtract_mean(pressureResult)
                     domain.setPressureResult(pressureResult)
                     domain.UpdateDomainData()
                     self._run_prep_fn("POST_PRESSURE_RESULT", domain=domain, local_step=step, time_step=time_step,
@@ -416,7 +425,7 @@ class Simulation:
                                 residual_reset_step=100, use_BiCG=self.pressure_use_BiCG, tol=self.pressure_tol,
                                 return_best_result=self.pressure_return_best_result)
                         if self.normalize_pressure_result:
-                            pressureResult = pressureResult - torch.mean(pressureResult)
+                            pressureResult = B._subtract_mean(pressureResult)
                         domain.setPressureResult(pressureResult)
                         domain.UpdateDomainData()
                         self._run_prep_fn("POST_PRESSURE_RESULT", domain=domain, local_step=step,
@@ -445,6 +454,77 @@ class Simulation:
                 break
             self.end_step(time_step)
         return solve_ok
+
+    # --- native step (pict_piso_step): the orthogonal branch above as one library call ---
+    def _native_step_eligible(self):
+        """True when _PISO_split_step would run exactly the stage sequence pict_piso_step queues: orthogonal,
+        not differentiable, no passive scalar, no hooks, no early-stop checks, every solve from zero, no solver
+        fallbacks and a finite-difference velocity corrector."""
+        d = self.domain
+        return (self.native_step and not self.differentiable and not self.non_orthogonal
+                and not (self.advect_passive_scalar and d.hasPassiveScalar())
+                and not self._prep_fn and self.stop_fn is _never_stop and self.convergence_tol is None
+                and not self.advect_reuse_result and not self.pressure_reuse_result
+                and not self.solver_double_fallback and not self.preconditionBiCG
+                and self._velocity_corrector_version in (0, 1) and d.getDevice().type == "cuda")
+
+    def _native_solve_opts(self, use_BiCG, tol, return_best):
+        dtype = self._get_dtype()
+        return PISOtorch._solve_opts(self.linear_solve_max_iterations,
+                                     PISOtorch_diff._get_solver_tolerance(tol, dtype),
+                                     PISOtorch.ConvergenceCriterion.NORM2_NORMALIZED, use_BiCG, False, 0, False,
+                                     False, return_best, False, zero_rhs_shortcut=True)
+
+    def _native_steps(self, iterations, time_step):
+        from . import _abi
+        domain = self.domain
+        D = domain.getSpatialDims()
+        K = self.corrector_steps
+        o = _abi.StepOpts()
+        o.time_step = float(ntonp(time_step)[0])
+        o.corrector_steps = K
+        o.velocity_corrector = self._velocity_corrector_version
+        o.time_step_norm = int(bool(self.pressure_time_step_normalized))
+        o.normalize_pressure = int(bool(self.normalize_pressure_result))
+        o.advection = self._native_solve_opts(self.advection_use_BiCG, self.advection_tol, False)
+        o.pressure = self._native_solve_opts(self.pressure_use_BiCG, self.pressure_tol,
+                                             self.pressure_return_best_result)
+        lib = _abi.load_library()
+        log = PISOtorch.solve_log
+        if log is not None:
+            _abi.check(lib.pict_set_step_timing(1))
+        n_csr = domain.C.getSize()
+        for _ in range(iterations):
+            domain.UpdateDomainData()
+            infos = (_abi.SolveInfo * (D + K))()
+            _abi.check(lib.pict_piso_step(domain._atlas, ctypes.byref(o), infos,
+                                          PISOtorch._stream(domain.getDevice())), "PISO step")
+            res = [PISOtorch.LinearSolverResultInfo(i.final_residual, i.used_iterations, i.converged,
+                                                    i.is_finite_residual) for i in infos]
+            adv, prs = res[:D], res[D:]
+            if log is not None:
+                ms = (ctypes.c_double * (1 + K))()
+                lib.pict_last_step_solve_ms(ms, 1 + K)
+                es = domain.velocityRHS.element_size()
+                log.append(("BiCG" if self.advection_use_BiCG else "CG", domain.getTotalSize(), n_csr, D, es, adv,
+                            ms[0], False))
+                for k in range(K):
+                    log.append(("BiCG" if self.pressure_use_BiCG else "CG", domain.getTotalSize(), n_csr, 1, es,
+                                [prs[k]], ms[1 + k], False))
+            if self.record_solver_infos:
+                self.solver_infos.append(("BiCG" if self.advection_use_BiCG else "CG", adv))
+                for k in range(K):
+                    self.solver_infos.append(("BiCG" if self.pressure_use_BiCG else "CG", [prs[k]]))
+            PISOtorch_diff._check_solver_return_infos(adv, False, self.advection_use_BiCG, o.advection.tolerance,
+                                                      o.advection.max_iterations, False, is_FWD=True)
+            for k in range(K):
+                PISOtorch_diff._check_solver_return_infos([prs[k]], False, self.pressure_use_BiCG,
+                                                          o.pressure.tolerance, o.pressure.max_iterations,
+                                                          self.pressure_return_best_result, is_FWD=True)
+            self.end_step(time_step)
+        if log is not None:
+            _abi.check(lib.pict_set_step_timing(0))
+        return True
 
     def run(self, iterations, static=False, log_domain=True):
         """Simulation.run (:1374-1518) without image/VTK output."""
@@ -507,7 +587,7 @@ class Simulation:
                                                                  return_best_result=self.pressure_return_best_result,
                                                                  max_iter=max_iter)
                 if self.normalize_pressure_result:
-                    pressureResult = pressureResult - torch.mean(pressureResult)
+                    pressureResult = B._subtract_mean(pressureResult)
                 domain.setPressureResult(pressureResult)
                 domain.UpdateDomainData()
                 if not solve_ok:

@@ -107,6 +107,7 @@ class CsrDesc(Structure):
         ("row", c_void_p),
         ("index", c_void_p),
         ("value", c_void_p),
+        ("transpose_perm", c_void_p),
     ]
 
 
@@ -133,6 +134,18 @@ class SolveInfo(Structure):
         ("converged", c_int32),
         ("is_finite_residual", c_int32),
         ("reserved", c_int32),
+    ]
+
+
+class StepOpts(Structure):
+    _fields_ = [
+        ("time_step", c_double),
+        ("corrector_steps", c_int32),
+        ("velocity_corrector", c_int32),
+        ("time_step_norm", c_int32),
+        ("normalize_pressure", c_int32),
+        ("advection", SolveOpts),
+        ("pressure", SolveOpts),
     ]
 
 
@@ -169,6 +182,11 @@ SIGNATURES = {
     "pict_copy_pressure_result_grad_from_blocks": (c_int32, [_P, _P]),
     "pict_solve_linear": (c_int32, [POINTER(CsrDesc), _P, _P, c_int32, POINTER(SolveOpts), POINTER(SolveInfo), _P]),
     "pict_sparse_outer_product": (c_int32, [_P, _P, POINTER(CsrDesc), _P, _P]),
+    "pict_csr_transpose_permutation": (c_int32, [POINTER(CsrDesc), _P, _P]),
+    "pict_piso_step": (c_int32, [_P, POINTER(StepOpts), POINTER(SolveInfo), _P]),
+    "pict_subtract_mean": (c_int32, [_P, c_int32, c_int32, _P]),
+    "pict_set_step_timing": (c_int32, [c_int32]),
+    "pict_last_step_solve_ms": (c_int32, [POINTER(c_double), c_int32]),
     "pict_set_kernel_timing": (c_int32, [c_int32]),
     "pict_last_solve_kernel_ms": (c_double, []),
     "pict_coords_to_transforms": (c_int32, [_P, c_int32, POINTER(c_int32), c_int32, _P, _P]),

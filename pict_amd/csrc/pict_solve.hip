@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "pict_host.hpp"
+#include "pict_solver.hpp"
 
 namespace pict {
 
@@ -1108,9 +1109,12 @@ const void *kernel_for_nrhs(Kind kind, int nrhs) {
     }
 }
 
+// Queues one solve on `stream`.  When it has run, the final state of each right-hand side is in
+// state_out[0..nrhs) and the barrier error flag in *err_out (device memory); ev0/ev1 (optional) bracket
+// the solver kernel.
 template <typename R>
-pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
-                        pict_solve_info *infos, hipStream_t stream) {
+pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
+                          RhsState *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream) {
     const int n = A->n;
     // PICT_CG_TEXTBOOK=1 selects the two-barrier textbook CG kernel (diagnostics / parity checks)
     const char *tb_env = getenv("PICT_CG_TEXTBOOK");
@@ -1141,7 +1145,7 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
     const int nvec = kind == Kind::kCGTextbook ? (o->return_best_result ? 5 : 4) : 9;
     const size_t bytes_vec = sizeof(R) * vec * nvec;
     const size_t bytes_part = sizeof(double) * (size_t)2 * nwg * kPartStride;
-    const size_t bytes_perm = o->transpose_a ? sizeof(int32_t) * (size_t)A->nnz : 0;
+    const size_t bytes_perm = (o->transpose_a && !A->transpose_perm) ? sizeof(int32_t) * (size_t)A->nnz : 0;
     const size_t off_part = (bytes_vec + 255) & ~size_t(255);
     const size_t off_ctl = off_part + ((bytes_part + 255) & ~size_t(255));
     const size_t off_st = off_ctl + ((sizeof(Ctl) + 255) & ~size_t(255));
@@ -1189,7 +1193,9 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
     a.tol = o->tolerance;
     a.nwg = (unsigned)nwg;
     PICT_HIP_TRY(hipMemsetAsync(ws + off_ctl, 0, sizeof(Ctl), stream));
-    if (o->transpose_a) {
+    if (o->transpose_a && A->transpose_perm) {
+        a.perm = A->transpose_perm;
+    } else if (o->transpose_a) {
         int32_t *perm = (int32_t *)(ws + off_perm);
         int *missing = &a.ctl->zero_all;
         hipLaunchKernelGGL(k_transpose_perm, dim3((n + 255) / 256), dim3(256), 0, stream, A->row, A->index, n, perm,
@@ -1205,36 +1211,95 @@ pict_status solve_typed(const pict_csr_desc *A, const void *rhs, void *x, int nr
         a.perm = perm;
     }
     void *args[] = {&a};
-    if (g_timing) {
-        if (!g_ev[0]) {
-            PICT_HIP_TRY(hipEventCreate(&g_ev[0]));
-            PICT_HIP_TRY(hipEventCreate(&g_ev[1]));
-        }
-        PICT_HIP_TRY(hipEventRecord(g_ev[0], stream));
-    }
+    if (ev0) PICT_HIP_TRY(hipEventRecord(ev0, stream));
     PICT_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3((unsigned)nwg), dim3(kSolveThreads), args, 0, stream));
-    if (g_timing) PICT_HIP_TRY(hipEventRecord(g_ev[1], stream));
-    std::vector<RhsState> hs(nrhs);
-    Ctl hctl;
-    PICT_HIP_TRY(hipMemcpyAsync(hs.data(), a.st, sizeof(RhsState) * nrhs, hipMemcpyDeviceToHost, stream));
-    PICT_HIP_TRY(hipMemcpyAsync(&hctl, a.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    if (ev1) PICT_HIP_TRY(hipEventRecord(ev1, stream));
+    PICT_HIP_TRY(hipMemcpyAsync(state_out, a.st, sizeof(RhsState) * nrhs, hipMemcpyDeviceToDevice, stream));
+    PICT_HIP_TRY(hipMemcpyAsync(err_out, &a.ctl->err, sizeof(int), hipMemcpyDeviceToDevice, stream));
+    return PICT_OK;
+}
+
+pict_status validate_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *opts) {
+    PICT_CHECK_ARG(A && opts, "NULL argument");
+    PICT_CHECK_ARG(A->n > 0 && A->row && A->index && A->value, "invalid matrix");
+    PICT_CHECK_ARG(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs must be 1..%d (got %d)", kMaxRhs, nrhs);
+    PICT_CHECK_ARG(opts->max_iterations >= 0, "max_iterations must be >= 0");
+    PICT_CHECK_ARG(opts->convergence_criterion >= 0 && opts->convergence_criterion <= 4, "unknown criterion");
+    PICT_CHECK_ARG(A->dtype == PICT_DTYPE_F32 || A->dtype == PICT_DTYPE_F64, "unknown dtype %d", A->dtype);
+    if (opts->matrix_rank_deficient) {
+        set_error("matrixRankDeficient is not implemented by libpict_hip");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    if (opts->use_bicg && opts->bicg_with_preconditioner) {
+        set_error("ILU(0)-preconditioned BiCGStab is not implemented by libpict_hip");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    return PICT_OK;
+}
+
+}  // namespace
+
+namespace pict {
+
+size_t solve_state_bytes() { return sizeof(RhsState); }
+
+pict_status solve_enqueue(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
+                          void *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream) {
+    pict_status st = validate_solve(A, nrhs, o);
+    if (st) return st;
+    if (A->dtype == PICT_DTYPE_F32)
+        return enqueue_typed<float>(A, rhs, x, nrhs, o, (RhsState *)state_out, err_out, ev0, ev1, stream);
+    return enqueue_typed<double>(A, rhs, x, nrhs, o, (RhsState *)state_out, err_out, ev0, ev1, stream);
+}
+
+void solve_state_to_info(const void *states, int nrhs, pict_solve_info *out) {
+    const RhsState *hs = (const RhsState *)states;
+    for (int k = 0; k < nrhs; ++k) {
+        out[k].final_residual = hs[k].final_res;
+        out[k].used_iterations = hs[k].used;
+        out[k].converged = hs[k].converged;
+        out[k].is_finite_residual = hs[k].finite;
+        out[k].reserved = 0;
+    }
+}
+
+}  // namespace pict
+
+namespace {
+
+// Synchronous solve: queue it, wait, report the states as pict_solve_info.
+pict_status solve_sync(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
+                       pict_solve_info *infos, hipStream_t stream) {
+    struct Out {
+        RhsState st[kMaxRhs];
+        int err;
+    };
+    Out *dev = nullptr;
+    PICT_HIP_TRY(hipMallocAsync((void **)&dev, sizeof(Out), stream));
+    struct Guard {
+        Out *p; hipStream_t s;
+        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+    } guard{dev, stream};
+    if (g_timing && !g_ev[0]) {
+        PICT_HIP_TRY(hipEventCreate(&g_ev[0]));
+        PICT_HIP_TRY(hipEventCreate(&g_ev[1]));
+    }
+    pict_status st = pict::solve_enqueue(A, rhs, x, nrhs, o, dev->st, &dev->err, g_timing ? g_ev[0] : nullptr,
+                                         g_timing ? g_ev[1] : nullptr, stream);
+    if (st) return st;
+    Out host;
+    PICT_HIP_TRY(hipMemcpyAsync(&host, dev, sizeof(Out), hipMemcpyDeviceToHost, stream));
     PICT_HIP_TRY(hipStreamSynchronize(stream));
     if (g_timing) {
         float ms = 0.f;
         PICT_HIP_TRY(hipEventElapsedTime(&ms, g_ev[0], g_ev[1]));
         g_last_ms = ms;
     }
-    if (hctl.err) {
+    if (host.err) {
         set_error("solver grid barrier exceeded its spin limit");
         return PICT_ERR_DEVICE_SYNC;
     }
-    for (int k = 0; k < nrhs; ++k) {
-        infos[k].final_residual = hs[k].final_res;
-        infos[k].used_iterations = hs[k].used;
-        infos[k].converged = hs[k].converged;
-        infos[k].is_finite_residual = hs[k].finite;
-        infos[k].reserved = 0;
-    }
+    pict::solve_state_to_info(host.st, nrhs, infos);
     return PICT_OK;
 }
 
@@ -1245,24 +1310,9 @@ extern "C" {
 pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, int32_t nrhs,
                               const pict_solve_opts *opts, pict_solve_info *infos_out, void *stream) {
     PICT_CHECK_ARG(A && rhs && x && opts && infos_out, "NULL argument");
-    PICT_CHECK_ARG(A->n > 0 && A->row && A->index && A->value, "invalid matrix");
-    PICT_CHECK_ARG(nrhs >= 1 && nrhs <= kMaxRhs, "nrhs must be 1..%d (got %d)", kMaxRhs, nrhs);
-    PICT_CHECK_ARG(opts->max_iterations >= 0, "max_iterations must be >= 0");
-    PICT_CHECK_ARG(opts->convergence_criterion >= 0 && opts->convergence_criterion <= 4, "unknown criterion");
-    if (opts->matrix_rank_deficient) {
-        set_error("matrixRankDeficient is not implemented by libpict_hip");
-        return PICT_ERR_UNSUPPORTED;
-    }
-    if (opts->use_bicg && opts->bicg_with_preconditioner) {
-        set_error("ILU(0)-preconditioned BiCGStab is not implemented by libpict_hip");
-        return PICT_ERR_UNSUPPORTED;
-    }
-    if (A->dtype == PICT_DTYPE_F32)
-        return solve_typed<float>(A, rhs, x, nrhs, opts, infos_out, (hipStream_t)stream);
-    if (A->dtype == PICT_DTYPE_F64)
-        return solve_typed<double>(A, rhs, x, nrhs, opts, infos_out, (hipStream_t)stream);
-    set_error("unknown dtype %d", A->dtype);
-    return PICT_ERR_INVALID;
+    pict_status st = validate_solve(A, nrhs, opts);
+    if (st) return st;
+    return solve_sync(A, rhs, x, nrhs, opts, infos_out, (hipStream_t)stream);
 }
 
 pict_status pict_set_kernel_timing(int32_t enable) {
@@ -1270,6 +1320,31 @@ pict_status pict_set_kernel_timing(int32_t enable) {
     return PICT_OK;
 }
 double pict_last_solve_kernel_ms(void) { return g_last_ms; }
+
+pict_status pict_csr_transpose_permutation(const pict_csr_desc *P, int32_t *perm_out, void *stream) {
+    PICT_CHECK_ARG(P && P->row && P->index && perm_out, "pattern or output missing");
+    PICT_CHECK_ARG(P->n >= 0 && P->nnz >= 0, "invalid pattern size");
+    if (P->n == 0) return PICT_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int *missing = nullptr;
+    PICT_HIP_TRY(hipMallocAsync((void **)&missing, sizeof(int), s));
+    struct Guard {
+        int *p; hipStream_t s;
+        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+    } guard{missing, s};
+    PICT_HIP_TRY(hipMemsetAsync(missing, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_transpose_perm, dim3((P->n + 255) / 256), dim3(256), 0, s, P->row, P->index, P->n, perm_out,
+                       missing);
+    PICT_HIP_TRY(hipGetLastError());
+    int h_missing = 0;
+    PICT_HIP_TRY(hipMemcpyAsync(&h_missing, missing, sizeof(int), hipMemcpyDeviceToHost, s));
+    PICT_HIP_TRY(hipStreamSynchronize(s));
+    if (h_missing) {
+        set_error("transposed solve requires a structurally symmetric CSR pattern");
+        return PICT_ERR_UNSUPPORTED;
+    }
+    return PICT_OK;
+}
 
 pict_status pict_sparse_outer_product(const void *a, const void *b, const pict_csr_desc *pat, void *out,
                                       void *stream) {

@@ -37,6 +37,7 @@ def test_struct_layout_matches_header():
         "pict_csr_desc": _abi.CsrDesc,
         "pict_solve_opts": _abi.SolveOpts,
         "pict_solve_info": _abi.SolveInfo,
+        "pict_step_opts": _abi.StepOpts,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for cname, py in structs.items():
