@@ -327,15 +327,38 @@ typedef struct pict_step_opts {
     pict_solve_opts pressure;     /* each pressure solve                                */
 } pict_step_opts;
 
+/* Intermediates of one step for its reverse pass (device buffers of the domain dtype, K = corrector_steps). */
+typedef struct pict_step_tape {
+    void *C_value;          /* [nnz]     advection matrix values                        */
+    void *A;                /* [N]       its diagonal                                   */
+    void *P_value;          /* [nnz]     pressure matrix values                         */
+    void *velocity_star;    /* [D*N]     advection solve result                         */
+    void *pressure_rhs;     /* [K][D*N]  pressure RHS of every corrector                */
+    void *pressure_raw;     /* [K][N]    pressure solve results before the mean subtraction */
+    void *velocity_result;  /* [K][D*N]  velocity result after every correction         */
+} pict_step_tape;
+
 /* One PISO step, orthogonal branch of the reference driver without hooks or passive
    scalars (PISOtorch_simulation.py:1054-1327): the stages and solves of
    PISOtorch_simulation._PISO_split_step queued back to back on `stream`, with one host
    wait at the end.  Block velocities are read and overwritten; block pressures and the
    domain's C, A, P, RHS and result vectors are written.  infos_out receives num_dims
    advection entries followed by one entry per corrector.  No counterpart in the
-   reference, whose driver calls each stage from Python. */
-pict_status pict_piso_step(const pict_atlas *atlas, const pict_step_opts *opts, pict_solve_info *infos_out,
-                           void *stream);
+   reference, whose driver calls each stage from Python.  tape (may be NULL) receives the
+   intermediates pict_piso_step_vjp needs. */
+pict_status pict_piso_step(const pict_atlas *atlas, const pict_step_opts *opts, const pict_step_tape *tape,
+                           pict_solve_info *infos_out, void *stream);
+/* Reverse pass of one pict_piso_step recorded on `tape` (no counterpart in the reference,
+   whose backward is PISOtorch_diff's autograd graph, PISOtorch_diff.py:295-1234; this is
+   the same sequence of stage VJPs, adjoint solves on A^T and -lambda x^T products).
+   velocity_grad [D*N, global layout]: in: gradient w.r.t. the step's output block velocities;
+   out: gradient w.r.t. its input block velocities.  Velocity-source, viscosity and fixed-
+   boundary velocity gradients are accumulated (+=) into the descriptor's gradient tensors.
+   Every other gradient tensor of the descriptor (C, P, A, the RHS and result vectors, block
+   velocity and pressure) must be present; they, the domain's C, A, P, RHS and result vectors and
+   the block pressures are overwritten.  infos_out: num_dims + corrector_steps adjoint solves. */
+pict_status pict_piso_step_vjp(const pict_atlas *atlas, const pict_step_opts *opts, const pict_step_tape *tape,
+                               void *velocity_grad, pict_solve_info *infos_out, void *stream);
 /* v -= mean(v) in place, the mean accumulated in FP64 in a fixed order: the pressure
    normalisation of the driver (PISOtorch_simulation.py:1243-1244, a torch expression there). */
 pict_status pict_subtract_mean(void *v, int32_t n, int32_t dtype, void *stream);

@@ -475,55 +475,74 @@ class Simulation:
                                      PISOtorch.ConvergenceCriterion.NORM2_NORMALIZED, use_BiCG, False, 0, False,
                                      False, return_best, False, zero_rhs_shortcut=True)
 
-    def _native_steps(self, iterations, time_step):
+    def _native_opts(self, time_step):
         from . import _abi
-        domain = self.domain
-        D = domain.getSpatialDims()
-        K = self.corrector_steps
         o = _abi.StepOpts()
         o.time_step = float(ntonp(time_step)[0])
-        o.corrector_steps = K
+        o.corrector_steps = self.corrector_steps
         o.velocity_corrector = self._velocity_corrector_version
         o.time_step_norm = int(bool(self.pressure_time_step_normalized))
         o.normalize_pressure = int(bool(self.normalize_pressure_result))
         o.advection = self._native_solve_opts(self.advection_use_BiCG, self.advection_tol, False)
         o.pressure = self._native_solve_opts(self.pressure_use_BiCG, self.pressure_tol,
                                              self.pressure_return_best_result)
-        lib = _abi.load_library()
+        return o
+
+    def _native_record(self, infos, o, kms=None, adjoint=False):
+        """Solver bookkeeping of one native step or reverse pass: solve log, recorded infos, LinsolveError."""
+        domain = self.domain
+        D, K = domain.getSpatialDims(), o.corrector_steps
+        res = [PISOtorch.LinearSolverResultInfo(i.final_residual, i.used_iterations, i.converged,
+                                                i.is_finite_residual) for i in infos]
+        adv, prs = res[:D], res[D:]
+        akind = "BiCG" if self.advection_use_BiCG else "CG"
+        pkind = "BiCG" if self.pressure_use_BiCG else "CG"
         log = PISOtorch.solve_log
         if log is not None:
-            _abi.check(lib.pict_set_step_timing(1))
-        n_csr = domain.C.getSize()
-        for _ in range(iterations):
-            domain.UpdateDomainData()
-            infos = (_abi.SolveInfo * (D + K))()
-            _abi.check(lib.pict_piso_step(domain._atlas, ctypes.byref(o), infos,
-                                          PISOtorch._stream(domain.getDevice())), "PISO step")
-            res = [PISOtorch.LinearSolverResultInfo(i.final_residual, i.used_iterations, i.converged,
-                                                    i.is_finite_residual) for i in infos]
-            adv, prs = res[:D], res[D:]
-            if log is not None:
-                ms = (ctypes.c_double * (1 + K))()
-                lib.pict_last_step_solve_ms(ms, 1 + K)
-                es = domain.velocityRHS.element_size()
-                log.append(("BiCG" if self.advection_use_BiCG else "CG", domain.getTotalSize(), n_csr, D, es, adv,
-                            ms[0], False))
-                for k in range(K):
-                    log.append(("BiCG" if self.pressure_use_BiCG else "CG", domain.getTotalSize(), n_csr, 1, es,
-                                [prs[k]], ms[1 + k], False))
-            if self.record_solver_infos:
-                self.solver_infos.append(("BiCG" if self.advection_use_BiCG else "CG", adv))
-                for k in range(K):
-                    self.solver_infos.append(("BiCG" if self.pressure_use_BiCG else "CG", [prs[k]]))
-            PISOtorch_diff._check_solver_return_infos(adv, False, self.advection_use_BiCG, o.advection.tolerance,
-                                                      o.advection.max_iterations, False, is_FWD=True)
+            es = domain.velocityRHS.element_size()
+            n, nnz = domain.getTotalSize(), domain.C.getSize()
+            log.append((akind, n, nnz, D, es, adv, kms[0] if kms else 0.0, adjoint))
             for k in range(K):
-                PISOtorch_diff._check_solver_return_infos([prs[k]], False, self.pressure_use_BiCG,
-                                                          o.pressure.tolerance, o.pressure.max_iterations,
-                                                          self.pressure_return_best_result, is_FWD=True)
-            self.end_step(time_step)
-        if log is not None:
-            _abi.check(lib.pict_set_step_timing(0))
+                log.append((pkind, n, nnz, 1, es, [prs[k]], kms[1 + k] if kms else 0.0, adjoint))
+        if self.record_solver_infos:
+            self.solver_infos.append((akind, adv))
+            for k in range(K):
+                self.solver_infos.append((pkind, [prs[k]]))
+        PISOtorch_diff._check_solver_return_infos(adv, adjoint, self.advection_use_BiCG, o.advection.tolerance,
+                                                  o.advection.max_iterations, False, is_FWD=not adjoint)
+        for k in range(K):
+            PISOtorch_diff._check_solver_return_infos([prs[k]], adjoint, self.pressure_use_BiCG, o.pressure.tolerance,
+                                                      o.pressure.max_iterations, self.pressure_return_best_result,
+                                                      is_FWD=not adjoint)
+
+    def _native_steps(self, iterations, time_step, tape=None):
+        """`iterations` native steps; `tape` (a pict_amd._abi.StepTape) records the last one for its reverse
+        pass."""
+        from . import _abi
+        domain = self.domain
+        D, K = domain.getSpatialDims(), self.corrector_steps
+        o = self._native_opts(time_step)
+        lib = _abi.load_library()
+        timing = PISOtorch.solve_log is not None
+        if timing:
+            _abi.check(lib.pict_set_step_timing(1))
+        try:
+            for _ in range(iterations):
+                domain.UpdateDomainData()
+                infos = (_abi.SolveInfo * (D + K))()
+                _abi.check(lib.pict_piso_step(domain._atlas, ctypes.byref(o),
+                                              ctypes.byref(tape) if tape is not None else None, infos,
+                                              PISOtorch._stream(domain.getDevice())), "PISO step")
+                kms = None
+                if timing:
+                    ms = (ctypes.c_double * (1 + K))()
+                    lib.pict_last_step_solve_ms(ms, 1 + K)
+                    kms = list(ms)
+                self._native_record(infos, o, kms)
+                self.end_step(time_step)
+        finally:
+            if timing:
+                _abi.check(lib.pict_set_step_timing(0))
         return True
 
     def run(self, iterations, static=False, log_domain=True):

@@ -149,6 +149,18 @@ class StepOpts(Structure):
     ]
 
 
+class StepTape(Structure):
+    _fields_ = [
+        ("C_value", c_void_p),
+        ("A", c_void_p),
+        ("P_value", c_void_p),
+        ("velocity_star", c_void_p),
+        ("pressure_rhs", c_void_p),
+        ("pressure_raw", c_void_p),
+        ("velocity_result", c_void_p),
+    ]
+
+
 # exported symbol -> (restype, argtypes)
 _P = c_void_p
 SIGNATURES = {
@@ -183,7 +195,8 @@ SIGNATURES = {
     "pict_solve_linear": (c_int32, [POINTER(CsrDesc), _P, _P, c_int32, POINTER(SolveOpts), POINTER(SolveInfo), _P]),
     "pict_sparse_outer_product": (c_int32, [_P, _P, POINTER(CsrDesc), _P, _P]),
     "pict_csr_transpose_permutation": (c_int32, [POINTER(CsrDesc), _P, _P]),
-    "pict_piso_step": (c_int32, [_P, POINTER(StepOpts), POINTER(SolveInfo), _P]),
+    "pict_piso_step": (c_int32, [_P, POINTER(StepOpts), _P, POINTER(SolveInfo), _P]),
+    "pict_piso_step_vjp": (c_int32, [_P, POINTER(StepOpts), POINTER(StepTape), _P, POINTER(SolveInfo), _P]),
     "pict_subtract_mean": (c_int32, [_P, c_int32, c_int32, _P]),
     "pict_set_step_timing": (c_int32, [c_int32]),
     "pict_last_step_solve_ms": (c_int32, [POINTER(c_double), c_int32]),

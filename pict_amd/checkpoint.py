@@ -1,19 +1,169 @@
 """Per-step checkpointing of differentiable PISO rollouts.
 
 A rollout of K steps through the PISOtorch_diff backend keeps every stage's saved tensors alive until backward
-(matrices, right-hand sides, solver results of all K steps).  `checkpointed_step` instead runs each step's forward
-with the plain PISOtorch backend (no autograd bookkeeping), keeps only the step's input velocities, and re-runs that
-one step through PISOtorch_diff when its gradient is needed, so that memory is one step's graph regardless of K.
-The recomputed step is bit-identical to the forward one: every kernel, including the Krylov reductions, is
-deterministic.
+(matrices, right-hand sides, solver results of all K steps).  `checkpointed_step` instead keeps only each step's
+input velocities and recomputes the step when its gradient is needed, so that memory is one step's intermediates
+regardless of K.
+
+Two implementations of the same contract:
+- native: when the step is what pict_piso_step runs (orthogonal, no hooks, ...), the forward is one library call;
+  the backward re-runs it recording its intermediates on a tape (pict_step_tape) and runs the reverse pass as one
+  more library call (pict_piso_step_vjp).  With keep_tape=True the tape of the forward is kept instead (no
+  recomputation; memory of one tape per step).
+- general: the forward runs the plain PISOtorch backend, the backward re-runs the step through PISOtorch_diff and
+  lets autograd differentiate it.
+Every kernel is deterministic, so a recomputed step is bit-identical to its forward.
 """
+import ctypes
+
 import torch
+
+from . import PISOtorch, _abi
+
+
+# ----------------------------------------------------------------------------
+# parameters a checkpointed step may differentiate: where they live on the domain and where their gradient goes
+# ----------------------------------------------------------------------------
+
+def _param_slots(domain):
+    """(getter, setter, gradient getter) of every tensor a parameter may be: block velocity sources and
+    viscosities, fixed-boundary velocities and the domain viscosity."""
+    for b in domain.getBlocks():
+        yield (lambda b=b: b.velocitySource, b.setVelocitySource, lambda b=b: b.velocitySourceGrad)
+        yield (lambda b=b: b.viscosity, b.setViscosity, lambda b=b: b.viscosityGrad)
+        for _, fb in b.getFixedBoundaries():
+            yield (lambda fb=fb: fb.velocity, fb.setVelocity, lambda fb=fb: fb.velocityGrad)
+    yield (lambda: domain.viscosity, domain.setViscosity, lambda: domain.viscosityGrad)
+
+
+def _same(a, b):
+    return a is b or (a is not None and b is not None and a.data_ptr() == b.data_ptr() and a.shape == b.shape
+                      and a.device == b.device)
+
+
+def _swap_params(domain, params, leaves):
+    swaps = []
+    for get, set_, _ in _param_slots(domain):
+        cur = get()
+        for p, leaf in zip(params, leaves):
+            if _same(cur, p):
+                set_(leaf)
+                swaps.append((set_, p))
+    return swaps
+
+
+def _restore_params(domain, swaps):
+    for set_, p in swaps:
+        set_(p)
+    domain.UpdateDomainData()
+
+
+def _param_grads(domain, params):
+    """The accumulated gradient of each parameter tensor (None where it is not a domain parameter)."""
+    out = []
+    for p in params:
+        g = None
+        for get, _, grad in _param_slots(domain):
+            if _same(get(), p):
+                gt = grad()
+                if gt is not None:
+                    g = gt.detach().clone().to(p.device)
+                break
+        out.append(g)
+    return out
+
+
+# ----------------------------------------------------------------------------
+# native tape and reverse pass
+# ----------------------------------------------------------------------------
+
+class _Tape:
+    """Device buffers of one pict_step_tape."""
+
+    def __init__(self, domain, K):
+        n, nnz, D = domain.getTotalSize(), domain.C.getSize(), domain.getSpatialDims()
+        opts = dict(dtype=domain.getDtype(), device=domain.getDevice())
+        self.tensors = [torch.empty(nnz, **opts), torch.empty(n, **opts), torch.empty(nnz, **opts),
+                        torch.empty(D * n, **opts), torch.empty(max(K, 1) * D * n, **opts),
+                        torch.empty(max(K, 1) * n, **opts), torch.empty(max(K, 1) * D * n, **opts)]
+        self.desc = _abi.StepTape(*[t.data_ptr() for t in self.tensors])
+
+
+def _ensure_grad_tensors(domain):
+    """Gradient tensors the reverse pass writes; parameter gradients (accumulated by the kernels) start at 0."""
+    domain.CreateAGrad()
+    domain.CGrad.CreateValue()
+    domain.PGrad.CreateValue()
+    domain.CreateVelocityRHSGrad()
+    domain.CreateVelocityResultGrad()
+    domain.CreatePressureRHSGrad()
+    domain.CreatePressureRHSdivGrad()
+    domain.CreatePressureResultGrad()
+    domain.CreateVelocityGradOnBlocks()
+    domain.CreatePressureGradOnBlocks()
+    domain.CreateVelocitySourceGradOnBlocks()
+    domain.CreateViscosityGrad()
+    domain.CreateVelocityGradOnBoundaries()
+
+
+def _to_global(tensors, D):
+    """Per-block [1,D,...] tensors -> global component-major vector."""
+    return torch.cat([torch.cat([t[0, c].reshape(-1) for t in tensors]) for c in range(D)]).contiguous()
+
+
+def _from_global(v, like, D):
+    out = [torch.empty_like(t) for t in like]
+    n = sum(t[0, 0].numel() for t in like)
+    for c in range(D):
+        off = c * n
+        for o in out:
+            k = o[0, c].numel()
+            o[0, c].copy_(v[off:off + k].view_as(o[0, c]))
+            off += k
+    return out
+
+
+def _native_forward(sim, tape):
+    """One native step (recording `tape`)."""
+    return sim._native_steps(1, sim._get_time_step_torch(), tape=tape.desc if tape is not None else None)
+
+
+def _native_backward(sim, tape, vel_grads):
+    """Reverse pass of the step recorded on `tape`: gradient w.r.t. the input block velocities; parameter
+    gradients are left on the domain's gradient tensors."""
+    domain = sim.domain
+    D = domain.getSpatialDims()
+    _ensure_grad_tensors(domain)
+    domain.UpdateDomainData()
+    g = _to_global(vel_grads, D)
+    o = sim._native_opts(sim._get_time_step_torch())
+    K = o.corrector_steps
+    infos = (_abi.SolveInfo * (D + K))()
+    lib = _abi.load_library()
+    timing = PISOtorch.solve_log is not None
+    if timing:
+        _abi.check(lib.pict_set_step_timing(1))
+    try:
+        _abi.check(lib.pict_piso_step_vjp(domain._atlas, ctypes.byref(o), ctypes.byref(tape.desc),
+                                          ctypes.c_void_p(g.data_ptr()), infos,
+                                          PISOtorch._stream(domain.getDevice())), "PISO step reverse pass")
+        kms = None
+        if timing:
+            ms = (ctypes.c_double * (1 + K))()
+            lib.pict_last_step_solve_ms(ms, 1 + K)
+            kms = list(ms)
+    finally:
+        if timing:
+            _abi.check(lib.pict_set_step_timing(0))
+    sim._native_record(infos, o, kms, adjoint=True)
+    return g
 
 
 class _PISOStep(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, sim, n_blocks, *tensors):
+    def forward(ctx, sim, n_blocks, keep_tape, *tensors):
         """tensors = block velocities (n_blocks) followed by the parameter tensors already set on the domain."""
+        ctx.set_materialize_grads(False)
         ctx.sim = sim
         ctx.n_blocks = n_blocks
         ctx.save_for_backward(*tensors)
@@ -23,15 +173,21 @@ class _PISOStep(torch.autograd.Function):
         with torch.no_grad():
             for b, v in zip(domain.getBlocks(), tensors[:n_blocks]):
                 b.setVelocity(v.detach().clone())
-                # the plain backend writes results into the block tensors in place: never into a tensor that an
-                # earlier step returned
+                # the forward writes results into the block tensors in place: never into a tensor that an earlier
+                # step returned
                 b.setPressure(b.pressure.detach().clone())
             domain.UpdateDomainData()
             # the differentiable recomputation starts every solve from zero; so does this forward
             sim.differentiable = False
             sim.advect_reuse_result = sim.pressure_reuse_result = False
             try:
-                ok = sim._PISO_split_step(1)
+                ctx.native = sim._native_step_eligible()
+                ctx.tape = None
+                if ctx.native:
+                    ctx.tape = _Tape(domain, sim.corrector_steps) if keep_tape else None
+                    ok = _native_forward(sim, ctx.tape)
+                else:
+                    ok = sim._PISO_split_step(1)
             finally:
                 sim.differentiable = was_diff
                 sim.advect_reuse_result, sim.pressure_reuse_result = reuse
@@ -43,6 +199,53 @@ class _PISOStep(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, *grads):
+        n = ctx.n_blocks
+        pressure_grads = [g for g in grads[n:] if g is not None and bool((g != 0).any())]
+        if ctx.native and not pressure_grads:
+            return (None, None, None, *_PISOStep._backward_native(ctx, grads[:n]))
+        return (None, None, None, *_PISOStep._backward_autograd(ctx, grads))
+
+    @staticmethod
+    def _backward_native(ctx, vel_grads):
+        sim, n = ctx.sim, ctx.n_blocks
+        domain = sim.domain
+        saved = ctx.saved_tensors
+        params = list(saved[n:])
+        vel_grads = [torch.zeros_like(v) if g is None else g.contiguous() for g, v in zip(vel_grads, saved[:n])]
+        state = [(b.velocity, b.pressure) for b in domain.getBlocks()]
+        counters = (sim.total_step, sim.total_time)
+        tape = ctx.tape
+        try:
+            with torch.no_grad():
+                # the reverse pass overwrites the block pressures: let it write into scratch tensors
+                for b, (v, p) in zip(domain.getBlocks(), state):
+                    b.setPressure(torch.empty_like(p))
+                if tape is None:
+                    # recompute the step from its input, recording the tape
+                    tape = _Tape(domain, sim.corrector_steps)
+                    for b, v in zip(domain.getBlocks(), saved[:n]):
+                        b.setVelocity(v.detach().clone())
+                    domain.UpdateDomainData()
+                    reuse = (sim.advect_reuse_result, sim.pressure_reuse_result)
+                    sim.advect_reuse_result = sim.pressure_reuse_result = False
+                    try:
+                        _native_forward(sim, tape)
+                    finally:
+                        sim.advect_reuse_result, sim.pressure_reuse_result = reuse
+                g = _native_backward(sim, tape, vel_grads)
+                g_vel = _from_global(g, list(saved[:n]), domain.getSpatialDims())
+                g_par = [gp if p.requires_grad else None for gp, p in zip(_param_grads(domain, params), params)]
+        finally:
+            sim.total_step, sim.total_time = counters
+            for b, (v, p) in zip(domain.getBlocks(), state):
+                b.setVelocity(v)
+                b.setPressure(p)
+            domain.UpdateDomainData()
+        ctx.tape = None
+        return (*g_vel, *g_par)
+
+    @staticmethod
+    def _backward_autograd(ctx, grads):
         sim, n = ctx.sim, ctx.n_blocks
         saved = ctx.saved_tensors
         domain = sim.domain
@@ -79,45 +282,18 @@ class _PISOStep(torch.autograd.Function):
         it = iter(got[n:])
         for x in leaves:
             g_par.append(next(it) if x.requires_grad else None)
-        return (None, None, *g_vel, *g_par)
+        return (*g_vel, *g_par)
 
 
-def _param_slots(domain):
-    """Where a parameter tensor may live on the domain: block velocity sources and viscosities, the domain
-    viscosity and fixed-boundary velocities."""
-    for b in domain.getBlocks():
-        yield (lambda b=b: b.velocitySource, b.setVelocitySource)
-        yield (lambda b=b: b.viscosity, b.setViscosity)
-        for _, fb in b.getFixedBoundaries():
-            yield (lambda fb=fb: fb.velocity, fb.setVelocity)
-    yield (lambda: domain.viscosity, domain.setViscosity)
-
-
-def _swap_params(domain, params, leaves):
-    swaps = []
-    for get, set_ in _param_slots(domain):
-        cur = get()
-        for p, leaf in zip(params, leaves):
-            if cur is not None and cur is p:
-                set_(leaf)
-                swaps.append((set_, p))
-    return swaps
-
-
-def _restore_params(domain, swaps):
-    for set_, p in swaps:
-        set_(p)
-    domain.UpdateDomainData()
-
-
-def checkpointed_step(sim, params=()):
-    """One PISO step of `sim` whose autograd graph is recomputed in backward.  `params` are tensors already set on
-    the domain (e.g. a learnable block.velocitySource) whose gradients are wanted.  The block velocities on entry
-    are the step's differentiable input; on return the blocks hold the step's output velocities and pressures,
-    which carry the gradient path."""
+def checkpointed_step(sim, params=(), keep_tape=False):
+    """One PISO step of `sim` whose gradient is computed by recomputing the step in backward (keep_tape=True on
+    the native path: by keeping its tape).  `params` are tensors already set on the domain (e.g. a learnable
+    block.velocitySource) whose gradients are wanted.  The block velocities on entry are the step's
+    differentiable input; on return the blocks hold the step's output velocities and pressures, which carry the
+    gradient path."""
     domain = sim.domain
     blocks = domain.getBlocks()
-    outs = _PISOStep.apply(sim, len(blocks), *[b.velocity for b in blocks], *params)
+    outs = _PISOStep.apply(sim, len(blocks), keep_tape, *[b.velocity for b in blocks], *params)
     n = len(blocks)
     for b, v, p in zip(blocks, outs[:n], outs[n:]):
         b.setVelocity(v)
@@ -126,12 +302,12 @@ def checkpointed_step(sim, params=()):
     return outs[:n]
 
 
-def rollout(sim, steps, params=(), checkpoint=True):
+def rollout(sim, steps, params=(), checkpoint=True, keep_tape=False):
     """`steps` differentiable PISO steps; with checkpoint=False the whole rollout graph is kept (PISOtorch_diff)."""
     if checkpoint:
         out = None
         for _ in range(steps):
-            out = checkpointed_step(sim, params)
+            out = checkpointed_step(sim, params, keep_tape)
         return out
     was_diff = sim.differentiable
     sim.differentiable = True

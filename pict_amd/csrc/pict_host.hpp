@@ -92,6 +92,9 @@ struct pict_atlas {
     void *pinned[kRing] = {};
     hipEvent_t ev[kRing] = {};
     int ring_pos = 0;
+    // A^T map of the CSR pattern (built on first use by a reverse pass, for the pattern it was built from)
+    mutable int32_t *tperm = nullptr;
+    mutable const int32_t *tperm_row = nullptr, *tperm_index = nullptr;
 
     template <typename R>
     pict::DomainK<R> view() const {

@@ -390,6 +390,7 @@ static void atlas_free(pict_atlas *a) {
         if (a->pinned[i]) (void)hipHostFree(a->pinned[i]);
     }
     if (a->dev) (void)hipFree(a->dev);
+    if (a->tperm) (void)hipFree(a->tperm);
     delete a;
 }
 

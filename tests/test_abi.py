@@ -38,6 +38,7 @@ def test_struct_layout_matches_header():
         "pict_solve_opts": _abi.SolveOpts,
         "pict_solve_info": _abi.SolveInfo,
         "pict_step_opts": _abi.StepOpts,
+        "pict_step_tape": _abi.StepTape,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for cname, py in structs.items():

@@ -102,31 +102,42 @@ def test_rollout_gradients_match_finite_differences(name):
     assert checks and all(rel <= 1e-4 for _, rel in checks), checks
 
 
-def test_checkpointed_rollout_matches_full_graph():
-    """Per-step checkpointing (forward on the plain backend, recomputation in backward) gives the same gradients as
-    keeping the whole rollout graph, and leaves the domain at the rollout's final state."""
+@pytest.mark.parametrize("name", ["periodic_two_block", "rotated_connection", "box3d", "cavity_static_fields"])
+@pytest.mark.parametrize("keep_tape", [False, True])
+def test_checkpointed_rollout_matches_full_graph(name, keep_tape):
+    """Per-step checkpointing -- on these orthogonal steps the native path: forward as one library call, reverse
+    pass (pict_piso_step_vjp) from a recomputed or kept tape -- gives the gradients of the whole PISOtorch_diff
+    autograd graph, w.r.t. the input velocities, the velocity sources and the domain viscosity, and leaves the
+    domain at the rollout's final state."""
     from pict_amd import checkpoint as CK
+    print("case", name, keep_tape, flush=True)
     steps = 3
     grads = {}
     finals = {}
     for ck in (False, True):
-        d = _setup("periodic_two_block")
+        d = _setup(name)
         v0, s0 = _state(d)
         g = torch.Generator().manual_seed(5)
         target = [((torch.rand(v.shape, generator=g, dtype=torch.float64) * 2 - 1) * 0.3).cuda() for v in v0]
         srcs = [s.clone().requires_grad_(True) for s in s0]
         vin = [v.clone().requires_grad_(True) for v in v0]
+        visc = d.viscosity.clone().requires_grad_(True)
         for b, s, v in zip(d.blocks, srcs, vin):
             b.setVelocitySource(s)
             b.setVelocity(v)
+        d.setViscosity(visc)
         d.UpdateDomainData()
         sim = _sim(d, False)
-        out = CK.rollout(sim, steps, params=srcs, checkpoint=ck)
+        if ck:
+            assert sim._native_step_eligible()
+        out = CK.rollout(sim, steps, params=srcs + [visc], checkpoint=ck, keep_tape=keep_tape)
         L = sum(((o - t) ** 2).sum() for o, t in zip(out, target))
         L.backward()
-        grads[ck] = [x.grad.detach().clone() for x in srcs + vin]
+        grads[ck] = [x.grad.detach().clone() for x in srcs + vin] + \
+            ([visc.grad.detach().clone()] if visc.grad is not None else [])
         finals[ck] = [b.velocity.detach().clone() for b in d.blocks]
         assert sim.total_step == steps
+    assert len(grads[True]) == len(grads[False])
     for a, b in zip(grads[True], grads[False]):
         err = float((a - b).abs().max() / b.abs().max())
         print("  checkpointed vs full graph: %.3e" % err, flush=True)
