@@ -71,14 +71,14 @@ def target_state(dom, args, torch):
     return [(torch.rand(b.velocity.shape, generator=g) * 0.2).cuda() for b in dom.blocks]
 
 
-def rollout_fwdbwd(dom, sim, steps, target, torch):
+def rollout_fwdbwd(dom, sim, steps, target, torch, keep_tape=False):
     from pict_amd import checkpoint as CK
     u0 = [b.velocity.detach().clone() for b in dom.blocks]
     srcs = [torch.zeros_like(b.velocity).requires_grad_(True) for b in dom.blocks]
     for b, s in zip(dom.blocks, srcs):
         b.setVelocitySource(s)
     dom.UpdateDomainData()
-    out = CK.rollout(sim, steps, params=srcs, checkpoint=True)
+    out = CK.rollout(sim, steps, params=srcs, checkpoint=True, keep_tape=keep_tape)
     loss = sum(((o - t) ** 2).mean() for o, t in zip(out, target))
     loss.backward()
     # back to the initial state for the next rollout
@@ -158,6 +158,19 @@ def main():
     ms_per_step = ms_total / args.steps
     value = N * args.steps / (ms_total * 1e-3)
 
+    extra = {}
+    if args.mode == "fwdbwd":
+        # the same rollout keeping each step's tape (no recomputation), and the forward alone, for context
+        for label, fn in (("keep_tape", lambda: rollout_fwdbwd(dom, sim, args.steps, target, torch, True)),
+                          ("forward_only", lambda: sim._PISO_split_step(args.steps))):
+            torch.cuda.synchronize()
+            f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            f0.record()
+            fn()
+            f1.record()
+            torch.cuda.synchronize()
+            extra[label + "_cells_steps_per_s"] = N * args.steps / (f0.elapsed_time(f1) * 1e-3)
+
     if args.copy_check > 0:
         src = torch.ones(args.copy_check * (1 << 18), dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -188,7 +201,7 @@ def main():
                    "solver_adjoint_launches": {k: v["adjoint_launches"] for k, v in agg.items()},
                    "solver_iterations_per_step": {k: v["iters"] / args.steps for k, v in agg.items()},
                    "solver_ms_per_step": {k: v["ms"] / args.steps for k, v in agg.items()},
-                   "wall_s": wall},
+                   "wall_s": wall, **extra},
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},

@@ -269,67 +269,115 @@ __device__ __forceinline__ void acc_res(double (&acc)[NR][NS], int k, R rv) {
 // --------------------------------------------------------------------------
 constexpr int kPartStride = kMaxRhs * kSlots;
 
+// Wave64 reductions in DPP (quad and row rotations within 16-lane rows, then the
+// four row totals in a fixed order): every lane of the wave gets the same total,
+// and the result depends only on the inputs, so every workgroup reducing the same
+// values gets bitwise the same result.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_d(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce_dpp(double v) {
+    auto op = [](double x, double y) { return MAX ? fmax(x, y) : x + y; };
+    v = op(v, dpp_d<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = op(v, dpp_d<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = op(v, dpp_d<0x124>(v));  // row_ror:4
+    v = op(v, dpp_d<0x128>(v));  // row_ror:8
+    return op(op(lane_d(v, 0), lane_d(v, 16)), op(lane_d(v, 32), lane_d(v, 48)));
+}
+
 template <int NR, int NS>
 __device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg, unsigned gen,
                                             double (&acc)[NR][NS], const int (&ids)[NS],
                                             const bool (&ismax)[NS], double *red, int &s_err) {
-    constexpr int nrhs = NR;
+    __shared__ double wsum[kSolveThreads / 64][NR * NS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // 1. wave totals, then the workgroup total (thread 0, fixed order)
 #pragma unroll
-    for (int k = 0; k < NR; ++k) block_reduce<NS>(acc[k], ismax);
+    for (int k = 0; k < NR; ++k)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const double w = ismax[s] ? wave_reduce_dpp<true>(acc[k][s]) : wave_reduce_dpp<false>(acc[k][s]);
+            if (lane == 0) wsum[wave][k * NS + s] = w;
+        }
+    __syncthreads();
+    if (wave != 0) return;
     double *base = part + (size_t)(gen & 1) * nwg * kPartStride;
-    if (threadIdx.x == 0) {
+    // 2. publish this workgroup's partials and arrive
+    if (lane == 0) {
         double *mine = base + (size_t)blockIdx.x * kPartStride;
-        for (int k = 0; k < nrhs; ++k)
 #pragma unroll
-            for (int s = 0; s < NS; ++s) mine[k * kSlots + ids[s]] = acc[k][s];
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                double t = wsum[0][k * NS + s];
+#pragma unroll
+                for (int w = 1; w < kSolveThreads / 64; ++w)
+                    t = ismax[s] ? fmax(t, wsum[w][k * NS + s]) : t + wsum[w][k * NS + s];
+                mine[k * kSlots + ids[s]] = t;
+            }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         const unsigned prev = __hip_atomic_fetch_add(&ctl->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int err = 0;
         if (prev == nwg - 1) {
             __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
             __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
-                    __hip_atomic_store(&ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    err = 1;
-                    break;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        s_err = err;
     }
-    __syncthreads();
-    for (int k = 0; k < nrhs; ++k) {
-        double v[NS];
+    // 3. the whole wave waits for the generation to move (relaxed loads; one acquire after)
+    unsigned spins = 0;
+    int err = 0;
+    while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSpinLimit) {
+            if (lane == 0) __hip_atomic_store(&ctl->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            err = 1;
+            break;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // 4. every workgroup's partials, reduced in the same fixed order by every workgroup
+    double v[NR][NS];
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[k][s] = 0.0;
+    for (unsigned w = lane; w < nwg; w += 64) {
+        const double *pw = base + (size_t)w * kPartStride;
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const double pv = pw[k * kSlots + ids[s]];
+                v[k][s] = ismax[s] ? fmax(v[k][s], pv) : v[k][s] + pv;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            double sum = 0.0;
-            for (unsigned w = threadIdx.x; w < nwg; w += kSolveThreads) {
-                const double pv = base[(size_t)w * kPartStride + k * kSlots + ids[s]];
-                sum = ismax[s] ? fmax(sum, pv) : sum + pv;
-            }
-            v[s] = sum;
+            const double t = ismax[s] ? wave_reduce_dpp<true>(v[k][s]) : wave_reduce_dpp<false>(v[k][s]);
+            if (lane == 0) red[k * kSlots + ids[s]] = t;
         }
-        block_reduce<NS>(v, ismax);
-        if (threadIdx.x == 0)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) red[k * kSlots + ids[s]] = v[s];
-    }
-    __syncthreads();
+    if (lane == 0) s_err = err;
 }
 
+// Reduce-barrier: flat_reduce leaves the totals and the error flag in LDS (written by
+// thread 0, which then runs the scalar update); one workgroup barrier publishes both.
 #define PICT_FLAT_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                            \
     do {                                                                                          \
         flat_reduce<NRHS, NS>(a.partials, a.ctl, a.nwg, gen, ACC, IDS, ISMAX, red, s_err);       \
         ++gen;                                                                                    \
-        if (s_err) return;                                                                        \
-        if (threadIdx.x == 0) { UPDATE; }                                                         \
+        if (threadIdx.x == 0 && !s_err) { UPDATE; }                                               \
         __syncthreads();                                                                          \
+        if (s_err) return;                                                                        \
     } while (0)
 
 // Matrix rows of one thread kept in registers across iterations: RPT rows,
@@ -755,7 +803,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                     for (int e = rs; e < re; ++e) f(a.col[e], mat_val(a, e));
                 });
             };
-            if (cached) {
+            // register budget: the gathered form for up to 8 bytes of right-hand-side values per row
+            constexpr bool kGather = NRHS * sizeof(R) <= 8;
+            if (cached && !kGather) {
 #pragma unroll
                 for (int q = 0; q < kRowsCached; ++q) {
                     const size_t i = i0 + (size_t)q * stride;
@@ -767,6 +817,67 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                         });
                     } else if (rc.len[q] == -2) {
                         mem_row(i);
+                    }
+                }
+            } else if (cached) {
+                // gather every load of this thread's cached rows before any store (one memory round trip),
+                // then update and store
+                R gr[kRowsCached][NRHS][kMaxNz], gw[kRowsCached][NRHS][kMaxNz], gs[kRowsCached][NRHS][kMaxNz];
+                R op[kRowsCached][NRHS], ox[kRowsCached][NRHS], ori[kRowsCached][NRHS], owi[kRowsCached][NRHS],
+                    osi[kRowsCached][NRHS];
+#pragma unroll
+                for (int q = 0; q < kRowsCached; ++q) {
+                    const size_t i = i0 + (size_t)q * stride;
+                    if (rc.len[q] < 0) continue;
+#pragma unroll
+                    for (int k = 0; k < NRHS; ++k) {
+                        if (!act[k]) continue;
+                        const size_t o = (size_t)k * n;
+                        op[q][k] = p[o + i];
+                        ox[q][k] = a.x[o + i];
+                        ori[q][k] = r[o + i];
+                        owi[q][k] = w[o + i];
+                        osi[q][k] = so[o + i];
+#pragma unroll
+                        for (int e = 0; e < kMaxNz; ++e) {
+                            if (e < rc.len[q]) {
+                                const size_t c = o + rc.col[q][e];
+                                gr[q][k][e] = r[c];
+                                gw[q][k][e] = w[c];
+                                gs[q][k][e] = so[c];
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kRowsCached; ++q) {
+                    const size_t i = i0 + (size_t)q * stride;
+                    if (rc.len[q] == -2) {
+                        mem_row(i);
+                        continue;
+                    }
+                    if (rc.len[q] < 0) continue;
+#pragma unroll
+                    for (int k = 0; k < NRHS; ++k) {
+                        if (!act[k]) continue;
+                        const size_t o = (size_t)k * n;
+                        if (save[k]) a.best[o + i] = ox[q][k];
+                        const R pm = fma(beta[k], op[q][k], ori[q][k]);
+                        const R sm = fma(beta[k], osi[q][k], owi[q][k]);
+                        p[o + i] = pm;
+                        sn[o + i] = sm;
+                        R xv = ox[q][k];
+                        xv += alpha[k] * pm;
+                        a.x[o + i] = xv;
+                        const R rm = fma(-alpha[k], sm, ori[q][k]);
+                        rn[o + i] = rm;
+                        R wv = 0;
+#pragma unroll
+                        for (int e = 0; e < kMaxNz; ++e)
+                            if (e < rc.len[q]) wv += rc.val[q][e] * fma(-alpha[k], fma(beta[k], gs[q][k][e], gw[q][k][e]), gr[q][k][e]);
+                        wn[o + i] = wv;
+                        acc_res(acc, k, rm);
+                        acc[k][3] += (double)wv * (double)rm;
                     }
                 }
             } else {
