@@ -18,6 +18,51 @@ __device__ __forceinline__ void add_to(R *p, R v) {
     unsafeAtomicAdd(p, v);
 }
 
+// Per-thread partial sums of the gradient contributions whose target is the same for the
+// whole workgroup (a workgroup covers cells of one block): the domain viscosity, the block's
+// static viscosity and its static velocity source.  Summed over the workgroup at the end of
+// the kernel and added with one atomic each, instead of one atomic per cell and face on a
+// single address.
+template <typename R>
+struct UniformAcc {
+    double visc_dom = 0.0, visc_own = 0.0, src[3] = {0.0, 0.0, 0.0};
+};
+
+// workgroup sum of v (all threads call it; result valid in thread 0)
+__device__ __forceinline__ double wg_sum(double v) {
+    __shared__ double lds[kCellsPerTile / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int w = 0; w < kCellsPerTile / 64; ++w) t += lds[w];
+    return t;
+}
+
+// Adds the workgroup's uniform-target sums (every thread of the workgroup must call it).
+template <typename R, int D>
+__device__ __forceinline__ void flush_uniform(const DomainK<R> &dom, const pict_block_desc *b, const UniformAcc<R> &u,
+                                              bool want_src) {
+    const double vd = wg_sum(u.visc_dom);
+    const double vo = wg_sum(u.visc_own);
+    double sv[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) sv[d] = want_src ? wg_sum(u.src[d]) : 0.0;
+    if (threadIdx.x == 0) {
+        if (dom.visc_grad && vd != 0.0) add_to(dom.visc_grad, (R)vd);
+        if (b->viscosity && b->viscosity_static && b->viscosity_grad && vo != 0.0)
+            add_to(reinterpret_cast<R *>(b->viscosity_grad), (R)vo);
+        if (want_src && b->velocity_source_grad && b->velocity_source_static)
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+                if (sv[d] != 0.0) add_to(reinterpret_cast<R *>(b->velocity_source_grad) + d, (R)sv[d]);
+    }
+}
+
 // Position of each row slot (diag, -x, +x, ...) in the cell's sorted CSR row
 // (LoadCSRrowNeighborSorted :3503-3563): rank of the slot's column among the
 // row's columns.  -1 for prescribed faces.
@@ -40,13 +85,20 @@ __device__ __forceinline__ void slot_positions(const DomainK<R> &dom, const pict
     }
 }
 
-// scatterViscosityBlock_GRAD :1852-1873 (+ scatterViscosity_GRAD :1834-1849)
+// scatterViscosityBlock_GRAD :1852-1873 (+ scatterViscosity_GRAD :1834-1849); `own` is the
+// workgroup's block: its uniform targets go to the accumulator u.
 template <typename R>
-__device__ __forceinline__ void scatter_viscosity(const DomainK<R> &dom, const pict_block_desc *b, I4 p, R g) {
+__device__ __forceinline__ void scatter_viscosity(const DomainK<R> &dom, const pict_block_desc *b, I4 p, R g,
+                                                  const pict_block_desc *own, UniformAcc<R> &u) {
     R *dst;
     if (!b->viscosity) {
-        dst = dom.visc_grad;
+        u.visc_dom += (double)g;
+        return;
     } else if (b->viscosity_static) {
+        if (b == own) {
+            u.visc_own += (double)g;
+            return;
+        }
         dst = reinterpret_cast<R *>(b->viscosity_grad);
     } else {
         p.a[3] = 0;
@@ -55,12 +107,16 @@ __device__ __forceinline__ void scatter_viscosity(const DomainK<R> &dom, const p
     if (dst) add_to(dst, g);
 }
 
-// scatterBlockVelocitySource_GRAD :2255-2264 (p.w = component)
+// scatterBlockVelocitySource_GRAD :2255-2264 (p.w = component); a static source of the
+// workgroup's own block goes to the accumulator u.
 template <typename R>
-__device__ __forceinline__ void scatter_velocity_source(const pict_block_desc *b, const I4 &p, R g) {
+__device__ __forceinline__ void scatter_velocity_source(const pict_block_desc *b, const I4 &p, R g, UniformAcc<R> &u) {
     if (!b->velocity_source_grad) return;
-    R *dst = reinterpret_cast<R *>(b->velocity_source_grad);
-    add_to(b->velocity_source_static ? dst + p.a[3] : dst + flat4(p, b->stride), g);
+    if (b->velocity_source_static) {
+        u.src[p.a[3]] += (double)g;
+        return;
+    }
+    add_to(reinterpret_cast<R *>(b->velocity_source_grad) + flat4(p, b->stride), g);
 }
 
 // scatterFixedBoundaryData (VELOCITY_GRAD) :2451-2476, p on the boundary grid, w = component
@@ -162,7 +218,7 @@ __device__ __forceinline__ void scatter_fluxes(const DomainK<R> &dom, const pict
 // G = gradient of the source term of component d (already divided by det).
 template <typename R, int D>
 __device__ __forceinline__ void fixed_boundary_source_grad(const DomainK<R> &dom, const pict_block_desc *b,
-                                                           const I4 &pos, int d, R G) {
+                                                           const I4 &pos, int d, R G, UniformAcc<R> &u) {
 #pragma unroll
     for (int face = 0; face < 2 * D; ++face) {
         if (!at_bound(pos, face, b)) continue;
@@ -185,7 +241,7 @@ __device__ __forceinline__ void fixed_boundary_source_grad(const DomainK<R> &dom
         scatter_fixed_contravariant<R, D>(fb, t, flux_grad * normal);
         t.a[3] = d;
         scatter_fixed_velocity<R>(fb, t, vel_grad);
-        scatter_viscosity<R>(dom, b, pos, visc_grad);
+        scatter_viscosity<R>(dom, b, pos, visc_grad, b, u);
     }
 }
 
@@ -194,11 +250,8 @@ __device__ __forceinline__ void fixed_boundary_source_grad(const DomainK<R> &dom
 // block velocity grad, viscosity grads.
 // ---------------------------------------------------------------------------
 template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_grad(DomainK<R> dom, R dt) {
-    const pict_block_desc *b;
-    int bi, f;
-    if (!tile_cell(dom, b, bi, f)) return;
-    (void)dt;
+__device__ __forceinline__ void adv_matrix_grad_cell(const DomainK<R> &dom, const pict_block_desc *b, int bi, int f,
+                                                     UniformAcc<R> &u) {
     const I4 pos = unflat(f, b);
     const int g = b->global_offset + f;
     const R rdet = R(1) / cell_det<R, D>(b, pos);
@@ -230,7 +283,7 @@ __global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_grad(DomainK<R> do
             const R vc_grad = diag_grad - cg[face + 1];
             visc_grad += R(0.5) * alpha_p[dim] * vc_grad;
             t.a[3] = 0;
-            scatter_viscosity<R>(dom, nbp, t, R(0.5) * alpha * vc_grad);
+            scatter_viscosity<R>(dom, nbp, t, R(0.5) * alpha * vc_grad, b, u);
         } else {
             fg[face] = 0;
             const R slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? R(0) : R(1);
@@ -238,7 +291,17 @@ __global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_grad(DomainK<R> do
         }
     }
     if (b->velocity_grad) scatter_fluxes<R, D>(dom, b, pos, fg, nullptr);
-    scatter_viscosity<R>(dom, b, pos, visc_grad);
+    scatter_viscosity<R>(dom, b, pos, visc_grad, b, u);
+}
+
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_grad(DomainK<R> dom, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    (void)dt;
+    UniformAcc<R> u;
+    if (tile_cell(dom, b, bi, f)) adv_matrix_grad_cell<R, D>(dom, b, bi, f, u);
+    flush_uniform<R, D>(dom, b, u, false);
 }
 
 // kPISO_build_advection_RHS_GRAD :4388-4476: velocity_rhs_grad -> velocity grad (=),
@@ -247,21 +310,24 @@ template <typename R, int D>
 __global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs_grad(DomainK<R> dom, R dt) {
     const pict_block_desc *b;
     int bi, f;
-    if (!tile_cell(dom, b, bi, f)) return;
-    const I4 pos = unflat(f, b);
-    const int g = b->global_offset + f;
-    const R det = cell_det<R, D>(b, pos);
+    UniformAcc<R> u;
+    if (tile_cell(dom, b, bi, f)) {
+        const I4 pos = unflat(f, b);
+        const int g = b->global_offset + f;
+        const R det = cell_det<R, D>(b, pos);
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        I4 t = pos;
-        t.a[3] = d;
-        R G = dom.vrhs_grad[(size_t)d * dom.N + g];
-        scatter_velocity_source<R>(b, t, G);
-        G /= det;
-        fixed_boundary_source_grad<R, D>(dom, b, pos, d, G);
-        G *= det;
-        if (b->velocity_grad) reinterpret_cast<R *>(b->velocity_grad)[flat4(t, b->stride)] = G / dt;
+        for (int d = 0; d < D; ++d) {
+            I4 t = pos;
+            t.a[3] = d;
+            R G = dom.vrhs_grad[(size_t)d * dom.N + g];
+            scatter_velocity_source<R>(b, t, G, u);
+            G /= det;
+            fixed_boundary_source_grad<R, D>(dom, b, pos, d, G, u);
+            G *= det;
+            if (b->velocity_grad) reinterpret_cast<R *>(b->velocity_grad)[flat4(t, b->stride)] = G / dt;
+        }
     }
+    flush_uniform<R, D>(dom, b, u, true);
 }
 
 // PISO_build_pressure_matrix_GRAD :4967-5115 (orthogonal): P_grad -> A_grad (+=).
@@ -321,10 +387,8 @@ __global__ __launch_bounds__(kCellsPerTile) void k_div_grad(DomainK<R> dom, R dt
 // velocity_result grad (+=), C_grad (= over the row), A_grad (+=), source and
 // boundary grads.
 template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_p_rhs_grad(DomainK<R> dom, R dt) {
-    const pict_block_desc *b;
-    int bi, f;
-    if (!tile_cell(dom, b, bi, f)) return;
+__device__ __forceinline__ void p_rhs_grad_cell(const DomainK<R> &dom, const pict_block_desc *b, int f, R dt,
+                                                UniformAcc<R> &u) {
     const I4 pos = unflat(f, b);
     const int g = b->global_offset + f;
     const R rDiag = R(1) / dom.A[g];
@@ -354,13 +418,22 @@ __global__ __launch_bounds__(kCellsPerTile) void k_p_rhs_grad(DomainK<R> dom, R 
             cvg[k] -= dom.vres[col[k] + o] * w;
         }
         R S_grad = w;
-        scatter_velocity_source<R>(b, t, S_grad);
+        scatter_velocity_source<R>(b, t, S_grad, u);
         S_grad /= det;
-        fixed_boundary_source_grad<R, D>(dom, b, pos, d, S_grad);
+        fixed_boundary_source_grad<R, D>(dom, b, pos, d, S_grad, u);
     }
     if (dom.C_grad)
         for (int k = 0; k < rs; ++k) dom.C_grad[cs + k] = cvg[k];
     if (dom.A_grad) dom.A_grad[g] += rDiag_grad * (-rDiag * rDiag);
+}
+
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_rhs_grad(DomainK<R> dom, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    UniformAcc<R> u;
+    if (tile_cell(dom, b, bi, f)) p_rhs_grad_cell<R, D>(dom, b, f, dt, u);
+    flush_uniform<R, D>(dom, b, u, true);
 }
 
 // scatterPressureGradToWithBounds :3411-3477 (transpose of pressure_at)

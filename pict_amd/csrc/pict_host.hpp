@@ -15,6 +15,11 @@ namespace pict {
 
 void set_error(const char *fmt, ...);
 
+// Stream-ordered workspace memory from a library-owned pool of the current device that keeps
+// what is freed, so repeated calls reuse it instead of allocating device memory each time.
+hipError_t ws_alloc(void **p, size_t bytes, hipStream_t s);
+hipError_t ws_free(void *p, hipStream_t s);
+
 #define PICT_HIP_TRY(expr)                                                                  \
     do {                                                                                    \
         hipError_t _e = (expr);                                                             \

@@ -21,6 +21,8 @@
 // error flag, after which all workgroups leave the kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -71,6 +73,9 @@ struct SolveArgs {
     int n, nrhs, maxit, conv, reset_steps, return_best, zero_shortcut;
     double tol;
     unsigned nwg;
+    unsigned long long *tagged;                    // k_cg_local: tagged partials [2][nwg][kSlots], zeroed
+    void *pub;                                     // k_cg_local: published rows r, w, s [3][2][n], zeroed
+    int salt;                                      // k_cg_local: stamp of this launch's foreign-read marks
 };
 
 // --------------------------------------------------------------------------
@@ -591,6 +596,58 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg(SolveArgs<R> a) {
     }
 }
 
+// Scalar step of the single-reduction CG after the reduction of iteration `it`
+// (red = this right-hand side's totals): convergence test (cg_solver_kernel.cu:306-313),
+// best-result bookkeeping (:330-346, :399-412), then the next alpha and beta.
+template <typename R>
+__device__ __forceinline__ void cg1_update(const SolveArgs<R> &a, RhsState &st, const double *red, int it) {
+    if (!st.active) return;
+    st.save_best = 0;
+    const double gam = red[S_RR];
+    const double crit = criterion(a.conv, gam, red[S_ASUM], red[S_AMAX], a.n);
+    if (!isfinite(crit)) {
+        st.converged = 0; st.used = it; st.final_res = crit; st.finite = 0; st.active = 0;
+        return;
+    }
+    if (a.return_best) {
+        if (it == 0 || crit < st.best_crit) { st.best_crit = crit; st.best_it = it; st.save_best = 1; }
+        if (it > 0 && crit >= st.last_crit) ++st.rising; else st.rising = 0;
+        st.last_crit = crit;
+    }
+    st.used = it;
+    st.final_res = crit;
+    if (crit < a.tol) { st.converged = 1; st.finite = 1; st.active = 0; return; }
+    if (a.return_best && (it == a.maxit - 1 || st.rising >= 100)) {
+        st.restore_best = st.best_it != it;
+        st.converged = 0; st.used = st.best_it; st.final_res = st.best_crit; st.finite = 1;
+        st.active = 0;
+        return;
+    }
+    const double beta_n = gam / st.rho;
+    st.alpha = gam / (red[S_D0] - beta_n * gam / st.alpha);
+    st.beta = beta_n;
+    st.rho = gam;
+}
+
+// first reduction of a solve: rho = (r0, r0); zero-RHS shortcut flag in omega
+__device__ __forceinline__ void cg1_init(RhsState &st, double rr, int zero_all) {
+    st.rho = rr;
+    st.alpha = 0;
+    st.beta = 0;
+    st.omega = zero_all;
+    st.active = zero_all ? 0 : 1;
+    st.converged = zero_all;
+    st.finite = 1;
+    st.used = -1;
+    st.final_res = 0;
+    st.best_crit = 0;
+    st.last_crit = 0;
+    st.best_it = -1;
+    st.rising = 0;
+    st.save_best = 0;
+    st.restore_best = 0;
+}
+
 // ==========================================================================
 // CG, single-reduction form (Chronopoulos & Gear 1989): the same Krylov
 // iterates as cgSolveGPU (cg_solver_kernel.cu:114-456) in exact arithmetic,
@@ -647,24 +704,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
             bool all_zero = true;
             for (int k = 0; k < NRHS; ++k) all_zero = all_zero && red[k * kSlots + S_BMAX] == 0.0;
             const int za = (a.zero_shortcut && all_zero) ? 1 : 0;
-            for (int k = 0; k < NRHS; ++k) {
-                RhsState &st = sst[k];
-                st.rho = red[k * kSlots + S_RR];
-                st.alpha = 0;
-                st.beta = 0;
-                st.omega = za;  // zero-RHS flag travels in omega (unused by CG)
-                st.active = za ? 0 : 1;
-                st.converged = za;
-                st.finite = 1;
-                st.used = -1;
-                st.final_res = 0;
-                st.best_crit = 0;
-                st.last_crit = 0;
-                st.best_it = -1;
-                st.rising = 0;
-                st.save_best = 0;
-                st.restore_best = 0;
-            }
+            for (int k = 0; k < NRHS; ++k) cg1_init(sst[k], red[k * kSlots + S_RR], za);
         });
         zero_all = sst[0].omega != 0.0;
     }
@@ -884,35 +924,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                 for (size_t i = i0; i < (size_t)n; i += stride) mem_row(i);
             }
             PICT_FLAT_BARRIER(5, acc, IDS_G, MAX_G, {
-                for (int k = 0; k < NRHS; ++k) {
-                    RhsState &st = sst[k];
-                    if (!st.active) continue;
-                    st.save_best = 0;
-                    const double gam = red[k * kSlots + S_RR];
-                    const double crit = criterion(a.conv, gam, red[k * kSlots + S_ASUM], red[k * kSlots + S_AMAX], n);
-                    if (!isfinite(crit)) {
-                        st.converged = 0; st.used = it; st.final_res = crit; st.finite = 0; st.active = 0;
-                        continue;
-                    }
-                    if (a.return_best) {
-                        if (it == 0 || crit < st.best_crit) { st.best_crit = crit; st.best_it = it; st.save_best = 1; }
-                        if (it > 0 && crit >= st.last_crit) ++st.rising; else st.rising = 0;
-                        st.last_crit = crit;
-                    }
-                    st.used = it;
-                    st.final_res = crit;
-                    if (crit < a.tol) { st.converged = 1; st.finite = 1; st.active = 0; continue; }
-                    if (a.return_best && (it == a.maxit - 1 || st.rising >= 100)) {
-                        st.restore_best = st.best_it != it;
-                        st.converged = 0; st.used = st.best_it; st.final_res = st.best_crit; st.finite = 1;
-                        st.active = 0;
-                        continue;
-                    }
-                    const double beta_n = gam / st.rho;
-                    st.alpha = gam / (red[k * kSlots + S_D0] - beta_n * gam / st.alpha);
-                    st.beta = beta_n;
-                    st.rho = gam;
-                }
+                for (int k = 0; k < NRHS; ++k) cg1_update(a, sst[k], red + k * kSlots, it);
             });
         }
         cur = nxt;
@@ -928,6 +940,485 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         for (int k = 0; k < NRHS; ++k) a.st[k] = sst[k];
+}
+
+// ==========================================================================
+// Register-resident single-reduction CG for one right-hand side.
+//
+// The recurrences of k_cg1 with another placement of the data: workgroup b owns
+// the contiguous rows [b*C, min(n, (b+1)*C)), C = 256*RPT, and thread t the rows
+// lo + t + q*256 (q < RPT).  Those rows' x, p, r, w, s and matrix entries (up to
+// NZ per row; longer rows are read from memory each time) stay in registers for
+// the whole solve.  Each iteration r' of the workgroup's own rows goes through
+// LDS; for a column owned by another workgroup, r' is formed from the owner's
+// published r, w, s of the previous iteration with the owner's fma sequence
+// (bitwise equal).  Only rows that another workgroup reads are published (found
+// in the first pass: every workgroup stamps the foreign columns it holds), double
+// buffered by iteration parity.
+//
+// Whatever one workgroup writes for another is stored and loaded with
+// agent-scope atomic operations, coherent across the XCDs' L2 caches by
+// themselves, so no L2 writeback or invalidate is needed; and every such word
+// carries the generation it belongs to, so a reader waits on the data itself
+// instead of on a separate flag that would have to be ordered after it:
+//  - FP64 partial sums carry (generation + 1) mod 4 in the low 2 bits of the
+//    mantissa, which readers clear (every workgroup reduces the same truncated
+//    values in the same order, so all get bitwise identical totals; truncation
+//    changes a partial by < 2^-50 relative, and an exact zero stays zero);
+//  - FP32 published rows are 8-byte words (value, iteration + 1).  FP64 rows
+//    cannot be packed with a tag in one atomic word: they are plain values and
+//    a workgroup waits for its row stores to complete before it stores its
+//    partials.
+// Both regions are zeroed before the launch (tag 0 is never awaited).
+// ==========================================================================
+template <typename T>
+__device__ __forceinline__ void st_coh(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_coh(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until every memory operation this thread issued has completed
+__device__ __forceinline__ void ops_done() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+typedef unsigned long long u64;
+// A slot is rewritten every second generation, so a stale slot is exactly two generations
+// old: (generation + 1) mod 4 tells them apart, and the zeroed start (tag 0) is not awaited
+// before every slot has been written once.
+__device__ __forceinline__ u64 tag_partial(double v, unsigned gen) {
+    return (__double_as_longlong(v) & ~3ull) | (u64)((gen + 1u) & 3u);
+}
+__device__ __forceinline__ bool partial_ready(u64 u, unsigned gen) { return (unsigned)(u & 3u) == ((gen + 1u) & 3u); }
+__device__ __forceinline__ double partial_value(u64 u) { return __longlong_as_double((long long)(u & ~3ull)); }
+
+// Published row values of one vector: [2][n] (parity), FP32 as (value, tag) words.
+template <typename R>
+struct Pub;
+template <>
+struct Pub<float> {
+    u64 *base;
+    int n;
+    __device__ __forceinline__ void put(int par, int i, float v, unsigned tag) const {
+        st_coh(base + (size_t)par * n + i, ((u64)tag << 32) | (u64)__float_as_uint(v));
+    }
+    // value of row c (parity par) once it carries `tag`; NaN after a bounded wait
+    __device__ __forceinline__ float get(int par, int c, unsigned tag, int &err) const {
+        if (err) return __int_as_float(0x7fc00000);
+        const u64 *p = base + (size_t)par * n + c;
+        u64 u = ld_coh(p);
+        unsigned spins = 0;
+        while ((unsigned)(u >> 32) != tag) {
+            if (++spins > kSpinLimit) {
+                err = 1;
+                return __int_as_float(0x7fc00000);
+            }
+            u = ld_coh(p);
+        }
+        return __uint_as_float((unsigned)u);
+    }
+    // raw word of row c (one load, no wait), then its value and whether it carries `tag`
+    using Word = u64;
+    __device__ __forceinline__ u64 word(int par, int c) const { return ld_coh(base + (size_t)par * n + c); }
+    static __device__ __forceinline__ float open(u64 u, unsigned tag, bool &ok) {
+        ok = ok && (unsigned)(u >> 32) == tag;
+        return __uint_as_float((unsigned)u);
+    }
+    static constexpr bool kTagged = true;
+};
+template <>
+struct Pub<double> {
+    double *base;
+    int n;
+    __device__ __forceinline__ void put(int par, int i, double v, unsigned) const {
+        st_coh(base + (size_t)par * n + i, v);
+    }
+    __device__ __forceinline__ double get(int par, int c, unsigned, int &) const {
+        return ld_coh(base + (size_t)par * n + c);
+    }
+    using Word = double;
+    __device__ __forceinline__ double word(int par, int c) const { return ld_coh(base + (size_t)par * n + c); }
+    static __device__ __forceinline__ double open(double u, unsigned, bool &) { return u; }
+    static constexpr bool kTagged = false;
+};
+
+// Reduce-barrier on tagged partials: the workgroup's totals of acc go to its line of
+// the generation's set (tagged); wave 0 then reads all nwg lines until every slot
+// carries the generation's tag and reduces them in a fixed order (red[ids[s]], thread
+// 0's view).  The two sets alternate by generation parity: a set is written again
+// only after every workgroup has passed the next generation.  `wait_rows`: the
+// workgroup's row stores must complete before its partials go out (untagged rows).
+template <int NS>
+__device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned nwg, unsigned gen,
+                                              const double (&acc)[NS], const int (&ids)[NS], const bool (&ismax)[NS],
+                                              bool wait_rows, double *red, int &s_err) {
+    __shared__ double wsum[kSolveThreads / 64][NS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wait_rows) ops_done();
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const double w = ismax[s] ? wave_reduce_dpp<true>(acc[s]) : wave_reduce_dpp<false>(acc[s]);
+        if (lane == 0) wsum[wave][s] = w;
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    u64 *base = part + (size_t)(gen & 1u) * nwg * kSlots;
+    if (lane == 0) {
+        u64 *mine = base + (size_t)blockIdx.x * kSlots;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double t = wsum[0][s];
+#pragma unroll
+            for (int w = 1; w < kSolveThreads / 64; ++w) t = ismax[s] ? fmax(t, wsum[w][s]) : t + wsum[w][s];
+            st_coh(mine + ids[s], tag_partial(t, gen));
+        }
+    }
+    unsigned spins = 0;
+    int err = 0;
+    double v[NS];
+    for (;;) {
+        bool ready = true;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = 0.0;
+        for (unsigned w = lane; w < nwg; w += 64) {
+            const u64 *pw = base + (size_t)w * kSlots;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const u64 u = ld_coh(pw + ids[s]);
+                ready = ready && partial_ready(u, gen);
+                const double pv = partial_value(u);
+                v[s] = ismax[s] ? fmax(v[s], pv) : v[s] + pv;
+            }
+        }
+        if (__all(ready)) break;
+        if (++spins > kSpinLimit) {
+            err = 1;
+            break;
+        }
+    }
+    if (err && lane == 0) st_coh(err_flag, 1);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const double t = ismax[s] ? wave_reduce_dpp<true>(v[s]) : wave_reduce_dpp<false>(v[s]);
+        if (lane == 0) red[ids[s]] = t;
+    }
+    if (lane == 0) s_err = err;
+}
+
+#define PICT_TAG_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                                          \
+    do {                                                                                                       \
+        tagged_reduce<NS>(a.tagged, &a.ctl->err, a.nwg, gen, ACC, IDS, ISMAX, !Pub<R>::kTagged, red, s_err);   \
+        ++gen;                                                                                                 \
+        if (threadIdx.x == 0 && !s_err) { UPDATE; }                                                            \
+        __syncthreads();                                                                                       \
+        if (s_err) return;                                                                                     \
+    } while (0)
+
+// Reduction slots of k_cg_local: with a norm-2 criterion only (r,r) and (w,r) (and max|b| in
+// the first pass) are needed; the sum and the maximum of |r| only for the absolute criteria.
+template <bool FULL>
+struct CgSlots;
+template <>
+struct CgSlots<true> {
+    static constexpr int NS = 4;
+    static constexpr int R0[4] = {S_RR, S_ASUM, S_AMAX, S_BMAX};
+    static constexpr bool M0[4] = {false, false, true, true};
+    static constexpr int G[4] = {S_RR, S_ASUM, S_AMAX, S_D0};
+    static constexpr bool MG[4] = {false, false, true, false};
+};
+template <>
+struct CgSlots<false> {
+    static constexpr int NS = 2;
+    static constexpr int R0[2] = {S_RR, S_BMAX};
+    static constexpr bool M0[2] = {false, true};
+    static constexpr int G[2] = {S_RR, S_D0};
+    static constexpr bool MG[2] = {false, false};
+};
+// accumulate the residual terms of one row (slot order of CgSlots)
+template <bool FULL, int NS>
+__device__ __forceinline__ void cg_acc(double (&acc)[NS], double rv, double last, bool last_is_max) {
+    acc[0] += rv * rv;
+    if constexpr (FULL) {
+        acc[1] += fabs(rv);
+        acc[2] = fmax(acc[2], fabs(rv));
+    }
+    acc[NS - 1] = last_is_max ? fmax(acc[NS - 1], last) : acc[NS - 1] + last;
+}
+
+// Foreign entries (columns owned by another workgroup) of a thread's rows are kept in a
+// compact list of at most NH entries whose loads are issued unconditionally and back to
+// back (unused entries load word 0, one line per wave); a row whose foreign entries do not
+// fit is read from memory like a long row.  NH covers the rows of a 2-D grid whose chunk
+// edges cut grid lines (two foreign entries per thread) and the z neighbours of a 3-D grid.
+template <typename R, int RPT, int NZ, bool FULL>
+__global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
+    constexpr int C = kSolveThreads * RPT;
+    constexpr int NH = NZ <= 5 ? 2 : 2 * RPT + 2;
+    using SL = CgSlots<FULL>;
+    constexpr int NS = SL::NS;
+    __shared__ R lv[C];  // r (first product) / r' (iterations) of this workgroup's rows
+    __shared__ double red[kSlots];
+    __shared__ RhsState sst;
+    __shared__ int s_err;
+    const int n = a.n;
+    const int lo = (int)blockIdx.x * C;
+    const int cnt = min(C, n - lo);
+    unsigned gen = 0;
+    using PubT = decltype(Pub<R>::base);
+    const Pub<R> pr{reinterpret_cast<PubT>(a.pub), n}, pw{reinterpret_cast<PubT>(a.pub) + 2 * (size_t)n, n},
+        ps{reinterpret_cast<PubT>(a.pub) + 4 * (size_t)n, n};
+    int *mark = reinterpret_cast<int *>(a.p0);
+    const int salt = a.salt;
+    int herr = 0;  // a published row did not arrive in time
+    auto local = [&](int c) { return (unsigned)(c - lo) < (unsigned)cnt; };
+    if (threadIdx.x < kSlots) red[threadIdx.x] = 0.0;
+
+    // register rows: entries of local columns (foreign and padding entries: column lo, value 0)
+    int col[RPT][NZ];
+    R val[RPT][NZ];
+    bool row[RPT], mem[RPT], pub[RPT];  // row present / read from memory / read by another workgroup
+    R x[RPT], p[RPT], r[RPT], w[RPT], s[RPT], xb[RPT];
+    // compact foreign entries: row slot, column, value
+    int nh = 0, hq[NH], hc[NH];
+    R hv[NH];
+
+    // ---- rows into registers; stamp the foreign columns; r0 = b - A x0 (published, tag 1) ----
+    {
+        double acc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            hq[k] = -1;
+            hc[k] = 0;
+            hv[k] = R(0);
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int li = (int)threadIdx.x + q * kSolveThreads;
+            row[q] = li < cnt;
+            mem[q] = false;
+            pub[q] = false;
+            x[q] = p[q] = r[q] = w[q] = s[q] = xb[q] = R(0);
+#pragma unroll
+            for (int e = 0; e < NZ; ++e) {
+                col[q][e] = lo;
+                val[q][e] = R(0);
+            }
+            if (!row[q]) continue;
+            const int i = lo + li;
+            const int rs = a.row[i], re = a.row[i + 1];
+            int nf = 0;
+            for (int e = rs; e < re; ++e) nf += local(a.col[e]) ? 0 : 1;
+            mem[q] = re - rs > NZ || nh + nf > NH;
+            R sum = 0;
+            for (int e = rs; e < re; ++e) {
+                const int c = a.col[e];
+                const R v = mat_val(a, e);
+                if (!mem[q]) {
+                    if (local(c)) {
+#pragma unroll
+                        for (int k = 0; k < NZ; ++k)
+                            if (k == e - rs) {
+                                col[q][k] = c;
+                                val[q][k] = v;
+                            }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < NH; ++k)
+                            if (k == nh) {
+                                hq[k] = q;
+                                hc[k] = c;
+                                hv[k] = v;
+                            }
+                        ++nh;
+                    }
+                }
+                if (!local(c)) st_coh(mark + c, salt);
+                sum += v * a.x[c];
+            }
+            const R bv = a.b[i];
+            const R rv = bv - sum;
+            x[q] = xb[q] = a.x[i];
+            r[q] = rv;
+            lv[li] = rv;
+            pr.put(0, i, rv, 1u);
+            ps.put(0, i, R(0), 1u);
+            cg_acc<FULL, NS>(acc, (double)rv, fabs((double)bv), true);
+        }
+        // the stamps must be visible when the generation completes
+        ops_done();
+        PICT_TAG_BARRIER(NS, acc, SL::R0, SL::M0, {
+            const int za = (a.zero_shortcut && red[S_BMAX] == 0.0) ? 1 : 0;
+            cg1_init(sst, red[S_RR], za);
+        });
+    }
+    if (sst.omega != 0.0) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (row[q]) a.x[lo + (int)threadIdx.x + q * kSolveThreads] = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.st[0] = sst;
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+        if (row[q]) pub[q] = ld_coh(mark + lo + (int)threadIdx.x + q * kSolveThreads) == salt;
+
+    // one row read from memory: sum of A_ij * v(j), v(j) = lv for local columns, else `far(j)`
+    auto mem_row = [&](int i, auto far) {
+        const int rs = a.row[i], re = a.row[i + 1];
+        R acc = 0;
+        for (int e = rs; e < re; ++e) {
+            const int c = a.col[e];
+            acc += mat_val(a, e) * (local(c) ? lv[c - lo] : far(c));
+        }
+        return acc;
+    };
+
+    // ---- w = A r (published, tag 1); gamma = (r,r), delta = (w,r); alpha = gamma/delta ----
+    {
+        // all loads first (no use in between), then the tag checks
+        typename Pub<R>::Word ur[NH];
+#pragma unroll
+        for (int k = 0; k < NH; ++k) ur[k] = pr.word(0, hc[k]);
+        R fr[NH];
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < NH; ++k) fr[k] = k < nh ? Pub<R>::open(ur[k], 1u, ok) : R(0);
+        if (!ok) {
+#pragma unroll
+            for (int k = 0; k < NH; ++k)
+                if (k < nh) fr[k] = pr.get(0, hc[k], 1u, herr);
+        }
+        double acc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (!row[q]) continue;
+            const int i = lo + (int)threadIdx.x + q * kSolveThreads;
+            R wv = 0;
+            if (!mem[q]) {
+#pragma unroll
+                for (int e = 0; e < NZ; ++e) wv += val[q][e] * lv[col[q][e] - lo];
+#pragma unroll
+                for (int k = 0; k < NH; ++k)
+                    if (hq[k] == q) wv += hv[k] * fr[k];
+            } else {
+                wv = mem_row(i, [&](int c) { return pr.get(0, c, 1u, herr); });
+            }
+            w[q] = wv;
+            if (pub[q]) pw.put(0, i, wv, 1u);
+            cg_acc<FULL, NS>(acc, (double)r[q], (double)wv * (double)r[q], false);
+        }
+        PICT_TAG_BARRIER(NS, acc, SL::G, SL::MG, {
+            if (sst.active) {
+                sst.rho = red[S_RR];
+                sst.alpha = sst.rho / red[S_D0];
+                sst.beta = 0;
+            }
+        });
+    }
+
+    for (int it = 0; it < a.maxit && sst.active; ++it) {
+        const int cur = it & 1, nxt = cur ^ 1;
+        const unsigned tin = (unsigned)it + 1u, tout = (unsigned)it + 2u;
+        const R alpha = (R)sst.alpha, beta = (R)sst.beta;
+        const bool save = sst.save_best;
+        // the owners' r, w, s of the foreign entries: all loads first
+        typename Pub<R>::Word ur[NH], uw[NH], us[NH];
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            ur[k] = pr.word(cur, hc[k]);
+            uw[k] = pw.word(cur, hc[k]);
+            us[k] = ps.word(cur, hc[k]);
+        }
+        // own rows: p, s, x, r (while the loads are in flight)
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (!row[q]) continue;
+            if (save) xb[q] = x[q];
+            const R pm = fma(beta, p[q], r[q]);
+            const R sm = fma(beta, s[q], w[q]);
+            x[q] = fma(alpha, pm, x[q]);
+            const R rm = fma(-alpha, sm, r[q]);
+            p[q] = pm;
+            s[q] = sm;
+            r[q] = rm;
+            lv[(int)threadIdx.x + q * kSolveThreads] = rm;
+        }
+        R fr[NH], fw[NH], fs[NH];
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            fr[k] = fw[k] = fs[k] = R(0);
+            if (k < nh) {
+                fr[k] = Pub<R>::open(ur[k], tin, ok);
+                fw[k] = Pub<R>::open(uw[k], tin, ok);
+                fs[k] = Pub<R>::open(us[k], tin, ok);
+            }
+        }
+        if (!ok) {
+            // some owner's words had not arrived: wait for each
+#pragma unroll
+            for (int k = 0; k < NH; ++k)
+                if (k < nh) {
+                    fr[k] = pr.get(cur, hc[k], tin, herr);
+                    fw[k] = pw.get(cur, hc[k], tin, herr);
+                    fs[k] = ps.get(cur, hc[k], tin, herr);
+                }
+        }
+        // r' of the foreign entries, the owner's fma sequence
+        R fx[NH];
+#pragma unroll
+        for (int k = 0; k < NH; ++k) fx[k] = fma(-alpha, fma(beta, fs[k], fw[k]), fr[k]);
+        __syncthreads();
+        // w' = A r'
+        double acc[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            if (!row[q]) continue;
+            const int i = lo + (int)threadIdx.x + q * kSolveThreads;
+            R wv = 0;
+            if (!mem[q]) {
+#pragma unroll
+                for (int e = 0; e < NZ; ++e) wv += val[q][e] * lv[col[q][e] - lo];
+#pragma unroll
+                for (int k = 0; k < NH; ++k)
+                    if (hq[k] == q) wv += hv[k] * fx[k];
+            } else {
+                wv = mem_row(i, [&](int c) {
+                    return fma(-alpha, fma(beta, ps.get(cur, c, tin, herr), pw.get(cur, c, tin, herr)),
+                               pr.get(cur, c, tin, herr));
+                });
+            }
+            w[q] = wv;
+            if (pub[q]) {
+                pr.put(nxt, i, r[q], tout);
+                pw.put(nxt, i, wv, tout);
+                ps.put(nxt, i, s[q], tout);
+            }
+            cg_acc<FULL, NS>(acc, (double)r[q], (double)wv * (double)r[q], false);
+        }
+        PICT_TAG_BARRIER(NS, acc, SL::G, SL::MG, cg1_update(a, sst, red, it));
+    }
+    if (herr) st_coh(&a.ctl->err, 1);
+    const bool restore = sst.restore_best;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+        if (row[q]) a.x[lo + (int)threadIdx.x + q * kSolveThreads] = restore ? xb[q] : x[q];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st[0] = sst;
+}
+
+// zero n 64-bit words (the tagged regions of k_cg_local, before its launch)
+__global__ void k_zero64(u64 *p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
 // ==========================================================================
@@ -1223,6 +1714,36 @@ const void *kernel_for_nrhs(Kind kind, int nrhs) {
 // Queues one solve on `stream`.  When it has run, the final state of each right-hand side is in
 // state_out[0..nrhs) and the barrier error flag in *err_out (device memory); ev0/ev1 (optional) bracket
 // the solver kernel.
+// k_cg_local instance for rows per thread `rpt` and cached row length `nz`
+template <typename R, int NZ, bool FULL>
+const void *local_cg_kernel_nz(int rpt) {
+    switch (rpt) {
+        case 1: return (const void *)k_cg_local<R, 1, NZ, FULL>;
+        case 2: return (const void *)k_cg_local<R, 2, NZ, FULL>;
+        case 4: return (const void *)k_cg_local<R, 4, NZ, FULL>;
+        default:
+            if constexpr (sizeof(R) == 4) return (const void *)k_cg_local<R, 8, NZ, FULL>;
+            return nullptr;
+    }
+}
+// `full`: the convergence criterion needs the sum and maximum of |r|
+template <typename R>
+const void *local_cg_kernel(int rpt, int nz, bool full) {
+    if (full) return nz <= 5 ? local_cg_kernel_nz<R, 5, true>(rpt) : local_cg_kernel_nz<R, 7, true>(rpt);
+    return nz <= 5 ? local_cg_kernel_nz<R, 5, false>(rpt) : local_cg_kernel_nz<R, 7, false>(rpt);
+}
+
+// Workgroup budget of the register-resident CG: the cost of its barrier grows with the
+// number of workgroups, the per-thread work with rows per thread.
+static int local_cg_target_wgs() {
+    static const int v = [] {
+        const char *e = getenv("PICT_CG_LOCAL_WGS");
+        const int t = e ? atoi(e) : 64;
+        return t;
+    }();
+    return v;
+}
+
 template <typename R>
 pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
                           RhsState *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream) {
@@ -1239,6 +1760,30 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
         set_error("solver kernel cannot be resident (occupancy 0)");
         return PICT_ERR_HIP;
     }
+    // one right-hand side of the single-reduction CG without residual resets: the
+    // register-resident kernel when its workgroups fit on the device at once
+    long long local_wgs = 0;
+    const int target = local_cg_target_wgs();
+    if (kind == Kind::kCG1 && nrhs == 1 && o->residual_reset_steps <= 0 && target > 0) {
+        const int nz = (long long)A->nnz <= 5LL * n ? 5 : 7;
+        const int rmax = sizeof(R) == 4 ? 8 : 4;  // register budget of the FP64 rows
+        for (int rpt = 1; rpt <= rmax; rpt *= 2) {
+            const bool full = o->convergence_criterion != PICT_CONV_NORM2 &&
+                              o->convergence_criterion != PICT_CONV_NORM2_NORMALIZED;
+            const void *k = local_cg_kernel<R>(rpt, nz, full);
+            const long long per = (long long)kSolveThreads * rpt;
+            const long long wgs = ((long long)n + per - 1) / per;
+            if (wgs > target && rpt < rmax) continue;
+            int locc = 0, lcus = 0;
+            st = occupancy(k, locc, lcus);
+            if (st) return st;
+            if (wgs <= (long long)locc * lcus) {
+                kernel = k;
+                local_wgs = wgs;
+            }
+            break;
+        }
+    }
     // rows per thread: more rows per workgroup means fewer barrier participants
     static const int rows_per_thread = [] {
         const char *e = getenv("PICT_SOLVER_ROWS_PER_THREAD");
@@ -1251,6 +1796,7 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     if (nwg > cap) nwg = cap;
     if (nwg > 2048) nwg = 2048;
     if (nwg < 1) nwg = 1;
+    if (local_wgs > 0) nwg = local_wgs;
 
     const size_t vec = (size_t)n * nrhs;
     const int nvec = kind == Kind::kCGTextbook ? (o->return_best_result ? 5 : 4) : 9;
@@ -1261,13 +1807,17 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     const size_t off_ctl = off_part + ((bytes_part + 255) & ~size_t(255));
     const size_t off_st = off_ctl + ((sizeof(Ctl) + 255) & ~size_t(255));
     const size_t off_perm = off_st + ((sizeof(RhsState) * nrhs + 255) & ~size_t(255));
-    const size_t total = off_perm + bytes_perm + 256;
+    // k_cg_local: tagged partials [2][nwg][kSlots] and published rows [3][2][n], 8-byte words, zeroed
+    const size_t words_tagged = local_wgs > 0 ? (size_t)2 * nwg * kSlots : 0;
+    const size_t words_zero = local_wgs > 0 ? words_tagged + (size_t)6 * n : 0;
+    const size_t off_zero = off_perm + ((bytes_perm + 255) & ~size_t(255));
+    const size_t total = off_zero + sizeof(unsigned long long) * words_zero + 256;
 
     char *ws = nullptr;
-    PICT_HIP_TRY(hipMallocAsync((void **)&ws, total, stream));
+    PICT_HIP_TRY(ws_alloc((void **)&ws, total, stream));
     struct Guard {
         char *p; hipStream_t s;
-        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+        ~Guard() { if (p) (void)ws_free(p, s); }
     } guard{ws, stream};
 
     SolveArgs<R> a;
@@ -1303,7 +1853,21 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     a.zero_shortcut = o->zero_rhs_shortcut;
     a.tol = o->tolerance;
     a.nwg = (unsigned)nwg;
+    a.tagged = reinterpret_cast<unsigned long long *>(ws + off_zero);
+    a.pub = a.tagged + words_tagged;
+    {
+        // stamp of this launch's foreign-read marks (a stale mark equal to it only costs a store)
+        static std::atomic<unsigned> launches{0};
+        unsigned sv = ++launches;
+        if (sv == 0) sv = ++launches;
+        a.salt = (int)sv;
+    }
     PICT_HIP_TRY(hipMemsetAsync(ws + off_ctl, 0, sizeof(Ctl), stream));
+    if (words_zero) {
+        const unsigned blocks = (unsigned)std::min<size_t>((words_zero + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_zero64, dim3(blocks), dim3(256), 0, stream, a.tagged, words_zero);
+        PICT_HIP_TRY(hipGetLastError());
+    }
     if (o->transpose_a && A->transpose_perm) {
         a.perm = A->transpose_perm;
     } else if (o->transpose_a) {
@@ -1386,10 +1950,10 @@ pict_status solve_sync(const pict_csr_desc *A, const void *rhs, void *x, int nrh
         int err;
     };
     Out *dev = nullptr;
-    PICT_HIP_TRY(hipMallocAsync((void **)&dev, sizeof(Out), stream));
+    PICT_HIP_TRY(ws_alloc((void **)&dev, sizeof(Out), stream));
     struct Guard {
         Out *p; hipStream_t s;
-        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+        ~Guard() { if (p) (void)ws_free(p, s); }
     } guard{dev, stream};
     if (g_timing && !g_ev[0]) {
         PICT_HIP_TRY(hipEventCreate(&g_ev[0]));
@@ -1438,10 +2002,10 @@ pict_status pict_csr_transpose_permutation(const pict_csr_desc *P, int32_t *perm
     if (P->n == 0) return PICT_OK;
     hipStream_t s = (hipStream_t)stream;
     int *missing = nullptr;
-    PICT_HIP_TRY(hipMallocAsync((void **)&missing, sizeof(int), s));
+    PICT_HIP_TRY(ws_alloc((void **)&missing, sizeof(int), s));
     struct Guard {
         int *p; hipStream_t s;
-        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+        ~Guard() { if (p) (void)ws_free(p, s); }
     } guard{missing, s};
     PICT_HIP_TRY(hipMemsetAsync(missing, 0, sizeof(int), s));
     hipLaunchKernelGGL(k_transpose_perm, dim3((P->n + 255) / 256), dim3(256), 0, s, P->row, P->index, P->n, perm_out,

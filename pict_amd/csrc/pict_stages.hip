@@ -29,6 +29,50 @@ void set_error(const char *fmt, ...) {
     g_err = buf;
 }
 
+// One stream-ordered pool per device, owned by the library, that keeps freed memory (release
+// threshold at its maximum): the solver and step workspaces are allocated and freed on every
+// call, and a pool that trims at each synchronisation makes each of those a fresh device
+// allocation.
+static hipMemPool_t device_pool(int dev, hipError_t &err) {
+    static std::mutex mu;
+    static hipMemPool_t pools[64] = {};
+    err = hipSuccess;
+    if (dev < 0 || dev >= 64) {
+        err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!pools[dev]) {
+        hipMemPoolProps props;
+        memset(&props, 0, sizeof props);
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        err = hipMemPoolCreate(&pool, &props);
+        if (err != hipSuccess) return nullptr;
+        uint64_t keep = ~uint64_t(0);
+        err = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        if (err != hipSuccess) {
+            (void)hipMemPoolDestroy(pool);
+            return nullptr;
+        }
+        pools[dev] = pool;
+    }
+    return pools[dev];
+}
+
+hipError_t ws_alloc(void **p, size_t bytes, hipStream_t s) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    hipMemPool_t pool = device_pool(dev, e);
+    if (!pool) return e;
+    return hipMallocFromPoolAsync(p, bytes, pool, s);
+}
+
+hipError_t ws_free(void *p, hipStream_t s) { return hipFreeAsync(p, s); }
+
 // ---------------------------------------------------------------------------
 // CSR pattern: row pointer from the closed form of
 // getCSRMatrixRowEndOffsetFromBlockBoundaries3D (:247-312), columns of

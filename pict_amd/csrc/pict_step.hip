@@ -188,10 +188,10 @@ pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const p
     const size_t slot = ((4 * sb + sizeof(int)) + 255) & ~size_t(255);
     const size_t bytes = slot * nsolves + sizeof(double) * kMeanBlocks;
     char *dev = nullptr;
-    PICT_HIP_TRY(hipMallocAsync((void **)&dev, bytes, s));
+    PICT_HIP_TRY(ws_alloc((void **)&dev, bytes, s));
     struct Guard {
         char *p; hipStream_t s;
-        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+        ~Guard() { if (p) (void)ws_free(p, s); }
     } guard{dev, s};
     double *part = (double *)(dev + slot * nsolves);
     hipEvent_t *ev = nullptr;
@@ -303,10 +303,10 @@ pict_status pict_piso_step_vjp(const pict_atlas *a, const pict_step_opts *o, con
     const size_t o_part = slot * nsolves, o_gu = o_part + al(sizeof(double) * kMeanBlocks), o_at = o_gu + al(vecD),
                  o_ct = o_at + al(vec1), o_lam = o_ct + al(nzb), bytes = o_lam + al(vecD);
     char *dev = nullptr;
-    PICT_HIP_TRY(hipMallocAsync((void **)&dev, bytes, s));
+    PICT_HIP_TRY(ws_alloc((void **)&dev, bytes, s));
     struct Guard {
         char *p; hipStream_t s;
-        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+        ~Guard() { if (p) (void)ws_free(p, s); }
     } guard{dev, s};
     double *part = (double *)(dev + o_part);
     void *g_u = dev + o_gu, *A_tot = dev + o_at, *C_tot = dev + o_ct, *lam = dev + o_lam;
@@ -446,10 +446,10 @@ pict_status pict_subtract_mean(void *v, int32_t n, int32_t dtype, void *stream) 
     PICT_CHECK_ARG(dtype == PICT_DTYPE_F32 || dtype == PICT_DTYPE_F64, "unknown dtype %d", dtype);
     hipStream_t s = (hipStream_t)stream;
     double *part = nullptr;
-    PICT_HIP_TRY(hipMallocAsync((void **)&part, sizeof(double) * kMeanBlocks, s));
+    PICT_HIP_TRY(ws_alloc((void **)&part, sizeof(double) * kMeanBlocks, s));
     struct Guard {
         double *p; hipStream_t s;
-        ~Guard() { if (p) (void)hipFreeAsync(p, s); }
+        ~Guard() { if (p) (void)ws_free(p, s); }
     } guard{part, s};
     return dtype == PICT_DTYPE_F32 ? subtract_mean<float>(v, n, part, s) : subtract_mean<double>(v, n, part, s);
 }
