@@ -141,8 +141,7 @@ def main():
 
     # per-kernel roofline from the solver log
     agg = {}
-    for kind, n, nnz, nrhs, s, infos, kms, transposed in log:
-        name = "k_cg1<float,%d>" % nrhs if kind == "CG" else "k_bicgstab<float,%d>" % nrhs
+    for kind, n, nnz, nrhs, s, infos, kms, transposed, name in log:
         a = agg.setdefault(name, {"ms": 0.0, "bytes": 0, "launches": 0, "iters": 0, "adjoint_launches": 0})
         its = [i.usedIterations + 1 for i in infos]
         a["ms"] += kms

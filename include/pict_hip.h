@@ -308,6 +308,13 @@ pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, 
                               const pict_solve_opts *opts, pict_solve_info *infos_out,
                               void *stream);
 
+/* Name of the device kernel pict_solve_linear would launch for this system
+   (as a profiler shows it, e.g. "pict::k_cg_local<float, 4, 5, false>"),
+   written NUL-terminated into buf[len].  No counterpart in the reference:
+   it labels measurements of the solver kernel. */
+pict_status pict_solver_kernel_name(const pict_csr_desc *A, int32_t nrhs, const pict_solve_opts *opts, char *buf,
+                                    int32_t len);
+
 /* Transpose map of a structurally symmetric pattern (no counterpart in the reference, which
    transposes inside cuSPARSE): perm_out[e] = position of entry (j,i) for entry e = (i,j).
    Returns PICT_ERR_UNSUPPORTED if some (j,i) is missing.  Synchronises `stream`. */

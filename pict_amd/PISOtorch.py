@@ -2037,9 +2037,19 @@ def _solve_opts(maxIterations, tolerance, convergenceCriterion, useBiCG, matrixR
 
 
 # Solver diagnostics: when enabled, every solve appends
-# (kind, n, nnz, nrhs, dtype bytes, [LinearSolverResultInfo], kernel ms) -- kernel time from HIP events
-# recorded by the library around its launch on the launch stream.
+# (kind, n, nnz, nrhs, dtype bytes, [LinearSolverResultInfo], kernel ms, transposed, kernel name) -- kernel time
+# from HIP events recorded by the library around its launch on the launch stream.
 solve_log = None
+
+
+def solver_kernel_name(A, nrhs, opts):
+    """Name of the device kernel a solve of `A` with `nrhs` right-hand sides and `opts` (a SolveOpts) runs, as a
+    profiler lists it."""
+    buf = ctypes.create_string_buffer(128)
+    desc = A._desc()
+    _abi.check(_abi.load_library().pict_solver_kernel_name(ctypes.byref(desc), nrhs, ctypes.byref(opts), buf, 128),
+               "solver_kernel_name")
+    return buf.value.decode()
 
 
 def enable_solve_log(enable=True):
@@ -2071,7 +2081,7 @@ def _solve(A, RHS, x, opts):
            for i in infos]
     if solve_log is not None:
         solve_log.append(("BiCG" if opts.use_bicg else "CG", n, A.getSize(), nb, RHS.element_size(), res,
-                          lib.pict_last_solve_kernel_ms(), bool(opts.transpose_a)))
+                          lib.pict_last_solve_kernel_ms(), bool(opts.transpose_a), solver_kernel_name(A, nb, opts)))
     return res
 
 

@@ -501,9 +501,11 @@ class Simulation:
         if log is not None:
             es = domain.velocityRHS.element_size()
             n, nnz = domain.getTotalSize(), domain.C.getSize()
-            log.append((akind, n, nnz, D, es, adv, kms[0] if kms else 0.0, adjoint))
+            aname = PISOtorch.solver_kernel_name(domain.C, D, o.advection)
+            pname = PISOtorch.solver_kernel_name(domain.P, 1, o.pressure)
+            log.append((akind, n, nnz, D, es, adv, kms[0] if kms else 0.0, adjoint, aname))
             for k in range(K):
-                log.append((pkind, n, nnz, 1, es, [prs[k]], kms[1 + k] if kms else 0.0, adjoint))
+                log.append((pkind, n, nnz, 1, es, [prs[k]], kms[1 + k] if kms else 0.0, adjoint, pname))
         if self.record_solver_infos:
             self.solver_infos.append((akind, adv))
             for k in range(K):

@@ -1735,26 +1735,33 @@ const void *local_cg_kernel(int rpt, int nz, bool full) {
 
 // Workgroup budget of the register-resident CG: the cost of its barrier grows with the
 // number of workgroups, the per-thread work with rows per thread.
+// (PICT_CG_LOCAL_WGS: budget; 0 turns the kernel off)
 static int local_cg_target_wgs() {
-    static const int v = [] {
-        const char *e = getenv("PICT_CG_LOCAL_WGS");
-        const int t = e ? atoi(e) : 64;
-        return t;
-    }();
-    return v;
+    const char *e = getenv("PICT_CG_LOCAL_WGS");
+    return e ? atoi(e) : 64;
 }
 
+// Which kernel a solve runs, and on how many workgroups.
+struct Plan {
+    Kind kind;
+    const void *kernel;
+    long long nwg;
+    bool local;     // k_cg_local
+    int rpt, nz;    // k_cg_local rows per thread, cached row length
+    bool full;      // k_cg_local reduces the sum and maximum of |r| too
+};
+
 template <typename R>
-pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
-                          RhsState *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream) {
+pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *o, Plan &pl) {
     const int n = A->n;
     // PICT_CG_TEXTBOOK=1 selects the two-barrier textbook CG kernel (diagnostics / parity checks)
     const char *tb_env = getenv("PICT_CG_TEXTBOOK");
     const bool textbook_cg = tb_env && atoi(tb_env) != 0;
-    const Kind kind = o->use_bicg ? Kind::kBiCG : (textbook_cg ? Kind::kCGTextbook : Kind::kCG1);
-    const void *kernel = kernel_for_nrhs<R>(kind, nrhs);
+    pl = Plan{};
+    pl.kind = o->use_bicg ? Kind::kBiCG : (textbook_cg ? Kind::kCGTextbook : Kind::kCG1);
+    pl.kernel = kernel_for_nrhs<R>(pl.kind, nrhs);
     int occ = 0, cus = 0;
-    pict_status st = occupancy(kernel, occ, cus);
+    pict_status st = occupancy(pl.kernel, occ, cus);
     if (st) return st;
     if (occ < 1) {
         set_error("solver kernel cannot be resident (occupancy 0)");
@@ -1762,14 +1769,13 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     }
     // one right-hand side of the single-reduction CG without residual resets: the
     // register-resident kernel when its workgroups fit on the device at once
-    long long local_wgs = 0;
     const int target = local_cg_target_wgs();
-    if (kind == Kind::kCG1 && nrhs == 1 && o->residual_reset_steps <= 0 && target > 0) {
+    if (pl.kind == Kind::kCG1 && nrhs == 1 && o->residual_reset_steps <= 0 && target > 0) {
         const int nz = (long long)A->nnz <= 5LL * n ? 5 : 7;
         const int rmax = sizeof(R) == 4 ? 8 : 4;  // register budget of the FP64 rows
+        const bool full = o->convergence_criterion != PICT_CONV_NORM2 &&
+                          o->convergence_criterion != PICT_CONV_NORM2_NORMALIZED;
         for (int rpt = 1; rpt <= rmax; rpt *= 2) {
-            const bool full = o->convergence_criterion != PICT_CONV_NORM2 &&
-                              o->convergence_criterion != PICT_CONV_NORM2_NORMALIZED;
             const void *k = local_cg_kernel<R>(rpt, nz, full);
             const long long per = (long long)kSolveThreads * rpt;
             const long long wgs = ((long long)n + per - 1) / per;
@@ -1778,8 +1784,13 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
             st = occupancy(k, locc, lcus);
             if (st) return st;
             if (wgs <= (long long)locc * lcus) {
-                kernel = k;
-                local_wgs = wgs;
+                pl.kernel = k;
+                pl.nwg = wgs;
+                pl.local = true;
+                pl.rpt = rpt;
+                pl.nz = nz;
+                pl.full = full;
+                return PICT_OK;
             }
             break;
         }
@@ -1796,7 +1807,21 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     if (nwg > cap) nwg = cap;
     if (nwg > 2048) nwg = 2048;
     if (nwg < 1) nwg = 1;
-    if (local_wgs > 0) nwg = local_wgs;
+    pl.nwg = nwg;
+    return PICT_OK;
+}
+
+template <typename R>
+pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
+                          RhsState *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream) {
+    const int n = A->n;
+    Plan pl;
+    pict_status st = plan_solve<R>(A, nrhs, o, pl);
+    if (st) return st;
+    const Kind kind = pl.kind;
+    const void *kernel = pl.kernel;
+    const long long nwg = pl.nwg;
+    const long long local_wgs = pl.local ? pl.nwg : 0;
 
     const size_t vec = (size_t)n * nrhs;
     const int nvec = kind == Kind::kCGTextbook ? (o->return_best_result ? 5 : 4) : 9;
@@ -1988,6 +2013,28 @@ pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, 
     pict_status st = validate_solve(A, nrhs, opts);
     if (st) return st;
     return solve_sync(A, rhs, x, nrhs, opts, infos_out, (hipStream_t)stream);
+}
+
+pict_status pict_solver_kernel_name(const pict_csr_desc *A, int32_t nrhs, const pict_solve_opts *opts, char *buf,
+                                    int32_t len) {
+    PICT_CHECK_ARG(A && opts && buf && len > 0, "NULL argument");
+    pict_status st = validate_solve(A, nrhs, opts);
+    if (st) return st;
+    Plan pl;
+    const bool f32 = A->dtype == PICT_DTYPE_F32;
+    st = f32 ? plan_solve<float>(A, nrhs, opts, pl) : plan_solve<double>(A, nrhs, opts, pl);
+    if (st) return st;
+    const char *t = f32 ? "float" : "double";
+    const int nr = nrhs < kMaxRhs ? nrhs : kMaxRhs;
+    if (pl.local)
+        snprintf(buf, (size_t)len, "pict::k_cg_local<%s, %d, %d, %s>", t, pl.rpt, pl.nz, pl.full ? "true" : "false");
+    else if (pl.kind == Kind::kBiCG)
+        snprintf(buf, (size_t)len, "pict::k_bicgstab<%s, %d>", t, nr);
+    else if (pl.kind == Kind::kCGTextbook)
+        snprintf(buf, (size_t)len, "pict::k_cg<%s, %d>", t, nr);
+    else
+        snprintf(buf, (size_t)len, "pict::k_cg1<%s, %d>", t, nr);
+    return PICT_OK;
 }
 
 pict_status pict_set_kernel_timing(int32_t enable) {
