@@ -73,6 +73,7 @@ struct SolveArgs {
     int n, nrhs, maxit, conv, reset_steps, return_best, zero_shortcut;
     double tol;
     unsigned nwg;
+    double crit_scale;                             // criterion factor: 1/sqrt(n) (NORM2_NORMALIZED), 1/n (ABS_MEAN), 1
     unsigned long long *tagged;                    // k_cg_local: tagged partials [2][nwg][kSlots], zeroed
     void *pub;                                     // k_cg_local: published rows r, w, s [3][2][n], zeroed
     int salt;                                      // k_cg_local: stamp of this launch's foreign-read marks
@@ -604,7 +605,10 @@ __device__ __forceinline__ void cg1_update(const SolveArgs<R> &a, RhsState &st, 
     if (!st.active) return;
     st.save_best = 0;
     const double gam = red[S_RR];
-    const double crit = criterion(a.conv, gam, red[S_ASUM], red[S_AMAX], a.n);
+    // criterion (ComputeConvergenceCriterion) with the size factor precomputed on the host
+    const double crit = (a.conv == PICT_CONV_NORM2 || a.conv == PICT_CONV_NORM2_NORMALIZED) ? sqrt(gam) * a.crit_scale
+                        : a.conv == PICT_CONV_ABS_MAX                                      ? red[S_AMAX]
+                                                                                           : red[S_ASUM] * a.crit_scale;
     if (!isfinite(crit)) {
         st.converged = 0; st.used = it; st.final_res = crit; st.finite = 0; st.active = 0;
         return;
@@ -1025,7 +1029,7 @@ struct Pub<float> {
     using Word = u64;
     __device__ __forceinline__ u64 word(int par, int c) const { return ld_coh(base + (size_t)par * n + c); }
     static __device__ __forceinline__ float open(u64 u, unsigned tag, bool &ok) {
-        ok = ok && (unsigned)(u >> 32) == tag;
+        ok = ok & ((unsigned)(u >> 32) == tag);
         return __uint_as_float((unsigned)u);
     }
     static constexpr bool kTagged = true;
@@ -1052,16 +1056,52 @@ struct Pub<double> {
 // 0's view).  The two sets alternate by generation parity: a set is written again
 // only after every workgroup has passed the next generation.  `wait_rows`: the
 // workgroup's row stores must complete before its partials go out (untagged rows).
-template <int NS>
-__device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned nwg, unsigned gen,
-                                              const double (&acc)[NS], const int (&ids)[NS], const bool (&ismax)[NS],
-                                              bool wait_rows, double *red, int &s_err) {
+// Reduction layouts of k_cg_local (slot ids and whether a slot is a maximum), as types so
+// that every use folds to constants.  With a norm-2 criterion only (r,r) and (w,r) (and
+// max|b| in the first pass) are needed; the sum and the maximum of |r| only for the
+// absolute criteria.
+struct CgFirstFull {
+    static constexpr int NS = 4;
+    static constexpr int ID[4] = {S_RR, S_ASUM, S_AMAX, S_BMAX};
+    static constexpr bool MAX[4] = {false, false, true, true};
+};
+struct CgIterFull {
+    static constexpr int NS = 4;
+    static constexpr int ID[4] = {S_RR, S_ASUM, S_AMAX, S_D0};
+    static constexpr bool MAX[4] = {false, false, true, false};
+};
+struct CgFirst {
+    static constexpr int NS = 2;
+    static constexpr int ID[2] = {S_RR, S_BMAX};
+    static constexpr bool MAX[2] = {false, true};
+};
+struct CgIter {
+    static constexpr int NS = 2;
+    static constexpr int ID[2] = {S_RR, S_D0};
+    static constexpr bool MAX[2] = {false, false};
+};
+template <bool FULL>
+struct CgSlots {
+    using First = CgFirstFull;
+    using Iter = CgIterFull;
+};
+template <>
+struct CgSlots<false> {
+    using First = CgFirst;
+    using Iter = CgIter;
+};
+
+template <class L>
+__device__ __forceinline__ void tagged_poll(u64 *part, int *err_flag, unsigned nwg, unsigned gen,
+                                            const double (&acc)[L::NS], bool wait_rows, double (&v)[L::NS],
+                                            int &s_err) {
+    constexpr int NS = L::NS;
     __shared__ double wsum[kSolveThreads / 64][NS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (wait_rows) ops_done();
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        const double w = ismax[s] ? wave_reduce_dpp<true>(acc[s]) : wave_reduce_dpp<false>(acc[s]);
+        const double w = L::MAX[s] ? wave_reduce_dpp<true>(acc[s]) : wave_reduce_dpp<false>(acc[s]);
         if (lane == 0) wsum[wave][s] = w;
     }
     __syncthreads();
@@ -1073,13 +1113,12 @@ __device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned
         for (int s = 0; s < NS; ++s) {
             double t = wsum[0][s];
 #pragma unroll
-            for (int w = 1; w < kSolveThreads / 64; ++w) t = ismax[s] ? fmax(t, wsum[w][s]) : t + wsum[w][s];
-            st_coh(mine + ids[s], tag_partial(t, gen));
+            for (int w = 1; w < kSolveThreads / 64; ++w) t = L::MAX[s] ? fmax(t, wsum[w][s]) : t + wsum[w][s];
+            st_coh(mine + L::ID[s], tag_partial(t, gen));
         }
     }
     unsigned spins = 0;
     int err = 0;
-    double v[NS];
     for (;;) {
         bool ready = true;
 #pragma unroll
@@ -1088,10 +1127,10 @@ __device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned
             const u64 *pw = base + (size_t)w * kSlots;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                const u64 u = ld_coh(pw + ids[s]);
+                const u64 u = ld_coh(pw + L::ID[s]);
                 ready = ready && partial_ready(u, gen);
                 const double pv = partial_value(u);
-                v[s] = ismax[s] ? fmax(v[s], pv) : v[s] + pv;
+                v[s] = L::MAX[s] ? fmax(v[s], pv) : v[s] + pv;
             }
         }
         if (__all(ready)) break;
@@ -1101,43 +1140,40 @@ __device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned
         }
     }
     if (err && lane == 0) st_coh(err_flag, 1);
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        const double t = ismax[s] ? wave_reduce_dpp<true>(v[s]) : wave_reduce_dpp<false>(v[s]);
-        if (lane == 0) red[ids[s]] = t;
-    }
     if (lane == 0) s_err = err;
 }
 
-#define PICT_TAG_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                                          \
-    do {                                                                                                       \
-        tagged_reduce<NS>(a.tagged, &a.ctl->err, a.nwg, gen, ACC, IDS, ISMAX, !Pub<R>::kTagged, red, s_err);   \
-        ++gen;                                                                                                 \
-        if (threadIdx.x == 0 && !s_err) { UPDATE; }                                                            \
-        __syncthreads();                                                                                       \
-        if (s_err) return;                                                                                     \
+// the totals of wave 0's polled lines (fixed order) into the slots of red (every lane's own array)
+template <class L>
+__device__ __forceinline__ void poll_total(const double (&v)[L::NS], double *red) {
+#pragma unroll
+    for (int s = 0; s < L::NS; ++s)
+        red[L::ID[s]] = L::MAX[s] ? wave_reduce_dpp<true>(v[s]) : wave_reduce_dpp<false>(v[s]);
+}
+
+template <class L>
+__device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned nwg, unsigned gen,
+                                              const double (&acc)[L::NS], bool wait_rows, double *red, int &s_err) {
+    double v[L::NS];
+    tagged_poll<L>(part, err_flag, nwg, gen, acc, wait_rows, v, s_err);
+    if (threadIdx.x < 64) {
+        double t[kSlots];
+        poll_total<L>(v, t);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int s = 0; s < L::NS; ++s) red[L::ID[s]] = t[L::ID[s]];
+    }
+}
+
+#define PICT_TAG_BARRIER(L, ACC, UPDATE)                                                                     \
+    do {                                                                                                      \
+        tagged_reduce<L>(a.tagged, &a.ctl->err, a.nwg, gen, ACC, !Pub<R>::kTagged, red, s_err);               \
+        ++gen;                                                                                                \
+        if (threadIdx.x == 0 && !s_err) { UPDATE; }                                                           \
+        __syncthreads();                                                                                      \
+        if (s_err) return;                                                                                    \
     } while (0)
 
-// Reduction slots of k_cg_local: with a norm-2 criterion only (r,r) and (w,r) (and max|b| in
-// the first pass) are needed; the sum and the maximum of |r| only for the absolute criteria.
-template <bool FULL>
-struct CgSlots;
-template <>
-struct CgSlots<true> {
-    static constexpr int NS = 4;
-    static constexpr int R0[4] = {S_RR, S_ASUM, S_AMAX, S_BMAX};
-    static constexpr bool M0[4] = {false, false, true, true};
-    static constexpr int G[4] = {S_RR, S_ASUM, S_AMAX, S_D0};
-    static constexpr bool MG[4] = {false, false, true, false};
-};
-template <>
-struct CgSlots<false> {
-    static constexpr int NS = 2;
-    static constexpr int R0[2] = {S_RR, S_BMAX};
-    static constexpr bool M0[2] = {false, true};
-    static constexpr int G[2] = {S_RR, S_D0};
-    static constexpr bool MG[2] = {false, false};
-};
 // accumulate the residual terms of one row (slot order of CgSlots)
 template <bool FULL, int NS>
 __device__ __forceinline__ void cg_acc(double (&acc)[NS], double rv, double last, bool last_is_max) {
@@ -1158,12 +1194,19 @@ template <typename R, int RPT, int NZ, bool FULL>
 __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
     constexpr int C = kSolveThreads * RPT;
     constexpr int NH = NZ <= 5 ? 2 : 2 * RPT + 2;
-    using SL = CgSlots<FULL>;
-    constexpr int NS = SL::NS;
+    using LF = typename CgSlots<FULL>::First;
+    using LI = typename CgSlots<FULL>::Iter;
+    constexpr int NS = LI::NS;
     __shared__ R lv[C];  // r (first product) / r' (iterations) of this workgroup's rows
     __shared__ double red[kSlots];
-    __shared__ RhsState sst;
+    __shared__ RhsState sst;  // solver state through the first two reductions
     __shared__ int s_err;
+    // afterwards thread 0 keeps the state in registers and publishes what every thread needs
+    struct Step {
+        double alpha, beta;
+        int active, save_best, restore_best;
+    };
+    __shared__ Step bc;
     const int n = a.n;
     const int lo = (int)blockIdx.x * C;
     const int cnt = min(C, n - lo);
@@ -1252,7 +1295,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
         }
         // the stamps must be visible when the generation completes
         ops_done();
-        PICT_TAG_BARRIER(NS, acc, SL::R0, SL::M0, {
+        PICT_TAG_BARRIER(LF, acc, {
             const int za = (a.zero_shortcut && red[S_BMAX] == 0.0) ? 1 : 0;
             cg1_init(sst, red[S_RR], za);
         });
@@ -1315,7 +1358,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
             if (pub[q]) pw.put(0, i, wv, 1u);
             cg_acc<FULL, NS>(acc, (double)r[q], (double)wv * (double)r[q], false);
         }
-        PICT_TAG_BARRIER(NS, acc, SL::G, SL::MG, {
+        PICT_TAG_BARRIER(LI, acc, {
             if (sst.active) {
                 sst.rho = red[S_RR];
                 sst.alpha = sst.rho / red[S_D0];
@@ -1324,23 +1367,34 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
         });
     }
 
-    for (int it = 0; it < a.maxit && sst.active; ++it) {
-        const int cur = it & 1, nxt = cur ^ 1;
-        const unsigned tin = (unsigned)it + 1u, tout = (unsigned)it + 2u;
-        const R alpha = (R)sst.alpha, beta = (R)sst.beta;
-        const bool save = sst.save_best;
-        // the owners' r, w, s of the foreign entries: all loads first
-        typename Pub<R>::Word ur[NH], uw[NH], us[NH];
+    // the owners' r, w, s of the foreign entries for the next iteration: all loads back to back, issued as
+    // soon as the generation that completes them has completed
+    typename Pub<R>::Word ur[NH], uw[NH], us[NH];
+    auto load_foreign = [&](int par) {
 #pragma unroll
         for (int k = 0; k < NH; ++k) {
-            ur[k] = pr.word(cur, hc[k]);
-            uw[k] = pw.word(cur, hc[k]);
-            us[k] = ps.word(cur, hc[k]);
+            ur[k] = pr.word(par, hc[k]);
+            uw[k] = pw.word(par, hc[k]);
+            us[k] = ps.word(par, hc[k]);
         }
+    };
+    load_foreign(0);
+    RhsState st = sst;  // thread 0's copy is the live one
+    Step cs{st.alpha, st.beta, st.active, st.save_best, st.restore_best};
+    // a wave whose rows are all present and all in registers runs the iteration without the per-row tests
+    bool irregular = false;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) irregular = irregular | !row[q] | mem[q];
+    const bool simple = !__any(irregular);
+    for (int it = 0; it < a.maxit && cs.active; ++it) {
+        const int cur = it & 1, nxt = cur ^ 1;
+        const unsigned tin = (unsigned)it + 1u, tout = (unsigned)it + 2u;
+        const R alpha = (R)cs.alpha, beta = (R)cs.beta;
+        const bool save = cs.save_best;
         // own rows: p, s, x, r (while the loads are in flight)
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
-            if (!row[q]) continue;
+            if (!simple && !row[q]) continue;
             if (save) xb[q] = x[q];
             const R pm = fma(beta, p[q], r[q]);
             const R sm = fma(beta, s[q], w[q]);
@@ -1355,14 +1409,13 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
         bool ok = true;
 #pragma unroll
         for (int k = 0; k < NH; ++k) {
-            fr[k] = fw[k] = fs[k] = R(0);
-            if (k < nh) {
-                fr[k] = Pub<R>::open(ur[k], tin, ok);
-                fw[k] = Pub<R>::open(uw[k], tin, ok);
-                fs[k] = Pub<R>::open(us[k], tin, ok);
-            }
+            bool m = true;
+            fr[k] = Pub<R>::open(ur[k], tin, m);
+            fw[k] = Pub<R>::open(uw[k], tin, m);
+            fs[k] = Pub<R>::open(us[k], tin, m);
+            ok = ok & (m | (k >= nh));  // unused entries loaded word 0
         }
-        if (!ok) {
+        if (__any(!ok)) {
             // some owner's words had not arrived: wait for each
 #pragma unroll
             for (int k = 0; k < NH; ++k)
@@ -1381,23 +1434,16 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
         double acc[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k) acc[k] = 0.0;
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            if (!row[q]) continue;
-            const int i = lo + (int)threadIdx.x + q * kSolveThreads;
+        auto product = [&](int q) {
             R wv = 0;
-            if (!mem[q]) {
 #pragma unroll
-                for (int e = 0; e < NZ; ++e) wv += val[q][e] * lv[col[q][e] - lo];
+            for (int e = 0; e < NZ; ++e) wv += val[q][e] * lv[col[q][e] - lo];
 #pragma unroll
-                for (int k = 0; k < NH; ++k)
-                    if (hq[k] == q) wv += hv[k] * fx[k];
-            } else {
-                wv = mem_row(i, [&](int c) {
-                    return fma(-alpha, fma(beta, ps.get(cur, c, tin, herr), pw.get(cur, c, tin, herr)),
-                               pr.get(cur, c, tin, herr));
-                });
-            }
+            for (int k = 0; k < NH; ++k) wv += hq[k] == q ? hv[k] * fx[k] : R(0);
+            return wv;
+        };
+        auto finish_row = [&](int q, R wv) {
+            const int i = lo + (int)threadIdx.x + q * kSolveThreads;
             w[q] = wv;
             if (pub[q]) {
                 pr.put(nxt, i, r[q], tout);
@@ -1405,15 +1451,48 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
                 ps.put(nxt, i, s[q], tout);
             }
             cg_acc<FULL, NS>(acc, (double)r[q], (double)wv * (double)r[q], false);
+        };
+        if (simple) {
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) finish_row(q, product(q));
+        } else {
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                if (!row[q]) continue;
+                const int i = lo + (int)threadIdx.x + q * kSolveThreads;
+                const R wv = !mem[q] ? product(q) : mem_row(i, [&](int c) {
+                    return fma(-alpha, fma(beta, ps.get(cur, c, tin, herr), pw.get(cur, c, tin, herr)),
+                               pr.get(cur, c, tin, herr));
+                });
+                finish_row(q, wv);
+            }
         }
-        PICT_TAG_BARRIER(NS, acc, SL::G, SL::MG, cg1_update(a, sst, red, it));
+        // reduce-barrier; the next iteration's loads go out while wave 0 forms the totals and the step
+        double pv[NS];
+        tagged_poll<LI>(a.tagged, &a.ctl->err, a.nwg, gen, acc, !Pub<R>::kTagged, pv, s_err);
+        ++gen;
+        __syncthreads();
+        if (s_err) return;
+        load_foreign(nxt);
+        if (threadIdx.x < 64) {
+            double t[kSlots];
+#pragma unroll
+            for (int k = 0; k < kSlots; ++k) t[k] = 0.0;
+            poll_total<LI>(pv, t);
+            if (threadIdx.x == 0) {
+                cg1_update(a, st, t, it);
+                bc = Step{st.alpha, st.beta, st.active, st.save_best, st.restore_best};
+            }
+        }
+        __syncthreads();
+        cs = bc;
     }
     if (herr) st_coh(&a.ctl->err, 1);
-    const bool restore = sst.restore_best;
+    const bool restore = cs.restore_best;
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
         if (row[q]) a.x[lo + (int)threadIdx.x + q * kSolveThreads] = restore ? xb[q] : x[q];
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.st[0] = sst;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st[0] = st;
 }
 
 // zero n 64-bit words (the tagged regions of k_cg_local, before its launch)
@@ -1722,7 +1801,7 @@ const void *local_cg_kernel_nz(int rpt) {
         case 2: return (const void *)k_cg_local<R, 2, NZ, FULL>;
         case 4: return (const void *)k_cg_local<R, 4, NZ, FULL>;
         default:
-            if constexpr (sizeof(R) == 4) return (const void *)k_cg_local<R, 8, NZ, FULL>;
+            if constexpr (sizeof(R) == 4 && NZ <= 5) return (const void *)k_cg_local<R, 8, NZ, FULL>;
             return nullptr;
     }
 }
@@ -1772,7 +1851,7 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
     const int target = local_cg_target_wgs();
     if (pl.kind == Kind::kCG1 && nrhs == 1 && o->residual_reset_steps <= 0 && target > 0) {
         const int nz = (long long)A->nnz <= 5LL * n ? 5 : 7;
-        const int rmax = sizeof(R) == 4 ? 8 : 4;  // register budget of the FP64 rows
+        const int rmax = sizeof(R) == 4 && nz <= 5 ? 8 : 4;  // register budget (FP64 rows, 7-entry rows)
         const bool full = o->convergence_criterion != PICT_CONV_NORM2 &&
                           o->convergence_criterion != PICT_CONV_NORM2_NORMALIZED;
         for (int rpt = 1; rpt <= rmax; rpt *= 2) {
@@ -1878,6 +1957,9 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     a.zero_shortcut = o->zero_rhs_shortcut;
     a.tol = o->tolerance;
     a.nwg = (unsigned)nwg;
+    a.crit_scale = o->convergence_criterion == PICT_CONV_NORM2_NORMALIZED ? 1.0 / sqrt((double)n)
+                   : o->convergence_criterion == PICT_CONV_ABS_MEAN      ? 1.0 / (double)n
+                                                                         : 1.0;
     a.tagged = reinterpret_cast<unsigned long long *>(ws + off_zero);
     a.pub = a.tagged + words_tagged;
     {
