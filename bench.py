@@ -102,6 +102,10 @@ def main():
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="after the timed region, run 3 plain device copies of MIB MiB (a known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--pmc-fetch", default=None, metavar="CSV",
+                    help="rocprofv3 --pmc FETCH_SIZE counter_collection.csv of a run with --copy-check 256: fills "
+                         "roofline.traffic (with --pmc-write)")
+    ap.add_argument("--pmc-write", default=None, metavar="CSV", help="the same for WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PISO step of one domain runs on a single MI355X "
@@ -182,6 +186,10 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    traffic = None
+    if args.pmc_fetch and args.pmc_write:
+        traffic = pmc_traffic(args.pmc_fetch, args.pmc_write, dom_kernel)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(dom, dt, args, target)
@@ -206,11 +214,36 @@ def main():
                    "solver_ms_per_step": {k: v["ms"] / args.steps for k, v in agg.items()},
                    "wall_s": wall, **extra},
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def pmc_traffic(fetch_csv, write_csv, kernel):
+    """HBM bytes per launch of `kernel` from rocprofv3 FETCH_SIZE / WRITE_SIZE counter files (KiB per dispatch),
+    each scaled by what the same counter reads for the 256 MiB device copy of --copy-check 256."""
+    import csv
+
+    def per_kernel(path, counter):
+        vals, copy = [], 0.0
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"].replace("void ", "").split("(")[0]
+                v = float(r["Counter_Value"])
+                if name == kernel:
+                    vals.append(v)
+                elif "copyBuffer" in name:
+                    copy = max(copy, v)
+        if not vals or copy <= 0:
+            return None
+        return sum(vals) / len(vals) * (256 * 1024 / copy) * 1024
+
+    f, w = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    return None if f is None or w is None else f + w
 
 
 def cpu_baseline(dom, dt, args, target):
