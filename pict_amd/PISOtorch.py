@@ -1684,8 +1684,10 @@ class Domain:
             for fb in b.boundaries:
                 if fb.type == BoundaryType.FIXED and b.hasTransform() != fb.hasTransform():
                     pass  # the reference only warns (domain_structs.cpp:1678)
-        self._desc, self._blocks_arr = self._pack()
-        if self._device.type == "cuda":
+        desc, arr = self._pack()
+        unchanged = desc is getattr(self, "_desc", None) and self._atlas is not None
+        self._desc, self._blocks_arr = desc, arr
+        if self._device.type == "cuda" and not unchanged:
             lib = _abi.load_library()
             stream = _stream(self._device)
             if self._atlas is None:

@@ -15,6 +15,7 @@ Two implementations of the same contract:
 Every kernel is deterministic, so a recomputed step is bit-identical to its forward.
 """
 import ctypes
+import weakref
 
 import torch
 
@@ -89,8 +90,65 @@ class _Tape:
         self.desc = _abi.StepTape(*[t.data_ptr() for t in self.tensors])
 
 
+def _grad_tensors(domain):
+    """Every gradient tensor the reverse pass reads or writes (None where one is missing)."""
+    ts = [domain.A_grad, domain.C_grad.value if domain.C_grad is not None else None,
+          domain.P_grad.value if domain.P_grad is not None else None, domain.velocityRHS_grad,
+          domain.velocityResult_grad, domain.pressureRHS_grad, domain.pressureRHSdiv_grad,
+          domain.pressureResult_grad, domain.viscosity_grad]
+    for b in domain.getBlocks():
+        ts += [b._velocity_grad, b._pressure_grad]
+    return ts + _param_grad_tensors(domain)
+
+
+def _param_grad_tensors(domain):
+    """The parameter gradients the reverse pass accumulates into (block sources and viscosities, fixed-boundary
+    velocities, the domain viscosity)."""
+    ts = [domain.viscosity_grad]
+    for b in domain.getBlocks():
+        ts += [b.velocitySource_grad, b._viscosity_grad]
+        ts += [fb._velocity_grad for _, fb in b.getFixedBoundaries()]
+    return [t for t in ts if t is not None]
+
+
+def _grad_key(domain):
+    ts = _grad_tensors(domain)
+    if any(t is None for t in ts[:9 + 2 * len(domain.getBlocks())]):
+        return None
+    return tuple(t.data_ptr() for t in ts)
+
+
+_GRAD_KEYS = weakref.WeakKeyDictionary()  # domain -> addresses of the gradient tensors created here
+
+
+_RECOMPUTE_TAPES = weakref.WeakKeyDictionary()  # domain -> tape reused by recomputing backward passes
+
+
+def _recompute_tape(domain, K):
+    """The tape a recomputing backward pass records into: filled and consumed within one backward call, so one per
+    domain is reused."""
+    t = _RECOMPUTE_TAPES.get(domain)
+    key = (K, domain.getTotalSize(), domain.C.getSize(), domain.getSpatialDims(), domain.getDtype(), domain.getDevice())
+    if t is None or t[0] != key:
+        t = (key, _Tape(domain, K))
+        _RECOMPUTE_TAPES[domain] = t
+    return t[1]
+
+
 def _ensure_grad_tensors(domain):
-    """Gradient tensors the reverse pass writes; parameter gradients (accumulated by the kernels) start at 0."""
+    """Gradient tensors the reverse pass writes.  They are created once per domain and reused: the reverse pass
+    overwrites every intermediate gradient before it reads it, so only the parameter gradients, which its kernels
+    accumulate into, are zeroed again."""
+    key = _grad_key(domain)
+    if key is not None and _GRAD_KEYS.get(domain) == key:
+        for t in _param_grad_tensors(domain):
+            t.zero_()
+        return
+    _create_grad_tensors(domain)
+    _GRAD_KEYS[domain] = _grad_key(domain)
+
+
+def _create_grad_tensors(domain):
     domain.CreateAGrad()
     domain.CGrad.CreateValue()
     domain.PGrad.CreateValue()
@@ -106,9 +164,22 @@ def _ensure_grad_tensors(domain):
     domain.CreateVelocityGradOnBoundaries()
 
 
-def _to_global(tensors, D):
-    """Per-block [1,D,...] tensors -> global component-major vector."""
-    return torch.cat([torch.cat([t[0, c].reshape(-1) for t in tensors]) for c in range(D)]).contiguous()
+_GRAD_VECTORS = weakref.WeakKeyDictionary()  # domain -> the global gradient vector of its reverse passes
+
+
+def _to_global(tensors, D, domain=None):
+    """Per-block [1,D,...] tensors -> global component-major vector (the domain's persistent one if given: its
+    address is part of what a captured reverse pass replays)."""
+    parts = [t[0, c].reshape(-1) for c in range(D) for t in tensors]
+    if domain is None:
+        return torch.cat(parts).contiguous()
+    n = D * domain.getTotalSize()
+    g = _GRAD_VECTORS.get(domain)
+    if g is None or g.numel() != n or g.dtype != tensors[0].dtype or g.device != tensors[0].device:
+        g = torch.empty(n, dtype=tensors[0].dtype, device=tensors[0].device)
+        _GRAD_VECTORS[domain] = g
+    torch.cat(parts, out=g)
+    return g
 
 
 def _from_global(v, like, D):
@@ -135,7 +206,7 @@ def _native_backward(sim, tape, vel_grads):
     D = domain.getSpatialDims()
     _ensure_grad_tensors(domain)
     domain.UpdateDomainData()
-    g = _to_global(vel_grads, D)
+    g = _to_global(vel_grads, D, domain)
     o = sim._native_opts(sim._get_time_step_torch())
     K = o.corrector_steps
     infos = (_abi.SolveInfo * (D + K))()
@@ -222,7 +293,7 @@ class _PISOStep(torch.autograd.Function):
                     b.setPressure(torch.empty_like(p))
                 if tape is None:
                     # recompute the step from its input, recording the tape
-                    tape = _Tape(domain, sim.corrector_steps)
+                    tape = _recompute_tape(domain, sim.corrector_steps)
                     for b, v in zip(domain.getBlocks(), saved[:n]):
                         b.setVelocity(v.detach().clone())
                     domain.UpdateDomainData()

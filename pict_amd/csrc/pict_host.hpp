@@ -20,6 +20,9 @@ void set_error(const char *fmt, ...);
 hipError_t ws_alloc(void **p, size_t bytes, hipStream_t s);
 hipError_t ws_free(void *p, hipStream_t s);
 
+// Frees the captured step graphs of an atlas (defined in pict_step.hip).
+void step_graphs_free(struct pict_atlas *a);
+
 #define PICT_HIP_TRY(expr)                                                                  \
     do {                                                                                    \
         hipError_t _e = (expr);                                                             \
@@ -81,6 +84,10 @@ inline pict_status check_non_ortho(int flags) {
     hipLaunchKernelGGL(KERNEL, dim3((atlas)->num_tiles), dim3(kCellsPerTile), 0,                \
                        (hipStream_t)(stream), __VA_ARGS__)
 
+namespace pict {
+struct GraphCache;
+}
+
 // Device-resident domain: descriptor copy + launch tables (one allocation).
 struct pict_atlas {
     pict_domain_desc desc;                // host copy; desc.blocks points to host_blocks
@@ -100,6 +107,8 @@ struct pict_atlas {
     // A^T map of the CSR pattern (built on first use by a reverse pass, for the pattern it was built from)
     mutable int32_t *tperm = nullptr;
     mutable const int32_t *tperm_row = nullptr, *tperm_index = nullptr;
+    // captured graphs of native steps and reverse passes (pict_step.hip)
+    mutable pict::GraphCache *step_graphs = nullptr;
 
     template <typename R>
     pict::DomainK<R> view() const {

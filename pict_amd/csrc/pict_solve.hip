@@ -1890,9 +1890,37 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
     return PICT_OK;
 }
 
+// Workspace of one solve: vectors | FP64 partials | final states | transpose map | control block |
+// k_cg_local's tagged partials and published rows.  The control block and the regions after it
+// are zeroed by one fill kernel before the launch.
+struct Layout {
+    size_t vec, off_part, off_st, off_perm, off_ctl, off_zero, words_tagged, words_zero, total;
+};
+
+template <typename R>
+Layout layout_of(const pict_csr_desc *A, int nrhs, const pict_solve_opts *o, const Plan &pl) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    Layout L;
+    const size_t n = (size_t)A->n;
+    L.vec = n * nrhs;
+    const int nvec = pl.kind == Kind::kCGTextbook ? (o->return_best_result ? 5 : 4) : 9;
+    const size_t bytes_part = sizeof(double) * (size_t)2 * pl.nwg * kPartStride;
+    const size_t bytes_perm = (o->transpose_a && !A->transpose_perm) ? sizeof(int32_t) * (size_t)A->nnz : 0;
+    L.off_part = al(sizeof(R) * L.vec * nvec);
+    L.off_st = L.off_part + al(bytes_part);
+    L.off_perm = L.off_st + al(sizeof(RhsState) * nrhs);
+    L.off_ctl = L.off_perm + al(bytes_perm);
+    L.off_zero = L.off_ctl + al(sizeof(Ctl));
+    L.words_tagged = pl.local ? (size_t)2 * pl.nwg * kSlots : 0;
+    L.words_zero = pl.local ? L.words_tagged + 6 * n : 0;
+    L.total = L.off_zero + sizeof(unsigned long long) * L.words_zero + 256;
+    return L;
+}
+
 template <typename R>
 pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
-                          RhsState *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream) {
+                          RhsState *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, void *ws_in,
+                          size_t ws_bytes, hipStream_t stream) {
     const int n = A->n;
     Plan pl;
     pict_status st = plan_solve<R>(A, nrhs, o, pl);
@@ -1900,29 +1928,22 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     const Kind kind = pl.kind;
     const void *kernel = pl.kernel;
     const long long nwg = pl.nwg;
-    const long long local_wgs = pl.local ? pl.nwg : 0;
+    const Layout L = layout_of<R>(A, nrhs, o, pl);
+    const size_t vec = L.vec;
 
-    const size_t vec = (size_t)n * nrhs;
-    const int nvec = kind == Kind::kCGTextbook ? (o->return_best_result ? 5 : 4) : 9;
-    const size_t bytes_vec = sizeof(R) * vec * nvec;
-    const size_t bytes_part = sizeof(double) * (size_t)2 * nwg * kPartStride;
-    const size_t bytes_perm = (o->transpose_a && !A->transpose_perm) ? sizeof(int32_t) * (size_t)A->nnz : 0;
-    const size_t off_part = (bytes_vec + 255) & ~size_t(255);
-    const size_t off_ctl = off_part + ((bytes_part + 255) & ~size_t(255));
-    const size_t off_st = off_ctl + ((sizeof(Ctl) + 255) & ~size_t(255));
-    const size_t off_perm = off_st + ((sizeof(RhsState) * nrhs + 255) & ~size_t(255));
-    // k_cg_local: tagged partials [2][nwg][kSlots] and published rows [3][2][n], 8-byte words, zeroed
-    const size_t words_tagged = local_wgs > 0 ? (size_t)2 * nwg * kSlots : 0;
-    const size_t words_zero = local_wgs > 0 ? words_tagged + (size_t)6 * n : 0;
-    const size_t off_zero = off_perm + ((bytes_perm + 255) & ~size_t(255));
-    const size_t total = off_zero + sizeof(unsigned long long) * words_zero + 256;
-
-    char *ws = nullptr;
-    PICT_HIP_TRY(ws_alloc((void **)&ws, total, stream));
+    char *ws = (char *)ws_in;
     struct Guard {
         char *p; hipStream_t s;
         ~Guard() { if (p) (void)ws_free(p, s); }
-    } guard{ws, stream};
+    } guard{nullptr, stream};
+    if (ws) {
+        PICT_CHECK_ARG(ws_bytes >= L.total, "solver workspace too small (%zu < %zu bytes)", ws_bytes, L.total);
+    } else {
+        PICT_HIP_TRY(ws_alloc((void **)&ws, L.total, stream));
+        guard.p = ws;
+    }
+    const size_t off_part = L.off_part, off_ctl = L.off_ctl, off_st = L.off_st, off_perm = L.off_perm,
+                 off_zero = L.off_zero, words_tagged = L.words_tagged;
 
     SolveArgs<R> a;
     memset(&a, 0, sizeof a);
@@ -1969,10 +1990,11 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
         if (sv == 0) sv = ++launches;
         a.salt = (int)sv;
     }
-    PICT_HIP_TRY(hipMemsetAsync(ws + off_ctl, 0, sizeof(Ctl), stream));
-    if (words_zero) {
-        const unsigned blocks = (unsigned)std::min<size_t>((words_zero + 255) / 256, 1024);
-        hipLaunchKernelGGL(k_zero64, dim3(blocks), dim3(256), 0, stream, a.tagged, words_zero);
+    {
+        // control block (+ k_cg_local's tagged regions): a fill kernel, which a captured graph replays safely
+        const size_t words = (off_zero - off_ctl) / 8 + L.words_zero;
+        const unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_zero64, dim3(blocks), dim3(256), 0, stream, (unsigned long long *)(ws + off_ctl), words);
         PICT_HIP_TRY(hipGetLastError());
     }
     if (o->transpose_a && A->transpose_perm) {
@@ -2025,13 +2047,30 @@ namespace pict {
 
 size_t solve_state_bytes() { return sizeof(RhsState); }
 
+pict_status solve_workspace_bytes(const pict_csr_desc *A, int nrhs, const pict_solve_opts *o, size_t *bytes) {
+    pict_status st = validate_solve(A, nrhs, o);
+    if (st) return st;
+    Plan pl;
+    if (A->dtype == PICT_DTYPE_F32) {
+        if ((st = plan_solve<float>(A, nrhs, o, pl))) return st;
+        *bytes = layout_of<float>(A, nrhs, o, pl).total;
+    } else {
+        if ((st = plan_solve<double>(A, nrhs, o, pl))) return st;
+        *bytes = layout_of<double>(A, nrhs, o, pl).total;
+    }
+    return PICT_OK;
+}
+
 pict_status solve_enqueue(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
-                          void *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream) {
+                          void *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream,
+                          void *ws, size_t ws_bytes) {
     pict_status st = validate_solve(A, nrhs, o);
     if (st) return st;
     if (A->dtype == PICT_DTYPE_F32)
-        return enqueue_typed<float>(A, rhs, x, nrhs, o, (RhsState *)state_out, err_out, ev0, ev1, stream);
-    return enqueue_typed<double>(A, rhs, x, nrhs, o, (RhsState *)state_out, err_out, ev0, ev1, stream);
+        return enqueue_typed<float>(A, rhs, x, nrhs, o, (RhsState *)state_out, err_out, ev0, ev1, ws, ws_bytes,
+                                    stream);
+    return enqueue_typed<double>(A, rhs, x, nrhs, o, (RhsState *)state_out, err_out, ev0, ev1, ws, ws_bytes,
+                                 stream);
 }
 
 void solve_state_to_info(const void *states, int nrhs, pict_solve_info *out) {

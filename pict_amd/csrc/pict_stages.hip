@@ -435,6 +435,7 @@ static void atlas_free(pict_atlas *a) {
     }
     if (a->dev) (void)hipFree(a->dev);
     if (a->tperm) (void)hipFree(a->tperm);
+    step_graphs_free(a);
     delete a;
 }
 

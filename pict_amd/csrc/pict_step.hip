@@ -8,6 +8,8 @@
 // solve, and no interpreter runs between the stages.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "pict_host.hpp"
@@ -163,49 +165,86 @@ thread_local HostStates g_host_states;
 
 }  // namespace
 
-extern "C" {
-
 static pict_status copy_dev(void *dst, const void *src, size_t bytes, hipStream_t s) {
     PICT_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
     return PICT_OK;
 }
 
-pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
-                           pict_solve_info *infos_out, void *stream) {
-    PICT_CHECK_ARG(a && o && infos_out, "NULL argument");
+namespace {
+
+// Device and host places of one native step or reverse pass.
+//   dev:  per solve a slot [kMaxRhs states | error flag]; the mean partial sums; for the
+//         reverse pass its vectors g_u, A_tot, C_tot, lambda.
+//   solve_ws[i]: workspace of solve i (null: from the library pool).
+//   host: pinned destination of the solve slots (copied at the end of the queued work).
+struct Work {
+    char *dev = nullptr;
+    void *solve_ws[65] = {};
+    size_t solve_ws_bytes[65] = {};
+    void *host = nullptr;
+    hipEvent_t *ev = nullptr;
+};
+
+struct Sizes {
+    size_t slot, o_part, o_gu, o_at, o_ct, o_lam, dev_bytes;
+    int nsolves;
+};
+
+Sizes sizes_of(const pict_atlas *a, const pict_step_opts *o, bool vjp) {
     const pict_domain_desc &d = a->desc;
-    PICT_CHECK_ARG(d.C_value && d.P_value && d.A && d.velocity_rhs && d.velocity_result && d.pressure_rhs &&
-                       d.pressure_rhs_div && d.pressure_result,
-                   "domain tensors missing");
-    PICT_CHECK_ARG(o->corrector_steps >= 0 && o->corrector_steps <= 64, "corrector_steps must be 0..64");
-    PICT_CHECK_ARG(o->advection.transpose_a == 0 && o->pressure.transpose_a == 0, "transposed solves in a step");
-    hipStream_t s = (hipStream_t)stream;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t es = d.dtype == PICT_DTYPE_F32 ? sizeof(float) : sizeof(double);
+    const size_t vecD = es * d.num_dims * d.num_cells, vec1 = es * d.num_cells, nzb = es * d.nnz;
+    Sizes z;
+    z.nsolves = 1 + o->corrector_steps;
+    z.slot = al(4 * solve_state_bytes() + sizeof(int));
+    z.o_part = z.slot * z.nsolves;
+    z.o_gu = z.o_part + al(sizeof(double) * kMeanBlocks);
+    if (!vjp) {
+        z.o_at = z.o_ct = z.o_lam = 0;
+        z.dev_bytes = z.o_gu;
+        return z;
+    }
+    z.o_at = z.o_gu + al(vecD);
+    z.o_ct = z.o_at + al(vec1);
+    z.o_lam = z.o_ct + al(nzb);
+    z.dev_bytes = z.o_lam + al(vecD);
+    return z;
+}
+
+// the solves of a step (vjp: transposed, zero-RHS shortcut): i = 0 advection, 1.. pressure
+void solve_systems(const pict_atlas *a, const pict_step_opts *o, bool vjp, pict_csr_desc &C, pict_csr_desc &P,
+                   pict_solve_opts &adv, pict_solve_opts &prs) {
+    C = csr_of(a, a->desc.C_value);
+    P = csr_of(a, a->desc.P_value);
+    adv = o->advection;
+    prs = o->pressure;
+    if (vjp) {
+        C.transpose_perm = a->tperm;
+        P.transpose_perm = a->tperm;
+        adv.transpose_a = prs.transpose_a = 1;
+        adv.zero_rhs_shortcut = prs.zero_rhs_shortcut = 1;
+    }
+}
+
+// the forward step queued on s (no allocation, no host wait when w's workspaces are given)
+pict_status enqueue_step(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape, const Work &w,
+                         const Sizes &z, hipStream_t s) {
+    const pict_domain_desc &d = a->desc;
+    void *stream = s;
     const int D = d.num_dims, N = d.num_cells, K = o->corrector_steps;
     const bool f32 = d.dtype == PICT_DTYPE_F32;
     const size_t sb = solve_state_bytes();
-    const int nsolves = 1 + K;
-    // device: per solve kMaxRhs states + an error flag; mean partial sums
-    const size_t slot = ((4 * sb + sizeof(int)) + 255) & ~size_t(255);
-    const size_t bytes = slot * nsolves + sizeof(double) * kMeanBlocks;
-    char *dev = nullptr;
-    PICT_HIP_TRY(ws_alloc((void **)&dev, bytes, s));
-    struct Guard {
-        char *p; hipStream_t s;
-        ~Guard() { if (p) (void)ws_free(p, s); }
-    } guard{dev, s};
-    double *part = (double *)(dev + slot * nsolves);
-    hipEvent_t *ev = nullptr;
-    pict_status st = timing_events(nsolves, ev);
-    if (st) return st;
-
+    char *dev = w.dev;
+    double *part = (double *)(dev + z.o_part);
+    hipEvent_t *ev = w.ev;
+    pict_status st;
     const double dt = o->time_step;
     const size_t es = f32 ? sizeof(float) : sizeof(double);
     const size_t vecD = es * D * N, vec1 = es * N, nzb = es * d.nnz;
-    if (tape)
-        PICT_CHECK_ARG(tape->C_value && tape->A && tape->P_value && tape->velocity_star &&
-                           (K == 0 || (tape->pressure_rhs && tape->pressure_raw && tape->velocity_result)),
-                       "tape buffers missing");
-    const pict_csr_desc C = csr_of(a, d.C_value), P = csr_of(a, d.P_value);
+    pict_csr_desc C, P;
+    pict_solve_opts adv, prs;
+    solve_systems(a, o, false, C, P, adv, prs);
     // ---- advection (PISOtorch_simulation.py:1161-1188) ----
     if ((st = pict_setup_advection_matrix(a, dt, 0, stream))) return st;
     if ((st = pict_setup_advection_velocity(a, dt, 0, 0, stream))) return st;
@@ -213,20 +252,21 @@ pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const p
         return st;
     if ((st = f32 ? fill<float>(d.velocity_result, (size_t)D * N, 0.0, s) : fill<double>(d.velocity_result, (size_t)D * N, 0.0, s)))
         return st;
-    if ((st = solve_enqueue(&C, d.velocity_rhs, d.velocity_result, D, &o->advection, dev,
-                            (int *)(dev + 4 * sb), ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, s)))
+    if ((st = solve_enqueue(&C, d.velocity_rhs, d.velocity_result, D, &adv, dev, (int *)(dev + 4 * sb),
+                            ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, s, w.solve_ws[0], w.solve_ws_bytes[0])))
         return st;
     if (tape && (st = copy_dev(tape->velocity_star, d.velocity_result, vecD, s))) return st;
     // ---- correctors (:1229-1320) ----
     for (int k = 0; k < K; ++k) {
-        char *sl = dev + slot * (1 + k);
+        char *sl = dev + z.slot * (1 + k);
         if ((st = pict_setup_pressure_correction(a, dt, 0, 0, o->time_step_norm, stream))) return st;
         if (tape && k == 0 && (st = copy_dev(tape->P_value, d.P_value, nzb, s))) return st;
         if (tape && (st = copy_dev((char *)tape->pressure_rhs + vecD * k, d.pressure_rhs, vecD, s))) return st;
         if ((st = f32 ? fill<float>(d.pressure_result, (size_t)N, 0.0, s) : fill<double>(d.pressure_result, (size_t)N, 0.0, s)))
             return st;
-        if ((st = solve_enqueue(&P, d.pressure_rhs_div, d.pressure_result, 1, &o->pressure, sl, (int *)(sl + 4 * sb),
-                                ev ? ev[2 * (1 + k)] : nullptr, ev ? ev[2 * (1 + k) + 1] : nullptr, s)))
+        if ((st = solve_enqueue(&P, d.pressure_rhs_div, d.pressure_result, 1, &prs, sl, (int *)(sl + 4 * sb),
+                                ev ? ev[2 * (1 + k)] : nullptr, ev ? ev[2 * (1 + k) + 1] : nullptr, s,
+                                w.solve_ws[1 + k], w.solve_ws_bytes[1 + k])))
             return st;
         if (tape && (st = copy_dev((char *)tape->pressure_raw + vec1 * k, d.pressure_result, vec1, s))) return st;
         if (o->normalize_pressure) {
@@ -239,29 +279,125 @@ pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const p
         if (tape && (st = copy_dev((char *)tape->velocity_result + vecD * k, d.velocity_result, vecD, s))) return st;
     }
     if ((st = pict_copy_velocity_result_to_blocks(a, stream))) return st;
+    PICT_HIP_TRY(hipMemcpyAsync(w.host, dev, z.slot * z.nsolves, hipMemcpyDeviceToHost, s));
+    return PICT_OK;
+}
 
-    // ---- one wait: the solvers' final states ----
-    HostStates &h = g_host_states;
-    if (h.bytes < slot * nsolves) {
-        if (h.p) PICT_HIP_TRY(hipHostFree(h.p));
-        h.p = nullptr;
-        PICT_HIP_TRY(hipHostMalloc(&h.p, slot * nsolves, hipHostMallocDefault));
-        h.bytes = slot * nsolves;
+// the reverse pass queued on s (the order in which PISOtorch_diff's autograd graph visits the
+// stages of the orthogonal step, PISOtorch_diff.py:295-1234)
+pict_status enqueue_vjp(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape, void *velocity_grad,
+                        const Work &w, const Sizes &z, hipStream_t s) {
+    const pict_domain_desc &d = a->desc;
+    void *stream = s;
+    const int D = d.num_dims, N = d.num_cells, K = o->corrector_steps;
+    const bool f32 = d.dtype == PICT_DTYPE_F32;
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    const size_t vecD = es * D * N, vec1 = es * N, nzb = es * d.nnz;
+    const size_t sb = solve_state_bytes();
+    char *dev = w.dev;
+    double *part = (double *)(dev + z.o_part);
+    void *g_u = dev + z.o_gu, *A_tot = dev + z.o_at, *C_tot = dev + z.o_ct, *lam = dev + z.o_lam;
+    const double dt = o->time_step;
+    hipEvent_t *ev = w.ev;
+    pict_status st;
+#define PICT_TRY(expr) do { if ((st = (expr))) return st; } while (0)
+#define PICT_T(fn, ...) (f32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
+    auto block_grads = [&](void *gv, int field, int mode) -> pict_status {
+        if (f32) PICT_LAUNCH_CELLS(a, stream, (k_block_grads<float>), a->view<float>(), (float *)gv, field, mode);
+        else PICT_LAUNCH_CELLS(a, stream, (k_block_grads<double>), a->view<double>(), (double *)gv, field, mode);
+        return check_launch();
+    };
+    auto outer_acc = [&](void *out, const void *av, const void *bv, double alpha) -> pict_status {
+        const unsigned blocks = (unsigned)((N + 255) / 256);
+        if (f32)
+            hipLaunchKernelGGL(k_outer_acc<float>, dim3(blocks), dim3(256), 0, s, (float *)out, (const float *)av,
+                               (const float *)bv, d.csr_row, d.csr_index, N, (float)alpha);
+        else
+            hipLaunchKernelGGL(k_outer_acc<double>, dim3(blocks), dim3(256), 0, s, (double *)out, (const double *)av,
+                               (const double *)bv, d.csr_row, d.csr_index, N, alpha);
+        return check_launch();
+    };
+    pict_csr_desc C, P;
+    pict_solve_opts adv, prs;
+    solve_systems(a, o, true, C, P, adv, prs);
+
+    // forward state the VJP kernels read
+    PICT_TRY(copy_dev(d.C_value, tape->C_value, nzb, s));
+    PICT_TRY(copy_dev(d.A, tape->A, vec1, s));
+    PICT_TRY(copy_dev(d.P_value, tape->P_value, nzb, s));
+    PICT_TRY(PICT_T(fill, g_u, (size_t)D * N, 0.0, s));
+    PICT_TRY(PICT_T(fill, A_tot, (size_t)N, 0.0, s));
+    PICT_TRY(PICT_T(fill, C_tot, (size_t)d.nnz, 0.0, s));
+    // CopyVelocityResultToBlocks: the output block-velocity gradient is the velocity-result gradient
+    PICT_TRY(copy_dev(d.velocity_result_grad, velocity_grad, vecD, s));
+    for (int k = K - 1; k >= 0; --k) {
+        char *sl = dev + z.slot * (1 + k);
+        // ---- CorrectVelocity (inputs A, block pressure p^_k, pressure RHS) ----
+        PICT_TRY(copy_dev(d.pressure_result, (const char *)tape->pressure_raw + vec1 * k, vec1, s));
+        if (o->normalize_pressure) PICT_TRY(PICT_T(subtract_mean, d.pressure_result, N, part, s));
+        PICT_TRY(pict_copy_pressure_result_to_blocks(a, stream));
+        PICT_TRY(block_grads(nullptr, 3, 1));
+        PICT_TRY(pict_correct_velocity_grad(a, dt, o->time_step_norm, stream));
+        PICT_TRY(PICT_T(axpy, A_tot, d.A_grad, (size_t)N, 1.0, s));
+        // ---- CopyPressureResultToBlocks, mean subtraction, pressure solve ----
+        PICT_TRY(pict_copy_pressure_result_grad_from_blocks(a, stream));
+        if (o->normalize_pressure) PICT_TRY(PICT_T(subtract_mean, d.pressure_result_grad, N, part, s));
+        PICT_TRY(PICT_T(fill, d.pressure_rhs_div_grad, (size_t)N, 0.0, s));
+        PICT_TRY(solve_enqueue(&P, d.pressure_result_grad, d.pressure_rhs_div_grad, 1, &prs, sl,
+                               (int *)(sl + 4 * sb), ev ? ev[2 * (1 + k)] : nullptr,
+                               ev ? ev[2 * (1 + k) + 1] : nullptr, s, w.solve_ws[1 + k], w.solve_ws_bytes[1 + k]));
+        PICT_TRY(PICT_T(fill, d.P_grad, (size_t)d.nnz, 0.0, s));
+        PICT_TRY(outer_acc(d.P_grad, d.pressure_rhs_div_grad, (const char *)tape->pressure_raw + vec1 * k, -1.0));
+        // ---- SetupPressureCorrection (inputs velocity, source, velocity result, boundaries, viscosity, A, C) ----
+        PICT_TRY(copy_dev(d.velocity_result, k == 0 ? tape->velocity_star
+                                                     : (const char *)tape->velocity_result + vecD * (k - 1), vecD, s));
+        PICT_TRY(copy_dev(d.pressure_rhs, (const char *)tape->pressure_rhs + vecD * k, vecD, s));
+        PICT_TRY(PICT_T(fill, d.velocity_result_grad, (size_t)D * N, 0.0, s));
+        PICT_TRY(PICT_T(fill, d.A_grad, (size_t)N, 0.0, s));
+        PICT_TRY(pict_setup_pressure_correction_grad(a, dt, 0, 0, o->time_step_norm, stream));
+        PICT_TRY(PICT_T(axpy, A_tot, d.A_grad, (size_t)N, 1.0, s));
+        PICT_TRY(PICT_T(axpy, C_tot, d.C_grad, (size_t)d.nnz, 1.0, s));
+        PICT_TRY(block_grads(g_u, 2, 0));
     }
-    PICT_HIP_TRY(hipMemcpyAsync(h.p, dev, slot * nsolves, hipMemcpyDeviceToHost, s));
-    PICT_HIP_TRY(hipStreamSynchronize(s));
-    const char *hp = (const char *)h.p;
+    // ---- advection solve C u* = rhs: lambda = C^-T g, C_grad -= lambda u*^T ----
+    PICT_TRY(PICT_T(fill, lam, (size_t)D * N, 0.0, s));
+    PICT_TRY(solve_enqueue(&C, d.velocity_result_grad, lam, D, &adv, dev, (int *)(dev + 4 * sb),
+                           ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, s, w.solve_ws[0], w.solve_ws_bytes[0]));
+    for (int c = 0; c < D; ++c)
+        PICT_TRY(outer_acc(C_tot, (const char *)lam + vec1 * c, (const char *)tape->velocity_star + vec1 * c, -1.0));
+    // ---- SetupAdvectionVelocity (rhs gradient = lambda) ----
+    PICT_TRY(copy_dev(d.velocity_rhs_grad, lam, vecD, s));
+    PICT_TRY(pict_setup_advection_velocity_grad(a, dt, 0, 0, stream));
+    PICT_TRY(block_grads(g_u, 2, 0));
+    // ---- SetupAdvectionMatrix (A and C gradients of every consumer) ----
+    PICT_TRY(copy_dev(d.A_grad, A_tot, vec1, s));
+    PICT_TRY(copy_dev(d.C_grad, C_tot, nzb, s));
+    PICT_TRY(block_grads(nullptr, 2, 1));
+    PICT_TRY(pict_setup_advection_matrix_grad(a, dt, 0, stream));
+    PICT_TRY(block_grads(g_u, 2, 0));
+    PICT_TRY(copy_dev(velocity_grad, g_u, vecD, s));
+#undef PICT_TRY
+#undef PICT_T
+    PICT_HIP_TRY(hipMemcpyAsync(w.host, dev, z.slot * z.nsolves, hipMemcpyDeviceToHost, s));
+    return PICT_OK;
+}
+
+// solver infos from the host copy of the solve slots
+pict_status collect(const void *host, const Sizes &z, const pict_step_opts *o, int D, pict_solve_info *infos_out,
+                    hipEvent_t *ev) {
+    const char *hp = (const char *)host;
+    const size_t sb = solve_state_bytes();
     int err = 0;
-    for (int i = 0; i < nsolves; ++i) err |= *(const int *)(hp + slot * i + 4 * sb);
+    for (int i = 0; i < z.nsolves; ++i) err |= *(const int *)(hp + z.slot * i + 4 * sb);
     if (err) {
         set_error("solver grid barrier exceeded its spin limit");
         return PICT_ERR_DEVICE_SYNC;
     }
     solve_state_to_info(hp, D, infos_out);
-    for (int k = 0; k < K; ++k) solve_state_to_info(hp + slot * (1 + k), 1, infos_out + D + k);
+    for (int k = 0; k < o->corrector_steps; ++k) solve_state_to_info(hp + z.slot * (1 + k), 1, infos_out + D + k);
     if (ev) {
-        g_step_timing.ms.assign(nsolves, 0.0);
-        for (int i = 0; i < nsolves; ++i) {
+        g_step_timing.ms.assign(z.nsolves, 0.0);
+        for (int i = 0; i < z.nsolves; ++i) {
             float ms = 0.f;
             PICT_HIP_TRY(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
             g_step_timing.ms[i] = ms;
@@ -270,8 +406,218 @@ pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const p
     return PICT_OK;
 }
 
-// Reverse pass of one pict_piso_step (the adjoint of the orthogonal step; the order in which
-// PISOtorch_diff's autograd graph visits the stages, PISOtorch_diff.py:295-1234).
+// ---------------------------------------------------------------------------
+// Graphs: the queued work of a step or reverse pass is captured once and replayed.  Its
+// short kernels (2-30 us) otherwise wait on the host's launches (about 10 us each); a
+// replay dispatches them about 2 us apart.  Everything a capture bakes in is in the key:
+// the domain-level descriptor (vector addresses, sizes, viscosity), the options, the tape
+// and gradient addresses and the transpose map.  Block tensors are read through the
+// atlas's device block table at run time, so swapping them needs no new capture.  Each
+// graph owns its workspaces (one device arena) and the pinned buffer of its solver states;
+// every replay is followed by a host wait, so an entry can be replaced at any call.
+// ---------------------------------------------------------------------------
+}  // namespace
+
+namespace pict {
+
+struct StepKey {
+    int kind;  // 0 step, 1 step recording a tape, 2 reverse pass
+    int device;
+    pict_domain_desc d;
+    pict_step_opts o;
+    pict_step_tape t;
+    const void *vgrad;
+    const int32_t *tperm;
+};
+
+struct GraphEntry {
+    bool valid = false;
+    StepKey key;
+    hipGraphExec_t exec = nullptr;
+    void *arena = nullptr;
+    void *host = nullptr;
+    Sizes z;
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (arena) (void)hipFree(arena);
+        if (host) (void)hipHostFree(host);
+        exec = nullptr;
+        arena = host = nullptr;
+        valid = false;
+    }
+};
+
+struct GraphCache {
+    GraphEntry e[3];
+    StepKey miss[3];       // the last key of each kind that had no graph
+    bool miss_valid[3] = {};
+    hipStream_t capture = nullptr;
+    ~GraphCache() {
+        for (GraphEntry &g : e) g.release();
+        if (capture) (void)hipStreamDestroy(capture);
+    }
+};
+
+}  // namespace pict
+
+namespace {
+
+bool graphs_enabled() {
+    const char *e = getenv("PICT_STEP_GRAPHS");
+    return !e || atoi(e) != 0;
+}
+
+StepKey key_of(const pict_atlas *a, int kind, const pict_step_opts *o, const pict_step_tape *tape, const void *vgrad) {
+    StepKey k;
+    memset(&k, 0, sizeof k);
+    k.kind = kind;
+    k.device = a->device;
+    k.d = a->desc;
+    k.d.blocks = nullptr;
+    k.o = *o;
+    if (tape) k.t = *tape;
+    k.vgrad = vgrad;
+    k.tperm = kind == 2 ? a->tperm : nullptr;
+    return k;
+}
+
+// The graph of `kind` for this call; null when the key is new: a key is captured when it comes
+// twice in a row, so calls whose addresses change every time (a new tape per step) run directly.
+pict_status graph_for(const pict_atlas *a, int kind, const pict_step_opts *o, const pict_step_tape *tape,
+                      void *vgrad, GraphEntry *&out) {
+    out = nullptr;
+    if (!a->step_graphs) a->step_graphs = new GraphCache();
+    GraphCache &c = *a->step_graphs;
+    GraphEntry &g = c.e[kind];
+    const StepKey key = key_of(a, kind, o, tape, vgrad);
+    if (g.valid && memcmp(&g.key, &key, sizeof key) == 0) {
+        out = &g;
+        return PICT_OK;
+    }
+    if (!(c.miss_valid[kind] && memcmp(&c.miss[kind], &key, sizeof key) == 0)) {
+        c.miss[kind] = key;
+        c.miss_valid[kind] = true;
+        return PICT_OK;
+    }
+    c.miss_valid[kind] = false;
+    g.release();
+    if (!c.capture) PICT_HIP_TRY(hipStreamCreateWithFlags(&c.capture, hipStreamNonBlocking));
+    const bool vjp = kind == 2;
+    const Sizes z = sizes_of(a, o, vjp);
+    pict_csr_desc C, P;
+    pict_solve_opts adv, prs;
+    solve_systems(a, o, vjp, C, P, adv, prs);
+    size_t wsb[65];
+    size_t total = z.dev_bytes;
+    pict_status st = solve_workspace_bytes(&C, a->desc.num_dims, &adv, &wsb[0]);
+    if (st) return st;
+    for (int k = 0; k < o->corrector_steps; ++k)
+        if ((st = solve_workspace_bytes(&P, 1, &prs, &wsb[1 + k]))) return st;
+    for (int i = 0; i < z.nsolves; ++i) total += (wsb[i] + 255) & ~size_t(255);
+    PICT_HIP_TRY(hipMalloc(&g.arena, total));
+    PICT_HIP_TRY(hipHostMalloc(&g.host, z.slot * z.nsolves, hipHostMallocDefault));
+    Work w;
+    w.dev = (char *)g.arena;
+    char *p = w.dev + z.dev_bytes;
+    for (int i = 0; i < z.nsolves; ++i) {
+        w.solve_ws[i] = p;
+        w.solve_ws_bytes[i] = wsb[i];
+        p += (wsb[i] + 255) & ~size_t(255);
+    }
+    w.host = g.host;
+    hipGraph_t graph = nullptr;
+    PICT_HIP_TRY(hipStreamBeginCapture(c.capture, hipStreamCaptureModeThreadLocal));
+    st = vjp ? enqueue_vjp(a, o, tape, vgrad, w, z, c.capture) : enqueue_step(a, o, tape, w, z, c.capture);
+    const hipError_t ce = hipStreamEndCapture(c.capture, &graph);
+    if (st || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        g.release();
+        if (st) return st;
+        set_error("step capture failed: %s", hipGetErrorString(ce));
+        return PICT_ERR_HIP;
+    }
+    const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        g.release();
+        set_error("step graph instantiation failed: %s", hipGetErrorString(ie));
+        return PICT_ERR_HIP;
+    }
+    g.key = key;
+    g.z = z;
+    g.valid = true;
+    out = &g;
+    return PICT_OK;
+}
+
+// one step or reverse pass: replayed graph, or (timing, graphs off) queued directly
+pict_status run(const pict_atlas *a, int kind, const pict_step_opts *o, const pict_step_tape *tape, void *vgrad,
+                pict_solve_info *infos_out, hipStream_t s) {
+    const bool vjp = kind == 2;
+    hipEvent_t *ev = nullptr;
+    const Sizes z = sizes_of(a, o, vjp);
+    pict_status st = timing_events(z.nsolves, ev);
+    if (st) return st;
+    if (!ev && graphs_enabled()) {
+        GraphEntry *g = nullptr;
+        if ((st = graph_for(a, kind, o, tape, vgrad, g))) return st;
+        if (g) {
+            PICT_HIP_TRY(hipGraphLaunch(g->exec, s));
+            PICT_HIP_TRY(hipStreamSynchronize(s));
+            return collect(g->host, g->z, o, a->desc.num_dims, infos_out, nullptr);
+        }
+    }
+    char *dev = nullptr;
+    PICT_HIP_TRY(ws_alloc((void **)&dev, z.dev_bytes, s));
+    struct Guard {
+        char *p; hipStream_t s;
+        ~Guard() { if (p) (void)ws_free(p, s); }
+    } guard{dev, s};
+    HostStates &h = g_host_states;
+    if (h.bytes < z.slot * z.nsolves) {
+        if (h.p) PICT_HIP_TRY(hipHostFree(h.p));
+        h.p = nullptr;
+        PICT_HIP_TRY(hipHostMalloc(&h.p, z.slot * z.nsolves, hipHostMallocDefault));
+        h.bytes = z.slot * z.nsolves;
+    }
+    Work w;
+    w.dev = dev;
+    w.host = h.p;
+    w.ev = ev;
+    st = vjp ? enqueue_vjp(a, o, tape, vgrad, w, z, s) : enqueue_step(a, o, tape, w, z, s);
+    if (st) return st;
+    PICT_HIP_TRY(hipStreamSynchronize(s));
+    return collect(h.p, z, o, a->desc.num_dims, infos_out, ev);
+}
+
+}  // namespace
+
+namespace pict {
+void step_graphs_free(pict_atlas *a) {
+    delete a->step_graphs;
+    a->step_graphs = nullptr;
+}
+}  // namespace pict
+
+extern "C" {
+
+pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
+                           pict_solve_info *infos_out, void *stream) {
+    PICT_CHECK_ARG(a && o && infos_out, "NULL argument");
+    const pict_domain_desc &d = a->desc;
+    PICT_CHECK_ARG(d.C_value && d.P_value && d.A && d.velocity_rhs && d.velocity_result && d.pressure_rhs &&
+                       d.pressure_rhs_div && d.pressure_result,
+                   "domain tensors missing");
+    PICT_CHECK_ARG(o->corrector_steps >= 0 && o->corrector_steps <= 64, "corrector_steps must be 0..64");
+    PICT_CHECK_ARG(o->advection.transpose_a == 0 && o->pressure.transpose_a == 0, "transposed solves in a step");
+    if (tape)
+        PICT_CHECK_ARG(tape->C_value && tape->A && tape->P_value && tape->velocity_star &&
+                           (o->corrector_steps == 0 || (tape->pressure_rhs && tape->pressure_raw && tape->velocity_result)),
+                       "tape buffers missing");
+    return run(a, tape ? 1 : 0, o, tape, nullptr, infos_out, (hipStream_t)stream);
+}
+
+// Reverse pass of one pict_piso_step (the adjoint of the orthogonal step).
 pict_status pict_piso_step_vjp(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
                                void *velocity_grad, pict_solve_info *infos_out, void *stream) {
     PICT_CHECK_ARG(a && o && tape && velocity_grad && infos_out, "NULL argument");
@@ -291,48 +637,7 @@ pict_status pict_piso_step_vjp(const pict_atlas *a, const pict_step_opts *o, con
                        (o->corrector_steps == 0 || (tape->pressure_rhs && tape->pressure_raw && tape->velocity_result)),
                    "tape buffers missing");
     hipStream_t s = (hipStream_t)stream;
-    const int D = d.num_dims, N = d.num_cells, K = o->corrector_steps;
-    const bool f32 = d.dtype == PICT_DTYPE_F32;
-    const size_t es = f32 ? sizeof(float) : sizeof(double);
-    const size_t vecD = es * D * N, vec1 = es * N, nzb = es * d.nnz;
-    const size_t sb = solve_state_bytes();
-    const int nsolves = 1 + K;
-    const size_t slot = ((4 * sb + sizeof(int)) + 255) & ~size_t(255);
-    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    // workspace: states | mean partials | g_u [D*N] | A_tot [N] | C_tot [nnz] | lambda [D*N]
-    const size_t o_part = slot * nsolves, o_gu = o_part + al(sizeof(double) * kMeanBlocks), o_at = o_gu + al(vecD),
-                 o_ct = o_at + al(vec1), o_lam = o_ct + al(nzb), bytes = o_lam + al(vecD);
-    char *dev = nullptr;
-    PICT_HIP_TRY(ws_alloc((void **)&dev, bytes, s));
-    struct Guard {
-        char *p; hipStream_t s;
-        ~Guard() { if (p) (void)ws_free(p, s); }
-    } guard{dev, s};
-    double *part = (double *)(dev + o_part);
-    void *g_u = dev + o_gu, *A_tot = dev + o_at, *C_tot = dev + o_ct, *lam = dev + o_lam;
-    const double dt = o->time_step;
-    hipEvent_t *ev = nullptr;
-    pict_status st = timing_events(nsolves, ev);
-    if (st) return st;
-#define PICT_TRY(expr) do { if ((st = (expr))) return st; } while (0)
-#define PICT_T(fn, ...) (f32 ? fn<float>(__VA_ARGS__) : fn<double>(__VA_ARGS__))
-    auto block_grads = [&](void *gv, int field, int mode) -> pict_status {
-        if (f32) PICT_LAUNCH_CELLS(a, stream, (k_block_grads<float>), a->view<float>(), (float *)gv, field, mode);
-        else PICT_LAUNCH_CELLS(a, stream, (k_block_grads<double>), a->view<double>(), (double *)gv, field, mode);
-        return check_launch();
-    };
-    auto outer_acc = [&](void *out, const void *av, const void *bv, double alpha) -> pict_status {
-        const unsigned blocks = (unsigned)((N + 255) / 256);
-        if (f32)
-            hipLaunchKernelGGL(k_outer_acc<float>, dim3(blocks), dim3(256), 0, s, (float *)out, (const float *)av,
-                               (const float *)bv, d.csr_row, d.csr_index, N, (float)alpha);
-        else
-            hipLaunchKernelGGL(k_outer_acc<double>, dim3(blocks), dim3(256), 0, s, (double *)out, (const double *)av,
-                               (const double *)bv, d.csr_row, d.csr_index, N, alpha);
-        return check_launch();
-    };
     // transposed matrices share the pattern's A^T map, built on the atlas's first reverse pass
-    pict_csr_desc C = csr_of(a, d.C_value), P = csr_of(a, d.P_value);
     if (!a->tperm || a->tperm_row != d.csr_row || a->tperm_index != d.csr_index) {
         if (a->tperm) {
             PICT_HIP_TRY(hipStreamSynchronize(s));
@@ -340,105 +645,13 @@ pict_status pict_piso_step_vjp(const pict_atlas *a, const pict_step_opts *o, con
             a->tperm = nullptr;
         }
         PICT_HIP_TRY(hipMalloc((void **)&a->tperm, sizeof(int32_t) * (size_t)(d.nnz > 0 ? d.nnz : 1)));
-        PICT_TRY(pict_csr_transpose_permutation(&C, a->tperm, stream));
+        pict_csr_desc C = csr_of(a, d.C_value);
+        pict_status st = pict_csr_transpose_permutation(&C, a->tperm, stream);
+        if (st) return st;
         a->tperm_row = d.csr_row;
         a->tperm_index = d.csr_index;
     }
-    C.transpose_perm = a->tperm;
-    P.transpose_perm = a->tperm;
-    pict_solve_opts adv = o->advection, prs = o->pressure;
-    adv.transpose_a = 1;
-    prs.transpose_a = 1;
-    adv.zero_rhs_shortcut = 1;
-    prs.zero_rhs_shortcut = 1;
-    prs.return_best_result = o->pressure.return_best_result;
-
-    // forward state the VJP kernels read
-    PICT_TRY(copy_dev(d.C_value, tape->C_value, nzb, s));
-    PICT_TRY(copy_dev(d.A, tape->A, vec1, s));
-    PICT_TRY(copy_dev(d.P_value, tape->P_value, nzb, s));
-    PICT_TRY(PICT_T(fill, g_u, (size_t)D * N, 0.0, s));
-    PICT_TRY(PICT_T(fill, A_tot, (size_t)N, 0.0, s));
-    PICT_TRY(PICT_T(fill, C_tot, (size_t)d.nnz, 0.0, s));
-    // CopyVelocityResultToBlocks: the output block-velocity gradient is the velocity-result gradient
-    PICT_TRY(copy_dev(d.velocity_result_grad, velocity_grad, vecD, s));
-    for (int k = K - 1; k >= 0; --k) {
-        char *sl = dev + slot * (1 + k);
-        // ---- CorrectVelocity (inputs A, block pressure p^_k, pressure RHS) ----
-        PICT_TRY(copy_dev(d.pressure_result, (const char *)tape->pressure_raw + vec1 * k, vec1, s));
-        if (o->normalize_pressure) PICT_TRY(PICT_T(subtract_mean, d.pressure_result, N, part, s));
-        PICT_TRY(pict_copy_pressure_result_to_blocks(a, stream));
-        PICT_TRY(block_grads(nullptr, 3, 1));
-        PICT_TRY(pict_correct_velocity_grad(a, dt, o->time_step_norm, stream));
-        PICT_TRY(PICT_T(axpy, A_tot, d.A_grad, (size_t)N, 1.0, s));
-        // ---- CopyPressureResultToBlocks, mean subtraction, pressure solve ----
-        PICT_TRY(pict_copy_pressure_result_grad_from_blocks(a, stream));
-        if (o->normalize_pressure) PICT_TRY(PICT_T(subtract_mean, d.pressure_result_grad, N, part, s));
-        PICT_TRY(PICT_T(fill, d.pressure_rhs_div_grad, (size_t)N, 0.0, s));
-        PICT_TRY(solve_enqueue(&P, d.pressure_result_grad, d.pressure_rhs_div_grad, 1, &prs, sl,
-                               (int *)(sl + 4 * sb), ev ? ev[2 * (1 + k)] : nullptr,
-                               ev ? ev[2 * (1 + k) + 1] : nullptr, s));
-        PICT_TRY(PICT_T(fill, d.P_grad, (size_t)d.nnz, 0.0, s));
-        PICT_TRY(outer_acc(d.P_grad, d.pressure_rhs_div_grad, (const char *)tape->pressure_raw + vec1 * k, -1.0));
-        // ---- SetupPressureCorrection (inputs velocity, source, velocity result, boundaries, viscosity, A, C) ----
-        PICT_TRY(copy_dev(d.velocity_result, k == 0 ? tape->velocity_star
-                                                     : (const char *)tape->velocity_result + vecD * (k - 1), vecD, s));
-        PICT_TRY(copy_dev(d.pressure_rhs, (const char *)tape->pressure_rhs + vecD * k, vecD, s));
-        PICT_TRY(PICT_T(fill, d.velocity_result_grad, (size_t)D * N, 0.0, s));
-        PICT_TRY(PICT_T(fill, d.A_grad, (size_t)N, 0.0, s));
-        PICT_TRY(pict_setup_pressure_correction_grad(a, dt, 0, 0, o->time_step_norm, stream));
-        PICT_TRY(PICT_T(axpy, A_tot, d.A_grad, (size_t)N, 1.0, s));
-        PICT_TRY(PICT_T(axpy, C_tot, d.C_grad, (size_t)d.nnz, 1.0, s));
-        PICT_TRY(block_grads(g_u, 2, 0));
-    }
-    // ---- advection solve C u* = rhs: lambda = C^-T g, C_grad -= lambda u*^T ----
-    PICT_TRY(PICT_T(fill, lam, (size_t)D * N, 0.0, s));
-    PICT_TRY(solve_enqueue(&C, d.velocity_result_grad, lam, D, &adv, dev, (int *)(dev + 4 * sb),
-                           ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, s));
-    for (int c = 0; c < D; ++c)
-        PICT_TRY(outer_acc(C_tot, (const char *)lam + vec1 * c, (const char *)tape->velocity_star + vec1 * c, -1.0));
-    // ---- SetupAdvectionVelocity (rhs gradient = lambda) ----
-    PICT_TRY(copy_dev(d.velocity_rhs_grad, lam, vecD, s));
-    PICT_TRY(pict_setup_advection_velocity_grad(a, dt, 0, 0, stream));
-    PICT_TRY(block_grads(g_u, 2, 0));
-    // ---- SetupAdvectionMatrix (A and C gradients of every consumer) ----
-    PICT_TRY(copy_dev(d.A_grad, A_tot, vec1, s));
-    PICT_TRY(copy_dev(d.C_grad, C_tot, nzb, s));
-    PICT_TRY(block_grads(nullptr, 2, 1));
-    PICT_TRY(pict_setup_advection_matrix_grad(a, dt, 0, stream));
-    PICT_TRY(block_grads(g_u, 2, 0));
-    PICT_TRY(copy_dev(velocity_grad, g_u, vecD, s));
-#undef PICT_TRY
-#undef PICT_T
-
-    // one wait: the adjoint solvers' final states
-    HostStates &h = g_host_states;
-    if (h.bytes < slot * nsolves) {
-        if (h.p) PICT_HIP_TRY(hipHostFree(h.p));
-        h.p = nullptr;
-        PICT_HIP_TRY(hipHostMalloc(&h.p, slot * nsolves, hipHostMallocDefault));
-        h.bytes = slot * nsolves;
-    }
-    PICT_HIP_TRY(hipMemcpyAsync(h.p, dev, slot * nsolves, hipMemcpyDeviceToHost, s));
-    PICT_HIP_TRY(hipStreamSynchronize(s));
-    const char *hp = (const char *)h.p;
-    int err = 0;
-    for (int i = 0; i < nsolves; ++i) err |= *(const int *)(hp + slot * i + 4 * sb);
-    if (err) {
-        set_error("solver grid barrier exceeded its spin limit");
-        return PICT_ERR_DEVICE_SYNC;
-    }
-    solve_state_to_info(hp, D, infos_out);
-    for (int k = 0; k < K; ++k) solve_state_to_info(hp + slot * (1 + k), 1, infos_out + D + k);
-    if (ev) {
-        g_step_timing.ms.assign(nsolves, 0.0);
-        for (int i = 0; i < nsolves; ++i) {
-            float ms = 0.f;
-            PICT_HIP_TRY(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
-            g_step_timing.ms[i] = ms;
-        }
-    }
-    return PICT_OK;
+    return run(a, 2, o, tape, velocity_grad, infos_out, s);
 }
 
 pict_status pict_subtract_mean(void *v, int32_t n, int32_t dtype, void *stream) {
