@@ -1049,10 +1049,25 @@ static void spmv(const Mat *m, const real *x, real *y) {
             for (int k = m->row[i]; k < m->row[i + 1]; ++k) y[m->col[k]] += m->val[k] * x[i];
     }
 }
+/* FP64 dot product in a fixed order whatever the thread count: 4096-element blocks summed
+   sequentially, then the block sums in block order (an OpenMP reduction combines the threads'
+   sums in an unspecified order, which makes iteration counts vary from run to run). */
 static double dotd(const real *a, const real *b, int n) {
+    enum { BLK = 4096 };
+    const int nb = (n + BLK - 1) / BLK;
+    double stack_sums[64];
+    double *bs = nb <= 64 ? stack_sums : (double *)malloc(sizeof(double) * (size_t)nb);
+    if (!bs) return NAN;
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < nb; ++k) {
+        const int e = (k + 1) * BLK < n ? (k + 1) * BLK : n;
+        double t = 0;
+        for (int i = k * BLK; i < e; ++i) t += (double)a[i] * (double)b[i];
+        bs[k] = t;
+    }
     double s = 0;
-    #pragma omp parallel for reduction(+ : s) schedule(static)
-    for (int i = 0; i < n; ++i) s += (double)a[i] * (double)b[i];
+    for (int k = 0; k < nb; ++k) s += bs[k];
+    if (bs != stack_sums) free(bs);
     return s;
 }
 /* ComputeConvergenceCriterion, cg_solver_kernel.cu:77-110 */
