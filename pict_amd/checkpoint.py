@@ -194,6 +194,49 @@ def _from_global(v, like, D):
     return out
 
 
+class _Work:
+    """Private block velocity and pressure tensors a domain steps in place during native checkpointed rollouts.
+    Installed once, their addresses stay in the domain's descriptor, so a step needs no repacking; step inputs
+    and outputs are copied in and out.  `last` holds the outputs (autograd tensors) of the last native step whose
+    values the buffers hold."""
+
+    def __init__(self, domain):
+        blocks = domain.getBlocks()
+        self.vel = [torch.empty_like(b.velocity, requires_grad=False) for b in blocks]
+        self.pres = [torch.empty_like(b.pressure, requires_grad=False) for b in blocks]
+        self.last = None
+
+    def matches(self, domain):
+        return all(w.shape == b.velocity.shape and w.dtype == b.velocity.dtype and w.device == b.velocity.device
+                   for w, b in zip(self.vel, domain.getBlocks())) and len(self.vel) == len(domain.getBlocks())
+
+    def installed(self, domain):
+        return all(b.velocity is v and b.pressure is p for b, v, p in zip(domain.getBlocks(), self.vel, self.pres))
+
+    def install(self, domain):
+        """Make the domain step in these buffers (their values are set by the caller)."""
+        for b, v, p in zip(domain.getBlocks(), self.vel, self.pres):
+            b.setVelocity(v)
+            b.setPressure(p)
+        domain.UpdateDomainData()
+        self.last = None
+
+
+_WORK = weakref.WeakKeyDictionary()  # domain -> _Work
+
+
+def _work(domain):
+    w = _WORK.get(domain)
+    if w is None or not w.matches(domain):
+        w = _Work(domain)
+        _WORK[domain] = w
+    return w
+
+
+def _same_storage(a, b):
+    return a is b or (a.data_ptr() == b.data_ptr() and a.shape == b.shape and a.dtype == b.dtype)
+
+
 def _native_forward(sim, tape):
     """One native step (recording `tape`)."""
     return sim._native_steps(1, sim._get_time_step_torch(), tape=tape.desc if tape is not None else None)
@@ -241,13 +284,8 @@ class _PISOStep(torch.autograd.Function):
         domain = sim.domain
         was_diff = sim.differentiable
         reuse = (sim.advect_reuse_result, sim.pressure_reuse_result)
+        vin = tensors[:n_blocks]
         with torch.no_grad():
-            for b, v in zip(domain.getBlocks(), tensors[:n_blocks]):
-                b.setVelocity(v.detach().clone())
-                # the forward writes results into the block tensors in place: never into a tensor that an earlier
-                # step returned
-                b.setPressure(b.pressure.detach().clone())
-            domain.UpdateDomainData()
             # the differentiable recomputation starts every solve from zero; so does this forward
             sim.differentiable = False
             sim.advect_reuse_result = sim.pressure_reuse_result = False
@@ -255,17 +293,35 @@ class _PISOStep(torch.autograd.Function):
                 ctx.native = sim._native_step_eligible()
                 ctx.tape = None
                 if ctx.native:
+                    # step in the domain's private buffers: the native step does not read the pressure, and the
+                    # velocity is copied in unless the buffers already hold this input (the previous step's output)
+                    w = _work(domain)
+                    held = w.installed(domain) and w.last is not None and all(
+                        _same_storage(v, l) for v, l in zip(vin, w.last))
+                    if not w.installed(domain):
+                        w.install(domain)
+                    if not held:
+                        for wv, v in zip(w.vel, vin):
+                            wv.copy_(v)
                     ctx.tape = _Tape(domain, sim.corrector_steps) if keep_tape else None
                     ok = _native_forward(sim, ctx.tape)
+                    outs = [t.clone() for t in w.vel + w.pres]
                 else:
+                    for b, v in zip(domain.getBlocks(), vin):
+                        b.setVelocity(v.detach().clone())
+                        # the forward writes results into the block tensors in place: never into a tensor that an
+                        # earlier step returned
+                        b.setPressure(b.pressure.detach().clone())
+                    domain.UpdateDomainData()
                     ok = sim._PISO_split_step(1)
+                    outs = [b.velocity.detach().clone() for b in domain.getBlocks()]
+                    outs += [b.pressure.detach().clone() for b in domain.getBlocks()]
             finally:
                 sim.differentiable = was_diff
                 sim.advect_reuse_result, sim.pressure_reuse_result = reuse
         if not ok:
             raise RuntimeError("PISO step failed (linear solve did not converge)")
-        outs = [b.velocity.detach().clone() for b in domain.getBlocks()]
-        outs += [b.pressure.detach().clone() for b in domain.getBlocks()]
+        sim._checkpoint_native = ctx.native
         return tuple(outs)
 
     @staticmethod
@@ -283,20 +339,23 @@ class _PISOStep(torch.autograd.Function):
         saved = ctx.saved_tensors
         params = list(saved[n:])
         vel_grads = [torch.zeros_like(v) if g is None else g.contiguous() for g, v in zip(vel_grads, saved[:n])]
-        state = [(b.velocity, b.pressure) for b in domain.getBlocks()]
         counters = (sim.total_step, sim.total_time)
         tape = ctx.tape
+        # the recomputation and the reverse pass overwrite the block velocities and pressures: they run in the
+        # private buffers, whose values (or the domain's own tensors) are put back afterwards
+        w = _work(domain)
+        state = None if w.installed(domain) else [(b.velocity, b.pressure) for b in domain.getBlocks()]
+        last = w.last
         try:
             with torch.no_grad():
-                # the reverse pass overwrites the block pressures: let it write into scratch tensors
-                for b, (v, p) in zip(domain.getBlocks(), state):
-                    b.setPressure(torch.empty_like(p))
+                snapshot = [t.clone() for t in w.vel + w.pres] if state is None else None
+                if state is not None:
+                    w.install(domain)
                 if tape is None:
                     # recompute the step from its input, recording the tape
                     tape = _recompute_tape(domain, sim.corrector_steps)
-                    for b, v in zip(domain.getBlocks(), saved[:n]):
-                        b.setVelocity(v.detach().clone())
-                    domain.UpdateDomainData()
+                    for wv, v in zip(w.vel, saved[:n]):
+                        wv.copy_(v)
                     reuse = (sim.advect_reuse_result, sim.pressure_reuse_result)
                     sim.advect_reuse_result = sim.pressure_reuse_result = False
                     try:
@@ -308,10 +367,16 @@ class _PISOStep(torch.autograd.Function):
                 g_par = [gp if p.requires_grad else None for gp, p in zip(_param_grads(domain, params), params)]
         finally:
             sim.total_step, sim.total_time = counters
-            for b, (v, p) in zip(domain.getBlocks(), state):
-                b.setVelocity(v)
-                b.setPressure(p)
-            domain.UpdateDomainData()
+            with torch.no_grad():
+                if state is None:
+                    for t, v in zip(w.vel + w.pres, snapshot):
+                        t.copy_(v)
+                    w.last = last
+                else:
+                    for b, (v, p) in zip(domain.getBlocks(), state):
+                        b.setVelocity(v)
+                        b.setPressure(p)
+                    domain.UpdateDomainData()
         ctx.tape = None
         return (*g_vel, *g_par)
 
@@ -360,16 +425,27 @@ def checkpointed_step(sim, params=(), keep_tape=False):
     """One PISO step of `sim` whose gradient is computed by recomputing the step in backward (keep_tape=True on
     the native path: by keeping its tape).  `params` are tensors already set on the domain (e.g. a learnable
     block.velocitySource) whose gradients are wanted.  The block velocities on entry are the step's
-    differentiable input; on return the blocks hold the step's output velocities and pressures, which carry the
-    gradient path."""
+    differentiable input (after a native checkpointed step: that step's returned outputs).  Returns the output
+    velocities, which carry the gradient path.  On return the blocks hold the step's output velocities and
+    pressures: as those tensors themselves on the stage-by-stage path, as their values in private buffers the
+    domain keeps stepping in on the native path."""
     domain = sim.domain
     blocks = domain.getBlocks()
-    outs = _PISOStep.apply(sim, len(blocks), keep_tape, *[b.velocity for b in blocks], *params)
     n = len(blocks)
-    for b, v, p in zip(blocks, outs[:n], outs[n:]):
-        b.setVelocity(v)
-        b.setPressure(p)
-    domain.UpdateDomainData()
+    w = _WORK.get(domain)
+    if w is not None and w.last is not None and w.installed(domain):
+        vin = w.last  # the domain holds the values of the previous native step's outputs
+    else:
+        vin = [b.velocity for b in blocks]
+    outs = _PISOStep.apply(sim, n, keep_tape, *vin, *params)
+    if sim._checkpoint_native:
+        # the domain keeps stepping in its private buffers, which hold these outputs' values
+        _WORK[domain].last = list(outs[:n])
+    else:
+        for b, v, p in zip(blocks, outs[:n], outs[n:]):
+            b.setVelocity(v)
+            b.setPressure(p)
+        domain.UpdateDomainData()
     return outs[:n]
 
 
