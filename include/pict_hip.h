@@ -312,6 +312,20 @@ pict_status pict_solve_linear(const pict_csr_desc *A, const void *rhs, void *x, 
    (as a profiler shows it, e.g. "pict::k_cg_local<float, 4, 5, false>"),
    written NUL-terminated into buf[len].  No counterpart in the reference:
    it labels measurements of the solver kernel. */
+/* Mixed-precision solve of A x = b to a relative FP64 residual: x (FP64, initial guess in,
+   solution out; A and b in the matrix dtype).  Each sweep forms r = b - A x in FP64, solves
+   A d = r / rms(r) in the matrix dtype with `inner` (the CG/BiCGStab kernels of
+   pict_solve_linear) and adds rms(r) * d to x, until ||r||_2 / ||b||_2 < refine_tol, the
+   residual stops halving, or max_sweeps.  null_space: 1 = the constant vector spans A's null
+   space (pure Neumann / periodic pressure systems): the constant part of b and of every
+   residual is projected out, since no x can reduce it; 0 = none; -1 = detect it from the row
+   sums.  info_out: final relative residual, inner iterations in total, converged, finite;
+   `reserved` = sweeps.  No counterpart in the reference, whose pressure solve is one solve in
+   the matrix dtype: this is the FP64 refinement of the FP32 pressure solve. */
+pict_status pict_solve_refined(const pict_csr_desc *A, const void *rhs, double *x, const pict_solve_opts *inner,
+                               double refine_tol, int32_t max_sweeps, int32_t null_space, pict_solve_info *info_out,
+                               void *stream);
+
 pict_status pict_solver_kernel_name(const pict_csr_desc *A, int32_t nrhs, const pict_solve_opts *opts, char *buf,
                                     int32_t len);
 

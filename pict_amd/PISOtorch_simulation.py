@@ -74,7 +74,8 @@ class Simulation:
                  exclude_advection_solve_gradients=False, exclude_pressure_solve_gradients=False, log_dir=None,
                  log_interval=0, log_images=True, log_vtk=False, norm_vel=False, block_layout=None,
                  output_mode3D="slice", log_fn=None, output_resampling_coords=None, output_resampling_shape=None,
-                 output_resampling_fill_max_steps=0, save_domain_name=None, stop_fn=_never_stop, native_step=True):
+                 output_resampling_fill_max_steps=0, save_domain_name=None, stop_fn=_never_stop, native_step=True,
+                 pressure_refine_tol=None, pressure_refine_inner_tol=1e-5, pressure_refine_max_sweeps=8):
         if density_viscosity is not None:
             raise NotImplementedError("Simulation.density_viscosity is superseded by Domain.scalarViscosity.")
         if scipy_solve_advection or scipy_solve_pressure:
@@ -121,6 +122,12 @@ class Simulation:
         self.stop_fn = stop_fn
         # run eligible steps as one native call (pict_piso_step); not a reference option
         self.native_step = native_step
+        # FP64 iterative refinement of the pressure solve of a non-differentiable step to this relative residual
+        # (PISOtorch.SolveLinearRefined; not a reference option): each correction solve runs in the domain dtype
+        # to pressure_refine_inner_tol on its unit-RMS right-hand side
+        self.pressure_refine_tol = pressure_refine_tol
+        self.pressure_refine_inner_tol = pressure_refine_inner_tol
+        self.pressure_refine_max_sweeps = pressure_refine_max_sweeps
         self.solver_infos = []  # (kind, [LinearSolverResultInfo]) of the last step, for iteration-count logging
         self.record_solver_infos = False
         self.reset_step_counters()
@@ -306,16 +313,32 @@ class Simulation:
                                               BiCG_precondition_fallback=BiCG_precondition_fallback), True
 
     def linear_solve(self, csrMat, rhs, x=None, use_BiCG=False, tol=None, max_iter=None, matrix_rank_deficient=False,
-                     residual_reset_step=0, use_scipy=False, return_best_result=False):
+                     residual_reset_step=0, use_scipy=False, return_best_result=False, refine_tol=None):
         if use_scipy:
             raise NotImplementedError("the scipy solve path is not part of this build")
         if max_iter is None:
             max_iter = self.linear_solve_max_iterations
+        if refine_tol is not None and not self.differentiable:
+            return self._linear_solve_refined(csrMat, rhs, x, use_BiCG, refine_tol, max_iter)
         return self.linear_solve_GPU(csrMat, rhs, x, use_BiCG, tol, max_iter, matrix_rank_deficient,
                                      residual_reset_step, return_best_result,
                                      double_fallback=self.solver_double_fallback,
                                      BiCG_with_preconditioner=self.preconditionBiCG,
                                      BiCG_precondition_fallback=self.BiCG_precondition_fallback)
+
+    def _linear_solve_refined(self, csrMat, rhs, x, use_BiCG, refine_tol, max_iter):
+        """FP64-refined solve: the FP64 solution rounded to the domain dtype."""
+        x64 = torch.zeros(rhs.numel(), dtype=torch.float64, device=rhs.device) if x is None else \
+            x.detach().to(torch.float64).contiguous()
+        info = PISOtorch.SolveLinearRefined(csrMat, rhs.detach().contiguous(), x64, max_iter,
+                                            self.pressure_refine_inner_tol, refine_tol,
+                                            self.pressure_refine_max_sweeps, useBiCG=use_BiCG)
+        if self.record_solver_infos:
+            self.solver_infos.append(("BiCG" if use_BiCG else "CG", [info]))
+        if not (info.converged and info.isFiniteResidual):
+            raise PISOtorch_diff.LinsolveError("FP64-refined solve reached relative residual %.3e (target %.1e)"
+                                               % (info.finalResidual, refine_tol))
+        return x64.to(rhs.dtype), True
 
     # --- the PISO step (:1054-1327) ---
     def _PISO_split_step(self, iterations, time_step=None):
@@ -398,7 +421,8 @@ class Simulation:
                         pressureResult, solve_ok = self.linear_solve(domain.P, domain.pressureRHSdiv, x=None,
                                                                      use_BiCG=self.pressure_use_BiCG,
                                                                      tol=self.pressure_tol,
-                                                                     return_best_result=self.pressure_return_best_result)
+                                                                     return_best_result=self.pressure_return_best_result,
+                                                                     refine_tol=self.pressure_refine_tol)
                     if not solve_ok:
                         return solve_ok
                     if self.normalize_pressure_result:
@@ -423,7 +447,8 @@ class Simulation:
                             pressureResult, solve_ok = self.linear_solve(
                                 domain.P, domain.pressureRHSdiv, x=x, matrix_rank_deficient=False,
                                 residual_reset_step=100, use_BiCG=self.pressure_use_BiCG, tol=self.pressure_tol,
-                                return_best_result=self.pressure_return_best_result)
+                                return_best_result=self.pressure_return_best_result,
+                                refine_tol=self.pressure_refine_tol)
                         if self.normalize_pressure_result:
                             pressureResult = B._subtract_mean(pressureResult)
                         domain.setPressureResult(pressureResult)
@@ -465,7 +490,7 @@ class Simulation:
                 and not (self.advect_passive_scalar and d.hasPassiveScalar())
                 and not self._prep_fn and self.stop_fn is _never_stop and self.convergence_tol is None
                 and not self.advect_reuse_result and not self.pressure_reuse_result
-                and not self.solver_double_fallback and not self.preconditionBiCG
+                and not self.solver_double_fallback and not self.preconditionBiCG and self.pressure_refine_tol is None
                 and self._velocity_corrector_version in (0, 1) and d.getDevice().type == "cuda")
 
     def _native_solve_opts(self, use_BiCG, tol, return_best):

@@ -1495,6 +1495,79 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) a.st[0] = st;
 }
 
+// ==========================================================================
+// Mixed-precision iterative refinement (no counterpart in the reference, whose pressure
+// solve is a single CG in the matrix dtype): r = b - A x in FP64 with x kept in FP64,
+// the correction solved in the matrix dtype by the Krylov kernels above.
+// ==========================================================================
+constexpr int kRefineBlocks = 256;
+
+// r = b - A x (FP64); part[4*blk + {0,1,2,3}] = sum r, sum r^2, sum b, sum b^2 of this workgroup's
+// rows (grid-stride rows, fixed order: the host sums the parts in order)
+template <typename R>
+__global__ __launch_bounds__(256) void k_residual64(const int32_t *row, const int32_t *col, const R *val,
+                                                    const R *b, const double *x, int n, double *r, double *part) {
+    __shared__ double lds[4][4];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double bi = (double)b[i];
+        double s = bi;
+        for (int e = row[i]; e < row[i + 1]; ++e) s -= (double)val[e] * x[col[e]];
+        r[i] = s;
+        acc[0] += s;
+        acc[1] += s * s;
+        acc[2] += bi;
+        acc[3] += bi * bi;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0) lds[k][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] =
+        lds[threadIdx.x][0] + lds[threadIdx.x][1] + lds[threadIdx.x][2] + lds[threadIdx.x][3];
+}
+
+// largest |sum of a row| / max |entry of the row| over this workgroup's rows: ~0 when every row sums to
+// zero, i.e. the constant vector spans a null space (pure Neumann or periodic pressure systems)
+template <typename R>
+__global__ __launch_bounds__(256) void k_row_sum_ratio(const int32_t *row, const R *val, int n, double *part) {
+    __shared__ double lds[4];
+    double m = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        double s = 0.0, a = 0.0;
+        for (int e = row[i]; e < row[i + 1]; ++e) {
+            s += (double)val[e];
+            a = fmax(a, fabs((double)val[e]));
+        }
+        m = fmax(m, a > 0.0 ? fabs(s) / a : 1.0);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+}
+
+// out = scale * (r - shift) (rounded to the matrix dtype), d = 0
+template <typename R>
+__global__ __launch_bounds__(256) void k_refine_rhs(const double *r, double shift, double scale, R *out, R *d,
+                                                   int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        out[i] = (R)((r[i] - shift) * scale);
+        d[i] = R(0);
+    }
+}
+
+// x += scale * d (FP64)
+template <typename R>
+__global__ __launch_bounds__(256) void k_refine_update(double *x, const R *d, double scale, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) x[i] += scale * (double)d[i];
+}
+
 // zero n 64-bit words (the tagged regions of k_cg_local, before its launch)
 __global__ void k_zero64(u64 *p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
@@ -2155,6 +2228,108 @@ pict_status pict_solver_kernel_name(const pict_csr_desc *A, int32_t nrhs, const 
         snprintf(buf, (size_t)len, "pict::k_cg<%s, %d>", t, nr);
     else
         snprintf(buf, (size_t)len, "pict::k_cg1<%s, %d>", t, nr);
+    return PICT_OK;
+}
+
+pict_status pict_solve_refined(const pict_csr_desc *A, const void *rhs, double *x, const pict_solve_opts *inner,
+                               double refine_tol, int32_t max_sweeps, int32_t null_space, pict_solve_info *info_out,
+                               void *stream) {
+    PICT_CHECK_ARG(A && rhs && x && inner && info_out, "NULL argument");
+    pict_status st = validate_solve(A, 1, inner);
+    if (st) return st;
+    PICT_CHECK_ARG(!inner->transpose_a, "transposed refined solves are not supported");
+    PICT_CHECK_ARG(refine_tol > 0.0 && max_sweeps >= 0, "refine_tol must be > 0 and max_sweeps >= 0");
+    PICT_CHECK_ARG(null_space >= -1 && null_space <= 1, "null_space must be -1 (detect), 0 or 1");
+    hipStream_t s = (hipStream_t)stream;
+    const int n = A->n;
+    const bool f32 = A->dtype == PICT_DTYPE_F32;
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_d = al(sizeof(double) * n), o_rr = o_d + al(es * n), o_part = o_rr + al(es * n),
+                 bytes = o_part + sizeof(double) * 4 * kRefineBlocks;
+    char *ws = nullptr;
+    PICT_HIP_TRY(ws_alloc((void **)&ws, bytes, s));
+    struct Guard {
+        char *p; hipStream_t s;
+        ~Guard() { if (p) (void)ws_free(p, s); }
+    } guard{ws, s};
+    double *r = (double *)ws, *part = (double *)(ws + o_part);
+    void *d = ws + o_d, *rr_vec = ws + o_rr;
+    const unsigned blocks = (unsigned)std::min<long long>(((long long)n + 255) / 256, kRefineBlocks);
+    std::vector<double> hp(4 * kRefineBlocks);
+    // the constant null space: given, or detected from the row sums
+    bool project = null_space == 1;
+    if (null_space == -1) {
+        if (f32)
+            hipLaunchKernelGGL(k_row_sum_ratio<float>, dim3(blocks), dim3(256), 0, s, A->row, (const float *)A->value,
+                               n, part);
+        else
+            hipLaunchKernelGGL(k_row_sum_ratio<double>, dim3(blocks), dim3(256), 0, s, A->row,
+                               (const double *)A->value, n, part);
+        PICT_HIP_TRY(hipGetLastError());
+        PICT_HIP_TRY(hipMemcpyAsync(hp.data(), part, sizeof(double) * blocks, hipMemcpyDeviceToHost, s));
+        PICT_HIP_TRY(hipStreamSynchronize(s));
+        double m = 0.0;
+        for (unsigned k = 0; k < blocks; ++k) m = std::max(m, hp[k]);
+        project = m < (f32 ? 1e-5 : 1e-12);
+    }
+    double rel = 0.0, prev = 0.0;
+    int inner_its = 0, sweep = 0;
+    bool converged = false, finite = true;
+    pict_solve_info ii;
+    for (;; ++sweep) {
+        if (f32)
+            hipLaunchKernelGGL(k_residual64<float>, dim3(blocks), dim3(256), 0, s, A->row, A->index,
+                               (const float *)A->value, (const float *)rhs, (const double *)x, n, r, part);
+        else
+            hipLaunchKernelGGL(k_residual64<double>, dim3(blocks), dim3(256), 0, s, A->row, A->index,
+                               (const double *)A->value, (const double *)rhs, (const double *)x, n, r, part);
+        PICT_HIP_TRY(hipGetLastError());
+        PICT_HIP_TRY(hipMemcpyAsync(hp.data(), part, sizeof(double) * 4 * blocks, hipMemcpyDeviceToHost, s));
+        PICT_HIP_TRY(hipStreamSynchronize(s));
+        double sr = 0.0, rr = 0.0, sb = 0.0, bb = 0.0;
+        for (unsigned k = 0; k < blocks; ++k) {
+            sr += hp[4 * k];
+            rr += hp[4 * k + 1];
+            sb += hp[4 * k + 2];
+            bb += hp[4 * k + 3];
+        }
+        // with a constant null space only the part of r (and b) orthogonal to it can be solved for
+        const double shift = project ? sr / n : 0.0;
+        if (project) {
+            rr = std::max(rr - sr * sr / n, 0.0);
+            bb = std::max(bb - sb * sb / n, 0.0);
+        }
+        rel = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+        finite = std::isfinite(rel);
+        if (!finite || rel < refine_tol) {
+            converged = finite;
+            break;
+        }
+        // stop when the sweeps no longer reduce the residual (the inner solver's precision floor)
+        if (sweep >= max_sweeps || (sweep > 0 && rel > 0.5 * prev)) break;
+        prev = rel;
+        const double rms = sqrt(rr / (double)n);
+        if (f32)
+            hipLaunchKernelGGL(k_refine_rhs<float>, dim3(blocks), dim3(256), 0, s, (const double *)r, shift,
+                               1.0 / rms, (float *)rr_vec, (float *)d, n);
+        else
+            hipLaunchKernelGGL(k_refine_rhs<double>, dim3(blocks), dim3(256), 0, s, (const double *)r, shift,
+                               1.0 / rms, (double *)rr_vec, (double *)d, n);
+        PICT_HIP_TRY(hipGetLastError());
+        if ((st = solve_sync(A, rr_vec, d, 1, inner, &ii, s))) return st;
+        inner_its += ii.used_iterations + 1;
+        if (f32)
+            hipLaunchKernelGGL(k_refine_update<float>, dim3(blocks), dim3(256), 0, s, x, (const float *)d, rms, n);
+        else
+            hipLaunchKernelGGL(k_refine_update<double>, dim3(blocks), dim3(256), 0, s, x, (const double *)d, rms, n);
+        PICT_HIP_TRY(hipGetLastError());
+    }
+    info_out->final_residual = rel;
+    info_out->used_iterations = inner_its;
+    info_out->converged = converged ? 1 : 0;
+    info_out->is_finite_residual = finite ? 1 : 0;
+    info_out->reserved = sweep;
     return PICT_OK;
 }
 
