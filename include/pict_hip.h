@@ -387,6 +387,55 @@ pict_status pict_subtract_mean(void *v, int32_t n, int32_t dtype, void *stream);
 pict_status pict_set_step_timing(int32_t enable);
 int32_t     pict_last_step_solve_ms(double *ms_out, int32_t max_count);
 
+/* ------------------------------------------------------------------------ */
+/* batches of independent simulations (no counterpart in the reference,     */
+/* which steps one Domain per PISOtorch_simulation.Simulation)              */
+/* ------------------------------------------------------------------------ */
+
+/* A batch steps `count` independent domains (same dtype and dimension; any sizes,
+   viscosities and boundaries) together: each stage is one launch over the tiles of every
+   domain, each solve one cooperative launch in which every resident workgroup runs a list
+   of (system, workgroup) jobs, and the step is captured into a HIP graph and replayed.
+   Every domain gets the result of pict_piso_step on it alone, up to the reduction order
+   of its Krylov dot products (the workgroup count of a system may differ). */
+typedef struct pict_batch pict_batch;
+
+/* Launch shapes (0: default).  The autotuner (pict_batch_tune) picks them by timing. */
+typedef struct pict_batch_config {
+    int32_t tiles_per_workgroup;        /* 256-cell tiles per workgroup of the stage kernels: 1, 2 or 4 */
+    int32_t pressure_rows_per_thread;   /* CG: matrix rows each thread keeps in registers: 1, 2, 4, 8   */
+    int32_t advection_rows_per_workgroup; /* BiCGStab: rows per workgroup (>= 64)                     */
+    int32_t adaptive_schedule;          /* 1: after the first step, schedule the solves by the
+                                           iteration counts it took; 0: by system size only         */
+} pict_batch_config;
+
+typedef struct pict_batch_info {
+    int32_t count, tiles, tiles_per_workgroup;
+    int32_t advection_grid, advection_jobs, advection_rows_per_workgroup;
+    int32_t pressure_grid, pressure_jobs, pressure_rows_per_thread;
+    int32_t graph;                      /* the last step was a graph replay                          */
+    char advection_kernel[64];
+    char pressure_kernel[64];
+} pict_batch_info;
+
+/* The atlases stay owned by the caller and must outlive the batch. */
+pict_status pict_batch_create(const pict_atlas *const *atlases, int32_t count, pict_batch **out);
+pict_status pict_batch_destroy(pict_batch *batch);
+pict_status pict_batch_set_config(pict_batch *batch, const pict_batch_config *config);
+pict_status pict_batch_get_config(const pict_batch *batch, pict_batch_config *config);
+/* One PISO step of every domain (pict_piso_step's sequence).  time_steps: count entries (NULL:
+   opts->time_step for all).  infos_out: count * (num_dims + corrector_steps) entries, domain by
+   domain in pict_piso_step's order. */
+pict_status pict_batch_step(pict_batch *batch, const pict_step_opts *opts, const double *time_steps,
+                            pict_solve_info *infos_out, void *stream);
+/* Autotuner: runs the stages and solves of a step on the batch's current state, timing every
+   candidate launch shape `repeats` times (the block velocities and pressures are not changed;
+   the domains' matrices, RHS and result vectors are), and keeps the fastest; *chosen (may be
+   NULL) receives it. */
+pict_status pict_batch_tune(pict_batch *batch, const pict_step_opts *opts, const double *time_steps,
+                            int32_t repeats, pict_batch_config *chosen, void *stream);
+pict_status pict_batch_describe(const pict_batch *batch, pict_batch_info *out);
+
 /* Diagnostics (no counterpart in the reference): when enabled, the solver
    records HIP events around its kernel launch on the launch stream; the
    elapsed time of the last solve's kernel is then available (milliseconds). */

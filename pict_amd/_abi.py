@@ -161,6 +161,32 @@ class StepTape(Structure):
     ]
 
 
+class BatchConfig(Structure):
+    _fields_ = [
+        ("tiles_per_workgroup", c_int32),
+        ("pressure_rows_per_thread", c_int32),
+        ("advection_rows_per_workgroup", c_int32),
+        ("adaptive_schedule", c_int32),
+    ]
+
+
+class BatchInfo(Structure):
+    _fields_ = [
+        ("count", c_int32),
+        ("tiles", c_int32),
+        ("tiles_per_workgroup", c_int32),
+        ("advection_grid", c_int32),
+        ("advection_jobs", c_int32),
+        ("advection_rows_per_workgroup", c_int32),
+        ("pressure_grid", c_int32),
+        ("pressure_jobs", c_int32),
+        ("pressure_rows_per_thread", c_int32),
+        ("graph", c_int32),
+        ("advection_kernel", ctypes.c_char * 64),
+        ("pressure_kernel", ctypes.c_char * 64),
+    ]
+
+
 # exported symbol -> (restype, argtypes)
 _P = c_void_p
 SIGNATURES = {
@@ -201,6 +227,13 @@ SIGNATURES = {
     "pict_piso_step": (c_int32, [_P, POINTER(StepOpts), _P, POINTER(SolveInfo), _P]),
     "pict_piso_step_vjp": (c_int32, [_P, POINTER(StepOpts), POINTER(StepTape), _P, POINTER(SolveInfo), _P]),
     "pict_subtract_mean": (c_int32, [_P, c_int32, c_int32, _P]),
+    "pict_batch_create": (c_int32, [POINTER(c_void_p), c_int32, POINTER(c_void_p)]),
+    "pict_batch_destroy": (c_int32, [_P]),
+    "pict_batch_set_config": (c_int32, [_P, POINTER(BatchConfig)]),
+    "pict_batch_get_config": (c_int32, [_P, POINTER(BatchConfig)]),
+    "pict_batch_step": (c_int32, [_P, POINTER(StepOpts), POINTER(c_double), POINTER(SolveInfo), _P]),
+    "pict_batch_tune": (c_int32, [_P, POINTER(StepOpts), POINTER(c_double), c_int32, POINTER(BatchConfig), _P]),
+    "pict_batch_describe": (c_int32, [_P, POINTER(BatchInfo)]),
     "pict_set_step_timing": (c_int32, [c_int32]),
     "pict_last_step_solve_ms": (c_int32, [POINTER(c_double), c_int32]),
     "pict_set_kernel_timing": (c_int32, [c_int32]),

@@ -5,6 +5,7 @@
 The library is written to pict_amd/lib/libpict_hip.so.  Every HIP source is
 compiled with packed-FP32 instructions disabled.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -13,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libpict_hip.so")
-SOURCES = ["pict_stages.hip", "pict_solve.hip", "pict_metrics.hip", "pict_grad.hip", "pict_step.hip"]
+SOURCES = ["pict_stages.hip", "pict_solve.hip", "pict_metrics.hip", "pict_grad.hip", "pict_step.hip", "pict_batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PICT_OFFLOAD_ARCH", "gfx950")
@@ -25,19 +26,23 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
+def _compile(src, verbose):
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
+    cmd = [HIPCC, *FLAGS, "-c", path, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source (in parallel: `jobs` processes, default up to 6) and link the library."""
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
-    for src in SOURCES:
-        path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
-        obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    jobs = jobs or max(1, min(len(srcs), 6, os.cpu_count() or 1))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+        objs = list(pool.map(lambda src: _compile(src, verbose), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp]
     if verbose:

@@ -94,3 +94,26 @@ def periodic_box_c3(n=256, viscosity=1e-3, dtype=torch.float32, device="cuda", s
     h = 2 * math.pi / n
     dt = 0.25 * h / torch.linalg.vector_norm(vel, dim=0).max().item()
     return dom, dt
+
+
+C4_SIDES = (32, 48, 64, 96, 128, 192, 256)
+
+
+def cavity_batch_params(count=64, seed=0, sides=C4_SIDES, visc_range=(1e-3, 1e-1)):
+    """Side lengths (uniform over `sides`) and viscosities (log-uniform over visc_range) of a batch of lid
+    cavities, drawn from torch.Generator seeded with `seed`: sides first, then viscosities."""
+    g = torch.Generator().manual_seed(seed)
+    idx = torch.randint(0, len(sides), (count,), generator=g)
+    lo, hi = math.log10(visc_range[0]), math.log10(visc_range[1])
+    visc = 10.0 ** (lo + (hi - lo) * torch.rand(count, generator=g, dtype=torch.float64))
+    return [int(sides[i]) for i in idx.tolist()], [float(v) for v in visc.tolist()]
+
+
+def cavity_batch_c4(count=64, seed=0, dtype=torch.float32, device="cuda", sides=C4_SIDES):
+    """Batching configuration: `count` independent lid-driven cavities on the unit square (lid velocity 1 on +y,
+    zero initial state), sides and viscosities from cavity_batch_params.  Returns (domains, sides, viscosities,
+    time_step) with dt = 0.4 / max(sides) (CFL <= 0.4 at the lid speed)."""
+    ns, visc = cavity_batch_params(count, seed, sides)
+    doms = [lid_cavity(n, viscosity=v, dtype=dtype, device=device, name="cavity%d" % i)
+            for i, (n, v) in enumerate(zip(ns, visc))]
+    return doms, ns, visc, 0.4 / max(sides)

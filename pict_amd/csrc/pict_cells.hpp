@@ -1,0 +1,237 @@
+// pict_cells.hpp -- per-cell bodies of the forward PISO stage kernels.
+//
+// Each body handles one 256-cell tile of one block (tile = index into the domain's
+// tile tables, one thread per cell).  The single-domain kernels (pict_stages.hip)
+// run tile blockIdx.x of their domain; the batched kernels (pict_batch.hip) run the
+// tiles of many domains in one launch, each workgroup reading its domain's view
+// from a table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pict_device.hpp"
+
+namespace pict {
+
+// ---------------------------------------------------------------------------
+// PISO_build_matrix :3601-3865 (velocity matrix C and its diagonal A),
+// orthogonal terms.  Values only; the pattern is written by k_csr_pattern.
+// ---------------------------------------------------------------------------
+template <typename R, int D>
+__device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    R fl[2 * D];
+    fluxes_nd<R, D>(dom, b, pos, fl, nullptr);
+    int idx[7];
+    R rv[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { idx[k] = -1; rv[k] = 0; }
+    const R det = cell_det<R, D>(b, pos);
+    R diag = det / dt;
+    idx[0] = f + b->global_offset;
+    const R visc_p = viscosity(dom, b, pos);
+    R alpha_p[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        alpha_p[d] = laplace_orth<R, D>(b, t);
+    }
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        const int dim = face >> 1;
+        const R fs = (face & 1) ? R(1) : R(-1);
+        if (!(at_bound(pos, face, b) && prescribed(face, b))) {
+            const R ff = fs * R(0.5) * fl[face];
+            diag += ff;
+            rv[face + 1] += ff;
+            int nb;
+            I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
+            const pict_block_desc *nbp = dom.blocks + nb;
+            const R alpha = laplace_orth<R, D>(nbp, t);
+            t.a[3] = 0;
+            const R visc_n = viscosity(dom, nbp, t);
+            idx[face + 1] = global_index(t, nbp);
+            const R vc = (alpha_p[dim] * visc_p + alpha * visc_n) * R(0.5);
+            diag += vc;
+            rv[face + 1] -= vc;
+        } else {
+            const int slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
+            diag += R((1 - slip) * 2) * visc_p * alpha_p[dim];
+        }
+    }
+    rv[0] = diag;
+    const int g = f + b->global_offset;
+    const int start = dom.csr_row[g], end = dom.csr_row[g + 1];
+    for (int i = 0; i < end - start; ++i) {
+        const int s = lowest_slot(idx);
+        if (s < 0) break;
+        rv[s] /= det;
+        dom.C[start + i] = rv[s];
+        idx[s] = -1;
+    }
+    dom.A[g] = rv[0];
+}
+
+// kPISO_build_advection_RHS :4281-4385 (orthogonal)
+template <typename R, int D>
+__device__ __forceinline__ void adv_rhs_cell(const DomainK<R> &dom, int tile, R dt, int apply_pg) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const R det = cell_det<R, D>(b, pos);
+    R pg[D];
+    if (apply_pg) pressure_gradient<R, D>(dom, b, pos, pg);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        R r = det * reinterpret_cast<const R *>(b->velocity)[flat4(t, b->stride)] / dt;
+        r += fixed_boundary_source<R, D>(dom, b, pos, d);
+        r /= det;
+        r += velocity_source<R>(b, t);
+        if (apply_pg) r -= pg[d] * dt;
+        dom.vrhs[(size_t)d * dom.N + b->global_offset + f] = r;
+    }
+}
+
+// PISO_build_pressure_matrix :4797-4963 (orthogonal, cell metrics)
+template <typename R, int D>
+__device__ __forceinline__ void p_matrix_cell(const DomainK<R> &dom, int tile) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    int idx[7];
+    R rv[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { idx[k] = -1; rv[k] = 0; }
+    R alphaP[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        alphaP[d] = laplace_orth<R, D>(b, t);
+    }
+    const R raP = R(1.0) / dom.A[g];
+    idx[0] = g;
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+        const int dim = face >> 1;
+        int nb;
+        I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
+        const pict_block_desc *nbp = dom.blocks + nb;
+        const R alphaN = laplace_orth<R, D>(nbp, t);
+        t.a[3] = 0;
+        const int gn = global_index(t, nbp);
+        const R raN = R(1.0) / dom.A[gn];
+        const R coef = R(0.5) * (alphaP[dim] * raP + alphaN * raN);
+        rv[0] -= coef;
+        rv[face + 1] += coef;
+        idx[face + 1] = gn;
+    }
+    const int start = dom.csr_row[g], end = dom.csr_row[g + 1];
+    for (int i = 0; i < end - start; ++i) {
+        const int s = lowest_slot(idx);
+        if (s < 0) break;
+        dom.P[start + i] = rv[s];
+        idx[s] = -1;
+    }
+}
+
+// PISO_build_pressure_rhs :5121-5240
+template <typename R, int D>
+__device__ __forceinline__ void p_rhs_cell(const DomainK<R> &dom, int tile, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R rDiag = R(1) / dom.A[g];
+    const R det = cell_det<R, D>(b, pos);
+    const int cs = dom.csr_row[g], ce = dom.csr_row[g + 1];
+    int col[7];
+    R val[7];
+    const int rs = ce - cs < 7 ? ce - cs : 7;
+    for (int k = 0; k < rs; ++k) {
+        col[k] = dom.csr_index[cs + k];
+        val[k] = dom.C[cs + k];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        I4 t = pos;
+        t.a[3] = d;
+        const R velOld = reinterpret_cast<const R *>(b->velocity)[flat4(t, b->stride)] / dt;
+        R H = 0;
+        for (int k = 0; k < rs; ++k)
+            if (col[k] != g) H += val[k] * dom.vres[col[k] + (size_t)dom.N * d];
+        R S = fixed_boundary_source<R, D>(dom, b, pos, d);
+        S /= det;
+        S += velocity_source<R>(b, t);
+        dom.prhs[g + (size_t)dom.N * d] = rDiag * (velOld - H + S);
+    }
+}
+
+// k_computePressureRHSdivergenceFromFlux :5374-5419 (vg = pressure RHS) and
+// k_computeVelocityDivergenceFromFlux :6333-6354 (vg = nullptr, block velocity).
+template <typename R, int D>
+__device__ __forceinline__ void divergence_cell(const DomainK<R> &dom, int tile, const R *vg, R *out, R dt, int tsn) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    R fl[2 * D];
+    fluxes_nd<R, D>(dom, b, pos, fl, vg);
+    R div = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) div += fl[2 * d + 1] - fl[2 * d];
+    if (tsn) div /= dt;
+    out[f + b->global_offset] = div;
+}
+
+// PISO_update_velocity :5947-5980
+template <typename R, int D>
+__device__ __forceinline__ void correct_velocity_cell(const DomainK<R> &dom, int tile, R dt, int tsn) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R rDiag = R(1) / dom.A[g];
+    R pg[D];
+    pressure_gradient<R, D>(dom, b, pos, pg);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        R u = -rDiag * pg[d];
+        if (tsn) u *= dt;
+        u += dom.prhs[g + (size_t)dom.N * d];
+        dom.vres[g + (size_t)dom.N * d] = u;
+    }
+}
+
+// Copy*Result*Blocks :6541-6733 and Copy*ResultGrad*Blocks :6819-6915 -- one
+// launch instead of a memcpy per block and component.  `field` selects the
+// block tensor: 0 velocity, 1 pressure, 2 velocity_grad, 3 pressure_grad;
+// `gv` is the matching global vector (component-major for velocities).
+template <typename R>
+__device__ __forceinline__ void copy_result_cell(const DomainK<R> &dom, int tile, R *gv, int field, int to_blocks) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const int g = b->global_offset + f;
+    void *bp = field == 0 ? b->velocity : field == 1 ? b->pressure : field == 2 ? b->velocity_grad : b->pressure_grad;
+    R *bv = reinterpret_cast<R *>(bp);
+    const int comps = (field & 1) ? 1 : dom.D;
+    for (int d = 0; d < comps; ++d) {
+        if (to_blocks) bv[(size_t)d * b->stride[3] + f] = gv[(size_t)d * dom.N + g];
+        else gv[(size_t)d * dom.N + g] = bv[(size_t)d * b->stride[3] + f];
+    }
+}
+
+}  // namespace pict

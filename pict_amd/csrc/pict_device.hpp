@@ -384,11 +384,16 @@ __device__ __forceinline__ int lowest_slot(const int *idx) {
 
 // Resolve the workgroup's (block, first cell) from the launch tables.
 template <typename R>
-__device__ __forceinline__ bool tile_cell(const DomainK<R> &dom, const pict_block_desc *&b, int &bi, int &f) {
-    bi = dom.tile_block[blockIdx.x];
+__device__ __forceinline__ bool tile_cell(const DomainK<R> &dom, int tile, const pict_block_desc *&b, int &bi,
+                                          int &f) {
+    bi = dom.tile_block[tile];
     b = dom.blocks + bi;
-    f = dom.tile_start[blockIdx.x] + (int)threadIdx.x;
+    f = dom.tile_start[tile] + (int)threadIdx.x;
     return f < b->stride[3];
+}
+template <typename R>
+__device__ __forceinline__ bool tile_cell(const DomainK<R> &dom, const pict_block_desc *&b, int &bi, int &f) {
+    return tile_cell(dom, (int)blockIdx.x, b, bi, f);
 }
 
 }  // namespace pict

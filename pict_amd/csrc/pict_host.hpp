@@ -15,6 +15,12 @@ namespace pict {
 
 void set_error(const char *fmt, ...);
 
+// Mean subtraction of a global vector: kMeanBlocks workgroups of kMeanThreads threads form FP64
+// partial sums in a fixed order (the single-domain and the batched step use the same shape, so
+// their means are bitwise equal).
+constexpr int kMeanThreads = 256;
+constexpr int kMeanBlocks = 128;
+
 // Stream-ordered workspace memory from a library-owned pool of the current device that keeps
 // what is freed, so repeated calls reuse it instead of allocating device memory each time.
 hipError_t ws_alloc(void **p, size_t bytes, hipStream_t s);

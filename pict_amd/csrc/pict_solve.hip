@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "pict_host.hpp"
@@ -300,7 +301,7 @@ __device__ __forceinline__ double wave_reduce_dpp(double v) {
 }
 
 template <int NR, int NS>
-__device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg, unsigned gen,
+__device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg, unsigned bid, unsigned gen,
                                             double (&acc)[NR][NS], const int (&ids)[NS],
                                             const bool (&ismax)[NS], double *red, int &s_err) {
     __shared__ double wsum[kSolveThreads / 64][NR * NS];
@@ -318,7 +319,7 @@ __device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg
     double *base = part + (size_t)(gen & 1) * nwg * kPartStride;
     // 2. publish this workgroup's partials and arrive
     if (lane == 0) {
-        double *mine = base + (size_t)blockIdx.x * kPartStride;
+        double *mine = base + (size_t)bid * kPartStride;
 #pragma unroll
         for (int k = 0; k < NR; ++k)
 #pragma unroll
@@ -379,7 +380,7 @@ __device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg
 // thread 0, which then runs the scalar update); one workgroup barrier publishes both.
 #define PICT_FLAT_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                            \
     do {                                                                                          \
-        flat_reduce<NRHS, NS>(a.partials, a.ctl, a.nwg, gen, ACC, IDS, ISMAX, red, s_err);       \
+        flat_reduce<NRHS, NS>(a.partials, a.ctl, a.nwg, bid, gen, ACC, IDS, ISMAX, red, s_err);  \
         ++gen;                                                                                    \
         if (threadIdx.x == 0 && !s_err) { UPDATE; }                                               \
         __syncthreads();                                                                          \
@@ -671,8 +672,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
     __shared__ RhsState sst[NRHS];
     __shared__ int s_err;
     const int n = a.n;
+    const unsigned bid = blockIdx.x;
     const size_t stride = (size_t)a.nwg * kSolveThreads;
-    const size_t i0 = (size_t)blockIdx.x * kSolveThreads + threadIdx.x;
+    const size_t i0 = (size_t)bid * kSolveThreads + threadIdx.x;
     unsigned gen = 0;
     R *rb[2] = {a.r, a.rhat}, *wb[2] = {a.v0, a.v1}, *sb[2] = {a.s, a.t};
     R *p = a.p0;
@@ -1092,7 +1094,7 @@ struct CgSlots<false> {
 };
 
 template <class L>
-__device__ __forceinline__ void tagged_poll(u64 *part, int *err_flag, unsigned nwg, unsigned gen,
+__device__ __forceinline__ void tagged_poll(u64 *part, int *err_flag, unsigned nwg, unsigned bid, unsigned gen,
                                             const double (&acc)[L::NS], bool wait_rows, double (&v)[L::NS],
                                             int &s_err) {
     constexpr int NS = L::NS;
@@ -1108,7 +1110,7 @@ __device__ __forceinline__ void tagged_poll(u64 *part, int *err_flag, unsigned n
     if (wave != 0) return;
     u64 *base = part + (size_t)(gen & 1u) * nwg * kSlots;
     if (lane == 0) {
-        u64 *mine = base + (size_t)blockIdx.x * kSlots;
+        u64 *mine = base + (size_t)bid * kSlots;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             double t = wsum[0][s];
@@ -1152,10 +1154,10 @@ __device__ __forceinline__ void poll_total(const double (&v)[L::NS], double *red
 }
 
 template <class L>
-__device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned nwg, unsigned gen,
+__device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned nwg, unsigned bid, unsigned gen,
                                               const double (&acc)[L::NS], bool wait_rows, double *red, int &s_err) {
     double v[L::NS];
-    tagged_poll<L>(part, err_flag, nwg, gen, acc, wait_rows, v, s_err);
+    tagged_poll<L>(part, err_flag, nwg, bid, gen, acc, wait_rows, v, s_err);
     if (threadIdx.x < 64) {
         double t[kSlots];
         poll_total<L>(v, t);
@@ -1167,7 +1169,7 @@ __device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned
 
 #define PICT_TAG_BARRIER(L, ACC, UPDATE)                                                                     \
     do {                                                                                                      \
-        tagged_reduce<L>(a.tagged, &a.ctl->err, a.nwg, gen, ACC, !Pub<R>::kTagged, red, s_err);               \
+        tagged_reduce<L>(a.tagged, &a.ctl->err, a.nwg, bid, gen, ACC, !Pub<R>::kTagged, red, s_err);          \
         ++gen;                                                                                                \
         if (threadIdx.x == 0 && !s_err) { UPDATE; }                                                           \
         __syncthreads();                                                                                      \
@@ -1191,7 +1193,7 @@ __device__ __forceinline__ void cg_acc(double (&acc)[NS], double rv, double last
 // fit is read from memory like a long row.  NH covers the rows of a 2-D grid whose chunk
 // edges cut grid lines (two foreign entries per thread) and the z neighbours of a 3-D grid.
 template <typename R, int RPT, int NZ, bool FULL>
-__global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
+__device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
     constexpr int C = kSolveThreads * RPT;
     constexpr int NH = NZ <= 5 ? 2 : 2 * RPT + 2;
     using LF = typename CgSlots<FULL>::First;
@@ -1208,7 +1210,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
     };
     __shared__ Step bc;
     const int n = a.n;
-    const int lo = (int)blockIdx.x * C;
+    const int lo = (int)bid * C;
     const int cnt = min(C, n - lo);
     unsigned gen = 0;
     using PubT = decltype(Pub<R>::base);
@@ -1304,7 +1306,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
 #pragma unroll
         for (int q = 0; q < RPT; ++q)
             if (row[q]) a.x[lo + (int)threadIdx.x + q * kSolveThreads] = 0;
-        if (blockIdx.x == 0 && threadIdx.x == 0) a.st[0] = sst;
+        if (bid == 0 && threadIdx.x == 0) a.st[0] = sst;
         return;
     }
 #pragma unroll
@@ -1469,7 +1471,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
         }
         // reduce-barrier; the next iteration's loads go out while wave 0 forms the totals and the step
         double pv[NS];
-        tagged_poll<LI>(a.tagged, &a.ctl->err, a.nwg, gen, acc, !Pub<R>::kTagged, pv, s_err);
+        tagged_poll<LI>(a.tagged, &a.ctl->err, a.nwg, bid, gen, acc, !Pub<R>::kTagged, pv, s_err);
         ++gen;
         __syncthreads();
         if (s_err) return;
@@ -1492,7 +1494,29 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
         if (row[q]) a.x[lo + (int)threadIdx.x + q * kSolveThreads] = restore ? xb[q] : x[q];
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.st[0] = st;
+    if (bid == 0 && threadIdx.x == 0) a.st[0] = st;
+}
+
+template <typename R, int RPT, int NZ, bool FULL>
+__global__ __launch_bounds__(kSolveThreads) void k_cg_local(SolveArgs<R> a) {
+    cg_local<R, RPT, NZ, FULL>(a, blockIdx.x);
+}
+
+// Batched launch: workgroup w runs jobs[job_off[w] .. job_off[w+1]) in order, job = (system << 16) |
+// workgroup index within the system.  Every workgroup's list follows one global order of the systems,
+// and the launch is cooperative (all workgroups resident), so the first unfinished system always has
+// all its workgroups on it.
+template <typename R, int RPT, int NZ, bool FULL>
+__global__ __launch_bounds__(kSolveThreads) void k_cg_local_jobs(const SolveArgs<R> *__restrict__ args,
+                                                                 const unsigned *__restrict__ job_off,
+                                                                 const unsigned *__restrict__ jobs) {
+    const unsigned j0 = job_off[blockIdx.x], j1 = job_off[blockIdx.x + 1];
+    for (unsigned j = j0; j < j1; ++j) {
+        const unsigned job = jobs[j];
+        const SolveArgs<R> a = args[job >> 16];
+        cg_local<R, RPT, NZ, FULL>(a, job & 0xffffu);
+        __syncthreads();
+    }
 }
 
 // ==========================================================================
@@ -1580,13 +1604,13 @@ __global__ void k_zero64(u64 *p, size_t n) {
 // into the following SpMVs, evaluated on the fly for the neighbours.
 // ==========================================================================
 template <typename R, int NRHS>
-__global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
+__device__ __forceinline__ void bicgstab(const SolveArgs<R> &a, unsigned bid) {
     __shared__ double red[NRHS * kSlots];
     __shared__ RhsState sst[NRHS];
     __shared__ int s_err;
     const int n = a.n;
     const size_t stride = (size_t)a.nwg * kSolveThreads;
-    const size_t i0 = (size_t)blockIdx.x * kSolveThreads + threadIdx.x;
+    const size_t i0 = (size_t)bid * kSolveThreads + threadIdx.x;
     unsigned gen = 0;
     constexpr int IDS_R[4] = {S_RR, S_ASUM, S_AMAX, S_BMAX};
     constexpr bool MAX_R[4] = {false, false, true, true};
@@ -1785,8 +1809,27 @@ __global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
             });
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (bid == 0 && threadIdx.x == 0)
         for (int k = 0; k < NRHS; ++k) a.st[k] = sst[k];
+}
+
+template <typename R, int NRHS>
+__global__ __launch_bounds__(kSolveThreads) void k_bicgstab(SolveArgs<R> a) {
+    bicgstab<R, NRHS>(a, blockIdx.x);
+}
+
+// batched launch (job lists as for k_cg_local_jobs)
+template <typename R, int NRHS>
+__global__ __launch_bounds__(kSolveThreads) void k_bicgstab_jobs(const SolveArgs<R> *__restrict__ args,
+                                                                 const unsigned *__restrict__ job_off,
+                                                                 const unsigned *__restrict__ jobs) {
+    const unsigned j0 = job_off[blockIdx.x], j1 = job_off[blockIdx.x + 1];
+    for (unsigned j = j0; j < j1; ++j) {
+        const unsigned job = jobs[j];
+        const SolveArgs<R> a = args[job >> 16];
+        bicgstab<R, NRHS>(a, job & 0xffffu);
+        __syncthreads();
+    }
 }
 
 // A^T permutation for a structurally symmetric pattern: perm[e] = index of (j,i) for e = (i,j).
@@ -1844,6 +1887,14 @@ pict_status occupancy(const void *kernel, int &occ, int &cus) {
 }
 
 enum class Kind { kCG1, kCGTextbook, kBiCG };
+
+// stamp of a launch's foreign-read marks in k_cg_local (a stale mark equal to it only costs a store)
+int next_salt() {
+    static std::atomic<unsigned> launches{0};
+    unsigned sv = ++launches;
+    if (sv == 0) sv = ++launches;
+    return (int)sv;
+}
 
 template <typename R, int NR>
 const void *kernel_for(Kind kind) {
@@ -2056,13 +2107,7 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
                                                                          : 1.0;
     a.tagged = reinterpret_cast<unsigned long long *>(ws + off_zero);
     a.pub = a.tagged + words_tagged;
-    {
-        // stamp of this launch's foreign-read marks (a stale mark equal to it only costs a store)
-        static std::atomic<unsigned> launches{0};
-        unsigned sv = ++launches;
-        if (sv == 0) sv = ++launches;
-        a.salt = (int)sv;
-    }
+    a.salt = next_salt();
     {
         // control block (+ k_cg_local's tagged regions): a fill kernel, which a captured graph replays safely
         const size_t words = (off_zero - off_ctl) / 8 + L.words_zero;
@@ -2155,6 +2200,351 @@ void solve_state_to_info(const void *states, int nrhs, pict_solve_info *out) {
         out[k].is_finite_residual = hs[k].finite;
         out[k].reserved = 0;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched solves: `count` independent systems (same dtype, right-hand sides per
+// system and options, any sizes) in one cooperative launch of a job-list kernel.
+// Each system runs the single-system algorithm on its own workspace; the host fixes
+// every system's workgroup count and deals the (system, workgroup) jobs to the
+// launch's resident workgroups by list scheduling: systems in order of decreasing
+// estimated cost, each on the workgroups that become free earliest.
+// ---------------------------------------------------------------------------
+struct BatchSolve {
+    int count = 0, nrhs = 0, dtype = 0;
+    bool local = false;  // k_cg_local_jobs, else k_bicgstab_jobs
+    int rpt = 0, nz = 0, rows_per_wg = 0;
+    const void *kernel = nullptr;
+    unsigned grid = 0;   // workgroups of the launch, all resident
+    std::vector<int> n, nwg;
+    size_t off_args = 0, off_joboff = 0, off_jobs = 0, off_states = 0, off_err = 0, off_zero = 0;
+    size_t zero_words = 0, total = 0, args_stride = 0;
+    char *arena = nullptr;
+    std::vector<char> args_host;
+    std::vector<unsigned> job_off, jobs;
+    double makespan = 0.0;
+    ~BatchSolve() {
+        if (arena) (void)hipFree(arena);
+    }
+};
+
+namespace {
+
+template <typename R, int NZ>
+const void *local_cg_jobs_kernel_nz(int rpt) {
+    switch (rpt) {
+        case 1: return (const void *)k_cg_local_jobs<R, 1, NZ, false>;
+        case 2: return (const void *)k_cg_local_jobs<R, 2, NZ, false>;
+        case 4: return (const void *)k_cg_local_jobs<R, 4, NZ, false>;
+        case 8:
+            if constexpr (sizeof(R) == 4 && NZ <= 5) return (const void *)k_cg_local_jobs<R, 8, NZ, false>;
+            return nullptr;
+        default: return nullptr;
+    }
+}
+template <typename R>
+const void *bicgstab_jobs_kernel(int nrhs) {
+    switch (nrhs) {
+        case 1: return (const void *)k_bicgstab_jobs<R, 1>;
+        case 2: return (const void *)k_bicgstab_jobs<R, 2>;
+        case 3: return (const void *)k_bicgstab_jobs<R, 3>;
+        default: return (const void *)k_bicgstab_jobs<R, 4>;
+    }
+}
+
+// OR of the systems' barrier error flags
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_err(const SolveArgs<R> *args, int count, int *out) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < count; s += blockDim.x)
+        if (__hip_atomic_load(&args[s].ctl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicOr(&any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = any;
+}
+
+// List scheduling of the systems' jobs onto b.grid workgroups (cost: estimated time of each solve).
+void schedule(BatchSolve &b, const std::vector<double> &cost) {
+    const unsigned G = b.grid;
+    std::vector<int> order(b.count);
+    for (int s = 0; s < b.count; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    std::vector<double> avail(G, 0.0);
+    std::vector<std::vector<unsigned>> lists(G);
+    std::vector<unsigned> idx(G);
+    b.makespan = 0.0;
+    for (int s : order) {
+        const unsigned k = (unsigned)b.nwg[s];
+        for (unsigned g = 0; g < G; ++g) idx[g] = g;
+        auto earlier = [&](unsigned x, unsigned y) { return avail[x] < avail[y] || (avail[x] == avail[y] && x < y); };
+        std::nth_element(idx.begin(), idx.begin() + (k - 1), idx.end(), earlier);
+        std::sort(idx.begin(), idx.begin() + k);
+        double start = 0.0;
+        for (unsigned j = 0; j < k; ++j) start = std::max(start, avail[idx[j]]);
+        const double fin = start + cost[s];
+        for (unsigned j = 0; j < k; ++j) {
+            lists[idx[j]].push_back(((unsigned)s << 16) | j);
+            avail[idx[j]] = fin;
+        }
+        b.makespan = std::max(b.makespan, fin);
+    }
+    b.job_off.assign(G + 1, 0);
+    b.jobs.clear();
+    for (unsigned g = 0; g < G; ++g) {
+        b.job_off[g] = (unsigned)b.jobs.size();
+        b.jobs.insert(b.jobs.end(), lists[g].begin(), lists[g].end());
+    }
+    b.job_off[G] = (unsigned)b.jobs.size();
+}
+
+// default schedule estimate: CG iterations grow like the square root of the size (2-D and 3-D
+// Poisson-like systems); a BiCGStab iteration streams its rows, so its time grows with the rows
+// per workgroup on top of the barrier latency
+std::vector<double> default_cost(const BatchSolve &b) {
+    std::vector<double> c(b.count);
+    for (int s = 0; s < b.count; ++s)
+        c[s] = b.local ? std::sqrt((double)b.n[s]) : (2048.0 + (double)b.n[s] / b.nwg[s]);
+    return c;
+}
+
+template <typename R>
+void fill_args(BatchSolve &b, int s, const pict_csr_desc &A, const void *rhs, void *x, const pict_solve_opts *o,
+               const std::vector<size_t> &off_vec, const std::vector<size_t> &off_part,
+               const std::vector<size_t> &off_ctl) {
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    SolveArgs<R> a;
+    memset(&a, 0, sizeof a);
+    const size_t vec = (size_t)A.n * b.nrhs;
+    a.row = A.row;
+    a.col = A.index;
+    a.val = (const R *)A.value;
+    a.b = (const R *)rhs;
+    a.x = (R *)x;
+    R *v = (R *)(b.arena + off_vec[s]);
+    R **vs[9] = {&a.r, &a.p0, &a.p1, &a.v0, &a.rhat, &a.v1, &a.s, &a.t, &a.best};
+    for (R **pp : vs) {
+        *pp = v;
+        v += vec;
+    }
+    a.partials = (double *)(b.arena + off_part[s]);
+    a.ctl = (Ctl *)(b.arena + off_ctl[s]);
+    a.st = (RhsState *)(b.arena + b.off_states) + (size_t)s * b.nrhs;
+    a.n = A.n;
+    a.nrhs = b.nrhs;
+    a.maxit = o->max_iterations;
+    a.conv = o->convergence_criterion;
+    a.reset_steps = o->residual_reset_steps;
+    a.return_best = o->return_best_result;
+    a.zero_shortcut = o->zero_rhs_shortcut;
+    a.tol = o->tolerance;
+    a.nwg = (unsigned)b.nwg[s];
+    a.crit_scale = o->convergence_criterion == PICT_CONV_NORM2_NORMALIZED ? 1.0 / sqrt((double)A.n)
+                   : o->convergence_criterion == PICT_CONV_ABS_MEAN      ? 1.0 / (double)A.n
+                                                                         : 1.0;
+    a.tagged = reinterpret_cast<unsigned long long *>(b.arena + off_ctl[s] + al(sizeof(Ctl)));
+    a.pub = a.tagged + (b.local ? (size_t)2 * b.nwg[s] * kSlots : 0);
+    a.salt = next_salt();
+    memcpy(b.args_host.data() + (size_t)s * sizeof(SolveArgs<R>), &a, sizeof a);
+}
+
+template <typename R>
+pict_status batch_create_typed(const pict_csr_desc *A, const void *const *rhs, void *const *x, int count, int nrhs,
+                               const pict_solve_opts *o, const BatchSolveConfig &cfg, const double *cost,
+                               BatchSolve **out, hipStream_t stream) {
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    std::unique_ptr<BatchSolve> b(new BatchSolve());
+    b->count = count;
+    b->nrhs = nrhs;
+    b->dtype = A[0].dtype;
+    b->local = !o->use_bicg;
+    int occ = 0, cus = 0;
+    if (b->local) {
+        PICT_CHECK_ARG(nrhs == 1, "batched CG solves take one right-hand side per system");
+        PICT_CHECK_ARG(o->residual_reset_steps <= 0, "batched CG solves do not reset the residual");
+        if (o->convergence_criterion != PICT_CONV_NORM2 && o->convergence_criterion != PICT_CONV_NORM2_NORMALIZED) {
+            set_error("batched CG solves support the norm-2 convergence criteria");
+            return PICT_ERR_UNSUPPORTED;
+        }
+        bool all5 = true;
+        for (int s = 0; s < count; ++s) all5 = all5 && (long long)A[s].nnz <= 5LL * A[s].n;
+        b->nz = all5 ? 5 : 7;
+        const int rmax = sizeof(R) == 4 && b->nz <= 5 ? 8 : 4;
+        b->rpt = cfg.rows_per_thread > 0 ? cfg.rows_per_thread : (rmax >= 4 ? 4 : rmax);
+        PICT_CHECK_ARG(b->rpt == 1 || b->rpt == 2 || b->rpt == 4 || b->rpt == 8, "rows per thread must be 1, 2, 4 or 8");
+        PICT_CHECK_ARG(b->rpt <= rmax, "rows per thread %d exceeds the register budget (%d)", b->rpt, rmax);
+        b->kernel = b->nz <= 5 ? local_cg_jobs_kernel_nz<R, 5>(b->rpt) : local_cg_jobs_kernel_nz<R, 7>(b->rpt);
+    } else {
+        b->rows_per_wg = cfg.rows_per_wg > 0 ? cfg.rows_per_wg : 1024;
+        PICT_CHECK_ARG(b->rows_per_wg >= 64, "rows per workgroup must be at least 64");
+        b->kernel = bicgstab_jobs_kernel<R>(nrhs);
+    }
+    PICT_CHECK_ARG(b->kernel != nullptr, "no batched solver kernel for this configuration");
+    pict_status st = occupancy(b->kernel, occ, cus);
+    if (st) return st;
+    if (occ < 1) {
+        set_error("batched solver kernel cannot be resident (occupancy 0)");
+        return PICT_ERR_HIP;
+    }
+    const long long cap = (long long)occ * cus;
+    long long sum = 0;
+    b->n.resize(count);
+    b->nwg.resize(count);
+    for (int s = 0; s < count; ++s) {
+        const long long n = A[s].n;
+        long long w = b->local ? (n + (long long)kSolveThreads * b->rpt - 1) / ((long long)kSolveThreads * b->rpt)
+                               : (n + b->rows_per_wg - 1) / b->rows_per_wg;
+        if (w < 1) w = 1;
+        if (!b->local && w > cap) w = cap;
+        if (w > cap || w > 65535) {
+            set_error("system %d (%lld rows) needs %lld workgroups; at most %lld are resident", s, n, w, cap);
+            return PICT_ERR_UNSUPPORTED;
+        }
+        b->n[s] = (int)n;
+        b->nwg[s] = (int)w;
+        sum += w;
+    }
+    b->grid = (unsigned)std::min(sum, cap);
+    // arena: tables | states | error flag | per-system vectors and partials | per-system control blocks and
+    // tagged regions (contiguous: one fill kernel clears them before each launch)
+    b->args_stride = sizeof(SolveArgs<R>);
+    size_t off = 0;
+    b->off_args = off;
+    off += al(b->args_stride * count);
+    b->off_joboff = off;
+    off += al(sizeof(unsigned) * (b->grid + 1));
+    b->off_jobs = off;
+    off += al(sizeof(unsigned) * (size_t)sum);
+    b->off_states = off;
+    off += al(sizeof(RhsState) * (size_t)count * nrhs);
+    b->off_err = off;
+    off += 256;
+    std::vector<size_t> off_vec(count), off_part(count), off_ctl(count);
+    for (int s = 0; s < count; ++s) {
+        off_vec[s] = off;
+        off += al(sizeof(R) * (size_t)b->n[s] * nrhs * 9);
+        off_part[s] = off;
+        off += al(sizeof(double) * (size_t)2 * b->nwg[s] * kPartStride);
+    }
+    b->off_zero = off;
+    for (int s = 0; s < count; ++s) {
+        off_ctl[s] = off;
+        const size_t words = b->local ? (size_t)2 * b->nwg[s] * kSlots + 6 * (size_t)b->n[s] : 0;
+        off += al(sizeof(Ctl)) + al(sizeof(unsigned long long) * words);
+    }
+    b->zero_words = (off - b->off_zero) / sizeof(unsigned long long);
+    b->total = off;
+    PICT_HIP_TRY(hipMalloc((void **)&b->arena, b->total));
+    b->args_host.assign(b->args_stride * count, 0);
+    for (int s = 0; s < count; ++s) fill_args<R>(*b, s, A[s], rhs[s], x[s], o, off_vec, off_part, off_ctl);
+    std::vector<double> c = cost ? std::vector<double>(cost, cost + count) : default_cost(*b);
+    schedule(*b, c);
+    PICT_HIP_TRY(hipMemcpyAsync(b->arena + b->off_args, b->args_host.data(), b->args_host.size(),
+                                hipMemcpyHostToDevice, stream));
+    PICT_HIP_TRY(hipMemcpyAsync(b->arena + b->off_joboff, b->job_off.data(), sizeof(unsigned) * b->job_off.size(),
+                                hipMemcpyHostToDevice, stream));
+    PICT_HIP_TRY(hipMemcpyAsync(b->arena + b->off_jobs, b->jobs.data(), sizeof(unsigned) * b->jobs.size(),
+                                hipMemcpyHostToDevice, stream));
+    PICT_HIP_TRY(hipStreamSynchronize(stream));
+    *out = b.release();
+    return PICT_OK;
+}
+
+}  // namespace
+
+pict_status batch_solve_create(const pict_csr_desc *A, const void *const *rhs, void *const *x, int count, int nrhs,
+                               const pict_solve_opts *o, const BatchSolveConfig &cfg, const double *cost,
+                               BatchSolve **out, hipStream_t stream) {
+    PICT_CHECK_ARG(A && rhs && x && o && out, "NULL argument");
+    PICT_CHECK_ARG(count >= 1 && count <= 65535, "batch size must be 1..65535 (got %d)", count);
+    for (int s = 0; s < count; ++s) {
+        pict_status st = validate_solve(&A[s], nrhs, o);
+        if (st) return st;
+        PICT_CHECK_ARG(A[s].dtype == A[0].dtype, "system %d: dtype differs from system 0", s);
+        PICT_CHECK_ARG(rhs[s] && x[s], "system %d: vectors missing", s);
+    }
+    PICT_CHECK_ARG(!o->transpose_a, "batched solves are not transposed");
+    return A[0].dtype == PICT_DTYPE_F32 ? batch_create_typed<float>(A, rhs, x, count, nrhs, o, cfg, cost, out, stream)
+                                        : batch_create_typed<double>(A, rhs, x, count, nrhs, o, cfg, cost, out, stream);
+}
+
+void batch_solve_destroy(BatchSolve *b) { delete b; }
+
+pict_status batch_solve_update(BatchSolve *b, const pict_csr_desc *A, const void *const *rhs, void *const *x,
+                               hipStream_t stream) {
+    bool changed = false;
+    for (int s = 0; s < b->count; ++s) {
+        char *h = b->args_host.data() + (size_t)s * b->args_stride;
+        // the leading members of SolveArgs: row, col, perm, val, b, x
+        const void *want[6] = {A[s].row, A[s].index, nullptr, A[s].value, rhs[s], x[s]};
+        PICT_CHECK_ARG(A[s].n == b->n[s], "system %d changed size", s);
+        if (memcmp(h, want, sizeof want) != 0) {
+            memcpy(h, want, sizeof want);
+            changed = true;
+        }
+    }
+    if (!changed) return PICT_OK;
+    // queued launches read the table: replace it only when they are done
+    PICT_HIP_TRY(hipStreamSynchronize(stream));
+    PICT_HIP_TRY(hipMemcpyAsync(b->arena + b->off_args, b->args_host.data(), b->args_host.size(),
+                                hipMemcpyHostToDevice, stream));
+    PICT_HIP_TRY(hipStreamSynchronize(stream));
+    return PICT_OK;
+}
+
+pict_status batch_solve_reschedule(BatchSolve *b, const double *cost, hipStream_t stream) {
+    std::vector<double> c(cost, cost + b->count);
+    schedule(*b, c);
+    PICT_HIP_TRY(hipStreamSynchronize(stream));
+    PICT_HIP_TRY(hipMemcpyAsync(b->arena + b->off_joboff, b->job_off.data(), sizeof(unsigned) * b->job_off.size(),
+                                hipMemcpyHostToDevice, stream));
+    PICT_HIP_TRY(hipMemcpyAsync(b->arena + b->off_jobs, b->jobs.data(), sizeof(unsigned) * b->jobs.size(),
+                                hipMemcpyHostToDevice, stream));
+    PICT_HIP_TRY(hipStreamSynchronize(stream));
+    return PICT_OK;
+}
+
+pict_status batch_solve_enqueue(const BatchSolve *b, void *states_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1,
+                                hipStream_t stream) {
+    {
+        const unsigned blocks = (unsigned)std::min<size_t>((b->zero_words + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_zero64, dim3(blocks ? blocks : 1), dim3(256), 0, stream,
+                           (unsigned long long *)(b->arena + b->off_zero), b->zero_words);
+        PICT_HIP_TRY(hipGetLastError());
+    }
+    void *args_dev = b->arena + b->off_args;
+    unsigned *job_off = (unsigned *)(b->arena + b->off_joboff), *jobs = (unsigned *)(b->arena + b->off_jobs);
+    void *kargs[] = {&args_dev, &job_off, &jobs};
+    if (ev0) PICT_HIP_TRY(hipEventRecord(ev0, stream));
+    PICT_HIP_TRY(hipLaunchCooperativeKernel(b->kernel, dim3(b->grid), dim3(kSolveThreads), kargs, 0, stream));
+    if (ev1) PICT_HIP_TRY(hipEventRecord(ev1, stream));
+    int *err = err_out ? err_out : (int *)(b->arena + b->off_err);
+    if (b->dtype == PICT_DTYPE_F32)
+        hipLaunchKernelGGL(k_batch_err<float>, dim3(1), dim3(256), 0, stream, (const SolveArgs<float> *)args_dev,
+                           b->count, err);
+    else
+        hipLaunchKernelGGL(k_batch_err<double>, dim3(1), dim3(256), 0, stream, (const SolveArgs<double> *)args_dev,
+                           b->count, err);
+    PICT_HIP_TRY(hipGetLastError());
+    if (states_out)
+        PICT_HIP_TRY(hipMemcpyAsync(states_out, b->arena + b->off_states, sizeof(RhsState) * (size_t)b->count * b->nrhs,
+                                    hipMemcpyDeviceToDevice, stream));
+    return PICT_OK;
+}
+
+void batch_solve_describe(const BatchSolve *b, BatchSolveInfo *out) {
+    out->grid = (int)b->grid;
+    out->rows_per_thread = b->rpt;
+    out->rows_per_wg = b->rows_per_wg;
+    out->local = b->local ? 1 : 0;
+    out->jobs = (int)b->jobs.size();
+    out->makespan = b->makespan;
+    const bool f32 = b->dtype == PICT_DTYPE_F32;
+    if (b->local)
+        snprintf(out->kernel, sizeof out->kernel, "pict::k_cg_local_jobs<%s, %d, %d, false>", f32 ? "float" : "double",
+                 b->rpt, b->nz);
+    else
+        snprintf(out->kernel, sizeof out->kernel, "pict::k_bicgstab_jobs<%s, %d>", f32 ? "float" : "double", b->nrhs);
 }
 
 }  // namespace pict

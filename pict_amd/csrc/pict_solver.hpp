@@ -29,3 +29,39 @@ pict_status solve_workspace_bytes(const pict_csr_desc *A, int nrhs, const pict_s
 void solve_state_to_info(const void *states, int nrhs, pict_solve_info *out);
 
 }  // namespace pict
+
+namespace pict {
+
+// ---- batched solves: many independent systems in one cooperative launch ----
+struct BatchSolve;
+
+struct BatchSolveConfig {
+    int rows_per_thread = 0;  // CG (k_cg_local): rows per thread, 0 = default
+    int rows_per_wg = 0;      // BiCGStab: rows per workgroup, 0 = default
+};
+
+struct BatchSolveInfo {
+    int grid, rows_per_thread, rows_per_wg, local, jobs;
+    double makespan;  // of the schedule, in units of the cost estimates
+    char kernel[64];
+};
+
+// Plans a batched solve of systems A[s] x[s] = rhs[s] (nrhs right-hand sides each, options o; CG
+// when !o->use_bicg, then one right-hand side), allocates its workspaces and uploads its tables.
+// `cost` (may be null): the estimated time of each solve, for the schedule.
+pict_status batch_solve_create(const pict_csr_desc *A, const void *const *rhs, void *const *x, int count, int nrhs,
+                               const pict_solve_opts *o, const BatchSolveConfig &cfg, const double *cost,
+                               BatchSolve **out, hipStream_t stream);
+void batch_solve_destroy(BatchSolve *b);
+// new matrix or vector addresses (same sizes); waits for queued work when the table changes
+pict_status batch_solve_update(BatchSolve *b, const pict_csr_desc *A, const void *const *rhs, void *const *x,
+                               hipStream_t stream);
+// new schedule from measured costs (waits for queued work)
+pict_status batch_solve_reschedule(BatchSolve *b, const double *cost, hipStream_t stream);
+// Queues the solves (no allocation, no host wait: capturable).  states_out (may be null): count*nrhs
+// final states (solve_state_bytes() each); err_out (may be null): nonzero when a barrier timed out.
+pict_status batch_solve_enqueue(const BatchSolve *b, void *states_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1,
+                                hipStream_t stream);
+void batch_solve_describe(const BatchSolve *b, BatchSolveInfo *out);
+
+}  // namespace pict

@@ -17,9 +17,6 @@
 
 namespace pict {
 
-constexpr int kMeanThreads = 256;
-constexpr int kMeanBlocks = 128;
-
 template <typename R>
 __global__ __launch_bounds__(256) void k_fill(R *v, size_t n, R value) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

@@ -39,6 +39,8 @@ def test_struct_layout_matches_header():
         "pict_solve_info": _abi.SolveInfo,
         "pict_step_opts": _abi.StepOpts,
         "pict_step_tape": _abi.StepTape,
+        "pict_batch_config": _abi.BatchConfig,
+        "pict_batch_info": _abi.BatchInfo,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "%s"' % HEADER, "int main(void){"]
     for cname, py in structs.items():
