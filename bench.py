@@ -1,6 +1,6 @@
 """Benchmark: cells*PISO-steps/s of the PISO time step on one MI355X.
 
-    python bench.py [--mode fwdbwd|fwd] [--steps K] [--warmup W] [--config c3|c1] [--no-cpu-baseline]
+    python bench.py [--mode fwdbwd|fwd] [--steps K] [--warmup W] [--config c3|c1|c4] [--no-cpu-baseline]
 
 Default workload (differentiation configuration): the 2-block 256x256 periodic box (2 blocks of 256x128,
 [0,2pi]^2, nu=1e-3, spectral k^-5/3 initial velocity with RMS 1, dt = 0.25 h/max|u0|), FP32, BiCGStab advection
@@ -8,6 +8,11 @@ and CG pressure at tol 1e-6, 2 correctors; a learnable per-cell forcing (block v
 a K-step rollout (default 16) with per-step checkpointing, MSE loss to a seeded target state, and backward.
 A bench step is one PISO step forward + its backward (recomputation included); the timed region is one whole
 rollout plus backward, inputs resident in HBM.  `--mode fwd` times K forward steps only.
+
+`--config c4` (batching configuration): 64 independent lid cavities (sides uniform over 32..256, viscosities
+log-uniform over 1e-3..1e-1), K = 100 forward steps each, stepped as one batch (one launch per stage and per
+solve, graph replay, autotuned launch shapes) and, for comparison, as the plain loop of one native step per
+cavity; `value` is the batch's, `config.batched_vs_plain` the ratio.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the solver kernel with the largest total time in the
 timed region, timed with HIP events on its launch stream): algorithmic bytes per launch / average launch duration
@@ -93,9 +98,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--mode", default="fwdbwd", choices=["fwdbwd", "fwd"])
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=None, help="default: 16 (c3, c1), 100 (c4)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c3", "c1"])
+    ap.add_argument("--config", default="c3", choices=["c3", "c1", "c4"])
+    ap.add_argument("--batch", type=int, default=64, help="c4: number of cavities")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU baseline sample: whole steps until this much time has passed (at most 64 steps)")
@@ -107,6 +113,8 @@ def main():
                          "roofline.traffic (with --pmc-write)")
     ap.add_argument("--pmc-write", default=None, metavar="CSV", help="the same for WRITE_SIZE")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 100 if args.config == "c4" else 16
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PISO step of one domain runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
@@ -115,6 +123,9 @@ def main():
     import torch
     from pict_amd import PISOtorch as P
 
+    if args.config == "c4":
+        print(json.dumps(bench_c4(args, torch)))
+        return
     dom, sim, dt, workload = make_workload(args, torch)
     N = dom.getTotalSize()
     target = target_state(dom, args, torch)
@@ -219,6 +230,155 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def _c4_sims(torch, count, dtype):
+    from pict_amd import PISOtorch_simulation as S
+    from pict_amd import configs
+    doms, ns, visc, dt = configs.cavity_batch_c4(count, 0, dtype, "cuda")
+    sims = [S.Simulation(d, time_step=dt, corrector_steps=2, non_orthogonal=False, advection_tol=1e-6,
+                         pressure_tol=1e-6, advect_passive_scalar=False) for d in doms]
+    return doms, sims, ns, visc, dt
+
+
+def _c4_error(doms, ref):
+    """Max over cavities of max|u - u_ref| / max|u_ref| (and the same for the pressure)."""
+    ev = ep = 0.0
+    for d, r in zip(doms, ref):
+        for b, rb in zip(d.blocks, r.blocks):
+            ev = max(ev, float((b.velocity.double() - rb.velocity.double()).abs().max() /
+                               rb.velocity.double().abs().max()))
+            ep = max(ep, float((b.pressure.double() - rb.pressure.double()).abs().max() /
+                               rb.pressure.double().abs().max()))
+    return ev, ep
+
+
+def bench_c4(args, torch):
+    """Batching configuration: `--batch` independent lid cavities (sides 32..256, viscosities 1e-3..1e-1, FP32,
+    BiCGStab advection and CG pressure at tolerance 1e-6), `--steps` PISO steps each.  Times the batched step
+    (SimulationBatch: one launch per stage and per solve, graph replay, autotuned launch shapes) and the plain loop
+    over the same simulations (one native step per simulation and step), both from the state after `--warmup`
+    steps; both end states are compared with FP64 batched steps of the same cavities and solver settings."""
+    import numpy as np
+    from pict_amd.batch import SimulationBatch
+    doms_b, sims_b, ns, visc, dt = _c4_sims(torch, args.batch, torch.float32)
+    doms_p, sims_p, _, _, _ = _c4_sims(torch, args.batch, torch.float32)
+    N = sum(n * n for n in ns)
+    batch = SimulationBatch(sims_b)
+    t_tune = time.perf_counter()
+    tuned = batch.tune(repeats=3)
+    t_tune = time.perf_counter() - t_tune
+    for _ in range(args.warmup):
+        batch.step()
+        for s in sims_p:
+            assert s._PISO_split_step(1)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    iters = []
+
+    def run_batched():
+        for _ in range(args.steps):
+            iters.append(batch.step()["used_iterations"].copy())
+
+    def run_plain():
+        for _ in range(args.steps):
+            for s in sims_p:
+                assert s._PISO_split_step(1)
+
+    ms_b = timed(run_batched)
+    ms_p = timed(run_plain)
+    info = batch.describe()
+    # FP64 steps of the same cavities and settings: the error of both FP32 paths
+    doms_r, sims_r, _, _, _ = _c4_sims(torch, args.batch, torch.float64)
+    ref = SimulationBatch(sims_r)
+    for _ in range(args.warmup + args.steps):
+        ref.step()
+    ev_b, ep_b = _c4_error(doms_b, doms_r)
+    ev_p, ep_p = _c4_error(doms_p, doms_r)
+    # kernel times of one more batched step (events around each solve launch, no graph): roofline of the
+    # batched pressure CG, the dominant kernel
+    batch.set_timing(True)
+    arr = batch.step()
+    kms = batch.last_solve_ms()
+    batch.set_timing(False)
+    K = 2
+    p_bytes = []
+    for k in range(K):
+        b = 0
+        for i, d in enumerate(doms_b):
+            n, nnz = d.getTotalSize(), d.C.getSize()
+            b += solver_bytes("CG", n, nnz, 1, 4, [int(arr["used_iterations"][i][2 + k]) + 1])
+        p_bytes.append(b)
+    avg_ms = sum(kms[1:1 + K]) / K
+    bytes_per_launch = sum(p_bytes) / K
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    its = np.array(iters)
+    cpu = None if args.no_cpu_baseline else cpu_baseline_c4(doms_b, dt, args)
+    value = N * args.steps / (ms_b * 1e-3)
+    plain = N * args.steps / (ms_p * 1e-3)
+    return {
+        "metric": "cells*PISO-steps/s (forward, batched %d cavities)" % args.batch,
+        "value": value,
+        "unit": "cells*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_b / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 Krylov reductions)",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "%d lid-driven cavities, sides %s (uniform over 32..256, seed 0), viscosities "
+                               "log-uniform 1e-3..1e-1, FP32, dt %.4g, BiCGStab/CG tolerance 1e-6, 2 correctors"
+                               % (args.batch, sorted(set(ns)), dt),
+                   "cells": N, "config": "c4", "mode": "fwd",
+                   "batched_vs_plain": value / plain,
+                   "plain_loop": {"value": plain, "ms_per_step": ms_p / args.steps},
+                   "error_vs_fp64": {"batched": {"velocity": ev_b, "pressure": ep_b},
+                                     "plain": {"velocity": ev_p, "pressure": ep_p},
+                                     "reference": "FP64 batched steps, same cavities and solver settings"},
+                   "tuned": tuned, "tune_s": t_tune, "launch": info,
+                   "iterations_per_step": {"advection_max": int(its[:, :, :2].max()),
+                                           "pressure_max": int(its[:, :, 2:].max()),
+                                           "pressure_sum": float(its[:, :, 2:].sum() / args.steps)},
+                   "solver_ms_per_step": {"advection": kms[0], "pressure": sum(kms[1:])}},
+        "roofline": {"bound": "hbm", "kernel": info["pressure_kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_c4(doms, dt, args):
+    """FP32 C restatement (OpenMP) of one forward step of each cavity in turn, until --cpu-seconds have passed."""
+    import torch
+    from oracle import oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    O.lib(32).oracle_set_threads(cores)
+    t = 0.0
+    cells = steps = 0
+    for d in doms:
+        twin = O.cpu_twin(d, dtype=torch.float32)  # set-up outside the timed part
+        t1 = time.perf_counter()
+        O.piso_step(twin, dt, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6)
+        t += time.perf_counter() - t1
+        cells += d.getTotalSize()
+        steps += 1
+        if t > args.cpu_seconds:
+            break
+    return {"value": cells / t, "unit": "cells*steps/s", "cores": cores, "kind": "port",
+            "sample": "one PISO step of each of the first %d cavities of the same batch in %.1f s (FP32, OpenMP %d "
+                      "threads)" % (steps, t, cores)}
 
 
 def pmc_traffic(fetch_csv, write_csv, kernel):

@@ -435,6 +435,11 @@ pict_status pict_batch_step(pict_batch *batch, const pict_step_opts *opts, const
 pict_status pict_batch_tune(pict_batch *batch, const pict_step_opts *opts, const double *time_steps,
                             int32_t repeats, pict_batch_config *chosen, void *stream);
 pict_status pict_batch_describe(const pict_batch *batch, pict_batch_info *out);
+/* Diagnostics: when enabled, steps run without the graph and record HIP events around every solve
+   launch; the kernel times of the last step's solves (advection, then one per corrector, in
+   milliseconds) are then available. */
+pict_status pict_batch_set_timing(pict_batch *batch, int32_t enable);
+int32_t     pict_batch_last_solve_ms(const pict_batch *batch, double *ms_out, int32_t max_count);
 
 /* Diagnostics (no counterpart in the reference): when enabled, the solver
    records HIP events around its kernel launch on the launch stream; the

@@ -234,6 +234,8 @@ SIGNATURES = {
     "pict_batch_step": (c_int32, [_P, POINTER(StepOpts), POINTER(c_double), POINTER(SolveInfo), _P]),
     "pict_batch_tune": (c_int32, [_P, POINTER(StepOpts), POINTER(c_double), c_int32, POINTER(BatchConfig), _P]),
     "pict_batch_describe": (c_int32, [_P, POINTER(BatchInfo)]),
+    "pict_batch_set_timing": (c_int32, [_P, c_int32]),
+    "pict_batch_last_solve_ms": (c_int32, [_P, POINTER(c_double), c_int32]),
     "pict_set_step_timing": (c_int32, [c_int32]),
     "pict_last_step_solve_ms": (c_int32, [POINTER(c_double), c_int32]),
     "pict_set_kernel_timing": (c_int32, [c_int32]),

@@ -109,6 +109,16 @@ class SimulationBatch:
         out["pressure_kernel"] = i.pressure_kernel.decode()
         return out
 
+    def set_timing(self, enable):
+        """Diagnostics: HIP events around every solve launch (steps then run without the graph)."""
+        _abi.check(self._lib.pict_batch_set_timing(self._handle, int(bool(enable))), "batch timing")
+
+    def last_solve_ms(self):
+        """Kernel times of the last step's solves (advection, then one per corrector), milliseconds."""
+        ms = (ctypes.c_double * (1 + self.corrector_steps))()
+        n = self._lib.pict_batch_last_solve_ms(self._handle, ms, len(ms))
+        return list(ms)[:n]
+
     def _prepare(self):
         for i, s in enumerate(self.sims):
             s.domain.UpdateDomainData()

@@ -172,10 +172,15 @@ struct pict_batch {
     std::unique_ptr<BatchPlan> plan;
     hipStream_t capture = nullptr;
     bool last_graph = false;
+    // diagnostics: HIP events around every solve launch (steps then run without the graph)
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    std::vector<double> solve_ms;
     ~pict_batch() {
         plan.reset();
         if (dev) (void)hipFree(dev);
         if (capture) (void)hipStreamDestroy(capture);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
 };
 
@@ -355,8 +360,9 @@ pict_status run_stage(const pict_batch *b, const pict_step_opts *o, int part, in
     }
 }
 
-// the whole step queued on s (allocates nothing, never waits: capturable)
-pict_status enqueue_step(const pict_batch *b, const pict_step_opts *o, hipStream_t s) {
+// the whole step queued on s (allocates nothing, never waits: capturable); ev (may be null): 2 events per
+// solve, recorded around its launch
+pict_status enqueue_step(const pict_batch *b, const pict_step_opts *o, hipStream_t s, hipEvent_t *ev = nullptr) {
     const BatchPlan &p = *b->plan;
     const int tpw = p.cfg.tiles_per_workgroup > 0 ? p.cfg.tiles_per_workgroup : 1;
     const size_t sb = solve_state_bytes();
@@ -365,12 +371,12 @@ pict_status enqueue_step(const pict_batch *b, const pict_step_opts *o, hipStream
     pict_status st;
 #define PICT_TRY(expr) do { if ((st = (expr))) return st; } while (0)
     PICT_TRY(run_stage(b, o, 0, 0, tpw, s));
-    PICT_TRY(batch_solve_enqueue(p.adv, p.dev, err, nullptr, nullptr, s));
+    PICT_TRY(batch_solve_enqueue(p.adv, p.dev, err, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, s));
     for (int k = 0; k < K; ++k) {
         PICT_TRY(run_stage(b, o, 1, k, tpw, s));
         PICT_TRY(run_stage(b, o, 2, k, tpw, s));
-        PICT_TRY(batch_solve_enqueue(p.prs, p.dev + p.o_prs + sb * (size_t)b->count * k, err + 1 + k, nullptr,
-                                     nullptr, s));
+        PICT_TRY(batch_solve_enqueue(p.prs, p.dev + p.o_prs + sb * (size_t)b->count * k, err + 1 + k,
+                                     ev ? ev[2 * (1 + k)] : nullptr, ev ? ev[2 * (1 + k) + 1] : nullptr, s));
         if (o->normalize_pressure) PICT_TRY(run_stage(b, o, 3, k, tpw, s));
         PICT_TRY(run_stage(b, o, 4, k, tpw, s));
     }
@@ -524,7 +530,15 @@ pict_status pict_batch_step(pict_batch *b, const pict_step_opts *o, const double
     if ((st = ensure_plan(b, o, s))) return st;
     BatchPlan &p = *b->plan;
     b->last_graph = false;
-    if (batch_graphs_enabled() && p.steps >= 1) {
+    const int nsolves = 1 + o->corrector_steps;
+    if (b->timing) {
+        while ((int)b->ev.size() < 2 * nsolves) {
+            hipEvent_t e;
+            PICT_HIP_TRY(hipEventCreate(&e));
+            b->ev.push_back(e);
+        }
+        if ((st = enqueue_step(b, o, s, b->ev.data()))) return st;
+    } else if (batch_graphs_enabled() && p.steps >= 1) {
         if (!p.exec && (st = capture_step(b, o))) return st;
         PICT_HIP_TRY(hipGraphLaunch(p.exec, s));
         b->last_graph = true;
@@ -533,6 +547,10 @@ pict_status pict_batch_step(pict_batch *b, const pict_step_opts *o, const double
     }
     PICT_HIP_TRY(hipStreamSynchronize(s));
     ++p.steps;
+    if (b->timing) {
+        b->solve_ms.assign(nsolves, 0.0);
+        for (int i = 0; i < nsolves; ++i) b->solve_ms[i] = elapsed(b->ev[2 * i], b->ev[2 * i + 1]);
+    }
     return collect(b, o, infos_out, s);
 }
 
@@ -639,6 +657,19 @@ pict_status pict_batch_tune(pict_batch *b, const pict_step_opts *o, const double
     b->plan.reset();
     if (chosen) *chosen = best;
     return PICT_OK;
+}
+
+pict_status pict_batch_set_timing(pict_batch *b, int32_t enable) {
+    PICT_CHECK_ARG(b, "NULL argument");
+    b->timing = enable != 0;
+    return PICT_OK;
+}
+
+int32_t pict_batch_last_solve_ms(const pict_batch *b, double *ms_out, int32_t max_count) {
+    if (!b) return 0;
+    const int n = (int)b->solve_ms.size();
+    for (int i = 0; i < n && i < max_count; ++i) ms_out[i] = b->solve_ms[i];
+    return n;
 }
 
 pict_status pict_batch_describe(const pict_batch *b, pict_batch_info *out) {
