@@ -407,11 +407,15 @@ typedef struct pict_batch_config {
     int32_t advection_rows_per_workgroup; /* BiCGStab: rows per workgroup (>= 64)                     */
     int32_t adaptive_schedule;          /* 1: after the first step, schedule the solves by the
                                            iteration counts it took; 0: by system size only         */
+    int32_t advection_rows_per_thread;  /* BiCGStab kept in registers, this many rows per thread: 1, 2,
+                                           4 (FP32, norm-2 criteria, up to 2 right-hand sides); -1: the
+                                           streaming kernel; 0: default (2 where eligible)           */
+    int32_t reserved;
 } pict_batch_config;
 
 typedef struct pict_batch_info {
     int32_t count, tiles, tiles_per_workgroup;
-    int32_t advection_grid, advection_jobs, advection_rows_per_workgroup;
+    int32_t advection_grid, advection_jobs, advection_rows_per_workgroup, advection_rows_per_thread;
     int32_t pressure_grid, pressure_jobs, pressure_rows_per_thread;
     int32_t graph;                      /* the last step was a graph replay                          */
     char advection_kernel[64];

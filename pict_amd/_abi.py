@@ -167,6 +167,8 @@ class BatchConfig(Structure):
         ("pressure_rows_per_thread", c_int32),
         ("advection_rows_per_workgroup", c_int32),
         ("adaptive_schedule", c_int32),
+        ("advection_rows_per_thread", c_int32),
+        ("reserved", c_int32),
     ]
 
 
@@ -178,6 +180,7 @@ class BatchInfo(Structure):
         ("advection_grid", c_int32),
         ("advection_jobs", c_int32),
         ("advection_rows_per_workgroup", c_int32),
+        ("advection_rows_per_thread", c_int32),
         ("pressure_grid", c_int32),
         ("pressure_jobs", c_int32),
         ("pressure_rows_per_thread", c_int32),

@@ -121,7 +121,9 @@ class SimulationBatch:
 
     def _prepare(self):
         for i, s in enumerate(self.sims):
-            s.domain.UpdateDomainData()
+            # the setters flag a changed domain (IsTensorChanged); an unchanged one keeps its uploaded descriptor
+            if s.domain.IsTensorChanged():
+                s.domain.UpdateDomainData()
             if s.domain._atlas != self._atlas_ids[i]:
                 # the batch holds the library's view of each domain; a re-prepared domain has a new one
                 raise RuntimeError("simulation %d: its domain was prepared again; create a new batch" % i)

@@ -168,7 +168,7 @@ struct pict_batch {
     size_t dom_stride = 0, o_tiles = 0;
     std::vector<char> dom_host;     // the uploaded domain table
     bool uploaded = false;
-    pict_batch_config cfg{1, 0, 0, 1};
+    pict_batch_config cfg{1, 0, 0, 1, 0, 0};
     std::unique_ptr<BatchPlan> plan;
     hipStream_t capture = nullptr;
     bool last_graph = false;
@@ -269,8 +269,13 @@ pict_status create_solve(const pict_batch *b, bool pressure, const pict_step_opt
     const SolveSet v = pressure ? pressure_set(b) : advection_set(b);
     BatchSolveConfig c;
     c.rows_per_thread = cfg.pressure_rows_per_thread;
-    c.rows_per_wg = pressure ? 0 : cfg.advection_rows_per_workgroup;
-    if (pressure && o->pressure.use_bicg) c.rows_per_wg = cfg.advection_rows_per_workgroup;
+    c.rows_per_wg = cfg.advection_rows_per_workgroup;
+    // BiCGStab in registers (where the kernel exists for the system; else the streaming kernel)
+    const int adv = cfg.advection_rows_per_thread == 0 ? 2 : cfg.advection_rows_per_thread;
+    if (!pressure) c.bicg_rows_per_thread = adv > 0 ? adv : 0;
+    else if (o->pressure.use_bicg)
+        c.bicg_rows_per_thread = cfg.pressure_rows_per_thread == 0 ? 2
+                                 : cfg.pressure_rows_per_thread <= 4 ? cfg.pressure_rows_per_thread : 0;
     return batch_solve_create(v.A.data(), v.rhs.data(), v.x.data(), b->count, pressure ? 1 : b->D,
                               pressure ? &o->pressure : &o->advection, c, nullptr, out, s);
 }
@@ -510,6 +515,9 @@ pict_status pict_batch_set_config(pict_batch *b, const pict_batch_config *c) {
                    "pressure_rows_per_thread must be 0, 1, 2, 4 or 8");
     PICT_CHECK_ARG(c->advection_rows_per_workgroup == 0 || c->advection_rows_per_workgroup >= 64,
                    "advection_rows_per_workgroup must be 0 or >= 64");
+    PICT_CHECK_ARG(c->advection_rows_per_thread >= -1 && c->advection_rows_per_thread <= 4 &&
+                       c->advection_rows_per_thread != 3,
+                   "advection_rows_per_thread must be -1, 0, 1, 2 or 4");
     b->cfg = *c;
     return PICT_OK;
 }
@@ -605,14 +613,19 @@ pict_status pict_batch_tune(pict_batch *b, const pict_step_opts *o, const double
             }
         }
     }
-    // 2. advection BiCGStab: rows per workgroup
+    // 2. advection BiCGStab: rows per thread of the register-resident kernel, rows per workgroup of the
+    // streaming one
     {
         float bt = 1e30f;
-        for (int rows : {256, 512, 1024, 2048, 4096}) {
+        const int cand[][2] = {{1, 0}, {2, 0}, {4, 0}, {-1, 512}, {-1, 1024}, {-1, 2048}};
+        for (const auto &cv : cand) {
             pict_batch_config c = best;
-            c.advection_rows_per_workgroup = rows;
+            c.advection_rows_per_thread = cv[0];
+            c.advection_rows_per_workgroup = cv[1];
             BatchSolve *bs = nullptr;
-            if ((st = create_solve(b, false, o, c, &bs, s))) return st;
+            st = create_solve(b, false, o, c, &bs, s);
+            if (st == PICT_ERR_UNSUPPORTED || st == PICT_ERR_INVALID) continue;  // a system too large for it
+            if (st) return st;
             float ms;
             st = timed([&]() -> pict_status {
                 pict_status q = run_stage(b, o, 0, 0, best.tiles_per_workgroup, s);
@@ -623,11 +636,13 @@ pict_status pict_batch_tune(pict_batch *b, const pict_step_opts *o, const double
             if (st) return st;
             if (ms < bt) {
                 bt = ms;
-                best.advection_rows_per_workgroup = rows;
+                best.advection_rows_per_thread = cv[0];
+                best.advection_rows_per_workgroup = cv[1];
             }
         }
     }
-    // 3. pressure CG: rows per thread (a BiCGStab pressure solve takes the advection's rows per workgroup)
+    // 3. pressure CG: rows per thread (a BiCGStab pressure solve takes the pressure rows per thread as its
+    // register-resident shape, else the advection's rows per workgroup)
     if (o->corrector_steps > 0 && !o->pressure.use_bicg) {
         float bt = 1e30f;
         for (int v : {1, 2, 4, 8}) {
@@ -686,6 +701,7 @@ pict_status pict_batch_describe(const pict_batch *b, pict_batch_info *out) {
         out->advection_grid = ia.grid;
         out->advection_jobs = ia.jobs;
         out->advection_rows_per_workgroup = ia.rows_per_wg;
+        out->advection_rows_per_thread = ia.rows_per_thread;
         out->pressure_grid = ip.grid;
         out->pressure_jobs = ip.jobs;
         out->pressure_rows_per_thread = ip.rows_per_thread;

@@ -1093,7 +1093,10 @@ struct CgSlots<false> {
     using Iter = CgIter;
 };
 
-template <class L>
+// FILL: the workgroup writes every slot of its line (unused ones as 0), for callers whose layouts differ from
+// one generation to the next: each slot is then rewritten every second generation, which the 2-bit tag needs
+// to tell a stale slot from a current one.
+template <class L, bool FILL = false>
 __device__ __forceinline__ void tagged_poll(u64 *part, int *err_flag, unsigned nwg, unsigned bid, unsigned gen,
                                             const double (&acc)[L::NS], bool wait_rows, double (&v)[L::NS],
                                             int &s_err) {
@@ -1111,12 +1114,20 @@ __device__ __forceinline__ void tagged_poll(u64 *part, int *err_flag, unsigned n
     u64 *base = part + (size_t)(gen & 1u) * nwg * kSlots;
     if (lane == 0) {
         u64 *mine = base + (size_t)bid * kSlots;
+        double line[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) line[j] = 0.0;
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             double t = wsum[0][s];
 #pragma unroll
             for (int w = 1; w < kSolveThreads / 64; ++w) t = L::MAX[s] ? fmax(t, wsum[w][s]) : t + wsum[w][s];
-            st_coh(mine + L::ID[s], tag_partial(t, gen));
+            if constexpr (FILL) line[L::ID[s]] = t;
+            else st_coh(mine + L::ID[s], tag_partial(t, gen));
+        }
+        if constexpr (FILL) {
+#pragma unroll
+            for (int j = 0; j < kSlots; ++j) st_coh(mine + j, tag_partial(line[j], gen));
         }
     }
     unsigned spins = 0;
@@ -1153,11 +1164,11 @@ __device__ __forceinline__ void poll_total(const double (&v)[L::NS], double *red
         red[L::ID[s]] = L::MAX[s] ? wave_reduce_dpp<true>(v[s]) : wave_reduce_dpp<false>(v[s]);
 }
 
-template <class L>
+template <class L, bool FILL = false>
 __device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned nwg, unsigned bid, unsigned gen,
                                               const double (&acc)[L::NS], bool wait_rows, double *red, int &s_err) {
     double v[L::NS];
-    tagged_poll<L>(part, err_flag, nwg, bid, gen, acc, wait_rows, v, s_err);
+    tagged_poll<L, FILL>(part, err_flag, nwg, bid, gen, acc, wait_rows, v, s_err);
     if (threadIdx.x < 64) {
         double t[kSlots];
         poll_total<L>(v, t);
@@ -1515,6 +1526,414 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg_local_jobs(const SolveArgs
         const unsigned job = jobs[j];
         const SolveArgs<R> a = args[job >> 16];
         cg_local<R, RPT, NZ, FULL>(a, job & 0xffffu);
+        __syncthreads();
+    }
+}
+
+// ==========================================================================
+// Register-resident BiCGStab (FP32, up to two right-hand sides, norm-2 criteria).
+//
+// The iteration of k_bicgstab (bicgstabSolveGPU, bicgstab_solver_kernel.cu:222-365) with
+// k_cg_local's placement of the data: workgroup b owns rows [b*C, min(n, (b+1)*C)),
+// C = 256*RPT; their matrix entries and x, r, rhat, p, v of every right-hand side stay
+// in registers for the whole solve.  Each iteration needs two products with foreign
+// columns: p (for v = A p) and s (for t = A s).  The owner of a row that another
+// workgroup reads publishes p and s as tagged words (value, iteration + 1); a reader
+// waits on the word itself.  One buffer per vector suffices: a neighbour reads the
+// owner's p (s) before it contributes to the reduce-barrier that follows, and the
+// owner writes the next value only after that barrier.  Three reduce-barriers per
+// iteration on tagged partials ((rhat, v) | |s|^2, (t, s), (t, t) | |r|^2, (rhat, r)),
+// as in k_bicgstab.
+// ==========================================================================
+template <int NR> struct BicgFirst;
+template <int NR> struct BicgA;
+template <int NR> struct BicgB;
+template <int NR> struct BicgC;
+// per right-hand side k: slot 2k = |r|^2, 2k+1 = max|b|
+template <> struct BicgFirst<1> { static constexpr int NS = 2; static constexpr int ID[2] = {0, 1}; static constexpr bool MAX[2] = {false, true}; };
+template <> struct BicgFirst<2> { static constexpr int NS = 4; static constexpr int ID[4] = {0, 1, 2, 3}; static constexpr bool MAX[4] = {false, true, false, true}; };
+// slot k = (rhat, v)
+template <> struct BicgA<1> { static constexpr int NS = 1; static constexpr int ID[1] = {0}; static constexpr bool MAX[1] = {false}; };
+template <> struct BicgA<2> { static constexpr int NS = 2; static constexpr int ID[2] = {0, 1}; static constexpr bool MAX[2] = {false, false}; };
+// slots 3k, 3k+1, 3k+2 = |s|^2, (t, s), (t, t)
+template <> struct BicgB<1> { static constexpr int NS = 3; static constexpr int ID[3] = {0, 1, 2}; static constexpr bool MAX[3] = {false, false, false}; };
+template <> struct BicgB<2> { static constexpr int NS = 6; static constexpr int ID[6] = {0, 1, 2, 3, 4, 5}; static constexpr bool MAX[6] = {false, false, false, false, false, false}; };
+// slots 2k, 2k+1 = |r|^2, (rhat, r)
+template <> struct BicgC<1> { static constexpr int NS = 2; static constexpr int ID[2] = {0, 1}; static constexpr bool MAX[2] = {false, false}; };
+template <> struct BicgC<2> { static constexpr int NS = 4; static constexpr int ID[4] = {0, 1, 2, 3}; static constexpr bool MAX[4] = {false, false, false, false}; };
+
+template <int RPT, int NZ, int NR>
+__device__ __forceinline__ void bicg_local(const SolveArgs<float> &a, unsigned bid) {
+    using R = float;
+    constexpr int C = kSolveThreads * RPT;
+    constexpr int NH = NZ <= 5 ? 2 : 2 * RPT + 2;
+    __shared__ R lv[NR][C];  // p, then s, of this workgroup's rows
+    __shared__ double red[kSlots];
+    __shared__ RhsState sst[NR];
+    __shared__ int s_err;
+    struct Step {
+        R coef[NR];   // beta (phase A), alpha (phase B), omega (phase C)
+        R coef2[NR];  // omega of the previous iteration (phase A)
+        int active[NR];
+    };
+    __shared__ Step bc;
+    Step cs{};  // this thread's copy of the step coefficients and active flags
+    const int n = a.n;
+    const int lo = (int)bid * C;
+    const int cnt = min(C, n - lo);
+    unsigned gen = 0;
+    const Pub<R> pp{reinterpret_cast<u64 *>(a.pub), n}, ps{reinterpret_cast<u64 *>(a.pub) + (size_t)NR * n, n};
+    int *mark = reinterpret_cast<int *>(a.p0);
+    const int salt = a.salt;
+    int herr = 0;
+    auto local = [&](int c) { return (unsigned)(c - lo) < (unsigned)cnt; };
+
+    int col[RPT][NZ];
+    R val[RPT][NZ];
+    bool row[RPT], mem[RPT], pub[RPT];
+    R x[NR][RPT], r[NR][RPT], rh[NR][RPT], p[NR][RPT], v[NR][RPT];
+    int nh = 0, hq[NH], hc[NH];
+    R hv[NH];
+
+    // ---- rows into registers; stamp foreign columns; r0 = b - A x0, rhat = r0 ----
+    {
+        double acc[BicgFirst<NR>::NS];
+#pragma unroll
+        for (int k = 0; k < BicgFirst<NR>::NS; ++k) acc[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            hq[k] = -1;
+            hc[k] = 0;
+            hv[k] = R(0);
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int li = (int)threadIdx.x + q * kSolveThreads;
+            row[q] = li < cnt;
+            mem[q] = false;
+            pub[q] = false;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) x[k][q] = r[k][q] = rh[k][q] = p[k][q] = v[k][q] = R(0);
+#pragma unroll
+            for (int e = 0; e < NZ; ++e) {
+                col[q][e] = lo;
+                val[q][e] = R(0);
+            }
+            if (!row[q]) continue;
+            const int i = lo + li;
+            const int rs = a.row[i], re = a.row[i + 1];
+            int nf = 0;
+            for (int e = rs; e < re; ++e) nf += local(a.col[e]) ? 0 : 1;
+            mem[q] = re - rs > NZ || nh + nf > NH;
+            R sum[NR];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) sum[k] = 0;
+            for (int e = rs; e < re; ++e) {
+                const int c = a.col[e];
+                const R m = mat_val(a, e);
+                if (!mem[q]) {
+                    if (local(c)) {
+#pragma unroll
+                        for (int z = 0; z < NZ; ++z)
+                            if (z == e - rs) {
+                                col[q][z] = c;
+                                val[q][z] = m;
+                            }
+                    } else {
+#pragma unroll
+                        for (int z = 0; z < NH; ++z)
+                            if (z == nh) {
+                                hq[z] = q;
+                                hc[z] = c;
+                                hv[z] = m;
+                            }
+                        ++nh;
+                    }
+                }
+                if (!local(c)) st_coh(mark + c, salt);
+#pragma unroll
+                for (int k = 0; k < NR; ++k) sum[k] += m * a.x[(size_t)k * n + c];
+            }
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const R bv = a.b[(size_t)k * n + i];
+                const R rv = bv - sum[k];
+                x[k][q] = a.x[(size_t)k * n + i];
+                r[k][q] = rh[k][q] = rv;
+                acc[2 * k] += (double)rv * (double)rv;
+                acc[2 * k + 1] = fmax(acc[2 * k + 1], fabs((double)bv));
+            }
+        }
+        ops_done();  // the stamps before the partials
+        if (threadIdx.x < kSlots) red[threadIdx.x] = 0.0;
+        tagged_reduce<BicgFirst<NR>, true>(a.tagged, &a.ctl->err, a.nwg, bid, gen, acc, false, red, s_err);
+        ++gen;
+        if (threadIdx.x == 0 && !s_err) {
+            bool all_zero = true;
+            for (int k = 0; k < NR; ++k) all_zero = all_zero && red[2 * k + 1] == 0.0;
+            const int za = (a.zero_shortcut && all_zero) ? 1 : 0;
+            for (int k = 0; k < NR; ++k) {
+                RhsState &st = sst[k];
+                const double nr0 = criterion(a.conv, red[2 * k], 0.0, 0.0, n);
+                st.finite = 1;
+                st.restore_best = 0;
+                st.save_best = 0;
+                st.rho = red[2 * k];
+                st.beta = 0;
+                st.alpha = 0;
+                st.omega = 0;
+                st.used = -1;
+                st.best_crit = za;
+                if (za) {
+                    st.active = 0; st.converged = 1; st.final_res = 0;
+                } else if (nr0 < a.tol) {
+                    st.active = 0; st.converged = 1; st.final_res = nr0;
+                } else {
+                    st.active = 1; st.converged = 0; st.final_res = 0;
+                }
+                bc.coef[k] = 0;
+                bc.coef2[k] = 0;
+                bc.active[k] = st.active;
+            }
+        }
+        __syncthreads();
+        if (s_err) return;
+    }
+    if (sst[0].best_crit != 0.0) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (row[q])
+#pragma unroll
+                for (int k = 0; k < NR; ++k) a.x[(size_t)k * n + lo + (int)threadIdx.x + q * kSolveThreads] = 0;
+        if (bid == 0 && threadIdx.x < NR) a.st[threadIdx.x] = sst[threadIdx.x];
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+        if (row[q]) pub[q] = ld_coh(mark + lo + (int)threadIdx.x + q * kSolveThreads) == salt;
+
+    // y = A z for the own rows of right-hand side k: z of local columns in LDS, foreign ones from `pv`
+    // (published words of the owners, tag `tag`)
+    auto product = [&](int k, int q, const Pub<R> &pv, unsigned tag, const R (&fz)[NR][NH]) {
+        const int i = lo + (int)threadIdx.x + q * kSolveThreads;
+        R y = 0;
+        if (!mem[q]) {
+#pragma unroll
+            for (int e = 0; e < NZ; ++e) y += val[q][e] * lv[k][col[q][e] - lo];
+#pragma unroll
+            for (int z = 0; z < NH; ++z) y += hq[z] == q ? hv[z] * fz[k][z] : R(0);
+        } else {
+            const int rs = a.row[i], re = a.row[i + 1];
+            for (int e = rs; e < re; ++e) {
+                const int c = a.col[e];
+                y += mat_val(a, e) * (local(c) ? lv[k][c - lo] : pv.get(0, k * n + c, tag, herr));
+            }
+        }
+        return y;
+    };
+    // the foreign values of vector `pv` (tag) of the active right-hand sides (only those are published): all
+    // loads back to back, then waits for those not yet there
+    auto gather = [&](const Pub<R> &pv, unsigned tag, R (&fz)[NR][NH]) {
+        u64 w[NR][NH];
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+            for (int z = 0; z < NH; ++z) w[k][z] = pv.word(0, k * n + hc[z]);
+        bool ok = true;
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+            for (int z = 0; z < NH; ++z) {
+                bool m = true;
+                fz[k][z] = Pub<R>::open(w[k][z], tag, m);
+                ok = ok & (m | (z >= nh) | !cs.active[k]);
+            }
+        if (__any(!ok)) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (!cs.active[k]) continue;
+#pragma unroll
+                for (int z = 0; z < NH; ++z)
+                    if (z < nh) fz[k][z] = pv.get(0, k * n + hc[z], tag, herr);
+            }
+        }
+    };
+
+    cs = bc;
+    for (int it = 0; it < a.maxit; ++it) {
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) any = any || cs.active[k];
+        if (!any) break;
+        const unsigned tag = (unsigned)it + 1u;
+        // ---- phase A: p = r + beta (p - omega v); v = A p; (rhat, v) ----
+        {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (!cs.active[k]) continue;
+                const R beta = cs.coef[k], mom = -cs.coef2[k];
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    if (!row[q]) continue;
+                    const R pn = fma(beta, fma(mom, v[k][q], p[k][q]), r[k][q]);
+                    p[k][q] = pn;
+                    const int li = (int)threadIdx.x + q * kSolveThreads;
+                    lv[k][li] = pn;
+                    if (pub[q]) pp.put(0, k * n + lo + li, pn, tag);
+                }
+            }
+            __syncthreads();
+            R fz[NR][NH];
+            gather(pp, tag, fz);
+            double acc[BicgA<NR>::NS];
+#pragma unroll
+            for (int k = 0; k < NR; ++k) acc[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (!cs.active[k]) continue;
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    if (!row[q]) continue;
+                    const R y = product(k, q, pp, tag, fz);
+                    v[k][q] = y;
+                    acc[k] += (double)rh[k][q] * (double)y;
+                }
+            }
+            tagged_reduce<BicgA<NR>, true>(a.tagged, &a.ctl->err, a.nwg, bid, gen, acc, false, red, s_err);
+            ++gen;
+            if (threadIdx.x == 0 && !s_err)
+                for (int k = 0; k < NR; ++k)
+                    if (sst[k].active) {
+                        sst[k].alpha = sst[k].rho / red[k];
+                        bc.coef[k] = (R)sst[k].alpha;
+                    }
+            __syncthreads();
+            if (s_err) break;
+            cs = bc;
+        }
+        // ---- phase B: s = r - alpha v; t = A s; x += alpha p; |s|^2, (t, s), (t, t) ----
+        R s_[NR][RPT], t_[NR][RPT];
+        {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const R al = cs.coef[k];
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    s_[k][q] = R(0);
+                    if (!cs.active[k] || !row[q]) continue;
+                    const R sv = fma(-al, v[k][q], r[k][q]);
+                    s_[k][q] = sv;
+                    const int li = (int)threadIdx.x + q * kSolveThreads;
+                    lv[k][li] = sv;
+                    if (pub[q]) ps.put(0, k * n + lo + li, sv, tag);
+                    x[k][q] += al * p[k][q];
+                }
+            }
+            __syncthreads();
+            R fz[NR][NH];
+            gather(ps, tag, fz);
+            double acc[BicgB<NR>::NS];
+#pragma unroll
+            for (int k = 0; k < BicgB<NR>::NS; ++k) acc[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    t_[k][q] = R(0);
+                    if (!cs.active[k] || !row[q]) continue;
+                    const R y = product(k, q, ps, tag, fz);
+                    t_[k][q] = y;
+                    const double sd = (double)s_[k][q], td = (double)y;
+                    acc[3 * k] += sd * sd;
+                    acc[3 * k + 1] += td * sd;
+                    acc[3 * k + 2] += td * td;
+                }
+            }
+            tagged_reduce<BicgB<NR>, true>(a.tagged, &a.ctl->err, a.nwg, bid, gen, acc, false, red, s_err);
+            ++gen;
+            if (threadIdx.x == 0 && !s_err)
+                for (int k = 0; k < NR; ++k) {
+                    RhsState &st = sst[k];
+                    if (!st.active) continue;
+                    const double nr = criterion(a.conv, red[3 * k], 0.0, 0.0, n);
+                    st.used = it;
+                    st.final_res = nr;
+                    if (!isfinite(nr)) {
+                        st.converged = 0; st.finite = 0; st.active = 0;
+                    } else if (nr < a.tol) {
+                        st.converged = 1; st.active = 0;
+                    } else {
+                        st.omega = red[3 * k + 1] / red[3 * k + 2];
+                        bc.coef[k] = (R)st.omega;
+                    }
+                    bc.active[k] = st.active;
+                }
+            __syncthreads();
+            if (s_err) break;
+            cs = bc;
+        }
+        // ---- phase C: x += omega s; r = s - omega t; |r|^2, (rhat, r) ----
+        {
+            double acc[BicgC<NR>::NS];
+#pragma unroll
+            for (int k = 0; k < BicgC<NR>::NS; ++k) acc[k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (!cs.active[k]) continue;
+                const R om = cs.coef[k];
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    if (!row[q]) continue;
+                    x[k][q] += om * s_[k][q];
+                    const R rv = s_[k][q] - om * t_[k][q];
+                    r[k][q] = rv;
+                    acc[2 * k] += (double)rv * (double)rv;
+                    acc[2 * k + 1] += (double)rh[k][q] * (double)rv;
+                }
+            }
+            tagged_reduce<BicgC<NR>, true>(a.tagged, &a.ctl->err, a.nwg, bid, gen, acc, false, red, s_err);
+            ++gen;
+            if (threadIdx.x == 0 && !s_err)
+                for (int k = 0; k < NR; ++k) {
+                    RhsState &st = sst[k];
+                    if (!st.active) continue;
+                    const double nr = criterion(a.conv, red[2 * k], 0.0, 0.0, n);
+                    st.final_res = nr;
+                    if (nr < a.tol) {
+                        st.converged = 1; st.used = it + 1; st.active = 0;
+                    } else {
+                        st.rho_prev = st.rho;
+                        st.rho = red[2 * k + 1];
+                        st.beta = (st.rho / st.rho_prev) * (st.alpha / st.omega);
+                    }
+                    bc.coef[k] = (R)st.beta;
+                    bc.coef2[k] = (R)st.omega;
+                    bc.active[k] = st.active;
+                }
+            __syncthreads();
+            if (s_err) break;
+            cs = bc;
+        }
+    }
+    if (herr) st_coh(&a.ctl->err, 1);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+        if (row[q])
+#pragma unroll
+            for (int k = 0; k < NR; ++k) a.x[(size_t)k * n + lo + (int)threadIdx.x + q * kSolveThreads] = x[k][q];
+    if (bid == 0 && threadIdx.x < NR) a.st[threadIdx.x] = sst[threadIdx.x];
+}
+
+// batched launch (job lists as for k_cg_local_jobs)
+template <int RPT, int NZ, int NR>
+__global__ __launch_bounds__(kSolveThreads) void k_bicg_local_jobs(const SolveArgs<float> *__restrict__ args,
+                                                                   const unsigned *__restrict__ job_off,
+                                                                   const unsigned *__restrict__ jobs) {
+    const unsigned j0 = job_off[blockIdx.x], j1 = job_off[blockIdx.x + 1];
+    for (unsigned j = j0; j < j1; ++j) {
+        const unsigned job = jobs[j];
+        const SolveArgs<float> a = args[job >> 16];
+        bicg_local<RPT, NZ, NR>(a, job & 0xffffu);
         __syncthreads();
     }
 }
@@ -2212,7 +2631,8 @@ void solve_state_to_info(const void *states, int nrhs, pict_solve_info *out) {
 // ---------------------------------------------------------------------------
 struct BatchSolve {
     int count = 0, nrhs = 0, dtype = 0;
-    bool local = false;  // k_cg_local_jobs, else k_bicgstab_jobs
+    bool local = false;       // k_cg_local_jobs
+    bool bicg_local = false;  // k_bicg_local_jobs (else, without `local`, k_bicgstab_jobs)
     int rpt = 0, nz = 0, rows_per_wg = 0;
     const void *kernel = nullptr;
     unsigned grid = 0;   // workgroups of the launch, all resident
@@ -2242,6 +2662,22 @@ const void *local_cg_jobs_kernel_nz(int rpt) {
         default: return nullptr;
     }
 }
+template <int NZ, int NR>
+const void *bicg_local_jobs_kernel_nz(int rpt) {
+    switch (rpt) {
+        case 1: return (const void *)k_bicg_local_jobs<1, NZ, NR>;
+        case 2: return (const void *)k_bicg_local_jobs<2, NZ, NR>;
+        case 4: return (const void *)k_bicg_local_jobs<4, NZ, NR>;
+        default: return nullptr;
+    }
+}
+// register-resident BiCGStab instances: FP32, 5-entry rows with one or two right-hand sides, 7-entry rows
+// with one
+const void *bicg_local_jobs_kernel(int rpt, int nz, int nrhs) {
+    if (nz <= 5) return nrhs == 1 ? bicg_local_jobs_kernel_nz<5, 1>(rpt) : nrhs == 2 ? bicg_local_jobs_kernel_nz<5, 2>(rpt) : nullptr;
+    return nrhs == 1 ? bicg_local_jobs_kernel_nz<7, 1>(rpt) : nullptr;
+}
+
 template <typename R>
 const void *bicgstab_jobs_kernel(int nrhs) {
     switch (nrhs) {
@@ -2304,7 +2740,7 @@ void schedule(BatchSolve &b, const std::vector<double> &cost) {
 std::vector<double> default_cost(const BatchSolve &b) {
     std::vector<double> c(b.count);
     for (int s = 0; s < b.count; ++s)
-        c[s] = b.local ? std::sqrt((double)b.n[s]) : (2048.0 + (double)b.n[s] / b.nwg[s]);
+        c[s] = b.local || b.bicg_local ? std::sqrt((double)b.n[s]) : (2048.0 + (double)b.n[s] / b.nwg[s]);
     return c;
 }
 
@@ -2343,7 +2779,7 @@ void fill_args(BatchSolve &b, int s, const pict_csr_desc &A, const void *rhs, vo
                    : o->convergence_criterion == PICT_CONV_ABS_MEAN      ? 1.0 / (double)A.n
                                                                          : 1.0;
     a.tagged = reinterpret_cast<unsigned long long *>(b.arena + off_ctl[s] + al(sizeof(Ctl)));
-    a.pub = a.tagged + (b.local ? (size_t)2 * b.nwg[s] * kSlots : 0);
+    a.pub = a.tagged + (b.local || b.bicg_local ? (size_t)2 * b.nwg[s] * kSlots : 0);
     a.salt = next_salt();
     memcpy(b.args_host.data() + (size_t)s * sizeof(SolveArgs<R>), &a, sizeof a);
 }
@@ -2375,9 +2811,21 @@ pict_status batch_create_typed(const pict_csr_desc *A, const void *const *rhs, v
         PICT_CHECK_ARG(b->rpt <= rmax, "rows per thread %d exceeds the register budget (%d)", b->rpt, rmax);
         b->kernel = b->nz <= 5 ? local_cg_jobs_kernel_nz<R, 5>(b->rpt) : local_cg_jobs_kernel_nz<R, 7>(b->rpt);
     } else {
-        b->rows_per_wg = cfg.rows_per_wg > 0 ? cfg.rows_per_wg : 1024;
-        PICT_CHECK_ARG(b->rows_per_wg >= 64, "rows per workgroup must be at least 64");
-        b->kernel = bicgstab_jobs_kernel<R>(nrhs);
+        bool all5 = true;
+        for (int s = 0; s < count; ++s) all5 = all5 && (long long)A[s].nnz <= 5LL * A[s].n;
+        b->nz = all5 ? 5 : 7;
+        const bool norm2 = o->convergence_criterion == PICT_CONV_NORM2 ||
+                           o->convergence_criterion == PICT_CONV_NORM2_NORMALIZED;
+        if (cfg.bicg_rows_per_thread > 0 && sizeof(R) == 4 && norm2 &&
+            bicg_local_jobs_kernel(cfg.bicg_rows_per_thread, b->nz, nrhs)) {
+            b->bicg_local = true;
+            b->rpt = cfg.bicg_rows_per_thread;
+            b->kernel = bicg_local_jobs_kernel(b->rpt, b->nz, nrhs);
+        } else {
+            b->rows_per_wg = cfg.rows_per_wg > 0 ? cfg.rows_per_wg : 1024;
+            PICT_CHECK_ARG(b->rows_per_wg >= 64, "rows per workgroup must be at least 64");
+            b->kernel = bicgstab_jobs_kernel<R>(nrhs);
+        }
     }
     PICT_CHECK_ARG(b->kernel != nullptr, "no batched solver kernel for this configuration");
     pict_status st = occupancy(b->kernel, occ, cus);
@@ -2392,10 +2840,11 @@ pict_status batch_create_typed(const pict_csr_desc *A, const void *const *rhs, v
     b->nwg.resize(count);
     for (int s = 0; s < count; ++s) {
         const long long n = A[s].n;
-        long long w = b->local ? (n + (long long)kSolveThreads * b->rpt - 1) / ((long long)kSolveThreads * b->rpt)
+        const bool resident = b->local || b->bicg_local;
+        long long w = resident ? (n + (long long)kSolveThreads * b->rpt - 1) / ((long long)kSolveThreads * b->rpt)
                                : (n + b->rows_per_wg - 1) / b->rows_per_wg;
         if (w < 1) w = 1;
-        if (!b->local && w > cap) w = cap;
+        if (!resident && w > cap) w = cap;
         if (w > cap || w > 65535) {
             set_error("system %d (%lld rows) needs %lld workgroups; at most %lld are resident", s, n, w, cap);
             return PICT_ERR_UNSUPPORTED;
@@ -2429,7 +2878,9 @@ pict_status batch_create_typed(const pict_csr_desc *A, const void *const *rhs, v
     b->off_zero = off;
     for (int s = 0; s < count; ++s) {
         off_ctl[s] = off;
-        const size_t words = b->local ? (size_t)2 * b->nwg[s] * kSlots + 6 * (size_t)b->n[s] : 0;
+        const size_t words = b->local        ? (size_t)2 * b->nwg[s] * kSlots + 6 * (size_t)b->n[s]
+                             : b->bicg_local ? (size_t)2 * b->nwg[s] * kSlots + 2 * (size_t)nrhs * b->n[s]
+                                             : 0;
         off += al(sizeof(Ctl)) + al(sizeof(unsigned long long) * words);
     }
     b->zero_words = (off - b->off_zero) / sizeof(unsigned long long);
@@ -2536,13 +2987,15 @@ void batch_solve_describe(const BatchSolve *b, BatchSolveInfo *out) {
     out->grid = (int)b->grid;
     out->rows_per_thread = b->rpt;
     out->rows_per_wg = b->rows_per_wg;
-    out->local = b->local ? 1 : 0;
+    out->local = b->local || b->bicg_local ? 1 : 0;
     out->jobs = (int)b->jobs.size();
     out->makespan = b->makespan;
     const bool f32 = b->dtype == PICT_DTYPE_F32;
     if (b->local)
         snprintf(out->kernel, sizeof out->kernel, "pict::k_cg_local_jobs<%s, %d, %d, false>", f32 ? "float" : "double",
                  b->rpt, b->nz);
+    else if (b->bicg_local)
+        snprintf(out->kernel, sizeof out->kernel, "pict::k_bicg_local_jobs<%d, %d, %d>", b->rpt, b->nz, b->nrhs);
     else
         snprintf(out->kernel, sizeof out->kernel, "pict::k_bicgstab_jobs<%s, %d>", f32 ? "float" : "double", b->nrhs);
 }

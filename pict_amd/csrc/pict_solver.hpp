@@ -36,8 +36,9 @@ namespace pict {
 struct BatchSolve;
 
 struct BatchSolveConfig {
-    int rows_per_thread = 0;  // CG (k_cg_local): rows per thread, 0 = default
-    int rows_per_wg = 0;      // BiCGStab: rows per workgroup, 0 = default
+    int rows_per_thread = 0;       // CG (k_cg_local): rows per thread, 0 = default
+    int rows_per_wg = 0;           // streaming BiCGStab: rows per workgroup, 0 = default
+    int bicg_rows_per_thread = 0;  // > 0: register-resident BiCGStab (FP32, norm-2 criteria) with these rows per thread
 };
 
 struct BatchSolveInfo {

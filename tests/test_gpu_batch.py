@@ -8,6 +8,9 @@ sizes, structures, viscosities and boundaries stepped with one launch per stage 
   settings: tolerance 1e-6 on the RMS residual) within 1e-5 of the field scale, like the plain per-simulation
   loop's.  (At that tolerance the solves stop well before full convergence, so the reference for the FP32
   error is the FP64 step with the same stopping rule, not the converged solution.)
+- The register-resident BiCGStab (rows and vectors in registers, halos through published tagged words) agrees
+  with the streaming BiCGStab kernel on systems split over many workgroups, periodic wraps and rotated block
+  connections, for 1, 2 and 4 rows per thread.
 - From the second step on a batch replays a captured graph: bitwise the steps queued directly.
 - The 64-cavity workload (sides 32..256, viscosities 1e-3..1e-1): batched and plain loop agree, every solve
   converges, and the schedule fits the device.
@@ -68,8 +71,10 @@ def test_batch_matches_native_steps_bitwise(dtype):
     ref, _, its_ref = _plain(dtype, steps)
     doms = _domains(dtype)
     sims = [_sim(d, dtype) for d in doms]
-    # rows per thread 1 and 512 rows per BiCGStab workgroup: the single-system plans of these sizes
-    batch = SimulationBatch(sims, config=dict(pressure_rows_per_thread=1, advection_rows_per_workgroup=512))
+    # rows per thread 1 and the streaming BiCGStab with 512 rows per workgroup: the single-system plans of these
+    # sizes
+    batch = SimulationBatch(sims, config=dict(pressure_rows_per_thread=1, advection_rows_per_thread=-1,
+                                              advection_rows_per_workgroup=512))
     its = [[] for _ in sims]
     for _ in range(steps):
         arr = batch.step()
@@ -129,7 +134,9 @@ def test_batch_fp32_error_against_fp64_oracle(tuned):
         print(" tuned:", cfg, flush=True)
         assert cfg["tiles_per_workgroup"] in (1, 2, 4)
         assert cfg["pressure_rows_per_thread"] in (1, 2, 4, 8)
-        assert cfg["advection_rows_per_workgroup"] in (256, 512, 1024, 2048, 4096)
+        assert cfg["advection_rows_per_thread"] in (-1, 1, 2, 4)
+        if cfg["advection_rows_per_thread"] == -1:
+            assert cfg["advection_rows_per_workgroup"] in (512, 1024, 2048)
         assert batch.config() == cfg
     batch.run(steps)
     ev_b, ep_b = _err(doms, refs)
@@ -138,6 +145,54 @@ def test_batch_fp32_error_against_fp64_oracle(tuned):
           % (ev_b, ep_b, ev_p, ep_p), flush=True)
     assert ev_b < 1e-5 and ep_b < 1e-5
     assert ev_p < 1e-5 and ep_p < 1e-5
+
+
+def _halo_domains(dtype):
+    """Systems split over several workgroups at every rows-per-thread shape: larger periodic 2-block boxes (columns
+    that wrap around, rows with more foreign entries than the register list holds), a rotated connection, lid
+    cavities."""
+    doms = [scenarios.periodic_two_block(dtype=dtype, device="cuda", nx=48, ny=20, seed=3),
+            scenarios.periodic_two_block(dtype=dtype, device="cuda", nx=130, ny=40, seed=4),
+            scenarios.rotated_connection(dtype=dtype, device="cuda")]
+    doms += [configs.lid_cavity(n, viscosity=v, dtype=dtype, device="cuda", name="c%d" % i)
+             for i, (n, v) in enumerate([(40, 0.08), (72, 0.01)])]
+    return doms
+
+
+@pytest.mark.parametrize("rpt", [1, 2, 4])
+def test_batch_register_bicgstab_matches_streaming(rpt):
+    print("case rows per thread", rpt, flush=True)
+    steps = 3
+    out = []
+    for cfg in (dict(advection_rows_per_thread=-1, advection_rows_per_workgroup=256),
+                dict(advection_rows_per_thread=rpt)):
+        doms = _halo_domains(torch.float32)
+        batch = SimulationBatch([_sim(d, torch.float32, 0.02) for d in doms], config=cfg)
+        its = [batch.step()["used_iterations"].copy() for _ in range(steps)]
+        info = batch.describe()
+        print(" ", info["advection_kernel"], "grid", info["advection_grid"], "jobs", info["advection_jobs"],
+              "advection iterations", [i[:, :2].tolist() for i in its], flush=True)
+        out.append((_state(doms), its, info))
+    (sa, ia, _), (sb, ib, info) = out
+    assert info["advection_kernel"] == "pict::k_bicg_local_jobs<%d, 5, 2>" % rpt
+    assert info["advection_jobs"] > len(sb)  # systems spread over several workgroups
+    # the two kernels sum a row's entries in different orders: on the stiffest system (the 40x40 cavity with
+    # viscosity 0.08) FP32 BiCGStab takes a few iterations more or fewer; both converge
+    for x, y in zip(ia, ib):
+        assert (np.abs(x[:, :2] - y[:, :2]) <= np.maximum(3, 0.15 * np.abs(x[:, :2]))).all()
+    worst = 0.0
+    for a, b in zip(sa, sb):
+        for x, y in zip(a, b):
+            worst = max(worst, float((x - y).abs().max()) / max(float(x.abs().max()), 1e-30))
+    print(" register vs streaming BiCGStab: %.2e" % worst, flush=True)
+    assert worst < 1e-4
+
+
+def test_batch_fp64_uses_streaming_bicgstab():
+    doms = _domains(torch.float64)
+    batch = SimulationBatch([_sim(d, torch.float64) for d in doms])
+    batch.step()
+    assert batch.describe()["advection_kernel"] == "pict::k_bicgstab_jobs<double, 2>"
 
 
 def test_batch_graph_replays_match_direct(monkeypatch):
@@ -170,7 +225,7 @@ def test_batch_tensors_swapped_between_steps():
     ref = _domains(torch.float32)
     batch = SimulationBatch([_sim(d, torch.float32) for d in doms])
     sims_ref = [_sim(d, torch.float32) for d in ref]
-    batch.set_config(pressure_rows_per_thread=1, advection_rows_per_workgroup=512)
+    batch.set_config(pressure_rows_per_thread=1, advection_rows_per_thread=-1, advection_rows_per_workgroup=512)
     for k in range(4):
         if k == 2:
             for d in (doms[0], ref[0]):
