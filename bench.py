@@ -12,7 +12,8 @@ rollout plus backward, inputs resident in HBM.  `--mode fwd` times K forward ste
 `--config c4` (batching configuration): 64 independent lid cavities (sides uniform over 32..256, viscosities
 log-uniform over 1e-3..1e-1), K = 100 forward steps each, stepped as one batch (one launch per stage and per
 solve, graph replay, autotuned launch shapes) and, for comparison, as the plain loop of one native step per
-cavity; `value` is the batch's, `config.batched_vs_plain` the ratio.
+cavity; `value` is the batch's, `config.batched_vs_plain` the ratio.  The default run appends the same measurement
+(without its CPU sample) to its line under `batched` (`--no-batched` skips it).
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the solver kernel with the largest total time in the
 timed region, timed with HIP events on its launch stream): algorithmic bytes per launch / average launch duration
@@ -102,6 +103,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=["c3", "c1", "c4"])
     ap.add_argument("--batch", type=int, default=64, help="c4: number of cavities")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="c3/c1: skip the batching configuration (64 cavities, 100 steps) reported under `batched`")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU baseline sample: whole steps until this much time has passed (at most 64 steps)")
@@ -205,6 +208,19 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(dom, dt, args, target)
 
+    batched = None
+    if not args.no_batched:
+        import copy
+        bargs = copy.copy(args)
+        bargs.steps, bargs.warmup, bargs.no_cpu_baseline = 100, 2, True
+        r = bench_c4(bargs, torch)
+        c4 = r["config"]
+        batched = {"metric": r["metric"], "value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"],
+                   "steps": bargs.steps, "cells": c4["cells"], "workload": c4["workload"],
+                   "plain_loop": c4["plain_loop"], "batched_vs_plain": c4["batched_vs_plain"],
+                   "error_vs_fp64": c4["error_vs_fp64"], "tuned": c4["tuned"],
+                   "solver_ms_per_step": c4["solver_ms_per_step"], "roofline": r["roofline"]}
+
     out = {
         "metric": "cells*PISO-steps/s (%s)" % ("forward+backward" if args.mode == "fwdbwd" else "forward"),
         "value": value,
@@ -229,6 +245,8 @@ def main():
                      "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
     }
+    if batched is not None:
+        out["batched"] = batched
     print(json.dumps(out))
 
 
