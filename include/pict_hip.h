@@ -457,6 +457,39 @@ pict_status pict_sparse_outer_product(const void *a, const void *b, const pict_c
                                       void *out_value, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* per-cell vector / matrix operations (channel dimension of NCDHW tensors)  */
+/* ------------------------------------------------------------------------ */
+/* A tensor [1, C, (Z), Y, X] of `cells` = X*Y*Z cells holds per cell a vector (C = num_dims)
+   or a row-major num_dims x num_dims matrix (C = num_dims^2); channel c of cell i at c*cells + i. */
+#define PICT_MATMUL_TRANSPOSE_A   1
+#define PICT_MATMUL_INVERT_A      2
+#define PICT_MATMUL_TRANSPOSE_B   4
+#define PICT_MATMUL_INVERT_B      8
+#define PICT_MATMUL_TRANSPOSE_OUT 16
+#define PICT_MATMUL_INVERT_OUT    32
+
+/* matmul (PISOtorch.cpp:604-611; matrix_vector_ops.cu:113-200): out = A B per cell, out a
+   matrix (both matrices), a vector (one matrix) or a scalar (two vectors).  Flags apply to
+   matrices only: an operand is loaded (transposed), then inverted; the output is inverted,
+   then stored (transposed).  Matrix x vector and vector x matrix ignore the output flags. */
+pict_status pict_matmul(const void *a, int32_t a_is_matrix, const void *b, int32_t b_is_matrix, void *out,
+                        int32_t num_dims, int64_t cells, int32_t dtype, int32_t flags, void *stream);
+/* matmulGrad (PISOtorch.cpp:613-621; matrix_vector_ops_grads.cu:217-310): a_grad, b_grad
+   (written completely) from out_grad.  An inverted operand's gradient is zero; with
+   matrix x matrix, both are zero when the output or both operands are inverted. */
+pict_status pict_matmul_grad(const void *a, int32_t a_is_matrix, const void *b, int32_t b_is_matrix,
+                             const void *out_grad, void *a_grad, void *b_grad, int32_t num_dims, int64_t cells,
+                             int32_t dtype, int32_t flags, void *stream);
+/* VectorToDiagMatrix (PISOtorch.cpp:622-623; matrix_vector_ops.cu:203-262): matrices [C = D^2]
+   with the vector on the diagonal and zeros elsewhere (written completely). */
+pict_status pict_vector_to_diag_matrix(const void *vectors, void *matrices, int32_t num_dims, int64_t cells,
+                                       int32_t dtype, void *stream);
+/* InvertMatrix (PISOtorch.cpp:624-625; matrix_vector_ops.cu:265-303): out = inverse per cell
+   (adjugate / determinant); out may be `matrices` (in place). */
+pict_status pict_invert_matrix(const void *matrices, void *out, int32_t num_dims, int64_t cells, int32_t dtype,
+                               void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* metrics                                                                   */
 /* ------------------------------------------------------------------------ */
 

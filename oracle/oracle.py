@@ -494,3 +494,120 @@ def piso_step_vjp(domain, time_step, velocity_grad_out, corrector_steps=2, advec
     for b, u in zip(blocks, u_out):
         b.setVelocity(u)
     return g_u, g_src, g_visc
+
+
+# ---------------------------------------------------------------------------
+# per-cell vector / matrix operations (matrix_vector_ops.cu:7-303, matrix_vector_ops_grads.cu:8-310), FP64 numpy
+# ---------------------------------------------------------------------------
+
+def _cells(t):
+    """[1, C, (Z), Y, X] -> numpy [cells, C] (float64)."""
+    a = t.detach().cpu().double().numpy()
+    return a.reshape(a.shape[1], -1).T.copy()
+
+
+def _from_cells(arr, like, channels):
+    """numpy [cells, channels] -> torch [1, channels, (Z), Y, X] in the dtype of `like`."""
+    out = torch.from_numpy(np.ascontiguousarray(arr.T)).reshape([1, channels] + list(like.shape[2:]))
+    return out.to(like.dtype)
+
+
+def _inverse(m):
+    """Per-cell adjugate / determinant of [cells, D, D] (transformations.h:418-482)."""
+    D = m.shape[-1]
+    if D == 1:
+        return 1.0 / m
+    if D == 2:
+        det = m[:, 0, 0] * m[:, 1, 1] - m[:, 1, 0] * m[:, 0, 1]
+        adj = np.stack([np.stack([m[:, 1, 1], -m[:, 0, 1]], -1), np.stack([-m[:, 1, 0], m[:, 0, 0]], -1)], -2)
+        return adj * (1.0 / det)[:, None, None]
+    # D == 3: cofactor C_ij = (-1)^(i+j) minor_ij; adjugate = C^T
+    cof = np.empty_like(m)
+    for i in range(3):
+        for j in range(3):
+            r = [k for k in range(3) if k != i]
+            c = [k for k in range(3) if k != j]
+            minor = m[:, r[0], c[0]] * m[:, r[1], c[1]] - m[:, r[0], c[1]] * m[:, r[1], c[0]]
+            cof[:, i, j] = (-1) ** (i + j) * minor
+    det = (m[:, 0, :] * cof[:, 0, :]).sum(-1)
+    return np.transpose(cof, (0, 2, 1)) * (1.0 / det)[:, None, None]
+
+
+def _operand(x, D, transpose, invert):
+    m = x.reshape(-1, D, D)
+    if transpose:
+        m = np.transpose(m, (0, 2, 1))
+    return _inverse(m) if invert else m
+
+
+def matmul(vectorMatrixA, vectorMatrixB, transposeA=False, invertA=False, transposeB=False, invertB=False,
+           transposeOutput=False, invertOutput=False):
+    """PISOtorch.matmul: per cell A B; flags act on matrices only, the output flags on matrix x matrix only
+    (matrix_vector_ops.cu:7-200)."""
+    D = vectorMatrixA.dim() - 2
+    A, B = _cells(vectorMatrixA), _cells(vectorMatrixB)
+    am, bm = A.shape[1] == D * D, B.shape[1] == D * D
+    if am and bm:
+        out = np.einsum("nij,njk->nik", _operand(A, D, transposeA, invertA), _operand(B, D, transposeB, invertB))
+        if invertOutput:
+            out = _inverse(out)
+        if transposeOutput:
+            out = np.transpose(out, (0, 2, 1))
+        return _from_cells(out.reshape(-1, D * D), vectorMatrixA, D * D)
+    if am:
+        out = np.einsum("nij,nj->ni", _operand(A, D, transposeA, invertA), B)
+        return _from_cells(out, vectorMatrixA, D)
+    if bm:
+        out = np.einsum("ni,nik->nk", A, _operand(B, D, transposeB, invertB))
+        return _from_cells(out, vectorMatrixA, D)
+    return _from_cells((A * B).sum(-1, keepdims=True), vectorMatrixA, 1)
+
+
+def matmulGrad(vectorMatrixA, vectorMatrixB, outputGrad, transposeA=False, invertA=False, transposeB=False,
+               invertB=False, transposeOutput=False, invertOutput=False):
+    """PISOtorch.matmulGrad (matrix_vector_ops_grads.cu:8-207): zero for an inverted operand; matrix x matrix gives
+    zeros with an inverted output or with both operands inverted; a transposed operand's gradient is transposed."""
+    D = vectorMatrixA.dim() - 2
+    A, B, G = _cells(vectorMatrixA), _cells(vectorMatrixB), _cells(outputGrad)
+    am, bm = A.shape[1] == D * D, B.shape[1] == D * D
+    gA, gB = np.zeros_like(A), np.zeros_like(B)
+
+    def put(m, transpose):
+        return (np.transpose(m, (0, 2, 1)) if transpose else m).reshape(-1, D * D)
+
+    if am and bm:
+        if not (invertOutput or (invertA and invertB)):
+            g = G.reshape(-1, D, D)
+            if transposeOutput:
+                g = np.transpose(g, (0, 2, 1))
+            if not invertA:
+                gA = put(np.einsum("nrk,nck->nrc", g, _operand(B, D, transposeB, invertB)), transposeA)
+            if not invertB:
+                gB = put(np.einsum("nkr,nkc->nrc", _operand(A, D, transposeA, invertA), g), transposeB)
+    elif am:
+        if not invertA:
+            gA = put(np.einsum("nr,nc->nrc", G, B), transposeA)
+        gB = np.einsum("nkc,nk->nc", _operand(A, D, transposeA, invertA), G)
+    elif bm:
+        gA = np.einsum("nrk,nk->nr", _operand(B, D, transposeB, invertB), G)
+        if not invertB:
+            gB = put(np.einsum("nr,nc->nrc", A, G), transposeB)
+    else:
+        gA, gB = B * G, A * G
+    return [_from_cells(gA, vectorMatrixA, A.shape[1]), _from_cells(gB, vectorMatrixB, B.shape[1])]
+
+
+def VectorToDiagMatrix(vectors):
+    """matrix_vector_ops.cu:203-262."""
+    D = vectors.dim() - 2
+    V = _cells(vectors)
+    out = np.zeros((V.shape[0], D, D))
+    for k in range(D):
+        out[:, k, k] = V[:, k]
+    return _from_cells(out.reshape(-1, D * D), vectors, D * D)
+
+
+def InvertMatrix(matrices):
+    """matrix_vector_ops.cu:265-303."""
+    D = matrices.dim() - 2
+    return _from_cells(_inverse(_cells(matrices).reshape(-1, D, D)).reshape(-1, D * D), matrices, D * D)

@@ -2181,6 +2181,110 @@ def CoordsToFaceTransforms(coords):
     return out
 
 
+# ----------------------------------------------------------------------------
+# per-cell vector / matrix operations (matrix_vector_ops.cu, matrix_vector_ops_grads.cu; bindings
+# PISOtorch.cpp:604-627): vectors or flat row-major square matrices in the channel dimension of NCDHW tensors
+# ----------------------------------------------------------------------------
+
+def _vm_check(t, name, allow_vector=True, allow_matrix=True):
+    _check(isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous(), "%s must be a contiguous CUDA tensor" % name)
+    _check(2 < t.dim() < 6, "%s must have batch and channel dimension and be 1-3D." % name)
+    _check(t.size(0) == 1, "%s batch dimension must be 1." % name)
+    _check(t.dtype in (torch.float32, torch.float64), "%s must be float32 or float64." % name)
+    dims = t.dim() - 2
+    ok = (allow_vector and t.size(1) == dims) or (allow_matrix and t.size(1) == dims * dims)
+    return dims, ok
+
+
+def _matmul_args(vectorMatrixA, vectorMatrixB):
+    dims, ok = _vm_check(vectorMatrixA, "vectorMatrixA")
+    _check(ok, "vectorMatrixA channel dimension must be a vector or flat square matrix that matches spatial "
+               "dimensionality.")
+    _check(isinstance(vectorMatrixB, torch.Tensor) and vectorMatrixB.dim() == dims + 2,
+           "vectorMatrixB dimensionality must match vectorMatrixA.")
+    _, okb = _vm_check(vectorMatrixB, "vectorMatrixB")
+    _check(okb, "vectorMatrixB channel dimension must be a vector or flat square matrix that matches spatial "
+                "dimensionality.")
+    _check(vectorMatrixA.dtype == vectorMatrixB.dtype, "Data type of vectorMatrixB does not match vectorMatrixA.")
+    _check(vectorMatrixA.shape[2:] == vectorMatrixB.shape[2:], "vectorMatrixB spatial size must match vectorMatrixA.")
+    _check(vectorMatrixA.device == vectorMatrixB.device, "vectorMatrixB device must match vectorMatrixA.")
+    # with one dimension a channel is both a vector and a 1x1 matrix: it counts as a matrix
+    a_mat = vectorMatrixA.size(1) == dims * dims
+    b_mat = vectorMatrixB.size(1) == dims * dims
+    out_ch = dims * dims if (a_mat and b_mat) else dims if (a_mat or b_mat) else 1
+    return dims, a_mat, b_mat, out_ch
+
+
+def _matmul_flags(transposeA, invertA, transposeB, invertB, transposeOutput, invertOutput):
+    return ((_abi.PICT_MATMUL_TRANSPOSE_A if transposeA else 0) | (_abi.PICT_MATMUL_INVERT_A if invertA else 0) |
+            (_abi.PICT_MATMUL_TRANSPOSE_B if transposeB else 0) | (_abi.PICT_MATMUL_INVERT_B if invertB else 0) |
+            (_abi.PICT_MATMUL_TRANSPOSE_OUT if transposeOutput else 0) |
+            (_abi.PICT_MATMUL_INVERT_OUT if invertOutput else 0))
+
+
+def matmul(vectorMatrixA, vectorMatrixB, transposeA=False, invertA=False, transposeB=False, invertB=False,
+           transposeOutput=False, invertOutput=False):
+    """Per-cell product matrix/vector x matrix/vector (PISOtorch.cpp:604-611, matrix_vector_ops.cu:113-200): a matrix
+    if both are matrices, a vector if one is, a scalar if both are vectors.  transpose/invert apply to matrices only:
+    an operand is loaded (transposed), then inverted; the product is inverted, then stored (transposed)."""
+    dims, a_mat, b_mat, out_ch = _matmul_args(vectorMatrixA, vectorMatrixB)
+    out = torch.empty([1, out_ch] + list(vectorMatrixA.shape[2:]), dtype=vectorMatrixA.dtype,
+                      device=vectorMatrixA.device)
+    cells = out[0, 0].numel()
+    lib = _abi.load_library()
+    _abi.check(lib.pict_matmul(_ptr(vectorMatrixA), int(a_mat), _ptr(vectorMatrixB), int(b_mat), _ptr(out), dims,
+                               cells, _dtype_code(out.dtype),
+                               _matmul_flags(transposeA, invertA, transposeB, invertB, transposeOutput, invertOutput),
+                               _stream(out.device)), "matmul")
+    return out
+
+
+def matmulGrad(vectorMatrixA, vectorMatrixB, outputGrad, transposeA=False, invertA=False, transposeB=False,
+               invertB=False, transposeOutput=False, invertOutput=False):
+    """Gradients of matmul (PISOtorch.cpp:613-621, matrix_vector_ops_grads.cu:217-310): [gradA, gradB].  An inverted
+    operand's gradient is zero; a matrix product with an inverted output, or with both operands inverted, has zero
+    gradients."""
+    dims, a_mat, b_mat, out_ch = _matmul_args(vectorMatrixA, vectorMatrixB)
+    _check(isinstance(outputGrad, torch.Tensor) and outputGrad.is_cuda and outputGrad.dim() == dims + 2,
+           "outputGrad dimensionality must match vectorMatrixA.")
+    _check(outputGrad.size(0) == 1, "outputGrad batch dimension must be 1.")
+    _check(outputGrad.size(1) == out_ch, "outputGrad channel dimension must match operation type.")
+    _check(outputGrad.dtype == vectorMatrixA.dtype, "Data type of outputGrad does not match vectorMatrixA.")
+    _check(outputGrad.shape[2:] == vectorMatrixA.shape[2:], "outputGrad spatial size must match vectorMatrixA.")
+    g = outputGrad.contiguous()
+    ga = torch.empty_like(vectorMatrixA)
+    gb = torch.empty_like(vectorMatrixB)
+    cells = ga[0, 0].numel()
+    lib = _abi.load_library()
+    _abi.check(lib.pict_matmul_grad(_ptr(vectorMatrixA), int(a_mat), _ptr(vectorMatrixB), int(b_mat), _ptr(g),
+                                    _ptr(ga), _ptr(gb), dims, cells, _dtype_code(ga.dtype),
+                                    _matmul_flags(transposeA, invertA, transposeB, invertB, transposeOutput,
+                                                  invertOutput), _stream(ga.device)), "matmulGrad")
+    return [ga, gb]
+
+
+def VectorToDiagMatrix(vectors):
+    """Matrices with the vectors on their diagonals (PISOtorch.cpp:622-623, matrix_vector_ops.cu:203-262)."""
+    dims, ok = _vm_check(vectors, "vectors", allow_matrix=False)
+    _check(ok, "vectors channel dimension must match spatial dimensionality.")
+    out = torch.empty([1, dims * dims] + list(vectors.shape[2:]), dtype=vectors.dtype, device=vectors.device)
+    lib = _abi.load_library()
+    _abi.check(lib.pict_vector_to_diag_matrix(_ptr(vectors), _ptr(out), dims, out[0, 0].numel(),
+                                              _dtype_code(out.dtype), _stream(out.device)), "VectorToDiagMatrix")
+    return out
+
+
+def InvertMatrix(matrices, inPlace=False):
+    """Per-cell inverse of square matrices (PISOtorch.cpp:624-625, matrix_vector_ops.cu:265-303)."""
+    dims, ok = _vm_check(matrices, "matrices", allow_vector=False)
+    _check(ok, "matrices channel dimension must match spatial dimensionality.")
+    out = matrices if inPlace else torch.empty_like(matrices)
+    lib = _abi.load_library()
+    _abi.check(lib.pict_invert_matrix(_ptr(matrices), _ptr(out), dims, out[0, 0].numel(), _dtype_code(out.dtype),
+                                      _stream(out.device)), "InvertMatrix")
+    return out
+
+
 def TransformVectors(vectors, transforms, inverse):
     """Per-cell M v (or Minv v) for NCDHW vectors and N(DHW)T transforms (k_TransformVectors :7152-7176)."""
     dims = vectors.size(1)

@@ -651,3 +651,39 @@ def is_tensor_empty(tensor):
 
 # Forward-only passthroughs (no gradient in the reference either)
 from .PISOtorch import CopyPressureResultFromBlocks  # noqa: E402,F401
+
+
+class _MatmulFunction(torch.autograd.Function):
+    """matmul with its VJP from PISOtorch.matmulGrad (PISOtorch_diff.py:1300-1371): a gradient is refused for an
+    inverted output, and for an inverted operand whose gradient is needed."""
+
+    @staticmethod
+    def forward(ctx, vectorMatrixA, vectorMatrixB, flags):
+        ctx.flags = flags
+        ctx.save_for_backward(vectorMatrixA, vectorMatrixB)
+        return PISOtorch.matmul(vectorMatrixA, vectorMatrixB, *flags)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, result_grad):
+        if not any(ctx.needs_input_grad[:2]):
+            return None, None, None
+        transposeA, invertA, transposeB, invertB, transposeOutput, invertOutput = ctx.flags
+        if invertOutput:
+            raise NotImplementedError("Can not compute gradients for inverted matrices (output).")
+        if ctx.needs_input_grad[0] and invertA:
+            raise NotImplementedError("Can not compute gradients for inverted matrices (input A).")
+        if ctx.needs_input_grad[1] and invertB:
+            raise NotImplementedError("Can not compute gradients for inverted matrices (input B).")
+        a, b = ctx.saved_tensors
+        ga, gb = PISOtorch.matmulGrad(a, b, result_grad.contiguous(), *ctx.flags)
+        return (ga if ctx.needs_input_grad[0] else None), (gb if ctx.needs_input_grad[1] else None), None
+
+
+def matmul(vectorMatrixA, vectorMatrixB, transposeA=False, invertA=False, transposeB=False, invertB=False,
+           transposeOutput=False, invertOutput=False):
+    """Differentiable per-cell product of vectors / flat row-major matrices in the channel dimension
+    (PISOtorch_diff.py:1300-1371): a matrix if both are matrices, a vector if exactly one is, a scalar otherwise."""
+    flags = (bool(transposeA), bool(invertA), bool(transposeB), bool(invertB), bool(transposeOutput),
+             bool(invertOutput))
+    return _MatmulFunction.apply(vectorMatrixA, vectorMatrixB, flags)

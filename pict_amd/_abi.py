@@ -24,6 +24,13 @@ PICT_BOUNDARY_CONNECTED = 20
 PICT_BOUNDARY_PERIODIC = 21
 PICT_BOUNDARY_FIXED = 30
 
+PICT_MATMUL_TRANSPOSE_A = 1
+PICT_MATMUL_INVERT_A = 2
+PICT_MATMUL_TRANSPOSE_B = 4
+PICT_MATMUL_INVERT_B = 8
+PICT_MATMUL_TRANSPOSE_OUT = 16
+PICT_MATMUL_INVERT_OUT = 32
+
 
 class BoundaryDesc(Structure):
     _fields_ = [
@@ -243,6 +250,11 @@ SIGNATURES = {
     "pict_last_step_solve_ms": (c_int32, [POINTER(c_double), c_int32]),
     "pict_set_kernel_timing": (c_int32, [c_int32]),
     "pict_last_solve_kernel_ms": (c_double, []),
+    "pict_matmul": (c_int32, [_P, c_int32, _P, c_int32, _P, c_int32, ctypes.c_int64, c_int32, c_int32, _P]),
+    "pict_matmul_grad": (c_int32, [_P, c_int32, _P, c_int32, _P, _P, _P, c_int32, ctypes.c_int64, c_int32, c_int32,
+                                   _P]),
+    "pict_vector_to_diag_matrix": (c_int32, [_P, _P, c_int32, ctypes.c_int64, c_int32, _P]),
+    "pict_invert_matrix": (c_int32, [_P, _P, c_int32, ctypes.c_int64, c_int32, _P]),
     "pict_coords_to_transforms": (c_int32, [_P, c_int32, POINTER(c_int32), c_int32, _P, _P]),
     "pict_coords_to_face_transforms": (c_int32, [_P, c_int32, POINTER(c_int32), c_int32, _P, _P]),
 }
