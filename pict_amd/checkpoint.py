@@ -42,34 +42,51 @@ def _same(a, b):
                       and a.device == b.device)
 
 
-def _swap_params(domain, params, leaves):
+def _slots_of(domain, params):
+    """Index into _param_slots(domain) of the slot each parameter occupies now (None: not a domain parameter).
+    Recorded when a step runs: a rollout may give every step other parameter tensors (a velocity source computed
+    from the velocities, say), and the step's backward must see the ones it ran with."""
+    slots = [get() for get, _, _ in _param_slots(domain)]
+    out = []
+    for p in params:
+        out.append(next((i for i, cur in enumerate(slots) if _same(cur, p)), None))
+    return out
+
+
+def _install_params(domain, tensors, slot_idx):
+    """Put each tensor into its recorded slot; returns what to put back (_restore_params)."""
+    slots = list(_param_slots(domain))
     swaps = []
-    for get, set_, _ in _param_slots(domain):
+    for t, i in zip(tensors, slot_idx):
+        if i is None:
+            continue
+        get, set_, _ = slots[i]
         cur = get()
-        for p, leaf in zip(params, leaves):
-            if _same(cur, p):
-                set_(leaf)
-                swaps.append((set_, p))
+        if cur is not t:
+            set_(t)
+            swaps.append((set_, cur))
+    if swaps:
+        domain.UpdateDomainData()
     return swaps
 
 
 def _restore_params(domain, swaps):
-    for set_, p in swaps:
+    for set_, p in reversed(swaps):
         set_(p)
     domain.UpdateDomainData()
 
 
-def _param_grads(domain, params):
-    """The accumulated gradient of each parameter tensor (None where it is not a domain parameter)."""
+def _param_grads(domain, params, slot_idx):
+    """The accumulated gradient of each parameter, from its recorded slot (None where it is not a domain
+    parameter)."""
+    slots = list(_param_slots(domain))
     out = []
-    for p in params:
+    for p, i in zip(params, slot_idx):
         g = None
-        for get, _, grad in _param_slots(domain):
-            if _same(get(), p):
-                gt = grad()
-                if gt is not None:
-                    g = gt.detach().clone().to(p.device)
-                break
+        if i is not None:
+            gt = slots[i][2]()
+            if gt is not None:
+                g = gt.detach().clone().to(p.device)
         out.append(g)
     return out
 
@@ -281,6 +298,7 @@ class _PISOStep(torch.autograd.Function):
         ctx.sim = sim
         ctx.n_blocks = n_blocks
         ctx.save_for_backward(*tensors)
+        ctx.slot_idx = _slots_of(sim.domain, tensors[n_blocks:])
         domain = sim.domain
         was_diff = sim.differentiable
         reuse = (sim.advect_reuse_result, sim.pressure_reuse_result)
@@ -346,8 +364,11 @@ class _PISOStep(torch.autograd.Function):
         w = _work(domain)
         state = None if w.installed(domain) else [(b.velocity, b.pressure) for b in domain.getBlocks()]
         last = w.last
+        swaps = []
         try:
             with torch.no_grad():
+                # the parameters this step ran with (a later step may have set others)
+                swaps = _install_params(domain, [p.detach() for p in params], ctx.slot_idx)
                 snapshot = [t.clone() for t in w.vel + w.pres] if state is None else None
                 if state is not None:
                     w.install(domain)
@@ -364,10 +385,13 @@ class _PISOStep(torch.autograd.Function):
                         sim.advect_reuse_result, sim.pressure_reuse_result = reuse
                 g = _native_backward(sim, tape, vel_grads)
                 g_vel = _from_global(g, list(saved[:n]), domain.getSpatialDims())
-                g_par = [gp if p.requires_grad else None for gp, p in zip(_param_grads(domain, params), params)]
+                g_par = [gp if p.requires_grad else None
+                         for gp, p in zip(_param_grads(domain, params, ctx.slot_idx), params)]
         finally:
             sim.total_step, sim.total_time = counters
             with torch.no_grad():
+                if swaps:
+                    _restore_params(domain, swaps)
                 if state is None:
                     for t, v in zip(w.vel + w.pres, snapshot):
                         t.copy_(v)
@@ -388,7 +412,7 @@ class _PISOStep(torch.autograd.Function):
         vel_in = [t.detach().clone().requires_grad_(True) for t in saved[:n]]
         params = list(saved[n:])
         leaves = [p.detach().requires_grad_(p.requires_grad) for p in params]
-        swaps = _swap_params(domain, params, leaves)
+        swaps = _install_params(domain, leaves, ctx.slot_idx)
         counters = (sim.total_step, sim.total_time)
         # the recomputation runs on the live domain: its block state is put back afterwards
         state = [(b.velocity, b.pressure) for b in domain.getBlocks()]
@@ -464,3 +488,21 @@ def rollout(sim, steps, params=(), checkpoint=True, keep_tape=False):
     finally:
         sim.differentiable = was_diff
     return [b.velocity for b in sim.domain.getBlocks()]
+
+
+def rollout_with_sources(sim, steps, source_fn, keep_tape=False):
+    """A checkpointed rollout whose block velocity sources are recomputed before every step from the current
+    velocities: sources = source_fn(velocities) (per block, differentiable: e.g. a static forcing plus a CNN
+    sub-grid model).  Each step's gradient reaches its sources, and through them whatever source_fn depends on
+    (its parameters and the velocities it read).  Returns the final velocities."""
+    domain = sim.domain
+    blocks = domain.getBlocks()
+    w = _WORK.get(domain)
+    u = w.last if (w is not None and w.last is not None and w.installed(domain)) else [b.velocity for b in blocks]
+    for _ in range(steps):
+        srcs = [s.contiguous() for s in source_fn(u)]
+        for b, s in zip(blocks, srcs):
+            b.setVelocitySource(s)
+        domain.UpdateDomainData()
+        u = checkpointed_step(sim, srcs, keep_tape)
+    return u

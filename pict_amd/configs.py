@@ -117,3 +117,55 @@ def cavity_batch_c4(count=64, seed=0, dtype=torch.float32, device="cuda", sides=
     doms = [lid_cavity(n, viscosity=v, dtype=dtype, device=device, name="cavity%d" % i)
             for i, (n, v) in enumerate(zip(ns, visc))]
     return doms, ns, visc, 0.4 / max(sides)
+
+
+def channel_coords(nx=192, ny=96, nz=96, stretch=2.0, dtype=torch.float32, device="cuda"):
+    """Vertex coordinates [1, 3, nz+1, ny+1, nx+1] of a channel [-pi, pi] x [-1, 1] x [-pi/2, pi/2] (half-height 1,
+    the proportions of the reference's channel grids, channel_flow_sim.py:110-141) with a wall-normal tanh stretch:
+    y_j = tanh(stretch (2j/ny - 1)) / tanh(stretch), cells cluster at the walls y = +-1."""
+    xs = torch.linspace(-math.pi, math.pi, nx + 1, dtype=torch.float64)
+    eta = torch.linspace(-1.0, 1.0, ny + 1, dtype=torch.float64)
+    ys = torch.tanh(stretch * eta) / math.tanh(stretch)
+    zs = torch.linspace(-math.pi / 2, math.pi / 2, nz + 1, dtype=torch.float64)
+    gz, gy, gx = torch.meshgrid(zs, ys, xs, indexing="ij")
+    return torch.stack([gx, gy, gz])[None].to(dtype=dtype, device=device).contiguous()
+
+
+def channel_c5(nx=192, ny=96, nz=96, stretch=2.0, re_tau=180.0, noise=0.05, bulk=15.7, seed=0,
+               dtype=torch.float32, device="cuda"):
+    """3-D channel configuration: one block of nx x ny x nz cells on channel_coords, periodic in x and z, no-slip walls at
+    y = +-1 (channel_flow_sim.py:385-398).  With half-height and friction velocity 1: viscosity 1/re_tau and a static
+    streamwise forcing (u_tau^2 / delta = 1).  Initial state (amplitudes not fixed by the reference, fixed here): the
+    laminar parabola with the bulk velocity of a turbulent channel at this Re_tau (u = 1.5 bulk (1 - y^2)) plus seeded
+    Gaussian noise of `noise` times its centre-line velocity in every component.  Returns (domain, time_step, forcing)
+    with dt = 0.4 min(dx) / (1.5 bulk), forcing the static source tensor [1, 3]."""
+    coords = channel_coords(nx, ny, nz, stretch, dtype, device)
+    dom = PISOtorch.Domain(3, torch.tensor([1.0 / re_tau], dtype=dtype), "channel", dtype, device,
+                           passiveScalarChannels=0)
+    blk = dom.CreateBlock(vertexCoordinates=coords, name="channel")
+    blk.MakePeriodic("x")
+    blk.MakePeriodic("z")
+    blk.CloseBoundary("-y")
+    blk.CloseBoundary("+y")
+    # cell-centre y of every cell [nz, ny, nx] (the grid is a tensor product: the mean of two y-adjacent vertices)
+    yc = 0.5 * (coords[0, 1, :-1, :-1, :-1] + coords[0, 1, :-1, 1:, :-1]).to(torch.float64).cpu()
+    ucl = 1.5 * bulk
+    g = torch.Generator().manual_seed(seed)
+    vel = torch.randn([1, 3, nz, ny, nx], generator=g, dtype=torch.float64) * (noise * ucl)
+    vel[0, 0] += ucl * (1.0 - yc ** 2)
+    blk.setVelocity(vel.to(dtype=dtype, device=device).contiguous())
+    forcing = torch.tensor([[1.0, 0.0, 0.0]], dtype=dtype, device=device)
+    blk.setVelocitySource(forcing)
+    dom.PrepareSolve()
+    dx = 2 * math.pi / nx
+    return dom, 0.4 * dx / ucl, forcing
+
+
+def sgs_cnn(channels=16, seed=0, dtype=torch.float32, device="cuda"):
+    """The SGS forcing model of the 3-D channel configuration: three 3^3 convolutions 3 -> channels -> channels -> 3
+    with 'same' padding and ReLU between them (channel_flow_learning.py:276-297), random initialisation from `seed`."""
+    torch.manual_seed(seed)
+    net = torch.nn.Sequential(torch.nn.Conv3d(3, channels, 3, padding=1), torch.nn.ReLU(),
+                              torch.nn.Conv3d(channels, channels, 3, padding=1), torch.nn.ReLU(),
+                              torch.nn.Conv3d(channels, 3, 3, padding=1))
+    return net.to(dtype=dtype, device=device)
