@@ -48,6 +48,15 @@ def make_workload(args, torch):
     from pict_amd import PISOtorch_simulation as S
     from pict_amd import configs
     dtype = torch.float32
+    if args.config == "c5":
+        dom, dt, _ = configs.channel_c5(dtype=dtype, device="cuda")
+        workload = ("3-D channel 192x96x96 (tanh wall-normal stretch 2.0, periodic x/z, no-slip walls), Re_tau 180, "
+                    "FP32, static forcing + random-init 3-layer Conv3d SGS forcing (16 channels) recomputed every "
+                    "step, %s" % ("%d-step rollout with per-step checkpointing, MSE loss, forward+backward"
+                                  % args.steps if args.mode == "fwdbwd" else "forward PISO steps"))
+        sim = S.Simulation(dom, time_step=dt, corrector_steps=2, non_orthogonal=False, advection_tol=1e-6,
+                           pressure_tol=1e-6, advect_passive_scalar=False)
+        return dom, sim, dt, workload
     if args.config == "c3":
         dom, dt = configs.periodic_box_c3(256, dtype=dtype, device="cuda")
         workload = ("periodic box 256x256 (2 blocks of 256x128), FP32, spectral k^-5/3 init, learnable per-cell "
@@ -77,6 +86,31 @@ def target_state(dom, args, torch):
     return [(torch.rand(b.velocity.shape, generator=g) * 0.2).cuda() for b in dom.blocks]
 
 
+_C5_NET = {}
+
+
+def rollout_c5(dom, sim, steps, target, torch, keep_tape=False):
+    """The 3-D channel's differentiable rollout: before every step the velocity source is the static forcing plus the
+    SGS CNN of the current velocity; MSE loss to the target; gradients of the CNN weights."""
+    from pict_amd import checkpoint as CK
+    from pict_amd import configs
+    net = _C5_NET.get("net")
+    if net is None:
+        net = _C5_NET["net"] = configs.sgs_cnn(16, seed=0)
+    blk = dom.blocks[0]
+    u0 = blk.velocity.detach().clone()
+    forcing = torch.tensor([1.0, 0.0, 0.0], dtype=u0.dtype, device=u0.device).reshape(1, 3, 1, 1, 1)
+    net.zero_grad(set_to_none=True)
+    out = CK.rollout_with_sources(sim, steps, lambda u: [forcing + 0.05 * net(u[0])], keep_tape=keep_tape)
+    loss = ((out[0] - target[0]) ** 2).mean()
+    loss.backward()
+    # back to the initial state (the source keeps its per-cell shape: its gradient tensor has that shape)
+    blk.setVelocity(u0)
+    blk.setVelocitySource(forcing.expand_as(u0).contiguous())
+    dom.UpdateDomainData()
+    return [p.grad for p in net.parameters()]
+
+
 def rollout_fwdbwd(dom, sim, steps, target, torch, keep_tape=False):
     from pict_amd import checkpoint as CK
     u0 = [b.velocity.detach().clone() for b in dom.blocks]
@@ -101,7 +135,7 @@ def main():
     ap.add_argument("--mode", default="fwdbwd", choices=["fwdbwd", "fwd"])
     ap.add_argument("--steps", type=int, default=None, help="default: 16 (c3, c1), 100 (c4)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c3", "c1", "c4"])
+    ap.add_argument("--config", default="c3", choices=["c3", "c1", "c4", "c5"])
     ap.add_argument("--batch", type=int, default=64, help="c4: number of cavities")
     ap.add_argument("--no-batched", action="store_true",
                     help="c3/c1: skip the batching configuration (64 cavities, 100 steps) reported under `batched`")
@@ -117,7 +151,10 @@ def main():
     ap.add_argument("--pmc-write", default=None, metavar="CSV", help="the same for WRITE_SIZE")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 100 if args.config == "c4" else 16
+        args.steps = {"c4": 100, "c5": 8}.get(args.config, 16)
+    if args.config == "c5":
+        # one FP32 oracle step of 1.77 M cells takes minutes on the host: no CPU sample for this configuration
+        args.no_cpu_baseline = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the PISO step of one domain runs on a single MI355X "
               "(multi-GPU sharding is a design note, not built); run with --gpus 1", file=sys.stderr)
@@ -135,7 +172,7 @@ def main():
 
     def run(k):
         if args.mode == "fwdbwd":
-            return rollout_fwdbwd(dom, sim, k, target, torch)
+            return (rollout_c5 if args.config == "c5" else rollout_fwdbwd)(dom, sim, k, target, torch)
         assert sim._PISO_split_step(k)
         return None
 
@@ -182,7 +219,8 @@ def main():
     extra = {}
     if args.mode == "fwdbwd":
         # the same rollout keeping each step's tape (no recomputation), and the forward alone, for context
-        for label, fn in (("keep_tape", lambda: rollout_fwdbwd(dom, sim, args.steps, target, torch, True)),
+        roll = rollout_c5 if args.config == "c5" else rollout_fwdbwd
+        for label, fn in (("keep_tape", lambda: roll(dom, sim, args.steps, target, torch, True)),
                           ("forward_only", lambda: sim._PISO_split_step(args.steps))):
             torch.cuda.synchronize()
             f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -209,7 +247,7 @@ def main():
         cpu = cpu_baseline(dom, dt, args, target)
 
     batched = None
-    if not args.no_batched:
+    if not args.no_batched and args.config == "c3":
         import copy
         bargs = copy.copy(args)
         bargs.steps, bargs.warmup, bargs.no_cpu_baseline = 100, 2, True

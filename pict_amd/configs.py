@@ -163,9 +163,10 @@ def channel_c5(nx=192, ny=96, nz=96, stretch=2.0, re_tau=180.0, noise=0.05, bulk
 
 def sgs_cnn(channels=16, seed=0, dtype=torch.float32, device="cuda"):
     """The SGS forcing model of the 3-D channel configuration: three 3^3 convolutions 3 -> channels -> channels -> 3
-    with 'same' padding and ReLU between them (channel_flow_learning.py:276-297), random initialisation from `seed`."""
+    with 'same' padding and ReLU between them (channel_flow_learning.py:276-297), random initialisation from `seed`
+    (torch.nn.Conv3d's); the convolutions use sgs.Conv3dSame's backward."""
+    from .sgs import Conv3dSame
     torch.manual_seed(seed)
-    net = torch.nn.Sequential(torch.nn.Conv3d(3, channels, 3, padding=1), torch.nn.ReLU(),
-                              torch.nn.Conv3d(channels, channels, 3, padding=1), torch.nn.ReLU(),
-                              torch.nn.Conv3d(channels, 3, 3, padding=1))
+    net = torch.nn.Sequential(Conv3dSame(3, channels), torch.nn.ReLU(), Conv3dSame(channels, channels),
+                              torch.nn.ReLU(), Conv3dSame(channels, 3))
     return net.to(dtype=dtype, device=device)
