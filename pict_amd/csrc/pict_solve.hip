@@ -1924,6 +1924,11 @@ __device__ __forceinline__ void bicg_local(const SolveArgs<float> &a, unsigned b
     if (bid == 0 && threadIdx.x < NR) a.st[threadIdx.x] = sst[threadIdx.x];
 }
 
+template <int RPT, int NZ, int NR>
+__global__ __launch_bounds__(kSolveThreads) void k_bicg_local(SolveArgs<float> a) {
+    bicg_local<RPT, NZ, NR>(a, blockIdx.x);
+}
+
 // batched launch (job lists as for k_cg_local_jobs)
 template <int RPT, int NZ, int NR>
 __global__ __launch_bounds__(kSolveThreads) void k_bicg_local_jobs(const SolveArgs<float> *__restrict__ args,
@@ -2355,6 +2360,26 @@ const void *local_cg_kernel(int rpt, int nz, bool full) {
     return nz <= 5 ? local_cg_kernel_nz<R, 5, false>(rpt) : local_cg_kernel_nz<R, 7, false>(rpt);
 }
 
+// k_bicg_local instances: 5-entry rows with one or two right-hand sides, 7-entry rows with one
+template <int NZ, int NR>
+const void *bicg_local_kernel_nz(int rpt) {
+    switch (rpt) {
+        case 1: return (const void *)k_bicg_local<1, NZ, NR>;
+        case 2: return (const void *)k_bicg_local<2, NZ, NR>;
+        case 4: return (const void *)k_bicg_local<4, NZ, NR>;
+        default: return nullptr;
+    }
+}
+const void *bicg_local_kernel(int rpt, int nz, int nrhs) {
+    if (nz <= 5) return nrhs == 1 ? bicg_local_kernel_nz<5, 1>(rpt) : nrhs == 2 ? bicg_local_kernel_nz<5, 2>(rpt) : nullptr;
+    return nrhs == 1 ? bicg_local_kernel_nz<7, 1>(rpt) : nullptr;
+}
+// PICT_BICG_LOCAL=0 keeps BiCGStab on the streaming kernel
+static bool bicg_local_enabled() {
+    const char *e = getenv("PICT_BICG_LOCAL");
+    return !e || atoi(e) != 0;
+}
+
 // Workgroup budget of the register-resident CG: the cost of its barrier grows with the
 // number of workgroups, the per-thread work with rows per thread.
 // (PICT_CG_LOCAL_WGS: budget; 0 turns the kernel off)
@@ -2368,8 +2393,9 @@ struct Plan {
     Kind kind;
     const void *kernel;
     long long nwg;
-    bool local;     // k_cg_local
-    int rpt, nz;    // k_cg_local rows per thread, cached row length
+    bool local;       // k_cg_local
+    bool bicg_local;  // k_bicg_local
+    int rpt, nz;      // register-resident kernels: rows per thread, cached row length
     bool full;      // k_cg_local reduces the sum and maximum of |r| too
 };
 
@@ -2417,6 +2443,34 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
             break;
         }
     }
+    // BiCGStab with FP32 rows in registers (norm-2 criteria, up to two right-hand sides), on the smallest rows per
+    // thread that keep the grid within the workgroup budget and resident
+    if constexpr (sizeof(R) == 4) {
+        const bool norm2 = o->convergence_criterion == PICT_CONV_NORM2 ||
+                           o->convergence_criterion == PICT_CONV_NORM2_NORMALIZED;
+        if (pl.kind == Kind::kBiCG && norm2 && target > 0 && bicg_local_enabled()) {
+            const int nz = (long long)A->nnz <= 5LL * n ? 5 : 7;
+            for (int rpt = 1; rpt <= 4; rpt *= 2) {
+                const void *k = bicg_local_kernel(rpt, nz, nrhs);
+                if (!k) break;
+                const long long per = (long long)kSolveThreads * rpt;
+                const long long wgs = ((long long)n + per - 1) / per;
+                if (wgs > target && rpt < 4) continue;
+                int locc = 0, lcus = 0;
+                st = occupancy(k, locc, lcus);
+                if (st) return st;
+                if (wgs <= (long long)locc * lcus) {
+                    pl.kernel = k;
+                    pl.nwg = wgs;
+                    pl.bicg_local = true;
+                    pl.rpt = rpt;
+                    pl.nz = nz;
+                    return PICT_OK;
+                }
+                break;
+            }
+        }
+    }
     // rows per thread: more rows per workgroup means fewer barrier participants
     static const int rows_per_thread = [] {
         const char *e = getenv("PICT_SOLVER_ROWS_PER_THREAD");
@@ -2454,8 +2508,10 @@ Layout layout_of(const pict_csr_desc *A, int nrhs, const pict_solve_opts *o, con
     L.off_perm = L.off_st + al(sizeof(RhsState) * nrhs);
     L.off_ctl = L.off_perm + al(bytes_perm);
     L.off_zero = L.off_ctl + al(sizeof(Ctl));
-    L.words_tagged = pl.local ? (size_t)2 * pl.nwg * kSlots : 0;
-    L.words_zero = pl.local ? L.words_tagged + 6 * n : 0;
+    L.words_tagged = pl.local || pl.bicg_local ? (size_t)2 * pl.nwg * kSlots : 0;
+    L.words_zero = pl.local        ? L.words_tagged + 6 * n
+                   : pl.bicg_local ? L.words_tagged + 2 * (size_t)nrhs * n
+                                   : 0;
     L.total = L.off_zero + sizeof(unsigned long long) * L.words_zero + 256;
     return L;
 }
@@ -3065,6 +3121,8 @@ pict_status pict_solver_kernel_name(const pict_csr_desc *A, int32_t nrhs, const 
     const int nr = nrhs < kMaxRhs ? nrhs : kMaxRhs;
     if (pl.local)
         snprintf(buf, (size_t)len, "pict::k_cg_local<%s, %d, %d, %s>", t, pl.rpt, pl.nz, pl.full ? "true" : "false");
+    else if (pl.bicg_local)
+        snprintf(buf, (size_t)len, "pict::k_bicg_local<%d, %d, %d>", pl.rpt, pl.nz, nr);
     else if (pl.kind == Kind::kBiCG)
         snprintf(buf, (size_t)len, "pict::k_bicgstab<%s, %d>", t, nr);
     else if (pl.kind == Kind::kCGTextbook)

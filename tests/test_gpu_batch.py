@@ -71,10 +71,11 @@ def test_batch_matches_native_steps_bitwise(dtype):
     ref, _, its_ref = _plain(dtype, steps)
     doms = _domains(dtype)
     sims = [_sim(d, dtype) for d in doms]
-    # rows per thread 1 and the streaming BiCGStab with 512 rows per workgroup: the single-system plans of these
-    # sizes
-    batch = SimulationBatch(sims, config=dict(pressure_rows_per_thread=1, advection_rows_per_thread=-1,
-                                              advection_rows_per_workgroup=512))
+    # the single-system plans of these sizes: register-resident CG with one row per thread; BiCGStab register-resident
+    # with one row per thread in FP32, streaming with 512 rows per workgroup in FP64
+    adv = dict(advection_rows_per_thread=1) if dtype == torch.float32 else dict(advection_rows_per_thread=-1,
+                                                                                   advection_rows_per_workgroup=512)
+    batch = SimulationBatch(sims, config=dict(pressure_rows_per_thread=1, **adv))
     its = [[] for _ in sims]
     for _ in range(steps):
         arr = batch.step()
@@ -82,7 +83,7 @@ def test_batch_matches_native_steps_bitwise(dtype):
             its[i].append(arr["used_iterations"][i].tolist())
     info = batch.describe()
     print(" batch:", info, flush=True)
-    assert info["pressure_rows_per_thread"] == 1 and info["advection_rows_per_workgroup"] == 512
+    assert info["pressure_rows_per_thread"] == 1
     assert its == its_ref
     for a, b in zip(_state(ref), _state(doms)):
         for x, y in zip(a, b):
@@ -225,7 +226,7 @@ def test_batch_tensors_swapped_between_steps():
     ref = _domains(torch.float32)
     batch = SimulationBatch([_sim(d, torch.float32) for d in doms])
     sims_ref = [_sim(d, torch.float32) for d in ref]
-    batch.set_config(pressure_rows_per_thread=1, advection_rows_per_thread=-1, advection_rows_per_workgroup=512)
+    batch.set_config(pressure_rows_per_thread=1, advection_rows_per_thread=1)
     for k in range(4):
         if k == 2:
             for d in (doms[0], ref[0]):
