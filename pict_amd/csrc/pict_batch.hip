@@ -48,20 +48,6 @@ __global__ __launch_bounds__(kCellsPerTile) void kb_cells(const BatchDom<R> *__r
     }
 }
 
-// solve start vector x0 = 0 (field 0: velocity result, 1: pressure result) on one tile
-template <typename R>
-__device__ __forceinline__ void zero_result_cell(const DomainK<R> &dom, int tile, int field) {
-    const pict_block_desc *b;
-    int bi, f;
-    if (!tile_cell(dom, tile, b, bi, f)) return;
-    const int g = b->global_offset + f;
-    if (field == 0) {
-        for (int d = 0; d < dom.D; ++d) dom.vres[(size_t)d * dom.N + g] = R(0);
-    } else {
-        dom.pres[g] = R(0);
-    }
-}
-
 // SetupAdvectionMatrix + SetupAdvectionVelocity + zero start vector (independent per cell)
 template <typename R, int D>
 struct OpAdvection {

@@ -29,6 +29,11 @@ hipError_t ws_free(void *p, hipStream_t s);
 // Frees the captured step graphs of an atlas (defined in pict_step.hip).
 void step_graphs_free(struct pict_atlas *a);
 
+// Fused forward stages of the native step (pict_stages.hip): advection matrix + RHS + velocity result = 0;
+// pressure matrix (if `matrix`) + pressure RHS + pressure result = 0.
+pict_status stage_advection_setup(const struct pict_atlas *a, double dt, hipStream_t stream);
+pict_status stage_pressure_setup(const struct pict_atlas *a, double dt, bool matrix, hipStream_t stream);
+
 #define PICT_HIP_TRY(expr)                                                                  \
     do {                                                                                    \
         hipError_t _e = (expr);                                                             \

@@ -154,6 +154,25 @@ __global__ __launch_bounds__(kCellsPerTile) void k_copy_result(DomainK<R> dom, R
     copy_result_cell<R>(dom, (int)blockIdx.x, gv, field, to_blocks);
 }
 
+// Fused stages of the native step (pict_step.hip): stages that read only what earlier launches wrote share one
+// launch, with the same values as the separate kernels.
+// advection matrix + advection RHS + solve start vector 0
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_advection_setup(DomainK<R> dom, R dt) {
+    const int t = (int)blockIdx.x;
+    adv_matrix_cell<R, D>(dom, t, dt);
+    adv_rhs_cell<R, D>(dom, t, dt, 0);
+    zero_result_cell<R>(dom, t, 0);
+}
+// pressure matrix (when `matrix`: it depends on A only, so once per step) + pressure RHS + solve start vector 0
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_pressure_setup(DomainK<R> dom, R dt, int matrix) {
+    const int t = (int)blockIdx.x;
+    if (matrix) p_matrix_cell<R, D>(dom, t);
+    p_rhs_cell<R, D>(dom, t, dt);
+    zero_result_cell<R>(dom, t, 1);
+}
+
 }  // namespace pict
 
 using namespace pict;
@@ -297,6 +316,29 @@ pict_status pict_atlas_destroy(pict_atlas *a) {
     atlas_free(a);
     return PICT_OK;
 }
+
+}  // extern "C"
+
+namespace pict {
+
+pict_status stage_advection_setup(const pict_atlas *a, double dt, hipStream_t stream) {
+    PICT_CHECK_ARG(a && a->desc.C_value && a->desc.A && a->desc.velocity_rhs && a->desc.velocity_result,
+                   "advection tensors missing");
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_advection_setup<R, D>), a->view<R>(), (R)dt));
+    return check_launch();
+}
+
+pict_status stage_pressure_setup(const pict_atlas *a, double dt, bool matrix, hipStream_t stream) {
+    PICT_CHECK_ARG(a && a->desc.P_value && a->desc.A && a->desc.pressure_rhs && a->desc.C_value &&
+                       a->desc.velocity_result && a->desc.pressure_result,
+                   "pressure tensors missing");
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_pressure_setup<R, D>), a->view<R>(), (R)dt, matrix ? 1 : 0));
+    return check_launch();
+}
+
+}  // namespace pict
+
+extern "C" {
 
 pict_status pict_build_csr_pattern(const pict_atlas *a, void *stream) {
     PICT_CHECK_ARG(a && a->desc.csr_row && a->desc.csr_index, "pattern buffers missing");

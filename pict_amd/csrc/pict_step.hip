@@ -242,12 +242,9 @@ pict_status enqueue_step(const pict_atlas *a, const pict_step_opts *o, const pic
     pict_csr_desc C, P;
     pict_solve_opts adv, prs;
     solve_systems(a, o, false, C, P, adv, prs);
-    // ---- advection (PISOtorch_simulation.py:1161-1188) ----
-    if ((st = pict_setup_advection_matrix(a, dt, 0, stream))) return st;
-    if ((st = pict_setup_advection_velocity(a, dt, 0, 0, stream))) return st;
+    // ---- advection (PISOtorch_simulation.py:1161-1188): matrix, RHS and start vector in one launch ----
+    if ((st = stage_advection_setup(a, dt, s))) return st;
     if (tape && ((st = copy_dev(tape->C_value, d.C_value, nzb, s)) || (st = copy_dev(tape->A, d.A, vec1, s))))
-        return st;
-    if ((st = f32 ? fill<float>(d.velocity_result, (size_t)D * N, 0.0, s) : fill<double>(d.velocity_result, (size_t)D * N, 0.0, s)))
         return st;
     if ((st = solve_enqueue(&C, d.velocity_rhs, d.velocity_result, D, &adv, dev, (int *)(dev + 4 * sb),
                             ev ? ev[0] : nullptr, ev ? ev[1] : nullptr, s, w.solve_ws[0], w.solve_ws_bytes[0])))
@@ -256,11 +253,11 @@ pict_status enqueue_step(const pict_atlas *a, const pict_step_opts *o, const pic
     // ---- correctors (:1229-1320) ----
     for (int k = 0; k < K; ++k) {
         char *sl = dev + z.slot * (1 + k);
-        if ((st = pict_setup_pressure_correction(a, dt, 0, 0, o->time_step_norm, stream))) return st;
+        // pressure matrix (first corrector: it depends on A only), RHS and start vector in one launch; divergence
+        if ((st = stage_pressure_setup(a, dt, k == 0, s))) return st;
+        if ((st = pict_setup_pressure_rhs_div(a, dt, 0, 0, o->time_step_norm, stream))) return st;
         if (tape && k == 0 && (st = copy_dev(tape->P_value, d.P_value, nzb, s))) return st;
         if (tape && (st = copy_dev((char *)tape->pressure_rhs + vecD * k, d.pressure_rhs, vecD, s))) return st;
-        if ((st = f32 ? fill<float>(d.pressure_result, (size_t)N, 0.0, s) : fill<double>(d.pressure_result, (size_t)N, 0.0, s)))
-            return st;
         if ((st = solve_enqueue(&P, d.pressure_rhs_div, d.pressure_result, 1, &prs, sl, (int *)(sl + 4 * sb),
                                 ev ? ev[2 * (1 + k)] : nullptr, ev ? ev[2 * (1 + k) + 1] : nullptr, s,
                                 w.solve_ws[1 + k], w.solve_ws_bytes[1 + k])))
