@@ -238,9 +238,13 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    traffic = None
+    traffic, traffic_source = None, None
     if args.pmc_fetch and args.pmc_write:
-        traffic = pmc_traffic(args.pmc_fetch, args.pmc_write, dom_kernel)
+        traffic, traffic_source = pmc_traffic(args.pmc_fetch, args.pmc_write, dom_kernel), "counter files"
+    elif args.config == "c3" and args.mode == "fwdbwd":
+        prof = os.path.join(ROOT, "profiles", "pict_piso_step_fwdbwd_periodic_box_256x256.md")
+        traffic = profiled_traffic(prof, dom_kernel)
+        traffic_source = os.path.relpath(prof, ROOT) if traffic is not None else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -280,7 +284,7 @@ def main():
                    "wall_s": wall, **extra},
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
+                     "traffic_source": traffic_source, "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
     }
     if batched is not None:
@@ -460,6 +464,22 @@ def pmc_traffic(fetch_csv, write_csv, kernel):
 
     f, w = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
     return None if f is None or w is None else f + w
+
+
+def profiled_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` as recorded in a committed profile summary: the 'scaled HBM MB per launch'
+    column of its FETCH_SIZE / WRITE_SIZE table (same workload, rocprofv3 --pmc runs); None if absent."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) == 6 and cols[0] == "`%s`" % kernel:
+                try:
+                    return float(cols[4]) * 1e6
+                except ValueError:
+                    return None
+    return None
 
 
 def cpu_baseline(dom, dt, args, target):

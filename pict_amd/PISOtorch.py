@@ -1143,12 +1143,13 @@ class Block:
 
     # --- queries ---
     def getMaxVelocity(self, withBounds=True, computational=False):
-        v = self.getVelocity(computational).abs()
-        m = torch.amax(v, dim=tuple(d for d in range(v.dim()) if d != 1))
+        """Largest velocity component magnitude of the block and, withBounds, its fixed boundaries, as a 0-dim
+        tensor (domain_structs.cpp:1564-1600)."""
+        m = torch.amax(self.getVelocity(computational).abs())
         if withBounds:
             for _, b in self.getFixedBoundaries():
-                bv = b.getVelocity(computational).abs() if b.hasSize() else b.velocity.abs()
-                m = torch.maximum(m, torch.amax(bv, dim=tuple(d for d in range(bv.dim()) if d != 1)))
+                bv = b.getVelocity(computational) if b.hasSize() else b.velocity
+                m = torch.maximum(m, torch.amax(bv.abs()))
         return m
 
     def getMaxVelocityMagnitude(self, withBounds=True, computational=False):
@@ -1743,7 +1744,7 @@ class Domain:
 
     # --- queries ---
     def getMaxVelocity(self, withBounds=True, computational=False):
-        return torch.amax(torch.stack([b.getMaxVelocity(withBounds, computational) for b in self.blocks]), dim=0)
+        return torch.amax(torch.stack([b.getMaxVelocity(withBounds, computational) for b in self.blocks]))
 
     def getMaxVelocityMagnitude(self, withBounds=True, computational=False):
         return torch.amax(torch.cat([b.getMaxVelocityMagnitude(withBounds, computational) for b in self.blocks]))\

@@ -33,6 +33,191 @@ def ntonp(t):
     return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
 
 
+# --- module-level helpers (PISOtorch_simulation.py:55-386) ---
+
+def get_max_time_step(domain, time_step_target, CFL_cond=0.8, with_transformations=True):
+    """Largest time step <= time_step_target with max|u| dt <= CFL_cond, as (time step, substeps) (:55-65)."""
+    max_vel = float(ntonp(domain.getMaxVelocity(True, with_transformations)))
+    limit = CFL_cond / max_vel if max_vel > 0 else float("inf")
+    if limit >= time_step_target:
+        return time_step_target, 1
+    substeps = int(np.ceil(time_step_target / limit))
+    return time_step_target / substeps, substeps
+
+
+def getVelocityResultMaxMag(domain):
+    """Largest per-cell velocity magnitude of domain.velocityResult (:67-73)."""
+    vel = domain.velocityResult.reshape(domain.getSpatialDims(), domain.getTotalSize())
+    if float(vel.abs().max()) > 0:
+        return float(torch.linalg.vector_norm(vel, dim=0).max())
+    return 0
+
+
+def getVelocityResultMaxVel(domain):
+    """Largest velocity component magnitude of domain.velocityResult (:75-78)."""
+    return float(domain.velocityResult.abs().max())
+
+
+def _face_slice(t, face):
+    """The layer of cells next to `face` (0 = -x, 1 = +x, 2 = -y, ...) of an NC[D]HW tensor, one cell thick."""
+    dim = t.dim() - 1 - (face >> 1)
+    n = t.size(dim)
+    return t.narrow(dim, n - 1 if face & 1 else 0, 1)
+
+
+def get_fixed_boundary_flux(bound, bound_idx):
+    """Total boundary-normal flux through a FixedBoundary, in the boundary's own orientation (:80-90)."""
+    if not isinstance(bound, PISOtorch.FixedBoundary):
+        raise TypeError("get_fixed_boundary_flux needs a FixedBoundary")
+    if bound.velocityType != PISOtorch.BoundaryConditionType.DIRICHLET:
+        raise ValueError("Only DIRICHLET boundaries are supportet")
+    return torch.sum(bound.GetFluxes())
+
+
+def get_fixed_boundary_fluxes(list_idx_bound):
+    """Net outward flux through a list of (face index, FixedBoundary): a lower face's flux counts negative,
+    an upper face's positive (:92-107)."""
+    domain = list_idx_bound[0][1].getParentDomain()
+    total = torch.zeros([1], dtype=domain.getDtype(), device=domain.getDevice())
+    for face, bound in list_idx_bound:
+        if not isinstance(bound, PISOtorch.FixedBoundary):
+            raise TypeError("boundary type not supported.")
+        flux = get_fixed_boundary_flux(bound, face)
+        total = total + flux if face & 1 else total - flux
+    return total
+
+
+def get_advective_velocity(velms, velm_idx, bound, bound_idx):
+    """The boundary-normal advection speed, in computational units, with which an outflow boundary is advected:
+    velms is one [1, dims] tensor for every boundary or a list with one tensor per boundary, static [1, dims] or
+    varying [1, dims, <boundary shape>].  With a transformed boundary the speed is row `axis` of the inverse
+    transform times the velocity, per boundary cell; otherwise it is the velocity's `axis` component
+    (:145-172).  Returns [1, 1] (static, untransformed) or [1, 1, <boundary shape>]."""
+    velm = velms if isinstance(velms, torch.Tensor) else velms[velm_idx]
+    dims = bound.getSpatialDims()
+    axis = bound_idx >> 1
+    size = bound.getSizes()
+    shape = [size[dims - 1 - d] for d in range(dims)]  # z, y, x
+    static = velm.dim() == 2
+    if not static and list(velm.shape[2:]) != shape:
+        raise ValueError("varying velm size must match boundary")
+    if not bound.hasTransform():
+        return velm[:, axis:axis + 1]
+    if static:
+        velm = velm.reshape([1, dims] + [1] * dims).expand([1, dims] + shape)
+    row = bound.transform[..., dims * dims + axis * dims: dims * dims + (axis + 1) * dims]  # [1, z, y, x, dims]
+    return (row * torch.movedim(velm, 1, -1)).sum(-1).unsqueeze(1)
+
+
+def balance_boundary_fluxes(domain, free_bounds, tol=None):
+    """Scales the velocities of the FixedBoundaries in free_bounds (every component) so that the net flux over
+    all fixed boundaries of the domain vanishes, unless it already does within 1% of the solver tolerance
+    (:174-207)."""
+    with torch.no_grad():
+        faces = [fb for block in domain.getBlocks() for fb in block.getFixedBoundaries()]
+        fixed = [fb for fb in faces if not any(fb[1] is b for b in free_bounds)]
+        free = [fb for fb in faces if any(fb[1] is b for b in free_bounds)]
+        fixed_flux = get_fixed_boundary_fluxes(fixed) if fixed else \
+            torch.zeros([1], dtype=domain.getDtype(), device=domain.getDevice())
+        free_flux = get_fixed_boundary_fluxes(free) if free else torch.zeros_like(fixed_flux)
+        atol = PISOtorch_diff._get_solver_tolerance(tol, dtype=domain.getDtype()) * 0.01
+        if torch.allclose(fixed_flux + free_flux, torch.zeros_like(fixed_flux), atol=atol):
+            return
+        scale = -fixed_flux / free_flux
+        for _, bound in free:
+            bound.setVelocity(bound.velocity * scale)
+
+
+def update_advective_boundaries(domain, bounds, velms, dt, tol=None):
+    """Advective (convective) outflow: each boundary in `bounds` (varying FixedBoundaries of the domain) moves
+    towards the cells next to it, u_b <- u_b - t (u_b - u_cell) with t = 1 - 1/(1 + 2 dt a), a the advection
+    speed of get_advective_velocity; a varying Dirichlet passive scalar on the boundary follows the same rule.
+    Then balance_boundary_fluxes rescales these boundaries so that the domain's boundary flux is balanced
+    (:210-311)."""
+    dims = domain.getSpatialDims()
+    if not (isinstance(bounds, list) and len(bounds) > 0):
+        raise ValueError("bounds must be a non-empty list of boundaries")
+
+    def _ok(v, varying):
+        return (isinstance(v, torch.Tensor) and v.size(0) == 1 and v.size(1) == dims
+                and (v.dim() == 2 or (varying and v.dim() == dims + 2)))
+
+    if not (_ok(velms, False) or (isinstance(velms, list) and len(velms) == len(bounds)
+                                  and all(_ok(v, True) for v in velms))):
+        raise ValueError("velms tensors have wrong shape")
+    with torch.no_grad():
+        found = []
+        for block in domain.getBlocks():
+            for face in range(2 * dims):
+                bound = block.getBoundary(face)
+                if isinstance(bound, PISOtorch.FixedBoundary) and any(bound is b for b in bounds):
+                    found.append((block, face, bound))
+        if len(found) != len(bounds):
+            raise ValueError("Bounds passed as advective outflow are not (all) part of the domain.")
+        if isinstance(dt, torch.Tensor):
+            dt = dt.to(domain.getDevice())
+        for block, face, bound in found:
+            idx = next(i for i, b in enumerate(bounds) if b is bound)
+            alpha = dt * 2 * get_advective_velocity(velms, idx, bound, face)
+            t = 1 - 1 / (1 + alpha)
+            if bound.isVelocityStatic:
+                raise ValueError("outflow boundaries must be varying. Use FixedBoundary.makeVelocityVarying() to "
+                                 "make the velocity varying.")
+            scalar = None
+            if block.hasPassiveScalar() and bound.hasPassiveScalar():
+                if bound.isPassiveScalarStatic():
+                    raise ValueError("outflow boundary passive scalar must be varying.")
+                if any(c != PISOtorch.BoundaryConditionType.DIRICHLET for c in bound.passiveScalarTypes):
+                    raise ValueError("passive scalar outflow boundary condition must be Dirichlet.")
+                scalar = bound.passiveScalar
+            vel = bound.velocity
+            bound.setVelocity(vel - t * (vel - _face_slice(block.velocity, face)))
+            if scalar is not None:
+                bound.setPassiveScalar(scalar - t * (scalar - _face_slice(block.passiveScalar, face)))
+    balance_boundary_fluxes(domain, bounds, tol=tol)
+
+
+def update_advective_boundaries_static(domain, bounds, velms, dt):
+    """The reference's static variant (:313-347) acts only on the deprecated Varying/StaticDirichletBoundary
+    types, which this package does not have; over FixedBoundaries it changes nothing, and so it does here."""
+    return None
+
+
+def append_prep_fn(prep_fn, name, fn):
+    """Adds fn after the hooks already registered under `name` (:349-361)."""
+    return _add_prep_fn(prep_fn, name, fn, False)
+
+
+def prepend_prep_fn(prep_fn, name, fn):
+    """Adds fn before the hooks already registered under `name` (:363-375)."""
+    return _add_prep_fn(prep_fn, name, fn, True)
+
+
+def _add_prep_fn(prep_fn, name, fn, first):
+    if not isinstance(prep_fn, dict):
+        raise TypeError("prep_fn must be a dict")
+    fns = prep_fn.get(name, [])
+    fns = list(fns) if isinstance(fns, (list, tuple)) else [fns]
+    fns.insert(0, fn) if first else fns.append(fn)
+    prep_fn[name] = fns
+    return prep_fn
+
+
+def add_pressure_ts_norm(prep_fn):
+    """Hooks that solve for dt * p instead of p: the pressure RHS divergence is multiplied by the time step after
+    the pressure setup and the result divided by it again after the pressure solve (:377-386)."""
+    def scale_rhs(domain, time_step, **kwargs):
+        domain.setPressureRHSdiv(domain.pressureRHSdiv * time_step.to(domain.getDevice()))
+        domain.UpdateDomainData()
+
+    def unscale_result(domain, time_step, **kwargs):
+        domain.setPressureResult(domain.pressureResult / time_step.to(domain.getDevice()))
+        domain.UpdateDomainData()
+    prepend_prep_fn(prep_fn, "POST_PRESSURE_SETUP", scale_rhs)
+    prepend_prep_fn(prep_fn, "POST_PRESSURE_NON_ORTHO", unscale_result)
+    return prep_fn
+
+
 class StopHandler:
     """PISOtorch_simulation.py:27-49 (without signal registration side effects unless asked)."""
 
@@ -578,10 +763,10 @@ class Simulation:
         if static:
             raise NotImplementedError("static (passive-scalar only) runs are not part of this build")
         domain = self.domain
-        flux = domain.GetBoundaryFluxBalance()
-        if float(ntonp(torch.abs(flux))) > float(self.pressure_tol):
+        flux = float(domain.GetBoundaryFluxBalance().reshape(-1)[0])
+        if abs(flux) > float(self.pressure_tol):
             _LOG.warning("Domain boundary is not divergence free (flux balance: %.03e). This can prevent pressure "
-                         "solve convergence.", float(ntonp(flux)))
+                         "solve convergence.", flux)
             return False
         sim_ok = True
         for it in range(1, iterations + 1):

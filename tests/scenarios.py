@@ -153,6 +153,31 @@ def cavity_static_fields(dtype=torch.float64, device="cuda", n=6, seed=5):
     return d
 
 
+def outflow_channel(dtype=torch.float64, device="cuda", coords=False, seed=6, nx=10, ny=8):
+    """Two blocks in a row (block0 +x joined to block1 -x), walls on +-y, inflow u = 1 on block0 -x and an
+    advective outflow on block1 +x (varying, initially u = 1).  With coords the grid is stretched in x and y, so
+    the outflow boundary carries a non-uniform transform."""
+    d = _domain(2, 0.02, dtype, device)
+    g = torch.Generator().manual_seed(seed)
+    bs = []
+    for i in range(2):
+        if coords:
+            b = _coords_block(d, stretched_coords(nx, ny, dtype, device, ratio=1.08, x0=float(i)), "B%d" % i)
+        else:
+            b = d.CreateBlockWithSize(P.Int4(nx, ny, 1), name="B%d" % i)
+        vel = _rand(b.velocity.shape, g, dtype, device, 0.2)
+        vel[:, 0] += 1.0
+        b.setVelocity(vel.contiguous())
+        b.CloseAllBoundaries()
+        bs.append(b)
+    bs[0].ConnectBlock("+x", bs[1], "-x", "+y")
+    bs[0].CloseBoundary("-x", velocity=torch.tensor([[1.0, 0.0]], dtype=dtype, device=device))
+    bs[1].CloseBoundary("+x", velocity=torch.tensor([[1.0, 0.0]], dtype=dtype, device=device))
+    bs[1].getBoundary("+x").makeVelocityVarying()
+    d.PrepareSolve()
+    return d
+
+
 ALL = {
     "cavity": cavity,
     "cavity_nocoords": lambda **kw: cavity(coords=False, **kw),

@@ -1,0 +1,143 @@
+"""Advective outflow and the other module-level helpers of PISOtorch_simulation (PISOtorch_simulation.py:55-386),
+checked on host-memory domains against hand-computed values:
+- update_advective_boundaries: t = 1 - 1/(1 + 2 dt a) per boundary cell, then balance_boundary_fluxes; known
+  answers for a plain and a transformed outflow boundary (the transform's inverse row of the boundary axis sets
+  a, and its determinant enters the flux), for a varying advection velocity, the limits dt -> 0 and dt -> inf,
+  and the argument errors of the reference;
+- balance_boundary_fluxes leaves a balanced domain unchanged;
+- get_max_time_step, append/prepend_prep_fn, add_pressure_ts_norm."""
+import pytest
+import torch
+
+from pict_amd import PISOtorch as P
+from pict_amd import PISOtorch_simulation as S
+
+F64 = torch.float64
+
+
+def _channel(nx=6, ny=4, u_in=1.0, last_column=(2.0, 4.0, 2.0, 4.0), v_last=0.2):
+    """One 2-D block, -x inflow u_in, +x outflow (varying, initially u = 1), +-y walls.  The interior velocity is
+    random except for the last column, which is set to u = last_column[j], v = v_last."""
+    d = P.Domain(2, torch.tensor([0.1], dtype=F64), "c", F64, "cpu", passiveScalarChannels=0, _allow_cpu=True)
+    g = torch.Generator().manual_seed(3)
+    vel = torch.rand(1, 2, ny, nx, generator=g, dtype=F64)
+    vel[0, 0, :, -1] = torch.tensor(last_column, dtype=F64)
+    vel[0, 1, :, -1] = v_last
+    b = d.CreateBlock(velocity=vel, name="b")
+    b.CloseAllBoundaries()
+    b.CloseBoundary("-x", velocity=torch.tensor([[u_in, 0.0]], dtype=F64))
+    b.CloseBoundary("+x", velocity=torch.tensor([[1.0, 0.0]], dtype=F64))
+    out = b.getBoundary("+x")
+    out.makeVelocityVarying()
+    d.PrepareSolve()
+    return d, b, out
+
+
+def test_outflow_known_answer():
+    d, b, out = _channel()
+    # a = 1 (velm x), dt = 0.5: t = 1 - 1/2 = 1/2, u_b = 1 - (1 - u_cell)/2 = [1.5, 2.5, 1.5, 2.5], v_b = 0.1;
+    # inflow flux 4, outflow flux 8 before balancing: every component is scaled by 1/2
+    S.update_advective_boundaries(d, [out], torch.tensor([[1.0, 0.0]], dtype=F64), torch.tensor([0.5], dtype=F64))
+    ub = out.velocity
+    assert ub.shape == (1, 2, 4, 1)
+    assert torch.allclose(ub[0, 0, :, 0], torch.tensor([0.75, 1.25, 0.75, 1.25], dtype=F64), atol=1e-15)
+    assert torch.allclose(ub[0, 1, :, 0], torch.full([4], 0.05, dtype=F64), atol=1e-15)
+    assert abs(float(d.GetBoundaryFluxBalance())) < 1e-14
+
+
+def test_outflow_transformed_boundary():
+    d, b, out = _channel()
+    # boundary transform: M = diag(1/2, 1/3), M^-1 = diag(2, 3), det M = 1/6 (channels M, M^-1, det)
+    tr = torch.tensor([0.5, 0, 0, 1 / 3, 2, 0, 0, 3, 1 / 6], dtype=F64).reshape(1, 1, 1, 9).repeat(1, 4, 1, 1)
+    out.setTransform(tr.contiguous())
+    # a = row x of M^-1 . (1, 0) = 2, dt = 0.25: t = 1/2, u_b = [1.5, 2.5, 1.5, 2.5]; outflow flux
+    # det * (M^-1 u)_x = (1/6) * 2 * 8 = 8/3 against the inflow 4: every component is scaled by 3/2
+    S.update_advective_boundaries(d, [out], torch.tensor([[1.0, 0.0]], dtype=F64), torch.tensor([0.25], dtype=F64))
+    assert torch.allclose(out.velocity[0, 0, :, 0], torch.tensor([2.25, 3.75, 2.25, 3.75], dtype=F64), atol=1e-14)
+    assert torch.allclose(out.velocity[0, 1, :, 0], torch.full([4], 0.15, dtype=F64), atol=1e-14)
+    assert abs(float(d.GetBoundaryFluxBalance())) < 1e-13
+    # the y row of M^-1 would give a = 0 for this velocity: t = 0, no update, only the balance
+    adv = S.get_advective_velocity(torch.tensor([[0.0, 1.0]], dtype=F64), 0, out, 1)
+    assert torch.allclose(adv, torch.zeros(1, 1, 4, 1, dtype=F64))
+    adv = S.get_advective_velocity(torch.tensor([[1.0, 1.0]], dtype=F64), 0, out, 1)
+    assert torch.allclose(adv, torch.full([1, 1, 4, 1], 2.0, dtype=F64))
+
+
+def test_outflow_varying_advection_velocity():
+    d, b, out = _channel()
+    velm = torch.tensor([1.0, 0.0, 3.0, 0.5], dtype=F64).reshape(1, 1, 4, 1) * torch.tensor([1.0, 7.0], dtype=F64)\
+        .reshape(1, 2, 1, 1)
+    # dt = 0.5: a = velm x = [1, 0, 3, 0.5], t = 1 - 1/(1 + a) = [1/2, 0, 3/4, 1/3]
+    S.update_advective_boundaries(d, [out], [velm.contiguous()], torch.tensor([0.5], dtype=F64))
+    u_cell = torch.tensor([2.0, 4.0, 2.0, 4.0], dtype=F64)
+    t = torch.tensor([0.5, 0.0, 0.75, 1 / 3], dtype=F64)
+    pre = 1 - t * (1 - u_cell)
+    scale = 4.0 / float(pre.sum())
+    assert torch.allclose(out.velocity[0, 0, :, 0], pre * scale, atol=1e-14)
+    assert torch.allclose(out.velocity[0, 1, :, 0], t * 0.2 * scale, atol=1e-14)
+
+
+def test_outflow_limits():
+    # dt -> 0: no interpolation; the domain is balanced (inflow 4, outflow 4), so nothing changes
+    d, b, out = _channel()
+    before = out.velocity.clone()
+    S.update_advective_boundaries(d, [out], torch.tensor([[1.0, 0.0]], dtype=F64), torch.tensor([0.0], dtype=F64))
+    assert torch.equal(out.velocity, before)
+    # dt -> inf: the boundary takes the values of the cells next to it, then the balance scale 4 / 12
+    d, b, out = _channel()
+    S.update_advective_boundaries(d, [out], torch.tensor([[1.0, 0.0]], dtype=F64), torch.tensor([1e30], dtype=F64))
+    assert torch.allclose(out.velocity[0, :, :, 0], b.velocity[0, :, :, -1] / 3, atol=1e-14)
+
+
+def test_balance_leaves_balanced_domain_unchanged():
+    d, b, out = _channel()
+    u = torch.zeros(1, 2, 4, 1, dtype=F64)
+    u[0, 0, :, 0] = torch.tensor([2.0, 0.0, 1.0, 1.0], dtype=F64)
+    out.setVelocity(u)
+    before = out.velocity.clone()
+    S.balance_boundary_fluxes(d, [out])
+    assert torch.equal(out.velocity, before)
+    # fluxes of (face, boundary) lists: lower faces count negative, upper faces positive
+    assert float(S.get_fixed_boundary_fluxes([(0, b.getBoundary("-x"))])) == -4.0
+    assert float(S.get_fixed_boundary_fluxes([(1, out)])) == 4.0
+    assert float(S.get_fixed_boundary_flux(out, 1)) == 4.0
+
+
+def test_outflow_errors():
+    d, b, out = _channel()
+    vm = torch.tensor([[1.0, 0.0]], dtype=F64)
+    with pytest.raises(ValueError):  # a static outflow boundary
+        S.update_advective_boundaries(d, [b.getBoundary("-x")], vm, 0.1)
+    other, _, out2 = _channel()
+    with pytest.raises(ValueError):  # a boundary of another domain
+        S.update_advective_boundaries(d, [out2], vm, 0.1)
+    with pytest.raises(ValueError):  # velocity of the wrong dimensionality
+        S.update_advective_boundaries(d, [out], torch.tensor([[1.0, 0.0, 0.0]], dtype=F64), 0.1)
+    with pytest.raises(ValueError):  # one velocity per boundary
+        S.update_advective_boundaries(d, [out], [vm, vm], 0.1)
+    with pytest.raises(ValueError):
+        S.update_advective_boundaries(d, [], vm, 0.1)
+    with pytest.raises(TypeError):
+        S.get_fixed_boundary_fluxes([(2, b.getBoundary("-y")), (0, P.PeriodicBoundary(d))])
+
+
+def test_max_time_step_and_prep_fn_lists():
+    d, b, out = _channel()
+    vmax = float(d.getMaxVelocity(True, False))
+    assert d.getMaxVelocity(True, False).dim() == 0
+    assert vmax == pytest.approx(float(max(b.velocity.abs().max(), out.velocity.abs().max())))
+    # CFL 0.8 with |u|max = 4: dt_max = 0.2
+    assert S.get_max_time_step(d, 0.1, with_transformations=False) == (0.1, 1)
+    ts, ss = S.get_max_time_step(d, 0.5, with_transformations=False)
+    assert ss == 3 and ts == pytest.approx(0.5 / 3)
+    calls = []
+    pf = {"PRE": lambda **kw: calls.append("a")}
+    S.append_prep_fn(pf, "PRE", lambda **kw: calls.append("b"))
+    S.prepend_prep_fn(pf, "PRE", lambda **kw: calls.append("c"))
+    S.append_prep_fn(pf, "POST", lambda **kw: calls.append("d"))
+    for name in ("PRE", "POST"):
+        for fn in pf[name]:
+            fn()
+    assert calls == ["c", "a", "b", "d"]
+    S.add_pressure_ts_norm(pf)
+    assert len(pf["POST_PRESSURE_SETUP"]) == 1 and len(pf["POST_PRESSURE_NON_ORTHO"]) == 1
