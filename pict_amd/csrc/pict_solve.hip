@@ -50,6 +50,7 @@ struct Ctl {
     unsigned pad2[30];
     unsigned group_count[8][32]; // per-group arrivals
 };
+constexpr int kLeanWgsPerCu = 4;  // resident workgroups per compute unit of the lean streaming CG
 constexpr int kGroups = 8;       // arrival tree fan-in (one group per XCD when blocks are dealt round-robin)
 
 struct RhsState {
@@ -665,8 +666,11 @@ __device__ __forceinline__ void cg1_init(RhsState &st, double rr, int zero_all) 
 //   beta' = gamma'/gamma, alpha' = gamma' / (delta' - beta' gamma' / alpha).
 // r, w, s are double-buffered (neighbours read the previous values); p and x
 // are only touched by their owner.
+// LEAN: without the register row cache and its gathered form, for systems whose
+// rows outnumber what the cache holds (every thread then streams its rows from
+// memory); the cache's registers would only cut the resident waves.
 // ==========================================================================
-template <typename R, int NRHS>
+template <typename R, int NRHS, bool LEAN>
 __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
     __shared__ double red[NRHS * kSlots];
     __shared__ RhsState sst[NRHS];
@@ -756,10 +760,10 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
     PICT_CG1_W_PHASE(0)
 
     // matrix rows of this thread in registers when every thread owns at most kRowsCached rows
-    RowCache<R, kRowsCached> rc;
+    RowCache<R, LEAN ? 1 : kRowsCached> rc;
     // rc.len[q]: -1 no row, -2 row longer than kMaxNz (read from memory), else cached length
-    const bool cached = stride * kRowsCached >= (size_t)n;
-    if (cached) {
+    const bool cached = !LEAN && stride * kRowsCached >= (size_t)n;
+    if constexpr (!LEAN) if (cached) {
 #pragma unroll
         for (int q = 0; q < kRowsCached; ++q) {
             const size_t i = i0 + (size_t)q * stride;
@@ -851,7 +855,9 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
             };
             // register budget: the gathered form for up to 8 bytes of right-hand-side values per row
             constexpr bool kGather = NRHS * sizeof(R) <= 8;
-            if (cached && !kGather) {
+            if constexpr (LEAN) {
+                for (size_t i = i0; i < (size_t)n; i += stride) mem_row(i);
+            } else if (cached && !kGather) {
 #pragma unroll
                 for (int q = 0; q < kRowsCached; ++q) {
                     const size_t i = i0 + (size_t)q * stride;
@@ -2325,7 +2331,16 @@ const void *kernel_for(Kind kind) {
     switch (kind) {
         case Kind::kBiCG: return (const void *)k_bicgstab<R, NR>;
         case Kind::kCGTextbook: return (const void *)k_cg<R, NR>;
-        default: return (const void *)k_cg1<R, NR>;
+        default: return (const void *)k_cg1<R, NR, false>;
+    }
+}
+template <typename R>
+const void *cg1_lean_for_nrhs(int nrhs) {
+    switch (nrhs) {
+        case 1: return (const void *)k_cg1<R, 1, true>;
+        case 2: return (const void *)k_cg1<R, 2, true>;
+        case 3: return (const void *)k_cg1<R, 3, true>;
+        default: return (const void *)k_cg1<R, 4, true>;
     }
 }
 template <typename R>
@@ -2397,6 +2412,7 @@ struct Plan {
     bool bicg_local;  // k_bicg_local
     int rpt, nz;      // register-resident kernels: rows per thread, cached row length
     bool full;      // k_cg_local reduces the sum and maximum of |r| too
+    bool lean;      // k_cg1 without the register row cache
 };
 
 template <typename R>
@@ -2478,12 +2494,30 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
         return v < 1 ? 1 : v;
     }();
     const long long per_wg = (long long)kSolveThreads * rows_per_thread;
-    long long nwg = ((long long)n + per_wg - 1) / per_wg;
-    const long long cap = (long long)occ * cus;
-    if (nwg > cap) nwg = cap;
-    if (nwg > 2048) nwg = 2048;
-    if (nwg < 1) nwg = 1;
-    pl.nwg = nwg;
+    auto grid = [&](int occ_k) {
+        long long nwg = ((long long)n + per_wg - 1) / per_wg;
+        const long long cap = (long long)occ_k * cus;
+        if (nwg > cap) nwg = cap;
+        if (nwg > 2048) nwg = 2048;
+        if (nwg < 1) nwg = 1;
+        return nwg;
+    };
+    pl.nwg = grid(occ);
+    // rows beyond the register row cache: the lean kernel, with the waves its registers allow, up to four
+    // workgroups per compute unit (with seven per unit, which the occupancy query allows, the 1.77 M-row
+    // pressure solve of the 192 x 96 x 96 channel ended at the barrier's spin limit; four per unit is the
+    // grid the streaming BiCGStab runs on that system)
+    if (pl.kind == Kind::kCG1 && (long long)pl.nwg * kSolveThreads * kRowsCached < (long long)n) {
+        const void *k = cg1_lean_for_nrhs<R>(nrhs);
+        int locc = 0, lcus = 0;
+        st = occupancy(k, locc, lcus);
+        if (st) return st;
+        if (locc > occ) {
+            pl.kernel = k;
+            pl.lean = true;
+            pl.nwg = grid(locc < kLeanWgsPerCu ? locc : kLeanWgsPerCu);
+        }
+    }
     return PICT_OK;
 }
 
@@ -3128,7 +3162,7 @@ pict_status pict_solver_kernel_name(const pict_csr_desc *A, int32_t nrhs, const 
     else if (pl.kind == Kind::kCGTextbook)
         snprintf(buf, (size_t)len, "pict::k_cg<%s, %d>", t, nr);
     else
-        snprintf(buf, (size_t)len, "pict::k_cg1<%s, %d>", t, nr);
+        snprintf(buf, (size_t)len, "pict::k_cg1<%s, %d, %s>", t, nr, pl.lean ? "true" : "false");
     return PICT_OK;
 }
 

@@ -208,3 +208,36 @@ def test_local_cg_deterministic():
     for _ in range(3):
         x2, i2 = solve(A, b, 1e-6)
         assert i2.usedIterations == i1.usedIterations and torch.equal(x1, x2)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_streaming_cg_lean_kernel(dtype, monkeypatch):
+    """A 3-D system with more rows than the grid-wide CG's register row cache holds (64 x 64 x 80 = 327,680 rows)
+    runs the lean streaming kernel; the textbook two-barrier CG (the reference's iterates) agrees on the iteration
+    count within 3% + 2, and the true FP64 residual |b - A x|_2 / sqrt(n) of both is at the solver tolerance."""
+    print("case lean streaming CG", dtype, flush=True)
+    m = periodic_laplacian((64, 64, 80), 0.05)
+    n = m.shape[0]
+    A = to_pict(m, dtype)
+    b = rhs(n, 5, dtype)
+    monkeypatch.setenv("PICT_CG_LOCAL_WGS", "0")
+    t = "float" if dtype == torch.float32 else "double"
+    assert kernel_name(A) == "pict::k_cg1<%s, 1, true>" % t
+    tol = 1e-10 if dtype == torch.float64 else 1e-6
+    x, info = solve(A, b, tol)
+    assert info.converged and info.isFiniteResidual
+    monkeypatch.setenv("PICT_CG_TEXTBOOK", "1")
+    assert kernel_name(A) == "pict::k_cg<%s, 1>" % t
+    xt, it = solve(A, b, tol)
+    assert it.converged
+    print(" its lean %d textbook %d" % (info.usedIterations, it.usedIterations), flush=True)
+    assert abs(info.usedIterations - it.usedIterations) <= 2 + 0.03 * it.usedIterations
+    bd = b.double().cpu().numpy()
+    for sol in (x, xt):
+        res = np.linalg.norm(bd - m @ sol.double().cpu().numpy()) / np.sqrt(n)
+        print(" true residual %.3e" % res, flush=True)
+        assert res < (2 * tol if dtype == torch.float64 else 1e-5)
+    # a system the row cache holds keeps the cached kernel
+    small = to_pict(periodic_laplacian((64, 64, 16), 0.05), dtype)
+    monkeypatch.delenv("PICT_CG_TEXTBOOK")
+    assert kernel_name(small) == "pict::k_cg1<%s, 1, false>" % t
