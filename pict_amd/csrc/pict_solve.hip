@@ -50,7 +50,11 @@ struct Ctl {
     unsigned pad2[30];
     unsigned group_count[8][32]; // per-group arrivals
 };
-constexpr int kLeanWgsPerCu = 4;  // resident workgroups per compute unit of the lean streaming CG
+// Resident workgroups (256 threads) per compute unit that a persistent solver grid is sized for.  The
+// occupancy query allows up to eight for the leaner kernels, but grids of seven and eight per unit (1792 and
+// 2048 workgroups: the 1.77 M-row channel pressure system on the lean grid-wide CG and on the textbook CG)
+// ended at the barrier's spin limit, while grids of four per unit run.
+constexpr int kMaxResidentWgsPerCu = 4;
 constexpr int kGroups = 8;       // arrival tree fan-in (one group per XCD when blocks are dealt round-robin)
 
 struct RhsState {
@@ -2298,7 +2302,8 @@ thread_local bool g_timing = false;
 thread_local double g_last_ms = 0.0;
 thread_local hipEvent_t g_ev[2] = {nullptr, nullptr};
 
-// Occupancy (resident workgroups per CU) of a solver kernel on the current device, cached.
+// Occupancy (resident workgroups per CU) of a solver kernel on the current device, cached, at most
+// kMaxResidentWgsPerCu: every workgroup of a persistent solver grid has to be resident at once.
 pict_status occupancy(const void *kernel, int &occ, int &cus) {
     struct Entry { int dev; const void *k; int occ; int cus; };
     static thread_local std::vector<Entry> cache;
@@ -2312,6 +2317,7 @@ pict_status occupancy(const void *kernel, int &occ, int &cus) {
         }
     PICT_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     PICT_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kSolveThreads, 0));
+    if (occ > kMaxResidentWgsPerCu) occ = kMaxResidentWgsPerCu;
     cache.push_back({dev, kernel, occ, cus});
     return PICT_OK;
 }
@@ -2503,10 +2509,7 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
         return nwg;
     };
     pl.nwg = grid(occ);
-    // rows beyond the register row cache: the lean kernel, with the waves its registers allow, up to four
-    // workgroups per compute unit (with seven per unit, which the occupancy query allows, the 1.77 M-row
-    // pressure solve of the 192 x 96 x 96 channel ended at the barrier's spin limit; four per unit is the
-    // grid the streaming BiCGStab runs on that system)
+    // rows beyond the register row cache: the lean kernel, with the waves its registers allow
     if (pl.kind == Kind::kCG1 && (long long)pl.nwg * kSolveThreads * kRowsCached < (long long)n) {
         const void *k = cg1_lean_for_nrhs<R>(nrhs);
         int locc = 0, lcus = 0;
@@ -2515,7 +2518,7 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
         if (locc > occ) {
             pl.kernel = k;
             pl.lean = true;
-            pl.nwg = grid(locc < kLeanWgsPerCu ? locc : kLeanWgsPerCu);
+            pl.nwg = grid(locc);
         }
     }
     return PICT_OK;

@@ -241,3 +241,25 @@ def test_streaming_cg_lean_kernel(dtype, monkeypatch):
     small = to_pict(periodic_laplacian((64, 64, 16), 0.05), dtype)
     monkeypatch.delenv("PICT_CG_TEXTBOOK")
     assert kernel_name(small) == "pict::k_cg1<%s, 1, false>" % t
+
+
+@pytest.mark.parametrize("bicg", [False, True])
+def test_large_grid_wide_solves_stay_resident(bicg, monkeypatch):
+    """Grid-wide kernels with few registers on the 3-D channel's size (192 x 96 x 96 = 1,769,472 rows): the
+    textbook CG and the one-right-hand-side BiCGStab.  Their persistent grids are sized at no more than four
+    workgroups per compute unit; with the occupancy query's seven or eight, the textbook CG's barrier ran into
+    its spin limit on this system.  Thirty iterations run without a barrier error and reduce the residual."""
+    print("case large grid-wide", "bicgstab" if bicg else "textbook cg", flush=True)
+    m = periodic_laplacian((192, 96, 96), 0.05)
+    A = to_pict(m, torch.float32)
+    b = rhs(m.shape[0], 7, torch.float32)
+    monkeypatch.setenv("PICT_CG_LOCAL_WGS", "0")
+    if not bicg:
+        monkeypatch.setenv("PICT_CG_TEXTBOOK", "1")
+    x = torch.zeros_like(b)
+    info = P.SolveLinear(A, b, x, 30, 1e-30, NORM2N, bicg, False, 0, False, False, False, False)[0]
+    assert info.isFiniteResidual and info.usedIterations >= 29
+    bd = b.double().cpu().numpy()
+    res = np.linalg.norm(bd - m @ x.double().cpu().numpy()) / np.linalg.norm(bd)
+    print(" relative residual after 30 iterations %.3e" % res, flush=True)
+    assert res < 0.5
