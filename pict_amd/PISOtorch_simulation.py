@@ -68,7 +68,7 @@ def _face_slice(t, face):
 def get_fixed_boundary_flux(bound, bound_idx):
     """Total boundary-normal flux through a FixedBoundary, in the boundary's own orientation (:80-90)."""
     if not isinstance(bound, PISOtorch.FixedBoundary):
-        raise TypeError("get_fixed_boundary_flux needs a FixedBoundary")
+        raise AssertionError("get_fixed_boundary_flux needs a FixedBoundary")
     if bound.velocityType != PISOtorch.BoundaryConditionType.DIRICHLET:
         raise ValueError("Only DIRICHLET boundaries are supportet")
     return torch.sum(bound.GetFluxes())
@@ -100,7 +100,7 @@ def get_advective_velocity(velms, velm_idx, bound, bound_idx):
     shape = [size[dims - 1 - d] for d in range(dims)]  # z, y, x
     static = velm.dim() == 2
     if not static and list(velm.shape[2:]) != shape:
-        raise ValueError("varying velm size must match boundary")
+        raise AssertionError("varying velm size must match boundary")
     if not bound.hasTransform():
         return velm[:, axis:axis + 1]
     if static:
@@ -135,8 +135,9 @@ def update_advective_boundaries(domain, bounds, velms, dt, tol=None):
     Then balance_boundary_fluxes rescales these boundaries so that the domain's boundary flux is balanced
     (:210-311)."""
     dims = domain.getSpatialDims()
+    # argument checks are assertions in the reference: the same exception type here
     if not (isinstance(bounds, list) and len(bounds) > 0):
-        raise ValueError("bounds must be a non-empty list of boundaries")
+        raise AssertionError("bounds must be a non-empty list of boundaries")
 
     def _ok(v, varying):
         return (isinstance(v, torch.Tensor) and v.size(0) == 1 and v.size(1) == dims
@@ -144,7 +145,7 @@ def update_advective_boundaries(domain, bounds, velms, dt, tol=None):
 
     if not (_ok(velms, False) or (isinstance(velms, list) and len(velms) == len(bounds)
                                   and all(_ok(v, True) for v in velms))):
-        raise ValueError("velms tensors have wrong shape")
+        raise AssertionError("velms tensors have wrong shape")
     with torch.no_grad():
         found = []
         for block in domain.getBlocks():

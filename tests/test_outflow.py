@@ -111,12 +111,17 @@ def test_outflow_errors():
     other, _, out2 = _channel()
     with pytest.raises(ValueError):  # a boundary of another domain
         S.update_advective_boundaries(d, [out2], vm, 0.1)
-    with pytest.raises(ValueError):  # velocity of the wrong dimensionality
+    # argument checks: assertions, as in the reference
+    with pytest.raises(AssertionError):  # velocity of the wrong dimensionality
         S.update_advective_boundaries(d, [out], torch.tensor([[1.0, 0.0, 0.0]], dtype=F64), 0.1)
-    with pytest.raises(ValueError):  # one velocity per boundary
+    with pytest.raises(AssertionError):  # one velocity per boundary
         S.update_advective_boundaries(d, [out], [vm, vm], 0.1)
-    with pytest.raises(ValueError):
+    with pytest.raises(AssertionError):
         S.update_advective_boundaries(d, [], vm, 0.1)
+    with pytest.raises(AssertionError):  # a varying velocity of another shape than the boundary
+        S.update_advective_boundaries(d, [out], [torch.zeros(1, 2, 3, 1, dtype=F64)], 0.1)
+    with pytest.raises(AssertionError):
+        S.get_fixed_boundary_flux(P.PeriodicBoundary(d), 0)
     with pytest.raises(TypeError):
         S.get_fixed_boundary_fluxes([(2, b.getBoundary("-y")), (0, P.PeriodicBoundary(d))])
 
