@@ -56,6 +56,7 @@ def make_workload(args, torch):
                                   % args.steps if args.mode == "fwdbwd" else "forward PISO steps"))
         sim = S.Simulation(dom, time_step=dt, corrector_steps=2, non_orthogonal=False, advection_tol=1e-6,
                            pressure_tol=1e-6, advect_passive_scalar=False)
+        sim.defer_solver_checks = not args.sync_checks
         return dom, sim, dt, workload
     if args.config == "c3":
         dom, dt = configs.periodic_box_c3(256, dtype=dtype, device="cuda")
@@ -72,6 +73,7 @@ def make_workload(args, torch):
     dom.UpdateDomainData()
     sim = S.Simulation(dom, time_step=dt, corrector_steps=2, non_orthogonal=False, advection_tol=1e-6,
                        pressure_tol=1e-6, advect_passive_scalar=False)
+    sim.defer_solver_checks = not args.sync_checks
     return dom, sim, dt, workload
 
 
@@ -104,6 +106,7 @@ def rollout_c5(dom, sim, steps, target, torch, keep_tape=False):
     out = CK.rollout_with_sources(sim, steps, lambda u: [forcing + 0.05 * net(u[0])], keep_tape=keep_tape)
     loss = ((out[0] - target[0]) ** 2).mean()
     loss.backward()
+    sim.flush_solver_checks()
     # back to the initial state (the source keeps its per-cell shape: its gradient tensor has that shape)
     blk.setVelocity(u0)
     blk.setVelocitySource(forcing.expand_as(u0).contiguous())
@@ -121,6 +124,7 @@ def rollout_fwdbwd(dom, sim, steps, target, torch, keep_tape=False):
     out = CK.rollout(sim, steps, params=srcs, checkpoint=True, keep_tape=keep_tape)
     loss = sum(((o - t) ** 2).mean() for o, t in zip(out, target))
     loss.backward()
+    sim.flush_solver_checks()
     # back to the initial state for the next rollout
     for b, u in zip(dom.blocks, u0):
         b.setVelocity(u)
@@ -137,6 +141,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=["c3", "c1", "c4", "c5"])
     ap.add_argument("--batch", type=int, default=64, help="c4: number of cavities")
+    ap.add_argument("--sync-checks", action="store_true",
+                    help="c1/c3/c5: check every native step's solver results before queueing the next, as the "
+                         "reference's driver does (default: queue without waiting and check them, in order, at the "
+                         "end of each rollout)")
     ap.add_argument("--no-batched", action="store_true",
                     help="c3/c1: skip the batching configuration (64 cavities, 100 steps) reported under `batched`")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -174,6 +182,7 @@ def main():
         if args.mode == "fwdbwd":
             return (rollout_c5 if args.config == "c5" else rollout_fwdbwd)(dom, sim, k, target, torch)
         assert sim._PISO_split_step(k)
+        sim.flush_solver_checks()
         return None
 
     for _ in range(args.warmup):
@@ -221,7 +230,7 @@ def main():
         # the same rollout keeping each step's tape (no recomputation), and the forward alone, for context
         roll = rollout_c5 if args.config == "c5" else rollout_fwdbwd
         for label, fn in (("keep_tape", lambda: roll(dom, sim, args.steps, target, torch, True)),
-                          ("forward_only", lambda: sim._PISO_split_step(args.steps))):
+                          ("forward_only", lambda: (sim._PISO_split_step(args.steps), sim.flush_solver_checks()))):
             torch.cuda.synchronize()
             f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             f0.record()
@@ -277,6 +286,7 @@ def main():
         "dtype": "f32 (FP64 Krylov reductions)",
         "data": "synthetic (seeded)",
         "config": {"workload": workload, "cells": N, "config": args.config, "mode": args.mode, "dt": dt,
+                   "solver_checks": "after every step" if args.sync_checks else "deferred to the end of the rollout",
                    "solver_launches": {k: v["launches"] for k, v in agg.items()},
                    "solver_adjoint_launches": {k: v["adjoint_launches"] for k, v in agg.items()},
                    "solver_iterations_per_step": {k: v["iters"] / args.steps for k, v in agg.items()},

@@ -380,6 +380,17 @@ pict_status pict_piso_step(const pict_atlas *atlas, const pict_step_opts *opts, 
    the block pressures are overwritten.  infos_out: num_dims + corrector_steps adjoint solves. */
 pict_status pict_piso_step_vjp(const pict_atlas *atlas, const pict_step_opts *opts, const pict_step_tape *tape,
                                void *velocity_grad, pict_solve_info *infos_out, void *stream);
+/* pict_piso_step and pict_piso_step_vjp without the host wait (no counterpart in the reference, whose driver
+   reads every solve's result on the host before going on): the work is queued on `stream` and *ticket names the
+   pending step; pict_step_wait(ticket, infos_out) waits for it and returns the same infos and errors as the
+   waiting call would have.  At most 16 steps may be pending per process; a ticket may be collected from any
+   thread.  Solver timing (pict_set_step_timing) must be off. */
+pict_status pict_piso_step_async(const pict_atlas *atlas, const pict_step_opts *opts, const pict_step_tape *tape,
+                                 int32_t *ticket, void *stream);
+pict_status pict_piso_step_vjp_async(const pict_atlas *atlas, const pict_step_opts *opts,
+                                     const pict_step_tape *tape, void *velocity_grad, int32_t *ticket,
+                                     void *stream);
+pict_status pict_step_wait(int32_t ticket, pict_solve_info *infos_out);
 /* v -= mean(v) in place, the mean accumulated in FP64 in a fixed order: the pressure
    normalisation of the driver (PISOtorch_simulation.py:1243-1244, a torch expression there). */
 pict_status pict_subtract_mean(void *v, int32_t n, int32_t dtype, void *stream);

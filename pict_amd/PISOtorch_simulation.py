@@ -316,6 +316,10 @@ class Simulation:
         self.pressure_refine_max_sweeps = pressure_refine_max_sweeps
         self.solver_infos = []  # (kind, [LinearSolverResultInfo]) of the last step, for iteration-count logging
         self.record_solver_infos = False
+        # native steps (and checkpointed reverse passes) queued without the host wait for their solver results;
+        # flush_solver_checks() waits for them and raises what the waiting calls would have raised
+        self.defer_solver_checks = False
+        self._pending_checks = []
         self.reset_step_counters()
 
     # --- properties (validation as in :552-835) ---
@@ -699,6 +703,39 @@ class Simulation:
                                              self.pressure_return_best_result)
         return o
 
+    def _defer_check(self, fn, o, adjoint, *args):
+        """Queue an asynchronous native call (`fn`, its arguments before the ticket) and keep its ticket."""
+        from . import _abi
+        if len(self._pending_checks) >= 16:
+            self.flush_solver_checks()
+        t = ctypes.c_int32(-1)
+        _abi.check(fn(*args, ctypes.byref(t), PISOtorch._stream(self.domain.getDevice())),
+                   "PISO step reverse pass" if adjoint else "PISO step")
+        self._pending_checks.append((t.value, o, adjoint))
+
+    def flush_solver_checks(self):
+        """Wait for the native calls queued with defer_solver_checks and check their solver results in order,
+        raising the first error (LinsolveError, or a solver barrier error) the waiting calls would have raised."""
+        if not self._pending_checks:
+            return
+        from . import _abi
+        lib = _abi.load_library()
+        pending, self._pending_checks = self._pending_checks, []
+        D = self.domain.getSpatialDims()
+        err = None
+        for ticket, o, adjoint in pending:
+            infos = (_abi.SolveInfo * (D + o.corrector_steps))()
+            st = lib.pict_step_wait(ticket, infos)
+            if err is not None:
+                continue
+            try:
+                _abi.check(st, "PISO step reverse pass" if adjoint else "PISO step")
+                self._native_record(infos, o, None, adjoint)
+            except Exception as e:  # every ticket is still collected
+                err = e
+        if err is not None:
+            raise err
+
     def _native_record(self, infos, o, kms=None, adjoint=False):
         """Solver bookkeeping of one native step or reverse pass: solve log, recorded infos, LinsolveError."""
         domain = self.domain
@@ -737,6 +774,14 @@ class Simulation:
         o = self._native_opts(time_step)
         lib = _abi.load_library()
         timing = PISOtorch.solve_log is not None
+        if self.defer_solver_checks and not timing:
+            for _ in range(iterations):
+                domain.UpdateDomainData()
+                self._defer_check(lib.pict_piso_step_async, o, False, domain._atlas, ctypes.byref(o),
+                                  ctypes.byref(tape) if tape is not None else None)
+                self.end_step(time_step)
+            return True
+        self.flush_solver_checks()
         if timing:
             _abi.check(lib.pict_set_step_timing(1))
         try:

@@ -236,6 +236,9 @@ SIGNATURES = {
     "pict_csr_transpose_permutation": (c_int32, [POINTER(CsrDesc), _P, _P]),
     "pict_piso_step": (c_int32, [_P, POINTER(StepOpts), _P, POINTER(SolveInfo), _P]),
     "pict_piso_step_vjp": (c_int32, [_P, POINTER(StepOpts), POINTER(StepTape), _P, POINTER(SolveInfo), _P]),
+    "pict_piso_step_async": (c_int32, [_P, POINTER(StepOpts), _P, POINTER(c_int32), _P]),
+    "pict_piso_step_vjp_async": (c_int32, [_P, POINTER(StepOpts), POINTER(StepTape), _P, POINTER(c_int32), _P]),
+    "pict_step_wait": (c_int32, [c_int32, POINTER(SolveInfo)]),
     "pict_subtract_mean": (c_int32, [_P, c_int32, c_int32, _P]),
     "pict_batch_create": (c_int32, [POINTER(c_void_p), c_int32, POINTER(c_void_p)]),
     "pict_batch_destroy": (c_int32, [_P]),

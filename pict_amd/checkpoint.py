@@ -272,6 +272,10 @@ def _native_backward(sim, tape, vel_grads):
     infos = (_abi.SolveInfo * (D + K))()
     lib = _abi.load_library()
     timing = PISOtorch.solve_log is not None
+    if sim.defer_solver_checks and not timing:
+        sim._defer_check(lib.pict_piso_step_vjp_async, o, True, domain._atlas, ctypes.byref(o),
+                         ctypes.byref(tape.desc), ctypes.c_void_p(g.data_ptr()))
+        return g
     if timing:
         _abi.check(lib.pict_set_step_timing(1))
     try:

@@ -6,6 +6,7 @@
 // correction}, copy to blocks) with every stage and solve queued back to back: the
 // host waits once per step, to read the solvers' final states, instead of once per
 // solve, and no interpreter runs between the stages.
+#include <mutex>
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -584,6 +585,87 @@ pict_status run(const pict_atlas *a, int kind, const pict_step_opts *o, const pi
     return collect(h.p, z, o, a->desc.num_dims, infos_out, ev);
 }
 
+// Queued steps whose solver states are still on their way to the host: a ring of pinned slots, each with the
+// event recorded after the step's last copy.  The caller collects a slot with pict_step_wait.  One ring per
+// process (a reverse pass may be queued from the autograd engine's thread and collected from another), kept
+// until the process ends.
+constexpr int kPendingSteps = 16;
+struct PendingStep {
+    bool busy = false;
+    hipEvent_t ev = nullptr;
+    void *host = nullptr;
+    size_t bytes = 0;
+    Sizes z{};
+    pict_step_opts o{};
+    int D = 0;
+};
+struct PendingRing {
+    std::mutex mu;
+    PendingStep slot[kPendingSteps];
+};
+PendingRing &pending_ring() {
+    static PendingRing *r = new PendingRing;
+    return *r;
+}
+
+pict_status run_async(const pict_atlas *a, int kind, const pict_step_opts *o, const pict_step_tape *tape,
+                      void *vgrad, int32_t *ticket, hipStream_t s) {
+    const bool vjp = kind == 2;
+    const Sizes z = sizes_of(a, o, vjp);
+    if (g_step_timing.enabled) {
+        set_error("an asynchronous step cannot be timed (solver timing is on)");
+        return PICT_ERR_INVALID;
+    }
+    PendingRing &ring = pending_ring();
+    int idx = -1;
+    {
+        std::lock_guard<std::mutex> lock(ring.mu);
+        for (int i = 0; i < kPendingSteps; ++i)
+            if (!ring.slot[i].busy) {
+                idx = i;
+                ring.slot[i].busy = true;  // reserved; released below if queueing fails
+                break;
+            }
+    }
+    if (idx < 0) {
+        set_error("%d asynchronous steps are pending: collect them with pict_step_wait", kPendingSteps);
+        return PICT_ERR_INVALID;
+    }
+    PendingStep &p = ring.slot[idx];
+    struct Release {
+        PendingRing &r; PendingStep &p; bool keep = false;
+        ~Release() { if (!keep) { std::lock_guard<std::mutex> lock(r.mu); p.busy = false; } }
+    } release{ring, p};
+    const size_t need = z.slot * z.nsolves;
+    if (p.bytes < need) {
+        if (p.host) PICT_HIP_TRY(hipHostFree(p.host));
+        p.host = nullptr;
+        p.bytes = 0;
+        PICT_HIP_TRY(hipHostMalloc(&p.host, need, hipHostMallocDefault));
+        p.bytes = need;
+    }
+    if (!p.ev) PICT_HIP_TRY(hipEventCreateWithFlags(&p.ev, hipEventDisableTiming));
+    char *dev = nullptr;
+    PICT_HIP_TRY(ws_alloc((void **)&dev, z.dev_bytes, s));
+    struct Guard {
+        char *p; hipStream_t s;
+        ~Guard() { if (p) (void)ws_free(p, s); }
+    } guard{dev, s};
+    Work w;
+    w.dev = dev;
+    w.host = p.host;
+    w.ev = nullptr;
+    pict_status st = vjp ? enqueue_vjp(a, o, tape, vgrad, w, z, s) : enqueue_step(a, o, tape, w, z, s);
+    if (st) return st;
+    PICT_HIP_TRY(hipEventRecord(p.ev, s));
+    p.z = z;
+    p.o = *o;
+    p.D = a->desc.num_dims;
+    release.keep = true;
+    *ticket = idx;
+    return PICT_OK;
+}
+
 }  // namespace
 
 namespace pict {
@@ -595,9 +677,9 @@ void step_graphs_free(pict_atlas *a) {
 
 extern "C" {
 
-pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
-                           pict_solve_info *infos_out, void *stream) {
-    PICT_CHECK_ARG(a && o && infos_out, "NULL argument");
+static pict_status step_entry(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
+                              pict_solve_info *infos_out, int32_t *ticket, void *stream) {
+    PICT_CHECK_ARG(a && o && (infos_out || ticket), "NULL argument");
     const pict_domain_desc &d = a->desc;
     PICT_CHECK_ARG(d.C_value && d.P_value && d.A && d.velocity_rhs && d.velocity_result && d.pressure_rhs &&
                        d.pressure_rhs_div && d.pressure_result,
@@ -608,13 +690,26 @@ pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const p
         PICT_CHECK_ARG(tape->C_value && tape->A && tape->P_value && tape->velocity_star &&
                            (o->corrector_steps == 0 || (tape->pressure_rhs && tape->pressure_raw && tape->velocity_result)),
                        "tape buffers missing");
+    if (ticket) return run_async(a, tape ? 1 : 0, o, tape, nullptr, ticket, (hipStream_t)stream);
     return run(a, tape ? 1 : 0, o, tape, nullptr, infos_out, (hipStream_t)stream);
 }
 
+pict_status pict_piso_step(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
+                           pict_solve_info *infos_out, void *stream) {
+    PICT_CHECK_ARG(infos_out, "NULL argument");
+    return step_entry(a, o, tape, infos_out, nullptr, stream);
+}
+
+pict_status pict_piso_step_async(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
+                                 int32_t *ticket, void *stream) {
+    PICT_CHECK_ARG(ticket, "NULL argument");
+    return step_entry(a, o, tape, nullptr, ticket, stream);
+}
+
 // Reverse pass of one pict_piso_step (the adjoint of the orthogonal step).
-pict_status pict_piso_step_vjp(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
-                               void *velocity_grad, pict_solve_info *infos_out, void *stream) {
-    PICT_CHECK_ARG(a && o && tape && velocity_grad && infos_out, "NULL argument");
+static pict_status vjp_entry(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
+                             void *velocity_grad, pict_solve_info *infos_out, int32_t *ticket, void *stream) {
+    PICT_CHECK_ARG(a && o && tape && velocity_grad && (infos_out || ticket), "NULL argument");
     const pict_domain_desc &d = a->desc;
     PICT_CHECK_ARG(d.C_value && d.P_value && d.A && d.velocity_rhs && d.velocity_result && d.pressure_rhs &&
                        d.pressure_rhs_div && d.pressure_result,
@@ -645,7 +740,37 @@ pict_status pict_piso_step_vjp(const pict_atlas *a, const pict_step_opts *o, con
         a->tperm_row = d.csr_row;
         a->tperm_index = d.csr_index;
     }
+    if (ticket) return run_async(a, 2, o, tape, velocity_grad, ticket, s);
     return run(a, 2, o, tape, velocity_grad, infos_out, s);
+}
+
+pict_status pict_piso_step_vjp(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
+                               void *velocity_grad, pict_solve_info *infos_out, void *stream) {
+    PICT_CHECK_ARG(infos_out, "NULL argument");
+    return vjp_entry(a, o, tape, velocity_grad, infos_out, nullptr, stream);
+}
+
+pict_status pict_piso_step_vjp_async(const pict_atlas *a, const pict_step_opts *o, const pict_step_tape *tape,
+                                     void *velocity_grad, int32_t *ticket, void *stream) {
+    PICT_CHECK_ARG(ticket, "NULL argument");
+    return vjp_entry(a, o, tape, velocity_grad, nullptr, ticket, stream);
+}
+
+pict_status pict_step_wait(int32_t ticket, pict_solve_info *infos_out) {
+    PICT_CHECK_ARG(infos_out, "NULL argument");
+    PICT_CHECK_ARG(ticket >= 0 && ticket < kPendingSteps, "no pending asynchronous step %d", ticket);
+    PendingRing &ring = pending_ring();
+    PendingStep &p = ring.slot[ticket];
+    {
+        std::lock_guard<std::mutex> lock(ring.mu);
+        PICT_CHECK_ARG(p.busy, "no pending asynchronous step %d", ticket);
+    }
+    struct Release {
+        PendingRing &r; PendingStep &p;
+        ~Release() { std::lock_guard<std::mutex> lock(r.mu); p.busy = false; }
+    } release{ring, p};
+    PICT_HIP_TRY(hipEventSynchronize(p.ev));
+    return collect(p.host, p.z, &p.o, p.D, infos_out, nullptr);
 }
 
 pict_status pict_subtract_mean(void *v, int32_t n, int32_t dtype, void *stream) {
