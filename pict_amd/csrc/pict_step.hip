@@ -447,9 +447,15 @@ struct GraphCache {
     StepKey miss[3];       // the last key of each kind that had no graph
     bool miss_valid[3] = {};
     hipStream_t capture = nullptr;
+    // device workspace of the directly queued steps of each kind (step slots, reverse-pass vectors and one
+    // workspace per solve), kept between calls: the library pool's per-solve allocations cost host time
+    void *arena[3] = {};
+    size_t arena_bytes[3] = {};
     ~GraphCache() {
         for (GraphEntry &g : e) g.release();
         if (capture) (void)hipStreamDestroy(capture);
+        for (void *p : arena)
+            if (p) (void)hipFree(p);
     }
 };
 
@@ -545,6 +551,43 @@ pict_status graph_for(const pict_atlas *a, int kind, const pict_step_opts *o, co
     return PICT_OK;
 }
 
+// The kept workspace of `kind` laid out for this call: w.dev (z.dev_bytes) and one workspace per solve.  Work
+// queued earlier on the atlas's stream may still use it; it is replaced (after a device wait) only when a call
+// needs more.
+pict_status direct_arena(const pict_atlas *a, int kind, const pict_step_opts *o, const Sizes &z, Work &w) {
+    if (!a->step_graphs) a->step_graphs = new GraphCache();
+    GraphCache &c = *a->step_graphs;
+    const bool vjp = kind == 2;
+    pict_csr_desc C, P;
+    pict_solve_opts adv, prs;
+    solve_systems(a, o, vjp, C, P, adv, prs);
+    size_t wsb[65];
+    pict_status st = solve_workspace_bytes(&C, a->desc.num_dims, &adv, &wsb[0]);
+    if (st) return st;
+    for (int k = 0; k < o->corrector_steps; ++k)
+        if ((st = solve_workspace_bytes(&P, 1, &prs, &wsb[1 + k]))) return st;
+    size_t total = (z.dev_bytes + 255) & ~size_t(255);
+    for (int i = 0; i < z.nsolves; ++i) total += (wsb[i] + 255) & ~size_t(255);
+    if (c.arena_bytes[kind] < total) {
+        if (c.arena[kind]) {
+            PICT_HIP_TRY(hipDeviceSynchronize());
+            PICT_HIP_TRY(hipFree(c.arena[kind]));
+            c.arena[kind] = nullptr;
+            c.arena_bytes[kind] = 0;
+        }
+        PICT_HIP_TRY(hipMalloc(&c.arena[kind], total));
+        c.arena_bytes[kind] = total;
+    }
+    w.dev = (char *)c.arena[kind];
+    char *p = w.dev + ((z.dev_bytes + 255) & ~size_t(255));
+    for (int i = 0; i < z.nsolves; ++i) {
+        w.solve_ws[i] = p;
+        w.solve_ws_bytes[i] = wsb[i];
+        p += (wsb[i] + 255) & ~size_t(255);
+    }
+    return PICT_OK;
+}
+
 // one step or reverse pass: replayed graph, or (timing, graphs off) queued directly
 pict_status run(const pict_atlas *a, int kind, const pict_step_opts *o, const pict_step_tape *tape, void *vgrad,
                 pict_solve_info *infos_out, hipStream_t s) {
@@ -562,12 +605,8 @@ pict_status run(const pict_atlas *a, int kind, const pict_step_opts *o, const pi
             return collect(g->host, g->z, o, a->desc.num_dims, infos_out, nullptr);
         }
     }
-    char *dev = nullptr;
-    PICT_HIP_TRY(ws_alloc((void **)&dev, z.dev_bytes, s));
-    struct Guard {
-        char *p; hipStream_t s;
-        ~Guard() { if (p) (void)ws_free(p, s); }
-    } guard{dev, s};
+    Work w;
+    if ((st = direct_arena(a, kind, o, z, w))) return st;
     HostStates &h = g_host_states;
     if (h.bytes < z.slot * z.nsolves) {
         if (h.p) PICT_HIP_TRY(hipHostFree(h.p));
@@ -575,8 +614,6 @@ pict_status run(const pict_atlas *a, int kind, const pict_step_opts *o, const pi
         PICT_HIP_TRY(hipHostMalloc(&h.p, z.slot * z.nsolves, hipHostMallocDefault));
         h.bytes = z.slot * z.nsolves;
     }
-    Work w;
-    w.dev = dev;
     w.host = h.p;
     w.ev = ev;
     st = vjp ? enqueue_vjp(a, o, tape, vgrad, w, z, s) : enqueue_step(a, o, tape, w, z, s);
@@ -645,14 +682,9 @@ pict_status run_async(const pict_atlas *a, int kind, const pict_step_opts *o, co
         p.bytes = need;
     }
     if (!p.ev) PICT_HIP_TRY(hipEventCreateWithFlags(&p.ev, hipEventDisableTiming));
-    char *dev = nullptr;
-    PICT_HIP_TRY(ws_alloc((void **)&dev, z.dev_bytes, s));
-    struct Guard {
-        char *p; hipStream_t s;
-        ~Guard() { if (p) (void)ws_free(p, s); }
-    } guard{dev, s};
     Work w;
-    w.dev = dev;
+    pict_status sa = direct_arena(a, kind, o, z, w);
+    if (sa) return sa;
     w.host = p.host;
     w.ev = nullptr;
     pict_status st = vjp ? enqueue_vjp(a, o, tape, vgrad, w, z, s) : enqueue_step(a, o, tape, w, z, s);
