@@ -2395,6 +2395,16 @@ const void *bicg_local_kernel(int rpt, int nz, int nrhs) {
     if (nz <= 5) return nrhs == 1 ? bicg_local_kernel_nz<5, 1>(rpt) : nrhs == 2 ? bicg_local_kernel_nz<5, 2>(rpt) : nullptr;
     return nrhs == 1 ? bicg_local_kernel_nz<7, 1>(rpt) : nullptr;
 }
+// Solver grids of at most one workgroup per compute unit (256 workgroups; every solver kernel has an occupancy
+// of at least one) are launched as ordinary kernels: on a device that runs nothing else they are all resident at
+// once, and the cooperative launch costs host time that leaves the GPU idle between a solve and the next kernel
+// (C3 forward+backward: 18.3 M against 16.8 M cells*steps/s).  If other work holds the compute units, waiting
+// workgroups start when it ends; a barrier that waits longer than its spin limit ends the solve with an error.
+// PICT_SOLVER_PLAIN_LAUNCH=0 keeps the cooperative launch.
+static bool plain_launch_enabled() {
+    const char *e = getenv("PICT_SOLVER_PLAIN_LAUNCH");
+    return !e || atoi(e) != 0;
+}
 // PICT_BICG_LOCAL=0 keeps BiCGStab on the streaming kernel
 static bool bicg_local_enabled() {
     const char *e = getenv("PICT_BICG_LOCAL");
@@ -2646,7 +2656,10 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     }
     void *args[] = {&a};
     if (ev0) PICT_HIP_TRY(hipEventRecord(ev0, stream));
-    PICT_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3((unsigned)nwg), dim3(kSolveThreads), args, 0, stream));
+    if (plain_launch_enabled() && nwg <= 256)
+        PICT_HIP_TRY(hipLaunchKernel(kernel, dim3((unsigned)nwg), dim3(kSolveThreads), args, 0, stream));
+    else
+        PICT_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3((unsigned)nwg), dim3(kSolveThreads), args, 0, stream));
     if (ev1) PICT_HIP_TRY(hipEventRecord(ev1, stream));
     PICT_HIP_TRY(hipMemcpyAsync(state_out, a.st, sizeof(RhsState) * nrhs, hipMemcpyDeviceToDevice, stream));
     PICT_HIP_TRY(hipMemcpyAsync(err_out, &a.ctl->err, sizeof(int), hipMemcpyDeviceToDevice, stream));
