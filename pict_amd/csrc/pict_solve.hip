@@ -2026,6 +2026,13 @@ __global__ __launch_bounds__(256) void k_refine_update(double *x, const R *d, do
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) x[i] += scale * (double)d[i];
 }
 
+// a solve's final states and barrier error flag to the caller's slots (one launch after the solver kernel)
+__global__ void k_solve_out(const RhsState *st, int nrhs, const int *err, RhsState *st_out, int *err_out) {
+    const int k = threadIdx.x;
+    if (k < nrhs) st_out[k] = st[k];
+    if (k == 0) *err_out = *err;
+}
+
 // zero n 64-bit words (the tagged regions of k_cg_local, before its launch)
 __global__ void k_zero64(u64 *p, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
@@ -2661,8 +2668,9 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     else
         PICT_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3((unsigned)nwg), dim3(kSolveThreads), args, 0, stream));
     if (ev1) PICT_HIP_TRY(hipEventRecord(ev1, stream));
-    PICT_HIP_TRY(hipMemcpyAsync(state_out, a.st, sizeof(RhsState) * nrhs, hipMemcpyDeviceToDevice, stream));
-    PICT_HIP_TRY(hipMemcpyAsync(err_out, &a.ctl->err, sizeof(int), hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_solve_out, dim3(1), dim3(64), 0, stream, (const RhsState *)a.st, nrhs,
+                       (const int *)&a.ctl->err, state_out, err_out);
+    PICT_HIP_TRY(hipGetLastError());
     return PICT_OK;
 }
 
