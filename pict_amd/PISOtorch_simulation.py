@@ -711,7 +711,7 @@ class Simulation:
         t = ctypes.c_int32(-1)
         _abi.check(fn(*args, ctypes.byref(t), PISOtorch._stream(self.domain.getDevice())),
                    "PISO step reverse pass" if adjoint else "PISO step")
-        self._pending_checks.append((t.value, o, adjoint))
+        self._pending_checks.append((t.value, o, adjoint, self.domain.getSpatialDims()))
 
     def flush_solver_checks(self):
         """Wait for the native calls queued with defer_solver_checks and check their solver results in order,
@@ -721,9 +721,8 @@ class Simulation:
         from . import _abi
         lib = _abi.load_library()
         pending, self._pending_checks = self._pending_checks, []
-        D = self.domain.getSpatialDims()
         err = None
-        for ticket, o, adjoint in pending:
+        for ticket, o, adjoint, D in pending:
             infos = (_abi.SolveInfo * (D + o.corrector_steps))()
             st = lib.pict_step_wait(ticket, infos)
             if err is not None:
