@@ -29,37 +29,41 @@ def lib(bits=64):
         path = os.path.join(BUILD, "liboracle_f%d.so" % bits)
         if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(path)
-        L.oracle_last_error.restype = ctypes.c_char_p
-        for n in ("oracle_setup_advection_matrix",):
-            getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
-        L.oracle_setup_advection_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
-        L.oracle_setup_pressure_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.oracle_setup_pressure_rhs_only.argtypes = [ctypes.c_void_p, ctypes.c_double]
-        L.oracle_setup_pressure_rhs_div.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
-        L.oracle_correct_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
-        L.oracle_compute_velocity_divergence.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        L.oracle_copy_velocity_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.oracle_copy_pressure_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.oracle_build_csr_pattern.argtypes = [ctypes.c_void_p]
-        L.oracle_csr_row_end_closed_form.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
-                                                     ctypes.POINTER(ctypes.c_int)]
-        L.oracle_solve_linear.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
-                                          ctypes.c_void_p, ctypes.c_void_p]
-        L.oracle_sparse_outer_product.argtypes = [ctypes.c_void_p] * 4
-        L.oracle_coords_to_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
-        L.oracle_coords_to_face_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
-                                                       ctypes.c_void_p]
-        L.oracle_set_threads.argtypes = [ctypes.c_int]
-        for n in ("oracle_setup_advection_matrix_grad", "oracle_setup_advection_velocity_grad",
-                  "oracle_setup_pressure_rhs_div_grad", "oracle_correct_velocity_grad"):
-            getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
-        L.oracle_setup_pressure_matrix_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.oracle_setup_pressure_rhs_only_grad.argtypes = [ctypes.c_void_p, ctypes.c_double]
-        L.oracle_copy_velocity_result_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.oracle_copy_pressure_result_grad_from_blocks.argtypes = [ctypes.c_void_p]
-        _libs[bits] = L
+        _libs[bits] = declare(ctypes.CDLL(path))
     return _libs[bits]
+
+
+def declare(L):
+    """Declares the argument types of the oracle's entry points on a loaded library; returns it."""
+    L.oracle_last_error.restype = ctypes.c_char_p
+    for n in ("oracle_setup_advection_matrix",):
+        getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.oracle_setup_advection_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.oracle_setup_pressure_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_setup_pressure_rhs_only.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    L.oracle_setup_pressure_rhs_div.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.oracle_correct_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.oracle_compute_velocity_divergence.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.oracle_copy_velocity_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_copy_pressure_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_build_csr_pattern.argtypes = [ctypes.c_void_p]
+    L.oracle_csr_row_end_closed_form.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_int)]
+    L.oracle_solve_linear.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_void_p, ctypes.c_void_p]
+    L.oracle_sparse_outer_product.argtypes = [ctypes.c_void_p] * 4
+    L.oracle_coords_to_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    L.oracle_coords_to_face_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                   ctypes.c_void_p]
+    L.oracle_set_threads.argtypes = [ctypes.c_int]
+    for n in ("oracle_setup_advection_matrix_grad", "oracle_setup_advection_velocity_grad",
+              "oracle_setup_pressure_rhs_div_grad", "oracle_correct_velocity_grad"):
+        getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.oracle_setup_pressure_matrix_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_setup_pressure_rhs_only_grad.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    L.oracle_copy_velocity_result_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_copy_pressure_result_grad_from_blocks.argtypes = [ctypes.c_void_p]
+    return L
 
 
 def _bits(domain):

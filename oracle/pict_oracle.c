@@ -11,6 +11,15 @@
  *   liboracle_f32.so  (ORACLE_REAL=float) : the FP32 CPU baseline ("port")
  * Every function names the reference lines it restates
  * (tum-pbs/PICT, extensions/PISO_multiblock_cuda_kernel.cu unless stated).
+ *
+ * Pinned (the reference ships no golden vectors) by checks that do not restate
+ * these formulas: closed forms and brute force for the CSR pattern; scipy for
+ * the solvers; finite differences for every VJP; for the pressure projection
+ * the divergence of linear fields, the continuum limit of the pressure matrix,
+ * the exact divergence left by one projection of a Fourier mode, the
+ * reference's step invariants and the Re 100 cavity against Ghia et al.
+ * (tests/test_oracle*.py; tests/test_oracle_mutations.py plants mistakes in
+ * copies of this file and shows the pins catch them).
  */
 #include <math.h>
 #include <stdint.h>
