@@ -42,6 +42,8 @@ def declare(L):
     L.oracle_setup_pressure_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_setup_pressure_rhs_only.argtypes = [ctypes.c_void_p, ctypes.c_double]
     L.oracle_setup_pressure_rhs_div.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.oracle_pressure_rhs_add_non_ortho.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_pressure_rhs_add_non_ortho_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_correct_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
     L.oracle_compute_velocity_divergence.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.oracle_copy_velocity_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -159,8 +161,18 @@ def SetupPressureMatrix(domain, timeStep, nonOrthoMode=0, useFaceTransform=False
 
 
 def SetupPressureRHSdiv(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    """Divergence of the pressure RHS, then its explicit non-orthogonal terms (PISO_multiblock_cuda_kernel.cu
+    :5659-5704)."""
     L = lib(_bits(domain))
     _chk(L, L.oracle_setup_pressure_rhs_div(_desc(domain), _dt(timeStep), int(timeStepNorm)), "pressure div")
+    _chk(L, L.oracle_pressure_rhs_add_non_ortho(_desc(domain), int(nonOrthoMode)), "pressure non-ortho rhs")
+
+
+def PressureRHSaddNonOrtho(domain, nonOrthoMode):
+    """pressureRHSdiv += the explicit non-orthogonal terms of the pressure Laplacian (k_pressureRHSaddNonOrthoComponents,
+    PISO_multiblock_cuda_kernel.cu:5455-5477)."""
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_pressure_rhs_add_non_ortho(_desc(domain), int(nonOrthoMode)), "pressure non-ortho rhs")
 
 
 def SetupPressureRHS(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
@@ -225,6 +237,7 @@ def SetupPressureMatrixGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=F
 
 def SetupPressureRHSdivGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
     L = lib(_bits(domain))
+    _chk(L, L.oracle_pressure_rhs_add_non_ortho_grad(_desc(domain), int(nonOrthoMode)), "non-ortho rhs grad")
     _chk(L, L.oracle_setup_pressure_rhs_div_grad(_desc(domain), _dt(timeStep), int(timeStepNorm)), "div grad")
 
 

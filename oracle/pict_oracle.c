@@ -340,18 +340,388 @@ int oracle_build_csr_pattern(const pict_domain_desc *d) {
 /* forward stages                                                       */
 /* ------------------------------------------------------------------ */
 
-static int check_ortho_flags(int flags) {
-    if (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX | PICT_NON_ORTHO_DIRECT_RHS | PICT_NON_ORTHO_DIAGONAL_RHS)) {
-        snprintf(g_err, sizeof g_err, "oracle: non-orthogonal terms not restated");
-        return PICT_ERR_UNSUPPORTED;
-    }
-    return PICT_OK;
+
+/* ------------------------------------------------------------------ */
+/* non-orthogonal terms (flags of PISO_multiblock_cuda.h:9-13)          */
+/* ------------------------------------------------------------------ */
+
+/* Data a corner or neighbour value is read from: the GridDataType values the reference's
+   stages pass (domain_structs_gpu.h:28-65).  Gradient variants scatter into the matching
+   gradient tensors. */
+enum { GD_VELOCITY = 0, GD_VELOCITY_RESULT = 1, GD_PRESSURE_RESULT = 2, GD_IS_FIXED = 3 };
+#define GD_GRAD 8
+#define GD_BASE(t) ((t) & 7)
+static inline int gd_is_pressure(int t) { return GD_BASE(t) == GD_PRESSURE_RESULT; }
+
+static inline int face_sign(int face) { return (face & 1) ? 1 : -1; }
+static inline int i4_eq(const I4 a, const I4 b) {
+    return a.a[0] == b.a[0] && a.a[1] == b.a[1] && a.a[2] == b.a[2] && a.a[3] == b.a[3];
 }
 
-/* PISO_build_matrix :3601-3865 (velocity, orthogonal terms) */
+/* getLaplaceCoefficientCenterIndex :1253-1255 */
+static inline int lap_ci(int c1, int c2) { return c1 + c2 + ((c1 >> 1) | (c2 >> 1)); }
+
+/* dot of Minv rows r1, r2 (transformations.h dot: ascending sum from 0) */
+static inline real minv_dot(const Ctx *c, const real *T, int r1, int r2) {
+    real s = 0;
+    for (int j = 0; j < c->D; ++j) s += T_MINV(c, T, r1, j) * T_MINV(c, T, r2, j);
+    return s;
+}
+
+/* getLaplaceCoefficientsFull :1261-1287: out[lap_ci(c1,c2)] = det Minv_c1 . Minv_c2 */
+static void lap_full(const Ctx *c, const pict_block_desc *b, I4 p, real *out) {
+    for (int c1 = 0; c1 < c->D; ++c1)
+        for (int c2 = 0; c2 <= c1; ++c2) {
+            if (b->has_transform) {
+                const real *T = cell_T(c, b, p);
+                out[lap_ci(c1, c2)] = T_DET(c, T) * minv_dot(c, T, c1, c2);
+            } else {
+                out[lap_ci(c1, c2)] = c1 == c2 ? 1 : 0;
+            }
+        }
+}
+/* getLaplaceCoefficientsSingleAxis :1289-1313 */
+static void lap_single_axis(const Ctx *c, const pict_block_desc *b, I4 p, int axis, real *out) {
+    for (int c1 = 0; c1 < c->D; ++c1) {
+        if (b->has_transform) {
+            const real *T = cell_T(c, b, p);
+            out[c1] = T_DET(c, T) * minv_dot(c, T, c1, axis);
+        } else {
+            out[c1] = c1 == axis ? 1 : 0;
+        }
+    }
+}
+/* getLaplaceCoefficientsNeighbourBoundary :1468-1490 (FixedBoundary) */
+static void lap_neighbour_boundary(const Ctx *c, const pict_boundary_desc *fb, I4 p, int axis, real *out) {
+    for (int c1 = 0; c1 < c->D; ++c1) {
+        if (fb->has_transform) {
+            I4 q = p; q.a[3] = 0; q.a[axis] = 0;
+            const real *T = bound_T(c, fb, q);
+            out[c1] = T_DET(c, T) * minv_dot(c, T, c1, axis);
+        } else {
+            out[c1] = c1 == axis ? 1 : 0;
+        }
+    }
+}
+
+/* computeConnectedDir :314-323 */
+static inline int connected_dir(const Ctx *c, int dir, int bdim, const pict_boundary_desc *cb) {
+    int rel = posmod((dir >> 1) - bdim, c->D);
+    return cb->axes[rel] ^ (dir & 1);
+}
+
+/* A resolved neighbour cell (resolveNeighborCell :431-476) with the axis mapping of the connection. */
+typedef struct { int bi; I4 pos; int map[3]; } NCell;
+static NCell resolve_neighbor(const Ctx *c, int bi, I4 pos, int dir) {
+    const pict_block_desc *b = blk(c, bi);
+    NCell n; n.bi = bi; n.pos = pos; n.map[0] = 0; n.map[1] = 2; n.map[2] = 4;
+    int axis = dir >> 1;
+    if (at_bound(pos, dir, b)) {
+        const pict_boundary_desc *bd = &b->bounds[dir];
+        if (bd->type == PICT_BOUNDARY_CONNECTED) {
+            n.bi = bd->connected_block;
+            n.pos = connected_pos(c, pos, axis, bd, 0);
+            for (int dd = 0; dd < c->D; ++dd) n.map[dd] = connected_dir(c, n.map[dd], axis, bd);
+        } else if (bd->type == PICT_BOUNDARY_PERIODIC) {
+            n.pos.a[axis] = (dir & 1) ? 0 : b->size[axis] - 1;
+        }
+    } else {
+        n.pos.a[axis] += face_sign(dir);
+    }
+    return n;
+}
+/* getLaplaceCoefficientsSingleAxisNeighbor(NeighborCellInfo) :1415-1452 (block cells only) */
+static void lap_single_axis_neighbor(const Ctx *c, const NCell *n, int dir, real *out) {
+    int axis = dir >> 1;
+    const pict_block_desc *nb = blk(c, n->bi);
+    for (int c1 = 0; c1 < c->D; ++c1) out[c1] = 0;
+    if (nb->has_transform) {
+        I4 q = n->pos; q.a[3] = 0;
+        const real *T = cell_T(c, nb, q);
+        int am = n->map[axis] >> 1;
+        for (int c1 = 0; c1 < c->D; ++c1) out[c1] = T_DET(c, T) * minv_dot(c, T, n->map[c1] >> 1, am);
+    } else {
+        out[axis] = 1;
+    }
+}
+
+/* getBlockData :2320-2331 through getBlockCellDataPtr :2305-2317 */
+static real block_data(const Ctx *c, int bi, I4 p, int type) {
+    const pict_block_desc *b = blk(c, bi);
+    switch (GD_BASE(type)) {
+        case GD_VELOCITY: return ((const real *)b->velocity)[flat4(p, b->stride)];
+        case GD_VELOCITY_RESULT: return ((const real *)c->d->velocity_result)[(size_t)p.a[3] * c->N + global_index(p, b)];
+        case GD_PRESSURE_RESULT: return ((const real *)c->d->pressure_result)[(size_t)p.a[3] * c->N + global_index(p, b)];
+        default: return 0;
+    }
+}
+/* getFixedBoundaryData :2376-2446 (FIXED; the pressure of a FixedBoundary is a static zero) */
+static real fixed_data(const Ctx *c, const pict_block_desc *b, I4 p, int face, int type) {
+    if (GD_BASE(type) == GD_IS_FIXED) return 1;
+    if (type & GD_GRAD) return 0;
+    const pict_boundary_desc *fb = &b->bounds[face];
+    if (fb->type != PICT_BOUNDARY_FIXED) return 0;
+    if (GD_BASE(type) == GD_PRESSURE_RESULT) return 0;
+    if (fb->velocity_static) return ((const real *)fb->velocity)[p.a[3]];
+    p.a[face >> 1] = 0;
+    return ((const real *)fb->velocity)[flat4(p, fb->stride)];
+}
+/* getFixedBoundaryType :2351-2374 (velocity: the boundary's type; pressure: Neumann) */
+static int fixed_type(const pict_block_desc *b, int face, int type) {
+    const pict_boundary_desc *fb = &b->bounds[face];
+    if (fb->type != PICT_BOUNDARY_FIXED) return PICT_BC_DIRICHLET;
+    switch (GD_BASE(type)) {
+        case GD_VELOCITY:
+        case GD_VELOCITY_RESULT: return fb->velocity_bc;
+        case GD_PRESSURE_RESULT: return PICT_BC_NEUMANN;
+        default: return PICT_BC_DIRICHLET;
+    }
+}
+
+/* getBlockDataNeighbor :2518-2557 (zeroBound false; across a connection it reads the
+   connected block at border offset 1, as the reference does) */
+static real data_neighbor(const Ctx *c, int bi, I4 pos, int dir, int type) {
+    const pict_block_desc *b = blk(c, bi);
+    int dim = dir >> 1;
+    I4 t = pos;
+    int tb = bi;
+    if (at_bound(pos, dir, b)) {
+        const pict_boundary_desc *bd = &b->bounds[dir];
+        switch (bd->type) {
+            case PICT_BOUNDARY_FIXED: return fixed_data(c, b, t, dir, type);
+            case PICT_BOUNDARY_CONNECTED: t = connected_pos(c, t, dim, bd, 1); tb = bd->connected_block; break;
+            case PICT_BOUNDARY_PERIODIC: t.a[dim] = (dir & 1) ? 0 : b->size[dim] - 1; break;
+            default: return 0;
+        }
+    } else {
+        t.a[dim] += face_sign(dir);
+    }
+    return block_data(c, tb, t, type);
+}
+/* getBlockDataNeighborDiagonal :2614-2656: two steps, the prescribed direction last */
+static real data_neighbor_diag(const Ctx *c, int bi, I4 pos, int dir1, int dir2, int type) {
+    const int d1e = prescribed(dir1, blk(c, bi));
+    const int dirs[2] = {d1e ? dir2 : dir1, d1e ? dir1 : dir2};
+    int tb = bi;
+    I4 t = pos;
+    for (int i = 0; i < 2; ++i) {
+        const pict_block_desc *b = blk(c, tb);
+        int bound = dirs[i], dim = bound >> 1;
+        if (at_bound(t, bound, b)) {
+            const pict_boundary_desc *bd = &b->bounds[bound];
+            switch (bd->type) {
+                case PICT_BOUNDARY_FIXED: return fixed_data(c, b, t, bound, type);
+                case PICT_BOUNDARY_CONNECTED: t = connected_pos(c, t, dim, bd, 1); tb = bd->connected_block; break;
+                case PICT_BOUNDARY_PERIODIC: t.a[dim] = (bound & 1) ? 0 : b->size[dim] - 1; break;
+                default: return 0;
+            }
+        } else {
+            t.a[dim] += face_sign(bound);
+        }
+    }
+    return block_data(c, tb, t, type);
+}
+
+/* getCornerValue :2742-2859: the mean of the cells around the corner of `pos` between faces dir1
+   and dir2, walking around it in both directions (to depth maxDepth), through connections (border
+   offset 1, as the reference) and periodic wraps; a FIXED boundary ends the walk with the boundary
+   value (num = 0) and its condition type. */
+typedef struct { real data; int num; int btype; } Corner;
+typedef struct { int d1, d2, bi; I4 pos; } Walker;
+static Corner corner_value(const Ctx *c, int bi, I4 pos, int dir1, int dir2, int inc0, int inc1, int max_depth,
+                           int type) {
+    real data = 0;
+    int num = 0, btype = PICT_BC_DIRICHLET;
+    if (inc0) data += block_data(c, bi, pos, type);
+    ++num;
+    Walker w[2] = {{dir1, dir2, bi, pos}, {dir2, dir1, bi, pos}};
+    for (int depth = 1; depth <= max_depth; ++depth) {
+        for (int d = 0; d < 2; ++d) {
+            const pict_block_desc *wb = blk(c, w[d].bi);
+            const int axis = w[d].d1 >> 1, fs = face_sign(w[d].d1);
+            if (at_bound(w[d].pos, w[d].d1, wb)) {
+                const pict_boundary_desc *bd = &wb->bounds[w[d].d1];
+                if (bd->type == PICT_BOUNDARY_FIXED) {
+                    const int at2 = at_bound(w[d].pos, w[d].d2, wb);
+                    btype = fixed_type(wb, w[d].d1, type);
+                    Corner r;
+                    r.num = 0; r.btype = btype;
+                    if (at2) {
+                        r.data = fixed_data(c, wb, w[d].pos, w[d].d1, type);
+                    } else {
+                        I4 np = w[d].pos;
+                        np.a[w[d].d2 >> 1] += face_sign(w[d].d2);
+                        r.data = (fixed_data(c, wb, w[d].pos, w[d].d1, type) + fixed_data(c, wb, np, w[d].d1, type)) *
+                                 (real)0.5;
+                    }
+                    return r;
+                } else if (bd->type == PICT_BOUNDARY_CONNECTED) {
+                    w[d].bi = bd->connected_block;
+                    w[d].pos.a[axis] += fs;
+                    w[d].pos = connected_pos(c, w[d].pos, axis, bd, 1);
+                    const int o1 = w[d].d1;
+                    w[d].d1 = connected_dir(c, w[d].d2, axis, bd);
+                    w[d].d2 = connected_dir(c, o1, axis, bd) ^ 1;
+                } else if (bd->type == PICT_BOUNDARY_PERIODIC) {
+                    w[d].pos.a[axis] = (w[d].d1 & 1) ? 0 : wb->size[axis] - 1;
+                    const int o1 = w[d].d1;
+                    w[d].d1 = w[d].d2;
+                    w[d].d2 = o1 ^ 1;
+                }
+            } else {
+                w[d].pos.a[axis] += fs;
+                const int o1 = w[d].d1;
+                w[d].d1 = w[d].d2;
+                w[d].d2 = o1 ^ 1;
+            }
+            if (w[d].bi == w[d ^ 1].bi && i4_eq(w[d].pos, w[d ^ 1].pos)) goto done;
+            if (depth > 1 || inc1) data += block_data(c, w[d].bi, w[d].pos, type);
+            ++num;
+        }
+    }
+done:;
+    Corner r;
+    r.data = data / (real)num; r.num = num; r.btype = btype;
+    return r;
+}
+
+/* interpolateNonOrthoLaplaceComponents :1911-1986 (cell metrics): out[face*(D-1) + i-1] is the
+   face value of alpha_{axis, (axis+i)%D} (times 1/A and the viscosity when asked).  Faces of a
+   cell at either end of an axis whose face on that axis is prescribed are left 0, as in the
+   reference's condition (0 < x < size-1 || face not prescribed). */
+static void interp_nonortho(const Ctx *c, int bi, I4 pos, int with_visc, int with_a, real *out) {
+    const pict_block_desc *b = blk(c, bi);
+    const real *A = (const real *)c->d->A;
+    real aP[6];
+    lap_full(c, b, pos, aP);
+    const real one = 1;
+    const real raP = (with_a ? one / A[global_index(pos, b)] : one) * (with_visc ? viscosity(c, b, pos) : one);
+    for (int face = 0; face < 2 * c->D; ++face) {
+        const int axis = face >> 1, up = face & 1;
+        if ((0 < pos.a[axis] && pos.a[axis] < b->size[axis] - 1) || !prescribed(face, b)) {
+            I4 pn = pos; pn.a[3] = axis;
+            int nbi = bi;
+            real aN[3];
+            if (at_bound(pos, face, b) && b->bounds[face].type == PICT_BOUNDARY_CONNECTED) {
+                nbi = b->bounds[face].connected_block;
+                pn = connected_pos_ch(c, pn, axis, &b->bounds[face]);
+                lap_single_axis(c, blk(c, nbi), pn, axis, aN);
+            } else {
+                if (!up && pos.a[axis] == 0 && b->bounds[face].type == PICT_BOUNDARY_PERIODIC) pn.a[axis] = b->size[axis] - 1;
+                else if (up && pos.a[axis] == b->size[axis] - 1 && b->bounds[face].type == PICT_BOUNDARY_PERIODIC) pn.a[axis] = 0;
+                else pn.a[axis] = pos.a[axis] + face_sign(face);
+                lap_single_axis(c, b, pn, axis, aN);
+            }
+            pn.a[3] = 0;
+            const pict_block_desc *nb = blk(c, nbi);
+            const real raN = (with_a ? one / A[global_index(pn, nb)] : one) * (with_visc ? viscosity(c, nb, pn) : one);
+            for (int i = 1; i < c->D; ++i) {
+                const int oa = (axis + i) % c->D;
+                out[face * (c->D - 1) + i - 1] = (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN) * (real)0.5;
+            }
+        } else {
+            for (int i = 1; i < c->D; ++i) out[face * (c->D - 1) + i - 1] = 0;
+        }
+    }
+}
+
+/* Non-orthogonal matrix entries of one face of a row (PISO_build_matrix :3738-3800 with sign -1,
+   PISO_build_pressure_matrix :4877-4932 with sign +1): the corners of `face` with its tangential
+   faces, weighted by the interpolated cross coefficient and the corner's cell count.  `rv` holds
+   the row in slot order (0 = diagonal, face+1 = neighbour); corner_type is the GridDataType of
+   the corner walk; grad_always: every boundary corner is a gradient (Neumann) corner. */
+static void nonortho_matrix_face(const Ctx *c, int bi, I4 pos, int face, const real *ai, int flags, real sign,
+                                 int corner_type, int grad_always, real *rv) {
+    const int D = c->D, dim = face >> 1, fs = face_sign(face);
+    const int nbrs = flags & PICT_NON_ORTHO_DIRECT_MATRIX, ctr = flags & PICT_NON_ORTHO_CENTER_MATRIX;
+    for (int i = 1; i < D; ++i) {
+        const int ta = (dim + i) % D;
+        const real alpha = ai[face * (D - 1) + i - 1];
+        if (alpha == 0) continue;
+        for (int tu = 0; tu < 2; ++tu) {
+            const int tf = (ta << 1) | tu, tfs = face_sign(tf);
+            const Corner cv = corner_value(c, bi, pos, face, tf, 0, 0, 2, corner_type);
+            if (cv.num < 1) {
+                if (grad_always || cv.btype == PICT_BC_NEUMANN) {
+                    const real v = (real)(fs * tfs) * alpha * (real)0.25;
+                    if (ctr) rv[0] += sign * 3 * v;
+                    if (nbrs) { rv[face + 1] += sign * 3 * v; rv[(tf ^ 1) + 1] -= sign * v; }
+                }
+            } else {
+                const real norm = (real)(1.0 / (double)cv.num);
+                const real v = (real)(fs * tfs) * alpha * norm;
+                if (ctr) rv[0] += sign * v;
+                if (nbrs) { rv[face + 1] += sign * v; rv[tf + 1] += sign * v; }
+            }
+        }
+    }
+}
+
+/* getNonOrthoLaplaceRHS_v2 :3033-3190 (no face transform): the explicit non-orthogonal part of the
+   Laplacian of `type` at cell `pos` (w = component), from the previous iterate. */
+static real nonortho_rhs(const Ctx *c, int bi, I4 pos, int flags, int type, int with_visc, int with_a) {
+    if (!(flags & (PICT_NON_ORTHO_DIRECT_RHS | PICT_NON_ORTHO_DIAGONAL_RHS))) return 0;
+    const pict_block_desc *b = blk(c, bi);
+    const real *A = (const real *)c->d->A;
+    const int D = c->D;
+    real S = 0;
+    real aP[6];
+    lap_full(c, b, pos, aP);
+    const real one = 1;
+    const real raP = (with_a ? one / A[global_index(pos, b)] : one) * (with_visc ? viscosity(c, b, pos) : one);
+    for (int face = 0; face < 2 * D; ++face) {
+        const int axis = face >> 1, fs = face_sign(face);
+        if (at_bound(pos, face, b) && prescribed(face, b)) {
+            if (fixed_type(b, face, type) == PICT_BC_NEUMANN || gd_is_pressure(type)) continue;
+            real bA[3];
+            lap_neighbour_boundary(c, &b->bounds[face], pos, axis, bA);
+            for (int i = 1; i < D; ++i) {
+                const int ta = (axis + i) % D;
+                const int tlo = pos.a[ta] == 0, thi = pos.a[ta] == b->size[ta] - 1;
+                I4 lo = pos, hi = pos;
+                real df = (real)0.5;
+                if (!tlo) lo.a[ta] -= 1;
+                if (!thi) hi.a[ta] += 1;
+                if (tlo || thi) df = 1;
+                const real tdg = df * (fixed_data(c, b, hi, face, type) - fixed_data(c, b, lo, face, type));
+                const real bv = with_visc ? viscosity(c, b, pos) : one;
+                S -= (real)fs * bA[ta] * tdg * bv;
+            }
+            continue;
+        }
+        const NCell n = resolve_neighbor(c, bi, pos, face);
+        real aN[3];
+        lap_single_axis_neighbor(c, &n, face, aN);
+        const pict_block_desc *nb = blk(c, n.bi);
+        const real raN = (with_a ? one / A[global_index(n.pos, nb)] : one) * (with_visc ? viscosity(c, nb, n.pos) : one);
+        for (int i = 1; i < D; ++i) {
+            const int ta = (axis + i) % D;
+            const real fa = (aP[lap_ci(axis, ta)] * raP + aN[ta] * raN) * (real)0.5;
+            real tdg = 0;
+            for (int tu = 0; tu < 2; ++tu) {
+                const int tf = (ta << 1) | tu, tfs = face_sign(tf);
+                const Corner cv = corner_value(c, bi, pos, face, tf, 0, (flags & PICT_NON_ORTHO_DIRECT_RHS) != 0, 2, type);
+                if (cv.num == 0 && (cv.btype == PICT_BC_NEUMANN || gd_is_pressure(type))) {
+                    const int to = tf ^ 1, tofs = -tfs;
+                    if (flags & PICT_NON_ORTHO_DIRECT_RHS) {
+                        tdg += (real)tfs * data_neighbor(c, bi, pos, face, type) * (real)0.75;
+                        tdg += (real)tofs * data_neighbor(c, bi, pos, to, type) * (real)0.25;
+                    }
+                    if (flags & PICT_NON_ORTHO_DIAGONAL_RHS)
+                        tdg += (real)tofs * data_neighbor_diag(c, bi, pos, face, to, type) * (real)0.25;
+                } else {
+                    tdg += (real)tfs * cv.data;
+                }
+            }
+            S -= (real)fs * fa * tdg;
+        }
+    }
+    return S;
+}
+
+/* PISO_build_matrix :3601-3865 (velocity) */
 int oracle_setup_advection_matrix(const pict_domain_desc *d, double time_step, int flags) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
-    if ((st = check_ortho_flags(flags))) return st;
     const real dt = (real)time_step;
     real *Cv = (real *)d->C_value, *A = (real *)d->A;
     for (int bi = 0; bi < d->num_blocks; ++bi) {
@@ -383,6 +753,13 @@ int oracle_setup_advection_matrix(const pict_domain_desc *d, double time_step, i
                     idx[face + 1] = global_index(t, blk(&c, nb));
                     real vc = (alpha_p[dim] * visc_p + alpha * visc_n) * (real)0.5;
                     diag += vc; rv[face + 1] -= vc;
+                    if (c.D > 1 && (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX))) {
+                        real ai[12];
+                        interp_nonortho(&c, bi, pos, 1, 0, ai);
+                        rv[0] = diag;
+                        nonortho_matrix_face(&c, bi, pos, face, ai, flags, (real)-1, GD_VELOCITY, 0, rv);
+                        diag = rv[0];
+                    }
                 } else {
                     int slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
                     diag += (real)((1 - slip) * 2) * visc_p * alpha_p[dim];
@@ -482,10 +859,9 @@ static void pressure_gradient(const Ctx *c, const pict_block_desc *b, I4 pos, re
     }
 }
 
-/* kPISO_build_advection_RHS :4281-4385 (orthogonal) */
+/* kPISO_build_advection_RHS :4281-4385 */
 int oracle_setup_advection_velocity(const pict_domain_desc *d, double time_step, int flags, int apply_pressure_gradient) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
-    if ((st = check_ortho_flags(flags))) return st;
     const real dt = (real)time_step;
     real *rhs = (real *)d->velocity_rhs;
     for (int bi = 0; bi < d->num_blocks; ++bi) {
@@ -500,6 +876,7 @@ int oracle_setup_advection_velocity(const pict_domain_desc *d, double time_step,
                 I4 t = pos; t.a[3] = dd;
                 real r = det * ((const real *)b->velocity)[flat4(t, b->stride)] / dt;
                 r += fixed_boundary_source(&c, b, pos, dd);
+                r -= nonortho_rhs(&c, bi, t, flags, GD_VELOCITY_RESULT, 1, 0);
                 r /= det;
                 r += velocity_source(b, t);
                 if (apply_pressure_gradient) r -= pg[dd] * dt;
@@ -510,10 +887,9 @@ int oracle_setup_advection_velocity(const pict_domain_desc *d, double time_step,
     return PICT_OK;
 }
 
-/* PISO_build_pressure_matrix :4797-4963 (orthogonal, no face transform) */
+/* PISO_build_pressure_matrix :4797-4963 (no face transform) */
 int oracle_setup_pressure_matrix(const pict_domain_desc *d, int flags) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
-    if ((st = check_ortho_flags(flags))) return st;
     real *Pv = (real *)d->P_value; const real *A = (const real *)d->A;
     for (int bi = 0; bi < d->num_blocks; ++bi) {
         const pict_block_desc *b = blk(&c, bi);
@@ -537,6 +913,11 @@ int oracle_setup_pressure_matrix(const pict_domain_desc *d, int flags) {
                 real raN = (real)1.0 / A[gn];
                 real coef = (real)0.5 * (alphaP[dim] * raP + alphaN * raN);
                 rv[0] -= coef; rv[face + 1] += coef; idx[face + 1] = gn;
+                if (c.D > 1 && (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX))) {
+                    real ai[12];
+                    interp_nonortho(&c, bi, pos, 0, 1, ai);
+                    nonortho_matrix_face(&c, bi, pos, face, ai, flags, (real)1, GD_IS_FIXED, 1, rv);
+                }
             }
             int start = d->csr_row[g];
             int n = 0; for (int s = 0; s < 7; ++s) n += idx[s] >= 0;
@@ -593,6 +974,24 @@ int oracle_setup_pressure_rhs_div(const pict_domain_desc *d, double time_step, i
             for (int dd = 0; dd < c.D; ++dd) div += fl[2 * dd + 1] - fl[2 * dd];
             if (time_step_norm) div /= dt;
             ((real *)d->pressure_rhs_div)[f + b->global_offset] = div;
+        }
+    }
+    return PICT_OK;
+}
+
+/* k_pressureRHSaddNonOrthoComponents :5455-5477: pressure_rhs_div += the explicit non-orthogonal
+   part of the pressure Laplacian from the previous pressure result */
+int oracle_pressure_rhs_add_non_ortho(const pict_domain_desc *d, int flags) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    if (!(flags & (PICT_NON_ORTHO_DIRECT_RHS | PICT_NON_ORTHO_DIAGONAL_RHS))) return PICT_OK;
+    real *div = (real *)d->pressure_rhs_div;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        #pragma omp parallel for schedule(static)
+        for (int f = 0; f < b->stride[3]; ++f) {
+            real S = div[f + b->global_offset];
+            S += nonortho_rhs(&c, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, 0, 1);
+            div[f + b->global_offset] = S;
         }
     }
     return PICT_OK;
@@ -787,10 +1186,313 @@ static void fixed_boundary_source_grad(const Ctx *c, const pict_block_desc *b, I
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* non-orthogonal terms: vector-Jacobian products                       */
+/* ------------------------------------------------------------------ */
+
+/* scatterBlockData :2340-2347 (gradient fields of the global results) */
+static void scatter_block_data(const Ctx *c, int bi, I4 p, int type, real g) {
+    const pict_block_desc *b = blk(c, bi);
+    real *dst = NULL;
+    switch (GD_BASE(type)) {
+        case GD_VELOCITY_RESULT: dst = (real *)c->d->velocity_result_grad; break;
+        case GD_PRESSURE_RESULT: dst = (real *)c->d->pressure_result_grad; break;
+        default: return;
+    }
+    if (dst) dst[(size_t)p.a[3] * c->N + global_index(p, b)] += g;
+}
+/* scatterFixedBoundaryData :2448-2478 (FixedBoundary velocity gradients) */
+static void scatter_fixed_data(const Ctx *c, const pict_block_desc *b, I4 p, int face, int type, real g) {
+    (void)c;
+    const pict_boundary_desc *fb = &b->bounds[face];
+    if (GD_BASE(type) == GD_IS_FIXED || fb->type != PICT_BOUNDARY_FIXED) return;
+    if (GD_BASE(type) != GD_VELOCITY_RESULT && GD_BASE(type) != GD_VELOCITY) return;
+    if (!fb->velocity_grad) return;
+    real *dst = (real *)fb->velocity_grad;
+    if (fb->velocity_static) { dst[p.a[3]] += g; return; }
+    p.a[face >> 1] = 0;
+    dst[flat4(p, fb->stride)] += g;
+}
+/* scatterBlockDataNeighbor :2560-2602 (boundary values get no gradient) */
+static void scatter_data_neighbor(const Ctx *c, int bi, I4 pos, int dir, int type, real g) {
+    const pict_block_desc *b = blk(c, bi);
+    int dim = dir >> 1, tb = bi;
+    I4 t = pos;
+    if (at_bound(pos, dir, b)) {
+        const pict_boundary_desc *bd = &b->bounds[dir];
+        switch (bd->type) {
+            case PICT_BOUNDARY_CONNECTED: t = connected_pos(c, t, dim, bd, 1); tb = bd->connected_block; break;
+            case PICT_BOUNDARY_PERIODIC: t.a[dim] = (dir & 1) ? 0 : b->size[dim] - 1; break;
+            default: return;
+        }
+    } else {
+        t.a[dim] += face_sign(dir);
+    }
+    scatter_block_data(c, tb, t, type, g);
+}
+/* scatterBlockDataNeighborDiagonal :2659-2700 */
+static void scatter_data_neighbor_diag(const Ctx *c, int bi, I4 pos, int dir1, int dir2, int type, real g) {
+    const int d1e = prescribed(dir1, blk(c, bi));
+    const int dirs[2] = {d1e ? dir2 : dir1, d1e ? dir1 : dir2};
+    int tb = bi;
+    I4 t = pos;
+    for (int i = 0; i < 2; ++i) {
+        const pict_block_desc *b = blk(c, tb);
+        int bound = dirs[i], dim = bound >> 1;
+        if (at_bound(t, bound, b)) {
+            const pict_boundary_desc *bd = &b->bounds[bound];
+            switch (bd->type) {
+                case PICT_BOUNDARY_CONNECTED: t = connected_pos(c, t, dim, bd, 1); tb = bd->connected_block; break;
+                case PICT_BOUNDARY_PERIODIC: t.a[dim] = (bound & 1) ? 0 : b->size[dim] - 1; break;
+                default: return;
+            }
+        } else {
+            t.a[dim] += face_sign(bound);
+        }
+    }
+    scatter_block_data(c, tb, t, type, g);
+}
+/* scatterCornerValue_GRAD :2869-2966: the gradient g of a corner value (with the forward walk's
+   cell count) to the cells it averaged, or to the boundary values it took. */
+static void scatter_corner_value(const Ctx *c, int bi, I4 pos, int dir1, int dir2, int inc0, int inc1, int max_depth,
+                                 int type, real g, int num) {
+    const int from_cells = num > 0;
+    real dg = g;
+    if (from_cells) dg /= (real)num;
+    if (inc0) scatter_block_data(c, bi, pos, type, dg);
+    Walker w[2] = {{dir1, dir2, bi, pos}, {dir2, dir1, bi, pos}};
+    for (int depth = 1; depth <= max_depth; ++depth) {
+        for (int d = 0; d < 2; ++d) {
+            const pict_block_desc *wb = blk(c, w[d].bi);
+            const int axis = w[d].d1 >> 1, fs = face_sign(w[d].d1);
+            if (at_bound(w[d].pos, w[d].d1, wb)) {
+                const pict_boundary_desc *bd = &wb->bounds[w[d].d1];
+                if (bd->type == PICT_BOUNDARY_FIXED) {
+                    if (at_bound(w[d].pos, w[d].d2, wb)) {
+                        scatter_fixed_data(c, wb, w[d].pos, w[d].d1, type, dg);
+                    } else {
+                        I4 np = w[d].pos;
+                        np.a[w[d].d2 >> 1] += face_sign(w[d].d2);
+                        dg *= (real)0.5;
+                        scatter_fixed_data(c, wb, w[d].pos, w[d].d1, type, dg);
+                        scatter_fixed_data(c, wb, np, w[d].d1, type, dg);
+                    }
+                    return;
+                } else if (bd->type == PICT_BOUNDARY_CONNECTED) {
+                    w[d].bi = bd->connected_block;
+                    w[d].pos.a[axis] += fs;
+                    w[d].pos = connected_pos(c, w[d].pos, axis, bd, 1);
+                    const int o1 = w[d].d1;
+                    w[d].d1 = connected_dir(c, w[d].d2, axis, bd);
+                    w[d].d2 = connected_dir(c, o1, axis, bd) ^ 1;
+                } else if (bd->type == PICT_BOUNDARY_PERIODIC) {
+                    w[d].pos.a[axis] = (w[d].d1 & 1) ? 0 : wb->size[axis] - 1;
+                    const int o1 = w[d].d1;
+                    w[d].d1 = w[d].d2;
+                    w[d].d2 = o1 ^ 1;
+                }
+            } else {
+                w[d].pos.a[axis] += fs;
+                const int o1 = w[d].d1;
+                w[d].d1 = w[d].d2;
+                w[d].d2 = o1 ^ 1;
+            }
+            if (w[d].bi == w[d ^ 1].bi && i4_eq(w[d].pos, w[d ^ 1].pos)) return;
+            if (from_cells && (depth > 1 || inc1)) scatter_block_data(c, w[d].bi, w[d].pos, type, dg);
+        }
+    }
+}
+
+/* scatterNonOrthoLaplaceComponents_GRAD :1989-2098: gradients of the interpolated cross coefficients
+   (ai_grad, laid out as interp_nonortho's output) to the viscosities and A of the cell and its
+   face neighbours. */
+static void scatter_interp_nonortho(const Ctx *c, int bi, I4 pos, const real *ai_grad, int with_visc, int with_a) {
+    const pict_block_desc *b = blk(c, bi);
+    const real *A = (const real *)c->d->A;
+    real *Ag = (real *)c->d->A_grad;
+    const int D = c->D;
+    real aP[6];
+    lap_full(c, b, pos, aP);
+    const real one = 1;
+    real raP_grad = 0;
+    for (int face = 0; face < 2 * D; ++face) {
+        const int axis = face >> 1, up = face & 1;
+        if (!((0 < pos.a[axis] && pos.a[axis] < b->size[axis] - 1) || !prescribed(face, b))) continue;
+        I4 pn = pos; pn.a[3] = axis;
+        int nbi = bi;
+        real aN[3];
+        if (at_bound(pos, face, b) && b->bounds[face].type == PICT_BOUNDARY_CONNECTED) {
+            nbi = b->bounds[face].connected_block;
+            pn = connected_pos_ch(c, pn, axis, &b->bounds[face]);
+            lap_single_axis(c, blk(c, nbi), pn, axis, aN);
+        } else {
+            if (!up && pos.a[axis] == 0 && b->bounds[face].type == PICT_BOUNDARY_PERIODIC) pn.a[axis] = b->size[axis] - 1;
+            else if (up && pos.a[axis] == b->size[axis] - 1 && b->bounds[face].type == PICT_BOUNDARY_PERIODIC) pn.a[axis] = 0;
+            else pn.a[axis] = pos.a[axis] + face_sign(face);
+            lap_single_axis(c, b, pn, axis, aN);
+        }
+        pn.a[3] = 0;
+        const pict_block_desc *nb = blk(c, nbi);
+        real raN_grad = 0;
+        for (int i = 1; i < D; ++i) {
+            const int oa = (axis + i) % D;
+            const real g = ai_grad[face * (D - 1) + i - 1];
+            raP_grad += aP[lap_ci(axis, oa)] * (real)0.5 * g;
+            raN_grad += aN[oa] * (real)0.5 * g;
+        }
+        const int gn = global_index(pn, nb);
+        const real raN = with_a ? one / A[gn] : one;
+        if (with_visc) {
+            real t = raN_grad;
+            if (with_a) t *= raN;
+            scatter_viscosity(c, nb, pn, t);
+        }
+        if (with_a) {
+            if (with_visc) raN_grad *= viscosity(c, nb, pn);
+            if (Ag) Ag[gn] += -raN_grad * raN * raN;
+        }
+    }
+    const int g = global_index(pos, b);
+    const real raP = with_a ? one / A[g] : one;
+    if (with_visc) {
+        real t = raP_grad;
+        if (with_a) t *= raP;
+        scatter_viscosity(c, b, pos, t);
+    }
+    if (with_a) {
+        if (with_visc) raP_grad *= viscosity(c, b, pos);
+        if (Ag) Ag[g] += -raP_grad * raP * raP;
+    }
+}
+
+/* VJP of nonortho_matrix_face (PISO_build_matrix_GRAD :3948-4019, PISO_build_pressure_matrix_GRAD
+   :5038-5104): accumulates into ai_grad the gradient of the face's cross coefficients from the row
+   gradient cg (slot order, cg[0] the diagonal's). */
+static void nonortho_matrix_face_grad(const Ctx *c, int bi, I4 pos, int face, const real *ai, int flags, real sign,
+                                      int corner_type, int grad_always, const real *cg, real *ai_grad) {
+    const int D = c->D, dim = face >> 1, fs = face_sign(face);
+    const int nbrs = flags & PICT_NON_ORTHO_DIRECT_MATRIX, ctr = flags & PICT_NON_ORTHO_CENTER_MATRIX;
+    for (int i = 1; i < D; ++i) {
+        const int ta = (dim + i) % D;
+        if (ai[face * (D - 1) + i - 1] == 0) continue;
+        real ag = 0;
+        for (int tu = 0; tu < 2; ++tu) {
+            const int tf = (ta << 1) | tu, tfs = face_sign(tf);
+            const Corner cv = corner_value(c, bi, pos, face, tf, 0, 0, 2, corner_type);
+            if (cv.num < 1) {
+                if (grad_always || cv.btype == PICT_BC_NEUMANN) {
+                    real vg = 0;
+                    if (ctr) vg += 3 * cg[0];
+                    if (nbrs) { vg += 3 * cg[face + 1]; vg -= cg[(tf ^ 1) + 1]; }
+                    ag += (real)(fs * tfs) * (real)0.25 * (sign * vg);
+                }
+            } else {
+                const real norm = (real)(1.0 / (double)cv.num);
+                real vg = 0;
+                if (ctr) vg += cg[0];
+                if (nbrs) { vg += cg[face + 1]; vg += cg[tf + 1]; }
+                ag += (real)(fs * tfs) * norm * (sign * vg);
+            }
+        }
+        ai_grad[face * (D - 1) + i - 1] += ag;
+    }
+}
+
+/* scatterNonOrthoLaplaceRHS_v2_GRAD :3210-3380 (no face transform): the VJP of nonortho_rhs for the
+   output gradient s_grad; `type` is the forward GridDataType (gradients go to the matching
+   result-gradient vector, FixedBoundary velocity gradients, viscosities and A). */
+static void nonortho_rhs_grad(const Ctx *c, int bi, I4 pos, int flags, int type, int with_visc, int with_a,
+                              real s_grad) {
+    if (!(flags & (PICT_NON_ORTHO_DIRECT_RHS | PICT_NON_ORTHO_DIAGONAL_RHS))) return;
+    const pict_block_desc *b = blk(c, bi);
+    const real *A = (const real *)c->d->A;
+    real *Ag = (real *)c->d->A_grad;
+    const int D = c->D;
+    const int direct = (flags & PICT_NON_ORTHO_DIRECT_RHS) != 0;
+    real aP[6];
+    lap_full(c, b, pos, aP);
+    const real one = 1;
+    const int g = global_index(pos, b);
+    const real raP = with_a ? one / A[g] : one;
+    const real viscP = with_visc ? viscosity(c, b, pos) : one;
+    const real rvP = raP * viscP;
+    const int diffable = with_visc || with_a;
+    real raP_grad = 0;
+    for (int face = 0; face < 2 * D; ++face) {
+        const int axis = face >> 1, fs = face_sign(face);
+        if (at_bound(pos, face, b) && prescribed(face, b)) {
+            if (fixed_type(b, face, type) == PICT_BC_NEUMANN || gd_is_pressure(type)) continue;
+            real bA[3];
+            lap_neighbour_boundary(c, &b->bounds[face], pos, axis, bA);
+            for (int i = 1; i < D; ++i) {
+                const int ta = (axis + i) % D;
+                const int tlo = pos.a[ta] == 0, thi = pos.a[ta] == b->size[ta] - 1;
+                I4 lo = pos, hi = pos;
+                real df = (real)0.5;
+                if (!tlo) lo.a[ta] -= 1;
+                if (!thi) hi.a[ta] += 1;
+                if (tlo || thi) df = 1;
+                const real tgg = -(real)fs * bA[ta] * s_grad * df;
+                if (with_visc) {
+                    const real tdg = fixed_data(c, b, hi, face, type) - fixed_data(c, b, lo, face, type);
+                    scatter_viscosity(c, b, pos, tgg * tdg);
+                }
+                const real bv = with_visc ? viscosity(c, b, pos) : one;
+                scatter_fixed_data(c, b, hi, face, type, tgg * bv);
+                scatter_fixed_data(c, b, lo, face, type, -tgg * bv);
+            }
+            continue;
+        }
+        const NCell n = resolve_neighbor(c, bi, pos, face);
+        real aN[3];
+        lap_single_axis_neighbor(c, &n, face, aN);
+        const pict_block_desc *nb = blk(c, n.bi);
+        const int gn = global_index(n.pos, nb);
+        const real raN = with_a ? one / A[gn] : one;
+        const real viscN = with_visc ? viscosity(c, nb, n.pos) : one;
+        const real rvN = raN * viscN;
+        real raN_grad = 0;
+        for (int i = 1; i < D; ++i) {
+            const int ta = (axis + i) % D;
+            const real fa = (aP[lap_ci(axis, ta)] * rvP + aN[ta] * rvN) * (real)0.5;
+            const real tgg = -s_grad * (real)fs * fa;
+            real tdg = 0;
+            for (int tu = 0; tu < 2; ++tu) {
+                const int tf = (ta << 1) | tu, tfs = face_sign(tf);
+                const Corner cv = corner_value(c, bi, pos, face, tf, 0, direct, 2, type);
+                if (cv.num == 0 && (cv.btype == PICT_BC_NEUMANN || gd_is_pressure(type))) {
+                    const int to = tf ^ 1, tofs = -tfs;
+                    if (direct) {
+                        if (diffable) tdg += (real)tfs * data_neighbor(c, bi, pos, face, type) * (real)0.75;
+                        scatter_data_neighbor(c, bi, pos, face, type, tgg * (real)tfs * (real)0.75);
+                        if (diffable) tdg += (real)tofs * data_neighbor(c, bi, pos, to, type) * (real)0.25;
+                        scatter_data_neighbor(c, bi, pos, to, type, tgg * (real)tofs * (real)0.25);
+                    }
+                    if (flags & PICT_NON_ORTHO_DIAGONAL_RHS) {
+                        if (diffable) tdg += (real)tofs * data_neighbor_diag(c, bi, pos, face, to, type) * (real)0.25;
+                        scatter_data_neighbor_diag(c, bi, pos, face, to, type, tgg * (real)tofs * (real)0.25);
+                    }
+                } else {
+                    if (diffable) tdg += (real)tfs * cv.data;
+                    scatter_corner_value(c, bi, pos, face, tf, 0, direct, 2, type, tgg * (real)tfs, cv.num);
+                }
+            }
+            if (diffable) {
+                const real fag = -s_grad * (real)fs * tdg;
+                raP_grad += aP[lap_ci(axis, ta)] * (real)0.5 * fag;
+                raN_grad += aN[ta] * (real)0.5 * fag;
+            }
+        }
+        if (with_visc) scatter_viscosity(c, nb, n.pos, raN_grad * raN);
+        if (with_a && Ag) Ag[gn] += raN_grad * viscN * (-raN * raN);
+    }
+    if (with_visc) scatter_viscosity(c, b, pos, raP_grad * raP);
+    if (with_a && Ag) Ag[g] += raP_grad * viscP * (-raP * raP);
+}
+
 /* PISO_build_matrix_GRAD :3869-4075 */
 int oracle_setup_advection_matrix_grad(const pict_domain_desc *d, double time_step, int flags) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
-    if ((st = check_ortho_flags(flags))) return st;
     (void)time_step;
     const real *Cg = (const real *)d->C_grad, *Ag = (const real *)d->A_grad;
     for (int bi = 0; bi < d->num_blocks; ++bi) {
@@ -806,6 +1508,9 @@ int oracle_setup_advection_matrix_grad(const pict_domain_desc *d, double time_st
             real alpha_p[3] = {1, 1, 1};
             for (int dd = 0; dd < c.D; ++dd) { I4 t = pos; t.a[3] = dd; alpha_p[dd] = laplace_orth(&c, b, t); }
             real fg[6] = {0, 0, 0, 0, 0, 0}, visc_grad = 0;
+            const int no = c.D > 1 && (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX));
+            real ai[12], ai_grad[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (no) interp_nonortho(&c, bi, pos, 0, 0, ai);
             for (int face = 0; face < 2 * c.D; ++face) {
                 int dim = face >> 1;
                 real fs = (face & 1) ? 1 : -1;
@@ -818,11 +1523,18 @@ int oracle_setup_advection_matrix_grad(const pict_domain_desc *d, double time_st
                     visc_grad += (real)0.5 * alpha_p[dim] * vcg;
                     t.a[3] = 0;
                     scatter_viscosity(&c, blk(&c, nb), t, (real)0.5 * alpha * vcg);
+                    if (no) {
+                        real cgd[7];
+                        for (int k = 0; k < 7; ++k) cgd[k] = cg[k];
+                        cgd[0] = diag_grad;
+                        nonortho_matrix_face_grad(&c, bi, pos, face, ai, flags, (real)-1, GD_VELOCITY, 0, cgd, ai_grad);
+                    }
                 } else {
                     real slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
                     visc_grad += (1 - slip) * 2 * alpha_p[dim] * diag_grad;
                 }
             }
+            if (no) scatter_interp_nonortho(&c, bi, pos, ai_grad, 1, 0);
             if (b->velocity_grad) scatter_fluxes(&c, bi, pos, fg, NULL);
             scatter_viscosity(&c, b, pos, visc_grad);
         }
@@ -833,7 +1545,6 @@ int oracle_setup_advection_matrix_grad(const pict_domain_desc *d, double time_st
 /* kPISO_build_advection_RHS_GRAD :4388-4476 */
 int oracle_setup_advection_velocity_grad(const pict_domain_desc *d, double time_step, int flags) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
-    if ((st = check_ortho_flags(flags))) return st;
     const real dt = (real)time_step;
     const real *rg = (const real *)d->velocity_rhs_grad;
     for (int bi = 0; bi < d->num_blocks; ++bi) {
@@ -846,6 +1557,7 @@ int oracle_setup_advection_velocity_grad(const pict_domain_desc *d, double time_
                 real G = rg[(size_t)dd * c.N + b->global_offset + f];
                 scatter_velocity_source(b, t, G);
                 G /= det;
+                nonortho_rhs_grad(&c, bi, t, flags, GD_VELOCITY_RESULT, 1, 0, -G);
                 fixed_boundary_source_grad(&c, b, pos, dd, G);
                 G *= det;
                 if (b->velocity_grad) ((real *)b->velocity_grad)[flat4(t, b->stride)] = G / dt;
@@ -858,7 +1570,6 @@ int oracle_setup_advection_velocity_grad(const pict_domain_desc *d, double time_
 /* PISO_build_pressure_matrix_GRAD :4967-5115 */
 int oracle_setup_pressure_matrix_grad(const pict_domain_desc *d, int flags) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
-    if ((st = check_ortho_flags(flags))) return st;
     const real *Pg = (const real *)d->P_grad, *A = (const real *)d->A;
     real *Ag = (real *)d->A_grad;
     for (int bi = 0; bi < d->num_blocks; ++bi) {
@@ -868,8 +1579,15 @@ int oracle_setup_pressure_matrix_grad(const pict_domain_desc *d, int flags) {
             int g = b->global_offset + f;
             int at[7]; slot_positions(&c, bi, pos, g, at);
             real raP_grad = 0;
+            const int no = c.D > 1 && (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX));
+            real ai[12], ai_grad[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, pcg[7];
+            if (no) {
+                interp_nonortho(&c, bi, pos, 0, 0, ai);
+                for (int k = 0; k < 7; ++k) pcg[k] = at[k] >= 0 ? Pg[at[k]] : 0;
+            }
             for (int face = 0; face < 2 * c.D; ++face) {
                 if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+                if (no) nonortho_matrix_face_grad(&c, bi, pos, face, ai, flags, (real)1, GD_IS_FIXED, 1, pcg, ai_grad);
                 int dim = face >> 1, nb;
                 I4 t = neighbor_cell(&c, b, pos, face, &nb, bi);
                 real alphaN = laplace_orth(&c, blk(&c, nb), t);
@@ -884,7 +1602,22 @@ int oracle_setup_pressure_matrix_grad(const pict_domain_desc *d, int flags) {
             }
             real raP = 1 / A[g];
             Ag[g] += -raP_grad * raP * raP;
+            if (no) scatter_interp_nonortho(&c, bi, pos, ai_grad, 0, 1);
         }
+    }
+    return PICT_OK;
+}
+
+/* k_pressureRHSaddNonOrthoComponents_GRAD :5479-5494: pressure_rhs_div_grad -> pressure_result_grad,
+   A_grad (+=) */
+int oracle_pressure_rhs_add_non_ortho_grad(const pict_domain_desc *d, int flags) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    if (!(flags & (PICT_NON_ORTHO_DIRECT_RHS | PICT_NON_ORTHO_DIAGONAL_RHS))) return PICT_OK;
+    const real *dg = (const real *)d->pressure_rhs_div_grad;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f)
+            nonortho_rhs_grad(&c, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, 0, 1, dg[f + b->global_offset]);
     }
     return PICT_OK;
 }

@@ -27,6 +27,8 @@ def input_tensors(d, name):
         out.append((d.C.value, lambda: d.CGrad.value))
     elif name == "PRESSURE_RHS":
         out.append((d.pressureRHS, lambda: d.pressureRHSGrad))
+    elif name == "PRESSURE_RESULT":
+        out.append((d.pressureResult, lambda: d.pressureResultGrad))
     return out
 
 

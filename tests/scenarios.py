@@ -188,3 +188,86 @@ ALL = {
     "three_d_connected": three_d_connected,
     "cavity_static_fields": cavity_static_fields,
 }
+
+
+def sheared_coords(size, shear, h=0.1, dtype=torch.float64, device="cpu", bend=0.0):
+    """Vertex coordinates of a non-orthogonal grid.  2-D: x = h (i + s0 j), y = h j; 3-D: x = h (i + s0 j + s1 k),
+    y = h (j + s2 k), z = h k (an affine map of the unit lattice: every cell has the same metrics).  `bend` adds
+    a smooth curvature, y += bend h sin(pi i / nx), so the metrics vary from cell to cell."""
+    vs = [n + 1 for n in size]
+    axes = [torch.arange(n, dtype=torch.float64) for n in vs]
+    if len(size) == 2:
+        j, i = torch.meshgrid(axes[1], axes[0], indexing="ij")
+        x = h * (i + shear[0] * j)
+        y = h * j + bend * h * torch.sin(math.pi * i / size[0])
+        c = torch.stack([x, y])
+    else:
+        k, j, i = torch.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+        x = h * (i + shear[0] * j + shear[1] * k)
+        y = h * (j + shear[2] * k) + bend * h * torch.sin(math.pi * i / size[0])
+        z = h * k
+        c = torch.stack([x, y, z])
+    return c[None].to(dtype=dtype, device=device).contiguous()
+
+
+def sheared_box(dtype=torch.float64, device="cuda", size=(9, 8), shear=(0.4,), periodic=(), walls_moving=True,
+                bend=0.0, seed=7, visc=0.04):
+    """One block on a sheared grid: closed walls on every face not listed in `periodic` (axis names), the +y
+    wall moving tangentially; random velocity and pressure."""
+    dims = len(size)
+    d = _domain(dims, visc, dtype, device)
+    g = torch.Generator().manual_seed(seed)
+    b = _coords_block(d, sheared_coords(size, shear, dtype=dtype, device=device, bend=bend))
+    b.CloseAllBoundaries()
+    for ax in periodic:
+        b.MakePeriodic(ax)
+    if walls_moving and "y" not in periodic:
+        v = [0.0] * dims
+        v[0] = 0.6
+        b.CloseBoundary("+y", velocity=torch.tensor([v], dtype=dtype, device=device))
+    b.setVelocity(_rand(b.velocity.shape, g, dtype, device, 0.4))
+    b.setPressure(_rand(b.pressure.shape, g, dtype, device, 0.2))
+    return _prepare(d)
+
+
+def _prepare(d):
+    d.PrepareSolve()
+    if str(d.getDevice()) == "cpu":
+        from oracle import oracle as O
+        O.build_csr_pattern(d)
+    return d
+
+
+def sheared_two_block(dtype=torch.float64, device="cuda", seed=8, rotate=True):
+    """Two sheared, bent blocks joined along x (block0 +x <-> block1, with block1's axes permuted and inverted
+    when `rotate`), walls elsewhere, one moving; random velocity and pressure, a per-cell viscosity."""
+    d = _domain(2, 0.03, dtype, device)
+    g = torch.Generator().manual_seed(seed)
+    b0 = _coords_block(d, sheared_coords((7, 6), (0.3,), dtype=dtype, device=device, bend=0.3))
+    if rotate:
+        b1 = _coords_block(d, sheared_coords((6, 5), (-0.25,), dtype=dtype, device=device, bend=0.2))
+    else:
+        b1 = _coords_block(d, sheared_coords((5, 6), (0.3,), dtype=dtype, device=device, bend=0.2))
+    b0.CloseAllBoundaries()
+    b1.CloseAllBoundaries()
+    if rotate:
+        b0.ConnectBlock("+x", b1, "-y", "+x")  # inverted axis
+    else:
+        b0.ConnectBlock("+x", b1, "-x", "+y")
+    b0.CloseBoundary("-y", velocity=torch.tensor([[0.7, 0.0]], dtype=dtype, device=device))
+    for b in (b0, b1):
+        b.setVelocity(_rand(b.velocity.shape, g, dtype, device, 0.4))
+        b.setPressure(_rand(b.pressure.shape, g, dtype, device, 0.2))
+    b0.setViscosity((torch.rand([1, 1, 6, 7], generator=g, dtype=torch.float64) * 0.05 + 0.01).to(dtype=dtype,
+                                                                                                   device=device))
+    return _prepare(d)
+
+
+NON_ORTHO = {
+    "sheared_walls": lambda **kw: sheared_box(bend=0.3, **kw),
+    "sheared_periodic_x": lambda **kw: sheared_box(periodic=("x",), **kw),
+    "sheared_two_block_rotated": sheared_two_block,
+    "sheared_two_block": lambda **kw: sheared_two_block(rotate=False, **kw),
+    "sheared_box3d": lambda **kw: sheared_box(size=(5, 4, 6), shear=(0.3, -0.2, 0.25), periodic=("z",), bend=0.2,
+                                              **kw),
+}

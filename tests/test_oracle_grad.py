@@ -148,3 +148,44 @@ def test_oracle_step_adjoint_matches_finite_differences(name, correctors):
     fd = (loss([u + EPS * v for u, v in zip(u0, V)]) - loss([u - EPS * v for u, v in zip(u0, V)])) / (2 * EPS)
     an = sum(float((a * v).sum()) for a, v in zip(gu, V))
     assert abs(fd - an) <= 1e-5 * abs(fd) + 1e-9, (fd, an)
+
+
+# Non-orthogonal terms: the stages with the driver's flags (25: centre and direct matrix terms, diagonal RHS
+# terms) and with the RHS carrying the direct terms too (26 = 16|2|8), on sheared and bent grids, across rotated
+# connections and periodic wraps.  The RHS stages then also depend on the previous iterate (velocity result,
+# pressure result) and the pressure RHS on A (PISOtorch_diff.py:546-560, :985-999).
+def _no_stages(flags):
+    return {
+        "SetupAdvectionMatrix": (
+            lambda d, dt: O.SetupAdvectionMatrix(d, dt, flags), lambda d, dt: O.SetupAdvectionMatrixGrad(d, dt, flags),
+            ["VELOCITY", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK"], ["A", "C"]),
+        "SetupAdvectionVelocity": (
+            lambda d, dt: O.SetupAdvectionVelocity(d, dt, flags),
+            lambda d, dt: O.SetupAdvectionVelocityGrad(d, dt, flags),
+            ["VELOCITY", "VELOCITY_SOURCE", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK", "VELOCITY_RESULT"],
+            ["VELOCITY_RHS"]),
+        "SetupPressureMatrix": (
+            lambda d, dt: O.SetupPressureMatrix(d, dt, flags), lambda d, dt: O.SetupPressureMatrixGrad(d, dt, flags),
+            ["A"], ["P"]),
+        "SetupPressureRHSdiv": (
+            lambda d, dt: O.SetupPressureRHSdiv(d, dt, flags), lambda d, dt: O.SetupPressureRHSdivGrad(d, dt, flags),
+            ["PRESSURE_RHS", "BOUNDARY_VELOCITY", "PRESSURE_RESULT", "A"], ["PRESSURE_RHS_DIV"]),
+        "SetupPressureCorrection": (
+            lambda d, dt: O.SetupPressureCorrection(d, dt, flags),
+            lambda d, dt: O.SetupPressureCorrectionGrad(d, dt, flags),
+            ["VELOCITY", "VELOCITY_SOURCE", "VELOCITY_RESULT", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK",
+             "C", "A", "PRESSURE_RESULT"], ["P", "PRESSURE_RHS", "PRESSURE_RHS_DIV"]),
+    }
+
+
+@pytest.mark.parametrize("flags", [25, 26])
+@pytest.mark.parametrize("stage", ["SetupAdvectionMatrix", "SetupAdvectionVelocity", "SetupPressureMatrix",
+                                   "SetupPressureRHSdiv", "SetupPressureCorrection"])
+@pytest.mark.parametrize("name", list(scenarios.NON_ORTHO))
+def test_oracle_non_ortho_vjp_matches_finite_differences(name, stage, flags, monkeypatch):
+    d = scenarios.NON_ORTHO[name](**CPU)
+    g = _gen(11)
+    d.setPressureResult(_rand_like(d.pressureResult, g, 0.3))
+    for k, v in _no_stages(flags).items():
+        monkeypatch.setitem(STAGES, k, v)
+    assert run_check(d, stage) > 0

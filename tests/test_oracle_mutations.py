@@ -70,3 +70,71 @@ def test_planted_mistake_fails_projection_pins(name, edits, tmp_path):
         O._libs[64] = good
     assert "fourier" in failed, (name, failed)
     assert len(failed) >= 2, (name, failed)
+
+
+# Planted mistakes in the non-orthogonal terms, each in the forward stage and its VJP where both have it; the
+# operator pins of test_oracle_nonortho.py (exactness for quadratics on sheared grids, one operator whatever the
+# split between matrix and right-hand side on a bent grid) must fail for each.  On a uniform affine grid the
+# matrix's corner entries cancel over the faces (the cross derivative comes from the diagonal cells alone), so the
+# matrix part is pinned by the split check, whose bent grid varies the cross coefficients.
+NON_ORTHO_MUTATIONS = [
+    ("interpolated cross coefficient without the 1/2",
+     [("out[face * (c->D - 1) + i - 1] = (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN) * (real)0.5;",
+       "out[face * (c->D - 1) + i - 1] = (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN);"),
+      ("raP_grad += aP[lap_ci(axis, oa)] * (real)0.5 * g;", "raP_grad += aP[lap_ci(axis, oa)] * g;"),
+      ("raN_grad += aN[oa] * (real)0.5 * g;", "raN_grad += aN[oa] * g;")]),
+    ("matrix corner terms without the tangential face sign",
+     [("const real v = (real)(fs * tfs) * alpha * norm;", "const real v = (real)(fs) * alpha * norm;"),
+      ("ag += (real)(fs * tfs) * norm * (sign * vg);", "ag += (real)(fs) * norm * (sign * vg);")]),
+    ("corner value averaged over one cell too many",
+     [("r.data = data / (real)num; r.num = num; r.btype = btype;",
+       "r.data = data / (real)(num + 1); r.num = num + 1; r.btype = btype;")]),
+    ("explicit non-orthogonal terms with the sign flipped",
+     [("            S -= (real)fs * fa * tdg;\n", "            S += (real)fs * fa * tdg;\n")]),
+    ("centre cross coefficient of the wrong axis pair",
+     [("const real fa = (aP[lap_ci(axis, ta)] * raP + aN[ta] * raN) * (real)0.5;",
+       "const real fa = (aP[lap_ci(axis, axis)] * raP + aN[ta] * raN) * (real)0.5;")]),
+]
+
+
+def _non_ortho_pins():
+    import test_oracle_nonortho as N
+    failed = []
+    for name, pin in (("pressure2d", lambda: N.test_pressure_operator_exact_for_quadratics(*N.CASES[0])),
+                      ("pressure3d", lambda: N.test_pressure_operator_exact_for_quadratics(*N.CASES[2])),
+                      ("velocity2d", lambda: N.test_velocity_operator_exact_for_quadratics(*N.CASES[1])),
+                      ("split", lambda: N.test_flag_split_gives_one_operator(("x",)))):
+        try:
+            pin()
+        except AssertionError:
+            failed.append(name)
+    return failed
+
+
+def _with_mutant(edits, tmp_path, fn):
+    src = open(SRC).read()
+    for old, new in edits:
+        assert src.count(old) == 1, old
+        src = src.replace(old, new)
+    mut = tmp_path / "pict_oracle_mut.c"
+    mut.write_text(src)
+    so = tmp_path / "liboracle_mut.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-std=c99", "-fopenmp", "-DORACLE_REAL=double", "-I",
+                    os.path.dirname(SRC), "-shared", str(mut), "-o", str(so), "-lm"], check=True,
+                   cwd=os.path.dirname(SRC))
+    good = O.lib(64)
+    O._libs[64] = O.declare(ctypes.CDLL(str(so)))
+    try:
+        return fn()
+    finally:
+        O._libs[64] = good
+
+
+def test_non_ortho_pins_pass_on_the_oracle():
+    assert _non_ortho_pins() == []
+
+
+@pytest.mark.parametrize("name,edits", NON_ORTHO_MUTATIONS, ids=[m[0] for m in NON_ORTHO_MUTATIONS])
+def test_planted_mistake_fails_non_ortho_pins(name, edits, tmp_path):
+    failed = _with_mutant(edits, tmp_path, _non_ortho_pins)
+    assert len(failed) >= 1, (name, failed)
