@@ -24,6 +24,15 @@ from . import PISOtorch_diff
 _LOG = logging.getLogger("PISOsim")
 cpu_device = torch.device("cpu")
 
+# Asynchronous native calls hold one of the library's 16 pending-step slots until their solver results are
+# collected (pict_step_wait); the slots are shared by every Simulation of the process.
+_MAX_PENDING_STEPS = 16
+_deferring = []  # Simulations that hold pending checks, in the order they queued their first one
+
+
+def _pending_steps_total():
+    return sum(len(s._pending_checks) for s in _deferring)
+
 
 def tensor_as_np(tensor):
     return tensor.detach().cpu().numpy()
@@ -704,24 +713,44 @@ class Simulation:
         return o
 
     def _defer_check(self, fn, o, adjoint, *args):
-        """Queue an asynchronous native call (`fn`, its arguments before the ticket) and keep its ticket."""
+        """Queue an asynchronous native call (`fn`, its arguments before the ticket) and keep its ticket.
+        When the process's pending-step slots are all taken, every Simulation holding some flushes first (its
+        own errors are raised by its own flush)."""
         from . import _abi
-        if len(self._pending_checks) >= 16:
-            self.flush_solver_checks()
+        if _pending_steps_total() >= _MAX_PENDING_STEPS:
+            first_err = None
+            for sim in list(_deferring):
+                try:
+                    sim.flush_solver_checks()
+                except Exception as e:  # collect every other Simulation's slots before raising
+                    if sim is self and first_err is None:
+                        first_err = e
+                    elif sim is not self:
+                        sim._flush_error = e
+            if first_err is not None:
+                raise first_err
         t = ctypes.c_int32(-1)
         _abi.check(fn(*args, ctypes.byref(t), PISOtorch._stream(self.domain.getDevice())),
                    "PISO step reverse pass" if adjoint else "PISO step")
+        if not self._pending_checks and self not in _deferring:
+            _deferring.append(self)
         self._pending_checks.append((t.value, o, adjoint, self.domain.getSpatialDims()))
 
     def flush_solver_checks(self):
         """Wait for the native calls queued with defer_solver_checks and check their solver results in order,
-        raising the first error (LinsolveError, or a solver barrier error) the waiting calls would have raised."""
+        raising the first error (LinsolveError, or a solver barrier error) the waiting calls would have raised.
+        Whoever sets defer_solver_checks calls this before reading the solver results (Simulation.run does after
+        every major step).  An error found while another Simulation freed the shared slots is raised here."""
+        err, self._flush_error = getattr(self, "_flush_error", None), None
         if not self._pending_checks:
+            if err is not None:
+                raise err
             return
         from . import _abi
         lib = _abi.load_library()
         pending, self._pending_checks = self._pending_checks, []
-        err = None
+        if self in _deferring:
+            _deferring.remove(self)
         for ticket, o, adjoint, D in pending:
             infos = (_abi.SolveInfo * (D + o.corrector_steps))()
             st = lib.pict_step_wait(ticket, infos)
@@ -817,6 +846,8 @@ class Simulation:
         for it in range(1, iterations + 1):
             try:
                 sim_ok = self._PISO_split_step(iterations=self.substeps)
+                # deferred solver results of this major step are checked before it counts as done
+                self.flush_solver_checks()
             except PISOtorch_diff.LinsolveError:
                 _LOG.exception("Simulation failed in major step %d (total step %d):", it, self.total_step)
                 break

@@ -2402,12 +2402,17 @@ const void *bicg_local_kernel(int rpt, int nz, int nrhs) {
     if (nz <= 5) return nrhs == 1 ? bicg_local_kernel_nz<5, 1>(rpt) : nrhs == 2 ? bicg_local_kernel_nz<5, 2>(rpt) : nullptr;
     return nrhs == 1 ? bicg_local_kernel_nz<7, 1>(rpt) : nullptr;
 }
-// Solver grids of at most one workgroup per compute unit (256 workgroups; every solver kernel has an occupancy
-// of at least one) are launched as ordinary kernels: on a device that runs nothing else they are all resident at
-// once, and the cooperative launch costs host time that leaves the GPU idle between a solve and the next kernel
-// (C3 forward+backward: 18.3 M against 16.8 M cells*steps/s).  If other work holds the compute units, waiting
-// workgroups start when it ends; a barrier that waits longer than its spin limit ends the solve with an error.
-// PICT_SOLVER_PLAIN_LAUNCH=0 keeps the cooperative launch.
+// Solver grids of at most one workgroup per compute unit (the device's queried CU count; every solver kernel has
+// an occupancy of at least one) are launched as ordinary kernels: the cooperative launch costs host time that
+// leaves the GPU idle between a solve and the next kernel (periodic box 256x256 forward+backward: 18.3 M against
+// 16.8 M cells*steps/s).  Residency: on an idle device the dispatcher places all of a grid of at most one
+// workgroup per CU at once.  Work of other queues that holds CUs delays the workgroups that find no room, and
+// they start when it releases them; kernels that never wait on other grids (every stage kernel, torch's kernels)
+// always do, so the grid becomes fully resident and its barriers complete.  Two persistent grids that wait on their own barriers and
+// together exceed what the CUs hold (two such solves on concurrent streams, or in two processes sharing the GPU)
+// can each be partly resident; the spin limit then ends the waiting solve with PICT_ERR_DEVICE_SYNC instead of a
+// hang.  The library's own solves are queued on one stream per call, so this only arises from concurrent callers.
+// PICT_SOLVER_PLAIN_LAUNCH=0 keeps the cooperative launch, whose residency the runtime checks.
 static bool plain_launch_enabled() {
     const char *e = getenv("PICT_SOLVER_PLAIN_LAUNCH");
     return !e || atoi(e) != 0;
@@ -2436,6 +2441,7 @@ struct Plan {
     int rpt, nz;      // register-resident kernels: rows per thread, cached row length
     bool full;      // k_cg_local reduces the sum and maximum of |r| too
     bool lean;      // k_cg1 without the register row cache
+    int cus;        // compute units of the device
 };
 
 template <typename R>
@@ -2445,11 +2451,18 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
     const char *tb_env = getenv("PICT_CG_TEXTBOOK");
     const bool textbook_cg = tb_env && atoi(tb_env) != 0;
     pl = Plan{};
-    pl.kind = o->use_bicg ? Kind::kBiCG : (textbook_cg ? Kind::kCGTextbook : Kind::kCG1);
+    // CG with residual resets runs the textbook kernel: the reference resets the residual only in the driver's
+    // non-orthogonal branch (PISOtorch_simulation.py:1280), whose pressure matrix is not symmetric, and on a
+    // non-symmetric matrix the single-reduction recurrences no longer follow textbook CG's iterates (they stall
+    // where textbook CG, the reference's algorithm, converges)
+    pl.kind = o->use_bicg ? Kind::kBiCG
+              : (textbook_cg || o->residual_reset_steps > 0) ? Kind::kCGTextbook
+                                                             : Kind::kCG1;
     pl.kernel = kernel_for_nrhs<R>(pl.kind, nrhs);
     int occ = 0, cus = 0;
     pict_status st = occupancy(pl.kernel, occ, cus);
     if (st) return st;
+    pl.cus = cus;
     if (occ < 1) {
         set_error("solver kernel cannot be resident (occupancy 0)");
         return PICT_ERR_HIP;
@@ -2663,7 +2676,7 @@ pict_status enqueue_typed(const pict_csr_desc *A, const void *rhs, void *x, int 
     }
     void *args[] = {&a};
     if (ev0) PICT_HIP_TRY(hipEventRecord(ev0, stream));
-    if (plain_launch_enabled() && nwg <= 256)
+    if (plain_launch_enabled() && nwg <= pl.cus)
         PICT_HIP_TRY(hipLaunchKernel(kernel, dim3((unsigned)nwg), dim3(kSolveThreads), args, 0, stream));
     else
         PICT_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3((unsigned)nwg), dim3(kSolveThreads), args, 0, stream));

@@ -263,3 +263,22 @@ def test_large_grid_wide_solves_stay_resident(bicg, monkeypatch):
     res = np.linalg.norm(bd - m @ x.double().cpu().numpy()) / np.linalg.norm(bd)
     print(" relative residual after 30 iterations %.3e" % res, flush=True)
     assert res < 0.5
+
+
+@pytest.mark.parametrize("bicg", [False, True])
+def test_small_grids_cooperative_and_plain_launch(bicg, monkeypatch):
+    """Solver grids of at most one workgroup per compute unit are ordinary launches by default;
+    PICT_SOLVER_PLAIN_LAUNCH=0 launches them cooperatively.  Both give bitwise the same solution and iteration
+    count (the same kernel on the same grid)."""
+    print("case launch kinds", "bicgstab" if bicg else "cg", flush=True)
+    m = periodic_laplacian((128, 96), 0.05)
+    A = to_pict(m, torch.float32)
+    b = rhs(m.shape[0], 11, torch.float32)
+    out = []
+    for plain in ("1", "0"):
+        monkeypatch.setenv("PICT_SOLVER_PLAIN_LAUNCH", plain)
+        x = torch.zeros_like(b)
+        info = P.SolveLinear(A, b, x, 2000, 1e-6, NORM2N, bicg, False, 0, False, False, False, False)[0]
+        assert info.converged and info.isFiniteResidual
+        out.append((x, info.usedIterations))
+    assert out[0][1] == out[1][1] and torch.equal(out[0][0], out[1][0])
