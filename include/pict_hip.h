@@ -217,7 +217,9 @@ pict_status pict_copy_pressure_result_from_blocks(const pict_atlas *atlas, void 
 /* Inputs are the *_grad tensors of the stage outputs; outputs are the
    *_grad tensors of its inputs, which the caller zero-initialises (the
    reference's Create*Grad).  Tensors a stage accumulates into are marked +=,
-   tensors it overwrites are marked =.  Orthogonal terms only. */
+   tensors it overwrites are marked =.  With non-orthogonal flags the stages also differentiate the explicit
+   terms: SetupAdvectionVelocityGrad into velocity_result_grad (+=), SetupPressureRHSdivGrad (and the calls that
+   include it) into pressure_result_grad and A_grad (+=). */
 
 /* SetupAdvectionMatrixGrad (PISOtorch.cpp:597-599; kernel PISO_build_matrix_GRAD
    :3869-4075): C_grad, A_grad -> block velocity_grad (+=), viscosity grads (+=). */

@@ -348,8 +348,12 @@ def csr_to_scipy(A):
 # ---------------------------------------------------------------------------
 
 def piso_step(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, max_iter=5000,
-              normalize_pressure=True, infos=None):
-    """One PISO step of the orthogonal driver branch on a CPU twin, with the oracle's solvers."""
+              normalize_pressure=True, infos=None, non_orthogonal=False, non_ortho_flags=25):
+    """One PISO step of the orthogonal driver branch on a CPU twin, with the oracle's solvers (non_orthogonal: the
+    driver's non-orthogonal branch)."""
+    if non_orthogonal:
+        return piso_step_non_ortho(domain, time_step, corrector_steps, advection_tol, pressure_tol, max_iter,
+                                   normalize_pressure, infos, non_ortho_flags)
     dt = torch.tensor([time_step], dtype=domain.getDtype())
     SetupAdvectionMatrix(domain, dt, 0)
     SetupAdvectionVelocity(domain, dt, 0, False)
@@ -367,6 +371,44 @@ def piso_step(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure
         if normalize_pressure:
             p = p - torch.mean(p)
         domain.setPressureResult(p.contiguous())
+        CopyPressureResultToBlocks(domain)
+        CorrectVelocity(domain, dt, 1)
+    CopyVelocityResultToBlocks(domain)
+    domain._set_changed(False)
+
+
+def piso_step_non_ortho(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, max_iter=5000,
+                        normalize_pressure=True, infos=None, flags=25, advect_non_ortho_steps=1,
+                        pressure_non_ortho_steps=1):
+    """One PISO step of the driver's non-orthogonal branch (PISOtorch_simulation.py:1152-1300, no passive scalar,
+    no result reuse): the explicit non-orthogonal terms of the advection RHS read the velocity result (the block
+    velocities copied in first), those of the pressure RHS the pressure result left by the previous solve; the
+    pressure CG recomputes its residual every 100 iterations."""
+    dt = torch.tensor([time_step], dtype=domain.getDtype())
+    SetupAdvectionMatrix(domain, dt, flags)
+    CopyVelocityResultFromBlocks(domain)
+    for _ in range(advect_non_ortho_steps):
+        SetupAdvectionVelocity(domain, dt, flags, False)
+        x = torch.zeros_like(domain.velocityRHS)
+        inf = SolveLinear(domain.C, domain.velocityRHS, x, max_iter, advection_tol, 1, useBiCG=True)
+        if infos is not None:
+            infos.append(("adv", inf))
+        domain.setVelocityResult(x)
+    for _ in range(corrector_steps):
+        SetupPressureMatrix(domain, dt, flags)
+        for pstep in range(pressure_non_ortho_steps):
+            if pstep == 0:
+                SetupPressureRHS(domain, dt, flags)
+            else:
+                SetupPressureRHSdiv(domain, dt, flags)
+            p = torch.zeros_like(domain.pressureRHSdiv)
+            inf = SolveLinear(domain.P, domain.pressureRHSdiv, p, max_iter, pressure_tol, 1, useBiCG=False,
+                              residualResetSteps=100)
+            if infos is not None:
+                infos.append(("p", inf))
+            if normalize_pressure:
+                p = p - torch.mean(p)
+            domain.setPressureResult(p.contiguous())
         CopyPressureResultToBlocks(domain)
         CorrectVelocity(domain, dt, 1)
     CopyVelocityResultToBlocks(domain)

@@ -338,11 +338,14 @@ _EXCLUDED_GRADIENTS["SetupAdvectionVelocity"] = (set(), set())
 
 
 def SetupAdvectionVelocity(domain, time_step, non_ortho_flags, apply_pressure_gradient=False):
-    """PISOtorch_diff.py:531-614: velocityRHS from block velocity, velocity source, boundary velocity, viscosities."""
-    if check_non_ortho_rhs(non_ortho_flags):
-        raise PISOtorch.PictUnsupportedError("non-orthogonal RHS terms are not implemented by libpict_hip")
+    """PISOtorch_diff.py:531-614: velocityRHS from block velocity, velocity source, boundary velocity, viscosities
+    (and the velocity result, through the explicit non-orthogonal terms)."""
+    is_non_ortho = check_non_ortho_rhs(non_ortho_flags)
     tracked = ["VELOCITY", "VELOCITY_SOURCE", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK"]
     saved = ["BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK", "VELOCITY_SOURCE"]
+    if is_non_ortho:
+        tracked.append("VELOCITY_RESULT")
+        saved.append("VELOCITY_RESULT")
     name = "SetupAdvectionVelocity"
 
     def fwd(d):
@@ -356,6 +359,8 @@ def SetupAdvectionVelocity(domain, time_step, non_ortho_flags, apply_pressure_gr
         d.CreateVelocityGradOnBlocks()
         d.CreateVelocityGradOnBoundaries()
         d.CreateVelocitySourceGradOnBlocks()
+        if is_non_ortho:
+            d.CreateVelocityResultGrad()
         d.UpdateDomainData()
         PISOtorch.SetupAdvectionVelocityGrad(d, dt, non_ortho_flags, apply_pressure_gradient)
 
@@ -462,7 +467,16 @@ _PRESSURE_SAVED = ["BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK", "A", "C"
                    "VELOCITY_SOURCE"]
 
 
-def _pressure_rhs_grad_setup(d, name, prhs_grad, prhs_div_grad):
+def _pressure_lists(non_ortho_flags):
+    """Tracked and saved tensors of the pressure RHS stages: with explicit non-orthogonal terms the RHS also
+    depends on the pressure result (PISOtorch_diff.py:807-818, :985-994).  It is saved for the backward pass here
+    (the reference leaves it out of its saved list and reads whatever the domain holds then)."""
+    if not check_non_ortho_rhs(non_ortho_flags):
+        return _PRESSURE_TRACKED, _PRESSURE_SAVED
+    return _PRESSURE_TRACKED + ["PRESSURE_RESULT"], _PRESSURE_SAVED + ["PRESSURE_RESULT"]
+
+
+def _pressure_rhs_grad_setup(d, name, prhs_grad, prhs_div_grad, non_ortho_flags=0):
     _set_gradient_input(prhs_grad, d.setPressureRHSGrad, name, "PRESSURE_RHS_GRAD")
     _set_gradient_input(prhs_div_grad, d.setPressureRHSdivGrad, name, "PRESSURE_RHS_DIV_GRAD")
     d.CreateAGrad()
@@ -472,6 +486,8 @@ def _pressure_rhs_grad_setup(d, name, prhs_grad, prhs_div_grad):
     d.CreateVelocityGradOnBlocks()
     d.CreateVelocitySourceGradOnBlocks()
     d.CreateVelocityGradOnBoundaries()
+    if check_non_ortho_rhs(non_ortho_flags):
+        d.CreatePressureResultGrad()
 
 
 _EXCLUDED_GRADIENTS["SetupPressureCorrection"] = (set(), set())
@@ -479,9 +495,8 @@ _EXCLUDED_GRADIENTS["SetupPressureCorrection"] = (set(), set())
 
 def SetupPressureCorrection(domain, time_step, non_ortho_flags, use_face_transform=False, timeStepNorm=False):
     """PISOtorch_diff.py:802-894: (P.value, pressureRHS, pressureRHSdiv)."""
-    if check_non_ortho_rhs(non_ortho_flags):
-        raise PISOtorch.PictUnsupportedError("non-orthogonal RHS terms are not implemented by libpict_hip")
     name = "SetupPressureCorrection"
+    tracked, saved = _pressure_lists(non_ortho_flags)
 
     def fwd(d):
         d.CreatePressureRHS()
@@ -494,13 +509,12 @@ def SetupPressureCorrection(domain, time_step, non_ortho_flags, use_face_transfo
         if use_face_transform:
             raise NotImplementedError("SetupPressureCorrection: Gradient does not support face transformations.")
         _set_gradient_input(grads[0], d.PGrad.setValue, name, "P_GRAD")
-        _pressure_rhs_grad_setup(d, name, grads[1], grads[2])
+        _pressure_rhs_grad_setup(d, name, grads[1], grads[2], non_ortho_flags)
         d.UpdateDomainData()
         PISOtorch.SetupPressureCorrectionGrad(d, dt, non_ortho_flags, use_face_transform, timeStepNorm)
 
-    fn = _stage(name, _PRESSURE_TRACKED, _PRESSURE_SAVED, fwd, bwd, lambda d: (d.P.value, d.pressureRHS,
-                                                                               d.pressureRHSdiv))
-    _apply(fn, domain, time_step, _PRESSURE_TRACKED)
+    fn = _stage(name, tracked, saved, fwd, bwd, lambda d: (d.P.value, d.pressureRHS, d.pressureRHSdiv))
+    _apply(fn, domain, time_step, tracked)
 
 
 _EXCLUDED_GRADIENTS["SetupPressureMatrix"] = (set(), set())
@@ -532,9 +546,8 @@ _EXCLUDED_GRADIENTS["SetupPressureRHS"] = (set(), set())
 
 def SetupPressureRHS(domain, time_step, non_ortho_flags, use_face_transform=False, timeStepNorm=False):
     """PISOtorch_diff.py:967-1064: (pressureRHS, pressureRHSdiv)."""
-    if check_non_ortho_rhs(non_ortho_flags):
-        raise PISOtorch.PictUnsupportedError("non-orthogonal RHS terms are not implemented by libpict_hip")
     name = "SetupPressureRHS"
+    tracked, saved = _pressure_lists(non_ortho_flags)
 
     def fwd(d):
         d.CreatePressureRHS()
@@ -545,23 +558,27 @@ def SetupPressureRHS(domain, time_step, non_ortho_flags, use_face_transform=Fals
     def bwd(d, dt, grads):
         if use_face_transform:
             raise NotImplementedError("SetupPressureRHS: Gradient does not support face transformations.")
-        _pressure_rhs_grad_setup(d, name, grads[0], grads[1])
+        _pressure_rhs_grad_setup(d, name, grads[0], grads[1], non_ortho_flags)
         d.UpdateDomainData()
         PISOtorch.SetupPressureRHSGrad(d, dt, non_ortho_flags, use_face_transform, timeStepNorm)
 
-    fn = _stage(name, _PRESSURE_TRACKED, _PRESSURE_SAVED, fwd, bwd, lambda d: (d.pressureRHS, d.pressureRHSdiv))
-    _apply(fn, domain, time_step, _PRESSURE_TRACKED)
+    fn = _stage(name, tracked, saved, fwd, bwd, lambda d: (d.pressureRHS, d.pressureRHSdiv))
+    _apply(fn, domain, time_step, tracked)
 
 
 _EXCLUDED_GRADIENTS["SetupPressureRHSdiv"] = (set(), set())
 
 
 def SetupPressureRHSdiv(domain, time_step, non_ortho_flags, use_face_transform=False, timeStepNorm=False):
-    """PISOtorch_diff.py:1067-1157: pressureRHSdiv from pressureRHS (and the boundary velocities)."""
-    if check_non_ortho_rhs(non_ortho_flags):
-        raise PISOtorch.PictUnsupportedError("non-orthogonal RHS terms are not implemented by libpict_hip")
+    """PISOtorch_diff.py:1067-1157: pressureRHSdiv from pressureRHS (and the boundary velocities; with explicit
+    non-orthogonal terms also from the pressure result and A)."""
     name = "SetupPressureRHSdiv"
+    is_non_ortho = check_non_ortho_rhs(non_ortho_flags)
     tracked = ["PRESSURE_RHS", "BOUNDARY_VELOCITY"]
+    saved = []
+    if is_non_ortho:
+        tracked += ["PRESSURE_RESULT", "A"]
+        saved = ["PRESSURE_RESULT", "A"]
 
     def fwd(d):
         d.CreatePressureRHSdiv()
@@ -572,12 +589,16 @@ def SetupPressureRHSdiv(domain, time_step, non_ortho_flags, use_face_transform=F
         if use_face_transform:
             raise NotImplementedError("SetupPressureRHSdiv: Gradient does not support face transformations.")
         _set_gradient_input(grads[0], d.setPressureRHSdivGrad, name, "PRESSURE_RHS_DIV_GRAD")
+        if is_non_ortho:
+            d.CreateAGrad()
         d.CreatePressureRHSGrad()
         d.CreateVelocityGradOnBoundaries()
+        if is_non_ortho:
+            d.CreatePressureResultGrad()
         d.UpdateDomainData()
         PISOtorch.SetupPressureRHSdivGrad(d, dt, non_ortho_flags, use_face_transform, timeStepNorm)
 
-    fn = _stage(name, tracked, [], fwd, bwd, lambda d: d.pressureRHSdiv)
+    fn = _stage(name, tracked, saved, fwd, bwd, lambda d: d.pressureRHSdiv)
     _apply(fn, domain, time_step, tracked)
 
 

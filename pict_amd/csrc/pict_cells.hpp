@@ -10,15 +10,17 @@
 #include <hip/hip_runtime.h>
 
 #include "pict_device.hpp"
+#include "pict_nonortho.hpp"
 
 namespace pict {
 
 // ---------------------------------------------------------------------------
-// PISO_build_matrix :3601-3865 (velocity matrix C and its diagonal A),
-// orthogonal terms.  Values only; the pattern is written by k_csr_pattern.
+// PISO_build_matrix :3601-3865 (velocity matrix C and its diagonal A).  Values
+// only; the pattern is written by k_csr_pattern.  NO: the non-orthogonal matrix
+// terms of `flags` are included (instances without them carry none of that code).
 // ---------------------------------------------------------------------------
-template <typename R, int D>
-__device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile, R dt) {
+template <typename R, int D, bool NO = false>
+__device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile, R dt, int flags = 0) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, tile, b, bi, f)) return;
@@ -58,6 +60,15 @@ __device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile,
             const R vc = (alpha_p[dim] * visc_p + alpha * visc_n) * R(0.5);
             diag += vc;
             rv[face + 1] -= vc;
+            if constexpr (NO && D > 1) {
+                if (flags & kNonOrthoMatrix) {
+                    R ai[2 * D * (D - 1)];
+                    interp_nonortho<R, D>(dom, bi, pos, true, false, ai);
+                    rv[0] = diag;
+                    nonortho_matrix_face<R, D>(dom, bi, pos, face, ai, flags, R(-1), GD_VELOCITY, false, rv);
+                    diag = rv[0];
+                }
+            }
         } else {
             const int slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
             diag += R((1 - slip) * 2) * visc_p * alpha_p[dim];
@@ -76,9 +87,10 @@ __device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile,
     dom.A[g] = rv[0];
 }
 
-// kPISO_build_advection_RHS :4281-4385 (orthogonal)
-template <typename R, int D>
-__device__ __forceinline__ void adv_rhs_cell(const DomainK<R> &dom, int tile, R dt, int apply_pg) {
+// kPISO_build_advection_RHS :4281-4385 (NO: with the explicit non-orthogonal terms of `flags`, from the
+// velocity result)
+template <typename R, int D, bool NO = false>
+__device__ __forceinline__ void adv_rhs_cell(const DomainK<R> &dom, int tile, R dt, int apply_pg, int flags = 0) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, tile, b, bi, f)) return;
@@ -86,12 +98,12 @@ __device__ __forceinline__ void adv_rhs_cell(const DomainK<R> &dom, int tile, R 
     const R det = cell_det<R, D>(b, pos);
     R pg[D];
     if (apply_pg) pressure_gradient<R, D>(dom, b, pos, pg);
-#pragma unroll
     for (int d = 0; d < D; ++d) {
         I4 t = pos;
         t.a[3] = d;
         R r = det * reinterpret_cast<const R *>(b->velocity)[flat4(t, b->stride)] / dt;
         r += fixed_boundary_source<R, D>(dom, b, pos, d);
+        if constexpr (NO && D > 1) r -= nonortho_rhs<R, D>(dom, bi, t, flags, GD_VELOCITY_RESULT, true, false);
         r /= det;
         r += velocity_source<R>(b, t);
         if (apply_pg) r -= pg[d] * dt;
@@ -99,9 +111,9 @@ __device__ __forceinline__ void adv_rhs_cell(const DomainK<R> &dom, int tile, R 
     }
 }
 
-// PISO_build_pressure_matrix :4797-4963 (orthogonal, cell metrics)
-template <typename R, int D>
-__device__ __forceinline__ void p_matrix_cell(const DomainK<R> &dom, int tile) {
+// PISO_build_pressure_matrix :4797-4963 (cell metrics; NO: with the non-orthogonal matrix terms of `flags`)
+template <typename R, int D, bool NO = false>
+__device__ __forceinline__ void p_matrix_cell(const DomainK<R> &dom, int tile, int flags = 0) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, tile, b, bi, f)) return;
@@ -135,6 +147,13 @@ __device__ __forceinline__ void p_matrix_cell(const DomainK<R> &dom, int tile) {
         rv[0] -= coef;
         rv[face + 1] += coef;
         idx[face + 1] = gn;
+        if constexpr (NO && D > 1) {
+            if (flags & kNonOrthoMatrix) {
+                R ai[2 * D * (D - 1)];
+                interp_nonortho<R, D>(dom, bi, pos, false, true, ai);
+                nonortho_matrix_face<R, D>(dom, bi, pos, face, ai, flags, R(1), GD_IS_FIXED, true, rv);
+            }
+        }
     }
     const int start = dom.csr_row[g], end = dom.csr_row[g + 1];
     for (int i = 0; i < end - start; ++i) {
@@ -193,6 +212,19 @@ __device__ __forceinline__ void divergence_cell(const DomainK<R> &dom, int tile,
     for (int d = 0; d < D; ++d) div += fl[2 * d + 1] - fl[2 * d];
     if (tsn) div /= dt;
     out[f + b->global_offset] = div;
+}
+
+// k_pressureRHSaddNonOrthoComponents :5455-5477: pressure_rhs_div += the explicit non-orthogonal part of the
+// pressure Laplacian from the pressure result
+template <typename R, int D>
+__device__ __forceinline__ void p_rhs_nonortho_cell(const DomainK<R> &dom, int tile, int flags) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const int g = f + b->global_offset;
+    R S = dom.prhs_div[g];
+    if constexpr (D > 1) S += nonortho_rhs<R, D>(dom, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, false, true);
+    dom.prhs_div[g] = S;
 }
 
 // PISO_update_velocity :5947-5980

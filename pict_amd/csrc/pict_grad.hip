@@ -5,11 +5,11 @@
 // tensors are written, and whether a kernel assigns (=) or accumulates (+=).
 // Scatters to cells other than the thread's own use device-scope float
 // atomics (the gradient tensors are zero-initialised by the caller, as in
-// the reference's Create*Grad functions).  Orthogonal terms only, like the
-// forward path.
+// the reference's Create*Grad functions).
 #include <hip/hip_runtime.h>
 
 #include "pict_host.hpp"
+#include "pict_nonortho.hpp"
 
 namespace pict {
 
@@ -246,12 +246,262 @@ __device__ __forceinline__ void fixed_boundary_source_grad(const DomainK<R> &dom
 }
 
 // ---------------------------------------------------------------------------
-// PISO_build_matrix_GRAD :3869-4075 (orthogonal): C_grad, A_grad ->
-// block velocity grad, viscosity grads.
+// Non-orthogonal terms: VJPs of the helpers in pict_nonortho.hpp
 // ---------------------------------------------------------------------------
+
+// scatterBlockData :2340-2347 into the result gradients (p.w = component)
+template <typename R>
+__device__ __forceinline__ void scatter_block_data(const DomainK<R> &dom, int bi, const I4 &p, int type, R g) {
+    R *dst = type == GD_VELOCITY_RESULT ? dom.vres_grad : type == GD_PRESSURE_RESULT ? dom.pres_grad : nullptr;
+    if (dst) add_to(dst + (size_t)p.a[3] * dom.N + global_index(p, dom.blocks + bi), g);
+}
+// scatterFixedBoundaryData :2448-2478 (FixedBoundary velocity gradients)
+template <typename R>
+__device__ __forceinline__ void scatter_fixed_data(const pict_block_desc *b, I4 p, int face, int type, R g) {
+    const pict_boundary_desc *fb = &b->bounds[face];
+    if (fb->type != PICT_BOUNDARY_FIXED || !(type == GD_VELOCITY_RESULT || type == GD_VELOCITY)) return;
+    if (!fb->velocity_grad) return;
+    R *dst = reinterpret_cast<R *>(fb->velocity_grad);
+    if (fb->velocity_static) {
+        add_to(dst + p.a[3], g);
+        return;
+    }
+    p.a[face >> 1] = 0;
+    add_to(dst + flat4(p, fb->stride), g);
+}
+// scatterBlockDataNeighbor :2560-2602 (values of FIXED faces get no gradient)
 template <typename R, int D>
+__device__ __forceinline__ void scatter_data_neighbor(const DomainK<R> &dom, int bi, const I4 &pos, int dir, int type,
+                                                      R g) {
+    int tb = bi;
+    I4 t = pos;
+    if (step_data<D>(dom.blocks, tb, t, dir)) scatter_block_data(dom, tb, t, type, g);
+}
+// scatterBlockDataNeighborDiagonal :2659-2700
+template <typename R, int D>
+__device__ __forceinline__ void scatter_data_neighbor_diag(const DomainK<R> &dom, int bi, const I4 &pos, int dir1,
+                                                           int dir2, int type, R g) {
+    const bool d1e = prescribed(dir1, dom.blocks + bi);
+    int tb = bi;
+    I4 t = pos;
+    if (!step_data<D>(dom.blocks, tb, t, d1e ? dir2 : dir1)) return;
+    if (!step_data<D>(dom.blocks, tb, t, d1e ? dir1 : dir2)) return;
+    scatter_block_data(dom, tb, t, type, g);
+}
+// scatterCornerValue_GRAD :2869-2966: gradient g of a corner value whose forward walk averaged `num` cells
+template <typename R, int D>
+__device__ __forceinline__ void scatter_corner_value(const DomainK<R> &dom, int bi, const I4 &pos, int dir1, int dir2,
+                                                     bool inc1, int type, R g, int num) {
+    const bool from_cells = num > 0;
+    R dg = g;
+    if (from_cells) dg /= (R)num;
+    Walker w[2] = {{dir1, dir2, bi, pos}, {dir2, dir1, bi, pos}};
+    for (int depth = 1; depth <= 2; ++depth) {
+        for (int d = 0; d < 2; ++d) {
+            if (!walk_step<D>(dom.blocks, w[d])) {
+                const pict_block_desc *wb = dom.blocks + w[d].bi;
+                if (at_bound(w[d].pos, w[d].d2, wb)) {
+                    scatter_fixed_data<R>(wb, w[d].pos, w[d].d1, type, dg);
+                } else {
+                    I4 np = w[d].pos;
+                    np.a[w[d].d2 >> 1] += face_sign(w[d].d2);
+                    dg *= R(0.5);
+                    scatter_fixed_data<R>(wb, w[d].pos, w[d].d1, type, dg);
+                    scatter_fixed_data<R>(wb, np, w[d].d1, type, dg);
+                }
+                return;
+            }
+            if (w[d].bi == w[d ^ 1].bi && i4_eq(w[d].pos, w[d ^ 1].pos)) return;
+            if (from_cells && (depth > 1 || inc1)) scatter_block_data(dom, w[d].bi, w[d].pos, type, dg);
+        }
+    }
+}
+
+// scatterNonOrthoLaplaceComponents_GRAD :1989-2098: gradients of the interpolated cross coefficients (layout of
+// interp_nonortho) to the viscosities (UniformAcc u for the workgroup's own uniform targets) and A.
+template <typename R, int D>
+__device__ __forceinline__ void scatter_interp_nonortho(const DomainK<R> &dom, int bi, const I4 &pos, const R *ai_grad,
+                                                        bool with_visc, bool with_a, UniformAcc<R> &u) {
+    const pict_block_desc *b = dom.blocks + bi;
+    R aP[D * (D + 1) / 2];
+    lap_full<R, D>(b, pos, aP);
+    const R one = 1;
+    R raP_grad = 0;
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        if (!interp_face_live(b, pos, face)) continue;
+        const int axis = face >> 1;
+        int nbi;
+        I4 pn;
+        R aN[D];
+        interp_neighbor<R, D>(dom.blocks, bi, pos, face, nbi, pn, aN);
+        const pict_block_desc *nb = dom.blocks + nbi;
+        R raN_grad = 0;
+#pragma unroll
+        for (int i = 1; i < D; ++i) {
+            const int oa = (axis + i) % D;
+            const R g = ai_grad[face * (D - 1) + i - 1];
+            raP_grad += aP[lap_ci(axis, oa)] * R(0.5) * g;
+            raN_grad += aN[oa] * R(0.5) * g;
+        }
+        const int gn = global_index(pn, nb);
+        const R raN = with_a ? one / dom.A[gn] : one;
+        if (with_visc) scatter_viscosity<R>(dom, nb, pn, with_a ? raN_grad * raN : raN_grad, b, u);
+        if (with_a && dom.A_grad) {
+            if (with_visc) raN_grad *= viscosity(dom, nb, pn);
+            add_to(dom.A_grad + gn, -raN_grad * raN * raN);
+        }
+    }
+    const int g = global_index(pos, b);
+    const R raP = with_a ? one / dom.A[g] : one;
+    if (with_visc) scatter_viscosity<R>(dom, b, pos, with_a ? raP_grad * raP : raP_grad, b, u);
+    if (with_a && dom.A_grad) {
+        if (with_visc) raP_grad *= viscosity(dom, b, pos);
+        add_to(dom.A_grad + g, -raP_grad * raP * raP);
+    }
+}
+
+// VJP of nonortho_matrix_face (PISO_build_matrix_GRAD :3948-4019, PISO_build_pressure_matrix_GRAD :5038-5104):
+// accumulates into ai_grad the gradient of the face's cross coefficients from the row gradient cg (slot order,
+// cg[0] the diagonal's).
+template <typename R, int D>
+__device__ __forceinline__ void nonortho_matrix_face_grad(const DomainK<R> &dom, int bi, const I4 &pos, int face,
+                                                          const R *ai, int flags, R sign, int corner_type,
+                                                          bool grad_always, const R *cg, R *ai_grad) {
+    const int dim = face >> 1, fs = face_sign(face);
+    const bool nbrs = flags & PICT_NON_ORTHO_DIRECT_MATRIX, ctr = flags & PICT_NON_ORTHO_CENTER_MATRIX;
+#pragma unroll
+    for (int i = 1; i < D; ++i) {
+        const int ta = (dim + i) % D;
+        if (ai[face * (D - 1) + i - 1] == R(0)) continue;
+        R ag = 0;
+#pragma unroll
+        for (int tu = 0; tu < 2; ++tu) {
+            const int tf = (ta << 1) | tu, tfs = face_sign(tf);
+            const Corner<R> cv = corner_value<R, D>(dom, bi, pos, face, tf, false, false, corner_type);
+            if (cv.num < 1) {
+                if (grad_always || cv.btype == PICT_BC_NEUMANN) {
+                    R vg = 0;
+                    if (ctr) vg += 3 * cg[0];
+                    if (nbrs) {
+                        vg += 3 * cg[face + 1];
+                        vg -= cg[(tf ^ 1) + 1];
+                    }
+                    ag += (R)(fs * tfs) * R(0.25) * (sign * vg);
+                }
+            } else {
+                const R norm = (R)(1.0 / (double)cv.num);
+                R vg = 0;
+                if (ctr) vg += cg[0];
+                if (nbrs) {
+                    vg += cg[face + 1];
+                    vg += cg[tf + 1];
+                }
+                ag += (R)(fs * tfs) * norm * (sign * vg);
+            }
+        }
+        ai_grad[face * (D - 1) + i - 1] += ag;
+    }
+}
+
+// scatterNonOrthoLaplaceRHS_v2_GRAD :3210-3380 (cell metrics): VJP of nonortho_rhs for the output gradient s_grad
+// into the result gradient of `type`, FixedBoundary velocity gradients, viscosities and A.
+template <typename R, int D>
+__device__ __forceinline__ void nonortho_rhs_grad(const DomainK<R> &dom, int bi, const I4 &pos, int flags, int type,
+                                                  bool with_visc, bool with_a, R s_grad, UniformAcc<R> &u) {
+    if (!(flags & kNonOrthoRhs)) return;
+    const pict_block_desc *b = dom.blocks + bi;
+    const bool direct = flags & PICT_NON_ORTHO_DIRECT_RHS;
+    R aP[D * (D + 1) / 2];
+    lap_full<R, D>(b, pos, aP);
+    const R one = 1;
+    const int g = global_index(pos, b);
+    const R raP = with_a ? one / dom.A[g] : one;
+    const R viscP = with_visc ? viscosity(dom, b, pos) : one;
+    const R rvP = raP * viscP;
+    const bool diffable = with_visc || with_a;
+    R raP_grad = 0;
+    for (int face = 0; face < 2 * D; ++face) {
+        const int axis = face >> 1, fs = face_sign(face);
+        if (at_bound(pos, face, b) && prescribed(face, b)) {
+            if (fixed_type(b, face, type) == PICT_BC_NEUMANN || type == GD_PRESSURE_RESULT) continue;
+            R bA[D];
+            lap_neighbour_boundary<R, D>(&b->bounds[face], pos, axis, bA);
+#pragma unroll
+            for (int i = 1; i < D; ++i) {
+                const int ta = (axis + i) % D;
+                const bool tlo = pos.a[ta] == 0, thi = pos.a[ta] == b->size[ta] - 1;
+                I4 lo = pos, hi = pos;
+                if (!tlo) lo.a[ta] -= 1;
+                if (!thi) hi.a[ta] += 1;
+                const R df = (tlo || thi) ? R(1) : R(0.5);
+                const R tgg = -(R)fs * bA[ta] * s_grad * df;
+                if (with_visc) {
+                    const R tdg = fixed_data<R>(b, hi, face, type) - fixed_data<R>(b, lo, face, type);
+                    scatter_viscosity<R>(dom, b, pos, tgg * tdg, b, u);
+                }
+                const R bv = with_visc ? viscosity(dom, b, pos) : one;
+                scatter_fixed_data<R>(b, hi, face, type, tgg * bv);
+                scatter_fixed_data<R>(b, lo, face, type, -tgg * bv);
+            }
+            continue;
+        }
+        const NCell n = resolve_neighbor<D>(dom.blocks, bi, pos, face);
+        R aN[D];
+        lap_single_axis_neighbor<R, D>(dom.blocks, n, face, aN);
+        const pict_block_desc *nb = dom.blocks + n.bi;
+        const int gn = global_index(n.pos, nb);
+        const R raN = with_a ? one / dom.A[gn] : one;
+        const R viscN = with_visc ? viscosity(dom, nb, n.pos) : one;
+        const R rvN = raN * viscN;
+        R raN_grad = 0;
+#pragma unroll
+        for (int i = 1; i < D; ++i) {
+            const int ta = (axis + i) % D;
+            const R fa = (aP[lap_ci(axis, ta)] * rvP + aN[ta] * rvN) * R(0.5);
+            const R tgg = -s_grad * (R)fs * fa;
+            R tdg = 0;
+#pragma unroll
+            for (int tu = 0; tu < 2; ++tu) {
+                const int tf = (ta << 1) | tu, tfs = face_sign(tf);
+                const Corner<R> cv = corner_value<R, D>(dom, bi, pos, face, tf, false, direct, type);
+                if (cv.num == 0 && (cv.btype == PICT_BC_NEUMANN || type == GD_PRESSURE_RESULT)) {
+                    const int to = tf ^ 1, tofs = -tfs;
+                    if (direct) {
+                        if (diffable) tdg += (R)tfs * data_neighbor<R, D>(dom, bi, pos, face, type) * R(0.75);
+                        scatter_data_neighbor<R, D>(dom, bi, pos, face, type, tgg * (R)tfs * R(0.75));
+                        if (diffable) tdg += (R)tofs * data_neighbor<R, D>(dom, bi, pos, to, type) * R(0.25);
+                        scatter_data_neighbor<R, D>(dom, bi, pos, to, type, tgg * (R)tofs * R(0.25));
+                    }
+                    if (flags & PICT_NON_ORTHO_DIAGONAL_RHS) {
+                        if (diffable) tdg += (R)tofs * data_neighbor_diag<R, D>(dom, bi, pos, face, to, type) * R(0.25);
+                        scatter_data_neighbor_diag<R, D>(dom, bi, pos, face, to, type, tgg * (R)tofs * R(0.25));
+                    }
+                } else {
+                    if (diffable) tdg += (R)tfs * cv.data;
+                    scatter_corner_value<R, D>(dom, bi, pos, face, tf, direct, type, tgg * (R)tfs, cv.num);
+                }
+            }
+            if (diffable) {
+                const R fag = -s_grad * (R)fs * tdg;
+                raP_grad += aP[lap_ci(axis, ta)] * R(0.5) * fag;
+                raN_grad += aN[ta] * R(0.5) * fag;
+            }
+        }
+        if (with_visc) scatter_viscosity<R>(dom, nb, n.pos, raN_grad * raN, b, u);
+        if (with_a && dom.A_grad) add_to(dom.A_grad + gn, raN_grad * viscN * (-raN * raN));
+    }
+    if (with_visc) scatter_viscosity<R>(dom, b, pos, raP_grad * raP, b, u);
+    if (with_a && dom.A_grad) add_to(dom.A_grad + g, raP_grad * viscP * (-raP * raP));
+}
+
+// ---------------------------------------------------------------------------
+// PISO_build_matrix_GRAD :3869-4075: C_grad, A_grad -> block velocity grad,
+// viscosity grads (NO: with the non-orthogonal matrix terms of `flags`).
+// ---------------------------------------------------------------------------
+template <typename R, int D, bool NO>
 __device__ __forceinline__ void adv_matrix_grad_cell(const DomainK<R> &dom, const pict_block_desc *b, int bi, int f,
-                                                     UniformAcc<R> &u) {
+                                                     int flags, UniformAcc<R> &u) {
     const I4 pos = unflat(f, b);
     const int g = b->global_offset + f;
     const R rdet = R(1) / cell_det<R, D>(b, pos);
@@ -270,6 +520,14 @@ __device__ __forceinline__ void adv_matrix_grad_cell(const DomainK<R> &dom, cons
     }
     R fg[2 * D];
     R visc_grad = 0;
+    constexpr int NA = NO && D > 1 ? 2 * D * (D - 1) : 1;
+    R ai[NA], ai_grad[NA];
+    const bool no = NO && D > 1 && (flags & kNonOrthoMatrix);
+    if constexpr (NO && D > 1) {
+        if (no) interp_nonortho<R, D>(dom, bi, pos, false, false, ai);
+#pragma unroll
+        for (int k = 0; k < NA; ++k) ai_grad[k] = 0;
+    }
 #pragma unroll
     for (int face = 0; face < 2 * D; ++face) {
         const int dim = face >> 1;
@@ -284,30 +542,44 @@ __device__ __forceinline__ void adv_matrix_grad_cell(const DomainK<R> &dom, cons
             visc_grad += R(0.5) * alpha_p[dim] * vc_grad;
             t.a[3] = 0;
             scatter_viscosity<R>(dom, nbp, t, R(0.5) * alpha * vc_grad, b, u);
+            if constexpr (NO && D > 1) {
+                if (no) {
+                    R cgd[7];
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) cgd[k] = cg[k];
+                    cgd[0] = diag_grad;
+                    nonortho_matrix_face_grad<R, D>(dom, bi, pos, face, ai, flags, R(-1), GD_VELOCITY, false, cgd,
+                                                    ai_grad);
+                }
+            }
         } else {
             fg[face] = 0;
             const R slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? R(0) : R(1);
             visc_grad += (R(1) - slip) * R(2) * alpha_p[dim] * diag_grad;
         }
     }
+    if constexpr (NO && D > 1) {
+        if (no) scatter_interp_nonortho<R, D>(dom, bi, pos, ai_grad, true, false, u);
+    }
     if (b->velocity_grad) scatter_fluxes<R, D>(dom, b, pos, fg, nullptr);
     scatter_viscosity<R>(dom, b, pos, visc_grad, b, u);
 }
 
-template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_grad(DomainK<R> dom, R dt) {
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_grad(DomainK<R> dom, R dt, int flags) {
     const pict_block_desc *b;
     int bi, f;
     (void)dt;
     UniformAcc<R> u;
-    if (tile_cell(dom, b, bi, f)) adv_matrix_grad_cell<R, D>(dom, b, bi, f, u);
+    if (tile_cell(dom, b, bi, f)) adv_matrix_grad_cell<R, D, NO>(dom, b, bi, f, flags, u);
     flush_uniform<R, D>(dom, b, u, false);
 }
 
 // kPISO_build_advection_RHS_GRAD :4388-4476: velocity_rhs_grad -> velocity grad (=),
-// velocity source grad, fixed boundary velocity and viscosity grads.
-template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs_grad(DomainK<R> dom, R dt) {
+// velocity source grad, fixed boundary velocity and viscosity grads (NO: and the VJP of the
+// explicit non-orthogonal terms of `flags`, into velocity_result_grad).
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs_grad(DomainK<R> dom, R dt, int flags) {
     const pict_block_desc *b;
     int bi, f;
     UniformAcc<R> u;
@@ -315,13 +587,13 @@ __global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs_grad(DomainK<R> dom, 
         const I4 pos = unflat(f, b);
         const int g = b->global_offset + f;
         const R det = cell_det<R, D>(b, pos);
-#pragma unroll
         for (int d = 0; d < D; ++d) {
             I4 t = pos;
             t.a[3] = d;
             R G = dom.vrhs_grad[(size_t)d * dom.N + g];
             scatter_velocity_source<R>(b, t, G, u);
             G /= det;
+            if constexpr (NO && D > 1) nonortho_rhs_grad<R, D>(dom, bi, t, flags, GD_VELOCITY_RESULT, true, false, -G, u);
             fixed_boundary_source_grad<R, D>(dom, b, pos, d, G, u);
             G *= det;
             if (b->velocity_grad) reinterpret_cast<R *>(b->velocity_grad)[flat4(t, b->stride)] = G / dt;
@@ -330,9 +602,10 @@ __global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs_grad(DomainK<R> dom, 
     flush_uniform<R, D>(dom, b, u, true);
 }
 
-// PISO_build_pressure_matrix_GRAD :4967-5115 (orthogonal): P_grad -> A_grad (+=).
-template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_grad(DomainK<R> dom) {
+// PISO_build_pressure_matrix_GRAD :4967-5115: P_grad -> A_grad (+=) (NO: with the non-orthogonal
+// matrix terms of `flags`).
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_grad(DomainK<R> dom, int flags) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, b, bi, f)) return;
@@ -342,9 +615,24 @@ __global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_grad(DomainK<R> dom)
     slot_positions<R, D>(dom, b, bi, pos, g, at);
     const R pg0 = dom.P_grad[at[0]];
     R raP_grad = 0;
+    constexpr int NA = NO && D > 1 ? 2 * D * (D - 1) : 1;
+    R ai[NA], ai_grad[NA], pcg[7];
+    const bool no = NO && D > 1 && (flags & kNonOrthoMatrix);
+    if constexpr (NO && D > 1) {
+        if (no) {
+            interp_nonortho<R, D>(dom, bi, pos, false, false, ai);
+#pragma unroll
+            for (int k = 0; k < 7; ++k) pcg[k] = at[k] >= 0 ? dom.P_grad[at[k]] : R(0);
+        }
+#pragma unroll
+        for (int k = 0; k < NA; ++k) ai_grad[k] = 0;
+    }
 #pragma unroll
     for (int face = 0; face < 2 * D; ++face) {
         if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+        if constexpr (NO && D > 1) {
+            if (no) nonortho_matrix_face_grad<R, D>(dom, bi, pos, face, ai, flags, R(1), GD_IS_FIXED, true, pcg, ai_grad);
+        }
         const int dim = face >> 1;
         int nb;
         I4 t = neighbor_cell<D>(dom.blocks, b, pos, face, bi, nb);
@@ -362,6 +650,24 @@ __global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_grad(DomainK<R> dom)
     }
     const R raP = R(1) / dom.A[g];
     add_to(dom.A_grad + g, -raP_grad * raP * raP);
+    if constexpr (NO && D > 1) {
+        if (no) {
+            UniformAcc<R> u;  // no viscosity targets here
+            scatter_interp_nonortho<R, D>(dom, bi, pos, ai_grad, false, true, u);
+        }
+    }
+}
+
+// k_pressureRHSaddNonOrthoComponents_GRAD :5479-5494: pressure_rhs_div_grad -> pressure_result_grad (+=), A_grad (+=)
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_rhs_nonortho_grad(DomainK<R> dom, int flags) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, b, bi, f)) return;
+    UniformAcc<R> u;  // no viscosity targets here
+    if constexpr (D > 1)
+        nonortho_rhs_grad<R, D>(dom, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, false, true,
+                                dom.prhs_div_grad[b->global_offset + f], u);
 }
 
 // k_computePressureRHSdivergenceFromFlux_GRAD :5423-5452: pressure_rhs_div_grad ->
@@ -527,9 +833,11 @@ extern "C" {
 
 pict_status pict_setup_advection_matrix_grad(const pict_atlas *a, double dt, int32_t flags, void *stream) {
     PICT_CHECK_ARG(a && a->desc.C_grad && a->desc.A_grad, "C_grad/A_grad missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
-    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_grad<R, D>), a->view<R>(), (R)dt));
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_grad<R, D, true>), a->view<R>(), (R)dt,
+                                           (int)flags));
+    else
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_grad<R, D, false>), a->view<R>(), (R)dt, 0));
     return check_launch();
 }
 
@@ -537,9 +845,13 @@ pict_status pict_setup_advection_velocity_grad(const pict_atlas *a, double dt, i
                                                void *stream) {
     (void)apply_pg;  // the reference's VJP omits the pressure-gradient term (:4388-4476)
     PICT_CHECK_ARG(a && a->desc.velocity_rhs_grad, "velocity_rhs_grad missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
-    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_rhs_grad<R, D>), a->view<R>(), (R)dt));
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
+        PICT_CHECK_ARG(a->desc.velocity_result && a->desc.velocity_result_grad,
+                       "velocity_result / its gradient missing (non-orthogonal RHS terms)");
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_rhs_grad<R, D, true>), a->view<R>(), (R)dt, (int)flags));
+    } else {
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_rhs_grad<R, D, false>), a->view<R>(), (R)dt, 0));
+    }
     return check_launch();
 }
 
@@ -547,24 +859,31 @@ pict_status pict_setup_pressure_matrix_grad(const pict_atlas *a, double dt, int3
                                             void *stream) {
     (void)dt;
     PICT_CHECK_ARG(a && a->desc.P_grad && a->desc.A_grad && a->desc.A, "P_grad/A_grad missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
     if (use_face_transform) {
         set_error("useFaceTransform gradients are not implemented (nor in the reference)");
         return PICT_ERR_UNSUPPORTED;
     }
-    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_grad<R, D>), a->view<R>()));
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_grad<R, D, true>), a->view<R>(), (int)flags));
+    else
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_grad<R, D, false>), a->view<R>(), 0));
     return check_launch();
 }
 
 pict_status pict_setup_pressure_rhs_div_grad(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
                                              int32_t tsn, void *stream) {
     PICT_CHECK_ARG(a && a->desc.pressure_rhs_div_grad && a->desc.pressure_rhs_grad, "pressure rhs grads missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
     if (use_face_transform) {
         set_error("useFaceTransform gradients are not implemented (nor in the reference)");
         return PICT_ERR_UNSUPPORTED;
+    }
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
+        // k_pressureRHSaddNonOrthoComponents_GRAD first (_SetupPressureRHS_GRAD :5792-5803)
+        PICT_CHECK_ARG(a->desc.pressure_result && a->desc.pressure_result_grad && a->desc.A && a->desc.A_grad,
+                       "pressure_result / its gradient or A_grad missing (non-orthogonal RHS terms)");
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs_nonortho_grad<R, D>), a->view<R>(), (int)flags));
+        pict_status st = check_launch();
+        if (st) return st;
     }
     PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_div_grad<R, D>), a->view<R>(), (R)dt, (int)tsn));
     return check_launch();

@@ -61,15 +61,6 @@ inline pict_status check_launch() {
     return PICT_OK;
 }
 
-// Non-orthogonal correction terms (PISO_multiblock_cuda.h flags) are not implemented.
-inline pict_status check_non_ortho(int flags) {
-    if (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX | PICT_NON_ORTHO_DIRECT_RHS |
-                 PICT_NON_ORTHO_DIAGONAL_RHS | PICT_NON_ORTHO_DIAGONAL_MATRIX)) {
-        set_error("non-orthogonal correction terms (flags %d) are not implemented by libpict_hip", flags);
-        return PICT_ERR_UNSUPPORTED;
-    }
-    return PICT_OK;
-}
 
 }  // namespace pict
 

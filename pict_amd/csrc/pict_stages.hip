@@ -118,20 +118,26 @@ __global__ __launch_bounds__(kCellsPerTile) void k_csr_pattern(DomainK<R> dom) {
     }
 }
 
-// Forward stage kernels: one workgroup per tile of the domain (bodies in pict_cells.hpp).
-template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix(DomainK<R> dom, R dt) {
-    adv_matrix_cell<R, D>(dom, (int)blockIdx.x, dt);
+// Forward stage kernels: one workgroup per tile of the domain (bodies in pict_cells.hpp).  NO: the instance with
+// the non-orthogonal terms of `flags` (the orthogonal instance carries none of their code).
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix(DomainK<R> dom, R dt, int flags) {
+    adv_matrix_cell<R, D, NO>(dom, (int)blockIdx.x, dt, flags);
+}
+
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs(DomainK<R> dom, R dt, int apply_pg, int flags) {
+    adv_rhs_cell<R, D, NO>(dom, (int)blockIdx.x, dt, apply_pg, flags);
+}
+
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix(DomainK<R> dom, int flags) {
+    p_matrix_cell<R, D, NO>(dom, (int)blockIdx.x, flags);
 }
 
 template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs(DomainK<R> dom, R dt, int apply_pg) {
-    adv_rhs_cell<R, D>(dom, (int)blockIdx.x, dt, apply_pg);
-}
-
-template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix(DomainK<R> dom) {
-    p_matrix_cell<R, D>(dom, (int)blockIdx.x);
+__global__ __launch_bounds__(kCellsPerTile) void k_p_rhs_nonortho(DomainK<R> dom, int flags) {
+    p_rhs_nonortho_cell<R, D>(dom, (int)blockIdx.x, flags);
 }
 
 template <typename R, int D>
@@ -348,18 +354,23 @@ pict_status pict_build_csr_pattern(const pict_atlas *a, void *stream) {
 
 pict_status pict_setup_advection_matrix(const pict_atlas *a, double dt, int32_t flags, void *stream) {
     PICT_CHECK_ARG(a && a->desc.C_value && a->desc.A, "C/A missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
-    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix<R, D>), a->view<R>(), (R)dt));
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix<R, D, true>), a->view<R>(), (R)dt, (int)flags));
+    else
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix<R, D, false>), a->view<R>(), (R)dt, 0));
     return check_launch();
 }
 
 pict_status pict_setup_advection_velocity(const pict_atlas *a, double dt, int32_t flags, int32_t apply_pg,
                                           void *stream) {
     PICT_CHECK_ARG(a && a->desc.velocity_rhs, "velocity_rhs missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
-    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_rhs<R, D>), a->view<R>(), (R)dt, (int)apply_pg));
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
+        PICT_CHECK_ARG(a->desc.velocity_result && a->desc.A, "velocity_result missing (non-orthogonal RHS terms)");
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_rhs<R, D, true>), a->view<R>(), (R)dt, (int)apply_pg,
+                                           (int)flags));
+    } else {
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_rhs<R, D, false>), a->view<R>(), (R)dt, (int)apply_pg, 0));
+    }
     return check_launch();
 }
 
@@ -367,21 +378,20 @@ pict_status pict_setup_pressure_matrix(const pict_atlas *a, double dt, int32_t f
                                        void *stream) {
     (void)dt;
     PICT_CHECK_ARG(a && a->desc.P_value && a->desc.A, "P/A missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
     if (use_face_transform) {
         set_error("useFaceTransform is not implemented by libpict_hip");
         return PICT_ERR_UNSUPPORTED;
     }
-    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix<R, D>), a->view<R>()));
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix<R, D, true>), a->view<R>(), (int)flags));
+    else
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix<R, D, false>), a->view<R>(), 0));
     return check_launch();
 }
 
 pict_status pict_setup_pressure_rhs_div(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
                                         int32_t tsn, void *stream) {
     PICT_CHECK_ARG(a && a->desc.pressure_rhs && a->desc.pressure_rhs_div, "pressure rhs missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
     if (use_face_transform) {
         set_error("useFaceTransform is not implemented by libpict_hip");
         return PICT_ERR_UNSUPPORTED;
@@ -389,14 +399,18 @@ pict_status pict_setup_pressure_rhs_div(const pict_atlas *a, double dt, int32_t 
     PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_divergence<R, D>), a->view<R>(),
                                        (const R *)a->desc.pressure_rhs, (R *)a->desc.pressure_rhs_div, (R)dt,
                                        (int)tsn));
+    pict_status st = check_launch();
+    if (st || a->desc.num_dims < 2 || !(flags & kNonOrthoRhs)) return st;
+    // k_pressureRHSaddNonOrthoComponents (:5455-5477, launched by SetupPressureRHSdiv :5659-5704)
+    PICT_CHECK_ARG(a->desc.pressure_result && a->desc.A, "pressure_result missing (non-orthogonal RHS terms)");
+    PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs_nonortho<R, D>), a->view<R>(), (int)flags));
     return check_launch();
 }
 
 pict_status pict_setup_pressure_rhs(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
                                     int32_t tsn, void *stream) {
     PICT_CHECK_ARG(a && a->desc.pressure_rhs && a->desc.C_value && a->desc.velocity_result, "pressure rhs inputs missing");
-    pict_status st = check_non_ortho(flags);
-    if (st) return st;
+    pict_status st;
     if (use_face_transform) {
         set_error("useFaceTransform is not implemented by libpict_hip");
         return PICT_ERR_UNSUPPORTED;
