@@ -331,6 +331,15 @@ pict_status pict_solve_refined(const pict_csr_desc *A, const void *rhs, double *
 pict_status pict_solver_kernel_name(const pict_csr_desc *A, int32_t nrhs, const pict_solve_opts *opts, char *buf,
                                     int32_t len);
 
+/* pict_solve_linear without device allocation.  No counterpart in the reference: it is for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   Same arguments, results and errors as pict_solve_linear; the solver workspace and the final states live in a
+   64 MiB device array of the library (one per GPU, held until the process ends) instead of the workspace pool.
+   A solve whose workspace exceeds it (about 84 bytes per row for one FP32 right-hand side) returns
+   PICT_ERR_INVALID.  Calls that overlap on one GPU share that array and race: issue them one at a time. */
+pict_status pict_solve_linear_noalloc(const pict_csr_desc *A, const void *rhs, void *x, int32_t nrhs,
+                                      const pict_solve_opts *opts, pict_solve_info *infos_out, void *stream);
+
 /* Transpose map of a structurally symmetric pattern (no counterpart in the reference, which
    transposes inside cuSPARSE): perm_out[e] = position of entry (j,i) for entry e = (i,j).
    Returns PICT_ERR_UNSUPPORTED if some (j,i) is missing.  Synchronises `stream`. */

@@ -229,6 +229,8 @@ SIGNATURES = {
     "pict_copy_velocity_result_grad_from_blocks": (c_int32, [_P, _P]),
     "pict_copy_pressure_result_grad_from_blocks": (c_int32, [_P, _P]),
     "pict_solve_linear": (c_int32, [POINTER(CsrDesc), _P, _P, c_int32, POINTER(SolveOpts), POINTER(SolveInfo), _P]),
+    "pict_solve_linear_noalloc": (c_int32, [POINTER(CsrDesc), _P, _P, c_int32, POINTER(SolveOpts), POINTER(SolveInfo),
+                                            _P]),
     "pict_solve_refined": (c_int32, [POINTER(CsrDesc), _P, _P, POINTER(SolveOpts), c_double, c_int32, c_int32,
                                      POINTER(SolveInfo), _P]),
     "pict_solver_kernel_name": (c_int32, [POINTER(CsrDesc), c_int32, POINTER(SolveOpts), ctypes.c_char_p, c_int32]),

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libpict_hip.so")
-SOURCES = ["pict_stages.hip", "pict_solve.hip", "pict_metrics.hip", "pict_grad.hip", "pict_step.hip", "pict_batch.hip", "pict_matvec.hip"]
+SOURCES = ["pict_stages.hip", "pict_solve.hip", "pict_metrics.hip", "pict_grad.hip", "pict_step.hip", "pict_batch.hip", "pict_matvec.hip",
+           "pict_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PICT_OFFLOAD_ARCH", "gfx950")
