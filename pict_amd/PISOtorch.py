@@ -2106,21 +2106,23 @@ def SolveLinear(A, RHS, x, maxIterations=1000, tolerance=1e-8, convergenceCriter
 
 
 def SolveLinearRefined(A, RHS, x, maxIterations=1000, tolerance=1e-5, refineTolerance=1e-10, maxSweeps=8,
-                       convergenceCriterion=ConvergenceCriterion.NORM2_NORMALIZED, useBiCG=False, nullSpace=None):
+                       convergenceCriterion=ConvergenceCriterion.NORM2_NORMALIZED, useBiCG=False, nullSpace=None,
+                       residualResetSteps=0):
     """Mixed-precision solve (no counterpart in the reference): x is an FP64 CUDA vector (initial guess in,
     solution out); A and RHS are in the matrix dtype.  Sweeps of r = b - A x in FP64 and a correction solve in the
     matrix dtype (maxIterations / tolerance / criterion / useBiCG apply to each correction solve, whose right-hand
     side is r scaled to unit RMS) until ||r||_2/||b||_2 < refineTolerance.  nullSpace: True if the constant vector
     spans A's null space (its component of b and of the residuals is projected out), False if not, None to detect
     it from the row sums.  Returns one LinearSolverResultInfo: finalResidual is the relative FP64 residual,
-    usedIterations the correction iterations in total."""
+    usedIterations the correction iterations in total.  residualResetSteps applies to each correction solve."""
     n = A.getRows()
     _check(RHS.is_cuda and RHS.is_contiguous() and RHS.dim() == 1 and RHS.size(0) == n,
            "RHS must be a contiguous CUDA vector of the matrix size.")
     _check(RHS.dtype == A.getDtype(), "Data type of RHS must match A.")
     _check(x.is_cuda and x.is_contiguous() and x.dtype == torch.float64 and x.dim() == 1 and x.size(0) == n,
            "x must be a contiguous FP64 CUDA vector of the matrix size.")
-    opts = _solve_opts(maxIterations, tolerance, convergenceCriterion, useBiCG, False, 0, False, False, False, False)
+    opts = _solve_opts(maxIterations, tolerance, convergenceCriterion, useBiCG, False, residualResetSteps, False,
+                       False, False, False)
     info = _abi.SolveInfo()
     desc = A._desc()
     _abi.check(_abi.load_library().pict_solve_refined(ctypes.byref(desc), ctypes.c_void_p(RHS.data_ptr()),

@@ -518,20 +518,21 @@ class Simulation:
         if max_iter is None:
             max_iter = self.linear_solve_max_iterations
         if refine_tol is not None and not self.differentiable:
-            return self._linear_solve_refined(csrMat, rhs, x, use_BiCG, refine_tol, max_iter)
+            return self._linear_solve_refined(csrMat, rhs, x, use_BiCG, refine_tol, max_iter, residual_reset_step)
         return self.linear_solve_GPU(csrMat, rhs, x, use_BiCG, tol, max_iter, matrix_rank_deficient,
                                      residual_reset_step, return_best_result,
                                      double_fallback=self.solver_double_fallback,
                                      BiCG_with_preconditioner=self.preconditionBiCG,
                                      BiCG_precondition_fallback=self.BiCG_precondition_fallback)
 
-    def _linear_solve_refined(self, csrMat, rhs, x, use_BiCG, refine_tol, max_iter):
+    def _linear_solve_refined(self, csrMat, rhs, x, use_BiCG, refine_tol, max_iter, residual_reset_step=0):
         """FP64-refined solve: the FP64 solution rounded to the domain dtype."""
         x64 = torch.zeros(rhs.numel(), dtype=torch.float64, device=rhs.device) if x is None else \
             x.detach().to(torch.float64).contiguous()
         info = PISOtorch.SolveLinearRefined(csrMat, rhs.detach().contiguous(), x64, max_iter,
                                             self.pressure_refine_inner_tol, refine_tol,
-                                            self.pressure_refine_max_sweeps, useBiCG=use_BiCG)
+                                            self.pressure_refine_max_sweeps, useBiCG=use_BiCG,
+                                            residualResetSteps=residual_reset_step)
         if self.record_solver_infos:
             self.solver_infos.append(("BiCG" if use_BiCG else "CG", [info]))
         if not (info.converged and info.isFiniteResidual):

@@ -170,3 +170,80 @@ def sgs_cnn(channels=16, seed=0, dtype=torch.float32, device="cuda"):
     net = torch.nn.Sequential(Conv3dSame(3, channels), torch.nn.ReLU(), Conv3dSame(channels, channels),
                               torch.nn.ReLU(), Conv3dSame(channels, 3))
     return net.to(dtype=dtype, device=device)
+
+
+def tanh_radii(r1, r2, n, ratio=1.05):
+    """n+1 radii from r1 to r2 with a one-sided tanh stretch, r_j = r1 + (r2 - r1) (1 - tanh(s (1 - j/n)) / tanh(s)),
+    cells clustered at r1; the stretch s is chosen (bisection) so that the second cell is `ratio` times the first."""
+    def radii(s):
+        eta = torch.linspace(0.0, 1.0, n + 1, dtype=torch.float64)
+        return r1 + (r2 - r1) * (1.0 - torch.tanh(s * (1.0 - eta)) / math.tanh(s))
+
+    def first_ratio(s):
+        r = radii(s)
+        return float((r[2] - r[1]) / (r[1] - r[0]))
+
+    lo, hi = 1e-6, 20.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if first_ratio(mid) < ratio:
+            lo = mid
+        else:
+            hi = mid
+    return radii(0.5 * (lo + hi))
+
+
+def ogrid_coords(n_angle, radii, start_deg, span_deg=-90.0, dtype=torch.float64, device="cpu"):
+    """Vertex coordinates [1, 2, len(radii), n_angle+1] of one O-grid block around the origin: x along the angle
+    (from start_deg, spanning span_deg; negative = clockwise), y along the radius (the layout of the reference's
+    make_torus_2D blocks, lib/data/shapes.py:554-600)."""
+    ang = torch.deg2rad(torch.tensor(start_deg, dtype=torch.float64)
+                        + torch.linspace(0.0, span_deg, n_angle + 1, dtype=torch.float64))
+    r = radii.to(torch.float64)
+    x = r[:, None] * torch.cos(ang)[None, :]
+    y = r[:, None] * torch.sin(ang)[None, :]
+    return torch.stack([x, y])[None].to(dtype=dtype, device=device).contiguous()
+
+
+def cylinder_ogrid_c2(n_angle=128, n_radius=128, r1=0.5, r2=5.0, ratio=1.05, viscosity=0.005, inflow=1.0,
+                      noise=0.01, seed=0, dtype=torch.float32, device="cuda"):
+    """Precision configuration: flow past a cylinder of radius r1 on an O-grid of 4 blocks (n_angle x n_radius cells
+    each, the quadrants clockwise from the top as in the reference's make4BlockTorus_VortexStreet,
+    tests/test_setups.py:204-350), blocks joined +x -> -x around the cylinder, radial spacing from tanh_radii.
+    No-slip wall on the cylinder (-y of every block); on the outer boundary (+y) uniform inflow (inflow, 0) on the
+    two upstream quadrants and an advective outflow, initially (inflow, 0), on the two downstream ones.  Initial
+    velocity: (inflow, 0) plus seeded Gaussian noise of std `noise` per cell and component.
+    Returns (domain, outflow boundaries, inflow velocity [1, 2]); step with
+    PISOtorch_simulation.update_advective_boundaries(domain, outflow, inflow_velocity, dt) as the PRE hook."""
+    radii = tanh_radii(r1, r2, n_radius, ratio)
+    dom = PISOtorch.Domain(2, torch.tensor([viscosity], dtype=dtype), "cylinder_ogrid", dtype, device,
+                           passiveScalarChannels=0)
+    names = ("top_right", "bottom_right", "bottom_left", "top_left")
+    blocks = []
+    for k, start in enumerate((90.0, 0.0, -90.0, -180.0)):
+        c = ogrid_coords(n_angle, radii, start, -90.0, dtype=dtype, device=device)
+        b = dom.CreateBlock(vertexCoordinates=c, name=names[k])
+        b.CloseAllBoundaries()
+        blocks.append(b)
+    for k in range(4):
+        blocks[k].ConnectBlock("+x", blocks[(k + 1) % 4], "-x", "-y")
+    vin = torch.tensor([[inflow, 0.0]], dtype=dtype, device=device)
+    for b in (blocks[2], blocks[3]):
+        b.getBoundary("+y").setVelocity(vin)
+    outflow = []
+    for b in (blocks[0], blocks[1]):
+        bd = b.getBoundary("+y")
+        bd.setVelocity(vin)
+        bd.makeVelocityVarying()
+        outflow.append(bd)
+    g = torch.Generator().manual_seed(seed)
+    for b in blocks:
+        u = torch.randn(b.velocity.shape, generator=g, dtype=torch.float64) * noise
+        u[:, 0] += inflow
+        b.setVelocity(u.to(dtype=dtype, device=device).contiguous())
+    dom.PrepareSolve()
+    # one update with zero advection speed balances the boundary fluxes before the first step (test_setups.py:341-343)
+    from . import PISOtorch_simulation
+    PISOtorch_simulation.update_advective_boundaries(dom, outflow, torch.zeros_like(vin),
+                                                     torch.ones([1], dtype=dtype, device=device))
+    return dom, outflow, vin
