@@ -137,9 +137,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--mode", default="fwdbwd", choices=["fwdbwd", "fwd"])
-    ap.add_argument("--steps", type=int, default=None, help="default: 16 (c3, c1), 100 (c4)")
+    ap.add_argument("--steps", type=int, default=None, help="default: 16 (c3, c1), 200 (c2), 100 (c4), 8 (c5)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c3", "c1", "c4", "c5"])
+    ap.add_argument("--config", default="c3", choices=["c3", "c1", "c2", "c4", "c5"])
     ap.add_argument("--batch", type=int, default=64, help="c4: number of cavities")
     ap.add_argument("--sync-checks", action="store_true",
                     help="c1/c3/c5: check every native step's solver results before queueing the next, as the "
@@ -159,7 +159,7 @@ def main():
     ap.add_argument("--pmc-write", default=None, metavar="CSV", help="the same for WRITE_SIZE")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"c4": 100, "c5": 8}.get(args.config, 16)
+        args.steps = {"c2": 200, "c4": 100, "c5": 8}.get(args.config, 16)
     if args.config == "c5":
         # one FP32 oracle step of 1.77 M cells takes minutes on the host: no CPU sample for this configuration
         args.no_cpu_baseline = True
@@ -173,6 +173,9 @@ def main():
 
     if args.config == "c4":
         print(json.dumps(bench_c4(args, torch)))
+        return
+    if args.config == "c2":
+        print(json.dumps(bench_c2(args, torch)))
         return
     dom, sim, dt, workload = make_workload(args, torch)
     N = dom.getTotalSize()
@@ -300,6 +303,139 @@ def main():
     if batched is not None:
         out["batched"] = batched
     print(json.dumps(out))
+
+
+def _c2_sim(torch, dtype, refine):
+    from pict_amd import PISOtorch_simulation as S
+    from pict_amd import configs
+    dom, out, vin = configs.cylinder_ogrid_c2(dtype=dtype, device="cuda")
+    sim = S.Simulation(dom, time_step=0.005, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6,
+                       advect_passive_scalar=False, pressure_refine_tol=refine,
+                       prep_fn={"PRE": lambda domain, time_step, **kw: S.update_advective_boundaries(
+                           domain, out, vin, time_step.cuda())})
+    return dom, sim
+
+
+def _max_rel(doms, refs, field):
+    e = 0.0
+    for d, r in zip(doms, refs):
+        for b, rb in zip(d.blocks, r.blocks):
+            a, c = getattr(b, field).double(), getattr(rb, field).double()
+            e = max(e, float((a - c).abs().max() / c.abs().max()))
+    return e
+
+
+def bench_c2(args, torch):
+    """Precision configuration: the 4-block O-grid cylinder (4 x 128 x 128 cells, inflow u = 1, advective outflow,
+    nu = 0.005, noisy uniform initial state) stepped with the reference driver's defaults (non-orthogonal branch,
+    flags 25), dt = 0.005, advection and pressure tolerance 1e-6, FP32 bulk with FP64 Krylov reductions and the
+    pressure solves refined in FP64 to 1e-10 relative residual.  Times `--steps` steps after `--warmup`; then runs
+    the same steps in FP64 (FP64 refined pressure solves) and in plain FP32 (no refinement) from the same initial
+    state and reports the error of both FP32 runs against the FP64 one."""
+    from pict_amd import PISOtorch as P
+    dom, sim = _c2_sim(torch, torch.float32, 1e-10)
+    N = dom.getTotalSize()
+    sim.record_solver_infos = True
+    if args.warmup > 0:
+        assert sim._PISO_split_step(args.warmup)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sim.solver_infos = []
+    t0 = time.perf_counter()
+    e0.record()
+    assert sim._PISO_split_step(args.steps)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = e0.elapsed_time(e1)
+    infos = sim.solver_infos
+    adv_its = [sum(i.usedIterations for i in inf) for k, inf in infos if k == "BiCG"]
+    p_its = [inf[0].usedIterations for k, inf in infos if k == "CG"]
+    # FP64 (refined) and plain FP32 runs of the same steps from the same initial state
+    ref, sref = _c2_sim(torch, torch.float64, 1e-10)
+    assert sref._PISO_split_step(args.warmup + args.steps)
+    plain, splain = _c2_sim(torch, torch.float32, None)
+    assert splain._PISO_split_step(args.warmup + args.steps)
+    err = {"refined": {"velocity": _max_rel([dom], [ref], "velocity"),
+                       "pressure": _max_rel([dom], [ref], "pressure")},
+           "plain_fp32": {"velocity": _max_rel([plain], [ref], "velocity"),
+                          "pressure": _max_rel([plain], [ref], "pressure")},
+           "reference": "FP64 steps of the same path with FP64 refined pressure solves (each step matches the FP64 "
+                        "oracle's non-orthogonal step to 1e-8 at reduced size, tests/test_gpu_c2.py)"}
+    # solver kernel times of two more steps (HIP events around every launch) for the roofline
+    P.enable_solve_log(True)
+    assert sim._PISO_split_step(2)
+    torch.cuda.synchronize()
+    log = P.solve_log
+    P.enable_solve_log(False)
+    agg = {}
+    for kind, n, nnz, nrhs, s_, inf, kms, transposed, name in log:
+        a = agg.setdefault(name, {"ms": 0.0, "bytes": 0, "launches": 0, "iters": 0})
+        its = [i.usedIterations + 1 for i in inf]
+        a["ms"] += kms
+        a["bytes"] += solver_bytes(kind, n, nnz, nrhs, s_, its)
+        a["launches"] += 1
+        a["iters"] += sum(its)
+    dk = max(agg, key=lambda k: agg[k]["ms"])
+    d = agg[dk]
+    avg_ms = d["ms"] / d["launches"]
+    bpl = d["bytes"] / d["launches"]
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    cpu = None if args.no_cpu_baseline else cpu_baseline_c2(args)
+    return {
+        "metric": "cells*PISO-steps/s (forward)",
+        "value": N * args.steps / (ms_total * 1e-3),
+        "unit": "cells*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 Krylov reductions, FP64-refined pressure solves)",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "O-grid around a cylinder r=0.5, 4 blocks of 128x128 (angle x radius, tanh radial "
+                               "stretch, first ratio 1.05, outer radius 5), inflow u=1 and advective outflow, "
+                               "nu=0.005, u0 = (1,0) + N(0, 0.01^2), dt=0.005, reference driver defaults "
+                               "(non-orthogonal branch, flags 25), tolerances 1e-6, pressure refined to 1e-10",
+                   "cells": N, "config": "c2", "mode": "fwd", "dt": 0.005, "wall_s": wall,
+                   "error_vs_fp64": err,
+                   "iterations_per_step": {"advection": sum(adv_its) / args.steps,
+                                           "pressure_inner": sum(p_its) / args.steps},
+                   "solver_ms_per_step": {k: v["ms"] / 2 for k, v in agg.items()},
+                   "solver_launches_per_step": {k: v["launches"] / 2 for k, v in agg.items()}},
+        "roofline": {"bound": "hbm", "kernel": dk, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": bpl,
+                     "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_c2(args):
+    """FP32 C restatement (OpenMP) of the precision configuration's non-orthogonal step (plain FP32 solves at the
+    same tolerances; the outflow update on the host twin), until --cpu-seconds have passed."""
+    import torch
+    from oracle import oracle as O
+    from pict_amd import PISOtorch_simulation as S
+    from pict_amd import configs
+    cores = min(16, os.cpu_count() or 1)
+    O.lib(32).oracle_set_threads(cores)
+    dom, out, vin = configs.cylinder_ogrid_c2(dtype=torch.float32, device="cuda")
+    twin = O.cpu_twin(dom, dtype=torch.float32)
+    tout = [twin.getBlock(i).getBoundary("+y") for i in (0, 1)]
+    vin = vin.cpu()
+    dt = torch.tensor([0.005], dtype=torch.float32)
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < 64 and (steps == 0 or time.perf_counter() - t0 < args.cpu_seconds):
+        S.update_advective_boundaries(twin, tout, vin, dt)
+        O.piso_step(twin, 0.005, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, non_orthogonal=True)
+        steps += 1
+    t = time.perf_counter() - t0
+    return {"value": dom.getTotalSize() * steps / t, "unit": "cells*steps/s", "cores": cores, "kind": "port",
+            "sample": "the first %d non-orthogonal PISO step(s) of the same workload in %.1f s (FP32 without "
+                      "refinement, OpenMP %d threads)" % (steps, t, cores)}
 
 
 def _c4_sims(torch, count, dtype):

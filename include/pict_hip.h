@@ -469,7 +469,8 @@ int32_t     pict_batch_last_solve_ms(const pict_batch *batch, double *ms_out, in
 
 /* Diagnostics (no counterpart in the reference): when enabled, the solver
    records HIP events around its kernel launch on the launch stream; the
-   elapsed time of the last solve's kernel is then available (milliseconds). */
+   elapsed time of the last solve's kernel is then available (milliseconds;
+   after pict_solve_refined, the sum over its correction solves). */
 pict_status pict_set_kernel_timing(int32_t enable);
 double      pict_last_solve_kernel_ms(void);
 

@@ -2131,7 +2131,12 @@ def SolveLinearRefined(A, RHS, x, maxIterations=1000, tolerance=1e-5, refineTole
                                                       -1 if nullSpace is None else int(bool(nullSpace)),
                                                       ctypes.byref(info),
                                                       _stream(RHS.device)), "SolveLinearRefined")
-    return LinearSolverResultInfo(info.final_residual, info.used_iterations, info.converged, info.is_finite_residual)
+    res = LinearSolverResultInfo(info.final_residual, info.used_iterations, info.converged, info.is_finite_residual)
+    if solve_log is not None:  # one entry for all correction solves (their iterations and kernel time summed)
+        lib = _abi.load_library()
+        solve_log.append(("BiCG" if useBiCG else "CG", n, A.getSize(), 1, RHS.element_size(), [res],
+                          lib.pict_last_solve_kernel_ms(), False, solver_kernel_name(A, 1, opts)))
+    return res
 
 
 def SparseOuterProduct(a, b, out_pattern):

@@ -518,7 +518,8 @@ class Simulation:
         if max_iter is None:
             max_iter = self.linear_solve_max_iterations
         if refine_tol is not None and not self.differentiable:
-            return self._linear_solve_refined(csrMat, rhs, x, use_BiCG, refine_tol, max_iter, residual_reset_step)
+            # the sweeps recompute the residual in FP64 themselves: the correction solves need no resets
+            return self._linear_solve_refined(csrMat, rhs, x, use_BiCG, refine_tol, max_iter)
         return self.linear_solve_GPU(csrMat, rhs, x, use_BiCG, tol, max_iter, matrix_rank_deficient,
                                      residual_reset_step, return_best_result,
                                      double_fallback=self.solver_double_fallback,

@@ -2306,6 +2306,7 @@ using namespace pict;
 namespace {
 
 thread_local bool g_timing = false;
+thread_local bool g_force_textbook = false;  // pict_solve_refined's fallback for its correction solves
 thread_local double g_last_ms = 0.0;
 thread_local hipEvent_t g_ev[2] = {nullptr, nullptr};
 
@@ -2449,7 +2450,7 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
     const int n = A->n;
     // PICT_CG_TEXTBOOK=1 selects the two-barrier textbook CG kernel (diagnostics / parity checks)
     const char *tb_env = getenv("PICT_CG_TEXTBOOK");
-    const bool textbook_cg = tb_env && atoi(tb_env) != 0;
+    const bool textbook_cg = (tb_env && atoi(tb_env) != 0) || g_force_textbook;
     pl = Plan{};
     // CG with residual resets runs the textbook kernel: the reference resets the residual only in the driver's
     // non-orthogonal branch (PISOtorch_simulation.py:1280), whose pressure matrix is not symmetric, and on a
@@ -3245,9 +3246,16 @@ pict_status pict_solve_refined(const pict_csr_desc *A, const void *rhs, double *
         for (unsigned k = 0; k < blocks; ++k) m = std::max(m, hp[k]);
         project = m < (f32 ? 1e-5 : 1e-12);
     }
-    double rel = 0.0, prev = 0.0;
+    double rel = 0.0, prev = 0.0, inner_ms = 0.0, last_rms = 0.0;
     int inner_its = 0, sweep = 0;
     bool converged = false, finite = true;
+    // A sweep whose single-reduction CG correction did not halve the residual (its recurrences can stall on a singular,
+    // not quite symmetric FP32 matrix where textbook CG still converges) hands the remaining sweeps to the textbook
+    // CG; a correction that increased the residual is taken back first.
+    bool textbook = false, redo = false;
+    struct Restore {
+        ~Restore() { g_force_textbook = false; }
+    } restore;
     pict_solve_info ii;
     for (;; ++sweep) {
         if (f32)
@@ -3274,12 +3282,27 @@ pict_status pict_solve_refined(const pict_csr_desc *A, const void *rhs, double *
         }
         rel = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
         finite = std::isfinite(rel);
+        if (sweep > 0 && !textbook && !inner->use_bicg && !(rel <= 0.5 * prev)) {
+            if (!(rel <= prev)) {  // take the correction back
+                if (f32)
+                    hipLaunchKernelGGL(k_refine_update<float>, dim3(blocks), dim3(256), 0, s, x, (const float *)d,
+                                       -last_rms, n);
+                else
+                    hipLaunchKernelGGL(k_refine_update<double>, dim3(blocks), dim3(256), 0, s, x, (const double *)d,
+                                       -last_rms, n);
+                PICT_HIP_TRY(hipGetLastError());
+            }
+            textbook = g_force_textbook = true;
+            redo = true;
+            continue;  // the residual of x again, then the next sweep with the textbook CG
+        }
         if (!finite || rel < refine_tol) {
             converged = finite;
             break;
         }
         // stop when the sweeps no longer reduce the residual (the inner solver's precision floor)
-        if (sweep >= max_sweeps || (sweep > 0 && rel > 0.5 * prev)) break;
+        if (sweep >= max_sweeps || (sweep > 0 && !redo && rel > 0.5 * prev)) break;
+        redo = false;
         prev = rel;
         const double rms = sqrt(rr / (double)n);
         if (f32)
@@ -3291,12 +3314,15 @@ pict_status pict_solve_refined(const pict_csr_desc *A, const void *rhs, double *
         PICT_HIP_TRY(hipGetLastError());
         if ((st = solve_sync(A, rr_vec, d, 1, inner, &ii, s))) return st;
         inner_its += ii.used_iterations + 1;
+        inner_ms += g_last_ms;
+        last_rms = rms;
         if (f32)
             hipLaunchKernelGGL(k_refine_update<float>, dim3(blocks), dim3(256), 0, s, x, (const float *)d, rms, n);
         else
             hipLaunchKernelGGL(k_refine_update<double>, dim3(blocks), dim3(256), 0, s, x, (const double *)d, rms, n);
         PICT_HIP_TRY(hipGetLastError());
     }
+    if (g_timing) g_last_ms = inner_ms;  // the kernel time of the refined solve: the sum over its correction solves
     info_out->final_residual = rel;
     info_out->used_iterations = inner_its;
     info_out->converged = converged ? 1 : 0;
