@@ -15,6 +15,11 @@ solve, graph replay, autotuned launch shapes) and, for comparison, as the plain 
 cavity; `value` is the batch's, `config.batched_vs_plain` the ratio.  The default run appends the same measurement
 (without its CPU sample) to its line under `batched` (`--no-batched` skips it).
 
+`config.error_vs_fp64` states the FP32 error against FP64 steps of the same workload: after the whole forward
+trajectory from the same initial state, and per step, from FP64 states along the trajectory, against the FP64
+step whose solves run the FP32 step's iteration counts (at tolerance 1e-6 a solve near the tolerance may stop one
+iteration earlier or later in either precision, which alone moves the state by up to about 1e-3).
+
 Prints one JSON line.  `roofline` is for the dominant kernel (the solver kernel with the largest total time in the
 timed region, timed with HIP events on its launch stream): algorithmic bytes per launch / average launch duration
 vs the 8 TB/s HBM peak.  `cpu_baseline` is the FP32 C restatement (oracle/pict_oracle.c, OpenMP) of the same
@@ -44,10 +49,10 @@ def solver_bytes(kind, n, nnz, nrhs, s, iters):
     return sum(max(it, 0) * per_it + spmv + n * s for it in iters)
 
 
-def make_workload(args, torch):
+def make_workload(args, torch, dtype=None):
     from pict_amd import PISOtorch_simulation as S
     from pict_amd import configs
-    dtype = torch.float32
+    dtype = dtype or torch.float32
     if args.config == "c5":
         dom, dt, _ = configs.channel_c5(dtype=dtype, device="cuda")
         workload = ("3-D channel 192x96x96 (tanh wall-normal stretch 2.0, periodic x/z, no-slip walls), Re_tau 180, "
@@ -137,10 +142,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--mode", default="fwdbwd", choices=["fwdbwd", "fwd"])
-    ap.add_argument("--steps", type=int, default=None, help="default: 16 (c3, c1), 200 (c2), 100 (c4), 8 (c5)")
+    ap.add_argument("--steps", type=int, default=None, help="default: 16 (c3), 50 (c1), 200 (c2), 100 (c4), 8 (c5)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=["c3", "c1", "c2", "c4", "c5"])
     ap.add_argument("--batch", type=int, default=64, help="c4: number of cavities")
+    ap.add_argument("--precision-samples", type=int, default=4,
+                    help="c3/c1/c5: FP64 states along the trajectory from which the FP32 step's error is measured")
+    ap.add_argument("--matched-samples", type=int, default=10,
+                    help="c4: FP64 states along the trajectory from which both FP32 paths take one step for the "
+                         "per-step error")
     ap.add_argument("--sync-checks", action="store_true",
                     help="c1/c3/c5: check every native step's solver results before queueing the next, as the "
                          "reference's driver does (default: queue without waiting and check them, in order, at the "
@@ -159,7 +169,7 @@ def main():
     ap.add_argument("--pmc-write", default=None, metavar="CSV", help="the same for WRITE_SIZE")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"c2": 200, "c4": 100, "c5": 8}.get(args.config, 16)
+        args.steps = {"c1": 50, "c2": 200, "c4": 100, "c5": 8}.get(args.config, 16)
     if args.config == "c5":
         # one FP32 oracle step of 1.77 M cells takes minutes on the host: no CPU sample for this configuration
         args.no_cpu_baseline = True
@@ -262,6 +272,9 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(dom, dt, args, target)
 
+    # FP32 error of the forward steps against FP64 steps of the same workload (outside the timed region)
+    precision = precision_vs_fp64(args, torch, args.steps, args.precision_samples)
+
     batched = None
     if not args.no_batched and args.config == "c3":
         import copy
@@ -289,6 +302,7 @@ def main():
         "dtype": "f32 (FP64 Krylov reductions)",
         "data": "synthetic (seeded)",
         "config": {"workload": workload, "cells": N, "config": args.config, "mode": args.mode, "dt": dt,
+                   "error_vs_fp64": precision,
                    "solver_checks": "after every step" if args.sync_checks else "deferred to the end of the rollout",
                    "solver_launches": {k: v["launches"] for k, v in agg.items()},
                    "solver_adjoint_launches": {k: v["adjoint_launches"] for k, v in agg.items()},
@@ -459,6 +473,164 @@ def _c4_error(doms, ref):
     return ev, ep
 
 
+def _fixed_count_sim(torch, dom, dt):
+    """Stage-path simulation whose solves each run a given number of iterations (tolerance 0, list `counts`, one
+    entry per solve and right-hand side in the driver's order): the reference step of a precision comparison takes
+    the iteration counts of the FP32 step it is compared with, so that the comparison measures rounding, not on
+    which side of the stopping tolerance a solve lands (one CG iteration near the tolerance moves a cavity's
+    velocity by about 1e-3 relative)."""
+    from pict_amd import PISOtorch as P
+    from pict_amd import PISOtorch_simulation as S
+
+    class FixedCountSim(S.Simulation):
+        counts = ()
+
+        def linear_solve(self, csrMat, rhs, x=None, use_BiCG=False, **kw):
+            if x is None:
+                x = torch.zeros_like(rhs)
+            n = csrMat.getRows()
+            for k in range(rhs.numel() // n):
+                # used iterations as the solvers report them: CG the index of its last iteration (-1: none),
+                # BiCGStab its full iterations (one converging at a half step reports the full ones before it)
+                it = self.counts.pop(0) + (0 if use_BiCG else 1)
+                if it > 0:
+                    P.SolveLinear(csrMat, rhs[k * n:(k + 1) * n], x[k * n:(k + 1) * n], maxIterations=it,
+                                  tolerance=0.0, useBiCG=use_BiCG, BiCGwithPreconditioner=False)
+            return x, True
+
+    s = FixedCountSim(dom, time_step=dt, corrector_steps=2, non_orthogonal=False, advection_tol=1e-6,
+                      pressure_tol=1e-6, advect_passive_scalar=False)
+    s.native_step = False
+    return s
+
+
+def _snapshot(dom):
+    return [(b.velocity.detach().clone(), b.pressure.detach().clone()) for b in dom.blocks]
+
+
+def _set_state(dom, snap):
+    """Velocity and pressure of every block from `snap`, copied in place (cast to the domain's dtype; the tensors
+    keep their addresses)."""
+    for b, (u, p) in zip(dom.blocks, snap):
+        b.velocity.detach().copy_(u)
+        b.pressure.detach().copy_(p)
+    dom.UpdateDomainData()
+
+
+def _dom_error(dom, ref):
+    """max|u - u_ref| / max|u_ref| over all blocks, and the same for the pressure."""
+    out = []
+    for f in ("velocity", "pressure"):
+        diff = max(float((getattr(b, f).detach().double() - getattr(r, f).detach().double()).abs().max())
+                   for b, r in zip(dom.blocks, ref.blocks))
+        scale = max(float(getattr(r, f).detach().double().abs().max()) for r in ref.blocks)
+        out.append(diff / scale)
+    return out
+
+
+def _counted_step(sim):
+    """One step of `sim` (FP32); returns the used iterations of its solves in the driver's order."""
+    sim.record_solver_infos = True
+    sim.solver_infos = []
+    assert sim._PISO_split_step(1)
+    sim.record_solver_infos = False
+    return [i.usedIterations for _, infos in sim.solver_infos for i in infos]
+
+
+def _ref_error(dom, counts, snap, ref_dom, ref_sim, source=None):
+    """Error of `dom`'s state after one FP32 step from the FP64 state `snap` that took the iteration counts
+    `counts`, against the FP64 step of ref_sim from `snap` with those counts (`source(dom)` sets the velocity
+    source from the state first)."""
+    _set_state(ref_dom, snap)
+    if source:
+        source(ref_dom)
+    ref_sim.counts = list(counts)
+    assert ref_sim._PISO_split_step(1)
+    assert not ref_sim.counts
+    return _dom_error(dom, ref_dom)
+
+
+def precision_vs_fp64(args, torch, steps, samples):
+    """FP32 error of this configuration's forward steps against FP64 (the library's FP64 steps, same inputs and
+    solver settings): the state after `steps` steps from the same initial state (trajectory), and one step from
+    each of `samples` FP64 states along that trajectory against the FP64 step with the FP32 step's iteration
+    counts (per step)."""
+    d32, s32, dt, _ = make_workload(args, torch, torch.float32)
+    d64, s64, _, _ = make_workload(args, torch, torch.float64)
+    dm, _, _, _ = make_workload(args, torch, torch.float64)
+    sm = _fixed_count_sim(torch, dm, dt)
+    s32.defer_solver_checks = s64.defer_solver_checks = False
+    source = None
+    if args.config == "c5":
+        from pict_amd import configs
+        nets = {dt_: configs.sgs_cnn(16, seed=0, dtype=dt_) for dt_ in (torch.float32, torch.float64)}
+
+        def source(dom):
+            b = dom.blocks[0]
+            u = b.velocity.detach()
+            f = torch.tensor([1.0, 0.0, 0.0], dtype=u.dtype, device=u.device).reshape(1, 3, 1, 1, 1)
+            with torch.no_grad():
+                b.setVelocitySource((f + 0.05 * nets[u.dtype](u)).contiguous())
+            dom.UpdateDomainData()
+    snaps = []
+    every = max(1, steps // samples)
+    for k in range(steps):
+        if k % every == 0:
+            snaps.append(_snapshot(d64))
+        for d, s in ((d32, s32), (d64, s64)):
+            if source:
+                source(d)
+            assert s._PISO_split_step(1)
+    ev, ep = _dom_error(d32, d64)
+    per = [0.0, 0.0]
+    for snap in snaps:
+        _set_state(d32, snap)
+        if source:
+            source(d32)
+        e = _ref_error(d32, _counted_step(s32), snap, dm, sm, source)
+        per = [max(per[0], e[0]), max(per[1], e[1])]
+    # the same states with the pressure solves refined in FP64 to 1e-10 relative residual in both precisions:
+    # what is left is the rounding of the FP32 stages, the advection solve and the FP32 pressure system
+    s32.pressure_refine_tol = s64.pressure_refine_tol = 1e-10
+    ref = [0.0, 0.0]
+    for snap in snaps:
+        for d, s in ((d32, s32), (d64, s64)):
+            _set_state(d, snap)
+            if source:
+                source(d)
+            assert s._PISO_split_step(1)
+        e = _dom_error(d32, d64)
+        ref = [max(ref[0], e[0]), max(ref[1], e[1])]
+    s32.pressure_refine_tol = s64.pressure_refine_tol = None
+    torch.cuda.synchronize()
+    return {"trajectory": {"velocity": ev, "pressure": ep, "steps": steps},
+            "per_step": {"velocity": per[0], "pressure": per[1], "states": len(snaps)},
+            "per_step_refined_pressure": {"velocity": ref[0], "pressure": ref[1]},
+            "reference": "FP64 steps of the same workload and solver settings; per step: one step from each FP64 "
+                         "state (cast to FP32) against the FP64 step from that state whose solves run the FP32 "
+                         "step's iteration counts; refined: both steps with FP64-refined pressure solves"}
+
+
+def _c4_matched_error(torch, snaps, doms_p, sims_p, batch, doms_b, dt):
+    """Per-step precision of both FP32 paths on the same inputs: from each FP64 snapshot (a state per cavity), one
+    FP32 step of the plain loop and one batched step against the FP64 step with the iteration counts of that FP32
+    step.  Returns {"plain": {"velocity", "pressure"}, "batched": ...}, maxima over snapshots and cavities."""
+    from pict_amd import configs
+    doms_r, _, _, _ = configs.cavity_batch_c4(len(doms_p), 0, torch.float64, "cuda")
+    sims_r = [_fixed_count_sim(torch, d, dt) for d in doms_r]
+    out = {"plain": [0.0, 0.0], "batched": [0.0, 0.0]}
+    for snap in snaps:
+        for d, sn in zip(doms_b, snap):
+            _set_state(d, sn)
+        counts_b = [[int(c) for c in row] for row in batch.step()["used_iterations"]]
+        for i, (dp, sp, db, dr, sr) in enumerate(zip(doms_p, sims_p, doms_b, doms_r, sims_r)):
+            _set_state(dp, snap[i])
+            res = [_ref_error(dp, _counted_step(sp), snap[i], dr, sr), _ref_error(db, counts_b[i], snap[i], dr, sr)]
+            for name, e in zip(("plain", "batched"), res):
+                out[name] = [max(out[name][0], e[0]), max(out[name][1], e[1])]
+    return {k: {"velocity": v[0], "pressure": v[1]} for k, v in out.items()}
+
+
 def bench_c4(args, torch):
     """Batching configuration: `--batch` independent lid cavities (sides 32..256, viscosities 1e-3..1e-1, FP32,
     BiCGStab advection and CG pressure at tolerance 1e-6), `--steps` PISO steps each.  Times the batched step
@@ -506,7 +678,11 @@ def bench_c4(args, torch):
     # FP64 steps of the same cavities and settings: the error of both FP32 paths
     doms_r, sims_r, _, _, _ = _c4_sims(torch, args.batch, torch.float64)
     ref = SimulationBatch(sims_r)
-    for _ in range(args.warmup + args.steps):
+    snaps = []
+    total = args.warmup + args.steps
+    for k in range(total):
+        if k % max(1, total // args.matched_samples) == 0:
+            snaps.append([_snapshot(d) for d in doms_r])
         ref.step()
     ev_b, ep_b = _c4_error(doms_b, doms_r)
     ev_p, ep_p = _c4_error(doms_p, doms_r)
@@ -528,6 +704,7 @@ def bench_c4(args, torch):
     bytes_per_launch = sum(p_bytes) / K
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     its = np.array(iters)
+    matched = _c4_matched_error(torch, snaps, doms_p, sims_p, batch, doms_b, dt)
     cpu = None if args.no_cpu_baseline else cpu_baseline_c4(doms_b, dt, args)
     value = N * args.steps / (ms_b * 1e-3)
     plain = N * args.steps / (ms_p * 1e-3)
@@ -550,9 +727,14 @@ def bench_c4(args, torch):
                    "cells": N, "config": "c4", "mode": "fwd",
                    "batched_vs_plain": value / plain,
                    "plain_loop": {"value": plain, "ms_per_step": ms_p / args.steps},
-                   "error_vs_fp64": {"batched": {"velocity": ev_b, "pressure": ep_b},
-                                     "plain": {"velocity": ev_p, "pressure": ep_p},
-                                     "reference": "FP64 batched steps, same cavities and solver settings"},
+                   "error_vs_fp64": {"per_step": dict(matched, reference=(
+                                         "one step from each of %d FP64 states along the trajectory (the state cast "
+                                         "to FP32) against the FP64 step from that state whose solves run the FP32 "
+                                         "step's iteration counts; max over states and cavities" % len(snaps))),
+                                     "trajectory": {"batched": {"velocity": ev_b, "pressure": ep_b},
+                                                    "plain": {"velocity": ev_p, "pressure": ep_p},
+                                                    "reference": "FP64 batched steps, same cavities and solver "
+                                                                 "settings (tolerance 1e-6 stopping rule)"}},
                    "tuned": tuned, "tune_s": t_tune, "launch": info,
                    "iterations_per_step": {"advection_max": int(its[:, :, :2].max()),
                                            "pressure_max": int(its[:, :, 2:].max()),
