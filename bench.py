@@ -426,6 +426,58 @@ def bench_c2(args, torch):
     }
 
 
+def _cpu_threads():
+    """OpenMP threads of the CPU sample: the cores this process may run on, at most the host's CPU share for one
+    GPU (16 threads, or OMP_NUM_THREADS when it is set lower)."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    except ValueError:
+        share = 16
+    return max(1, min(avail, share, 16))
+
+
+def _cpu_host():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"model": model, "logical_cpus": os.cpu_count(), "cpus_available": avail}
+
+
+def _cpu_sample(step, seconds, threads, max_steps=64):
+    """step() (returns the cells it stepped, and the seconds to count: None = all of it) on `threads` OpenMP threads
+    of the oracle until `seconds` have passed, at least once.  Returns (cells*steps/s, steps, seconds)."""
+    from oracle import oracle as O
+    O.lib(32).oracle_set_threads(threads)
+    t = 0.0
+    cells = steps = 0
+    while steps < max_steps and (steps == 0 or t < seconds):
+        t1 = time.perf_counter()
+        c, dt_counted = step(steps)
+        t += dt_counted if dt_counted is not None else time.perf_counter() - t1
+        cells += c
+        steps += 1
+    return cells / t, steps, t
+
+
+def _cpu_result(step, args, what):
+    """The CPU baseline entry: the same sample on the host's CPU share and on one thread."""
+    threads = _cpu_threads()
+    v, n, t = _cpu_sample(step, args.cpu_seconds, threads)
+    v1, n1, t1 = _cpu_sample(step, args.cpu_seconds, 1)
+    return {"value": v, "unit": "cells*steps/s", "cores": threads, "kind": "port",
+            "sample": "%d %s in %.1f s (FP32 C restatement, OpenMP %d threads)" % (n, what, t, threads),
+            "single_thread": {"value": v1, "sample": "%d %s in %.1f s, 1 thread" % (n1, what, t1)},
+            "host": _cpu_host()}
+
+
 def cpu_baseline_c2(args):
     """FP32 C restatement (OpenMP) of the precision configuration's non-orthogonal step (plain FP32 solves at the
     same tolerances; the outflow update on the host twin), until --cpu-seconds have passed."""
@@ -433,23 +485,18 @@ def cpu_baseline_c2(args):
     from oracle import oracle as O
     from pict_amd import PISOtorch_simulation as S
     from pict_amd import configs
-    cores = min(16, os.cpu_count() or 1)
-    O.lib(32).oracle_set_threads(cores)
     dom, out, vin = configs.cylinder_ogrid_c2(dtype=torch.float32, device="cuda")
     twin = O.cpu_twin(dom, dtype=torch.float32)
     tout = [twin.getBlock(i).getBoundary("+y") for i in (0, 1)]
     vin = vin.cpu()
     dt = torch.tensor([0.005], dtype=torch.float32)
-    t0 = time.perf_counter()
-    steps = 0
-    while steps < 64 and (steps == 0 or time.perf_counter() - t0 < args.cpu_seconds):
+
+    def step(_):
         S.update_advective_boundaries(twin, tout, vin, dt)
         O.piso_step(twin, 0.005, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, non_orthogonal=True)
-        steps += 1
-    t = time.perf_counter() - t0
-    return {"value": dom.getTotalSize() * steps / t, "unit": "cells*steps/s", "cores": cores, "kind": "port",
-            "sample": "the first %d non-orthogonal PISO step(s) of the same workload in %.1f s (FP32 without "
-                      "refinement, OpenMP %d threads)" % (steps, t, cores)}
+        return dom.getTotalSize(), None
+    return _cpu_result(step, args, "non-orthogonal PISO step(s) of the same workload from its initial state "
+                                   "onwards (FP32 without refinement)")
 
 
 def _c4_sims(torch, count, dtype):
@@ -751,22 +798,14 @@ def cpu_baseline_c4(doms, dt, args):
     """FP32 C restatement (OpenMP) of one forward step of each cavity in turn, until --cpu-seconds have passed."""
     import torch
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
-    O.lib(32).oracle_set_threads(cores)
-    t = 0.0
-    cells = steps = 0
-    for d in doms:
+
+    def step(i):
+        d = doms[i % len(doms)]
         twin = O.cpu_twin(d, dtype=torch.float32)  # set-up outside the timed part
         t1 = time.perf_counter()
         O.piso_step(twin, dt, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6)
-        t += time.perf_counter() - t1
-        cells += d.getTotalSize()
-        steps += 1
-        if t > args.cpu_seconds:
-            break
-    return {"value": cells / t, "unit": "cells*steps/s", "cores": cores, "kind": "port",
-            "sample": "one PISO step of each of the first %d cavities of the same batch in %.1f s (FP32, OpenMP %d "
-                      "threads)" % (steps, t, cores)}
+        return d.getTotalSize(), time.perf_counter() - t1
+    return _cpu_result(step, args, "PISO steps, one of each cavity of the same batch in turn from the first")
 
 
 def pmc_traffic(fetch_csv, write_csv, kernel):
@@ -815,14 +854,10 @@ def cpu_baseline(dom, dt, args, target):
     forward step, and for fwdbwd also the recomputed forward and the reverse pass of a checkpointed step."""
     import torch
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
-    O.lib(32).oracle_set_threads(cores)
     twin = O.cpu_twin(dom, dtype=torch.float32)
     g = torch.Generator().manual_seed(0)
-    t0 = time.perf_counter()
-    steps = 0
-    while steps < 64 and (steps == 0 or time.perf_counter() - t0 < args.cpu_seconds):
-        steps += 1
+
+    def step(_):
         u_in = [b.velocity.clone() for b in twin.blocks]
         O.piso_step(twin, dt, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6)
         if args.mode == "fwdbwd":
@@ -833,11 +868,9 @@ def cpu_baseline(dom, dt, args, target):
             O.piso_step_vjp(twin, dt, gout, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6)
             for b, u in zip(twin.blocks, out):
                 b.setVelocity(u)
-    t = time.perf_counter() - t0
-    what = "forward step + recomputed forward + reverse pass" if args.mode == "fwdbwd" else "forward step"
-    return {"value": dom.getTotalSize() * steps / t, "unit": "cells*steps/s", "cores": cores,
-            "kind": "port", "sample": "%d PISO step(s) of the same workload in %.1f s, %s (FP32, OpenMP %d threads)"
-            % (steps, t, what, cores)}
+        return dom.getTotalSize(), None
+    what = "forward step(s) + recomputed forward + reverse pass" if args.mode == "fwdbwd" else "forward step(s)"
+    return _cpu_result(step, args, "PISO %s of the same workload" % what)
 
 
 if __name__ == "__main__":

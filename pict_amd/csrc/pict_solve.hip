@@ -50,10 +50,12 @@ struct Ctl {
     unsigned pad2[30];
     unsigned group_count[8][32]; // per-group arrivals
 };
-// Resident workgroups (256 threads) per compute unit that a persistent solver grid is sized for.  The
-// occupancy query allows up to eight for the leaner kernels, but grids of seven and eight per unit (1792 and
-// 2048 workgroups: the 1.77 M-row channel pressure system on the lean grid-wide CG and on the textbook CG)
-// ended at the barrier's spin limit, while grids of four per unit run.
+// Resident workgroups (256 threads) per compute unit that a persistent solver grid is sized for.  Grids of seven
+// and eight per unit (1792 and 2048 workgroups: the 1.77 M-row channel pressure system on the lean grid-wide CG
+// and on the textbook CG) ended at the barrier's spin limit.  They sat exactly at the occupancy query's maximum,
+// which SGPRs set for those kernels (102-106 per wave: 7 waves per SIMD), while grids at a VGPR-set maximum run;
+// the counters and the spin budget do not depend on the grid size.  Four per unit stays below the SGPR limit of
+// every solver kernel; kernels whose VGPRs allow fewer take the query's figure.
 constexpr int kMaxResidentWgsPerCu = 4;
 constexpr int kGroups = 8;       // arrival tree fan-in (one group per XCD when blocks are dealt round-robin)
 
