@@ -87,6 +87,13 @@ typedef struct pict_boundary_desc {
     void   *velocity;         /* FIXED */
     void   *velocity_grad;    /* FIXED, may be NULL */
     void   *transform;        /* FIXED, [1,(z),y,x,2D^2+1] or NULL */
+    /* FIXED faces of a domain with passive-scalar channels (FixedBoundary
+       passiveScalar and its types, domain_structs.h:190-260)                   */
+    int32_t scalar_static;    /* passive scalar is [1,C] (else [1,C,(z),y,x])     */
+    int32_t scalar_bc_mask;   /* bit (c & 31) set: channel c is PICT_BC_NEUMANN
+                                 (per-channel types for at most 32 channels)     */
+    void   *scalar;           /* NULL when the domain has no passive scalar       */
+    void   *scalar_grad;      /* may be NULL                                      */
 } pict_boundary_desc;
 
 typedef struct pict_block_desc {
@@ -109,6 +116,8 @@ typedef struct pict_block_desc {
     void *viscosity_grad;
     void *velocity_source_grad;
     pict_boundary_desc bounds[6]; /* -x,+x,-y,+y,-z,+z                             */
+    void *scalar;             /* passive scalar [1,C,(z),y,x] or NULL              */
+    void *scalar_grad;        /* may be NULL                                        */
 } pict_block_desc;
 
 typedef struct pict_domain_desc {
@@ -139,6 +148,14 @@ typedef struct pict_domain_desc {
     void *pressure_rhs_grad;
     void *pressure_rhs_div_grad;
     void *pressure_result_grad;
+    int32_t scalar_channels;          /* passive-scalar channels C (0: none)        */
+    int32_t scalar_viscosity_static;  /* scalar_viscosity is [1] (else [C])         */
+    void *scalar_viscosity;   /* device [1] or [C]; NULL: the domain viscosity      */
+    void *scalar_viscosity_grad;      /* same shape, may be NULL                    */
+    void *scalar_rhs;         /* [C*N] */
+    void *scalar_result;      /* [C*N] */
+    void *scalar_rhs_grad;
+    void *scalar_result_grad;
 } pict_domain_desc;
 
 /* Device-resident copy of a domain descriptor plus its launch tables.
@@ -152,6 +169,8 @@ typedef struct pict_atlas pict_atlas;
 /* ------------------------------------------------------------------------ */
 
 const char *pict_last_error(void);
+/* Layout version of the descriptor structs above (2: passive-scalar fields). */
+#define PICT_ABI_VERSION 2
 int32_t     pict_abi_version(void);
 
 /* Domain::PrepareSolve -> SetupDomainGPU (extensions/domain_structs.cpp:2554-2676, :2992-3029) */
@@ -211,6 +230,20 @@ pict_status pict_copy_velocity_result_from_blocks(const pict_atlas *atlas, void 
 pict_status pict_copy_pressure_result_to_blocks(const pict_atlas *atlas, void *stream);
 pict_status pict_copy_pressure_result_from_blocks(const pict_atlas *atlas, void *stream);
 
+/* Passive scalars.  SetupAdvectionMatrix(forPassiveScalar=True, passiveScalarChannel)
+   (PISOtorch.cpp:503-505; kernel PISO_build_matrix :3601-3865, host :4510-4560): the
+   advection matrix of channel `channel` (scalar viscosity, the channel's boundary types)
+   into C values and A. */
+pict_status pict_setup_advection_matrix_scalar(const pict_atlas *atlas, double time_step,
+                                               int32_t non_ortho_flags, int32_t channel, void *stream);
+/* SetupAdvectionScalar (PISOtorch.cpp:506-508; kernel kPISO_build_scalar_advection_RHS
+   :4080-4190, host :4590-4623): scalar_rhs of every channel. */
+pict_status pict_setup_advection_scalar(const pict_atlas *atlas, double time_step,
+                                        int32_t non_ortho_flags, void *stream);
+/* CopyScalarResult{To,From}Blocks (PISOtorch.cpp:533-534; :6543-6610). */
+pict_status pict_copy_scalar_result_to_blocks(const pict_atlas *atlas, void *stream);
+pict_status pict_copy_scalar_result_from_blocks(const pict_atlas *atlas, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* backward stages (vector-Jacobian products)                                */
 /* ------------------------------------------------------------------------ */
@@ -260,6 +293,20 @@ pict_status pict_correct_velocity_grad(const pict_atlas *atlas, double time_step
 pict_status pict_copy_velocity_result_grad_to_blocks(const pict_atlas *atlas, void *stream);
 pict_status pict_copy_velocity_result_grad_from_blocks(const pict_atlas *atlas, void *stream);
 pict_status pict_copy_pressure_result_grad_from_blocks(const pict_atlas *atlas, void *stream);
+/* SetupAdvectionMatrixGrad(forPassiveScalar=True) (PISOtorch.cpp:568-570; kernel
+   PISO_build_matrix_GRAD :3869-4075): C_grad, A_grad -> block velocity_grad (+=),
+   fixed-boundary velocity_grad (+=), scalar viscosity grad (+=; the domain
+   viscosity grad without a scalar viscosity). */
+pict_status pict_setup_advection_matrix_scalar_grad(const pict_atlas *atlas, double time_step,
+                                                    int32_t non_ortho_flags, int32_t channel, void *stream);
+/* SetupAdvectionScalarGrad (PISOtorch.cpp:571-573; kernel :4193-4278, host :4626-4659): scalar_rhs_grad ->
+   block scalar_grad (=), fixed-boundary scalar_grad and velocity_grad (+=), scalar
+   viscosity grad (+=), scalar_result_grad (+=, explicit non-orthogonal terms). */
+pict_status pict_setup_advection_scalar_grad(const pict_atlas *atlas, double time_step,
+                                             int32_t non_ortho_flags, void *stream);
+/* CopyScalarResultGrad{From,To}Blocks (PISOtorch.cpp:588-589; :6737-6800). */
+pict_status pict_copy_scalar_result_grad_from_blocks(const pict_atlas *atlas, void *stream);
+pict_status pict_copy_scalar_result_grad_to_blocks(const pict_atlas *atlas, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* linear solvers                                                            */

@@ -65,6 +65,11 @@ def declare(L):
     L.oracle_setup_pressure_rhs_only_grad.argtypes = [ctypes.c_void_p, ctypes.c_double]
     L.oracle_copy_velocity_result_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_copy_pressure_result_grad_from_blocks.argtypes = [ctypes.c_void_p]
+    for n in ("oracle_setup_advection_matrix_scalar", "oracle_setup_advection_matrix_scalar_grad"):
+        getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    for n in ("oracle_setup_advection_scalar", "oracle_setup_advection_scalar_grad"):
+        getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.oracle_copy_scalar_result.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     return L
 
 
@@ -95,11 +100,14 @@ def cpu_twin(domain, dtype=torch.float64):
     def cv(t):
         return None if t is None else t.detach().to(device="cpu", dtype=dtype).contiguous()
 
+    channels = domain.getPassiveScalarChannels()
     d = P.Domain(domain.getSpatialDims(), domain.viscosity.to(dtype), domain.name + "_cpu", dtype, "cpu",
-                 passiveScalarChannels=0, _allow_cpu=True)
+                 passiveScalarChannels=channels, _allow_cpu=True)
+    if domain.hasPassiveScalarViscosity():
+        d.setScalarViscosity(cv(domain.passiveScalarViscosity))
     mapping = {}
     for b in domain.blocks:
-        nb = P.Block(d, b.name, velocity=cv(b.velocity), pressure=cv(b.pressure))
+        nb = P.Block(d, b.name, velocity=cv(b.velocity), pressure=cv(b.pressure), passiveScalar=cv(b.passiveScalar))
         if b.hasTransform():
             nb.setTransform(cv(b.transform), cv(b.faceTransform))
         if b.viscosity is not None:
@@ -112,7 +120,9 @@ def cpu_twin(domain, dtype=torch.float64):
         nb = mapping[id(b)]
         for f, bd in enumerate(b.boundaries):
             if bd.type == P.BoundaryType.FIXED:
-                fb = P.FixedBoundary(cv(bd.velocity), bd.velocityType, None, None, cv(bd.transform), d)
+                fb = P.FixedBoundary(cv(bd.velocity), bd.velocityType, cv(bd.passiveScalar),
+                                     list(bd.passiveScalarTypes) if bd.passiveScalarTypes else None,
+                                     cv(bd.transform), d)
                 fb._setSize(list(bd.getSizes()))
                 nb.boundaries[f] = fb
             elif bd.type == P.BoundaryType.CONNECTED:
@@ -144,9 +154,28 @@ def _dt(t):
     return float(t[0]) if isinstance(t, torch.Tensor) else float(t)
 
 
-def SetupAdvectionMatrix(domain, timeStep, nonOrthoFlags=0):
+def SetupAdvectionMatrix(domain, timeStep, nonOrthoFlags=0, forPassiveScalar=False, passiveScalarChannel=0):
     L = lib(_bits(domain))
+    if forPassiveScalar:
+        _chk(L, L.oracle_setup_advection_matrix_scalar(_desc(domain), _dt(timeStep), int(nonOrthoFlags),
+                                                       int(passiveScalarChannel)), "scalar advection matrix")
+        return
     _chk(L, L.oracle_setup_advection_matrix(_desc(domain), _dt(timeStep), int(nonOrthoFlags)), "advection matrix")
+
+
+def SetupAdvectionScalar(domain, timeStep, nonOrthoFlags=0):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_advection_scalar(_desc(domain), _dt(timeStep), int(nonOrthoFlags)), "scalar rhs")
+
+
+def CopyScalarResultToBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_scalar_result(_desc(domain), 1, 0), "copy")
+
+
+def CopyScalarResultFromBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_scalar_result(_desc(domain), 0, 0), "copy")
 
 
 def SetupAdvectionVelocity(domain, timeStep, nonOrthoFlags=0, applyPressureGradient=False):
@@ -220,9 +249,28 @@ def CopyPressureResultFromBlocks(domain):
 
 # backward stages (same names and arguments as PISOtorch's *Grad functions)
 
-def SetupAdvectionMatrixGrad(domain, timeStep, nonOrthoFlags=0):
+def SetupAdvectionMatrixGrad(domain, timeStep, nonOrthoFlags=0, forPassiveScalar=False, passiveScalarChannel=0):
     L = lib(_bits(domain))
+    if forPassiveScalar:
+        _chk(L, L.oracle_setup_advection_matrix_scalar_grad(_desc(domain), _dt(timeStep), int(nonOrthoFlags),
+                                                            int(passiveScalarChannel)), "scalar matrix grad")
+        return
     _chk(L, L.oracle_setup_advection_matrix_grad(_desc(domain), _dt(timeStep), int(nonOrthoFlags)), "adv matrix grad")
+
+
+def SetupAdvectionScalarGrad(domain, timeStep, nonOrthoFlags=0):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_setup_advection_scalar_grad(_desc(domain), _dt(timeStep), int(nonOrthoFlags)), "scalar rhs grad")
+
+
+def CopyScalarResultGradFromBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_scalar_result(_desc(domain), 0, 1), "copy")
+
+
+def CopyScalarResultGradToBlocks(domain):
+    L = lib(_bits(domain))
+    _chk(L, L.oracle_copy_scalar_result(_desc(domain), 1, 1), "copy")
 
 
 def SetupAdvectionVelocityGrad(domain, timeStep, nonOrthoFlags=0, applyPressureGradient=False):
@@ -347,15 +395,60 @@ def csr_to_scipy(A):
 # one PISO step (PISOtorch_simulation.py:1054-1327, orthogonal branch)
 # ---------------------------------------------------------------------------
 
+def advect_scalar_step(domain, dt, flags, non_orthogonal, tol, max_iter, infos=None):
+    """The driver's passive-scalar advection (PISOtorch_simulation.py:1085-1167, no result reuse): the advection
+    matrix per channel when the channels differ (scalar viscosity per channel or boundary types per channel), else
+    one matrix for all channels and one multi-right-hand-side BiCGStab; the non-orthogonal branch reads the scalar
+    result (the block scalars copied in first).  Returns whether that matrix serves the velocity too (no scalar
+    viscosity, no block viscosity, channels not split: the driver then skips the velocity matrix)."""
+    n = domain.getTotalSize()
+    channels = domain.getPassiveScalarChannels()
+    split = not (domain.isPassiveScalarViscosityStatic() and domain.isAllFixedBoundariesPassiveScalarTypeStatic())
+    reuse = not (domain.hasPassiveScalarViscosity() or domain.hasBlockViscosity() or split)
+    mats = []
+    if split:
+        for ch in range(channels):
+            SetupAdvectionMatrix(domain, dt, flags, True, ch)
+            mats.append(domain.C.value.clone())
+    else:
+        SetupAdvectionMatrix(domain, dt, flags, True, 0)
+    if non_orthogonal:
+        CopyScalarResultFromBlocks(domain)
+    SetupAdvectionScalar(domain, dt, flags if non_orthogonal else 0)
+    x = torch.zeros_like(domain.scalarRHS)
+    if split:
+        keep = domain.C.value.clone()
+        for ch in range(channels):
+            domain.C.value.copy_(mats[ch])
+            xs = torch.zeros(n, dtype=x.dtype)
+            inf = SolveLinear(domain.C, domain.scalarRHS[ch * n:(ch + 1) * n].contiguous(), xs, max_iter, tol, 1,
+                              useBiCG=True)
+            x[ch * n:(ch + 1) * n] = xs
+            if infos is not None:
+                infos.append(("scalar", inf))
+        domain.C.value.copy_(keep)
+    else:
+        inf = SolveLinear(domain.C, domain.scalarRHS, x, max_iter, tol, 1, useBiCG=True)
+        if infos is not None:
+            infos.append(("scalar", inf))
+    domain.setScalarResult(x)
+    CopyScalarResultToBlocks(domain)
+    return reuse
+
+
 def piso_step(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, max_iter=5000,
-              normalize_pressure=True, infos=None, non_orthogonal=False, non_ortho_flags=25):
+              normalize_pressure=True, infos=None, non_orthogonal=False, non_ortho_flags=25, advect_scalar=True):
     """One PISO step of the orthogonal driver branch on a CPU twin, with the oracle's solvers (non_orthogonal: the
-    driver's non-orthogonal branch)."""
+    driver's non-orthogonal branch); passive scalars are advected first when the domain has them."""
     if non_orthogonal:
         return piso_step_non_ortho(domain, time_step, corrector_steps, advection_tol, pressure_tol, max_iter,
-                                   normalize_pressure, infos, non_ortho_flags)
+                                   normalize_pressure, infos, non_ortho_flags, advect_scalar=advect_scalar)
     dt = torch.tensor([time_step], dtype=domain.getDtype())
-    SetupAdvectionMatrix(domain, dt, 0)
+    reuse = False
+    if advect_scalar and domain.hasPassiveScalar():
+        reuse = advect_scalar_step(domain, dt, 0, False, advection_tol, max_iter, infos)
+    if not reuse:
+        SetupAdvectionMatrix(domain, dt, 0)
     SetupAdvectionVelocity(domain, dt, 0, False)
     x = torch.zeros_like(domain.velocityRHS)
     inf = SolveLinear(domain.C, domain.velocityRHS, x, max_iter, advection_tol, 1, useBiCG=True)
@@ -379,13 +472,17 @@ def piso_step(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure
 
 def piso_step_non_ortho(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, max_iter=5000,
                         normalize_pressure=True, infos=None, flags=25, advect_non_ortho_steps=1,
-                        pressure_non_ortho_steps=1):
-    """One PISO step of the driver's non-orthogonal branch (PISOtorch_simulation.py:1152-1300, no passive scalar,
-    no result reuse): the explicit non-orthogonal terms of the advection RHS read the velocity result (the block
-    velocities copied in first), those of the pressure RHS the pressure result left by the previous solve; the
-    pressure CG recomputes its residual every 100 iterations."""
+                        pressure_non_ortho_steps=1, advect_scalar=True):
+    """One PISO step of the driver's non-orthogonal branch (PISOtorch_simulation.py:1085-1300, no result reuse):
+    passive scalars first (when the domain has them), then the explicit non-orthogonal terms of the advection RHS
+    read the velocity result (the block velocities copied in first), those of the pressure RHS the pressure result
+    left by the previous solve; the pressure CG recomputes its residual every 100 iterations."""
     dt = torch.tensor([time_step], dtype=domain.getDtype())
-    SetupAdvectionMatrix(domain, dt, flags)
+    reuse = False
+    if advect_scalar and domain.hasPassiveScalar():
+        reuse = advect_scalar_step(domain, dt, flags, True, advection_tol, max_iter, infos)
+    if not reuse:
+        SetupAdvectionMatrix(domain, dt, flags)
     CopyVelocityResultFromBlocks(domain)
     for _ in range(advect_non_ortho_steps):
         SetupAdvectionVelocity(domain, dt, flags, False)

@@ -348,10 +348,29 @@ int oracle_build_csr_pattern(const pict_domain_desc *d) {
 /* Data a corner or neighbour value is read from: the GridDataType values the reference's
    stages pass (domain_structs_gpu.h:28-65).  Gradient variants scatter into the matching
    gradient tensors. */
-enum { GD_VELOCITY = 0, GD_VELOCITY_RESULT = 1, GD_PRESSURE_RESULT = 2, GD_IS_FIXED = 3 };
+enum { GD_VELOCITY = 0, GD_VELOCITY_RESULT = 1, GD_PRESSURE_RESULT = 2, GD_IS_FIXED = 3,
+       GD_SCALAR = 4, GD_SCALAR_RESULT = 5 /* passive scalar: p.w is the channel */ };
 #define GD_GRAD 8
 #define GD_BASE(t) ((t) & 7)
 static inline int gd_is_pressure(int t) { return GD_BASE(t) == GD_PRESSURE_RESULT; }
+static inline int gd_is_scalar(int t) { return GD_BASE(t) == GD_SCALAR || GD_BASE(t) == GD_SCALAR_RESULT; }
+
+/* getViscosity(forPassiveScalar) :1788-1800: the scalar viscosity of channel ch, or the domain's */
+static real scalar_viscosity(const Ctx *c, int ch) {
+    const real *v = (const real *)c->d->scalar_viscosity;
+    if (!v) return (real)c->d->viscosity;
+    return v[c->d->scalar_viscosity_static ? 0 : ch];
+}
+/* FixedBoundary passive scalar of channel ch (getFixedBoundaryData :2419-2437), p on the boundary grid */
+static real fixed_scalar(const pict_boundary_desc *fb, I4 p, int ch) {
+    if (fb->scalar_static) return ((const real *)fb->scalar)[ch];
+    p.a[3] = ch;
+    return ((const real *)fb->scalar)[flat4(p, fb->stride)];
+}
+/* getFixedBoundaryType (PASSIVE_SCALAR) :2360-2370 */
+static int fixed_scalar_bc(const pict_boundary_desc *fb, int ch) {
+    return ((fb->scalar_bc_mask >> (ch & 31)) & 1) ? PICT_BC_NEUMANN : PICT_BC_DIRICHLET;
+}
 
 static inline int face_sign(int face) { return (face & 1) ? 1 : -1; }
 static inline int i4_eq(const I4 a, const I4 b) {
@@ -452,6 +471,8 @@ static real block_data(const Ctx *c, int bi, I4 p, int type) {
         case GD_VELOCITY: return ((const real *)b->velocity)[flat4(p, b->stride)];
         case GD_VELOCITY_RESULT: return ((const real *)c->d->velocity_result)[(size_t)p.a[3] * c->N + global_index(p, b)];
         case GD_PRESSURE_RESULT: return ((const real *)c->d->pressure_result)[(size_t)p.a[3] * c->N + global_index(p, b)];
+        case GD_SCALAR: return ((const real *)b->scalar)[flat4(p, b->stride)];
+        case GD_SCALAR_RESULT: return ((const real *)c->d->scalar_result)[(size_t)p.a[3] * c->N + global_index(p, b)];
         default: return 0;
     }
 }
@@ -462,18 +483,22 @@ static real fixed_data(const Ctx *c, const pict_block_desc *b, I4 p, int face, i
     const pict_boundary_desc *fb = &b->bounds[face];
     if (fb->type != PICT_BOUNDARY_FIXED) return 0;
     if (GD_BASE(type) == GD_PRESSURE_RESULT) return 0;
-    if (fb->velocity_static) return ((const real *)fb->velocity)[p.a[3]];
     p.a[face >> 1] = 0;
+    if (gd_is_scalar(type)) return fixed_scalar(fb, p, p.a[3]);
+    if (fb->velocity_static) return ((const real *)fb->velocity)[p.a[3]];
     return ((const real *)fb->velocity)[flat4(p, fb->stride)];
 }
-/* getFixedBoundaryType :2351-2374 (velocity: the boundary's type; pressure: Neumann) */
-static int fixed_type(const pict_block_desc *b, int face, int type) {
+/* getFixedBoundaryType :2351-2374 (velocity: the boundary's type; pressure: Neumann; passive scalar: the type
+   of channel ch, pos.w in the reference) */
+static int fixed_type(const pict_block_desc *b, int face, int type, int ch) {
     const pict_boundary_desc *fb = &b->bounds[face];
     if (fb->type != PICT_BOUNDARY_FIXED) return PICT_BC_DIRICHLET;
     switch (GD_BASE(type)) {
         case GD_VELOCITY:
         case GD_VELOCITY_RESULT: return fb->velocity_bc;
         case GD_PRESSURE_RESULT: return PICT_BC_NEUMANN;
+        case GD_SCALAR:
+        case GD_SCALAR_RESULT: return fixed_scalar_bc(fb, ch);
         default: return PICT_BC_DIRICHLET;
     }
 }
@@ -543,7 +568,7 @@ static Corner corner_value(const Ctx *c, int bi, I4 pos, int dir1, int dir2, int
                 const pict_boundary_desc *bd = &wb->bounds[w[d].d1];
                 if (bd->type == PICT_BOUNDARY_FIXED) {
                     const int at2 = at_bound(w[d].pos, w[d].d2, wb);
-                    btype = fixed_type(wb, w[d].d1, type);
+                    btype = fixed_type(wb, w[d].d1, type, w[d].pos.a[3]);
                     Corner r;
                     r.num = 0; r.btype = btype;
                     if (at2) {
@@ -672,7 +697,7 @@ static real nonortho_rhs(const Ctx *c, int bi, I4 pos, int flags, int type, int 
     for (int face = 0; face < 2 * D; ++face) {
         const int axis = face >> 1, fs = face_sign(face);
         if (at_bound(pos, face, b) && prescribed(face, b)) {
-            if (fixed_type(b, face, type) == PICT_BC_NEUMANN || gd_is_pressure(type)) continue;
+            if (fixed_type(b, face, type, pos.a[3]) == PICT_BC_NEUMANN || gd_is_pressure(type)) continue;
             real bA[3];
             lap_neighbour_boundary(c, &b->bounds[face], pos, axis, bA);
             for (int i = 1; i < D; ++i) {
@@ -719,8 +744,10 @@ static real nonortho_rhs(const Ctx *c, int bi, I4 pos, int flags, int type, int 
     return S;
 }
 
-/* PISO_build_matrix :3601-3865 (velocity) */
-int oracle_setup_advection_matrix(const pict_domain_desc *d, double time_step, int flags) {
+/* PISO_build_matrix :3601-3865: the velocity matrix (sc = 0), or the matrix of passive-scalar channel ch (sc = 1,
+   forPassiveScalar: scalar viscosity :3683, :3726-3727, cross coefficients without viscosity times the scalar
+   viscosity :3742-3755, the channel's boundary types for corners :3746-3750 and prescribed faces :3815-3826) */
+static int advection_matrix(const pict_domain_desc *d, double time_step, int flags, int sc, int ch) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
     const real dt = (real)time_step;
     real *Cv = (real *)d->C_value, *A = (real *)d->A;
@@ -736,7 +763,7 @@ int oracle_setup_advection_matrix(const pict_domain_desc *d, double time_step, i
             const real det = cell_det(&c, b, pos);
             real diag = det / dt;
             idx[0] = f + b->global_offset;
-            const real visc_p = viscosity(&c, b, pos);
+            const real visc_p = sc ? scalar_viscosity(&c, ch) : viscosity(&c, b, pos);
             real alpha_p[3] = {1, 1, 1};
             for (int dd = 0; dd < c.D; ++dd) { I4 t = pos; t.a[3] = dd; alpha_p[dd] = laplace_orth(&c, b, t); }
             for (int face = 0; face < 2 * c.D; ++face) {
@@ -749,19 +776,25 @@ int oracle_setup_advection_matrix(const pict_domain_desc *d, double time_step, i
                     I4 t = neighbor_cell(&c, b, pos, face, &nb, bi);
                     real alpha = laplace_orth(&c, blk(&c, nb), t);
                     t.a[3] = 0;
-                    real visc_n = viscosity(&c, blk(&c, nb), t);
+                    real visc_n = sc ? scalar_viscosity(&c, ch) : viscosity(&c, blk(&c, nb), t);
                     idx[face + 1] = global_index(t, blk(&c, nb));
                     real vc = (alpha_p[dim] * visc_p + alpha * visc_n) * (real)0.5;
                     diag += vc; rv[face + 1] -= vc;
                     if (c.D > 1 && (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX))) {
                         real ai[12];
-                        interp_nonortho(&c, bi, pos, 1, 0, ai);
+                        interp_nonortho(&c, bi, pos, !sc, 0, ai);
+                        I4 cp = pos;
+                        if (sc) {
+                            for (int k = 0; k < 2 * c.D * (c.D - 1); ++k) ai[k] *= visc_p;
+                            cp.a[3] = ch;
+                        }
                         rv[0] = diag;
-                        nonortho_matrix_face(&c, bi, pos, face, ai, flags, (real)-1, GD_VELOCITY, 0, rv);
+                        nonortho_matrix_face(&c, bi, cp, face, ai, flags, (real)-1, sc ? GD_SCALAR : GD_VELOCITY, 0, rv);
                         diag = rv[0];
                     }
                 } else {
-                    int slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
+                    int bc = sc ? fixed_scalar_bc(&b->bounds[face], ch) : b->bounds[face].velocity_bc;
+                    int slip = bc == PICT_BC_DIRICHLET ? 0 : 1;
                     diag += (real)((1 - slip) * 2) * visc_p * alpha_p[dim];
                 }
             }
@@ -775,6 +808,55 @@ int oracle_setup_advection_matrix(const pict_domain_desc *d, double time_step, i
                 idx[s] = -1;
             }
             A[f + b->global_offset] = rv[0];
+        }
+    }
+    return PICT_OK;
+}
+
+int oracle_setup_advection_matrix(const pict_domain_desc *d, double time_step, int flags) {
+    return advection_matrix(d, time_step, flags, 0, 0);
+}
+int oracle_setup_advection_matrix_scalar(const pict_domain_desc *d, double time_step, int flags, int channel) {
+    if (d->scalar_channels < 1 || channel < 0 || channel >= d->scalar_channels) {
+        snprintf(g_err, sizeof g_err, "passive scalar channel %d out of range", channel);
+        return PICT_ERR_INVALID;
+    }
+    return advection_matrix(d, time_step, flags, 1, channel);
+}
+
+/* kPISO_build_scalar_advection_RHS :4080-4190: scalar_rhs of every channel */
+int oracle_setup_advection_scalar(const pict_domain_desc *d, double time_step, int flags) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    const real dt = (real)time_step;
+    real *rhs = (real *)d->scalar_rhs;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        #pragma omp parallel for schedule(static)
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            const real det = cell_det(&c, b, pos);
+            for (int ch = 0; ch < d->scalar_channels; ++ch) {
+                I4 t = pos; t.a[3] = ch;
+                const real visc = scalar_viscosity(&c, ch);
+                real r = det * ((const real *)b->scalar)[flat4(t, b->stride)] / dt;
+                for (int face = 0; face < 2 * c.D; ++face) {
+                    if (!at_bound(pos, face, b) || b->bounds[face].type != PICT_BOUNDARY_FIXED) continue;
+                    const pict_boundary_desc *fb = &b->bounds[face];
+                    const int ax = face >> 1;
+                    const real normal = (face & 1) ? 1 : -1;
+                    I4 bp = pos; bp.a[ax] = 0;
+                    const real sv = fixed_scalar(fb, bp, ch);    /* value (Dirichlet) or gradient (Neumann) */
+                    bp.a[3] = ax;
+                    const real alpha = laplace_orth_fixed(&c, fb, bp);
+                    const real flux = fixed_contravariant(&c, fb, bp) * normal;
+                    r -= sv * flux;
+                    if (fixed_scalar_bc(fb, ch) == PICT_BC_DIRICHLET) r += sv * visc * 2 * alpha;
+                    else r += sv * visc;
+                }
+                if (c.D > 1) r -= nonortho_rhs(&c, bi, t, flags, GD_SCALAR_RESULT, 0, 0) * visc;
+                r /= det;
+                rhs[(size_t)ch * c.N + b->global_offset + f] = r;
+            }
         }
     }
     return PICT_OK;
@@ -1063,6 +1145,20 @@ int oracle_copy_pressure_result(const pict_domain_desc *d, int to_blocks) {
     }
     return PICT_OK;
 }
+/* CopyScalarResult{To,From}Blocks :6543-6610 (grad = 1: CopyScalarResultGrad{To,From}Blocks :6737-6800) */
+int oracle_copy_scalar_result(const pict_domain_desc *d, int to_blocks, int grad) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int ch = 0; ch < d->scalar_channels; ++ch) {
+            real *bv = (real *)(grad ? b->scalar_grad : b->scalar) + (size_t)ch * b->stride[3];
+            real *gv = (real *)(grad ? d->scalar_result_grad : d->scalar_result) + (size_t)ch * c.N + b->global_offset;
+            if (to_blocks) memcpy(bv, gv, sizeof(real) * b->stride[3]);
+            else memcpy(gv, bv, sizeof(real) * b->stride[3]);
+        }
+    }
+    return PICT_OK;
+}
 
 /* ------------------------------------------------------------------ */
 /* backward stages (vector-Jacobian products), serial                   */
@@ -1197,15 +1293,24 @@ static void scatter_block_data(const Ctx *c, int bi, I4 p, int type, real g) {
     switch (GD_BASE(type)) {
         case GD_VELOCITY_RESULT: dst = (real *)c->d->velocity_result_grad; break;
         case GD_PRESSURE_RESULT: dst = (real *)c->d->pressure_result_grad; break;
+        case GD_SCALAR_RESULT: dst = (real *)c->d->scalar_result_grad; break;
         default: return;
     }
     if (dst) dst[(size_t)p.a[3] * c->N + global_index(p, b)] += g;
 }
-/* scatterFixedBoundaryData :2448-2478 (FixedBoundary velocity gradients) */
+/* scatterFixedBoundaryData :2448-2478 (FixedBoundary velocity and passive-scalar gradients) */
 static void scatter_fixed_data(const Ctx *c, const pict_block_desc *b, I4 p, int face, int type, real g) {
     (void)c;
     const pict_boundary_desc *fb = &b->bounds[face];
     if (GD_BASE(type) == GD_IS_FIXED || fb->type != PICT_BOUNDARY_FIXED) return;
+    if (gd_is_scalar(type)) {
+        if (!fb->scalar_grad) return;
+        real *sdst = (real *)fb->scalar_grad;
+        if (fb->scalar_static) { sdst[p.a[3]] += g; return; }
+        p.a[face >> 1] = 0;
+        sdst[flat4(p, fb->stride)] += g;
+        return;
+    }
     if (GD_BASE(type) != GD_VELOCITY_RESULT && GD_BASE(type) != GD_VELOCITY) return;
     if (!fb->velocity_grad) return;
     real *dst = (real *)fb->velocity_grad;
@@ -1421,7 +1526,7 @@ static void nonortho_rhs_grad(const Ctx *c, int bi, I4 pos, int flags, int type,
     for (int face = 0; face < 2 * D; ++face) {
         const int axis = face >> 1, fs = face_sign(face);
         if (at_bound(pos, face, b) && prescribed(face, b)) {
-            if (fixed_type(b, face, type) == PICT_BC_NEUMANN || gd_is_pressure(type)) continue;
+            if (fixed_type(b, face, type, pos.a[3]) == PICT_BC_NEUMANN || gd_is_pressure(type)) continue;
             real bA[3];
             lap_neighbour_boundary(c, &b->bounds[face], pos, axis, bA);
             for (int i = 1; i < D; ++i) {
@@ -1491,7 +1596,19 @@ static void nonortho_rhs_grad(const Ctx *c, int bi, I4 pos, int flags, int type,
 }
 
 /* PISO_build_matrix_GRAD :3869-4075 */
-int oracle_setup_advection_matrix_grad(const pict_domain_desc *d, double time_step, int flags) {
+/* scatterViscosity_GRAD(forPassiveScalar) :1834-1849: into the scalar viscosity gradient of channel ch, or the
+   domain viscosity gradient when there is none */
+static void scatter_scalar_viscosity(const Ctx *c, int ch, real g) {
+    real *dst = (real *)c->d->scalar_viscosity_grad;
+    if (dst) dst += c->d->scalar_viscosity_static ? 0 : ch;
+    else dst = (real *)c->d->viscosity_grad;
+    if (dst) *dst += g;
+}
+
+/* PISO_build_matrix_GRAD :3869-4075: the velocity matrix (sc = 0) or that of passive-scalar channel ch (sc = 1:
+   viscosity gradients to the scalar viscosity, the cross coefficients' gradient times their viscosity-free value
+   :4004-4011, the channel's boundary types) */
+static int advection_matrix_grad(const pict_domain_desc *d, double time_step, int flags, int sc, int ch) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
     (void)time_step;
     const real *Cg = (const real *)d->C_grad, *Ag = (const real *)d->A_grad;
@@ -1522,21 +1639,91 @@ int oracle_setup_advection_matrix_grad(const pict_domain_desc *d, double time_st
                     real vcg = diag_grad - cg[face + 1];
                     visc_grad += (real)0.5 * alpha_p[dim] * vcg;
                     t.a[3] = 0;
-                    scatter_viscosity(&c, blk(&c, nb), t, (real)0.5 * alpha * vcg);
+                    if (sc) scatter_scalar_viscosity(&c, ch, (real)0.5 * alpha * vcg);
+                    else scatter_viscosity(&c, blk(&c, nb), t, (real)0.5 * alpha * vcg);
                     if (no) {
                         real cgd[7];
                         for (int k = 0; k < 7; ++k) cgd[k] = cg[k];
                         cgd[0] = diag_grad;
-                        nonortho_matrix_face_grad(&c, bi, pos, face, ai, flags, (real)-1, GD_VELOCITY, 0, cgd, ai_grad);
+                        I4 cp = pos;
+                        if (sc) cp.a[3] = ch;
+                        nonortho_matrix_face_grad(&c, bi, cp, face, ai, flags, (real)-1, sc ? GD_SCALAR : GD_VELOCITY, 0,
+                                                  cgd, ai_grad);
                     }
                 } else {
-                    real slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
+                    int bc = sc ? fixed_scalar_bc(&b->bounds[face], ch) : b->bounds[face].velocity_bc;
+                    real slip = bc == PICT_BC_DIRICHLET ? 0 : 1;
                     visc_grad += (1 - slip) * 2 * alpha_p[dim] * diag_grad;
                 }
             }
-            if (no) scatter_interp_nonortho(&c, bi, pos, ai_grad, 1, 0);
+            if (no) {
+                if (sc) {
+                    for (int k = 0; k < 2 * c.D * (c.D - 1); ++k) visc_grad += ai_grad[k] * ai[k];
+                } else {
+                    scatter_interp_nonortho(&c, bi, pos, ai_grad, 1, 0);
+                }
+            }
             if (b->velocity_grad) scatter_fluxes(&c, bi, pos, fg, NULL);
-            scatter_viscosity(&c, b, pos, visc_grad);
+            if (sc) scatter_scalar_viscosity(&c, ch, visc_grad);
+            else scatter_viscosity(&c, b, pos, visc_grad);
+        }
+    }
+    return PICT_OK;
+}
+int oracle_setup_advection_matrix_grad(const pict_domain_desc *d, double time_step, int flags) {
+    return advection_matrix_grad(d, time_step, flags, 0, 0);
+}
+int oracle_setup_advection_matrix_scalar_grad(const pict_domain_desc *d, double time_step, int flags, int channel) {
+    if (d->scalar_channels < 1 || channel < 0 || channel >= d->scalar_channels) {
+        snprintf(g_err, sizeof g_err, "passive scalar channel %d out of range", channel);
+        return PICT_ERR_INVALID;
+    }
+    return advection_matrix_grad(d, time_step, flags, 1, channel);
+}
+
+/* kPISO_build_scalar_advection_RHS_GRAD :4193-4278 */
+int oracle_setup_advection_scalar_grad(const pict_domain_desc *d, double time_step, int flags) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    const real dt = (real)time_step;
+    const real *rg = (const real *)d->scalar_rhs_grad;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            real det = cell_det(&c, b, pos);
+            for (int ch = 0; ch < d->scalar_channels; ++ch) {
+                I4 t = pos; t.a[3] = ch;
+                const real visc = scalar_viscosity(&c, ch);
+                real G = rg[(size_t)ch * c.N + b->global_offset + f] / det;
+                real vg = 0;
+                if (c.D > 1 && (flags & (PICT_NON_ORTHO_DIRECT_RHS | PICT_NON_ORTHO_DIAGONAL_RHS))) {
+                    nonortho_rhs_grad(&c, bi, t, flags, GD_SCALAR_RESULT, 0, 0, -G * visc);
+                    vg -= G * nonortho_rhs(&c, bi, t, flags, GD_SCALAR_RESULT, 0, 0);
+                }
+                for (int face = 0; face < 2 * c.D; ++face) {
+                    if (!at_bound(pos, face, b) || b->bounds[face].type != PICT_BOUNDARY_FIXED) continue;
+                    const pict_boundary_desc *fb = &b->bounds[face];
+                    const int ax = face >> 1;
+                    const real normal = (face & 1) ? 1 : -1;
+                    I4 bp = pos; bp.a[ax] = 0;
+                    const real sv = fixed_scalar(fb, bp, ch);
+                    bp.a[3] = ax;
+                    const real alpha = laplace_orth_fixed(&c, fb, bp);
+                    const real flux = fixed_contravariant(&c, fb, bp) * normal;
+                    real s_grad = -flux * G;
+                    if (fixed_scalar_bc(fb, ch) == PICT_BC_DIRICHLET) {
+                        s_grad += visc * 2 * alpha * G;
+                        vg += sv * 2 * alpha * G;
+                    } else {
+                        s_grad += visc * G;
+                        vg += sv * G;
+                    }
+                    scatter_fixed_contravariant(&c, fb, bp, -sv * G * normal);
+                    scatter_fixed_data(&c, b, t, face, GD_SCALAR, s_grad);
+                }
+                scatter_scalar_viscosity(&c, ch, vg);
+                if (b->scalar_grad) ((real *)b->scalar_grad)[flat4(t, b->stride)] = G * det / dt;
+            }
         }
     }
     return PICT_OK;

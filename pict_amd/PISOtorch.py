@@ -10,14 +10,14 @@ include/pict_hip.h).  This module only owns tensors and packs the descriptor
 the kernels read; there is no CPU fallback: a missing library or a non-GPU
 domain raises.
 
-Differences from the reference are deliberate and listed in DESIGN.md;
-the ones visible from Python:
+Differences from the reference visible from Python:
   * Only FixedBoundary / ConnectedBoundary / PeriodicBoundary exist (the
     reference's Static/VaryingDirichletBoundary are refused by its own
     MakeClosedBoundary, domain_structs.cpp:1866-1868).
-  * Non-orthogonal correction terms, passive-scalar advection, ILU-
-    preconditioned BiCGStab and the FVM velocity correctors raise
+  * ILU-preconditioned BiCGStab and the FVM velocity correctors raise
     NotImplementedError (PictUnsupportedError).
+  * Per-channel passive-scalar boundary types are limited to 32 channels
+    (one type for all channels has no limit).
 """
 import ctypes
 import enum
@@ -493,7 +493,7 @@ class FixedBoundary(Boundary):
                 return d
         return 0
 
-    # --- passive scalar (held, not advected by this build) ---
+    # --- passive scalar ---
     passiveScalar = property(lambda self: self._passiveScalar)
     passiveScalarTypes = property(lambda self: self._passiveScalarTypes)
     passiveScalarGrad = property(lambda self: self._passiveScalar_grad)
@@ -511,8 +511,19 @@ class FixedBoundary(Boundary):
         return self._passiveScalarTypes is None or len(self._passiveScalarTypes) < 2
 
     def setPassiveScalar(self, t):
+        dims = self._domain.getSpatialDims()
         self._domain._check_tensor_device(t)
+        _check(t.is_contiguous(), "passive scalar must be contiguous")
+        _check(t.size(0) == 1, "Batches are not yet supported (passive scalar).")
         _check(t.size(1) == self._domain.getPassiveScalarChannels(), "Passive scalar channels must match domain.")
+        _check(t.dim() == 2 or t.dim() == 2 + dims, "Passive scalar must be static (shape NC) or varying (shape "
+               "NCDHW).")
+        _check(t.dtype == self._domain.getDtype(), "Passive scalar dtype must match domain.")
+        if t.dim() > 2:
+            s = _spatial_size_xyz(t, dims, True)
+            if self._size is None:
+                self._size = [s[0], s[1], s[2], dims]
+            _check(list(s[:3]) == list(self._size[:3]), "Passive scalar shape must match the boundary's size.")
         self._passiveScalar = t
         self._changed = True
 
@@ -525,10 +536,30 @@ class FixedBoundary(Boundary):
             self.setPassiveScalar(torch.zeros(_nc_shape(self._domain.getSpatialDims(), c, self._size), **opts))
 
     def setPassiveScalarType(self, types):
+        """FixedBoundary::setPassiveScalarType: one type for every channel, or one per channel."""
         if isinstance(types, (list, tuple)):
-            self._passiveScalarTypes = [BoundaryConditionType(t) for t in types]
+            ts = [BoundaryConditionType(t) for t in types]
+            _check(len(ts) == 1 or len(ts) == self._domain.getPassiveScalarChannels(),
+                   "Passive scalar boundary types must be static or match passive scalar channels.")
+            _check(len(ts) <= 32, "libpict_hip supports per-channel passive scalar boundary types for at most 32 "
+                   "channels.")
+            self._passiveScalarTypes = ts
         else:
             self._passiveScalarTypes = [BoundaryConditionType(types)]
+        self._changed = True
+
+    def _scalar_bc_mask(self):
+        """Bit c set: channel c is NEUMANN (a static type sets every bit alike)."""
+        ts = self._passiveScalarTypes
+        if not ts:
+            return 0
+        if len(ts) == 1:
+            return -1 if ts[0] == BoundaryConditionType.NEUMANN else 0
+        m = 0
+        for c, t in enumerate(ts):
+            if t == BoundaryConditionType.NEUMANN:
+                m |= 1 << c
+        return m - (1 << 32) if m >= (1 << 31) else m
 
     def setPassiveScalarGrad(self, t):
         self._passiveScalar_grad = t
@@ -873,7 +904,7 @@ class Block:
     def CreatePressureGrad(self):
         self.setPressureGrad(torch.zeros_like(self.pressure))
 
-    # --- passive scalar (held; advection not implemented) ---
+    # --- passive scalar ---
     def setPassiveScalar(self, s):
         self._check_data(s, self._domain.getPassiveScalarChannels(), False, "PassiveScalar")
         self.passiveScalar = s
@@ -1208,13 +1239,15 @@ class Block:
             return None if x is None else (x.data_ptr(), x.dim())
         sig = [self.globalOffset, self.csrOffset, t(self.velocity), t(self.pressure), t(self._viscosity),
                t(self.velocitySource), t(self._transform), t(self._faceTransform), t(self._velocity_grad),
-               t(self._pressure_grad), t(self._viscosity_grad), t(self.velocitySource_grad)]
+               t(self._pressure_grad), t(self._viscosity_grad), t(self.velocitySource_grad), t(self.passiveScalar),
+               t(self._passiveScalar_grad)]
         for bd in self.boundaries:
             if bd.type == BoundaryType.CONNECTED:
                 sig.append((id(bd), block_index_of(bd.getConnectedBlock()), tuple(bd.axes)))
             elif bd.type == BoundaryType.FIXED:
                 sig.append((id(bd), t(bd.velocity), t(bd._velocity_grad), t(bd.transform), int(bd.velocityType),
-                            None if bd._size is None else tuple(bd._size)))
+                            None if bd._size is None else tuple(bd._size), t(bd._passiveScalar),
+                            t(bd._passiveScalar_grad), bd._scalar_bc_mask()))
             else:
                 sig.append(int(bd.type))
         return tuple(sig)
@@ -1249,6 +1282,8 @@ class Block:
         b.pressure_grad = _ptr(self._pressure_grad)
         b.viscosity_grad = _ptr(self._viscosity_grad)
         b.velocity_source_grad = _ptr(self.velocitySource_grad)
+        b.scalar = _ptr(self.passiveScalar)
+        b.scalar_grad = _ptr(self._passiveScalar_grad)
         for f, bd in enumerate(self.boundaries):
             d = b.bounds[f]
             d.type = int(bd.type)
@@ -1266,6 +1301,10 @@ class Block:
                 d.velocity = _ptr(bd.velocity)
                 d.velocity_grad = _ptr(bd._velocity_grad)
                 d.transform = _ptr(bd.transform)
+                d.scalar_static = int(bd.isPassiveScalarStatic())
+                d.scalar_bc_mask = bd._scalar_bc_mask()
+                d.scalar = _ptr(bd._passiveScalar)
+                d.scalar_grad = _ptr(bd._passiveScalar_grad)
         return b
 
 
@@ -1372,10 +1411,44 @@ class Domain:
         return any(b.hasViscosity() for b in self.blocks)
 
     def setScalarViscosity(self, v):
+        """Domain::setScalarViscosity (domain_structs.cpp:2530-2539): a device tensor [1] or [channels]."""
+        _check(isinstance(v, torch.Tensor) and v.dim() == 1, "Scalar viscosity must be 1D.")
+        _check(v.size(0) == 1 or v.size(0) == self._channels,
+               "Scalar viscosity must be static (a scalar) or match passive scalar channels.")
+        _check(v.dtype == self._dtype, "Data type of scalar viscosity does not match.")
+        self._check_tensor_device(v)
+        _check(v.is_contiguous(), "Scalar viscosity must be contiguous.")
         self.passiveScalarViscosity = v
+        self._changed = True
 
     def clearScalarViscosity(self):
         self.passiveScalarViscosity = None
+        self.passiveScalarViscosity_grad = None
+        self._changed = True
+
+    passiveScalarViscosityGrad = property(lambda self: getattr(self, "passiveScalarViscosity_grad", None))
+
+    def setPassiveScalarViscosityGrad(self, t):
+        """domain_structs.cpp:2884-2893."""
+        _check(self.hasPassiveScalarViscosity(), "Domain has no passive scalar viscosity.")
+        _check(t.dim() == 1 and t.size(0) == self.passiveScalarViscosity.size(0),
+               "Scalar viscosity grad shape must match Scalar viscosity.")
+        _check(t.dtype == self._dtype, "Data type of scalar viscosity does not match.")
+        self._check_tensor_device(t)
+        self.passiveScalarViscosity_grad = t
+        self._changed = True
+
+    def clearPassiveScalarViscosityGrad(self):
+        if getattr(self, "passiveScalarViscosity_grad", None) is not None:
+            self.passiveScalarViscosity_grad = None
+            self._changed = True
+
+    def CreatePassiveScalarViscosityGrad(self):
+        """domain_structs.cpp:2900-2907."""
+        if self.hasPassiveScalarViscosity():
+            self.setPassiveScalarViscosityGrad(torch.zeros_like(self.passiveScalarViscosity))
+        else:
+            self.clearPassiveScalarViscosityGrad()
 
     def hasPassiveScalarViscosity(self):
         return self.passiveScalarViscosity is not None
@@ -1405,6 +1478,7 @@ class Domain:
         self.setViscosityGrad(torch.zeros(1, dtype=self._dtype, device=self._device))
         for b in self.blocks:
             b.CreateViscosityGrad()
+        self.CreatePassiveScalarViscosityGrad()
 
     # --- blocks ---
     def CreateBlock(self, velocity=None, pressure=None, passiveScalar=None, vertexCoordinates=None, name="Block"):
@@ -1585,6 +1659,20 @@ class Domain:
     def CreatePressureResultGrad(self):
         self.setPressureResultGrad(self._vec(self._totalSize))
 
+    def setScalarRHSGrad(self, t):
+        self._set_vec("scalarRHS_grad", t, self._totalSize * self._channels)
+
+    def CreateScalarRHSGrad(self):
+        self.setScalarRHSGrad(self._vec(self._totalSize * self._channels))
+
+    def setScalarResultGrad(self, t):
+        self._set_vec("scalarResult_grad", t, self._totalSize * self._channels)
+
+    def CreateScalarResultGrad(self):
+        self.setScalarResultGrad(self._vec(self._totalSize * self._channels))
+
+    scalarRHSGrad = property(lambda self: self.scalarRHS_grad)
+    scalarResultGrad = property(lambda self: self.scalarResult_grad)
     velocityRHSGrad = property(lambda self: self.velocityRHS_grad)
     velocityResultGrad = property(lambda self: self.velocityResult_grad)
     pressureRHSGrad = property(lambda self: self.pressureRHS_grad)
@@ -1656,6 +1744,11 @@ class Domain:
                 g(self.P_grad.value if self.P_grad is not None else None), g(self.A_grad), g(self.velocityRHS_grad),
                 g(self.velocityResult_grad), g(self.pressureRHS_grad), g(self.pressureRHSdiv_grad),
                 g(self.pressureResult_grad))
+        sv = self.passiveScalarViscosity
+        sptrs = (self._channels, int(sv is not None and sv.size(0) == 1), g(sv),
+                 g(getattr(self, "passiveScalarViscosity_grad", None)), g(self.scalarRHS), g(self.scalarResult),
+                 g(self.scalarRHS_grad), g(self.scalarResult_grad))
+        ptrs = ptrs + sptrs
         sig = (ptrs, tuple(id(b) for b in blocks))
         cached = getattr(self, "_packed", None)
         if cached is not None and cached[0] == sig:
@@ -1674,7 +1767,9 @@ class Domain:
         d.blocks = ctypes.cast(arr, ctypes.POINTER(_abi.BlockDesc))
         (d.csr_row, d.csr_index, d.C_value, d.P_value, d.A, d.velocity_rhs, d.velocity_result, d.pressure_rhs,
          d.pressure_rhs_div, d.pressure_result, d.viscosity_grad, d.C_grad, d.P_grad, d.A_grad, d.velocity_rhs_grad,
-         d.velocity_result_grad, d.pressure_rhs_grad, d.pressure_rhs_div_grad, d.pressure_result_grad) = ptrs[3:]
+         d.velocity_result_grad, d.pressure_rhs_grad, d.pressure_rhs_div_grad, d.pressure_result_grad) = ptrs[3:22]
+        (d.scalar_channels, d.scalar_viscosity_static, d.scalar_viscosity, d.scalar_viscosity_grad, d.scalar_rhs,
+         d.scalar_result, d.scalar_rhs_grad, d.scalar_result_grad) = ptrs[22:]
         self._packed = (sig, d, arr)
         return d, arr
 
@@ -1853,16 +1948,27 @@ def _call(domain, name, *args):
     _abi.check(getattr(lib, name)(domain._atlas, *args, _stream(domain.getDevice())), name)
 
 
+def _check_scalar(domain, channel=None):
+    """PISO_multiblock_cuda_kernel.cu:4513-4516."""
+    _check(domain.hasPassiveScalar(), "domain does not have passive scalar.")
+    if channel is not None:
+        _check(0 <= channel < domain.getPassiveScalarChannels(), "Passive scalar channel index out of bounds.")
+
+
 def SetupAdvectionMatrix(domain, timeStep, nonOrthoFlags, forPassiveScalar=False, passiveScalarChannel=0):
     dt = _check_domain(domain, timeStep)
+    flags = domain._effective_non_ortho_flags(nonOrthoFlags)
     if forPassiveScalar:
-        raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
-    _call(domain, "pict_setup_advection_matrix", dt, domain._effective_non_ortho_flags(nonOrthoFlags))
+        _check_scalar(domain, passiveScalarChannel)
+        _call(domain, "pict_setup_advection_matrix_scalar", dt, flags, int(passiveScalarChannel))
+        return
+    _call(domain, "pict_setup_advection_matrix", dt, flags)
 
 
 def SetupAdvectionScalar(domain, timeStep, nonOrthoFlags):
-    _check_domain(domain, timeStep)
-    raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+    dt = _check_domain(domain, timeStep)
+    _check_scalar(domain)
+    _call(domain, "pict_setup_advection_scalar", dt, domain._effective_non_ortho_flags(nonOrthoFlags))
 
 
 def SetupAdvectionVelocity(domain, timeStep, nonOrthoFlags, applyPressureGradient=False):
@@ -1927,10 +2033,15 @@ def CopyPressureResultFromBlocks(domain):
 
 
 def CopyScalarResultToBlocks(domain):
-    raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+    _check_domain(domain)
+    _check_scalar(domain)
+    _call(domain, "pict_copy_scalar_result_to_blocks")
 
 
-CopyScalarResultFromBlocks = CopyScalarResultToBlocks
+def CopyScalarResultFromBlocks(domain):
+    _check_domain(domain)
+    _check_scalar(domain)
+    _call(domain, "pict_copy_scalar_result_from_blocks")
 
 
 # ----------------------------------------------------------------------------
@@ -1940,14 +2051,18 @@ CopyScalarResultFromBlocks = CopyScalarResultToBlocks
 
 def SetupAdvectionMatrixGrad(domain, timeStep, nonOrthoFlags, forPassiveScalar=False, passiveScalarChannel=0):
     dt = _check_domain(domain, timeStep)
+    flags = domain._effective_non_ortho_flags(nonOrthoFlags)
     if forPassiveScalar:
-        raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
-    _call(domain, "pict_setup_advection_matrix_grad", dt, domain._effective_non_ortho_flags(nonOrthoFlags))
+        _check_scalar(domain, passiveScalarChannel)
+        _call(domain, "pict_setup_advection_matrix_scalar_grad", dt, flags, int(passiveScalarChannel))
+        return
+    _call(domain, "pict_setup_advection_matrix_grad", dt, flags)
 
 
 def SetupAdvectionScalarGrad(domain, timeStep, nonOrthoFlags):
-    _check_domain(domain, timeStep)
-    raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+    dt = _check_domain(domain, timeStep)
+    _check_scalar(domain)
+    _call(domain, "pict_setup_advection_scalar_grad", dt, domain._effective_non_ortho_flags(nonOrthoFlags))
 
 
 def SetupAdvectionVelocityGrad(domain, timeStep, nonOrthoFlags, applyPressureGradient=False):
@@ -2012,10 +2127,15 @@ def _subtract_mean(t):
 
 
 def CopyScalarResultGradFromBlocks(domain):
-    raise PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+    _check_domain(domain)
+    _check_scalar(domain)
+    _call(domain, "pict_copy_scalar_result_grad_from_blocks")
 
 
-CopyScalarResultGradToBlocks = CopyScalarResultGradFromBlocks
+def CopyScalarResultGradToBlocks(domain):
+    _check_domain(domain)
+    _check_scalar(domain)
+    _call(domain, "pict_copy_scalar_result_grad_to_blocks")
 
 
 # ----------------------------------------------------------------------------

@@ -187,10 +187,18 @@ def _domain_entries(domain):
     yield ("D", "P_GRAD", lambda: d.PGrad.value, d.PGrad.setValue, None, False)
     yield ("D", "VISCOSITY", lambda: d.viscosity, d.setViscosity, None, False)
     yield ("D", "VISCOSITY_GRAD", lambda: d.viscosityGrad, None, None, True)
+    yield ("D", "PASSIVE_SCALAR_VISCOSITY", lambda: d.passiveScalarViscosity, d.setScalarViscosity,
+           d.clearScalarViscosity, False)
+    yield ("D", "PASSIVE_SCALAR_VISCOSITY_GRAD", lambda: d.passiveScalarViscosityGrad,
+           d.setPassiveScalarViscosityGrad, None, False)
+    yield ("D", "PASSIVE_SCALAR_RESULT", lambda: d.scalarResult, d.setScalarResult, None, False)
+    yield ("D", "PASSIVE_SCALAR_RESULT_GRAD", lambda: d.scalarResultGrad, d.setScalarResultGrad, None, False)
     yield ("D", "VELOCITY_RESULT", lambda: d.velocityResult, d.setVelocityResult, None, False)
     yield ("D", "VELOCITY_RESULT_GRAD", lambda: d.velocityResultGrad, d.setVelocityResultGrad, None, False)
     yield ("D", "PRESSURE_RESULT", lambda: d.pressureResult, d.setPressureResult, None, False)
     yield ("D", "PRESSURE_RESULT_GRAD", lambda: d.pressureResultGrad, d.setPressureResultGrad, None, False)
+    yield ("D", "PASSIVE_SCALAR_RHS", lambda: d.scalarRHS, d.setScalarRHS, None, False)
+    yield ("D", "PASSIVE_SCALAR_RHS_GRAD", lambda: d.scalarRHSGrad, d.setScalarRHSGrad, None, False)
     yield ("D", "VELOCITY_RHS", lambda: d.velocityRHS, d.setVelocityRHS, None, False)
     yield ("D", "VELOCITY_RHS_GRAD", lambda: d.velocityRHSGrad, d.setVelocityRHSGrad, None, False)
     yield ("D", "PRESSURE_RHS", lambda: d.pressureRHS, d.setPressureRHS, None, False)
@@ -206,12 +214,17 @@ def _domain_entries(domain):
         yield (sec, "VELOCITY_SOURCE", lambda b=b: b.velocitySource, b.setVelocitySource, b.clearVelocitySource,
                False)
         yield (sec, "VELOCITY_SOURCE_GRAD", lambda b=b: b.velocitySourceGrad, b.setVelocitySourceGrad, None, False)
+        yield (sec, "PASSIVE_SCALAR", lambda b=b: b.passiveScalar, b.setPassiveScalar, None, False)
+        yield (sec, "PASSIVE_SCALAR_GRAD", lambda b=b: b.passiveScalarGrad, b.setPassiveScalarGrad, None, False)
         yield (sec, "PRESSURE", lambda b=b: b.pressure, b.setPressure, None, False)
         yield (sec, "PRESSURE_GRAD", lambda b=b: b.pressureGrad, b.setPressureGrad, None, False)
         for fi, fb in b.getFixedBoundaries():
             bsec = ("F", bi, fi)
             yield (bsec, "BOUNDARY_VELOCITY", lambda fb=fb: fb.velocity, fb.setVelocity, None, False)
             yield (bsec, "BOUNDARY_VELOCITY_GRAD", lambda fb=fb: fb.velocityGrad, fb.setVelocityGrad, None, False)
+            yield (bsec, "BOUNDARY_PASSIVE_SCALAR", lambda fb=fb: fb.passiveScalar, fb.setPassiveScalar, None, False)
+            yield (bsec, "BOUNDARY_PASSIVE_SCALAR_GRAD", lambda fb=fb: fb.passiveScalarGrad, fb.setPassiveScalarGrad,
+                   None, False)
 
 
 def _entries(domain):
@@ -309,17 +322,18 @@ _EXCLUDED_GRADIENTS["SetupAdvectionMatrix"] = (set(), set())
 
 
 def SetupAdvectionMatrix(domain, time_step, non_ortho_flags, forPassiveScalar=False, passiveScalarChannel=0):
-    """PISOtorch_diff.py:377-441: (A, C.value) from block velocity, boundary velocity and viscosities."""
-    if forPassiveScalar:
-        raise PISOtorch.PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
-    tracked = ["VELOCITY", "BOUNDARY_VELOCITY", "VISCOSITY", "VISCOSITY_BLOCK"]
+    """PISOtorch_diff.py:377-441: (A, C.value) from block velocity, boundary velocity and viscosities (the velocity
+    matrix), or the matrix of passive-scalar channel `passiveScalarChannel` (with the scalar viscosity)."""
+    tracked = ["VELOCITY", "BOUNDARY_VELOCITY", "VISCOSITY"]
+    tracked.append("PASSIVE_SCALAR_VISCOSITY" if forPassiveScalar else "VISCOSITY_BLOCK")
     name = "SetupAdvectionMatrix"
 
     def fwd(d):
         d.CreateA()
         d.C.CreateValue()
         d.UpdateDomainData()
-        PISOtorch.SetupAdvectionMatrix(d, time_step, non_ortho_flags)
+        PISOtorch.SetupAdvectionMatrix(d, time_step, non_ortho_flags, forPassiveScalar=forPassiveScalar,
+                                       passiveScalarChannel=passiveScalarChannel)
 
     def bwd(d, dt, grads):
         _set_gradient_input(grads[0], d.setAGrad, name, "A_GRAD")
@@ -328,7 +342,8 @@ def SetupAdvectionMatrix(domain, time_step, non_ortho_flags, forPassiveScalar=Fa
         d.CreateVelocityGradOnBlocks()
         d.CreateVelocityGradOnBoundaries()
         d.UpdateDomainData()
-        PISOtorch.SetupAdvectionMatrixGrad(d, dt, non_ortho_flags)
+        PISOtorch.SetupAdvectionMatrixGrad(d, dt, non_ortho_flags, forPassiveScalar=forPassiveScalar,
+                                           passiveScalarChannel=passiveScalarChannel)
 
     fn = _stage(name, tracked, [], fwd, bwd, lambda d: (d.A, d.C.value))
     return _apply(fn, domain, time_step, tracked)
@@ -451,14 +466,76 @@ def CopyPressureResultToBlocks(domain):
 
 
 def CopyScalarResultToBlocks(domain):
-    raise PISOtorch.PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+    """PISOtorch_diff.py:616-651: block passive scalars from scalarResult."""
+    def fwd(d):
+        d.CreatePassiveScalarOnBlocks()
+        d.UpdateDomainData()
+        PISOtorch.CopyScalarResultToBlocks(d)
+
+    def bwd(d, grads):
+        for b, g in zip(d.getBlocks(), grads):
+            b.setPassiveScalarGrad(g.contiguous())
+        d.CreateScalarResultGrad()
+        d.UpdateDomainData()
+        PISOtorch.CopyScalarResultGradFromBlocks(d)
+        return [d.scalarResultGrad]
+
+    fn = _copy_stage(fwd, bwd, None, lambda d: tuple(b.passiveScalar for b in d.getBlocks()))
+    _CopyFunction.apply(fn, domain, domain.scalarResult)
 
 
-CopyScalarResultFromBlocks = CopyScalarResultToBlocks
+def CopyScalarResultFromBlocks(domain):
+    """PISOtorch_diff.py:653-695: scalarResult from the block passive scalars (its backward is
+    CopyScalarResultGradToBlocks, which the reference's wrapper leaves unimplemented)."""
+    def fwd(d):
+        d.CreateScalarResult()
+        d.UpdateDomainData()
+        PISOtorch.CopyScalarResultFromBlocks(d)
+
+    def bwd(d, grads):
+        d.setScalarResultGrad(grads[0].contiguous())
+        d.CreatePassiveScalarGradOnBlocks()
+        d.UpdateDomainData()
+        PISOtorch.CopyScalarResultGradToBlocks(d)
+        return [b.passiveScalarGrad for b in d.getBlocks()]
+
+    fn = _copy_stage(fwd, bwd, None, lambda d: d.scalarResult)
+    _CopyFunction.apply(fn, domain, *[b.passiveScalar for b in domain.getBlocks()])
+
+
+_EXCLUDED_GRADIENTS["SetupAdvectionScalar"] = (set(), set())
 
 
 def SetupAdvectionScalar(domain, time_step, non_ortho_flags):
-    raise PISOtorch.PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip")
+    """PISOtorch_diff.py:444-529: scalarRHS from the block passive scalars, boundary scalars and velocities, the
+    viscosities (and the scalar result, through the explicit non-orthogonal terms)."""
+    is_non_ortho = check_non_ortho_rhs(non_ortho_flags)
+    tracked = ["PASSIVE_SCALAR", "BOUNDARY_PASSIVE_SCALAR", "BOUNDARY_VELOCITY", "VISCOSITY",
+               "PASSIVE_SCALAR_VISCOSITY"]
+    saved = ["BOUNDARY_PASSIVE_SCALAR", "BOUNDARY_VELOCITY", "VISCOSITY", "PASSIVE_SCALAR_VISCOSITY"]
+    if is_non_ortho:
+        tracked.append("PASSIVE_SCALAR_RESULT")
+        saved.append("PASSIVE_SCALAR_RESULT")
+    name = "SetupAdvectionScalar"
+
+    def fwd(d):
+        d.CreateScalarRHS()
+        d.UpdateDomainData()
+        PISOtorch.SetupAdvectionScalar(d, time_step, non_ortho_flags)
+
+    def bwd(d, dt, grads):
+        _set_gradient_input(grads[0], d.setScalarRHSGrad, name, "PASSIVE_SCALAR_RHS_GRAD")
+        d.CreateViscosityGrad()
+        d.CreatePassiveScalarGradOnBlocks()
+        d.CreatePassiveScalarGradOnBoundaries()
+        d.CreateVelocityGradOnBoundaries()
+        if is_non_ortho:
+            d.CreateScalarResultGrad()
+        d.UpdateDomainData()
+        PISOtorch.SetupAdvectionScalarGrad(d, dt, non_ortho_flags)
+
+    fn = _stage(name, tracked, saved, fwd, bwd, lambda d: d.scalarRHS)
+    return _apply(fn, domain, time_step, tracked)[0]
 
 
 _PRESSURE_TRACKED = ["VELOCITY", "VELOCITY_SOURCE", "VELOCITY_RESULT", "BOUNDARY_VELOCITY", "VISCOSITY",

@@ -541,6 +541,76 @@ class Simulation:
                                                % (info.finalResidual, refine_tol))
         return x64.to(rhs.dtype), True
 
+    def _advect_scalar(self, step, time_step, non_ortho_flags, reuse_result):
+        """Passive-scalar advection of one step (PISOtorch_simulation.py:1085-1167): one advection matrix per
+        channel when the channels differ (scalar viscosity per channel or boundary types per channel), else one
+        matrix and one multi-right-hand-side solve.  Returns (whether that matrix serves the velocity advection too,
+        solve_ok, whether the step ends here: a failed solve or the stop check of the non-orthogonal loop)."""
+        domain = self.domain
+        B = self._backend
+        split = not (domain.isPassiveScalarViscosityStatic() and domain.isAllFixedBoundariesPassiveScalarTypeStatic())
+        channels = domain.getPassiveScalarChannels()
+        n = domain.getTotalSize()
+        for_velocity = not (domain.hasPassiveScalarViscosity() or domain.hasBlockViscosity() or split)
+        matrices = []
+        if split:
+            for channel in range(channels):
+                B.SetupAdvectionMatrix(domain, time_step, non_ortho_flags, forPassiveScalar=True,
+                                       passiveScalarChannel=channel)
+                matrices.append(domain.C.clone())
+        else:
+            B.SetupAdvectionMatrix(domain, time_step, non_ortho_flags, forPassiveScalar=True, passiveScalarChannel=0)
+        solve_ok = True
+        no_grad = torch.no_grad() if self.exclude_advection_solve_gradients else nullcontext()
+        if not self.non_orthogonal:
+            B.SetupAdvectionScalar(domain, time_step, 0)
+            self._run_prep_fn("POST_SCALAR_SETUP", domain=domain, no_step=0, local_step=step, time_step=time_step,
+                              total_step=self.total_step, total_time=self.total_time)
+            with no_grad:
+                if split:
+                    rhs = torch.split(domain.scalarRHS, n)
+                    parts = []
+                    for channel in range(channels):
+                        r, solve_ok = self.linear_solve(matrices[channel], rhs[channel], x=None,
+                                                        use_BiCG=self.advection_use_BiCG, tol=self.advection_tol)
+                        parts.append(r)
+                    scalar_result = torch.cat(parts)
+                else:
+                    scalar_result, solve_ok = self.linear_solve(domain.C, domain.scalarRHS, x=None,
+                                                                use_BiCG=self.advection_use_BiCG,
+                                                                tol=self.advection_tol)
+            domain.setScalarResult(scalar_result)
+            domain.UpdateDomainData()
+        else:
+            B.CopyScalarResultFromBlocks(domain)  # the explicit non-orthogonal RHS terms read it
+            for no_step in range(self.advect_non_ortho_steps):
+                B.SetupAdvectionScalar(domain, time_step, non_ortho_flags)
+                self._run_prep_fn("POST_SCALAR_SETUP", domain=domain, no_step=no_step, local_step=step,
+                                  time_step=time_step, total_step=self.total_step, total_time=self.total_time)
+                with no_grad:
+                    reuse = no_step > 0 and reuse_result
+                    if split:
+                        xs = torch.split(domain.scalarResult, n) if reuse else [None] * channels
+                        rhs = torch.split(domain.scalarRHS, n)
+                        parts = []
+                        for channel in range(channels):
+                            x = xs[channel].contiguous() if xs[channel] is not None else None
+                            r, solve_ok = self.linear_solve(matrices[channel], rhs[channel], x=x,
+                                                            use_BiCG=self.advection_use_BiCG, tol=self.advection_tol)
+                            parts.append(r)
+                        scalar_result = torch.cat(parts)
+                    else:
+                        x = domain.scalarResult if reuse else None
+                        scalar_result, solve_ok = self.linear_solve(domain.C, domain.scalarRHS, x=x,
+                                                                    use_BiCG=self.advection_use_BiCG,
+                                                                    tol=self.advection_tol)
+                domain.setScalarResult(scalar_result)
+                domain.UpdateDomainData()
+                if not solve_ok or self._check_stop():
+                    return for_velocity, solve_ok, True
+        B.CopyScalarResultToBlocks(domain)
+        return for_velocity, solve_ok, False
+
     # --- the PISO step (:1054-1327) ---
     def _PISO_split_step(self, iterations, time_step=None):
         self._check_domain()
@@ -570,14 +640,18 @@ class Simulation:
                               total_time=self.total_time)
             domain.UpdateDomainData()
 
+            advection_matrix_for_velocity = False
             if self.advect_passive_scalar and domain.hasPassiveScalar():
-                raise PISOtorch.PictUnsupportedError("passive-scalar advection is not implemented by libpict_hip; "
-                                                     "use advect_passive_scalar=False or passiveScalarChannels=0")
+                advection_matrix_for_velocity, solve_ok, stop = self._advect_scalar(
+                    step, time_step, non_ortho_flags, advect_non_ortho_reuse_result)
+                if not solve_ok or stop:
+                    return solve_ok
 
             apply_pressure_gradient = False
             self._run_prep_fn("PRE_VELOCITY_SETUP", domain=domain, local_step=step, time_step=time_step,
                               total_step=self.total_step, total_time=self.total_time)
-            B.SetupAdvectionMatrix(domain, time_step, non_ortho_flags)
+            if not advection_matrix_for_velocity:
+                B.SetupAdvectionMatrix(domain, time_step, non_ortho_flags)
             if not self.non_orthogonal:
                 B.SetupAdvectionVelocity(domain, time_step, 0, apply_pressure_gradient)
                 self._run_prep_fn("POST_VELOCITY_SETUP", domain=domain, no_step=0, local_step=step,

@@ -46,6 +46,10 @@ class BoundaryDesc(Structure):
         ("velocity", c_void_p),
         ("velocity_grad", c_void_p),
         ("transform", c_void_p),
+        ("scalar_static", c_int32),
+        ("scalar_bc_mask", c_int32),
+        ("scalar", c_void_p),
+        ("scalar_grad", c_void_p),
     ]
 
 
@@ -70,6 +74,8 @@ class BlockDesc(Structure):
         ("viscosity_grad", c_void_p),
         ("velocity_source_grad", c_void_p),
         ("bounds", BoundaryDesc * 6),
+        ("scalar", c_void_p),
+        ("scalar_grad", c_void_p),
     ]
 
 
@@ -102,6 +108,14 @@ class DomainDesc(Structure):
         ("pressure_rhs_grad", c_void_p),
         ("pressure_rhs_div_grad", c_void_p),
         ("pressure_result_grad", c_void_p),
+        ("scalar_channels", c_int32),
+        ("scalar_viscosity_static", c_int32),
+        ("scalar_viscosity", c_void_p),
+        ("scalar_viscosity_grad", c_void_p),
+        ("scalar_rhs", c_void_p),
+        ("scalar_result", c_void_p),
+        ("scalar_rhs_grad", c_void_p),
+        ("scalar_result_grad", c_void_p),
     ]
 
 
@@ -197,6 +211,9 @@ class BatchInfo(Structure):
     ]
 
 
+# layout version of the descriptor structs (PICT_ABI_VERSION)
+ABI_VERSION = 2
+
 # exported symbol -> (restype, argtypes)
 _P = c_void_p
 SIGNATURES = {
@@ -215,6 +232,14 @@ SIGNATURES = {
     "pict_correct_velocity": (c_int32, [_P, c_double, c_int32, c_int32, _P]),
     "pict_compute_velocity_divergence": (c_int32, [_P, _P, _P]),
     "pict_copy_velocity_result_to_blocks": (c_int32, [_P, _P]),
+    "pict_setup_advection_matrix_scalar": (c_int32, [_P, c_double, c_int32, c_int32, _P]),
+    "pict_setup_advection_scalar": (c_int32, [_P, c_double, c_int32, _P]),
+    "pict_copy_scalar_result_to_blocks": (c_int32, [_P, _P]),
+    "pict_copy_scalar_result_from_blocks": (c_int32, [_P, _P]),
+    "pict_setup_advection_matrix_scalar_grad": (c_int32, [_P, c_double, c_int32, c_int32, _P]),
+    "pict_setup_advection_scalar_grad": (c_int32, [_P, c_double, c_int32, _P]),
+    "pict_copy_scalar_result_grad_from_blocks": (c_int32, [_P, _P]),
+    "pict_copy_scalar_result_grad_to_blocks": (c_int32, [_P, _P]),
     "pict_copy_velocity_result_from_blocks": (c_int32, [_P, _P]),
     "pict_copy_pressure_result_to_blocks": (c_int32, [_P, _P]),
     "pict_copy_pressure_result_from_blocks": (c_int32, [_P, _P]),
@@ -291,6 +316,9 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.pict_abi_version() != ABI_VERSION:
+        raise ImportError(f"{path} has descriptor layout {lib.pict_abi_version()}, this module expects "
+                          f"{ABI_VERSION}; rebuild it with `python -m pict_amd.build`")
     if path == LIB_PATH:
         _lib = lib
     return lib

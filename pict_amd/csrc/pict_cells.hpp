@@ -18,9 +18,12 @@ namespace pict {
 // PISO_build_matrix :3601-3865 (velocity matrix C and its diagonal A).  Values
 // only; the pattern is written by k_csr_pattern.  NO: the non-orthogonal matrix
 // terms of `flags` are included (instances without them carry none of that code).
+// SC: the matrix of passive-scalar channel ch (forPassiveScalar): the scalar
+// viscosity, the channel's boundary types, cross coefficients times the scalar
+// viscosity (:3683, :3742-3755, :3815-3826).
 // ---------------------------------------------------------------------------
-template <typename R, int D, bool NO = false>
-__device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile, R dt, int flags = 0) {
+template <typename R, int D, bool NO = false, bool SC = false>
+__device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile, R dt, int flags = 0, int ch = 0) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, tile, b, bi, f)) return;
@@ -34,7 +37,7 @@ __device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile,
     const R det = cell_det<R, D>(b, pos);
     R diag = det / dt;
     idx[0] = f + b->global_offset;
-    const R visc_p = viscosity(dom, b, pos);
+    const R visc_p = SC ? scalar_viscosity(dom, ch) : viscosity(dom, b, pos);
     R alpha_p[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -55,7 +58,7 @@ __device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile,
             const pict_block_desc *nbp = dom.blocks + nb;
             const R alpha = laplace_orth<R, D>(nbp, t);
             t.a[3] = 0;
-            const R visc_n = viscosity(dom, nbp, t);
+            const R visc_n = SC ? visc_p : viscosity(dom, nbp, t);
             idx[face + 1] = global_index(t, nbp);
             const R vc = (alpha_p[dim] * visc_p + alpha * visc_n) * R(0.5);
             diag += vc;
@@ -63,14 +66,22 @@ __device__ __forceinline__ void adv_matrix_cell(const DomainK<R> &dom, int tile,
             if constexpr (NO && D > 1) {
                 if (flags & kNonOrthoMatrix) {
                     R ai[2 * D * (D - 1)];
-                    interp_nonortho<R, D>(dom, bi, pos, true, false, ai);
+                    interp_nonortho<R, D>(dom, bi, pos, !SC, false, ai);
+                    I4 cp = pos;
+                    if constexpr (SC) {
+#pragma unroll
+                        for (int k = 0; k < 2 * D * (D - 1); ++k) ai[k] *= visc_p;
+                        cp.a[3] = ch;
+                    }
                     rv[0] = diag;
-                    nonortho_matrix_face<R, D>(dom, bi, pos, face, ai, flags, R(-1), GD_VELOCITY, false, rv);
+                    nonortho_matrix_face<R, D>(dom, bi, cp, face, ai, flags, R(-1), SC ? GD_SCALAR : GD_VELOCITY,
+                                               false, rv);
                     diag = rv[0];
                 }
             }
         } else {
-            const int slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? 0 : 1;
+            const int bc = SC ? fixed_scalar_bc(&b->bounds[face], ch) : b->bounds[face].velocity_bc;
+            const int slip = bc == PICT_BC_DIRICHLET ? 0 : 1;
             diag += R((1 - slip) * 2) * visc_p * alpha_p[dim];
         }
     }
@@ -108,6 +119,42 @@ __device__ __forceinline__ void adv_rhs_cell(const DomainK<R> &dom, int tile, R 
         r += velocity_source<R>(b, t);
         if (apply_pg) r -= pg[d] * dt;
         dom.vrhs[(size_t)d * dom.N + b->global_offset + f] = r;
+    }
+}
+
+// kPISO_build_scalar_advection_RHS :4080-4190: scalar_rhs of every channel (NO: with the explicit non-orthogonal
+// terms of `flags`, from the scalar result, times the scalar viscosity)
+template <typename R, int D, bool NO = false>
+__device__ __forceinline__ void scalar_rhs_cell(const DomainK<R> &dom, int tile, R dt, int flags = 0) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const R det = cell_det<R, D>(b, pos);
+    for (int c = 0; c < dom.channels; ++c) {
+        I4 t = pos;
+        t.a[3] = c;
+        const R visc = scalar_viscosity(dom, c);
+        R r = det * reinterpret_cast<const R *>(b->scalar)[flat4(t, b->stride)] / dt;
+#pragma unroll
+        for (int face = 0; face < 2 * D; ++face) {
+            if (!at_bound(pos, face, b)) continue;
+            const pict_boundary_desc *fb = &b->bounds[face];
+            if (fb->type != PICT_BOUNDARY_FIXED) continue;
+            const int ax = face >> 1;
+            const R normal = (face & 1) ? R(1) : R(-1);
+            I4 bp = pos;
+            bp.a[ax] = 0;
+            const R sv = fixed_scalar<R>(fb, bp, c);  // value (Dirichlet) or gradient (Neumann)
+            bp.a[3] = ax;
+            const R alpha = laplace_orth_fixed<R, D>(fb, bp);
+            const R flux = fixed_contravariant<R, D>(fb, bp) * normal;
+            r -= sv * flux;
+            r += fixed_scalar_bc(fb, c) == PICT_BC_DIRICHLET ? sv * visc * R(2) * alpha : sv * visc;
+        }
+        if constexpr (NO && D > 1) r -= nonortho_rhs<R, D>(dom, bi, t, flags, GD_SCALAR_RESULT, false, false) * visc;
+        r /= det;
+        dom.srhs[(size_t)c * dom.N + b->global_offset + f] = r;
     }
 }
 
@@ -249,17 +296,19 @@ __device__ __forceinline__ void correct_velocity_cell(const DomainK<R> &dom, int
 
 // Copy*Result*Blocks :6541-6733 and Copy*ResultGrad*Blocks :6819-6915 -- one
 // launch instead of a memcpy per block and component.  `field` selects the
-// block tensor: 0 velocity, 1 pressure, 2 velocity_grad, 3 pressure_grad;
-// `gv` is the matching global vector (component-major for velocities).
+// block tensor: 0 velocity, 1 pressure, 2 velocity_grad, 3 pressure_grad,
+// 4 passive scalar, 5 passive-scalar grad; `gv` is the matching global vector
+// (component- or channel-major).
 template <typename R>
 __device__ __forceinline__ void copy_result_cell(const DomainK<R> &dom, int tile, R *gv, int field, int to_blocks) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, tile, b, bi, f)) return;
     const int g = b->global_offset + f;
-    void *bp = field == 0 ? b->velocity : field == 1 ? b->pressure : field == 2 ? b->velocity_grad : b->pressure_grad;
+    void *bp = field == 0 ? b->velocity : field == 1 ? b->pressure : field == 2 ? b->velocity_grad
+             : field == 3 ? b->pressure_grad : field == 4 ? b->scalar : b->scalar_grad;
     R *bv = reinterpret_cast<R *>(bp);
-    const int comps = (field & 1) ? 1 : dom.D;
+    const int comps = field >= 4 ? dom.channels : (field & 1) ? 1 : dom.D;
     for (int d = 0; d < comps; ++d) {
         if (to_blocks) bv[(size_t)d * b->stride[3] + f] = gv[(size_t)d * dom.N + g];
         else gv[(size_t)d * dom.N + g] = bv[(size_t)d * b->stride[3] + f];

@@ -32,6 +32,10 @@ struct DomainK {
     R *vrhs, *vres, *prhs, *prhs_div, *pres;
     R *visc_grad, *C_grad, *P_grad, *A_grad;
     R *vrhs_grad, *vres_grad, *prhs_grad, *prhs_div_grad, *pres_grad;
+    // passive scalar: channels, scalar viscosity ([1] or [C], null: the domain viscosity), global vectors [C*N]
+    int channels, svisc_static;
+    const R *svisc;
+    R *svisc_grad, *srhs, *sres, *srhs_grad, *sres_grad;
 };
 
 __device__ __forceinline__ int flat4(const I4 &p, const int32_t *stride) {
@@ -209,6 +213,24 @@ __device__ __forceinline__ R viscosity(const DomainK<R> &dom, const pict_block_d
     if (b->viscosity_static) return reinterpret_cast<const R *>(b->viscosity)[0];
     p.a[3] = 0;
     return reinterpret_cast<const R *>(b->viscosity)[flat4(p, b->stride)];
+}
+
+// getViscosity(forPassiveScalar) :1788-1800 (the scalar viscosity is global, getViscosityBlock :1803-1807)
+template <typename R>
+__device__ __forceinline__ R scalar_viscosity(const DomainK<R> &dom, int ch) {
+    if (!dom.svisc) return dom.viscosity;
+    return dom.svisc[dom.svisc_static ? 0 : ch];
+}
+// FixedBoundary passive scalar of channel ch, getFixedBoundaryData :2419-2437 (p on the boundary grid)
+template <typename R>
+__device__ __forceinline__ R fixed_scalar(const pict_boundary_desc *fb, I4 p, int ch) {
+    if (fb->scalar_static) return reinterpret_cast<const R *>(fb->scalar)[ch];
+    p.a[3] = ch;
+    return reinterpret_cast<const R *>(fb->scalar)[flat4(p, fb->stride)];
+}
+// getFixedBoundaryType (PASSIVE_SCALAR) :2360-2370
+__device__ __forceinline__ int fixed_scalar_bc(const pict_boundary_desc *fb, int ch) {
+    return ((fb->scalar_bc_mask >> (ch & 31)) & 1) ? PICT_BC_NEUMANN : PICT_BC_DIRICHLET;
 }
 
 // getBlockVelocitySource :2241-2252

@@ -26,6 +26,7 @@ __device__ __forceinline__ void add_to(R *p, R v) {
 template <typename R>
 struct UniformAcc {
     double visc_dom = 0.0, visc_own = 0.0, src[3] = {0.0, 0.0, 0.0};
+    double visc_scalar = 0.0;  // the scalar viscosity of one passive-scalar channel
 };
 
 // workgroup sum of v (all threads call it; result valid in thread 0)
@@ -60,6 +61,17 @@ __device__ __forceinline__ void flush_uniform(const DomainK<R> &dom, const pict_
 #pragma unroll
             for (int d = 0; d < D; ++d)
                 if (sv[d] != 0.0) add_to(reinterpret_cast<R *>(b->velocity_source_grad) + d, (R)sv[d]);
+    }
+}
+
+// scatterViscosity_GRAD(forPassiveScalar) :1834-1849: the workgroup's sum of scalar-viscosity gradients of channel
+// ch, into the scalar viscosity gradient, or the domain viscosity gradient when there is none (every thread calls it)
+template <typename R>
+__device__ __forceinline__ void flush_scalar_viscosity(const DomainK<R> &dom, double v, int ch) {
+    const double t = wg_sum(v);
+    if (threadIdx.x == 0 && t != 0.0) {
+        R *dst = dom.svisc_grad ? dom.svisc_grad + (dom.svisc_static ? 0 : ch) : dom.visc_grad;
+        if (dst) add_to(dst, (R)t);
     }
 }
 
@@ -252,14 +264,27 @@ __device__ __forceinline__ void fixed_boundary_source_grad(const DomainK<R> &dom
 // scatterBlockData :2340-2347 into the result gradients (p.w = component)
 template <typename R>
 __device__ __forceinline__ void scatter_block_data(const DomainK<R> &dom, int bi, const I4 &p, int type, R g) {
-    R *dst = type == GD_VELOCITY_RESULT ? dom.vres_grad : type == GD_PRESSURE_RESULT ? dom.pres_grad : nullptr;
+    R *dst = type == GD_VELOCITY_RESULT ? dom.vres_grad : type == GD_PRESSURE_RESULT ? dom.pres_grad
+           : type == GD_SCALAR_RESULT ? dom.sres_grad : nullptr;
     if (dst) add_to(dst + (size_t)p.a[3] * dom.N + global_index(p, dom.blocks + bi), g);
 }
-// scatterFixedBoundaryData :2448-2478 (FixedBoundary velocity gradients)
+// scatterFixedBoundaryData :2448-2478 (FixedBoundary velocity and passive-scalar gradients)
 template <typename R>
 __device__ __forceinline__ void scatter_fixed_data(const pict_block_desc *b, I4 p, int face, int type, R g) {
     const pict_boundary_desc *fb = &b->bounds[face];
-    if (fb->type != PICT_BOUNDARY_FIXED || !(type == GD_VELOCITY_RESULT || type == GD_VELOCITY)) return;
+    if (fb->type != PICT_BOUNDARY_FIXED) return;
+    if (gd_scalar(type)) {
+        if (!fb->scalar_grad) return;
+        R *dst = reinterpret_cast<R *>(fb->scalar_grad);
+        if (fb->scalar_static) {
+            add_to(dst + p.a[3], g);
+            return;
+        }
+        p.a[face >> 1] = 0;
+        add_to(dst + flat4(p, fb->stride), g);
+        return;
+    }
+    if (!(type == GD_VELOCITY_RESULT || type == GD_VELOCITY)) return;
     if (!fb->velocity_grad) return;
     R *dst = reinterpret_cast<R *>(fb->velocity_grad);
     if (fb->velocity_static) {
@@ -424,7 +449,7 @@ __device__ __forceinline__ void nonortho_rhs_grad(const DomainK<R> &dom, int bi,
     for (int face = 0; face < 2 * D; ++face) {
         const int axis = face >> 1, fs = face_sign(face);
         if (at_bound(pos, face, b) && prescribed(face, b)) {
-            if (fixed_type(b, face, type) == PICT_BC_NEUMANN || type == GD_PRESSURE_RESULT) continue;
+            if (fixed_type(b, face, type, pos.a[3]) == PICT_BC_NEUMANN || type == GD_PRESSURE_RESULT) continue;
             R bA[D];
             lap_neighbour_boundary<R, D>(&b->bounds[face], pos, axis, bA);
 #pragma unroll
@@ -499,9 +524,11 @@ __device__ __forceinline__ void nonortho_rhs_grad(const DomainK<R> &dom, int bi,
 // PISO_build_matrix_GRAD :3869-4075: C_grad, A_grad -> block velocity grad,
 // viscosity grads (NO: with the non-orthogonal matrix terms of `flags`).
 // ---------------------------------------------------------------------------
-template <typename R, int D, bool NO>
+// SC: the matrix of passive-scalar channel ch (forPassiveScalar): viscosity gradients go to the channel's scalar
+// viscosity (u.visc_scalar), the cross coefficients' gradient times their viscosity-free value (:4004-4011).
+template <typename R, int D, bool NO, bool SC = false>
 __device__ __forceinline__ void adv_matrix_grad_cell(const DomainK<R> &dom, const pict_block_desc *b, int bi, int f,
-                                                     int flags, UniformAcc<R> &u) {
+                                                     int flags, UniformAcc<R> &u, int ch = 0) {
     const I4 pos = unflat(f, b);
     const int g = b->global_offset + f;
     const R rdet = R(1) / cell_det<R, D>(b, pos);
@@ -541,28 +568,104 @@ __device__ __forceinline__ void adv_matrix_grad_cell(const DomainK<R> &dom, cons
             const R vc_grad = diag_grad - cg[face + 1];
             visc_grad += R(0.5) * alpha_p[dim] * vc_grad;
             t.a[3] = 0;
-            scatter_viscosity<R>(dom, nbp, t, R(0.5) * alpha * vc_grad, b, u);
+            if constexpr (SC) u.visc_scalar += (double)(R(0.5) * alpha * vc_grad);
+            else scatter_viscosity<R>(dom, nbp, t, R(0.5) * alpha * vc_grad, b, u);
             if constexpr (NO && D > 1) {
                 if (no) {
                     R cgd[7];
 #pragma unroll
                     for (int k = 0; k < 7; ++k) cgd[k] = cg[k];
                     cgd[0] = diag_grad;
-                    nonortho_matrix_face_grad<R, D>(dom, bi, pos, face, ai, flags, R(-1), GD_VELOCITY, false, cgd,
-                                                    ai_grad);
+                    I4 cp = pos;
+                    if constexpr (SC) cp.a[3] = ch;
+                    nonortho_matrix_face_grad<R, D>(dom, bi, cp, face, ai, flags, R(-1), SC ? GD_SCALAR : GD_VELOCITY,
+                                                    false, cgd, ai_grad);
                 }
             }
         } else {
             fg[face] = 0;
-            const R slip = b->bounds[face].velocity_bc == PICT_BC_DIRICHLET ? R(0) : R(1);
+            const int bc = SC ? fixed_scalar_bc(&b->bounds[face], ch) : b->bounds[face].velocity_bc;
+            const R slip = bc == PICT_BC_DIRICHLET ? R(0) : R(1);
             visc_grad += (R(1) - slip) * R(2) * alpha_p[dim] * diag_grad;
         }
     }
     if constexpr (NO && D > 1) {
-        if (no) scatter_interp_nonortho<R, D>(dom, bi, pos, ai_grad, true, false, u);
+        if (no) {
+            if constexpr (SC) {
+#pragma unroll
+                for (int k = 0; k < NA; ++k) visc_grad += ai_grad[k] * ai[k];
+            } else {
+                scatter_interp_nonortho<R, D>(dom, bi, pos, ai_grad, true, false, u);
+            }
+        }
     }
     if (b->velocity_grad) scatter_fluxes<R, D>(dom, b, pos, fg, nullptr);
-    scatter_viscosity<R>(dom, b, pos, visc_grad, b, u);
+    if constexpr (SC) u.visc_scalar += (double)visc_grad;
+    else scatter_viscosity<R>(dom, b, pos, visc_grad, b, u);
+}
+
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_scalar_grad(DomainK<R> dom, R dt, int flags, int ch) {
+    const pict_block_desc *b;
+    int bi, f;
+    (void)dt;
+    UniformAcc<R> u;
+    if (tile_cell(dom, b, bi, f)) adv_matrix_grad_cell<R, D, NO, true>(dom, b, bi, f, flags, u, ch);
+    flush_scalar_viscosity<R>(dom, u.visc_scalar, ch);
+}
+
+// kPISO_build_scalar_advection_RHS_GRAD :4193-4278: scalar_rhs_grad -> block scalar grad (=), fixed-boundary scalar
+// and velocity grads (+=), scalar viscosity grad (+=) (NO: and the VJP of the explicit non-orthogonal terms of
+// `flags`, into scalar_result_grad).  Every thread runs the channel loop (the viscosity sums are workgroup sums).
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_scalar_rhs_grad(DomainK<R> dom, R dt, int flags) {
+    const pict_block_desc *b;
+    int bi, f;
+    const bool live = tile_cell(dom, b, bi, f);
+    UniformAcc<R> u;  // unused: the explicit terms of a scalar carry no viscosity of their own
+    const I4 pos = live ? unflat(f, b) : I4{{0, 0, 0, 0}};
+    const R det = live ? cell_det<R, D>(b, pos) : R(1);
+    for (int c = 0; c < dom.channels; ++c) {
+        double vacc = 0.0;
+        if (live) {
+            I4 t = pos;
+            t.a[3] = c;
+            const int g = b->global_offset + f;
+            const R visc = scalar_viscosity(dom, c);
+            R G = dom.srhs_grad[(size_t)c * dom.N + g] / det;
+            if constexpr (NO && D > 1) {
+                nonortho_rhs_grad<R, D>(dom, bi, t, flags, GD_SCALAR_RESULT, false, false, -G * visc, u);
+                vacc += (double)(-G * nonortho_rhs<R, D>(dom, bi, t, flags, GD_SCALAR_RESULT, false, false));
+            }
+#pragma unroll
+            for (int face = 0; face < 2 * D; ++face) {
+                if (!at_bound(pos, face, b)) continue;
+                const pict_boundary_desc *fb = &b->bounds[face];
+                if (fb->type != PICT_BOUNDARY_FIXED) continue;
+                const int ax = face >> 1;
+                const R normal = (face & 1) ? R(1) : R(-1);
+                I4 bp = pos;
+                bp.a[ax] = 0;
+                const R sv = fixed_scalar<R>(fb, bp, c);
+                const bool dirichlet = fixed_scalar_bc(fb, c) == PICT_BC_DIRICHLET;
+                bp.a[3] = ax;
+                const R alpha = laplace_orth_fixed<R, D>(fb, bp);
+                const R flux = fixed_contravariant<R, D>(fb, bp) * normal;
+                R s_grad = -flux * G;
+                if (dirichlet) {
+                    s_grad += visc * R(2) * alpha * G;
+                    vacc += (double)(sv * R(2) * alpha * G);
+                } else {
+                    s_grad += visc * G;
+                    vacc += (double)(sv * G);
+                }
+                scatter_fixed_contravariant<R, D>(fb, bp, -sv * G * normal);
+                scatter_fixed_data<R>(b, t, face, GD_SCALAR, s_grad);
+            }
+            if (b->scalar_grad) reinterpret_cast<R *>(b->scalar_grad)[flat4(t, b->stride)] = G * det / dt;
+        }
+        flush_scalar_viscosity<R>(dom, vacc, c);
+    }
 }
 
 template <typename R, int D, bool NO>
@@ -838,6 +941,36 @@ pict_status pict_setup_advection_matrix_grad(const pict_atlas *a, double dt, int
                                            (int)flags));
     else
         PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_grad<R, D, false>), a->view<R>(), (R)dt, 0));
+    return check_launch();
+}
+
+pict_status pict_setup_advection_matrix_scalar_grad(const pict_atlas *a, double dt, int32_t flags, int32_t channel,
+                                                    void *stream) {
+    PICT_CHECK_ARG(a && a->desc.C_grad && a->desc.A_grad, "C_grad/A_grad missing");
+    PICT_CHECK_ARG(a->desc.scalar_channels > 0, "domain has no passive scalar");
+    PICT_CHECK_ARG(channel >= 0 && channel < a->desc.scalar_channels, "passive scalar channel %d out of range",
+                   channel);
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_scalar_grad<R, D, true>), a->view<R>(), (R)dt,
+                                           (int)flags, (int)channel));
+    else
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_scalar_grad<R, D, false>), a->view<R>(), (R)dt,
+                                           0, (int)channel));
+    return check_launch();
+}
+
+pict_status pict_setup_advection_scalar_grad(const pict_atlas *a, double dt, int32_t flags, void *stream) {
+    PICT_CHECK_ARG(a && a->desc.scalar_channels > 0, "domain has no passive scalar");
+    PICT_CHECK_ARG(a->desc.scalar_rhs_grad, "scalar_rhs_grad missing");
+    for (int i = 0; i < a->desc.num_blocks; ++i)
+        PICT_CHECK_ARG(a->host_blocks[i].scalar, "block %d: passive scalar missing", i);
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
+        PICT_CHECK_ARG(a->desc.scalar_result, "scalar_result missing (non-orthogonal RHS terms)");
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_scalar_rhs_grad<R, D, true>), a->view<R>(), (R)dt,
+                                           (int)flags));
+    } else {
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_scalar_rhs_grad<R, D, false>), a->view<R>(), (R)dt, 0));
+    }
     return check_launch();
 }
 

@@ -143,6 +143,14 @@ struct pict_atlas {
         k.prhs_grad = (R *)desc.pressure_rhs_grad;
         k.prhs_div_grad = (R *)desc.pressure_rhs_div_grad;
         k.pres_grad = (R *)desc.pressure_result_grad;
+        k.channels = desc.scalar_channels;
+        k.svisc_static = desc.scalar_viscosity_static;
+        k.svisc = (const R *)desc.scalar_viscosity;
+        k.svisc_grad = (R *)desc.scalar_viscosity_grad;
+        k.srhs = (R *)desc.scalar_rhs;
+        k.sres = (R *)desc.scalar_result;
+        k.srhs_grad = (R *)desc.scalar_rhs_grad;
+        k.sres_grad = (R *)desc.scalar_result_grad;
         return k;
     }
 };

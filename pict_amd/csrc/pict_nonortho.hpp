@@ -20,7 +20,11 @@ namespace pict {
 
 // Field a corner or neighbour value is read from (the GridDataType values the reference's stages pass,
 // domain_structs_gpu.h:28-65).
-enum GridData { GD_VELOCITY = 0, GD_VELOCITY_RESULT = 1, GD_PRESSURE_RESULT = 2, GD_IS_FIXED = 3 };
+enum GridData {
+    GD_VELOCITY = 0, GD_VELOCITY_RESULT = 1, GD_PRESSURE_RESULT = 2, GD_IS_FIXED = 3,
+    GD_SCALAR = 4, GD_SCALAR_RESULT = 5  // passive scalar: p.w is the channel
+};
+__host__ __device__ constexpr bool gd_scalar(int type) { return type == GD_SCALAR || type == GD_SCALAR_RESULT; }
 
 constexpr int kNonOrthoMatrix = PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX;
 constexpr int kNonOrthoRhs = PICT_NON_ORTHO_DIRECT_RHS | PICT_NON_ORTHO_DIAGONAL_RHS;
@@ -131,6 +135,8 @@ __device__ __forceinline__ R block_data(const DomainK<R> &dom, int bi, const I4 
         case GD_VELOCITY: return reinterpret_cast<const R *>(b->velocity)[flat4(p, b->stride)];
         case GD_VELOCITY_RESULT: return dom.vres[(size_t)p.a[3] * dom.N + global_index(p, b)];
         case GD_PRESSURE_RESULT: return dom.pres[(size_t)p.a[3] * dom.N + global_index(p, b)];
+        case GD_SCALAR: return reinterpret_cast<const R *>(b->scalar)[flat4(p, b->stride)];
+        case GD_SCALAR_RESULT: return dom.sres[(size_t)p.a[3] * dom.N + global_index(p, b)];
         default: return R(0);
     }
 }
@@ -140,16 +146,21 @@ __device__ __forceinline__ R fixed_data(const pict_block_desc *b, I4 p, int face
     if (type == GD_IS_FIXED) return R(1);
     const pict_boundary_desc *fb = &b->bounds[face];
     if (fb->type != PICT_BOUNDARY_FIXED || type == GD_PRESSURE_RESULT) return R(0);
+    if (gd_scalar(type)) {
+        p.a[face >> 1] = 0;
+        return fixed_scalar<R>(fb, p, p.a[3]);
+    }
     if (fb->velocity_static) return reinterpret_cast<const R *>(fb->velocity)[p.a[3]];
     p.a[face >> 1] = 0;
     return reinterpret_cast<const R *>(fb->velocity)[flat4(p, fb->stride)];
 }
-// getFixedBoundaryType :2351-2374
-__device__ __forceinline__ int fixed_type(const pict_block_desc *b, int face, int type) {
+// getFixedBoundaryType :2351-2374 (ch: the passive-scalar channel, pos.w in the reference)
+__device__ __forceinline__ int fixed_type(const pict_block_desc *b, int face, int type, int ch = 0) {
     const pict_boundary_desc *fb = &b->bounds[face];
     if (fb->type != PICT_BOUNDARY_FIXED) return PICT_BC_DIRICHLET;
     if (type == GD_VELOCITY || type == GD_VELOCITY_RESULT) return fb->velocity_bc;
     if (type == GD_PRESSURE_RESULT) return PICT_BC_NEUMANN;
+    if (gd_scalar(type)) return fixed_scalar_bc(fb, ch);
     return PICT_BC_DIRICHLET;
 }
 
@@ -253,7 +264,7 @@ __device__ __forceinline__ Corner<R> corner_value(const DomainK<R> &dom, int bi,
                 const pict_block_desc *wb = dom.blocks + w[d].bi;
                 Corner<R> r;
                 r.num = 0;
-                r.btype = fixed_type(wb, w[d].d1, type);
+                r.btype = fixed_type(wb, w[d].d1, type, w[d].pos.a[3]);
                 if (at_bound(w[d].pos, w[d].d2, wb)) {
                     r.data = fixed_data<R>(wb, w[d].pos, w[d].d1, type);
                 } else {
@@ -398,7 +409,7 @@ __device__ __forceinline__ R nonortho_rhs(const DomainK<R> &dom, int bi, const I
     for (int face = 0; face < 2 * D; ++face) {
         const int axis = face >> 1, fs = face_sign(face);
         if (at_bound(pos, face, b) && prescribed(face, b)) {
-            if (fixed_type(b, face, type) == PICT_BC_NEUMANN || type == GD_PRESSURE_RESULT) continue;
+            if (fixed_type(b, face, type, pos.a[3]) == PICT_BC_NEUMANN || type == GD_PRESSURE_RESULT) continue;
             R bA[D];
             lap_neighbour_boundary<R, D>(&b->bounds[face], pos, axis, bA);
 #pragma unroll

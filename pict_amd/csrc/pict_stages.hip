@@ -126,6 +126,16 @@ __global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix(DomainK<R> dom, R 
 }
 
 template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_adv_matrix_scalar(DomainK<R> dom, R dt, int flags, int ch) {
+    adv_matrix_cell<R, D, NO, true>(dom, (int)blockIdx.x, dt, flags, ch);
+}
+
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_scalar_rhs(DomainK<R> dom, R dt, int flags) {
+    scalar_rhs_cell<R, D, NO>(dom, (int)blockIdx.x, dt, flags);
+}
+
+template <typename R, int D, bool NO>
 __global__ __launch_bounds__(kCellsPerTile) void k_adv_rhs(DomainK<R> dom, R dt, int apply_pg, int flags) {
     adv_rhs_cell<R, D, NO>(dom, (int)blockIdx.x, dt, apply_pg, flags);
 }
@@ -206,6 +216,8 @@ static pict_status validate_desc(const pict_domain_desc *d) {
                                "block %d face %d: connected block out of range", i, f);
             } else if (bd.type == PICT_BOUNDARY_FIXED) {
                 PICT_CHECK_ARG(bd.velocity != nullptr, "block %d face %d: fixed boundary without velocity", i, f);
+                PICT_CHECK_ARG(d->scalar_channels == 0 || bd.scalar != nullptr,
+                               "block %d face %d: fixed boundary without passive scalar", i, f);
                 PICT_CHECK_ARG(!bd.has_transform || bd.transform, "block %d face %d: transform flag without data", i, f);
             } else if (bd.type != PICT_BOUNDARY_PERIODIC) {
                 set_error("block %d face %d: unsupported boundary type %d", i, f, bd.type);
@@ -215,13 +227,14 @@ static pict_status validate_desc(const pict_domain_desc *d) {
         cells += b.stride[3];
     }
     PICT_CHECK_ARG(cells == d->num_cells, "num_cells %d does not match blocks (%lld)", d->num_cells, cells);
+    PICT_CHECK_ARG(d->scalar_channels >= 0, "negative passive-scalar channel count");
     return PICT_OK;
 }
 
 extern "C" {
 
 const char *pict_last_error(void) { return g_err.c_str(); }
-int32_t pict_abi_version(void) { return 1; }
+int32_t pict_abi_version(void) { return PICT_ABI_VERSION; }
 
 pict_status pict_atlas_create(const pict_domain_desc *desc, pict_atlas **out, void *stream) {
     PICT_CHECK_ARG(out != nullptr, "out is NULL");
@@ -361,6 +374,45 @@ pict_status pict_setup_advection_matrix(const pict_atlas *a, double dt, int32_t 
     return check_launch();
 }
 
+// a domain with passive-scalar channels and, for per-block use, scalar tensors on every block
+static pict_status check_scalar(const pict_atlas *a, bool blocks) {
+    PICT_CHECK_ARG(a != nullptr, "atlas is NULL");
+    PICT_CHECK_ARG(a->desc.scalar_channels > 0, "domain has no passive scalar");
+    if (blocks)
+        for (int i = 0; i < a->desc.num_blocks; ++i)
+            PICT_CHECK_ARG(a->host_blocks[i].scalar != nullptr, "block %d: passive scalar missing", i);
+    return PICT_OK;
+}
+
+pict_status pict_setup_advection_matrix_scalar(const pict_atlas *a, double dt, int32_t flags, int32_t channel,
+                                               void *stream) {
+    pict_status st = check_scalar(a, false);
+    if (st) return st;
+    PICT_CHECK_ARG(a->desc.C_value && a->desc.A, "C/A missing");
+    PICT_CHECK_ARG(channel >= 0 && channel < a->desc.scalar_channels, "passive scalar channel %d out of range",
+                   channel);
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_scalar<R, D, true>), a->view<R>(), (R)dt,
+                                           (int)flags, (int)channel));
+    else
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_adv_matrix_scalar<R, D, false>), a->view<R>(), (R)dt, 0,
+                                           (int)channel));
+    return check_launch();
+}
+
+pict_status pict_setup_advection_scalar(const pict_atlas *a, double dt, int32_t flags, void *stream) {
+    pict_status st = check_scalar(a, true);
+    if (st) return st;
+    PICT_CHECK_ARG(a->desc.scalar_rhs, "scalar_rhs missing");
+    if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
+        PICT_CHECK_ARG(a->desc.scalar_result, "scalar_result missing (non-orthogonal RHS terms)");
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_scalar_rhs<R, D, true>), a->view<R>(), (R)dt, (int)flags));
+    } else {
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_scalar_rhs<R, D, false>), a->view<R>(), (R)dt, 0));
+    }
+    return check_launch();
+}
+
 pict_status pict_setup_advection_velocity(const pict_atlas *a, double dt, int32_t flags, int32_t apply_pg,
                                           void *stream) {
     PICT_CHECK_ARG(a && a->desc.velocity_rhs, "velocity_rhs missing");
@@ -448,12 +500,18 @@ pict_status pict_compute_velocity_divergence(const pict_atlas *a, void *out, voi
 
 static pict_status copy_result(const pict_atlas *a, int field, int to_blocks, void *stream) {
     PICT_CHECK_ARG(a != nullptr, "atlas is NULL");
+    if (field >= 4) {
+        pict_status st = check_scalar(a, false);
+        if (st) return st;
+    }
     void *gv = field == 0 ? a->desc.velocity_result : field == 1 ? a->desc.pressure_result
-             : field == 2 ? a->desc.velocity_result_grad : a->desc.pressure_result_grad;
+             : field == 2 ? a->desc.velocity_result_grad : field == 3 ? a->desc.pressure_result_grad
+             : field == 4 ? a->desc.scalar_result : a->desc.scalar_result_grad;
     PICT_CHECK_ARG(gv != nullptr, "global vector missing");
     for (int i = 0; i < a->desc.num_blocks; ++i) {
         const pict_block_desc &b = a->host_blocks[i];
-        const void *bp = field == 0 ? b.velocity : field == 1 ? b.pressure : field == 2 ? b.velocity_grad : b.pressure_grad;
+        const void *bp = field == 0 ? b.velocity : field == 1 ? b.pressure : field == 2 ? b.velocity_grad
+                       : field == 3 ? b.pressure_grad : field == 4 ? b.scalar : b.scalar_grad;
         PICT_CHECK_ARG(bp != nullptr, "block %d: tensor missing", i);
     }
     if (a->desc.dtype == PICT_DTYPE_F32)
@@ -469,5 +527,9 @@ pict_status pict_copy_pressure_result_from_blocks(const pict_atlas *a, void *s) 
 pict_status pict_copy_velocity_result_grad_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 2, 1, s); }
 pict_status pict_copy_velocity_result_grad_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 2, 0, s); }
 pict_status pict_copy_pressure_result_grad_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 3, 0, s); }
+pict_status pict_copy_scalar_result_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 4, 1, s); }
+pict_status pict_copy_scalar_result_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 4, 0, s); }
+pict_status pict_copy_scalar_result_grad_to_blocks(const pict_atlas *a, void *s) { return copy_result(a, 5, 1, s); }
+pict_status pict_copy_scalar_result_grad_from_blocks(const pict_atlas *a, void *s) { return copy_result(a, 5, 0, s); }
 
 }  // extern "C"

@@ -17,6 +17,12 @@ def input_tensors(d, name):
         elif name == "BOUNDARY_VELOCITY":
             for _, fb in b.getFixedBoundaries():
                 out.append((fb.velocity, lambda fb=fb: fb.velocityGrad))
+        elif name == "PASSIVE_SCALAR" and b.passiveScalar is not None:
+            out.append((b.passiveScalar, lambda b=b: b.passiveScalarGrad))
+        elif name == "BOUNDARY_PASSIVE_SCALAR":
+            for _, fb in b.getFixedBoundaries():
+                if fb.passiveScalar is not None:
+                    out.append((fb.passiveScalar, lambda fb=fb: fb.passiveScalarGrad))
     if name == "VISCOSITY":
         out.append((d.viscosity, lambda: d.viscosityGrad))
     elif name == "VELOCITY_RESULT":
@@ -29,6 +35,10 @@ def input_tensors(d, name):
         out.append((d.pressureRHS, lambda: d.pressureRHSGrad))
     elif name == "PRESSURE_RESULT":
         out.append((d.pressureResult, lambda: d.pressureResultGrad))
+    elif name == "PASSIVE_SCALAR_VISCOSITY" and d.passiveScalarViscosity is not None:
+        out.append((d.passiveScalarViscosity, lambda: d.passiveScalarViscosityGrad))
+    elif name == "PASSIVE_SCALAR_RESULT" and d.hasPassiveScalar():
+        out.append((d.scalarResult, lambda: d.scalarResultGrad))
     return out
 
 
@@ -40,6 +50,7 @@ OUTPUTS = {
     "PRESSURE_RHS": (lambda d: d.pressureRHS, lambda d, t: d.setPressureRHSGrad(t)),
     "PRESSURE_RHS_DIV": (lambda d: d.pressureRHSdiv, lambda d, t: d.setPressureRHSdivGrad(t)),
     "VELOCITY_RESULT": (lambda d: d.velocityResult, lambda d, t: d.setVelocityResultGrad(t)),
+    "PASSIVE_SCALAR_RHS": (lambda d: d.scalarRHS, lambda d, t: d.setScalarRHSGrad(t)),
 }
 
 def create_grads(d):
@@ -57,5 +68,10 @@ def create_grads(d):
     d.CreatePressureRHSGrad()
     d.CreatePressureRHSdivGrad()
     d.CreatePressureResultGrad()
+    if d.hasPassiveScalar():
+        d.CreatePassiveScalarGradOnBlocks()
+        d.CreatePassiveScalarGradOnBoundaries()
+        d.CreateScalarRHSGrad()
+        d.CreateScalarResultGrad()
 
 

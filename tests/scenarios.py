@@ -12,9 +12,13 @@ from pict_amd import PISOtorch as P
 from pict_amd.configs import uniform_coords
 
 
+# passive-scalar channels of the domains the builders create (set by with_scalars around a builder call)
+_SCALAR_CHANNELS = [0]
+
+
 def _domain(dims, visc, dtype, device):
-    return P.Domain(dims, torch.tensor([visc], dtype=dtype), "test", dtype, device, passiveScalarChannels=0,
-                    _allow_cpu=(str(device) == "cpu"))
+    return P.Domain(dims, torch.tensor([visc], dtype=dtype), "test", dtype, device,
+                    passiveScalarChannels=_SCALAR_CHANNELS[0], _allow_cpu=(str(device) == "cpu"))
 
 
 def _coords_block(d, coords, name="Block"):
@@ -271,3 +275,38 @@ NON_ORTHO = {
     "sheared_box3d": lambda **kw: sheared_box(size=(5, 4, 6), shear=(0.3, -0.2, 0.25), periodic=("z",), bend=0.2,
                                               **kw),
 }
+
+
+def with_scalars(build, channels=2, seed=41, viscosity="per_channel", neumann=True, **kw):
+    """`build(**kw)` with `channels` passive-scalar channels: random block scalars; a scalar viscosity per channel
+    (`viscosity` "per_channel"), one for all ("static") or none (the domain viscosity, None); random boundary
+    scalars on every FIXED face, per cell on the first, static on the others; with `neumann`, channel 1 is Neumann
+    on every second FIXED face (per-channel types)."""
+    _SCALAR_CHANNELS[0] = channels
+    try:
+        d = build(**kw)
+    finally:
+        _SCALAR_CHANNELS[0] = 0
+    g = torch.Generator().manual_seed(seed)
+    dtype, device = d.getDtype(), d.getDevice()
+    dims = d.getSpatialDims()
+    for b in d.blocks:
+        b.setPassiveScalar(_rand(b.passiveScalar.shape, g, dtype, device, 1.0))
+    if viscosity == "per_channel":
+        d.setScalarViscosity(torch.tensor([0.02 + 0.03 * c for c in range(channels)], dtype=dtype, device=device))
+    elif viscosity == "static":
+        d.setScalarViscosity(torch.tensor([0.035], dtype=dtype, device=device))
+    k = 0
+    for b in d.blocks:
+        for _, fb in b.getFixedBoundaries():
+            sz = fb.getSizes()
+            shape = [1, channels] + [sz[a] for a in reversed(range(dims))] if k == 0 else [1, channels]
+            fb.setPassiveScalar(_rand(shape, g, dtype, device, 0.8))
+            if neumann and channels > 1 and k % 2 == 1:
+                types = [P.BoundaryConditionType.DIRICHLET] * channels
+                types[1] = P.BoundaryConditionType.NEUMANN
+                fb.setPassiveScalarType(types)
+            k += 1
+    if str(device) != "cpu":
+        d.UpdateDomainData()
+    return d

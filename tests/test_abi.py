@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared if not hasattr(lib, n)]
     assert not missing, missing
     assert set(declared) == set(_abi.SIGNATURES), set(declared) ^ set(_abi.SIGNATURES)
-    assert lib.pict_abi_version() == 1
+    assert lib.pict_abi_version() == _abi.ABI_VERSION == 2
 
 
 def test_struct_layout_matches_header():

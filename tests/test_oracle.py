@@ -2,7 +2,7 @@
 (scipy) and brute force on tiny grids.  Each check is chosen so that a plausible restatement mistake (a dropped
 term, a wrong sign, a transposed index, a wrong neighbour across a connection) fails it.
 
-The reference ships no golden vectors for the PISO step (SURVEY: data/ holds only a placeholder); its own tests
+The reference ships no golden vectors for the PISO step (its data/ folder holds only a placeholder); its own tests
 check invariants (tests/pisosteps.py:206-250) which are restated here as well.
 """
 import math

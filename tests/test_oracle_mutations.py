@@ -138,3 +138,55 @@ def test_non_ortho_pins_pass_on_the_oracle():
 def test_planted_mistake_fails_non_ortho_pins(name, edits, tmp_path):
     failed = _with_mutant(edits, tmp_path, _non_ortho_pins)
     assert len(failed) >= 1, (name, failed)
+
+
+# Planted mistakes in the passive-scalar stages (forward and VJP together); the scalar pins of
+# test_oracle_scalar.py (Fourier-mode closed form, wall profiles, equality with a velocity component, exactness for
+# a linear field with Dirichlet and Neumann walls) must fail for each.
+SCALAR_MUTATIONS = [
+    ("Neumann wall term with the Dirichlet factor 2 alpha",
+     [("                    else r += sv * visc;\n", "                    else r += sv * visc * 2 * alpha;\n"),
+      ("                        s_grad += visc * G;\n                        vg += sv * G;\n",
+       "                        s_grad += visc * 2 * alpha * G;\n                        vg += sv * 2 * alpha * G;\n")]),
+    ("scalar matrix with the velocity viscosity",
+     [("const real visc_p = sc ? scalar_viscosity(&c, ch) : viscosity(&c, b, pos);",
+       "const real visc_p = viscosity(&c, b, pos);"),
+      ("real visc_n = sc ? scalar_viscosity(&c, ch) : viscosity(&c, blk(&c, nb), t);",
+       "real visc_n = viscosity(&c, blk(&c, nb), t);")]),
+    ("boundary flux term with the sign flipped",
+     [("                    r -= sv * flux;\n", "                    r += sv * flux;\n"),
+      ("real s_grad = -flux * G;", "real s_grad = flux * G;"),
+      ("scatter_fixed_contravariant(&c, fb, bp, -sv * G * normal);",
+       "scatter_fixed_contravariant(&c, fb, bp, sv * G * normal);")]),
+    ("scalar walls always Dirichlet in the corner walks",
+     [("        case GD_SCALAR_RESULT: return fixed_scalar_bc(fb, ch);",
+       "        case GD_SCALAR_RESULT: return PICT_BC_DIRICHLET;")]),
+    ("scalar viscosity of channel 0 for every channel",
+     [("return v[c->d->scalar_viscosity_static ? 0 : ch];", "return v[0];")]),
+]
+
+
+def _scalar_pins():
+    import test_oracle_scalar as S
+    failed = []
+    pins = (("fourier", S.test_fourier_mode_step_closed_form),
+            ("walls", S.test_wall_profiles_dirichlet_and_neumann),
+            ("velocity_inflow", lambda: S.test_scalar_channel_equals_velocity_component("outflow_channel", 0)),
+            ("velocity_sheared", lambda: S.test_scalar_channel_equals_velocity_component("sheared_walls", 26)),
+            ("linear_field", lambda: S.test_linear_field_is_exact_with_dirichlet_and_neumann_walls(26)))
+    for name, pin in pins:
+        try:
+            pin()
+        except AssertionError:
+            failed.append(name)
+    return failed
+
+
+def test_scalar_pins_pass_on_the_oracle():
+    assert _scalar_pins() == []
+
+
+@pytest.mark.parametrize("name,edits", SCALAR_MUTATIONS, ids=[m[0] for m in SCALAR_MUTATIONS])
+def test_planted_mistake_fails_scalar_pins(name, edits, tmp_path):
+    failed = _with_mutant(edits, tmp_path, _scalar_pins)
+    assert len(failed) >= 1, (name, failed)
