@@ -82,7 +82,8 @@ struct SolveArgs {
     double tol;
     unsigned nwg;
     double crit_scale;                             // criterion factor: 1/sqrt(n) (NORM2_NORMALIZED), 1/n (ABS_MEAN), 1
-    unsigned long long *tagged;                    // k_cg_local: tagged partials [2][nwg][kSlots], zeroed
+    unsigned long long *tagged;                    // tagged partials [2][nwg][kSlots], zeroed (k_cg_local packs its
+                                                   // NS values per workgroup: [2][nwg][NS])
     void *pub;                                     // k_cg_local: published rows r, w, s [3][2][n], zeroed
     int salt;                                      // k_cg_local: stamp of this launch's foreign-read marks
 };
@@ -1190,15 +1191,6 @@ __device__ __forceinline__ void tagged_reduce(u64 *part, int *err_flag, unsigned
     }
 }
 
-#define PICT_TAG_BARRIER(L, ACC, UPDATE)                                                                     \
-    do {                                                                                                      \
-        tagged_reduce<L>(a.tagged, &a.ctl->err, a.nwg, bid, gen, ACC, !Pub<R>::kTagged, red, s_err);          \
-        ++gen;                                                                                                \
-        if (threadIdx.x == 0 && !s_err) { UPDATE; }                                                           \
-        __syncthreads();                                                                                      \
-        if (s_err) return;                                                                                    \
-    } while (0)
-
 // accumulate the residual terms of one row (slot order of CgSlots)
 template <bool FULL, int NS>
 __device__ __forceinline__ void cg_acc(double (&acc)[NS], double rv, double last, bool last_is_max) {
@@ -1210,11 +1202,153 @@ __device__ __forceinline__ void cg_acc(double (&acc)[NS], double rv, double last
     acc[NS - 1] = last_is_max ? fmax(acc[NS - 1], last) : acc[NS - 1] + last;
 }
 
-// Foreign entries (columns owned by another workgroup) of a thread's rows are kept in a
-// compact list of at most NH entries whose loads are issued unconditionally and back to
-// back (unused entries load word 0, one line per wave); a row whose foreign entries do not
-// fit is read from memory like a long row.  NH covers the rows of a 2-D grid whose chunk
-// edges cut grid lines (two foreign entries per thread) and the z neighbours of a 3-D grid.
+// Reduce-barrier of k_cg_local on tagged partials packed by workgroup: the waves combine their
+// totals through LDS, thread 0 stores the workgroup's NS values (tagged) at [set][bid][0..NS), so the
+// whole generation's partials lie on a few cache lines (64 workgroups with two values: 1 KB), and
+// wave 0 reads them (lane l: workgroups l, l + 64, ..., always in that order) until every value
+// carries the generation's tag, so all workgroups form bitwise identical totals.  The two sets
+// alternate by generation parity.  `wait_rows`: the rows this workgroup published must have landed
+// before its partials go out (untagged FP64 rows).
+constexpr int kWaves = kSolveThreads / 64;
+// 64-bit words of a tagged partial region (k_cg_local packs NS <= kSlots values per workgroup)
+constexpr size_t tagged_words(long long nwg) { return (size_t)2 * (size_t)nwg * kSlots; }
+
+template <class L>
+__device__ __forceinline__ void packed_store(u64 *part, unsigned nwg, unsigned bid, unsigned gen, bool wait_rows,
+                                             const double (&acc)[L::NS]) {
+    constexpr int NS = L::NS;
+    __shared__ double wsum[kWaves][NS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wait_rows) ops_done();
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const double w = L::MAX[s] ? wave_reduce_dpp<true>(acc[s]) : wave_reduce_dpp<false>(acc[s]);
+        if (lane == 0) wsum[wave][s] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 *mine = part + ((size_t)(gen & 1u) * nwg + bid) * NS;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double t = wsum[0][s];
+#pragma unroll
+            for (int v = 1; v < kWaves; ++v) t = L::MAX[s] ? fmax(t, wsum[v][s]) : t + wsum[v][s];
+            st_coh(mine + s, tag_partial(t, gen));
+        }
+    }
+}
+
+// wave 0: the totals of generation gen into t[slot id] (the same value in every lane); false when
+// the bounded wait ran out.  Up to 256 workgroups (4 per lane) are read per pass, their loads back
+// to back; more in passes of 256.
+template <class L>
+__device__ __forceinline__ bool packed_poll(const u64 *part, unsigned nwg, unsigned gen, double (&t)[kSlots]) {
+    constexpr int NS = L::NS;
+    const unsigned lane = threadIdx.x & 63;
+    const u64 *set = part + (size_t)(gen & 1u) * nwg * NS;
+    double v[NS];
+    unsigned spins = 0;
+    for (;;) {
+        bool ready = true;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) v[s] = 0.0;
+        for (unsigned c0 = 0; c0 < nwg; c0 += 4 * 64) {
+            u64 u[4][NS];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const unsigned w = c0 + lane + 64u * j;
+                const u64 *pw = set + (size_t)(w < nwg ? w : 0u) * NS;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) u[j][s] = ld_coh(pw + s);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (c0 + lane + 64u * j >= nwg) continue;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    ready = ready & partial_ready(u[j][s], gen);
+                    const double pv = partial_value(u[j][s]);
+                    v[s] = L::MAX[s] ? fmax(v[s], pv) : v[s] + pv;
+                }
+            }
+        }
+        if (__all(ready)) break;
+        if (++spins > kSpinLimit) return false;
+    }
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) t[s] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) t[L::ID[s]] = L::MAX[s] ? wave_reduce_dpp<true>(v[s]) : wave_reduce_dpp<false>(v[s]);
+    return true;
+}
+
+// 1/d to about one ulp: the hardware estimate and two Newton steps
+__device__ __forceinline__ double rcp64(double d) {
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    return fma(y, e, y);
+}
+
+// Solver state of k_cg_local's iterations, held by every lane of wave 0 (all lanes compute the
+// same values from the same totals).
+struct CgRun {
+    double rho, inv_rho, den, alpha, beta;  // alpha = rho / den
+    double best_crit, last_crit, final_res;
+    int active, converged, finite, used, best_it, rising, save_best, restore_best;
+};
+
+// cg1_update in the form k_cg_local runs it: the same tests (convergence cg_solver_kernel.cu:306-313,
+// best result :330-346), and the next step from
+//   beta' = gamma' / rho,  delta' - beta' gamma' / alpha = delta' - beta'^2 den,  alpha' = gamma' / that,
+// equal to cg1_update's alpha' and beta' in exact arithmetic.  The candidates are formed before the
+// tests, so one reciprocal (not three dependent divisions) lies on the path to the next iteration.
+template <bool FULL, typename R>
+__device__ __forceinline__ void cg_run_update(const SolveArgs<R> &a, CgRun &st, const double (&t)[kSlots], int it) {
+    const double gam = t[S_RR];
+    const double beta_n = gam * st.inv_rho;
+    const double den_n = fma(-beta_n * beta_n, st.den, t[S_D0]);
+    const double alpha_n = gam * rcp64(den_n);
+    const double inv_rho_n = rcp64(gam);
+    double crit = sqrt(gam) * a.crit_scale;
+    if constexpr (FULL)
+        crit = (a.conv == PICT_CONV_NORM2 || a.conv == PICT_CONV_NORM2_NORMALIZED) ? crit
+               : a.conv == PICT_CONV_ABS_MAX                                      ? t[S_AMAX]
+                                                                                  : t[S_ASUM] * a.crit_scale;
+    st.save_best = 0;
+    if (!isfinite(crit)) {
+        st.converged = 0; st.used = it; st.final_res = crit; st.finite = 0; st.active = 0;
+        return;
+    }
+    if (a.return_best) {
+        if (it == 0 || crit < st.best_crit) { st.best_crit = crit; st.best_it = it; st.save_best = 1; }
+        if (it > 0 && crit >= st.last_crit) ++st.rising; else st.rising = 0;
+        st.last_crit = crit;
+    }
+    st.used = it;
+    st.final_res = crit;
+    if (crit < a.tol) { st.converged = 1; st.finite = 1; st.active = 0; return; }
+    if (a.return_best && (it == a.maxit - 1 || st.rising >= 100)) {
+        st.restore_best = st.best_it != it;
+        st.converged = 0; st.used = st.best_it; st.final_res = st.best_crit; st.finite = 1;
+        st.active = 0;
+        return;
+    }
+    st.alpha = alpha_n;
+    st.beta = beta_n;
+    st.den = den_n;
+    st.rho = gam;
+    st.inv_rho = inv_rho_n;
+}
+
+// Foreign entries (columns owned by another workgroup) of a thread's rows: at most NH per
+// thread, whose owners' words are loaded back to back (unused entries load word 0).  Each
+// iteration the thread forms their r' and stores it in its own LDS slots after the workgroup's
+// rows (lv[C + t*NH + k]), where the row's register entry points, so every row is one uniform
+// gather from LDS.  A row whose foreign entries do not fit is read from memory like a long row.
+// NH covers the rows of a 2-D grid whose chunk edges cut grid lines (two foreign entries per
+// thread) and the z neighbours of a 3-D grid.
 template <typename R, int RPT, int NZ, bool FULL>
 __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
     constexpr int C = kSolveThreads * RPT;
@@ -1222,11 +1356,13 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
     using LF = typename CgSlots<FULL>::First;
     using LI = typename CgSlots<FULL>::Iter;
     constexpr int NS = LI::NS;
-    __shared__ R lv[C];  // r (first product) / r' (iterations) of this workgroup's rows
+    constexpr bool kTagged = Pub<R>::kTagged;
+    __shared__ R lv[C + kSolveThreads * NH];  // r (first product) / r' (iterations) of this workgroup's rows, then
+                                              // each thread's foreign entries
     __shared__ double red[kSlots];
     __shared__ RhsState sst;  // solver state through the first two reductions
     __shared__ int s_err;
-    // afterwards thread 0 keeps the state in registers and publishes what every thread needs
+    // per iteration wave 0 forms the step and publishes what every thread needs
     struct Step {
         double alpha, beta;
         int active, save_best, restore_best;
@@ -1244,15 +1380,38 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
     int herr = 0;  // a published row did not arrive in time
     auto local = [&](int c) { return (unsigned)(c - lo) < (unsigned)cnt; };
     if (threadIdx.x < kSlots) red[threadIdx.x] = 0.0;
+    if (threadIdx.x == 0) s_err = 0;
+    const int hbase = C + (int)threadIdx.x * NH;  // this thread's LDS slots of foreign entries
 
-    // register rows: entries of local columns (foreign and padding entries: column lo, value 0)
+    // reduce-barrier of the first two phases: totals into red (thread 0), then UPDATE by thread 0
+    auto first_barrier = [&](auto layout, const auto &acc, auto update) {
+        using L = decltype(layout);
+        packed_store<L>(a.tagged, a.nwg, bid, gen, !kTagged, acc);
+        if (threadIdx.x < 64) {
+            double t[kSlots];
+            const bool ok = packed_poll<L>(a.tagged, a.nwg, gen, t);
+            if (threadIdx.x == 0) {
+                if (!ok) {
+                    st_coh(&a.ctl->err, 1);
+                    s_err = 1;
+                } else {
+#pragma unroll
+                    for (int s = 0; s < L::NS; ++s) red[L::ID[s]] = t[L::ID[s]];
+                    update();
+                }
+            }
+        }
+        ++gen;
+        __syncthreads();
+    };
+
+    // register rows: matrix entries with their LDS index (own column: c - lo; foreign: the thread's slot;
+    // padding: index 0, value 0)
     int col[RPT][NZ];
     R val[RPT][NZ];
     bool row[RPT], mem[RPT], pub[RPT];  // row present / read from memory / read by another workgroup
     R x[RPT], p[RPT], r[RPT], w[RPT], s[RPT], xb[RPT];
-    // compact foreign entries: row slot, column, value
-    int nh = 0, hq[NH], hc[NH];
-    R hv[NH];
+    int nh = 0, hc[NH];  // foreign entries: global column
 
     // ---- rows into registers; stamp the foreign columns; r0 = b - A x0 (published, tag 1) ----
     {
@@ -1260,11 +1419,7 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
 #pragma unroll
         for (int k = 0; k < NS; ++k) acc[k] = 0.0;
 #pragma unroll
-        for (int k = 0; k < NH; ++k) {
-            hq[k] = -1;
-            hc[k] = 0;
-            hv[k] = R(0);
-        }
+        for (int k = 0; k < NH; ++k) hc[k] = 0;
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int li = (int)threadIdx.x + q * kSolveThreads;
@@ -1274,7 +1429,7 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
             x[q] = p[q] = r[q] = w[q] = s[q] = xb[q] = R(0);
 #pragma unroll
             for (int e = 0; e < NZ; ++e) {
-                col[q][e] = lo;
+                col[q][e] = 0;
                 val[q][e] = R(0);
             }
             if (!row[q]) continue;
@@ -1288,23 +1443,19 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
                 const int c = a.col[e];
                 const R v = mat_val(a, e);
                 if (!mem[q]) {
-                    if (local(c)) {
-#pragma unroll
-                        for (int k = 0; k < NZ; ++k)
-                            if (k == e - rs) {
-                                col[q][k] = c;
-                                val[q][k] = v;
-                            }
-                    } else {
+                    const int li_c = local(c) ? c - lo : hbase + nh;
+                    if (!local(c)) {
 #pragma unroll
                         for (int k = 0; k < NH; ++k)
-                            if (k == nh) {
-                                hq[k] = q;
-                                hc[k] = c;
-                                hv[k] = v;
-                            }
+                            if (k == nh) hc[k] = c;
                         ++nh;
                     }
+#pragma unroll
+                    for (int k = 0; k < NZ; ++k)
+                        if (k == e - rs) {
+                            col[q][k] = li_c;
+                            val[q][k] = v;
+                        }
                 }
                 if (!local(c)) st_coh(mark + c, salt);
                 sum += v * a.x[c];
@@ -1320,10 +1471,11 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
         }
         // the stamps must be visible when the generation completes
         ops_done();
-        PICT_TAG_BARRIER(LF, acc, {
+        first_barrier(LF{}, acc, [&] {
             const int za = (a.zero_shortcut && red[S_BMAX] == 0.0) ? 1 : 0;
             cg1_init(sst, red[S_RR], za);
         });
+        if (s_err) return;
     }
     if (sst.omega != 0.0) {
 #pragma unroll
@@ -1346,10 +1498,17 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
         }
         return acc;
     };
+    // one register row: a gather from LDS
+    auto product = [&](int q) {
+        R wv = 0;
+#pragma unroll
+        for (int e = 0; e < NZ; ++e) wv += val[q][e] * lv[col[q][e]];
+        return wv;
+    };
 
     // ---- w = A r (published, tag 1); gamma = (r,r), delta = (w,r); alpha = gamma/delta ----
     {
-        // all loads first (no use in between), then the tag checks
+        // all loads first (no use in between), then the tag checks; each thread's own LDS slots
         typename Pub<R>::Word ur[NH];
 #pragma unroll
         for (int k = 0; k < NH; ++k) ur[k] = pr.word(0, hc[k]);
@@ -1362,6 +1521,8 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
             for (int k = 0; k < NH; ++k)
                 if (k < nh) fr[k] = pr.get(0, hc[k], 1u, herr);
         }
+#pragma unroll
+        for (int k = 0; k < NH; ++k) lv[hbase + k] = fr[k];
         double acc[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k) acc[k] = 0.0;
@@ -1369,31 +1530,22 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
         for (int q = 0; q < RPT; ++q) {
             if (!row[q]) continue;
             const int i = lo + (int)threadIdx.x + q * kSolveThreads;
-            R wv = 0;
-            if (!mem[q]) {
-#pragma unroll
-                for (int e = 0; e < NZ; ++e) wv += val[q][e] * lv[col[q][e] - lo];
-#pragma unroll
-                for (int k = 0; k < NH; ++k)
-                    if (hq[k] == q) wv += hv[k] * fr[k];
-            } else {
-                wv = mem_row(i, [&](int c) { return pr.get(0, c, 1u, herr); });
-            }
+            const R wv = !mem[q] ? product(q) : mem_row(i, [&](int c) { return pr.get(0, c, 1u, herr); });
             w[q] = wv;
             if (pub[q]) pw.put(0, i, wv, 1u);
             cg_acc<FULL, NS>(acc, (double)r[q], (double)wv * (double)r[q], false);
         }
-        PICT_TAG_BARRIER(LI, acc, {
+        first_barrier(LI{}, acc, [&] {
             if (sst.active) {
                 sst.rho = red[S_RR];
                 sst.alpha = sst.rho / red[S_D0];
                 sst.beta = 0;
             }
         });
+        if (s_err) return;
     }
 
-    // the owners' r, w, s of the foreign entries for the next iteration: all loads back to back, issued as
-    // soon as the generation that completes them has completed
+    // the owners' r, w, s of the foreign entries for the next iteration (parity par), all loads back to back
     typename Pub<R>::Word ur[NH], uw[NH], us[NH];
     auto load_foreign = [&](int par) {
 #pragma unroll
@@ -1403,9 +1555,49 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
             us[k] = ps.word(par, hc[k]);
         }
     };
-    load_foreign(0);
-    RhsState st = sst;  // thread 0's copy is the live one
-    Step cs{st.alpha, st.beta, st.active, st.save_best, st.restore_best};
+    // FP32 (tagged words): load until every used entry carries `tag` (bounded)
+    auto fetch_foreign = [&](int par, unsigned tag) {
+        unsigned spins = 0;
+        for (;;) {
+            load_foreign(par);
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                bool m = true;
+                Pub<R>::open(ur[k], tag, m);
+                Pub<R>::open(uw[k], tag, m);
+                Pub<R>::open(us[k], tag, m);
+                ok = ok & (m | (k >= nh));
+            }
+            if (ok) return;
+            if (++spins > kSpinLimit) {
+                herr = 1;
+                return;
+            }
+        }
+    };
+    if constexpr (kTagged) fetch_foreign(0, 1u);
+    else load_foreign(0);
+    CgRun run{};
+    if (threadIdx.x < 64) {
+        run.rho = sst.rho;
+        run.alpha = sst.alpha;
+        run.beta = sst.beta;
+        run.den = red[S_D0];
+        run.inv_rho = 1.0 / sst.rho;
+        run.best_crit = sst.best_crit;
+        run.last_crit = sst.last_crit;
+        run.final_res = sst.final_res;
+        run.active = sst.active;
+        run.converged = sst.converged;
+        run.finite = sst.finite;
+        run.used = sst.used;
+        run.best_it = sst.best_it;
+        run.rising = sst.rising;
+        run.save_best = sst.save_best;
+        run.restore_best = sst.restore_best;
+    }
+    Step cs{sst.alpha, sst.beta, sst.active, sst.save_best, sst.restore_best};
     // a wave whose rows are all present and all in registers runs the iteration without the per-row tests
     bool irregular = false;
 #pragma unroll
@@ -1416,7 +1608,7 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
         const unsigned tin = (unsigned)it + 1u, tout = (unsigned)it + 2u;
         const R alpha = (R)cs.alpha, beta = (R)cs.beta;
         const bool save = cs.save_best;
-        // own rows: p, s, x, r (while the loads are in flight)
+        // own rows: p, s, x, r
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             if (!simple && !row[q]) continue;
@@ -1430,43 +1622,31 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
             r[q] = rm;
             lv[(int)threadIdx.x + q * kSolveThreads] = rm;
         }
-        R fr[NH], fw[NH], fs[NH];
-        bool ok = true;
+        // r' of the foreign entries, the owner's fma sequence, into this thread's LDS slots
+        if constexpr (kTagged) {
+            bool ok = true;
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                bool m = true;
+                Pub<R>::open(ur[k], tin, m);
+                Pub<R>::open(uw[k], tin, m);
+                Pub<R>::open(us[k], tin, m);
+                ok = ok & (m | (k >= nh));
+            }
+            if (!ok) fetch_foreign(cur, tin);
+        }
 #pragma unroll
         for (int k = 0; k < NH; ++k) {
             bool m = true;
-            fr[k] = Pub<R>::open(ur[k], tin, m);
-            fw[k] = Pub<R>::open(uw[k], tin, m);
-            fs[k] = Pub<R>::open(us[k], tin, m);
-            ok = ok & (m | (k >= nh));  // unused entries loaded word 0
+            const R fr = Pub<R>::open(ur[k], tin, m), fw = Pub<R>::open(uw[k], tin, m),
+                    fs = Pub<R>::open(us[k], tin, m);
+            lv[hbase + k] = fma(-alpha, fma(beta, fs, fw), fr);
         }
-        if (__any(!ok)) {
-            // some owner's words had not arrived: wait for each
-#pragma unroll
-            for (int k = 0; k < NH; ++k)
-                if (k < nh) {
-                    fr[k] = pr.get(cur, hc[k], tin, herr);
-                    fw[k] = pw.get(cur, hc[k], tin, herr);
-                    fs[k] = ps.get(cur, hc[k], tin, herr);
-                }
-        }
-        // r' of the foreign entries, the owner's fma sequence
-        R fx[NH];
-#pragma unroll
-        for (int k = 0; k < NH; ++k) fx[k] = fma(-alpha, fma(beta, fs[k], fw[k]), fr[k]);
         __syncthreads();
         // w' = A r'
         double acc[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k) acc[k] = 0.0;
-        auto product = [&](int q) {
-            R wv = 0;
-#pragma unroll
-            for (int e = 0; e < NZ; ++e) wv += val[q][e] * lv[col[q][e] - lo];
-#pragma unroll
-            for (int k = 0; k < NH; ++k) wv += hq[k] == q ? hv[k] * fx[k] : R(0);
-            return wv;
-        };
         auto finish_row = [&](int q, R wv) {
             const int i = lo + (int)threadIdx.x + q * kSolveThreads;
             w[q] = wv;
@@ -1492,24 +1672,31 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
                 finish_row(q, wv);
             }
         }
-        // reduce-barrier; the next iteration's loads go out while wave 0 forms the totals and the step
-        double pv[NS];
-        tagged_poll<LI>(a.tagged, &a.ctl->err, a.nwg, bid, gen, acc, !Pub<R>::kTagged, pv, s_err);
+        // reduce-barrier.  Waves 1.. fetch the neighbours' words as soon as the workgroup's partials are out;
+        // wave 0 waits for the totals first (a wave's loads also wait for its earlier stores, and a word
+        // fetched before its owner stored it costs another round trip), issues its loads, and forms the step
+        // while they travel (checked at the top of the next iteration)
+        packed_store<LI>(a.tagged, a.nwg, bid, gen, !kTagged, acc);
+        if constexpr (kTagged)
+            if (threadIdx.x >= 64) fetch_foreign(nxt, tout);
+        if (threadIdx.x < 64) {
+            double t[kSlots];
+            if (!packed_poll<LI>(a.tagged, a.nwg, gen, t)) {
+                if (threadIdx.x == 0) {
+                    st_coh(&a.ctl->err, 1);
+                    s_err = 1;
+                }
+            } else {
+                if constexpr (kTagged) load_foreign(nxt);
+                cg_run_update<FULL>(a, run, t, it);
+                if (threadIdx.x == 0) bc = Step{run.alpha, run.beta, run.active, run.save_best, run.restore_best};
+            }
+        }
         ++gen;
         __syncthreads();
         if (s_err) return;
-        load_foreign(nxt);
-        if (threadIdx.x < 64) {
-            double t[kSlots];
-#pragma unroll
-            for (int k = 0; k < kSlots; ++k) t[k] = 0.0;
-            poll_total<LI>(pv, t);
-            if (threadIdx.x == 0) {
-                cg1_update(a, st, t, it);
-                bc = Step{st.alpha, st.beta, st.active, st.save_best, st.restore_best};
-            }
-        }
-        __syncthreads();
+        // FP64 rows carry no tag: their loads go out once the generation has completed
+        if constexpr (!kTagged) load_foreign(nxt);
         cs = bc;
     }
     if (herr) st_coh(&a.ctl->err, 1);
@@ -1517,7 +1704,24 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
         if (row[q]) a.x[lo + (int)threadIdx.x + q * kSolveThreads] = restore ? xb[q] : x[q];
-    if (bid == 0 && threadIdx.x == 0) a.st[0] = st;
+    if (bid == 0 && threadIdx.x == 0) {
+        RhsState o = sst;
+        o.rho = run.rho;
+        o.alpha = run.alpha;
+        o.beta = run.beta;
+        o.best_crit = run.best_crit;
+        o.last_crit = run.last_crit;
+        o.final_res = run.final_res;
+        o.active = run.active;
+        o.converged = run.converged;
+        o.finite = run.finite;
+        o.used = run.used;
+        o.best_it = run.best_it;
+        o.rising = run.rising;
+        o.save_best = run.save_best;
+        o.restore_best = run.restore_best;
+        a.st[0] = o;
+    }
 }
 
 template <typename R, int RPT, int NZ, bool FULL>
@@ -2578,7 +2782,7 @@ Layout layout_of(const pict_csr_desc *A, int nrhs, const pict_solve_opts *o, con
     L.off_perm = L.off_st + al(sizeof(RhsState) * nrhs);
     L.off_ctl = L.off_perm + al(bytes_perm);
     L.off_zero = L.off_ctl + al(sizeof(Ctl));
-    L.words_tagged = pl.local || pl.bicg_local ? (size_t)2 * pl.nwg * kSlots : 0;
+    L.words_tagged = pl.local || pl.bicg_local ? tagged_words(pl.nwg) : 0;
     L.words_zero = pl.local        ? L.words_tagged + 6 * n
                    : pl.bicg_local ? L.words_tagged + 2 * (size_t)nrhs * n
                                    : 0;
@@ -2909,7 +3113,7 @@ void fill_args(BatchSolve &b, int s, const pict_csr_desc &A, const void *rhs, vo
                    : o->convergence_criterion == PICT_CONV_ABS_MEAN      ? 1.0 / (double)A.n
                                                                          : 1.0;
     a.tagged = reinterpret_cast<unsigned long long *>(b.arena + off_ctl[s] + al(sizeof(Ctl)));
-    a.pub = a.tagged + (b.local || b.bicg_local ? (size_t)2 * b.nwg[s] * kSlots : 0);
+    a.pub = a.tagged + (b.local || b.bicg_local ? tagged_words(b.nwg[s]) : 0);
     a.salt = next_salt();
     memcpy(b.args_host.data() + (size_t)s * sizeof(SolveArgs<R>), &a, sizeof a);
 }
@@ -3008,8 +3212,8 @@ pict_status batch_create_typed(const pict_csr_desc *A, const void *const *rhs, v
     b->off_zero = off;
     for (int s = 0; s < count; ++s) {
         off_ctl[s] = off;
-        const size_t words = b->local        ? (size_t)2 * b->nwg[s] * kSlots + 6 * (size_t)b->n[s]
-                             : b->bicg_local ? (size_t)2 * b->nwg[s] * kSlots + 2 * (size_t)nrhs * b->n[s]
+        const size_t words = b->local        ? tagged_words(b->nwg[s]) + 6 * (size_t)b->n[s]
+                             : b->bicg_local ? tagged_words(b->nwg[s]) + 2 * (size_t)nrhs * b->n[s]
                                              : 0;
         off += al(sizeof(Ctl)) + al(sizeof(unsigned long long) * words);
     }
