@@ -32,9 +32,15 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+_T0 = time.perf_counter()
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def _note(msg):
+    """A progress line on stderr (the JSON line stays alone on stdout)."""
+    print("[bench %.0fs] %s" % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
 
 
 def solver_bytes(kind, n, nnz, nrhs, s, iters):
@@ -167,6 +173,9 @@ def main():
                     help="rocprofv3 --pmc FETCH_SIZE counter_collection.csv of a run with --copy-check 256: fills "
                          "roofline.traffic (with --pmc-write)")
     ap.add_argument("--pmc-write", default=None, metavar="CSV", help="the same for WRITE_SIZE")
+    ap.add_argument("--roofline-steps", type=int, default=5,
+                    help="c4 / batched: extra steps whose solve launches are timed one by one (HIP events) for the "
+                         "pressure CG's roofline")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"c1": 50, "c2": 200, "c4": 100, "c5": 8}.get(args.config, 16)
@@ -181,6 +190,14 @@ def main():
     import torch
     from pict_amd import PISOtorch as P
 
+    if args.copy_check > 0:
+        # a known-size kernel for the counter calibration (rocprofv3 --pmc runs)
+        src = torch.ones(args.copy_check * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if args.config == "c4":
         print(json.dumps(bench_c4(args, torch)))
         return
@@ -210,6 +227,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_time(e1)
+    _note("timed region: %d steps, %.3f ms/step" % (args.steps, ms_total / args.steps))
     if grads is not None:
         assert all(torch.isfinite(g).all() for g in grads)
     # the same steps again with the solver log on (HIP events around every solver launch, which add host
@@ -219,6 +237,7 @@ def main():
     torch.cuda.synchronize()
     log = P.solve_log
     P.enable_solve_log(False)
+    _note("solver log pass done")
 
     # per-kernel roofline from the solver log
     agg = {}
@@ -252,13 +271,6 @@ def main():
             torch.cuda.synchronize()
             extra[label + "_cells_steps_per_s"] = N * args.steps / (f0.elapsed_time(f1) * 1e-3)
 
-    if args.copy_check > 0:
-        src = torch.ones(args.copy_check * (1 << 18), dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     traffic, traffic_source = None, None
     if args.pmc_fetch and args.pmc_write:
@@ -271,9 +283,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(dom, dt, args, target)
+        _note("CPU baseline done")
 
     # FP32 error of the forward steps against FP64 steps of the same workload (outside the timed region)
     precision = precision_vs_fp64(args, torch, args.steps, args.precision_samples)
+    _note("precision against FP64 done")
 
     batched = None
     if not args.no_batched and args.config == "c3":
@@ -693,6 +707,7 @@ def bench_c4(args, torch):
     t_tune = time.perf_counter()
     tuned = batch.tune(repeats=3)
     t_tune = time.perf_counter() - t_tune
+    _note("batched: launch shapes tuned in %.1f s" % t_tune)
     for _ in range(args.warmup):
         batch.step()
         for s in sims_p:
@@ -720,7 +735,9 @@ def bench_c4(args, torch):
                 assert s._PISO_split_step(1)
 
     ms_b = timed(run_batched)
+    _note("batched: %d timed steps, %.3f ms/step" % (args.steps, ms_b / args.steps))
     ms_p = timed(run_plain)
+    _note("batched: plain loop %.3f ms/step" % (ms_p / args.steps))
     info = batch.describe()
     # FP64 steps of the same cavities and settings: the error of both FP32 paths
     doms_r, sims_r, _, _, _ = _c4_sims(torch, args.batch, torch.float64)
@@ -731,27 +748,43 @@ def bench_c4(args, torch):
         if k % max(1, total // args.matched_samples) == 0:
             snaps.append([_snapshot(d) for d in doms_r])
         ref.step()
+    _note("batched: FP64 reference steps done")
     ev_b, ep_b = _c4_error(doms_b, doms_r)
     ev_p, ep_p = _c4_error(doms_p, doms_r)
-    # kernel times of one more batched step (events around each solve launch, no graph): roofline of the
-    # batched pressure CG, the dominant kernel
-    batch.set_timing(True)
-    arr = batch.step()
-    kms = batch.last_solve_ms()
-    batch.set_timing(False)
+    # roofline of the batched pressure CG, the dominant kernel: `--roofline-steps` more batched steps with events
+    # around each solve launch (no graph), each launch's algorithmic bytes from its own iteration counts
     K = 2
-    p_bytes = []
-    for k in range(K):
-        b = 0
-        for i, d in enumerate(doms_b):
-            n, nnz = d.getTotalSize(), d.C.getSize()
-            b += solver_bytes("CG", n, nnz, 1, 4, [int(arr["used_iterations"][i][2 + k]) + 1])
-        p_bytes.append(b)
-    avg_ms = sum(kms[1:1 + K]) / K
-    bytes_per_launch = sum(p_bytes) / K
+
+    def launch_bytes(used, k):
+        return sum(solver_bytes("CG", d.getTotalSize(), d.C.getSize(), 1, 4, [int(used[i][2 + k]) + 1])
+                   for i, d in enumerate(doms_b))
+    batch.set_timing(True)
+    l_ms, l_bytes, kms_all = [], [], []
+    for _ in range(max(1, args.roofline_steps)):
+        arr = batch.step()
+        kms = batch.last_solve_ms()
+        kms_all.append(kms)
+        for k in range(K):
+            l_ms.append(kms[1 + k])
+            l_bytes.append(launch_bytes(arr["used_iterations"], k))
+    batch.set_timing(False)
+    kms = [float(np.mean([x[j] for x in kms_all])) for j in range(len(kms_all[0]))]
+    avg_ms = float(np.mean(l_ms))
+    bytes_per_launch = float(np.mean(l_bytes))
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # the same model over the timed (graph-replayed) steps, for a kernel trace of them
+    timed_bytes_per_launch = float(np.mean([launch_bytes(u, k) for u in iters for k in range(K)]))
+    traffic, traffic_source = None, None
+    if args.pmc_fetch and args.pmc_write:
+        traffic, traffic_source = pmc_traffic(args.pmc_fetch, args.pmc_write, info["pressure_kernel"]), "counter files"
+    else:
+        prof = os.path.join(ROOT, "profiles", "pict_batch_step_64_cavities.md")
+        traffic = profiled_traffic(prof, info["pressure_kernel"])
+        traffic_source = os.path.relpath(prof, ROOT) if traffic is not None else None
     its = np.array(iters)
+    _note("batched: roofline launches timed")
     matched = _c4_matched_error(torch, snaps, doms_p, sims_p, batch, doms_b, dt)
+    _note("batched: per-step precision done")
     cpu = None if args.no_cpu_baseline else cpu_baseline_c4(doms_b, dt, args)
     value = N * args.steps / (ms_b * 1e-3)
     plain = N * args.steps / (ms_p * 1e-3)
@@ -788,8 +821,9 @@ def bench_c4(args, torch):
                                            "pressure_sum": float(its[:, :, 2:].sum() / args.steps)},
                    "solver_ms_per_step": {"advection": kms[0], "pressure": sum(kms[1:])}},
         "roofline": {"bound": "hbm", "kernel": info["pressure_kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_source, "bytes_per_launch": bytes_per_launch, "avg_launch_ms": avg_ms,
+                     "launches": len(l_ms), "timed_bytes_per_launch": timed_bytes_per_launch},
         "cpu_baseline": cpu,
     }
 
