@@ -48,7 +48,7 @@ struct Ctl {
     int err;                     // a wait exceeded its spin limit
     int zero_all;                // every right-hand side is zero
     unsigned pad2[30];
-    unsigned group_count[8][32]; // per-group arrivals
+    unsigned group_count[32][32];  // per-group arrivals (a line per group)
 };
 // Resident workgroups (256 threads) per compute unit that a persistent solver grid is sized for.  Grids of seven
 // and eight per unit (1792 and 2048 workgroups: the 1.77 M-row channel pressure system on the lean grid-wide CG
@@ -58,6 +58,24 @@ struct Ctl {
 // every solver kernel; kernels whose VGPRs allow fewer take the query's figure.
 constexpr int kMaxResidentWgsPerCu = 4;
 constexpr int kGroups = 8;       // arrival tree fan-in (one group per XCD when blocks are dealt round-robin)
+
+// Loads and stores that one workgroup uses to exchange data with another: agent-scope atomic
+// operations (sc1 on gfx950), coherent across the XCDs' L2 caches without L2 writeback or
+// invalidation.
+template <typename T>
+__device__ __forceinline__ void st_coh(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ld_coh(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until every memory operation this thread issued has completed
+__device__ __forceinline__ void ops_done() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 struct RhsState {
     double rho, rho_prev, alpha, omega, beta;
@@ -75,7 +93,7 @@ struct SolveArgs {
     const R *b;
     R *x;
     R *r, *rhat, *p0, *p1, *v0, *v1, *s, *t, *best;  // each [nrhs*n]
-    double *partials;                              // [nwg * nrhs * kSlots]
+    double *partials;                              // partial_doubles(nwg) (grid-wide kernels)
     Ctl *ctl;
     RhsState *st;                                  // [nrhs]
     int n, nrhs, maxit, conv, reset_steps, return_best, zero_shortcut;
@@ -274,15 +292,26 @@ __device__ __forceinline__ void acc_res(double (&acc)[NR][NS], int k, R rv) {
     } while (0)
 
 // --------------------------------------------------------------------------
-// Flat reduce-barrier with replicated reduction (single-reduction CG and
-// BiCGStab): every workgroup stores its block-reduced FP64 partials (double
-// buffered by generation parity), publishes them with a release fence and one
-// relaxed atomic increment; the last arrival opens the next generation.
-// After the wait every workgroup reduces all partials itself in a fixed order,
-// so all workgroups hold bitwise identical totals and run the same scalar
-// update on their own LDS copy of the solver state.
+// Two-level reduce-barrier of the grid-wide kernels (single-reduction CG and BiCGStab).  Workgroup b
+// belongs to group g = b mod G, G = min(nwg, kTreeGroups); with blocks dealt round-robin over the
+// eight XCDs the members of a group share one XCD.
+//  1. Every workgroup stores its block-reduced FP64 partials (double buffered by generation parity)
+//     and, after a release fence (its vector stores and partials published), counts on its group's
+//     line.
+//  2. The last arrival of a group (acquire) reduces the group's partials with wave 0 (lane l takes
+//     member g + l G, in lane order), stores the group's totals and counts on the top line.
+//  3. The last group (acquire) reduces the G group totals in group order, stores the final totals
+//     and opens the next generation (release).
+//  4. The other workgroups wait for the generation, acquire, and read the final totals.
+// Every workgroup holds bitwise identical totals (one fixed summation order) and runs the same scalar
+// update on its own LDS copy of the solver state.  A counter sees at most max(nwg / G, G) arrivals
+// per generation: with one counter for all, its 1024 serialised atomics and every workgroup's read
+// of all 1024 partials cost 34 us per barrier on the 1.77 M-row channel system.
+// Partials region: [2][nwg] workgroup lines, then kTreeGroups group lines and one line of totals.
 // --------------------------------------------------------------------------
 constexpr int kPartStride = kMaxRhs * kSlots;
+constexpr int kTreeGroups = 32;
+constexpr size_t partial_doubles(long long nwg) { return ((size_t)2 * (size_t)nwg + kTreeGroups + 1) * kPartStride; }
 
 // Wave64 reductions in DPP (quad and row rotations within 16-lane rows, then the
 // four row totals in a fixed order): every lane of the wave gets the same total,
@@ -308,13 +337,32 @@ __device__ __forceinline__ double wave_reduce_dpp(double v) {
     return op(op(lane_d(v, 0), lane_d(v, 16)), op(lane_d(v, 32), lane_d(v, 48)));
 }
 
-template <int NR, int NS>
+// FENCE = false: the caller exchanges its vectors with agent-scope atomics (st_coh / ld_coh); every wave
+// waits for its stores before the workgroup arrives, the barrier's own data go the same way, and no L2
+// writeback or invalidation is needed.
+template <int NR, int NS, bool FENCE = true>
 __device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg, unsigned bid, unsigned gen,
                                             double (&acc)[NR][NS], const int (&ids)[NS],
                                             const bool (&ismax)[NS], double *red, int &s_err) {
     __shared__ double wsum[kSolveThreads / 64][NR * NS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // 1. wave totals, then the workgroup total (thread 0, fixed order)
+    auto release = [] {
+        if constexpr (FENCE) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        else ops_done();
+    };
+    auto acquire = [] {
+        if constexpr (FENCE) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    };
+    auto put = [](double *p, double v) {
+        if constexpr (FENCE) *p = v;
+        else st_coh(p, v);
+    };
+    auto get = [](const double *p) {
+        if constexpr (FENCE) return *p;
+        else return ld_coh(p);
+    };
+    if constexpr (!FENCE) ops_done();
+    // wave totals, then the workgroup total (fixed order)
 #pragma unroll
     for (int k = 0; k < NR; ++k)
 #pragma unroll
@@ -324,29 +372,89 @@ __device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg
         }
     __syncthreads();
     if (wave != 0) return;
-    double *base = part + (size_t)(gen & 1) * nwg * kPartStride;
-    // 2. publish this workgroup's partials and arrive
+    const unsigned G = nwg < (unsigned)kTreeGroups ? nwg : (unsigned)kTreeGroups;
+    const unsigned g = bid % G;
+    const unsigned gsize = (nwg - g + G - 1) / G;
+    const double *base = part + (size_t)(gen & 1) * nwg * kPartStride;
+    double *gpart = part + (size_t)2 * nwg * kPartStride;
+    double *fin = gpart + (size_t)kTreeGroups * kPartStride;
+    auto combine = [&](int s, double x, double y) { return ismax[s] ? fmax(x, y) : x + y; };
+    // 1. this workgroup's partials; arrive at the group's line
+    int role = 0;
     if (lane == 0) {
-        double *mine = base + (size_t)bid * kPartStride;
+        double *mine = part + ((size_t)(gen & 1) * nwg + bid) * kPartStride;
 #pragma unroll
         for (int k = 0; k < NR; ++k)
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
                 double t = wsum[0][k * NS + s];
 #pragma unroll
-                for (int w = 1; w < kSolveThreads / 64; ++w)
-                    t = ismax[s] ? fmax(t, wsum[w][k * NS + s]) : t + wsum[w][k * NS + s];
-                mine[k * kSlots + ids[s]] = t;
+                for (int w = 1; w < kSolveThreads / 64; ++w) t = combine(s, t, wsum[w][k * NS + s]);
+                put(mine + k * kSlots + ids[s], t);
             }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const unsigned prev = __hip_atomic_fetch_add(&ctl->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == nwg - 1) {
-            __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-            __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        release();
+        const unsigned prev = __hip_atomic_fetch_add(&ctl->group_count[g][0], 1u, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == gsize - 1) {
+            __hip_atomic_store(&ctl->group_count[g][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            role = 1;
         }
     }
-    // 3. the whole wave waits for the generation to move (relaxed loads; one acquire after)
+    role = __shfl(role, 0, 64);
+    if (role) {
+        // 2. the group's totals (members in lane order), then the top line
+        acquire();
+        double v[NR][NS];
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) v[k][s] = 0.0;
+        for (unsigned j = lane; j < gsize; j += 64) {
+            const double *pw = base + (size_t)(g + j * G) * kPartStride;
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+#pragma unroll
+                for (int s = 0; s < NS; ++s) v[k][s] = combine(s, v[k][s], get(pw + k * kSlots + ids[s]));
+        }
+        int top_last = 0;
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const double t = ismax[s] ? wave_reduce_dpp<true>(v[k][s]) : wave_reduce_dpp<false>(v[k][s]);
+                if (lane == 0) put(gpart + (size_t)g * kPartStride + k * kSlots + ids[s], t);
+            }
+        if (lane == 0) {
+            release();
+            const unsigned top = __hip_atomic_fetch_add(&ctl->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (top == G - 1) {
+                __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                top_last = 1;
+            }
+        }
+        top_last = __shfl(top_last, 0, 64);
+        if (top_last) {
+            // 3. the final totals (groups in lane order); open the next generation
+            acquire();
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    v[k][s] = (unsigned)lane < G ? get(gpart + (size_t)lane * kPartStride + k * kSlots + ids[s]) : 0.0;
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const double t = ismax[s] ? wave_reduce_dpp<true>(v[k][s]) : wave_reduce_dpp<false>(v[k][s]);
+                    if (lane == 0) put(fin + k * kSlots + ids[s], t);
+                }
+            if (lane == 0) {
+                release();
+                __hip_atomic_fetch_add(&ctl->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    // 4. wait for the generation to move (relaxed loads; one acquire after), then the totals
     unsigned spins = 0;
     int err = 0;
     while (__hip_atomic_load(&ctl->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
@@ -357,38 +465,22 @@ __device__ __forceinline__ void flat_reduce(double *part, Ctl *ctl, unsigned nwg
             break;
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // 4. every workgroup's partials, reduced in the same fixed order by every workgroup
-    double v[NR][NS];
-#pragma unroll
-    for (int k = 0; k < NR; ++k)
-#pragma unroll
-        for (int s = 0; s < NS; ++s) v[k][s] = 0.0;
-    for (unsigned w = lane; w < nwg; w += 64) {
-        const double *pw = base + (size_t)w * kPartStride;
+    acquire();
+    if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < NR; ++k)
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const double pv = pw[k * kSlots + ids[s]];
-                v[k][s] = ismax[s] ? fmax(v[k][s], pv) : v[k][s] + pv;
-            }
+            for (int s = 0; s < NS; ++s) red[k * kSlots + ids[s]] = get(fin + k * kSlots + ids[s]);
+        s_err = err;
     }
-#pragma unroll
-    for (int k = 0; k < NR; ++k)
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const double t = ismax[s] ? wave_reduce_dpp<true>(v[k][s]) : wave_reduce_dpp<false>(v[k][s]);
-            if (lane == 0) red[k * kSlots + ids[s]] = t;
-        }
-    if (lane == 0) s_err = err;
 }
 
 // Reduce-barrier: flat_reduce leaves the totals and the error flag in LDS (written by
 // thread 0, which then runs the scalar update); one workgroup barrier publishes both.
-#define PICT_FLAT_BARRIER(NS, ACC, IDS, ISMAX, UPDATE)                                            \
+#define PICT_FLAT_BARRIER(NS, ACC, IDS, ISMAX, UPDATE) PICT_FLAT_BARRIER_F(NS, ACC, IDS, ISMAX, true, UPDATE)
+#define PICT_FLAT_BARRIER_F(NS, ACC, IDS, ISMAX, FENCE, UPDATE)                                   \
     do {                                                                                          \
-        flat_reduce<NRHS, NS>(a.partials, a.ctl, a.nwg, bid, gen, ACC, IDS, ISMAX, red, s_err);  \
+        flat_reduce<NRHS, NS, FENCE>(a.partials, a.ctl, a.nwg, bid, gen, ACC, IDS, ISMAX, red, s_err); \
         ++gen;                                                                                    \
         if (threadIdx.x == 0 && !s_err) { UPDATE; }                                               \
         __syncthreads();                                                                          \
@@ -405,6 +497,29 @@ struct RowCache {
     int col[RPT][kMaxNz];
     R val[RPT][kMaxNz];
 };
+
+// One row's entries in registers, loaded back to back (one memory round trip instead of one per
+// entry).  nz < 0: the row is empty or longer than kMaxNz (the caller reads it from memory).  Entries
+// past nz repeat the last one's column (a valid address to load from) and are never used.
+template <typename R>
+struct RowRegs {
+    int nz;
+    int col[kMaxNz];
+    R val[kMaxNz];
+};
+template <typename R>
+__device__ __forceinline__ void load_row(const SolveArgs<R> &a, size_t i, RowRegs<R> &rr) {
+    const int rs = a.row[i], re = a.row[i + 1];
+    const int nz = re - rs;
+    rr.nz = nz > 0 && nz <= kMaxNz ? nz : -1;
+    if (rr.nz < 0) return;
+#pragma unroll
+    for (int e = 0; e < kMaxNz; ++e) {
+        const int idx = rs + (e < nz ? e : nz - 1);
+        rr.col[e] = a.col[idx];
+        rr.val[e] = mat_val(a, idx);
+    }
+}
 
 // ==========================================================================
 // CG  (cgSolveGPU, cg_solver_kernel.cu:114-456)
@@ -695,12 +810,25 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
     constexpr bool MAX_G[5] = {false, false, true, false, false};
     int cur = 0;  // buffer holding r_i, w_i, s_{i-1}
     int zero_all = 0;
+    // rows of this thread: grid-stride (i0, i0 + stride, ...) for the kernels with the row cache; the lean kernel
+    // walks a contiguous chunk per workgroup, and the chunks of one XCD lie next to each other (workgroup b runs on
+    // XCD b mod 8 under round-robin dispatch), so the neighbours a grid line or plane away are mostly rows of the
+    // same workgroup or of workgroups that share its L2 (the placement only affects speed)
+    size_t r_lo = i0, r_hi = (size_t)n, r_step = stride;
+    if constexpr (LEAN) {
+        const size_t nw = a.nwg;
+        const size_t chunk = ((size_t)n + nw - 1) / nw;
+        const size_t c = nw % 8 == 0 ? (size_t)(bid % 8) * (nw / 8) + bid / 8 : (size_t)bid;
+        r_lo = c * chunk + threadIdx.x;
+        r_hi = min((size_t)n, (c + 1) * chunk);
+        r_step = kSolveThreads;
+    }
 
     // ---- phase I0: r0 = b - A x0; p = s = 0 ----
     {
         double acc[NRHS][4];
         zero_acc(acc);
-        for (size_t i = i0; i < (size_t)n; i += stride) {
+        for (size_t i = r_lo; i < r_hi; i += r_step) {
             const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
             for (int k = 0; k < NRHS; ++k) {
@@ -726,7 +854,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
         zero_all = sst[0].omega != 0.0;
     }
     if (zero_all) {
-        for (size_t i = i0; i < (size_t)n; i += stride)
+        for (size_t i = r_lo; i < r_hi; i += r_step)
             for (int k = 0; k < NRHS; ++k) a.x[(size_t)k * n + i] = 0;
         if (blockIdx.x == 0 && threadIdx.x == 0)
             for (int k = 0; k < NRHS; ++k) a.st[k] = sst[k];
@@ -740,7 +868,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
         _Pragma("unroll") for (int k = 0; k < NRHS; ++k) act[k] = k < NRHS && sst[k].active; \
         double acc[NRHS][5];                                                                   \
         zero_acc(acc);                                                                            \
-        for (size_t i = i0; i < (size_t)n; i += stride) {                                         \
+        for (size_t i = r_lo; i < r_hi; i += r_step) {                                         \
             const int rs = a.row[i], re = a.row[i + 1];                                           \
             _Pragma("unroll") for (int k = 0; k < NRHS; ++k) {                                 \
                 if (!act[k]) continue;                                                            \
@@ -796,7 +924,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                 for (int k = 0; k < NRHS; ++k) act[k] = k < NRHS && sst[k].active;
                 double acc[NRHS][4];
                 zero_acc(acc);
-                for (size_t i = i0; i < (size_t)n; i += stride) {
+                for (size_t i = r_lo; i < r_hi; i += r_step) {
                     const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
                     for (int k = 0; k < NRHS; ++k) {
@@ -837,19 +965,20 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     if (save[k]) a.best[o + i] = a.x[o + i];
-                    const R pm = fma(beta[k], p[o + i], r[o + i]);
-                    const R sm = fma(beta[k], so[o + i], w[o + i]);
+                    const R ri = ld_coh(r + o + i);
+                    const R pm = fma(beta[k], p[o + i], ri);
+                    const R sm = fma(beta[k], ld_coh(so + o + i), ld_coh(w + o + i));
                     p[o + i] = pm;
-                    sn[o + i] = sm;
+                    st_coh(sn + o + i, sm);
                     a.x[o + i] += alpha[k] * pm;
-                    const R rm = fma(-alpha[k], sm, r[o + i]);
-                    rn[o + i] = rm;
+                    const R rm = fma(-alpha[k], sm, ri);
+                    st_coh(rn + o + i, rm);
                     R wv = 0;
                     ent(nz, [&](int cidx, R v) {
                         const size_t c = o + cidx;
-                        wv += v * fma(-alpha[k], fma(beta[k], so[c], w[c]), r[c]);
+                        wv += v * fma(-alpha[k], fma(beta[k], ld_coh(so + c), ld_coh(w + c)), ld_coh(r + c));
                     });
-                    wn[o + i] = wv;
+                    st_coh(wn + o + i, wv);
                     acc_res(acc, k, rm);
                     acc[k][3] += (double)wv * (double)rm;
                 }
@@ -863,7 +992,44 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
             // register budget: the gathered form for up to 8 bytes of right-hand-side values per row
             constexpr bool kGather = NRHS * sizeof(R) <= 8;
             if constexpr (LEAN) {
-                for (size_t i = i0; i < (size_t)n; i += stride) mem_row(i);
+                // per row: its entries (one round trip), then every gather (one more), then the update
+                for (size_t i = r_lo; i < r_hi; i += r_step) {
+                    RowRegs<R> rr;
+                    load_row(a, i, rr);
+                    if (rr.nz < 0) {
+                        mem_row(i);
+                        continue;
+                    }
+#pragma unroll
+                    for (int k = 0; k < NRHS; ++k) {
+                        if (!act[k]) continue;
+                        const size_t o = (size_t)k * n;
+                        R gr[kMaxNz], gw[kMaxNz], gs[kMaxNz];
+#pragma unroll
+                        for (int e = 0; e < kMaxNz; ++e) {
+                            const size_t c = o + rr.col[e];
+                            gr[e] = ld_coh(r + c);
+                            gw[e] = ld_coh(w + c);
+                            gs[e] = ld_coh(so + c);
+                        }
+                        const R ri = ld_coh(r + o + i), wi = ld_coh(w + o + i), si = ld_coh(so + o + i);
+                        if (save[k]) a.best[o + i] = a.x[o + i];
+                        const R pm = fma(beta[k], p[o + i], ri);
+                        const R sm = fma(beta[k], si, wi);
+                        p[o + i] = pm;
+                        st_coh(sn + o + i, sm);
+                        a.x[o + i] += alpha[k] * pm;
+                        const R rm = fma(-alpha[k], sm, ri);
+                        st_coh(rn + o + i, rm);
+                        R wv = 0;
+#pragma unroll
+                        for (int e = 0; e < kMaxNz; ++e)
+                            if (e < rr.nz) wv += rr.val[e] * fma(-alpha[k], fma(beta[k], gs[e], gw[e]), gr[e]);
+                        st_coh(wn + o + i, wv);
+                        acc_res(acc, k, rm);
+                        acc[k][3] += (double)wv * (double)rm;
+                    }
+                }
             } else if (cached && !kGather) {
 #pragma unroll
                 for (int q = 0; q < kRowsCached; ++q) {
@@ -894,16 +1060,16 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                         const size_t o = (size_t)k * n;
                         op[q][k] = p[o + i];
                         ox[q][k] = a.x[o + i];
-                        ori[q][k] = r[o + i];
-                        owi[q][k] = w[o + i];
-                        osi[q][k] = so[o + i];
+                        ori[q][k] = ld_coh(r + o + i);
+                        owi[q][k] = ld_coh(w + o + i);
+                        osi[q][k] = ld_coh(so + o + i);
 #pragma unroll
                         for (int e = 0; e < kMaxNz; ++e) {
                             if (e < rc.len[q]) {
                                 const size_t c = o + rc.col[q][e];
-                                gr[q][k][e] = r[c];
-                                gw[q][k][e] = w[c];
-                                gs[q][k][e] = so[c];
+                                gr[q][k][e] = ld_coh(r + c);
+                                gw[q][k][e] = ld_coh(w + c);
+                                gs[q][k][e] = ld_coh(so + c);
                             }
                         }
                     }
@@ -924,27 +1090,35 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
                         const R pm = fma(beta[k], op[q][k], ori[q][k]);
                         const R sm = fma(beta[k], osi[q][k], owi[q][k]);
                         p[o + i] = pm;
-                        sn[o + i] = sm;
+                        st_coh(sn + o + i, sm);
                         R xv = ox[q][k];
                         xv += alpha[k] * pm;
                         a.x[o + i] = xv;
                         const R rm = fma(-alpha[k], sm, ori[q][k]);
-                        rn[o + i] = rm;
+                        st_coh(rn + o + i, rm);
                         R wv = 0;
 #pragma unroll
                         for (int e = 0; e < kMaxNz; ++e)
                             if (e < rc.len[q]) wv += rc.val[q][e] * fma(-alpha[k], fma(beta[k], gs[q][k][e], gw[q][k][e]), gr[q][k][e]);
-                        wn[o + i] = wv;
+                        st_coh(wn + o + i, wv);
                         acc_res(acc, k, rm);
                         acc[k][3] += (double)wv * (double)rm;
                     }
                 }
             } else {
-                for (size_t i = i0; i < (size_t)n; i += stride) mem_row(i);
+                for (size_t i = r_lo; i < r_hi; i += r_step) mem_row(i);
             }
-            PICT_FLAT_BARRIER(5, acc, IDS_G, MAX_G, {
-                for (int k = 0; k < NRHS; ++k) cg1_update(a, sst[k], red + k * kSlots, it);
-            });
+            // r, w and s travel as agent-scope atomics: the barrier needs no L2 writeback or invalidation,
+            // except before a residual reset, which reads the neighbours' x (plain stores)
+            if (a.reset_steps > 0) {
+                PICT_FLAT_BARRIER(5, acc, IDS_G, MAX_G, {
+                    for (int k = 0; k < NRHS; ++k) cg1_update(a, sst[k], red + k * kSlots, it);
+                });
+            } else {
+                PICT_FLAT_BARRIER_F(5, acc, IDS_G, MAX_G, false, {
+                    for (int k = 0; k < NRHS; ++k) cg1_update(a, sst[k], red + k * kSlots, it);
+                });
+            }
         }
         cur = nxt;
         bool any = false;
@@ -955,7 +1129,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
     for (int k = 0; k < NRHS; ++k) {
         if (!sst[k].restore_best) continue;
         const size_t o = (size_t)k * n;
-        for (size_t i = i0; i < (size_t)n; i += stride) a.x[o + i] = a.best[o + i];
+        for (size_t i = r_lo; i < r_hi; i += r_step) a.x[o + i] = a.best[o + i];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         for (int k = 0; k < NRHS; ++k) a.st[k] = sst[k];
@@ -990,20 +1164,6 @@ __global__ __launch_bounds__(kSolveThreads) void k_cg1(SolveArgs<R> a) {
 //    partials.
 // Both regions are zeroed before the launch (tag 0 is never awaited).
 // ==========================================================================
-template <typename T>
-__device__ __forceinline__ void st_coh(T *p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T ld_coh(const T *p) {
-    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// wait until every memory operation this thread issued has completed
-__device__ __forceinline__ void ops_done() {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
 
 typedef unsigned long long u64;
 // A slot is rewritten every second generation, so a stale slot is exactly two generations
@@ -2248,7 +2408,9 @@ __global__ void k_zero64(u64 *p, size_t n) {
 // BiCGStab without preconditioner (bicgstabSolveGPU, bicgstab_solver_kernel.cu:222-365)
 // Three reduce-barriers per iteration: (rhat,v) | half-step norm, (t,s), (t,t) |
 // norm, (rhat,r).  p = r + beta (p - omega v) and s = r - alpha v are folded
-// into the following SpMVs, evaluated on the fly for the neighbours.
+// into the following SpMVs, evaluated on the fly for the neighbours.  The vectors
+// other workgroups read in the iterations (p, v, r) travel as agent-scope atomics,
+// so the iteration barriers need no L2 writeback or invalidation.
 // ==========================================================================
 template <typename R, int NRHS>
 __device__ __forceinline__ void bicgstab(const SolveArgs<R> &a, unsigned bid) {
@@ -2347,23 +2509,40 @@ __device__ __forceinline__ void bicgstab(const SolveArgs<R> &a, unsigned bid) {
             double acc[NRHS][1];
             zero_acc(acc);
             for (size_t i = i0; i < (size_t)n; i += stride) {
+                // the row's entries in registers when they fit (one round trip), then each right-hand side's gathers
+                RowRegs<R> rr;
+                load_row(a, i, rr);
                 const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
                 for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
-                    const R pn = fma(beta[k], fma(mom[k], vold[o + i], pold[o + i]), a.r[o + i]);
-                    pnew[o + i] = pn;
+                    const R pn = fma(beta[k], fma(mom[k], ld_coh(vold + o + i), ld_coh(pold + o + i)), ld_coh(a.r + o + i));
+                    st_coh(pnew + o + i, pn);
                     R s = 0;
-                    for (int e = rs; e < re; ++e) {
-                        const size_t c = o + a.col[e];
-                        s += mat_val(a, e) * fma(beta[k], fma(mom[k], vold[c], pold[c]), a.r[c]);
+                    if (rr.nz >= 0) {
+                        R gv[kMaxNz], gp[kMaxNz], gr[kMaxNz];
+#pragma unroll
+                        for (int e = 0; e < kMaxNz; ++e) {
+                            const size_t c = o + rr.col[e];
+                            gv[e] = ld_coh(vold + c);
+                            gp[e] = ld_coh(pold + c);
+                            gr[e] = ld_coh(a.r + c);
+                        }
+#pragma unroll
+                        for (int e = 0; e < kMaxNz; ++e)
+                            if (e < rr.nz) s += rr.val[e] * fma(beta[k], fma(mom[k], gv[e], gp[e]), gr[e]);
+                    } else {
+                        for (int e = rs; e < re; ++e) {
+                            const size_t c = o + a.col[e];
+                            s += mat_val(a, e) * fma(beta[k], fma(mom[k], ld_coh(vold + c), ld_coh(pold + c)), ld_coh(a.r + c));
+                        }
                     }
-                    vnew[o + i] = s;
+                    st_coh(vnew + o + i, s);
                     acc[k][0] += (double)a.rhat[o + i] * (double)s;
                 }
             }
-            PICT_FLAT_BARRIER(1, acc, IDS_D, MAX_D, {
+            PICT_FLAT_BARRIER_F(1, acc, IDS_D, MAX_D, false, {
                 for (int k = 0; k < NRHS; ++k)
                     if (sst[k].active) sst[k].alpha = sst[k].rho / red[k * kSlots + S_D0];
             });
@@ -2380,27 +2559,42 @@ __device__ __forceinline__ void bicgstab(const SolveArgs<R> &a, unsigned bid) {
             double acc[NRHS][5];
             zero_acc(acc);
             for (size_t i = i0; i < (size_t)n; i += stride) {
+                RowRegs<R> rr;
+                load_row(a, i, rr);
                 const int rs = a.row[i], re = a.row[i + 1];
 #pragma unroll
                 for (int k = 0; k < NRHS; ++k) {
                     if (!act[k]) continue;
                     const size_t o = (size_t)k * n;
                     const R mal = -alpha[k];
-                    const R sv = fma(mal, vnew[o + i], a.r[o + i]);
+                    const R sv = fma(mal, ld_coh(vnew + o + i), ld_coh(a.r + o + i));
                     a.s[o + i] = sv;
                     R tv = 0;
-                    for (int e = rs; e < re; ++e) {
-                        const size_t c = o + a.col[e];
-                        tv += mat_val(a, e) * fma(mal, vnew[c], a.r[c]);
+                    if (rr.nz >= 0) {
+                        R gv[kMaxNz], gr[kMaxNz];
+#pragma unroll
+                        for (int e = 0; e < kMaxNz; ++e) {
+                            const size_t c = o + rr.col[e];
+                            gv[e] = ld_coh(vnew + c);
+                            gr[e] = ld_coh(a.r + c);
+                        }
+#pragma unroll
+                        for (int e = 0; e < kMaxNz; ++e)
+                            if (e < rr.nz) tv += rr.val[e] * fma(mal, gv[e], gr[e]);
+                    } else {
+                        for (int e = rs; e < re; ++e) {
+                            const size_t c = o + a.col[e];
+                            tv += mat_val(a, e) * fma(mal, ld_coh(vnew + c), ld_coh(a.r + c));
+                        }
                     }
                     a.t[o + i] = tv;
-                    a.x[o + i] += alpha[k] * pnew[o + i];
+                    a.x[o + i] += alpha[k] * ld_coh(pnew + o + i);
                     acc_res(acc, k, sv);
                     acc[k][3] += (double)tv * (double)sv;
                     acc[k][4] += (double)tv * (double)tv;
                 }
             }
-            PICT_FLAT_BARRIER(5, acc, IDS_B, MAX_B, {
+            PICT_FLAT_BARRIER_F(5, acc, IDS_B, MAX_B, false, {
                 for (int k = 0; k < NRHS; ++k) {
                     RhsState &st = sst[k];
                     if (!st.active) continue;
@@ -2436,12 +2630,12 @@ __device__ __forceinline__ void bicgstab(const SolveArgs<R> &a, unsigned bid) {
                     const R sv = a.s[o + i];
                     a.x[o + i] += omega[k] * sv;
                     const R rv = sv - omega[k] * a.t[o + i];
-                    a.r[o + i] = rv;
+                    st_coh(a.r + o + i, rv);
                     acc_res(acc, k, rv);
                     acc[k][3] += (double)a.rhat[o + i] * (double)rv;
                 }
             }
-            PICT_FLAT_BARRIER(4, acc, IDS_C, MAX_C, {
+            PICT_FLAT_BARRIER_F(4, acc, IDS_C, MAX_C, false, {
                 for (int k = 0; k < NRHS; ++k) {
                     RhsState &st = sst[k];
                     if (!st.active) continue;
@@ -2756,6 +2950,8 @@ pict_status plan_solve(const pict_csr_desc *A, int nrhs, const pict_solve_opts *
             pl.kernel = k;
             pl.lean = true;
             pl.nwg = grid(locc);
+            // a multiple of eight workgroups: the lean kernel places the chunks of one XCD next to each other
+            if (pl.nwg >= 16) pl.nwg -= pl.nwg % 8;
         }
     }
     return PICT_OK;
@@ -2775,7 +2971,7 @@ Layout layout_of(const pict_csr_desc *A, int nrhs, const pict_solve_opts *o, con
     const size_t n = (size_t)A->n;
     L.vec = n * nrhs;
     const int nvec = pl.kind == Kind::kCGTextbook ? (o->return_best_result ? 5 : 4) : 9;
-    const size_t bytes_part = sizeof(double) * (size_t)2 * pl.nwg * kPartStride;
+    const size_t bytes_part = sizeof(double) * partial_doubles(pl.nwg);
     const size_t bytes_perm = (o->transpose_a && !A->transpose_perm) ? sizeof(int32_t) * (size_t)A->nnz : 0;
     L.off_part = al(sizeof(R) * L.vec * nvec);
     L.off_st = L.off_part + al(bytes_part);
@@ -3207,7 +3403,7 @@ pict_status batch_create_typed(const pict_csr_desc *A, const void *const *rhs, v
         off_vec[s] = off;
         off += al(sizeof(R) * (size_t)b->n[s] * nrhs * 9);
         off_part[s] = off;
-        off += al(sizeof(double) * (size_t)2 * b->nwg[s] * kPartStride);
+        off += al(sizeof(double) * partial_doubles(b->nwg[s]));
     }
     b->off_zero = off;
     for (int s = 0; s < count; ++s) {
