@@ -275,8 +275,9 @@ def main():
     traffic, traffic_source = None, None
     if args.pmc_fetch and args.pmc_write:
         traffic, traffic_source = pmc_traffic(args.pmc_fetch, args.pmc_write, dom_kernel), "counter files"
-    elif args.config == "c3" and args.mode == "fwdbwd":
-        prof = os.path.join(ROOT, "profiles", "pict_piso_step_fwdbwd_periodic_box_256x256.md")
+    elif args.mode == "fwdbwd" and args.config in ("c3", "c5"):
+        prof = os.path.join(ROOT, "profiles", "pict_piso_step_fwdbwd_periodic_box_256x256.md" if args.config == "c3"
+                            else "pict_piso_step_fwdbwd_channel_192x96x96.md")
         traffic = profiled_traffic(prof, dom_kernel)
         traffic_source = os.path.relpath(prof, ROOT) if traffic is not None else None
 
