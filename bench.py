@@ -377,14 +377,17 @@ def bench_c2(args, torch):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_time(e1)
+    _note("c2: timed region %d steps, %.1f ms/step" % (args.steps, ms_total / args.steps))
     infos = sim.solver_infos
     adv_its = [sum(i.usedIterations for i in inf) for k, inf in infos if k == "BiCG"]
     p_its = [inf[0].usedIterations for k, inf in infos if k == "CG"]
     # FP64 (refined) and plain FP32 runs of the same steps from the same initial state
     ref, sref = _c2_sim(torch, torch.float64, 1e-10)
     assert sref._PISO_split_step(args.warmup + args.steps)
+    _note("c2: FP64 run done")
     plain, splain = _c2_sim(torch, torch.float32, None)
     assert splain._PISO_split_step(args.warmup + args.steps)
+    _note("c2: plain FP32 run done")
     err = {"refined": {"velocity": _max_rel([dom], [ref], "velocity"),
                        "pressure": _max_rel([dom], [ref], "pressure")},
            "plain_fp32": {"velocity": _max_rel([plain], [ref], "velocity"),

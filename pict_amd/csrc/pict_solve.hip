@@ -1506,9 +1506,12 @@ __device__ __forceinline__ void cg_run_update(const SolveArgs<R> &a, CgRun &st, 
 // thread, whose owners' words are loaded back to back (unused entries load word 0).  Each
 // iteration the thread forms their r' and stores it in its own LDS slots after the workgroup's
 // rows (lv[C + t*NH + k]), where the row's register entry points, so every row is one uniform
-// gather from LDS.  A row whose foreign entries do not fit is read from memory like a long row.
-// NH covers the rows of a 2-D grid whose chunk edges cut grid lines (two foreign entries per
-// thread) and the z neighbours of a 3-D grid.
+// gather from LDS.  NH covers the rows of a 2-D grid whose chunk edges cut grid lines (two
+// foreign entries per thread) and the z neighbours of a 3-D grid.  Foreign entries beyond a
+// thread's NH (a grid whose block connections run along the chunk, such as an O-grid's seam:
+// a foreign neighbour in each of a thread's rows) go to the workgroup's shared slots
+// lv[C + 256*NH + j], one per thread: thread j loads that column's words and forms its r' with
+// the private ones.  A row whose entries fit neither is read from memory like a long row.
 template <typename R, int RPT, int NZ, bool FULL>
 __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
     constexpr int C = kSolveThreads * RPT;
@@ -1517,8 +1520,11 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
     using LI = typename CgSlots<FULL>::Iter;
     constexpr int NS = LI::NS;
     constexpr bool kTagged = Pub<R>::kTagged;
-    __shared__ R lv[C + kSolveThreads * NH];  // r (first product) / r' (iterations) of this workgroup's rows, then
-                                              // each thread's foreign entries
+    __shared__ R lv[C + kSolveThreads * NH + kSolveThreads];  // r (first product) / r' (iterations) of this
+                                                             // workgroup's rows, each thread's foreign entries,
+                                                             // then the shared foreign slots
+    __shared__ int s_ovf_col[kSolveThreads];                 // column of each shared slot (-1: unused)
+    __shared__ int s_novf;
     __shared__ double red[kSlots];
     __shared__ RhsState sst;  // solver state through the first two reductions
     __shared__ int s_err;
@@ -1540,8 +1546,14 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
     int herr = 0;  // a published row did not arrive in time
     auto local = [&](int c) { return (unsigned)(c - lo) < (unsigned)cnt; };
     if (threadIdx.x < kSlots) red[threadIdx.x] = 0.0;
-    if (threadIdx.x == 0) s_err = 0;
+    if (threadIdx.x == 0) {
+        s_err = 0;
+        s_novf = 0;
+    }
+    s_ovf_col[threadIdx.x] = -1;
     const int hbase = C + (int)threadIdx.x * NH;  // this thread's LDS slots of foreign entries
+    constexpr int obase = C + kSolveThreads * NH;  // the shared slots
+    __syncthreads();
 
     // reduce-barrier of the first two phases: totals into red (thread 0), then UPDATE by thread 0
     auto first_barrier = [&](auto layout, const auto &acc, auto update) {
@@ -1597,18 +1609,29 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
             const int rs = a.row[i], re = a.row[i + 1];
             int nf = 0;
             for (int e = rs; e < re; ++e) nf += local(a.col[e]) ? 0 : 1;
-            mem[q] = re - rs > NZ || nh + nf > NH;
+            // foreign entries: the thread's free private slots first, then shared slots
+            const int npriv = min(nf, NH - nh), nshared = nf - npriv;
+            int sj = 0;
+            if (nshared > 0 && re - rs <= NZ) sj = atomicAdd(&s_novf, nshared);
+            mem[q] = re - rs > NZ || sj + nshared > kSolveThreads;
             R sum = 0;
             for (int e = rs; e < re; ++e) {
                 const int c = a.col[e];
                 const R v = mat_val(a, e);
                 if (!mem[q]) {
-                    const int li_c = local(c) ? c - lo : hbase + nh;
+                    int li_c = c - lo;
                     if (!local(c)) {
+                        if (nh < NH) {
+                            li_c = hbase + nh;
 #pragma unroll
-                        for (int k = 0; k < NH; ++k)
-                            if (k == nh) hc[k] = c;
-                        ++nh;
+                            for (int k = 0; k < NH; ++k)
+                                if (k == nh) hc[k] = c;
+                            ++nh;
+                        } else {
+                            li_c = obase + sj;
+                            s_ovf_col[sj] = c;
+                            ++sj;
+                        }
                     }
 #pragma unroll
                     for (int k = 0; k < NZ; ++k)
@@ -1647,6 +1670,8 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
         if (row[q]) pub[q] = ld_coh(mark + lo + (int)threadIdx.x + q * kSolveThreads) == salt;
+    const int oc = s_ovf_col[threadIdx.x];  // the column of this thread's shared slot (-1: none)
+    const int oslot = obase + (int)threadIdx.x;
 
     // one row read from memory: sum of A_ij * v(j), v(j) = lv for local columns, else `far(j)`
     auto mem_row = [&](int i, auto far) {
@@ -1683,6 +1708,8 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
         }
 #pragma unroll
         for (int k = 0; k < NH; ++k) lv[hbase + k] = fr[k];
+        if (oc >= 0) lv[oslot] = pr.get(0, oc, 1u, herr);
+        __syncthreads();  // the shared slots are read by other threads
         double acc[NS];
 #pragma unroll
         for (int k = 0; k < NS; ++k) acc[k] = 0.0;
@@ -1706,13 +1733,18 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
     }
 
     // the owners' r, w, s of the foreign entries for the next iteration (parity par), all loads back to back
-    typename Pub<R>::Word ur[NH], uw[NH], us[NH];
+    typename Pub<R>::Word ur[NH], uw[NH], us[NH], uro = {}, uwo = {}, uso = {};
     auto load_foreign = [&](int par) {
 #pragma unroll
         for (int k = 0; k < NH; ++k) {
             ur[k] = pr.word(par, hc[k]);
             uw[k] = pw.word(par, hc[k]);
             us[k] = ps.word(par, hc[k]);
+        }
+        if (oc >= 0) {
+            uro = pr.word(par, oc);
+            uwo = pw.word(par, oc);
+            uso = ps.word(par, oc);
         }
     };
     // FP32 (tagged words): load until every used entry carries `tag` (bounded)
@@ -1728,6 +1760,13 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
                 Pub<R>::open(uw[k], tag, m);
                 Pub<R>::open(us[k], tag, m);
                 ok = ok & (m | (k >= nh));
+            }
+            {
+                bool m = true;
+                Pub<R>::open(uro, tag, m);
+                Pub<R>::open(uwo, tag, m);
+                Pub<R>::open(uso, tag, m);
+                ok = ok & (m | (oc < 0));
             }
             if (ok) return;
             if (++spins > kSpinLimit) {
@@ -1793,6 +1832,13 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
                 Pub<R>::open(us[k], tin, m);
                 ok = ok & (m | (k >= nh));
             }
+            {
+                bool m = true;
+                Pub<R>::open(uro, tin, m);
+                Pub<R>::open(uwo, tin, m);
+                Pub<R>::open(uso, tin, m);
+                ok = ok & (m | (oc < 0));
+            }
             if (!ok) fetch_foreign(cur, tin);
         }
 #pragma unroll
@@ -1801,6 +1847,11 @@ __device__ __forceinline__ void cg_local(const SolveArgs<R> &a, unsigned bid) {
             const R fr = Pub<R>::open(ur[k], tin, m), fw = Pub<R>::open(uw[k], tin, m),
                     fs = Pub<R>::open(us[k], tin, m);
             lv[hbase + k] = fma(-alpha, fma(beta, fs, fw), fr);
+        }
+        if (oc >= 0) {
+            bool m = true;
+            const R fr = Pub<R>::open(uro, tin, m), fw = Pub<R>::open(uwo, tin, m), fs = Pub<R>::open(uso, tin, m);
+            lv[oslot] = fma(-alpha, fma(beta, fs, fw), fr);
         }
         __syncthreads();
         // w' = A r'
