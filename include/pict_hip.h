@@ -223,6 +223,12 @@ pict_status pict_correct_velocity(const pict_atlas *atlas, double time_step, int
                                   int32_t time_step_norm, void *stream);
 /* ComputeVelocityDivergence (PISOtorch.cpp:528; kernel :6333-6354): out [N]. */
 pict_status pict_compute_velocity_divergence(const pict_atlas *atlas, void *out, void *stream);
+/* ComputeSpatialVelocityGradients (PISOtorch.cpp:530; kernel :6445-6467, getBlockDataGradient :2982-3025):
+   out [D][D][N], out[(c*D + i)*N + g] = d u_c / d x_i at global cell g (component-major per velocity component). */
+pict_status pict_compute_spatial_velocity_gradients(const pict_atlas *atlas, void *out, void *stream);
+/* SGSviscosityIncompressibleSmagorinsky (PISOtorch.cpp:540-541; kernel :6898-6952): out [N], the Smagorinsky
+   viscosity coefficient * Delta^2 * |S| of every cell. */
+pict_status pict_sgs_viscosity_smagorinsky(const pict_atlas *atlas, double coefficient, void *out, void *stream);
 
 /* Copy{Velocity,Pressure}Result{To,From}Blocks (PISOtorch.cpp:533-538; :6541-6733). */
 pict_status pict_copy_velocity_result_to_blocks(const pict_atlas *atlas, void *stream);

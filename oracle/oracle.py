@@ -46,6 +46,8 @@ def declare(L):
     L.oracle_pressure_rhs_add_non_ortho_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_correct_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
     L.oracle_compute_velocity_divergence.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.oracle_spatial_velocity_gradients.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.oracle_sgs_smagorinsky.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
     L.oracle_copy_velocity_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_copy_pressure_result.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_build_csr_pattern.argtypes = [ctypes.c_void_p]
@@ -224,6 +226,23 @@ def ComputeVelocityDivergence(domain):
     L = lib(_bits(domain))
     out = torch.zeros(domain.getTotalSize(), dtype=domain.getDtype())
     _chk(L, L.oracle_compute_velocity_divergence(_desc(domain), ctypes.c_void_p(out.data_ptr())), "divergence")
+    return out
+
+
+def SpatialVelocityGradients(domain):
+    """The spatial velocity gradients as one global vector [D, D, N]: [c, i, g] = d u_c / d x_i at cell g."""
+    L = lib(_bits(domain))
+    D, N = domain.getSpatialDims(), domain.getTotalSize()
+    out = torch.zeros(D * D * N, dtype=domain.getDtype())
+    _chk(L, L.oracle_spatial_velocity_gradients(_desc(domain), ctypes.c_void_p(out.data_ptr())), "velocity gradients")
+    return out.view(D, D, N)
+
+
+def SGSViscositySmagorinsky(domain, coefficient):
+    """The Smagorinsky viscosity of every cell as one global vector [N]."""
+    L = lib(_bits(domain))
+    out = torch.zeros(domain.getTotalSize(), dtype=domain.getDtype())
+    _chk(L, L.oracle_sgs_smagorinsky(_desc(domain), float(coefficient), ctypes.c_void_p(out.data_ptr())), "sgs")
     return out
 
 

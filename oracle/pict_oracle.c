@@ -1095,6 +1095,115 @@ int oracle_compute_velocity_divergence(const pict_domain_desc *d, real *out) {
     return PICT_OK;
 }
 
+/* getBlockDataGradient :2982-3025 for the velocity: per axis (value of the upper neighbour - value of the
+ * lower neighbour) / distance in cells, the distance 2 less 0.5 for a Dirichlet face (boundary value half a
+ * cell away) or 1 for a Neumann face (the cell's own value); a connected face gives the neighbour block's
+ * cell with the same component index (computeConnectedPos :327-347 leaves w as it is); a periodic face wraps.
+ * With a transform, out_j = sum_i g_i Minv_ij (matmul(vector, matrix), transformations.h:397-405). */
+static void velocity_gradient(const Ctx *c, const pict_block_desc *b, I4 pos, int comp, real *out) {
+    int D = c->D;
+    real cg[3];
+    pos.a[3] = comp;
+    real own = ((const real *)b->velocity)[flat4(pos, b->stride)];
+    for (int dim = 0; dim < D; ++dim) {
+        real dist = 2.0, acc = 0.0;
+        for (int up = 0; up < 2; ++up) {
+            int face = 2 * dim + up;
+            real v;
+            if (at_bound(pos, face, b) && prescribed(face, b)) {
+                const pict_boundary_desc *fb = &b->bounds[face];
+                if (fb->velocity_bc == PICT_BC_DIRICHLET) {
+                    I4 bp = pos;
+                    bp.a[dim] = 0;
+                    v = fixed_vel(c, fb, bp, comp);
+                    dist -= 0.5;
+                } else {
+                    v = own;
+                    dist -= 1.0;
+                }
+            } else {
+                const pict_block_desc *nb = b;
+                I4 t = pos;
+                if (at_bound(pos, face, b) && b->bounds[face].type == PICT_BOUNDARY_CONNECTED) {
+                    nb = blk(c, b->bounds[face].connected_block);
+                    t = connected_pos(c, pos, dim, &b->bounds[face], 0);
+                } else if (at_bound(pos, face, b)) {
+                    t.a[dim] = up ? 0 : b->size[dim] - 1;
+                } else {
+                    t.a[dim] += up ? 1 : -1;
+                }
+                v = ((const real *)nb->velocity)[flat4(t, nb->stride)];
+            }
+            acc += up ? v : -v;
+        }
+        cg[dim] = acc / dist;
+    }
+    if (b->has_transform) {
+        const real *T = cell_T(c, b, pos);
+        for (int j = 0; j < D; ++j) {
+            real s = 0;
+            for (int i = 0; i < D; ++i) s += T_MINV(c, T, i, j) * cg[i];
+            out[j] = s;
+        }
+    } else {
+        for (int j = 0; j < D; ++j) out[j] = cg[j];
+    }
+}
+
+/* k_computeSpatialVelocityGradients :6445-6467: out[(comp*D + i)*N + g] = d u_comp / d x_i */
+int oracle_spatial_velocity_gradients(const pict_domain_desc *d, real *out) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            for (int comp = 0; comp < c.D; ++comp) {
+                real g[3];
+                velocity_gradient(&c, b, pos, comp, g);
+                for (int i = 0; i < c.D; ++i) out[(size_t)(comp * c.D + i) * c.N + b->global_offset + f] = g[i];
+            }
+        }
+    }
+    return PICT_OK;
+}
+
+/* k_SGSviscosityIncompressibleSmagorinsky :6898-6952: coefficient * Delta^2 * |S|, |S| = sqrt(2 sum S_ij^2) over
+ * i <= j with the off-diagonal terms doubled (S = (G + G^T)/2, G_ij = d u_i / d x_j), Delta^2 the largest squared
+ * column norm of M (1 without a transform) */
+int oracle_sgs_smagorinsky(const pict_domain_desc *d, double coefficient, real *out) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    int D = c.D;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            real G[3][3];
+            for (int comp = 0; comp < D; ++comp) velocity_gradient(&c, b, pos, comp, G[comp]);
+            real s2 = 0;
+            for (int i = 0; i < D; ++i)
+                for (int j = i; j < D; ++j) {
+                    real sv = (real)0.5 * (G[i][j] + G[j][i]);
+                    sv *= sv;
+                    if (i != j) sv *= 2;
+                    s2 += sv;
+                }
+            real norm = sqrt(2 * s2);
+            real delta = 1;
+            if (b->has_transform) {
+                const real *T = cell_T(&c, b, pos);
+                delta = 0;
+                for (int i = 0; i < D; ++i) {
+                    real col = 0;
+                    for (int r = 0; r < D; ++r) col += T[r * D + i] * T[r * D + i];
+                    if (col > delta) delta = col;
+                }
+            }
+            out[b->global_offset + f] = (real)coefficient * delta * norm;
+        }
+    }
+    return PICT_OK;
+}
+
 /* PISO_update_velocity :5947-5980 */
 int oracle_correct_velocity(const pict_domain_desc *d, double time_step, int time_step_norm) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;

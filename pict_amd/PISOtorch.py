@@ -2012,6 +2012,47 @@ def ComputeVelocityDivergence(domain):
     return out
 
 
+def _block_tensors(domain, g, channels):
+    """Per-block tensors [1, channels, (z,) y, x] cut from a global component-major vector g [channels, N]."""
+    out = []
+    for b in domain.getBlocks():
+        n = int(b.pressure.numel())
+        out.append(g[:, b.globalOffset:b.globalOffset + n].reshape((1, channels) + tuple(b.pressure.shape[2:]))
+                   .contiguous())
+    return out
+
+
+def ComputeSpatialVelocityGradients(domain):
+    """Spatial gradients of the block velocities (PISOtorch.cpp:530; ComputeSpatialVelocityGradients
+    PISO_multiblock_cuda_kernel.cu:6516-6538): for every block a list of D tensors shaped like its velocity, the
+    d-th holding the gradient of velocity component d (channel i: d u_d / d x_i, in physical coordinates when the
+    block has a transform).  Central differences; a fixed boundary counts at half a cell; across a connection the
+    neighbour's component of the same index is used, as in the reference."""
+    _check_domain(domain)
+    D, N = domain.getSpatialDims(), domain.getTotalSize()
+    g = torch.empty(D * D * N, dtype=domain.getDtype(), device=domain.getDevice())
+    _call(domain, "pict_compute_spatial_velocity_gradients", ctypes.c_void_p(g.data_ptr()))
+    g = g.view(D, D, N)
+    per_comp = [_block_tensors(domain, g[d], D) for d in range(D)]
+    return [[per_comp[d][bi] for d in range(D)] for bi in range(domain.getNumBlocks())]
+
+
+def SGSviscosityIncompressibleSmagorinsky(domain, coefficient):
+    """Additive viscosities of a Smagorinsky SGS model from the velocity field (PISOtorch.cpp:540-541;
+    PISO_multiblock_cuda_kernel.cu:6898-7017): a list of one tensor per block shaped like its pressure, holding
+    coefficient * Delta^2 * |S| with |S| = sqrt(2 S_ij S_ij) of the strain rate and Delta^2 the largest squared
+    column norm of the block's transform M (1 without one).  `coefficient`: host tensor [1]."""
+    _check_domain(domain)
+    _check(isinstance(coefficient, torch.Tensor) and coefficient.device.type == "cpu",
+           "coefficient must be a host tensor")
+    _check(coefficient.dim() == 1 and coefficient.size(0) == 1, "coefficient tensor must have shape [1]")
+    N = domain.getTotalSize()
+    g = torch.empty(N, dtype=domain.getDtype(), device=domain.getDevice())
+    _call(domain, "pict_sgs_viscosity_smagorinsky", ctypes.c_double(float(coefficient[0])),
+          ctypes.c_void_p(g.data_ptr()))
+    return _block_tensors(domain, g.view(1, N), 1)
+
+
 def CopyVelocityResultToBlocks(domain):
     _check_domain(domain)
     _call(domain, "pict_copy_velocity_result_to_blocks")

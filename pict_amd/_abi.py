@@ -231,6 +231,8 @@ SIGNATURES = {
     "pict_setup_pressure_correction": (c_int32, [_P, c_double, c_int32, c_int32, c_int32, _P]),
     "pict_correct_velocity": (c_int32, [_P, c_double, c_int32, c_int32, _P]),
     "pict_compute_velocity_divergence": (c_int32, [_P, _P, _P]),
+    "pict_compute_spatial_velocity_gradients": (c_int32, [_P, _P, _P]),
+    "pict_sgs_viscosity_smagorinsky": (c_int32, [_P, c_double, _P, _P]),
     "pict_copy_velocity_result_to_blocks": (c_int32, [_P, _P]),
     "pict_setup_advection_matrix_scalar": (c_int32, [_P, c_double, c_int32, c_int32, _P]),
     "pict_setup_advection_scalar": (c_int32, [_P, c_double, c_int32, _P]),
