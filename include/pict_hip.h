@@ -218,7 +218,10 @@ pict_status pict_setup_pressure_rhs_div(const pict_atlas *atlas, double time_ste
 pict_status pict_setup_pressure_correction(const pict_atlas *atlas, double time_step,
                                            int32_t non_ortho_flags, int32_t use_face_transform,
                                            int32_t time_step_norm, void *stream);
-/* CorrectVelocity version 1 (PISOtorch.cpp:525-527; kernel :5947-5980). */
+/* CorrectVelocity (PISOtorch.cpp:525-527; version switch :6163-6203): version 4 the orthogonal flux form
+   (kernel :6052-6160), 5 and 6 the finite-volume pressure gradient with cell fluxes / face metrics (kernel
+   :6013-6049, gradient :991-1124; a transformed block needs boundary transforms on its fixed faces, and face
+   transforms for 6, else PICT_ERR_INVALID), every other version finite differences (kernel :5947-5980). */
 pict_status pict_correct_velocity(const pict_atlas *atlas, double time_step, int32_t version,
                                   int32_t time_step_norm, void *stream);
 /* ComputeVelocityDivergence (PISOtorch.cpp:528; kernel :6333-6354): out [N]. */

@@ -45,6 +45,8 @@ def declare(L):
     L.oracle_pressure_rhs_add_non_ortho.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_pressure_rhs_add_non_ortho_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_correct_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.oracle_correct_velocity_fvm.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.oracle_correct_velocity_v4.argtypes = [ctypes.c_void_p, ctypes.c_double]
     L.oracle_compute_velocity_divergence.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.oracle_spatial_velocity_gradients.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.oracle_sgs_smagorinsky.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
@@ -218,8 +220,18 @@ def SetupPressureCorrection(domain, timeStep, nonOrthoMode=0, useFaceTransform=F
 
 
 def CorrectVelocity(domain, timeStep, version=0, timeStepNorm=False):
+    """The version switch of _CorrectVelocity (PISO_multiblock_cuda_kernel.cu:6163-6203): 4 the orthogonal flux
+    form, 5 and 6 the finite-volume pressure gradient (cell fluxes / face metrics), every other version the
+    finite-difference corrector."""
     L = lib(_bits(domain))
-    _chk(L, L.oracle_correct_velocity(_desc(domain), _dt(timeStep), int(timeStepNorm)), "correct velocity")
+    version = int(version)
+    if version == 4:
+        st = L.oracle_correct_velocity_v4(_desc(domain), _dt(timeStep))
+    elif version in (5, 6):
+        st = L.oracle_correct_velocity_fvm(_desc(domain), _dt(timeStep), 0 if version == 5 else 3, int(timeStepNorm))
+    else:
+        st = L.oracle_correct_velocity(_desc(domain), _dt(timeStep), int(timeStepNorm))
+    _chk(L, st, "correct velocity")
 
 
 def ComputeVelocityDivergence(domain):
@@ -456,12 +468,15 @@ def advect_scalar_step(domain, dt, flags, non_orthogonal, tol, max_iter, infos=N
 
 
 def piso_step(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, max_iter=5000,
-              normalize_pressure=True, infos=None, non_orthogonal=False, non_ortho_flags=25, advect_scalar=True):
+              normalize_pressure=True, infos=None, non_orthogonal=False, non_ortho_flags=25, advect_scalar=True,
+              corrector_version=1):
     """One PISO step of the orthogonal driver branch on a CPU twin, with the oracle's solvers (non_orthogonal: the
-    driver's non-orthogonal branch); passive scalars are advected first when the domain has them."""
+    driver's non-orthogonal branch); passive scalars are advected first when the domain has them; the velocity
+    corrector of `corrector_version` (1 finite differences, 5 / 6 finite volumes with cell / face metrics)."""
     if non_orthogonal:
         return piso_step_non_ortho(domain, time_step, corrector_steps, advection_tol, pressure_tol, max_iter,
-                                   normalize_pressure, infos, non_ortho_flags, advect_scalar=advect_scalar)
+                                   normalize_pressure, infos, non_ortho_flags, advect_scalar=advect_scalar,
+                                   corrector_version=corrector_version)
     dt = torch.tensor([time_step], dtype=domain.getDtype())
     reuse = False
     if advect_scalar and domain.hasPassiveScalar():
@@ -484,14 +499,14 @@ def piso_step(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure
             p = p - torch.mean(p)
         domain.setPressureResult(p.contiguous())
         CopyPressureResultToBlocks(domain)
-        CorrectVelocity(domain, dt, 1)
+        CorrectVelocity(domain, dt, corrector_version)
     CopyVelocityResultToBlocks(domain)
     domain._set_changed(False)
 
 
 def piso_step_non_ortho(domain, time_step, corrector_steps=2, advection_tol=1e-6, pressure_tol=1e-6, max_iter=5000,
                         normalize_pressure=True, infos=None, flags=25, advect_non_ortho_steps=1,
-                        pressure_non_ortho_steps=1, advect_scalar=True):
+                        pressure_non_ortho_steps=1, advect_scalar=True, corrector_version=1):
     """One PISO step of the driver's non-orthogonal branch (PISOtorch_simulation.py:1085-1300, no result reuse):
     passive scalars first (when the domain has them), then the explicit non-orthogonal terms of the advection RHS
     read the velocity result (the block velocities copied in first), those of the pressure RHS the pressure result
@@ -526,7 +541,7 @@ def piso_step_non_ortho(domain, time_step, corrector_steps=2, advection_tol=1e-6
                 p = p - torch.mean(p)
             domain.setPressureResult(p.contiguous())
         CopyPressureResultToBlocks(domain)
-        CorrectVelocity(domain, dt, 1)
+        CorrectVelocity(domain, dt, corrector_version)
     CopyVelocityResultToBlocks(domain)
     domain._set_changed(False)
 

@@ -1230,6 +1230,184 @@ int oracle_correct_velocity(const pict_domain_desc *d, double time_step, int tim
     return PICT_OK;
 }
 
+/* getFaceTransformPtr :874-889: the transform of one face of a cell, from the block's face transforms laid out
+   [D][(z+1)][(y+1)][x+1] (one more vertex along every spatial axis; the face of `face` sits at the cell's index,
+   one further along its axis for an upper face) */
+static const real *face_T(const Ctx *c, const pict_block_desc *b, I4 pos, int face) {
+    int dim = face >> 1, fs[3];
+    pos.a[dim] += face & 1;
+    for (int i = 0; i < 3; ++i) fs[i] = b->size[i] + (i < c->D ? 1 : 0);
+    size_t idx = (size_t)pos.a[0] + (size_t)fs[0] * ((size_t)pos.a[1] + (size_t)fs[1] * ((size_t)pos.a[2] + (size_t)fs[2] * dim));
+    return (const real *)b->face_transform + idx * c->TS;
+}
+
+/* getPressureGradientFVM :991-1124, modes 0 (the mean of the two cells' fluxes det * Minv[dim] * p) and 3 (face
+   metrics times the mean pressure).  A fixed face extrapolates the pressure from the cell and its inner neighbour:
+   half a cell with the boundary's transform (mode 0), a whole cell then averaged with the cell's (mode 3).  Across a
+   connection the neighbour contributes its own axis' row, negated when both faces are on the same side. */
+static void pressure_gradient_fvm(const Ctx *c, const pict_block_desc *b, I4 pos, int mode, real *g) {
+    if (!b->has_transform) { pressure_gradient(c, b, pos, g); return; }
+    const int D = c->D;
+    pos.a[3] = 0;
+    const real *TP = cell_T(c, b, pos);
+    const real pP = ((const real *)b->pressure)[flat4(pos, b->stride)];
+    real acc[3] = {0, 0, 0};
+    for (int face = 0; face < 2 * D; ++face) {
+        const int dim = face >> 1, up = face & 1;
+        const real sgn = up ? 1 : -1;
+        const real *T;
+        int axn = dim;
+        real tsign = 1, p;
+        I4 t = pos;
+        if (at_bound(pos, face, b)) {
+            const pict_boundary_desc *fb = &b->bounds[face];
+            if (fb->type == PICT_BOUNDARY_FIXED) {
+                t.a[dim] = 0;
+                T = bound_T(c, fb, t);
+                t.a[dim] = pos.a[dim] - (up ? 1 : -1);
+                const real pin = ((const real *)b->pressure)[flat4(t, b->stride)];
+                if (mode != 3) {
+                    p = pP + (pP - pin) * (real)0.5;
+                    for (int j = 0; j < D; ++j) acc[j] += T_MINV(c, T, dim, j) * (T_DET(c, T) * p) * sgn;
+                    continue;
+                }
+                p = pP + (pP - pin);
+            } else if (fb->type == PICT_BOUNDARY_CONNECTED) {
+                const pict_block_desc *ob = blk(c, fb->connected_block);
+                t.a[3] = dim;
+                I4 o = connected_pos_ch(c, t, dim, fb);
+                axn = o.a[3];
+                if ((fb->axes[0] & 1) == up) tsign = -1;
+                o.a[3] = 0;
+                T = cell_T(c, ob, o);
+                p = ((const real *)ob->pressure)[flat4(o, ob->stride)];
+            } else if (fb->type == PICT_BOUNDARY_PERIODIC) {
+                t.a[dim] = up ? 0 : b->size[dim] - 1;
+                T = cell_T(c, b, t);
+                p = ((const real *)b->pressure)[flat4(t, b->stride)];
+            } else {
+                continue;
+            }
+        } else {
+            t.a[dim] += up ? 1 : -1;
+            T = cell_T(c, b, t);
+            p = ((const real *)b->pressure)[flat4(t, b->stride)];
+        }
+        if (mode == 3) {
+            const real *F = face_T(c, b, pos, face);
+            for (int j = 0; j < D; ++j) acc[j] += T_MINV(c, F, dim, j) * (T_DET(c, F) * (p + pP) * (real)0.5 * sgn);
+        } else {
+            for (int j = 0; j < D; ++j)
+                acc[j] += (T_MINV(c, TP, dim, j) * (T_DET(c, TP) * pP) + T_MINV(c, T, axn, j) * (T_DET(c, T) * p * tsign))
+                          * ((real)0.5 * sgn);
+        }
+    }
+    const real rdet = (real)1 / T_DET(c, TP);
+    for (int j = 0; j < D; ++j) g[j] = acc[j] * rdet;
+}
+
+/* PISO_update_velocity_PressureFVM :6013-6049 (CorrectVelocity versions 5 and 6, :6180-6191) */
+int oracle_correct_velocity_fvm(const pict_domain_desc *d, double time_step, int mode, int time_step_norm) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    const real dt = (real)time_step;
+    const real *A = (const real *)d->A, *prhs = (const real *)d->pressure_rhs;
+    real *vr = (real *)d->velocity_result;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        if (!b->has_transform) continue;
+        if (mode == 3 && !b->face_transform) {
+            snprintf(g_err, sizeof g_err, "block %d: finite-volume gradient with face metrics needs face transforms", bi);
+            return PICT_ERR_INVALID;
+        }
+        for (int f = 0; f < 2 * c.D; ++f)
+            if (b->bounds[f].type == PICT_BOUNDARY_FIXED && !b->bounds[f].transform) {
+                snprintf(g_err, sizeof g_err, "block %d face %d: fixed boundary without a transform", bi, f);
+                return PICT_ERR_INVALID;
+            }
+    }
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        #pragma omp parallel for schedule(static)
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            int g = b->global_offset + f;
+            real rDiag = 1 / A[g];
+            real pg[3];
+            pressure_gradient_fvm(&c, b, pos, mode, pg);
+            for (int dd = 0; dd < c.D; ++dd) {
+                real u = -rDiag * pg[dd];
+                if (time_step_norm) u *= dt;
+                vr[g + (size_t)c.N * dd] = u + prhs[g + (size_t)c.N * dd];
+            }
+        }
+    }
+    return PICT_OK;
+}
+
+/* getTransformMetricOrthogonal :1181-1192: Minv[w][w] of a cell (1 without a transform) */
+static real metric_orth(const Ctx *c, const pict_block_desc *b, I4 p) {
+    if (!b->has_transform) return 1;
+    const int comp = p.a[3];
+    const real *T = cell_T(c, b, p);
+    return T_MINV(c, T, comp, comp);
+}
+/* PISO_update_velocity_v4_orthogonal :6052-6160 (CorrectVelocity version 4): per component a face flux from the
+   two cells' covariant H/A and pressure difference, a quarter per side, over det*t.  The cell's own metric t is
+   read at channel 0 for every component, as the reference's unflattened index carries w = 0; the neighbours' at
+   the component's channel (remapped across connections).  Prescribed faces add nothing; the time step always
+   scales the pressure term. */
+int oracle_correct_velocity_v4(const pict_domain_desc *d, double time_step) {
+    Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    const real dt = (real)time_step;
+    const real *A = (const real *)d->A, *prhs = (const real *)d->pressure_rhs;
+    real *vr = (real *)d->velocity_result;
+    for (int bi = 0; bi < d->num_blocks; ++bi) {
+        const pict_block_desc *b = blk(&c, bi);
+        #pragma omp parallel for schedule(static)
+        for (int f = 0; f < b->stride[3]; ++f) {
+            I4 pos = unflat(f, b);
+            int g = b->global_offset + f;
+            const real ra = 1 / A[g];
+            const real p = ((const real *)b->pressure)[f];
+            const real det = cell_det(&c, b, pos);
+            const real t = metric_orth(&c, b, pos);
+            const real cc = det * t;
+            for (int dim = 0; dim < c.D; ++dim) {
+                const size_t gc = g + (size_t)c.N * dim;
+                const real hbyA = prhs[gc];
+                real vel = 0;
+                for (int up = 0; up < 2; ++up) {
+                    const int face = 2 * dim + up;
+                    if (at_bound(pos, face, b) && prescribed(face, b)) continue;
+                    const pict_block_desc *nb = b;
+                    I4 q = pos;
+                    q.a[3] = dim;
+                    if (at_bound(pos, face, b) && b->bounds[face].type == PICT_BOUNDARY_CONNECTED) {
+                        nb = blk(&c, b->bounds[face].connected_block);
+                        q = connected_pos_ch(&c, q, dim, &b->bounds[face]);
+                    } else if (at_bound(pos, face, b)) {
+                        q.a[dim] = up ? 0 : b->size[dim] - 1;
+                    } else {
+                        q.a[dim] += up ? 1 : -1;
+                    }
+                    const int gn = global_index(q, nb);
+                    const real hbyAN = prhs[gn + (size_t)c.N * q.a[3]];
+                    const real tN = metric_orth(&c, nb, q);
+                    q.a[3] = 0;
+                    const real detN = cell_det(&c, nb, q);
+                    const real pN = ((const real *)nb->pressure)[flat4(q, nb->stride)];
+                    const real raN = 1 / A[gn];
+                    const real ccN = detN * tN;
+                    const real dp = up ? pN - p : p - pN;
+                    vel += (cc * hbyA + ccN * hbyAN - (cc * t * ra + ccN * tN * raN) * dp * dt) * (real)0.25;
+                }
+                vr[gc] = vel / (det * t);
+            }
+        }
+    }
+    return PICT_OK;
+}
+
 /* Copy*Result*Blocks :6541-6733 (memcpy per block and component) */
 int oracle_copy_velocity_result(const pict_domain_desc *d, int to_blocks) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;

@@ -294,6 +294,177 @@ __device__ __forceinline__ void correct_velocity_cell(const DomainK<R> &dom, int
     }
 }
 
+// getFaceTransformPtr :874-889: a face's transform in the block's face transforms, laid out [D][(z+1)][(y+1)][x+1]
+// (one more vertex along every spatial axis; an upper face sits one index further along its axis)
+template <typename R, int D>
+__device__ __forceinline__ const R *face_T(const pict_block_desc *b, I4 pos, int face) {
+    const int dim = face >> 1;
+    pos.a[dim] += face & 1;
+    const size_t fx = b->size[0] + 1, fy = b->size[1] + (D > 1 ? 1 : 0), fz = b->size[2] + (D > 2 ? 1 : 0);
+    const size_t idx = (size_t)pos.a[0] + fx * ((size_t)pos.a[1] + fy * ((size_t)pos.a[2] + fz * dim));
+    return reinterpret_cast<const R *>(b->face_transform) + idx * TS<D>::n;
+}
+
+// getPressureGradientFVM :991-1124 (MODE 0: mean of the two cells' fluxes det * Minv[dim] * p; MODE 3: the face's
+// metrics times the mean pressure).  A fixed face extrapolates the pressure from the cell and its inner neighbour:
+// half a cell, with the boundary's transform, for MODE 0; a whole cell, then averaged with the cell, for MODE 3.
+// Across a connection the neighbour contributes its own axis' row, negated when both faces are on the same side.
+// Without a transform: the finite-difference gradient.
+template <typename R, int D, int MODE>
+__device__ __forceinline__ void pressure_gradient_fvm(const DomainK<R> &dom, const pict_block_desc *b, I4 pos,
+                                                      R *g) {
+    if (!b->has_transform) {
+        pressure_gradient<R, D>(dom, b, pos, g);
+        return;
+    }
+    pos.a[3] = 0;
+    const R *TP = cell_T<R, D>(b, pos);
+    const R pP = reinterpret_cast<const R *>(b->pressure)[flat4(pos, b->stride)];
+    const R detP = Tdet<R, D>(TP);
+    R acc[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc[j] = 0;
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        const int dim = face >> 1, up = face & 1;
+        const R sgn = up ? R(1) : R(-1);
+        const R *T;
+        int axn = dim;
+        R tsign = 1, p;
+        I4 t = pos;
+        if (at_bound(pos, face, b)) {
+            const pict_boundary_desc *fb = &b->bounds[face];
+            if (fb->type == PICT_BOUNDARY_FIXED) {
+                t.a[dim] = 0;
+                T = bound_T<R, D>(fb, t);
+                t.a[dim] = pos.a[dim] - (up ? 1 : -1);
+                const R pin = reinterpret_cast<const R *>(b->pressure)[flat4(t, b->stride)];
+                if (MODE != 3) {
+                    p = pP + (pP - pin) * R(0.5);
+                    const R s = Tdet<R, D>(T) * p * sgn;
+#pragma unroll
+                    for (int j = 0; j < D; ++j) acc[j] += Tminv<R, D>(T, dim, j) * s;
+                    continue;
+                }
+                p = pP + (pP - pin);
+            } else if (fb->type == PICT_BOUNDARY_CONNECTED) {
+                const pict_block_desc *ob = dom.blocks + fb->connected_block;
+                t.a[3] = dim;
+                I4 o = connected_pos_ch<D>(dom.blocks, t, dim, fb);
+                axn = o.a[3];
+                if ((fb->axes[0] & 1) == up) tsign = -1;
+                o.a[3] = 0;
+                T = cell_T<R, D>(ob, o);
+                p = reinterpret_cast<const R *>(ob->pressure)[flat4(o, ob->stride)];
+            } else {  // periodic
+                t.a[dim] = up ? 0 : b->size[dim] - 1;
+                T = cell_T<R, D>(b, t);
+                p = reinterpret_cast<const R *>(b->pressure)[flat4(t, b->stride)];
+            }
+        } else {
+            t.a[dim] += up ? 1 : -1;
+            T = cell_T<R, D>(b, t);
+            p = reinterpret_cast<const R *>(b->pressure)[flat4(t, b->stride)];
+        }
+        if (MODE == 3) {
+            const R *F = face_T<R, D>(b, pos, face);
+            const R s = Tdet<R, D>(F) * (p + pP) * R(0.5) * sgn;
+#pragma unroll
+            for (int j = 0; j < D; ++j) acc[j] += Tminv<R, D>(F, dim, j) * s;
+        } else {
+            const R sP = detP * pP, sN = Tdet<R, D>(T) * p * tsign, h = R(0.5) * sgn;
+#pragma unroll
+            for (int j = 0; j < D; ++j) acc[j] += (Tminv<R, D>(TP, dim, j) * sP + Tminv<R, D>(T, axn, j) * sN) * h;
+        }
+    }
+    const R rdet = R(1) / detP;
+#pragma unroll
+    for (int j = 0; j < D; ++j) g[j] = acc[j] * rdet;
+}
+
+// PISO_update_velocity_PressureFVM :6013-6049 (CorrectVelocity versions 5 and 6)
+template <typename R, int D, int MODE>
+__device__ __forceinline__ void correct_velocity_fvm_cell(const DomainK<R> &dom, int tile, R dt, int tsn) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R rDiag = R(1) / dom.A[g];
+    R pg[D];
+    pressure_gradient_fvm<R, D, MODE>(dom, b, pos, pg);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        R u = -rDiag * pg[d];
+        if (tsn) u *= dt;
+        dom.vres[g + (size_t)dom.N * d] = u + dom.prhs[g + (size_t)dom.N * d];
+    }
+}
+
+// getTransformMetricOrthogonal :1181-1192: Minv[w][w] of a cell (1 without a transform)
+template <typename R, int D>
+__device__ __forceinline__ R metric_orth(const pict_block_desc *b, const I4 &p) {
+    if (!b->has_transform) return R(1);
+    return Tminv<R, D>(cell_T<R, D>(b, p), p.a[3], p.a[3]);
+}
+template <typename R, int D>
+__device__ __forceinline__ R det_or_one(const pict_block_desc *b, const I4 &p) {
+    return b->has_transform ? Tdet<R, D>(cell_T<R, D>(b, p)) : R(1);
+}
+
+// PISO_update_velocity_v4_orthogonal :6052-6160 (CorrectVelocity version 4): per component a face flux from the
+// two cells' covariant H/A and pressure difference, a quarter per side, over det*t.  The cell's own metric t is read
+// at channel 0 for every component (the reference's unflattened cell index carries w = 0), the neighbours' at the
+// component's channel, remapped across connections.  Prescribed faces add nothing; the time step always scales the
+// pressure term.
+template <typename R, int D>
+__device__ __forceinline__ void correct_velocity_v4_cell(const DomainK<R> &dom, int tile, R dt) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    const R ra = R(1) / dom.A[g];
+    const R p = reinterpret_cast<const R *>(b->pressure)[f];
+    const R det = det_or_one<R, D>(b, pos);
+    const R t = metric_orth<R, D>(b, pos);
+    const R cc = det * t;
+#pragma unroll
+    for (int dim = 0; dim < D; ++dim) {
+        const size_t gc = g + (size_t)dom.N * dim;
+        const R hbyA = dom.prhs[gc];
+        R vel = 0;
+#pragma unroll
+        for (int up = 0; up < 2; ++up) {
+            const int face = 2 * dim + up;
+            const bool ab = at_bound(pos, face, b);
+            if (ab && prescribed(face, b)) continue;
+            const pict_block_desc *nb = b;
+            I4 q = pos;
+            q.a[3] = dim;
+            if (ab && b->bounds[face].type == PICT_BOUNDARY_CONNECTED) {
+                nb = dom.blocks + b->bounds[face].connected_block;
+                q = connected_pos_ch<D>(dom.blocks, q, dim, &b->bounds[face]);
+            } else if (ab) {
+                q.a[dim] = up ? 0 : b->size[dim] - 1;
+            } else {
+                q.a[dim] += up ? 1 : -1;
+            }
+            const int gn = global_index(q, nb);
+            const R hbyAN = dom.prhs[gn + (size_t)dom.N * q.a[3]];
+            const R tN = metric_orth<R, D>(nb, q);
+            q.a[3] = 0;
+            const R detN = det_or_one<R, D>(nb, q);
+            const R pN = reinterpret_cast<const R *>(nb->pressure)[flat4(q, nb->stride)];
+            const R raN = R(1) / dom.A[gn];
+            const R ccN = detN * tN;
+            const R dp = up ? pN - p : p - pN;
+            vel += (cc * hbyA + ccN * hbyAN - (cc * t * ra + ccN * tN * raN) * dp * dt) * R(0.25);
+        }
+        dom.vres[gc] = vel / (det * t);
+    }
+}
+
 // Copy*Result*Blocks :6541-6733 and Copy*ResultGrad*Blocks :6819-6915 -- one
 // launch instead of a memcpy per block and component.  `field` selects the
 // block tensor: 0 velocity, 1 pressure, 2 velocity_grad, 3 pressure_grad,

@@ -165,6 +165,16 @@ __global__ __launch_bounds__(kCellsPerTile) void k_correct_velocity(DomainK<R> d
     correct_velocity_cell<R, D>(dom, (int)blockIdx.x, dt, tsn);
 }
 
+template <typename R, int D, int MODE>
+__global__ __launch_bounds__(kCellsPerTile) void k_correct_velocity_fvm(DomainK<R> dom, R dt, int tsn) {
+    correct_velocity_fvm_cell<R, D, MODE>(dom, (int)blockIdx.x, dt, tsn);
+}
+
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_correct_velocity_v4(DomainK<R> dom, R dt) {
+    correct_velocity_v4_cell<R, D>(dom, (int)blockIdx.x, dt);
+}
+
 template <typename R>
 __global__ __launch_bounds__(kCellsPerTile) void k_copy_result(DomainK<R> dom, R *gv, int field, int to_blocks) {
     copy_result_cell<R>(dom, (int)blockIdx.x, gv, field, to_blocks);
@@ -482,10 +492,33 @@ pict_status pict_setup_pressure_correction(const pict_atlas *a, double dt, int32
 
 pict_status pict_correct_velocity(const pict_atlas *a, double dt, int32_t version, int32_t tsn, void *stream) {
     PICT_CHECK_ARG(a && a->desc.velocity_result && a->desc.pressure_rhs && a->desc.A, "inputs missing");
-    if (!(version == 0 || version == 1)) {
-        set_error("CorrectVelocity version %d is not implemented by libpict_hip (versions 0/1: finite differences)",
-                  version);
-        return PICT_ERR_UNSUPPORTED;
+    // the version switch of _CorrectVelocity :6163-6203: 4 the orthogonal flux form, 5 and 6 the finite-volume
+    // pressure gradient (cell fluxes / face metrics), every other version finite differences
+    if (version == 4) {
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_correct_velocity_v4<R, D>), a->view<R>(), (R)dt));
+        return check_launch();
+    }
+    if (version == 5 || version == 6) {
+        // a transformed block's walls read their boundary transforms (the face-transform slices), and version 6
+        // every face's transform: refuse what the kernel would otherwise read as NULL
+        for (int i = 0; i < a->desc.num_blocks; ++i) {
+            const pict_block_desc &b = a->host_blocks[i];
+            if (!b.has_transform) continue;
+            PICT_CHECK_ARG(version == 5 || (b.has_face_transform && b.face_transform),
+                           "CorrectVelocity version 6: block %d has a transform but no face transforms", i);
+            for (int f = 0; f < 2 * a->desc.num_dims; ++f)
+                PICT_CHECK_ARG(b.bounds[f].type != PICT_BOUNDARY_FIXED || (b.bounds[f].has_transform &&
+                                                                          b.bounds[f].transform),
+                               "CorrectVelocity version %d: block %d face %d is a fixed boundary without a transform "
+                               "(set the block's face transforms)", version, i, f);
+        }
+        if (version == 5)
+            PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_correct_velocity_fvm<R, D, 0>), a->view<R>(), (R)dt,
+                                               (int)tsn));
+        else
+            PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_correct_velocity_fvm<R, D, 3>), a->view<R>(), (R)dt,
+                                               (int)tsn));
+        return check_launch();
     }
     PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_correct_velocity<R, D>), a->view<R>(), (R)dt, (int)tsn));
     return check_launch();

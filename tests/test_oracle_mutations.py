@@ -190,3 +190,53 @@ def test_scalar_pins_pass_on_the_oracle():
 def test_planted_mistake_fails_scalar_pins(name, edits, tmp_path):
     failed = _with_mutant(edits, tmp_path, _scalar_pins)
     assert len(failed) >= 1, (name, failed)
+
+
+# Planted mistakes in the velocity correctors of versions 4, 5 and 6; the pins of test_oracle_correctors.py
+# (exactness for a linear pressure on an affine grid, the rotated cut, the closed forms of version 4) must fail for
+# each.
+CORRECTOR_MUTATIONS = [
+    ("finite-volume wall extrapolation over a whole cell",
+     [("p = pP + (pP - pin) * (real)0.5;", "p = pP + (pP - pin);")]),
+    ("face metrics with a column of Minv for its row",
+     [("acc[j] += T_MINV(c, F, dim, j) * (T_DET(c, F) * (p + pP) * (real)0.5 * sgn);",
+       "acc[j] += T_MINV(c, F, j, dim) * (T_DET(c, F) * (p + pP) * (real)0.5 * sgn);")]),
+    ("connection sign of the neighbour's flux inverted",
+     [("if ((fb->axes[0] & 1) == up) tsign = -1;", "if ((fb->axes[0] & 1) != up) tsign = -1;")]),
+    ("version 4 centre metric per component",
+     [("            const real t = metric_orth(&c, b, pos);\n            const real cc = det * t;\n"
+       "            for (int dim = 0; dim < c.D; ++dim) {\n",
+       "            for (int dim = 0; dim < c.D; ++dim) {\n                I4 pw = pos; pw.a[3] = dim;\n"
+       "                const real t = metric_orth(&c, b, pw);\n                const real cc = det * t;\n")]),
+    ("version 4 with both sides at a wall",
+     [("                    if (at_bound(pos, face, b) && prescribed(face, b)) continue;\n"
+       "                    const pict_block_desc *nb = b;\n",
+       "                    const pict_block_desc *nb = b;\n")]),
+]
+
+
+def _corrector_pins():
+    import test_oracle_correctors as C
+    failed = []
+    pins = (("linear5", lambda: C.test_fvm_gradient_of_a_linear_pressure_is_exact_on_an_affine_grid(2, 5)),
+            ("linear6_3d", lambda: C.test_fvm_gradient_of_a_linear_pressure_is_exact_on_an_affine_grid(3, 6)),
+            ("rotated5", lambda: C.test_fvm_on_a_rotated_cut_matches_the_uncut_grid(5)),
+            ("rotated6", lambda: C.test_fvm_on_a_rotated_cut_matches_the_uncut_grid(6)),
+            ("v4_aniso", C.test_v4_centre_metric_on_an_anisotropic_grid),
+            ("v4_walls", C.test_v4_wall_cells_take_only_the_inner_side))
+    for name, pin in pins:
+        try:
+            pin()
+        except (AssertionError, IndexError):
+            failed.append(name)
+    return failed
+
+
+def test_corrector_pins_pass_on_the_oracle():
+    assert _corrector_pins() == []
+
+
+@pytest.mark.parametrize("name,edits", CORRECTOR_MUTATIONS, ids=[m[0] for m in CORRECTOR_MUTATIONS])
+def test_planted_mistake_fails_corrector_pins(name, edits, tmp_path):
+    failed = _with_mutant(edits, tmp_path, _corrector_pins)
+    assert len(failed) >= 1, (name, failed)
