@@ -342,7 +342,10 @@ typedef struct pict_solve_opts {
     int32_t transpose_a;             /* solve A^T x = b */
     int32_t print_residual;
     int32_t return_best_result;      /* CG only */
-    int32_t bicg_with_preconditioner;/* ILU(0): not implemented -> UNSUPPORTED */
+    int32_t bicg_with_preconditioner;/* BiCGStab with the ILU(0) preconditioner M = L U
+                                        (bicgstab_solver_kernel.cu:139-365): pict_solve_linear and
+                                        pict_solve_refined only (host analysis, synchronous); the
+                                        enqueue-only entry points return UNSUPPORTED */
     int32_t zero_rhs_shortcut;       /* if every right-hand side is exactly 0: x = 0,
                                         no iterations (PISOtorch_diff.py:251, :292-293) */
     double  tolerance;

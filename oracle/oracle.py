@@ -58,6 +58,7 @@ def declare(L):
     L.oracle_solve_linear.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                       ctypes.c_void_p, ctypes.c_void_p]
     L.oracle_sparse_outer_product.argtypes = [ctypes.c_void_p] * 4
+    L.oracle_ilu0.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     L.oracle_coords_to_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     L.oracle_coords_to_face_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                    ctypes.c_void_p]
@@ -364,8 +365,9 @@ class SolveInfo:
 
 
 def SolveLinear(A, RHS, x, maxIterations=1000, tolerance=1e-8, convergenceCriterion=1, useBiCG=False,
-                residualResetSteps=0, transposeA=False, returnBestResult=False):
-    """CPU restatement of SolveLinear on host tensors (A: PISOtorch.CSRmatrix on CPU)."""
+                residualResetSteps=0, transposeA=False, returnBestResult=False, withPreconditioner=False):
+    """CPU restatement of SolveLinear on host tensors (A: PISOtorch.CSRmatrix on CPU); withPreconditioner: the
+    ILU(0)-preconditioned BiCGStab."""
     from pict_amd import _abi
     bits = 64 if A.value.dtype == torch.float64 else 32
     L = lib(bits)
@@ -377,13 +379,23 @@ def SolveLinear(A, RHS, x, maxIterations=1000, tolerance=1e-8, convergenceCriter
     o.residual_reset_steps = int(residualResetSteps)
     o.transpose_a = int(transposeA)
     o.return_best_result = int(returnBestResult)
-    o.bicg_with_preconditioner = 0
+    o.bicg_with_preconditioner = int(bool(withPreconditioner))
     o.tolerance = float(tolerance)
     nb = RHS.size(0) // A.getRows()
     infos = (_abi.SolveInfo * nb)()
     _chk(L, L.oracle_solve_linear(ctypes.byref(desc), ctypes.c_void_p(RHS.data_ptr()), ctypes.c_void_p(x.data_ptr()),
                                   nb, ctypes.byref(o), infos), "solve")
     return [SolveInfo(i) for i in infos]
+
+
+def ILU0(A, transpose=False):
+    """The ILU(0) factors of A (of A^T with transpose) on its effective pattern: L strictly below the diagonal (unit
+    diagonal implied), U on and above, in the CSR order of A (of A^T's rows with ascending columns)."""
+    bits = 64 if A.value.dtype == torch.float64 else 32
+    L = lib(bits)
+    out = torch.empty_like(A.value)
+    _chk(L, L.oracle_ilu0(ctypes.byref(A._desc()), int(bool(transpose)), ctypes.c_void_p(out.data_ptr())), "ilu0")
+    return out
 
 
 def SparseOuterProduct(a, b, pattern):

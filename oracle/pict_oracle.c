@@ -2373,10 +2373,146 @@ static void bicg_one(const Mat *m, const real *f, real *x, const pict_solve_opts
     }
 }
 
+/* The matrix the preconditioned solve works on: A, or A^T built explicitly (rows of A^T with ascending columns). */
+typedef struct { int n; int *row, *col; real *val; } Csr;
+
+static Csr csr_effective(const pict_csr_desc *A, int transpose) {
+    Csr B; int n = A->n, nnz = A->row[n];
+    const real *av = (const real *)A->value;
+    B.n = n;
+    B.row = (int *)calloc((size_t)n + 1, sizeof(int));
+    B.col = (int *)malloc(sizeof(int) * (size_t)(nnz ? nnz : 1));
+    B.val = (real *)malloc(sizeof(real) * (size_t)(nnz ? nnz : 1));
+    if (!transpose) {
+        memcpy(B.row, A->row, sizeof(int) * ((size_t)n + 1));
+        memcpy(B.col, A->index, sizeof(int) * (size_t)nnz);
+        memcpy(B.val, av, sizeof(real) * (size_t)nnz);
+        return B;
+    }
+    for (int k = 0; k < nnz; ++k) B.row[A->index[k] + 1]++;
+    for (int i = 0; i < n; ++i) B.row[i + 1] += B.row[i];
+    int *fill = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+    memcpy(fill, B.row, sizeof(int) * ((size_t)n + 1));
+    for (int i = 0; i < n; ++i)  /* rows of A in order: each row of A^T gets its columns ascending */
+        for (int k = A->row[i]; k < A->row[i + 1]; ++k) {
+            int d = fill[A->index[k]]++;
+            B.col[d] = i;
+            B.val[d] = av[k];
+        }
+    free(fill);
+    return B;
+}
+
+/* ILU(0) (csrilu02, bicgstab_solver_kernel.cu:201-209 with the factors' roles of :73-77): unit-lower L and upper U
+   on the matrix's own pattern with (LU)_ij = a_ij on that pattern, row by row (IKJ: each earlier row k of the
+   lower part scales a_ik by 1/u_kk and subtracts l_ik * u_kj from the row's later entries that row k has too).
+   lu holds L below and U on and above the diagonal; diag[i] is the position of (i,i).  Returns the first row
+   without a diagonal entry, or -1. */
+static int ilu0(const Csr *B, real *lu, int *diag) {
+    for (int i = 0; i < B->n; ++i) {
+        diag[i] = -1;
+        for (int k = B->row[i]; k < B->row[i + 1]; ++k) if (B->col[k] == i) diag[i] = k;
+        if (diag[i] < 0) return i;
+    }
+    memcpy(lu, B->val, sizeof(real) * (size_t)B->row[B->n]);
+    for (int i = 0; i < B->n; ++i)
+        for (int kk = B->row[i]; kk < diag[i]; ++kk) {
+            const int k = B->col[kk];
+            lu[kk] /= lu[diag[k]];
+            int q = diag[k] + 1;
+            for (int jj = kk + 1; jj < B->row[i + 1]; ++jj) {
+                const int j = B->col[jj];
+                while (q < B->row[k + 1] && B->col[q] < j) ++q;
+                if (q < B->row[k + 1] && B->col[q] == j) lu[jj] -= lu[kk] * lu[q];
+            }
+        }
+    return -1;
+}
+
+/* out = (LU)^-1 in: forward substitution with the unit-lower L into tmp, back substitution with U. */
+static void ilu_apply(const Csr *B, const real *lu, const int *diag, const real *in, real *tmp, real *out) {
+    for (int i = 0; i < B->n; ++i) {
+        real s = in[i];
+        for (int k = B->row[i]; k < diag[i]; ++k) s -= lu[k] * tmp[B->col[k]];
+        tmp[i] = s;
+    }
+    for (int i = B->n - 1; i >= 0; --i) {
+        real s = tmp[i];
+        for (int k = diag[i] + 1; k < B->row[i + 1]; ++k) s -= lu[k] * out[B->col[k]];
+        out[i] = s / lu[diag[i]];
+    }
+}
+
+/* bicgstabSolveGPU with preconditioner, bicgstab_solver_kernel.cu:222-365: right-preconditioned BiCGStab, the
+   preconditioner applied to the search direction (pw = M^-1 p) and to the intermediate residual (s = M^-1 r); the
+   stopping rule and the reported iterations as without preconditioner. */
+static void bicg_ilu_one(const Csr *B, const real *lu, const int *diag, const real *f, real *x,
+                         const pict_solve_opts *o, pict_solve_info *info, real *w) {
+    const int n = B->n;
+    real *r = w, *rw = w + n, *p = w + 2 * n, *pw = w + 3 * n, *v = w + 4 * n, *s = w + 5 * n, *t = w + 6 * n,
+         *tmp = w + 7 * n;
+    const Mat m = {B->row, B->col, B->val, n, 0};
+    info->final_residual = 0; info->used_iterations = -1; info->converged = 0; info->is_finite_residual = 1;
+    double rho = 0, rhop, alpha = 0, omega = 0, beta;
+    spmv(&m, x, r);
+    for (int i = 0; i < n; ++i) { r[i] = f[i] - r[i]; rw[i] = r[i]; p[i] = r[i]; v[i] = 0; }
+    double nrmr0 = criterion(r, n, o->convergence_criterion);
+    if (nrmr0 < o->tolerance) { info->converged = 1; info->used_iterations = -1; info->final_residual = nrmr0; return; }
+    for (int it = 0; it < o->max_iterations; ++it) {
+        rhop = rho;
+        rho = dotd(rw, r, n);
+        if (it > 0) {
+            beta = (rho / rhop) * (alpha / omega);
+            for (int i = 0; i < n; ++i) p[i] = r[i] + (real)beta * (p[i] - (real)omega * v[i]);
+        }
+        ilu_apply(B, lu, diag, p, tmp, pw);
+        spmv(&m, pw, v);
+        alpha = rho / dotd(rw, v, n);
+        for (int i = 0; i < n; ++i) { r[i] -= (real)alpha * v[i]; x[i] += (real)alpha * pw[i]; }
+        double nr = criterion(r, n, o->convergence_criterion);
+        if (!isfinite(nr)) { info->converged = 0; info->used_iterations = it; info->final_residual = nr; info->is_finite_residual = 0; break; }
+        info->used_iterations = it; info->final_residual = nr;
+        if (nr < o->tolerance) { info->converged = 1; break; }
+        ilu_apply(B, lu, diag, r, tmp, s);
+        spmv(&m, s, t);
+        omega = dotd(t, r, n) / dotd(t, t, n);
+        for (int i = 0; i < n; ++i) { x[i] += (real)omega * s[i]; r[i] -= (real)omega * t[i]; }
+        nr = criterion(r, n, o->convergence_criterion);
+        info->final_residual = nr;
+        if (nr < o->tolerance) { info->converged = 1; info->used_iterations = it + 1; break; }
+    }
+}
+
+/* The ILU(0) factors of A (or of A^T with transpose_a) as one array on A's effective pattern: for the checks of
+   the factorization itself. */
+int oracle_ilu0(const pict_csr_desc *A, int transpose, real *lu_out) {
+    if (A->dtype != ORACLE_DTYPE) { snprintf(g_err, sizeof g_err, "oracle: csr dtype mismatch"); return PICT_ERR_INVALID; }
+    Csr B = csr_effective(A, transpose);
+    int *diag = (int *)malloc(sizeof(int) * (size_t)A->n);
+    int miss = ilu0(&B, lu_out, diag);
+    free(diag); free(B.row); free(B.col); free(B.val);
+    if (miss >= 0) { snprintf(g_err, sizeof g_err, "ILU(0): A(%d,%d) is missing", miss, miss); return PICT_ERR_INVALID; }
+    return PICT_OK;
+}
+
 int oracle_solve_linear(const pict_csr_desc *A, const real *rhs, real *x, int nrhs, const pict_solve_opts *o, pict_solve_info *infos) {
     if (A->dtype != ORACLE_DTYPE) { snprintf(g_err, sizeof g_err, "oracle: csr dtype mismatch"); return PICT_ERR_INVALID; }
-    if (o->matrix_rank_deficient || (o->use_bicg && o->bicg_with_preconditioner)) {
+    if (o->matrix_rank_deficient) {
         snprintf(g_err, sizeof g_err, "oracle: option not restated"); return PICT_ERR_UNSUPPORTED;
+    }
+    if (o->use_bicg && o->bicg_with_preconditioner) {
+        const int n = A->n;
+        Csr B = csr_effective(A, o->transpose_a);
+        real *lu = (real *)malloc(sizeof(real) * (size_t)(B.row[n] ? B.row[n] : 1));
+        int *diag = (int *)malloc(sizeof(int) * (size_t)n);
+        real *w = (real *)malloc(sizeof(real) * (size_t)n * 8);
+        int miss = ilu0(&B, lu, diag);
+        if (miss < 0)
+            for (int k = 0; k < nrhs; ++k)
+                bicg_ilu_one(&B, lu, diag, rhs + (size_t)k * n, x + (size_t)k * n, o, infos + k, w);
+        free(w); free(diag); free(lu); free(B.row); free(B.col); free(B.val);
+        if (miss >= 0) { snprintf(g_err, sizeof g_err, "ILU(0): A(%d,%d) is missing", miss, miss); return PICT_ERR_INVALID; }
+        return PICT_OK;
     }
     Mat m = {A->row, A->index, (const real *)A->value, A->n, o->transpose_a};
     real *w = (real *)malloc(sizeof(real) * (size_t)A->n * 6);

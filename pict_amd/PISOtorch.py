@@ -14,8 +14,11 @@ Differences from the reference visible from Python:
   * Only FixedBoundary / ConnectedBoundary / PeriodicBoundary exist (the
     reference's Static/VaryingDirichletBoundary are refused by its own
     MakeClosedBoundary, domain_structs.cpp:1866-1868).
-  * ILU-preconditioned BiCGStab and the FVM velocity correctors raise
-    NotImplementedError (PictUnsupportedError).
+  * useFaceTransform=True in the pressure setup raises NotImplementedError
+    (PictUnsupportedError); the reference's driver never sets it.
+  * The ILU(0)-preconditioned BiCGStab applies M = L U (the factors' roles
+    bicgstab_solver_kernel.cu:73-77 declares); a missing diagonal entry is
+    an error.
   * Per-channel passive-scalar boundary types are limited to 32 channels
     (one type for all channels has no limit).
 """

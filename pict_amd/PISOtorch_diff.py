@@ -79,8 +79,11 @@ def _linear_solve_wrapper(csrMat, rhs, result, maxit_torch, tol_torch, convergen
         infos = PISOtorch._solve(csrMat.toType(dp), rhs.to(dp), result_dp, opts)
         result.copy_(result_dp.to(result.dtype))
     if BiCG_precondition_fallback and use_BiCG and not BiCG_with_preconditioner and not_solved(infos):
-        raise LinsolveError("Not preconditioned BiCG solve failed; the ILU(0)-preconditioned fallback is not "
-                            "implemented by libpict_hip. Solver infos: %s" % infos)
+        _LOG.warning("Not preconditioned BiCG solve failed, trying preconditioned.")
+        _LOG.info("Solver infos:\n%s", [str(i) for i in infos])
+        result.zero_()  # may hold non-finite values after the failed solve
+        opts.bicg_with_preconditioner = 1
+        infos = PISOtorch._solve(csrMat, rhs, result, opts)
     _check_solver_return_infos(infos, transpose, use_BiCG, tol, maxit, return_best_result, is_FWD=is_FWD,
                                debug_out=debug_out)
     return infos

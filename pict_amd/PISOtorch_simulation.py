@@ -612,7 +612,7 @@ class Simulation:
         return for_velocity, solve_ok, False
 
     # --- the PISO step (:1054-1327) ---
-    def _PISO_split_step(self, iterations, time_step=None):
+    def _PISO_split_step(self, iterations, time_step=None, native=True):
         self._check_domain()
         solve_ok = True
         if time_step is None:
@@ -629,7 +629,7 @@ class Simulation:
         non_ortho_flags = self._non_ortho_flags
         B = self._backend
         self.solver_infos = []
-        if self._native_step_eligible():
+        if native and self._native_step_eligible():
             return self._native_steps(iterations, time_step)
 
         for step in range(iterations):
@@ -886,11 +886,16 @@ class Simulation:
                 self.end_step(time_step)
             return True
         self.flush_solver_checks()
+        # a BiCGStab that fails inside the one-call step is retried with the ILU(0) preconditioner by the stage
+        # path (PISOtorch_diff.py:276-286): the step's inputs are kept to redo it there
+        retry = tape is None and self.BiCG_precondition_fallback and (self.advection_use_BiCG or
+                                                                      self.pressure_use_BiCG)
         if timing:
             _abi.check(lib.pict_set_step_timing(1))
         try:
             for _ in range(iterations):
                 domain.UpdateDomainData()
+                kept = [(b.velocity.clone(), b.pressure.clone()) for b in domain.getBlocks()] if retry else None
                 infos = (_abi.SolveInfo * (D + K))()
                 _abi.check(lib.pict_piso_step(domain._atlas, ctypes.byref(o),
                                               ctypes.byref(tape) if tape is not None else None, infos,
@@ -900,7 +905,18 @@ class Simulation:
                     ms = (ctypes.c_double * (1 + K))()
                     lib.pict_last_step_solve_ms(ms, 1 + K)
                     kms = list(ms)
-                self._native_record(infos, o, kms)
+                try:
+                    self._native_record(infos, o, kms)
+                except PISOtorch_diff.LinsolveError:
+                    if kept is None:
+                        raise
+                    _LOG.warning("A solve of the native step failed; redoing the step stage by stage.")
+                    for b, (v, p) in zip(domain.getBlocks(), kept):
+                        b.velocity.copy_(v)
+                        b.pressure.copy_(p)
+                    if not self._PISO_split_step(1, time_step, native=False):
+                        return False
+                    continue
                 self.end_step(time_step)
         finally:
             if timing:

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libpict_hip.so")
 SOURCES = ["pict_stages.hip", "pict_solve.hip", "pict_metrics.hip", "pict_grad.hip", "pict_step.hip", "pict_batch.hip", "pict_matvec.hip",
-           "pict_noalloc.hip", "pict_sgs.hip"]
+           "pict_noalloc.hip", "pict_sgs.hip", "pict_ilu.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PICT_OFFLOAD_ARCH", "gfx950")

@@ -26,6 +26,10 @@ constexpr int kMeanBlocks = 128;
 hipError_t ws_alloc(void **p, size_t bytes, hipStream_t s);
 hipError_t ws_free(void *p, hipStream_t s);
 
+// The ILU(0)-preconditioned BiCGStab (pict_ilu.hip): synchronous, infos written on return.
+pict_status ilu_bicgstab_solve(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
+                               pict_solve_info *infos, hipStream_t s);
+
 // Frees the captured step graphs of an atlas (defined in pict_step.hip).
 void step_graphs_free(struct pict_atlas *a);
 

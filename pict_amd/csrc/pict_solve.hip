@@ -3152,8 +3152,14 @@ pict_status validate_solve(const pict_csr_desc *A, int nrhs, const pict_solve_op
         set_error("matrixRankDeficient is not implemented by libpict_hip");
         return PICT_ERR_UNSUPPORTED;
     }
-    if (opts->use_bicg && opts->bicg_with_preconditioner) {
-        set_error("ILU(0)-preconditioned BiCGStab is not implemented by libpict_hip");
+    return PICT_OK;
+}
+
+// The ILU(0)-preconditioned BiCGStab runs only through the synchronous entry points (its analysis reads the
+// pattern on the host); the enqueue-only paths refuse it.
+pict_status refuse_preconditioned(const pict_solve_opts *o) {
+    if (o->use_bicg && o->bicg_with_preconditioner) {
+        set_error("the ILU(0)-preconditioned BiCGStab runs through pict_solve_linear only");
         return PICT_ERR_UNSUPPORTED;
     }
     return PICT_OK;
@@ -3167,7 +3173,7 @@ size_t solve_state_bytes() { return sizeof(RhsState); }
 
 pict_status solve_workspace_bytes(const pict_csr_desc *A, int nrhs, const pict_solve_opts *o, size_t *bytes) {
     pict_status st = validate_solve(A, nrhs, o);
-    if (st) return st;
+    if (st || (st = refuse_preconditioned(o))) return st;
     Plan pl;
     if (A->dtype == PICT_DTYPE_F32) {
         if ((st = plan_solve<float>(A, nrhs, o, pl))) return st;
@@ -3183,7 +3189,7 @@ pict_status solve_enqueue(const pict_csr_desc *A, const void *rhs, void *x, int 
                           void *state_out, int *err_out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t stream,
                           void *ws, size_t ws_bytes) {
     pict_status st = validate_solve(A, nrhs, o);
-    if (st) return st;
+    if (st || (st = refuse_preconditioned(o))) return st;
     if (A->dtype == PICT_DTYPE_F32)
         return enqueue_typed<float>(A, rhs, x, nrhs, o, (RhsState *)state_out, err_out, ev0, ev1, ws, ws_bytes,
                                     stream);
@@ -3588,6 +3594,7 @@ namespace {
 // Synchronous solve: queue it, wait, report the states as pict_solve_info.
 pict_status solve_sync(const pict_csr_desc *A, const void *rhs, void *x, int nrhs, const pict_solve_opts *o,
                        pict_solve_info *infos, hipStream_t stream) {
+    if (o->use_bicg && o->bicg_with_preconditioner) return pict::ilu_bicgstab_solve(A, rhs, x, nrhs, o, infos, stream);
     struct Out {
         RhsState st[kMaxRhs];
         int err;
@@ -3640,9 +3647,13 @@ pict_status pict_solver_kernel_name(const pict_csr_desc *A, int32_t nrhs, const 
     if (st) return st;
     Plan pl;
     const bool f32 = A->dtype == PICT_DTYPE_F32;
+    const char *t = f32 ? "float" : "double";
+    if (opts->use_bicg && opts->bicg_with_preconditioner) {
+        snprintf(buf, (size_t)len, "pict::k_bicgstab_ilu<%s>", t);
+        return PICT_OK;
+    }
     st = f32 ? plan_solve<float>(A, nrhs, opts, pl) : plan_solve<double>(A, nrhs, opts, pl);
     if (st) return st;
-    const char *t = f32 ? "float" : "double";
     const int nr = nrhs < kMaxRhs ? nrhs : kMaxRhs;
     if (pl.local)
         snprintf(buf, (size_t)len, "pict::k_cg_local<%s, %d, %d, %s>", t, pl.rpt, pl.nz, pl.full ? "true" : "false");

@@ -240,3 +240,40 @@ def test_corrector_pins_pass_on_the_oracle():
 def test_planted_mistake_fails_corrector_pins(name, edits, tmp_path):
     failed = _with_mutant(edits, tmp_path, _corrector_pins)
     assert len(failed) >= 1, (name, failed)
+
+
+# Planted mistakes in the ILU(0) factorization and the preconditioned iteration; the pins of test_oracle_ilu.py
+# must fail for each.
+ILU_MUTATIONS = [
+    ("ILU(0) without the pivot division", [("            lu[kk] /= lu[diag[k]];\n", "")]),
+    ("ILU(0) without the row update",
+     [("if (q < B->row[k + 1] && B->col[q] == j) lu[jj] -= lu[kk] * lu[q];",
+       "if (q < B->row[k + 1] && B->col[q] == j) lu[jj] -= 0 * lu[q];")]),
+    ("preconditioned step along p instead of M^-1 p",
+     [("for (int i = 0; i < n; ++i) { r[i] -= (real)alpha * v[i]; x[i] += (real)alpha * pw[i]; }",
+       "for (int i = 0; i < n; ++i) { r[i] -= (real)alpha * v[i]; x[i] += (real)alpha * p[i]; }")]),
+]
+
+
+def _ilu_pins():
+    import test_oracle_ilu as I
+    failed = []
+    pins = (("identity_P", lambda: I.test_ilu0_reproduces_the_matrix_on_its_pattern("rotated_connection", "P")),
+            ("tridiagonal", I.test_tridiagonal_ilu_is_exact_and_solves_in_one_half_step),
+            ("criterion", lambda: I.test_preconditioned_solution_meets_the_criterion("cavity", False)))
+    for name, pin in pins:
+        try:
+            pin()
+        except AssertionError:
+            failed.append(name)
+    return failed
+
+
+def test_ilu_pins_pass_on_the_oracle():
+    assert _ilu_pins() == []
+
+
+@pytest.mark.parametrize("name,edits", ILU_MUTATIONS, ids=[m[0] for m in ILU_MUTATIONS])
+def test_planted_mistake_fails_ilu_pins(name, edits, tmp_path):
+    failed = _with_mutant(edits, tmp_path, _ilu_pins)
+    assert len(failed) >= 1, (name, failed)
