@@ -39,9 +39,9 @@ def declare(L):
     for n in ("oracle_setup_advection_matrix",):
         getattr(L, n).argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
     L.oracle_setup_advection_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
-    L.oracle_setup_pressure_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_setup_pressure_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
     L.oracle_setup_pressure_rhs_only.argtypes = [ctypes.c_void_p, ctypes.c_double]
-    L.oracle_setup_pressure_rhs_div.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    L.oracle_setup_pressure_rhs_div.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int]
     L.oracle_pressure_rhs_add_non_ortho.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_pressure_rhs_add_non_ortho_grad.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_correct_velocity.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
@@ -190,15 +190,23 @@ def SetupAdvectionVelocity(domain, timeStep, nonOrthoFlags=0, applyPressureGradi
 
 
 def SetupPressureMatrix(domain, timeStep, nonOrthoMode=0, useFaceTransform=False):
+    """useFaceTransform: the face-transform coefficients of the orthogonal terms (PISO_multiblock_cuda_kernel.cu
+    :4857-4865); with non-orthogonal terms it is not restated (their face-transform interpolation)."""
     L = lib(_bits(domain))
-    _chk(L, L.oracle_setup_pressure_matrix(_desc(domain), int(nonOrthoMode)), "pressure matrix")
+    if useFaceTransform and domain.getSpatialDims() > 1 and int(nonOrthoMode) & (1 | 16):
+        raise NotImplementedError("useFaceTransform with non-orthogonal matrix terms")
+    _chk(L, L.oracle_setup_pressure_matrix(_desc(domain), int(nonOrthoMode), int(bool(useFaceTransform))),
+         "pressure matrix")
 
 
 def SetupPressureRHSdiv(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
     """Divergence of the pressure RHS, then its explicit non-orthogonal terms (PISO_multiblock_cuda_kernel.cu
     :5659-5704)."""
     L = lib(_bits(domain))
-    _chk(L, L.oracle_setup_pressure_rhs_div(_desc(domain), _dt(timeStep), int(timeStepNorm)), "pressure div")
+    if useFaceTransform and domain.getSpatialDims() > 1 and int(nonOrthoMode) & (2 | 8):
+        raise NotImplementedError("useFaceTransform with non-orthogonal right-hand-side terms")
+    _chk(L, L.oracle_setup_pressure_rhs_div(_desc(domain), _dt(timeStep), int(timeStepNorm),
+                                            int(bool(useFaceTransform))), "pressure div")
     _chk(L, L.oracle_pressure_rhs_add_non_ortho(_desc(domain), int(nonOrthoMode)), "pressure non-ortho rhs")
 
 

@@ -103,6 +103,18 @@ static inline const real *bound_T(const Ctx *c, const pict_boundary_desc *fb, I4
 #define T_MINV(c, T, i, j) ((T)[(c)->D * (c)->D + (i) * (c)->D + (j)])
 #define T_DET(c, T) ((T)[2 * (c)->D * (c)->D])
 
+/* getFaceTransformPtr :874-889: the transform of one face of a cell, from the block's face transforms laid out
+   [D][(z+1)][(y+1)][x+1] (one more vertex along every spatial axis; the face of `face` sits at the cell's index,
+   one further along its axis for an upper face) */
+static const real *face_T(const Ctx *c, const pict_block_desc *b, I4 pos, int face) {
+    int dim = face >> 1, fs[3];
+    pos.a[dim] += face & 1;
+    for (int i = 0; i < 3; ++i) fs[i] = b->size[i] + (i < c->D ? 1 : 0);
+    size_t idx = (size_t)pos.a[0] + (size_t)fs[0] * ((size_t)pos.a[1] + (size_t)fs[1] * ((size_t)pos.a[2] + (size_t)fs[2] * dim));
+    return (const real *)b->face_transform + idx * c->TS;
+}
+
+
 /* computeConnectedPos :325-347 */
 static I4 connected_pos(const Ctx *c, I4 pos, int bdim, const pict_boundary_desc *cb, int border) {
     const pict_block_desc *ob = blk(c, cb->connected_block);
@@ -193,6 +205,16 @@ static real laplace_orth(const Ctx *c, const pict_block_desc *b, I4 p) {
     const real *T = cell_T(c, b, p);
     real s = 0;
     for (int j = 0; j < c->D; ++j) s += T_MINV(c, T, p.a[3], j) * T_MINV(c, T, p.a[3], j);
+    return T_DET(c, T) * s;
+}
+/* getLaplaceCoefficientOrthogonalFace :1226-1241: det |Minv[axis]|^2 of the face's own transform (1 without face
+   transforms) */
+static real laplace_orth_face(const Ctx *c, const pict_block_desc *b, I4 p, int face) {
+    if (!b->has_face_transform) return 1;
+    const real *T = face_T(c, b, p, face);
+    const int ax = face >> 1;
+    real s = 0;
+    for (int j = 0; j < c->D; ++j) s += T_MINV(c, T, ax, j) * T_MINV(c, T, ax, j);
     return T_DET(c, T) * s;
 }
 /* getLaplaceCoefficientOrthogonalBoundary :1454-1466 */
@@ -970,7 +992,9 @@ int oracle_setup_advection_velocity(const pict_domain_desc *d, double time_step,
 }
 
 /* PISO_build_pressure_matrix :4797-4963 (no face transform) */
-int oracle_setup_pressure_matrix(const pict_domain_desc *d, int flags) {
+/* use_face: the face-transform coefficient det|Minv[axis]|^2 of the face times the mean of the two 1/A (:4857-4865,
+   orthogonal terms only) */
+int oracle_setup_pressure_matrix(const pict_domain_desc *d, int flags, int use_face) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
     real *Pv = (real *)d->P_value; const real *A = (const real *)d->A;
     for (int bi = 0; bi < d->num_blocks; ++bi) {
@@ -994,6 +1018,7 @@ int oracle_setup_pressure_matrix(const pict_domain_desc *d, int flags) {
                 int gn = global_index(t, blk(&c, nb));
                 real raN = (real)1.0 / A[gn];
                 real coef = (real)0.5 * (alphaP[dim] * raP + alphaN * raN);
+                if (use_face) coef = laplace_orth_face(&c, b, pos, face) * ((real)0.5 * (raP + raN));
                 rv[0] -= coef; rv[face + 1] += coef; idx[face + 1] = gn;
                 if (c.D > 1 && (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX))) {
                     real ai[12];
@@ -1042,16 +1067,81 @@ int oracle_setup_pressure_rhs_only(const pict_domain_desc *d, double time_step) 
     return PICT_OK;
 }
 
-/* k_computePressureRHSdivergenceFromFlux :5374-5419 */
-int oracle_setup_pressure_rhs_div(const pict_domain_desc *d, double time_step, int time_step_norm) {
+/* slerp2Halfpoint :937-971 with eps 1e-5: the direction of the mean of the two vectors, each scaled by the first
+   one's inverse length as the reference does (the direction is that of v1 + v2; the smallness test sees the scale),
+   with the mean of their lengths; the plain mean when a vector or the direction is too short */
+static void slerp2_half(const real *v1, const real *v2, int D, real *out) {
+    const real eps = (real)1e-5;
+    real s1 = 0, s2 = 0;
+    for (int k = 0; k < D; ++k) { s1 += v1[k] * v1[k]; s2 += v2[k] * v2[k]; }
+    if (s1 < eps || s2 < eps) { for (int k = 0; k < D; ++k) out[k] = (v1[k] + v2[k]) * (real)0.5; return; }
+    const real r1 = (real)1 / sqrt(s1);
+    real n[3], sn = 0;
+    for (int k = 0; k < D; ++k) { n[k] = (v1[k] * r1 + v2[k] * r1) * (real)0.5; sn += n[k] * n[k]; }
+    if (sn < eps) { for (int k = 0; k < D; ++k) out[k] = (v1[k] + v2[k]) * (real)0.5; return; }
+    const real rn = (real)1 / sqrt(sn);
+    const real m = (real)(((double)sqrt(s1) + (double)sqrt(s2)) * 0.5);
+    for (int k = 0; k < D; ++k) out[k] = n[k] * rn * m;
+}
+
+/* computeFluxesWithFaceTransforms :1703-1785: per face, the neighbour's vector of vg (a fixed face: the boundary
+   velocity) blended with the cell's by slerp2_half (the reference does so at fixed faces too), projected on the face's
+   det * Minv[axis] */
+static void fluxes_face(const Ctx *c, const pict_block_desc *b, I4 pos, real *fl, const real *vg) {
+    const int D = c->D, g = global_index(pos, b);
+    real vc[3];
+    for (int k = 0; k < D; ++k) vc[k] = vg[g + (size_t)c->N * k];
+    for (int face = 0; face < 2 * D; ++face) {
+        const int dim = face >> 1, up = face & 1;
+        real vn[3];
+        I4 t = pos; t.a[3] = dim;
+        if (at_bound(pos, face, b)) {
+            const pict_boundary_desc *fb = &b->bounds[face];
+            if (fb->type == PICT_BOUNDARY_FIXED) {
+                I4 bp = pos; bp.a[dim] = 0; bp.a[3] = 0;
+                for (int k = 0; k < D; ++k) vn[k] = fixed_vel(c, fb, bp, k);
+            } else if (fb->type == PICT_BOUNDARY_CONNECTED) {
+                const pict_block_desc *ob = blk(c, fb->connected_block);
+                I4 o = connected_pos(c, t, dim, fb, 0);
+                const int gn = global_index(o, ob);
+                for (int k = 0; k < D; ++k) vn[k] = vg[gn + (size_t)c->N * k];
+            } else {
+                t.a[dim] = up ? 0 : b->size[dim] - 1;
+                const int gn = global_index(t, b);
+                for (int k = 0; k < D; ++k) vn[k] = vg[gn + (size_t)c->N * k];
+            }
+        } else {
+            t.a[dim] += up ? 1 : -1;
+            const int gn = global_index(t, b);
+            for (int k = 0; k < D; ++k) vn[k] = vg[gn + (size_t)c->N * k];
+        }
+        real v[3];
+        slerp2_half(vc, vn, D, v);
+        const real *F = face_T(c, b, pos, face);
+        real s = 0;
+        for (int j = 0; j < D; ++j) s += T_MINV(c, F, dim, j) * v[j];
+        fl[face] = T_DET(c, F) * s;
+    }
+}
+
+/* k_computePressureRHSdivergenceFromFlux :5374-5419 (use_face: computeFluxesWithFaceTransforms; every block needs
+   face transforms) */
+int oracle_setup_pressure_rhs_div(const pict_domain_desc *d, double time_step, int time_step_norm, int use_face) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
     const real dt = (real)time_step;
+    if (use_face)
+        for (int bi = 0; bi < d->num_blocks; ++bi)
+            if (!d->blocks[bi].has_face_transform || !d->blocks[bi].face_transform) {
+                snprintf(g_err, sizeof g_err, "block %d: fluxes with face transforms need face transforms", bi);
+                return PICT_ERR_INVALID;
+            }
     for (int bi = 0; bi < d->num_blocks; ++bi) {
         const pict_block_desc *b = blk(&c, bi);
         #pragma omp parallel for schedule(static)
         for (int f = 0; f < b->stride[3]; ++f) {
             real fl[6];
-            fluxes_nd(&c, bi, unflat(f, b), fl, (const real *)d->pressure_rhs);
+            if (use_face) fluxes_face(&c, b, unflat(f, b), fl, (const real *)d->pressure_rhs);
+            else fluxes_nd(&c, bi, unflat(f, b), fl, (const real *)d->pressure_rhs);
             real div = 0;
             for (int dd = 0; dd < c.D; ++dd) div += fl[2 * dd + 1] - fl[2 * dd];
             if (time_step_norm) div /= dt;
@@ -1228,17 +1318,6 @@ int oracle_correct_velocity(const pict_domain_desc *d, double time_step, int tim
         }
     }
     return PICT_OK;
-}
-
-/* getFaceTransformPtr :874-889: the transform of one face of a cell, from the block's face transforms laid out
-   [D][(z+1)][(y+1)][x+1] (one more vertex along every spatial axis; the face of `face` sits at the cell's index,
-   one further along its axis for an upper face) */
-static const real *face_T(const Ctx *c, const pict_block_desc *b, I4 pos, int face) {
-    int dim = face >> 1, fs[3];
-    pos.a[dim] += face & 1;
-    for (int i = 0; i < 3; ++i) fs[i] = b->size[i] + (i < c->D ? 1 : 0);
-    size_t idx = (size_t)pos.a[0] + (size_t)fs[0] * ((size_t)pos.a[1] + (size_t)fs[1] * ((size_t)pos.a[2] + (size_t)fs[2] * dim));
-    return (const real *)b->face_transform + idx * c->TS;
 }
 
 /* getPressureGradientFVM :991-1124, modes 0 (the mean of the two cells' fluxes det * Minv[dim] * p) and 3 (face

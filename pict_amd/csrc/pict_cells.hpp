@@ -158,8 +158,9 @@ __device__ __forceinline__ void scalar_rhs_cell(const DomainK<R> &dom, int tile,
     }
 }
 
-// PISO_build_pressure_matrix :4797-4963 (cell metrics; NO: with the non-orthogonal matrix terms of `flags`)
-template <typename R, int D, bool NO = false>
+// PISO_build_pressure_matrix :4797-4963 (cell metrics; NO: with the non-orthogonal matrix terms of `flags`; FACE:
+// the orthogonal coefficient from the face's transform times the mean of the two 1/A, useFaceTransform :4857-4865)
+template <typename R, int D, bool NO = false, bool FACE = false>
 __device__ __forceinline__ void p_matrix_cell(const DomainK<R> &dom, int tile, int flags = 0) {
     const pict_block_desc *b;
     int bi, f;
@@ -190,7 +191,8 @@ __device__ __forceinline__ void p_matrix_cell(const DomainK<R> &dom, int tile, i
         t.a[3] = 0;
         const int gn = global_index(t, nbp);
         const R raN = R(1.0) / dom.A[gn];
-        const R coef = R(0.5) * (alphaP[dim] * raP + alphaN * raN);
+        const R coef = FACE ? laplace_orth_face<R, D>(b, pos, face) * (R(0.5) * (raP + raN))
+                            : R(0.5) * (alphaP[dim] * raP + alphaN * raN);
         rv[0] -= coef;
         rv[face + 1] += coef;
         idx[face + 1] = gn;
@@ -261,6 +263,92 @@ __device__ __forceinline__ void divergence_cell(const DomainK<R> &dom, int tile,
     out[f + b->global_offset] = div;
 }
 
+// slerp2Halfpoint :937-971 with eps 1e-5: the direction of the mean of the two vectors (both scaled by the first one's
+// inverse length, as the reference does: the direction is that of v1 + v2, the smallness test sees the scale) with
+// the mean of their lengths; the plain mean when a vector or the direction is too short
+template <typename R, int D>
+__device__ __forceinline__ void slerp2_half(const R (&v1)[D], const R (&v2)[D], R (&out)[D]) {
+    const R eps = R(1e-5);
+    R s1 = 0, s2 = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        s1 += v1[k] * v1[k];
+        s2 += v2[k] * v2[k];
+    }
+    R n[D], sn = 0;
+    const R r1 = R(1) / sqrt(s1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        n[k] = (v1[k] * r1 + v2[k] * r1) * R(0.5);
+        sn += n[k] * n[k];
+    }
+    if (s1 < eps || s2 < eps || sn < eps) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) out[k] = (v1[k] + v2[k]) * R(0.5);
+        return;
+    }
+    const R rn = R(1) / sqrt(sn);
+    const R m = (R)(((double)sqrt(s1) + (double)sqrt(s2)) * 0.5);
+#pragma unroll
+    for (int k = 0; k < D; ++k) out[k] = n[k] * rn * m;
+}
+
+// computeFluxesWithFaceTransforms :1703-1785: per face, the neighbour's vector of vg (a fixed face: the boundary
+// velocity), blended with the cell's by slerp2_half (at fixed faces too, as in the reference), projected on the
+// face's det * Minv[axis]; the divergence as k_computePressureRHSdivergenceFromFlux :5374-5419
+template <typename R, int D>
+__device__ __forceinline__ void divergence_face_cell(const DomainK<R> &dom, int tile, const R *vg, R *out, R dt,
+                                                     int tsn) {
+    const pict_block_desc *b;
+    int bi, f;
+    if (!tile_cell(dom, tile, b, bi, f)) return;
+    const I4 pos = unflat(f, b);
+    const int g = b->global_offset + f;
+    R vc[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vc[k] = vg[g + (size_t)dom.N * k];
+    R div = 0;
+#pragma unroll
+    for (int face = 0; face < 2 * D; ++face) {
+        const int dim = face >> 1, up = face & 1;
+        R vn[D];
+        I4 t = pos;
+        t.a[3] = dim;
+        int gn = -1;
+        if (at_bound(pos, face, b)) {
+            const pict_boundary_desc *fb = &b->bounds[face];
+            if (fb->type == PICT_BOUNDARY_FIXED) {
+                I4 bp = pos;
+                bp.a[dim] = 0;
+#pragma unroll
+                for (int k = 0; k < D; ++k) vn[k] = fixed_vel<R>(fb, bp, k);
+            } else if (fb->type == PICT_BOUNDARY_CONNECTED) {
+                const I4 o = connected_pos<D>(dom.blocks, t, dim, fb, 0);
+                gn = global_index(o, dom.blocks + fb->connected_block);
+            } else {
+                t.a[dim] = up ? 0 : b->size[dim] - 1;
+                gn = global_index(t, b);
+            }
+        } else {
+            t.a[dim] += up ? 1 : -1;
+            gn = global_index(t, b);
+        }
+        if (gn >= 0)
+#pragma unroll
+            for (int k = 0; k < D; ++k) vn[k] = vg[gn + (size_t)dom.N * k];
+        R v[D];
+        slerp2_half<R, D>(vc, vn, v);
+        const R *F = face_T<R, D>(b, pos, face);
+        R s = 0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) s += Tminv<R, D>(F, dim, j) * v[j];
+        const R flux = Tdet<R, D>(F) * s;
+        div += up ? flux : -flux;
+    }
+    if (tsn) div /= dt;
+    out[g] = div;
+}
+
 // k_pressureRHSaddNonOrthoComponents :5455-5477: pressure_rhs_div += the explicit non-orthogonal part of the
 // pressure Laplacian from the pressure result
 template <typename R, int D>
@@ -292,17 +380,6 @@ __device__ __forceinline__ void correct_velocity_cell(const DomainK<R> &dom, int
         u += dom.prhs[g + (size_t)dom.N * d];
         dom.vres[g + (size_t)dom.N * d] = u;
     }
-}
-
-// getFaceTransformPtr :874-889: a face's transform in the block's face transforms, laid out [D][(z+1)][(y+1)][x+1]
-// (one more vertex along every spatial axis; an upper face sits one index further along its axis)
-template <typename R, int D>
-__device__ __forceinline__ const R *face_T(const pict_block_desc *b, I4 pos, int face) {
-    const int dim = face >> 1;
-    pos.a[dim] += face & 1;
-    const size_t fx = b->size[0] + 1, fy = b->size[1] + (D > 1 ? 1 : 0), fz = b->size[2] + (D > 2 ? 1 : 0);
-    const size_t idx = (size_t)pos.a[0] + fx * ((size_t)pos.a[1] + fy * ((size_t)pos.a[2] + fz * dim));
-    return reinterpret_cast<const R *>(b->face_transform) + idx * TS<D>::n;
 }
 
 // getPressureGradientFVM :991-1124 (MODE 0: mean of the two cells' fluxes det * Minv[dim] * p; MODE 3: the face's

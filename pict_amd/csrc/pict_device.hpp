@@ -87,6 +87,30 @@ __device__ __forceinline__ R Tminv(const R *T, int i, int j) { return T[D * D + 
 template <typename R, int D>
 __device__ __forceinline__ R Tdet(const R *T) { return T[2 * D * D]; }
 
+// getFaceTransformPtr :874-889: a face's transform in the block's face transforms, laid out [D][(z+1)][(y+1)][x+1]
+// (one more vertex along every spatial axis; an upper face sits one index further along its axis)
+template <typename R, int D>
+__device__ __forceinline__ const R *face_T(const pict_block_desc *b, I4 pos, int face) {
+    const int dim = face >> 1;
+    pos.a[dim] += face & 1;
+    const size_t fx = b->size[0] + 1, fy = b->size[1] + (D > 1 ? 1 : 0), fz = b->size[2] + (D > 2 ? 1 : 0);
+    const size_t idx = (size_t)pos.a[0] + fx * ((size_t)pos.a[1] + fy * ((size_t)pos.a[2] + fz * dim));
+    return reinterpret_cast<const R *>(b->face_transform) + idx * TS<D>::n;
+}
+
+// getLaplaceCoefficientOrthogonalFace :1226-1241: det |Minv[axis]|^2 of the face's own transform (1 without face
+// transforms)
+template <typename R, int D>
+__device__ __forceinline__ R laplace_orth_face(const pict_block_desc *b, const I4 &pos, int face) {
+    if (!b->has_face_transform) return R(1);
+    const R *T = face_T<R, D>(b, pos, face);
+    const int ax = face >> 1;
+    R s = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) s += Tminv<R, D>(T, ax, j) * Tminv<R, D>(T, ax, j);
+    return Tdet<R, D>(T) * s;
+}
+
 // computeConnectedPos :325-347
 template <int D>
 __device__ __forceinline__ I4 connected_pos(const pict_block_desc *blocks, I4 pos, int bdim,

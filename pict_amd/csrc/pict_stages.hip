@@ -161,6 +161,17 @@ __global__ __launch_bounds__(kCellsPerTile) void k_divergence(DomainK<R> dom, co
 }
 
 template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_face(DomainK<R> dom) {
+    p_matrix_cell<R, D, false, true>(dom, (int)blockIdx.x, 0);
+}
+
+template <typename R, int D>
+__global__ __launch_bounds__(kCellsPerTile) void k_divergence_face(DomainK<R> dom, const R *vg, R *out, R dt,
+                                                                   int tsn) {
+    divergence_face_cell<R, D>(dom, (int)blockIdx.x, vg, out, dt, tsn);
+}
+
+template <typename R, int D>
 __global__ __launch_bounds__(kCellsPerTile) void k_correct_velocity(DomainK<R> dom, R dt, int tsn) {
     correct_velocity_cell<R, D>(dom, (int)blockIdx.x, dt, tsn);
 }
@@ -441,8 +452,13 @@ pict_status pict_setup_pressure_matrix(const pict_atlas *a, double dt, int32_t f
     (void)dt;
     PICT_CHECK_ARG(a && a->desc.P_value && a->desc.A, "P/A missing");
     if (use_face_transform) {
-        set_error("useFaceTransform is not implemented by libpict_hip");
-        return PICT_ERR_UNSUPPORTED;
+        // useFaceTransform (:4857-4865): the orthogonal coefficients from the faces' transforms
+        if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix)) {
+            set_error("useFaceTransform with non-orthogonal matrix terms is not implemented by libpict_hip");
+            return PICT_ERR_UNSUPPORTED;
+        }
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_face<R, D>), a->view<R>()));
+        return check_launch();
     }
     if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
         PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix<R, D, true>), a->view<R>(), (int)flags));
@@ -455,8 +471,18 @@ pict_status pict_setup_pressure_rhs_div(const pict_atlas *a, double dt, int32_t 
                                         int32_t tsn, void *stream) {
     PICT_CHECK_ARG(a && a->desc.pressure_rhs && a->desc.pressure_rhs_div, "pressure rhs missing");
     if (use_face_transform) {
-        set_error("useFaceTransform is not implemented by libpict_hip");
-        return PICT_ERR_UNSUPPORTED;
+        // computeFluxesWithFaceTransforms :1703-1785 reads every face's transform
+        if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
+            set_error("useFaceTransform with non-orthogonal right-hand-side terms is not implemented by libpict_hip");
+            return PICT_ERR_UNSUPPORTED;
+        }
+        for (int i = 0; i < a->desc.num_blocks; ++i)
+            PICT_CHECK_ARG(a->host_blocks[i].has_face_transform && a->host_blocks[i].face_transform,
+                           "useFaceTransform: block %d has no face transforms", i);
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_divergence_face<R, D>), a->view<R>(),
+                                           (const R *)a->desc.pressure_rhs, (R *)a->desc.pressure_rhs_div, (R)dt,
+                                           (int)tsn));
+        return check_launch();
     }
     PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_divergence<R, D>), a->view<R>(),
                                        (const R *)a->desc.pressure_rhs, (R *)a->desc.pressure_rhs_div, (R)dt,
@@ -473,10 +499,6 @@ pict_status pict_setup_pressure_rhs(const pict_atlas *a, double dt, int32_t flag
                                     int32_t tsn, void *stream) {
     PICT_CHECK_ARG(a && a->desc.pressure_rhs && a->desc.C_value && a->desc.velocity_result, "pressure rhs inputs missing");
     pict_status st;
-    if (use_face_transform) {
-        set_error("useFaceTransform is not implemented by libpict_hip");
-        return PICT_ERR_UNSUPPORTED;
-    }
     PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs<R, D>), a->view<R>(), (R)dt));
     st = check_launch();
     if (st) return st;

@@ -1,0 +1,64 @@
+"""useFaceTransform in the pressure setup (computeFluxesWithFaceTransforms PISO_multiblock_cuda_kernel.cu:1703-1785
+in the pressure-RHS divergence :5374-5419; getLaplaceCoefficientOrthogonalFace :1226-1241 in the pressure matrix
+:4857-4865) on the GPU against the FP64 oracle (tests/test_oracle_face_transform.py pins it), on every scenario with
+vertex coordinates: FP64 1e-11, FP32 1e-5 relative (the divergence, a difference of fluxes, 10x that).  Blocks
+without face transforms: the matrix takes the coefficient 1 as in the reference, the divergence is refused;
+non-orthogonal terms with face transforms are refused."""
+import pytest
+import torch
+
+import scenarios
+from compare import assert_close, tol_for
+from oracle import oracle as O
+from pict_amd import PISOtorch as P
+
+pytestmark = pytest.mark.gpu
+
+CASES = dict(scenarios.ALL, **scenarios.NON_ORTHO, outflow_channel=scenarios.outflow_channel)
+
+
+def _rand(shape, seed, scale=1.0):
+    g = torch.Generator().manual_seed(seed)
+    return (torch.rand(shape, generator=g, dtype=torch.float64) * 2 - 1) * scale
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_face_transform_pressure_setup_matches_the_oracle(name, dtype):
+    print("case", name, dtype, flush=True)
+    g = CASES[name](dtype=dtype, device="cuda")
+    c = O.cpu_twin(g)
+    tol = tol_for(dtype)
+    dt = 0.05
+    tdt = torch.tensor([dt], dtype=dtype)
+    P.SetupAdvectionMatrix(g, tdt, 0)
+    O.SetupAdvectionMatrix(c, dt)
+    P.SetupPressureMatrix(g, tdt, 0, True)
+    O.SetupPressureMatrix(c, dt, 0, True)
+    assert_close(g.P.value, c.P.value, tol, "pressure matrix with face transforms")
+    H = _rand(g.pressureRHS.shape, 23, 0.5)
+    g.setPressureRHS(H.to(dtype=dtype, device="cuda"))
+    c.setPressureRHS(H.to(dtype=dtype).double())
+    g.UpdateDomainData()
+    c.UpdateDomainData()
+    faces = all(b.hasFaceTransform() for b in g.getBlocks())
+    for tsn in (False, True):
+        if faces:
+            P.SetupPressureRHSdiv(g, tdt, 0, True, tsn)
+            O.SetupPressureRHSdiv(c, dt, 0, True, tsn)
+            assert_close(g.pressureRHSdiv, c.pressureRHSdiv, 10 * tol, "divergence with face transforms")
+        else:
+            with pytest.raises(P.PictError):
+                P.SetupPressureRHSdiv(g, tdt, 0, True, tsn)
+            with pytest.raises(RuntimeError):
+                O.SetupPressureRHSdiv(c, dt, 0, True, tsn)
+
+
+def test_face_transform_with_non_orthogonal_terms_is_refused():
+    g = scenarios.NON_ORTHO["sheared_walls"](dtype=torch.float32, device="cuda")
+    tdt = torch.tensor([0.05], dtype=torch.float32)
+    P.SetupAdvectionMatrix(g, tdt, 0)
+    with pytest.raises(P.PictUnsupportedError):
+        P.SetupPressureMatrix(g, tdt, 25, True)
+    with pytest.raises(P.PictUnsupportedError):
+        P.SetupPressureRHSdiv(g, tdt, 25, True)

@@ -1,0 +1,129 @@
+"""useFaceTransform in the pressure setup, in the oracle: the divergence of the pressure RHS from
+computeFluxesWithFaceTransforms (PISO_multiblock_cuda_kernel.cu:1703-1785, used by :5374-5419) and the orthogonal
+pressure-matrix coefficient of getLaplaceCoefficientOrthogonalFace (:1226-1241, :4857-4865), pinned by:
+- a constant vector field on a warped closed grid: the face metrics of a closed cell sum to zero, so every cell's
+  divergence vanishes (the cell-metric divergence does not);
+- u = (c x, 0) on a uniform orthogonal grid away from x = 0: interior cells c hx hy; a cell next to an x wall
+  0.75 c hx hy, because the reference blends the wall's velocity with the cell's (the mean of collinear vectors);
+- the matrix: on an affine grid the face coefficients are the cell coefficients; on a grid stretched along x the
+  coefficient between two cells is hy over the distance of their centres (face area over centre distance).
+"""
+import numpy as np
+import pytest
+import torch
+
+import scenarios
+from oracle import oracle as O
+from pict_amd import PISOtorch as P
+from pict_amd.configs import uniform_coords
+
+from test_oracle_correctors import _warped
+
+
+def _domain(dims):
+    return P.Domain(dims, torch.tensor([0.05], dtype=torch.float64), "face", torch.float64, "cpu",
+                    passiveScalarChannels=0, _allow_cpu=True)
+
+
+def _prep(d):
+    d.PrepareSolve()
+    O.build_csr_pattern(d)
+    return d
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_constant_field_has_no_divergence_with_face_metrics(dims):
+    size = (8, 6, 4)[:dims]
+    shear = (0.3,) if dims == 2 else (0.3, 0.1, -0.2)
+    vel = torch.tensor([0.7, -0.4, 0.25][:dims], dtype=torch.float64)
+    d = _domain(dims)
+    b = scenarios._coords_block(d, _warped(size, shear))
+    b.CloseAllBoundaries()
+    for face in range(2 * dims):
+        b.CloseBoundary(face, velocity=vel[None].clone())
+    _prep(d)
+    N = d.getTotalSize()
+    d.setPressureRHS(vel.repeat_interleave(N).clone())
+    O.SetupPressureRHSdiv(d, 0.1, 0, True)
+    np.testing.assert_allclose(d.pressureRHSdiv.numpy(), 0.0, atol=1e-13)
+    O.SetupPressureRHSdiv(d, 0.1, 0, False)
+    assert np.abs(d.pressureRHSdiv.numpy()).max() > 1e-6
+
+
+@pytest.mark.parametrize("tsn", [False, True])
+def test_linear_field_divergence_closed_form(tsn):
+    nx, ny, hx, hy, x0, c, dt = 6, 4, 0.2, 0.15, 0.5, 1.3, 0.25
+    d = _domain(2)
+    b = scenarios._coords_block(d, uniform_coords(nx, ny, lx=nx * hx, ly=ny * hy, x0=x0, dtype=torch.float64,
+                                                  device="cpu"))
+    b.CloseAllBoundaries()
+    xc = x0 + (torch.arange(nx, dtype=torch.float64) + 0.5) * hx
+    b.CloseBoundary("-x", velocity=torch.tensor([[c * x0, 0.0]], dtype=torch.float64))
+    b.CloseBoundary("+x", velocity=torch.tensor([[c * (x0 + nx * hx), 0.0]], dtype=torch.float64))
+    wall = torch.stack([c * xc, torch.zeros(nx, dtype=torch.float64)]).view(1, 2, 1, nx).contiguous()
+    b.CloseBoundary("-y", velocity=wall.clone())
+    b.CloseBoundary("+y", velocity=wall.clone())
+    _prep(d)
+    u = torch.stack([(c * xc).expand(ny, nx), torch.zeros(ny, nx, dtype=torch.float64)])
+    d.setPressureRHS(u.reshape(-1).clone())
+    O.SetupPressureRHSdiv(d, dt, 0, True, tsn)
+    div = d.pressureRHSdiv.numpy().reshape(ny, nx)
+    want = np.full((ny, nx), c * hx * hy)
+    want[:, [0, -1]] = 0.75 * c * hx * hy
+    if tsn:
+        want /= dt
+    np.testing.assert_allclose(div, want, rtol=1e-12)
+
+
+def _matrix(d, face, A=None):
+    N = d.getTotalSize()
+    d.setA(A if A is not None else torch.rand(N, generator=torch.Generator().manual_seed(3), dtype=torch.float64)
+           + 0.5)
+    O.SetupPressureMatrix(d, 0.1, 0, face)
+    return d.P.value.clone()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_face_matrix_equals_cell_matrix_on_an_affine_grid(dims):
+    size = (6, 5, 4)[:dims]
+    shear = (0.4,) if dims == 2 else (0.3, -0.2, 0.25)
+    d = _domain(dims)
+    b = scenarios._coords_block(d, scenarios.sheared_coords(size, shear, h=0.1))
+    b.CloseAllBoundaries()
+    b.MakePeriodic("x")
+    _prep(d)
+    np.testing.assert_allclose(_matrix(d, True).numpy(), _matrix(d, False).numpy(), rtol=1e-12, atol=1e-13)
+
+
+def test_face_matrix_coefficient_is_area_over_centre_distance():
+    nx, ny, hy = 7, 4, 0.2
+    xs = torch.cumsum(torch.cat([torch.zeros(1), 0.1 * 1.2 ** torch.arange(nx, dtype=torch.float64)]), 0)
+    ys = torch.arange(ny + 1, dtype=torch.float64) * hy
+    gy, gx = torch.meshgrid(ys, xs.double(), indexing="ij")
+    coords = torch.stack([gx, gy])[None].contiguous()
+    d = _domain(2)
+    b = scenarios._coords_block(d, coords)
+    b.CloseAllBoundaries()
+    _prep(d)
+    Pv = _matrix(d, True, A=torch.ones(nx * ny, dtype=torch.float64)).numpy()
+    row, col = d.P.row.numpy(), d.P.index.numpy()
+    xc = 0.5 * (xs[1:] + xs[:-1]).numpy()
+    for j in range(ny):
+        for i in range(nx - 1):
+            g = j * nx + i
+            k = row[g] + list(col[row[g]:row[g + 1]]).index(g + 1)
+            np.testing.assert_allclose(Pv[k], hy / (xc[i + 1] - xc[i]), rtol=1e-12)
+    cell = _matrix(d, False, A=torch.ones(nx * ny, dtype=torch.float64)).numpy()
+    assert np.abs(cell - Pv).max() > 1e-3
+
+
+def test_face_options_with_non_orthogonal_terms_are_refused():
+    d = _domain(2)
+    b = scenarios._coords_block(d, scenarios.sheared_coords((5, 4), (0.3,), h=0.1))
+    b.CloseAllBoundaries()
+    _prep(d)
+    d.setA(torch.ones(d.getTotalSize(), dtype=torch.float64))
+    with pytest.raises(NotImplementedError):
+        O.SetupPressureMatrix(d, 0.1, 25, True)
+    with pytest.raises(NotImplementedError):
+        O.SetupPressureRHSdiv(d, 0.1, 25, True)

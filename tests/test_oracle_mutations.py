@@ -277,3 +277,42 @@ def test_ilu_pins_pass_on_the_oracle():
 def test_planted_mistake_fails_ilu_pins(name, edits, tmp_path):
     failed = _with_mutant(edits, tmp_path, _ilu_pins)
     assert len(failed) >= 1, (name, failed)
+
+
+# Planted mistakes in the face-transform pressure setup; the pins of test_oracle_face_transform.py must fail.
+FACE_MUTATIONS = [
+    ("face flux with a column of Minv for its row",
+     [("for (int j = 0; j < D; ++j) s += T_MINV(c, F, dim, j) * v[j];",
+       "for (int j = 0; j < D; ++j) s += T_MINV(c, F, j, dim) * v[j];")]),
+    ("face matrix coefficient with the cell's metric",
+     [("if (use_face) coef = laplace_orth_face(&c, b, pos, face) * ((real)0.5 * (raP + raN));",
+       "if (use_face) coef = alphaP[dim] * ((real)0.5 * (raP + raN));")]),
+    ("fixed-face velocity taken as it is",
+     [("        real v[3];\n        slerp2_half(vc, vn, D, v);\n",
+       "        real v[3];\n        if (at_bound(pos, face, b) && b->bounds[face].type == PICT_BOUNDARY_FIXED)\n"
+       "            for (int k = 0; k < D; ++k) v[k] = vn[k];\n        else slerp2_half(vc, vn, D, v);\n")]),
+]
+
+
+def _face_pins():
+    import test_oracle_face_transform as F
+    failed = []
+    pins = (("constant2d", lambda: F.test_constant_field_has_no_divergence_with_face_metrics(2)),
+            ("linear", lambda: F.test_linear_field_divergence_closed_form(False)),
+            ("stretched", F.test_face_matrix_coefficient_is_area_over_centre_distance))
+    for name, pin in pins:
+        try:
+            pin()
+        except AssertionError:
+            failed.append(name)
+    return failed
+
+
+def test_face_pins_pass_on_the_oracle():
+    assert _face_pins() == []
+
+
+@pytest.mark.parametrize("name,edits", FACE_MUTATIONS, ids=[m[0] for m in FACE_MUTATIONS])
+def test_planted_mistake_fails_face_pins(name, edits, tmp_path):
+    failed = _with_mutant(edits, tmp_path, _face_pins)
+    assert len(failed) >= 1, (name, failed)
