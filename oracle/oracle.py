@@ -319,11 +319,19 @@ def SetupAdvectionVelocityGrad(domain, timeStep, nonOrthoFlags=0, applyPressureG
 
 
 def SetupPressureMatrixGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False):
+    """PISO_build_pressure_matrix_GRAD (PISO_multiblock_cuda_kernel.cu:4966-5100): useFaceTransform reaches only its
+    non-orthogonal terms (not restated with face transforms); the orthogonal VJP is the cell-metric one."""
+    if useFaceTransform and domain.getSpatialDims() > 1 and int(nonOrthoMode) & (1 | 16):
+        raise NotImplementedError("useFaceTransform with non-orthogonal matrix terms")
     L = lib(_bits(domain))
     _chk(L, L.oracle_setup_pressure_matrix_grad(_desc(domain), int(nonOrthoMode)), "pressure matrix grad")
 
 
 def SetupPressureRHSdivGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
+    """k_computePressureRHSdivergenceFromFlux_GRAD (:5422-5452) scatters through the cell-metric fluxes whatever
+    useFaceTransform says; the flag reaches only the non-orthogonal terms (not restated with face transforms)."""
+    if useFaceTransform and domain.getSpatialDims() > 1 and int(nonOrthoMode) & (2 | 8):
+        raise NotImplementedError("useFaceTransform with non-orthogonal right-hand-side terms")
     L = lib(_bits(domain))
     _chk(L, L.oracle_pressure_rhs_add_non_ortho_grad(_desc(domain), int(nonOrthoMode)), "non-ortho rhs grad")
     _chk(L, L.oracle_setup_pressure_rhs_div_grad(_desc(domain), _dt(timeStep), int(timeStepNorm)), "div grad")

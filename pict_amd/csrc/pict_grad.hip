@@ -992,8 +992,10 @@ pict_status pict_setup_pressure_matrix_grad(const pict_atlas *a, double dt, int3
                                             void *stream) {
     (void)dt;
     PICT_CHECK_ARG(a && a->desc.P_grad && a->desc.A_grad && a->desc.A, "P_grad/A_grad missing");
-    if (use_face_transform) {
-        set_error("useFaceTransform gradients are not implemented (nor in the reference)");
+    // PISO_build_pressure_matrix_GRAD (:4966-5100) takes useFaceTransform but uses it only in the non-orthogonal
+    // terms: the orthogonal VJP is the cell-metric one with or without it
+    if (use_face_transform && a->desc.num_dims > 1 && (flags & kNonOrthoMatrix)) {
+        set_error("useFaceTransform with non-orthogonal matrix terms is not implemented by libpict_hip");
         return PICT_ERR_UNSUPPORTED;
     }
     if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
@@ -1006,8 +1008,10 @@ pict_status pict_setup_pressure_matrix_grad(const pict_atlas *a, double dt, int3
 pict_status pict_setup_pressure_rhs_div_grad(const pict_atlas *a, double dt, int32_t flags, int32_t use_face_transform,
                                              int32_t tsn, void *stream) {
     PICT_CHECK_ARG(a && a->desc.pressure_rhs_div_grad && a->desc.pressure_rhs_grad, "pressure rhs grads missing");
-    if (use_face_transform) {
-        set_error("useFaceTransform gradients are not implemented (nor in the reference)");
+    // k_computePressureRHSdivergenceFromFlux_GRAD (:5422-5452) scatters through the cell-metric fluxes whatever
+    // useFaceTransform says; only the non-orthogonal terms would use the face transforms
+    if (use_face_transform && a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
+        set_error("useFaceTransform with non-orthogonal right-hand-side terms is not implemented by libpict_hip");
         return PICT_ERR_UNSUPPORTED;
     }
     if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {

@@ -62,3 +62,32 @@ def test_face_transform_with_non_orthogonal_terms_is_refused():
         P.SetupPressureMatrix(g, tdt, 25, True)
     with pytest.raises(P.PictUnsupportedError):
         P.SetupPressureRHSdiv(g, tdt, 25, True)
+
+
+@pytest.mark.parametrize("name", ["rotated_connection", "box3d", "sheared_two_block_rotated"])
+def test_face_flag_leaves_the_orthogonal_vjps_unchanged(name):
+    """The reference's pressure-matrix and divergence VJPs do not use useFaceTransform outside the non-orthogonal
+    terms (:4966-5100, :5422-5452): with the flag, the VJP of the pressure correction equals the one without it (to
+    rounding: the scatters accumulate with atomics, whose order varies between runs)."""
+    from grad_util import OUTPUTS, create_grads, input_tensors
+    print("case", name, flush=True)
+    tdt = torch.tensor([0.05], dtype=torch.float64)
+    grads = []
+    for face in (False, True):
+        g = CASES[name](dtype=torch.float64, device="cuda")
+        P.SetupAdvectionMatrix(g, tdt, 0)
+        P.SetupAdvectionVelocity(g, tdt, 0)
+        g.setVelocityResult(_rand(g.velocityResult.shape, 27, 0.5).cuda())
+        g.UpdateDomainData()
+        P.SetupPressureCorrection(g, tdt, 0)
+        create_grads(g)
+        for k, o in enumerate(("P", "PRESSURE_RHS", "PRESSURE_RHS_DIV")):
+            OUTPUTS[o][1](g, _rand(OUTPUTS[o][0](g).shape, 30 + k).cuda())
+        g.UpdateDomainData()
+        P.SetupPressureCorrectionGrad(g, tdt, 0, face)
+        torch.cuda.synchronize()
+        grads.append([gg().clone() for inp in ("VELOCITY_RESULT", "A", "C", "VELOCITY")
+                      for _, gg in input_tensors(g, inp)])
+    assert len(grads[0]) > 0
+    for a, b in zip(*grads):
+        assert_close(a, b, 1e-13, "VJP with and without useFaceTransform")
