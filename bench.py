@@ -521,8 +521,11 @@ def _c4_sims(torch, count, dtype):
     from pict_amd import PISOtorch_simulation as S
     from pict_amd import configs
     doms, ns, visc, dt = configs.cavity_batch_c4(count, 0, dtype, "cuda")
+    # no preconditioned retry of a failed BiCGStab in either leg: the batched solver has none, and the plain loop
+    # would otherwise keep each step's input velocity for it
     sims = [S.Simulation(d, time_step=dt, corrector_steps=2, non_orthogonal=False, advection_tol=1e-6,
-                         pressure_tol=1e-6, advect_passive_scalar=False) for d in doms]
+                         pressure_tol=1e-6, advect_passive_scalar=False, BiCG_precondition_fallback=False)
+            for d in doms]
     return doms, sims, ns, visc, dt
 
 

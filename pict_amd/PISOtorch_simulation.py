@@ -887,7 +887,8 @@ class Simulation:
             return True
         self.flush_solver_checks()
         # a BiCGStab that fails inside the one-call step is retried with the ILU(0) preconditioner by the stage
-        # path (PISOtorch_diff.py:276-286): the step's inputs are kept to redo it there
+        # path (PISOtorch_diff.py:276-286): the step's input velocity is kept to redo it there (the block pressure
+        # is no input of an eligible step: its solves start from zero and the corrections write the pressure first)
         retry = tape is None and self.BiCG_precondition_fallback and (self.advection_use_BiCG or
                                                                       self.pressure_use_BiCG)
         if timing:
@@ -895,7 +896,7 @@ class Simulation:
         try:
             for _ in range(iterations):
                 domain.UpdateDomainData()
-                kept = [(b.velocity.clone(), b.pressure.clone()) for b in domain.getBlocks()] if retry else None
+                kept = [b.velocity.clone() for b in domain.getBlocks()] if retry else None
                 infos = (_abi.SolveInfo * (D + K))()
                 _abi.check(lib.pict_piso_step(domain._atlas, ctypes.byref(o),
                                               ctypes.byref(tape) if tape is not None else None, infos,
@@ -911,9 +912,8 @@ class Simulation:
                     if kept is None:
                         raise
                     _LOG.warning("A solve of the native step failed; redoing the step stage by stage.")
-                    for b, (v, p) in zip(domain.getBlocks(), kept):
+                    for b, v in zip(domain.getBlocks(), kept):
                         b.velocity.copy_(v)
-                        b.pressure.copy_(p)
                     if not self._PISO_split_step(1, time_step, native=False):
                         return False
                     continue
