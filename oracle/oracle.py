@@ -59,6 +59,7 @@ def declare(L):
                                       ctypes.c_void_p, ctypes.c_void_p]
     L.oracle_sparse_outer_product.argtypes = [ctypes.c_void_p] * 4
     L.oracle_ilu0.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.oracle_pressure_rhs_add_non_ortho_face.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_coords_to_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     L.oracle_coords_to_face_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                    ctypes.c_void_p]
@@ -190,11 +191,9 @@ def SetupAdvectionVelocity(domain, timeStep, nonOrthoFlags=0, applyPressureGradi
 
 
 def SetupPressureMatrix(domain, timeStep, nonOrthoMode=0, useFaceTransform=False):
-    """useFaceTransform: the face-transform coefficients of the orthogonal terms (PISO_multiblock_cuda_kernel.cu
-    :4857-4865); with non-orthogonal terms it is not restated (their face-transform interpolation)."""
+    """useFaceTransform: the coefficients from the faces' transforms (PISO_multiblock_cuda_kernel.cu:4857-4865, the
+    non-orthogonal ones :1962-1972)."""
     L = lib(_bits(domain))
-    if useFaceTransform and domain.getSpatialDims() > 1 and int(nonOrthoMode) & (1 | 16):
-        raise NotImplementedError("useFaceTransform with non-orthogonal matrix terms")
     _chk(L, L.oracle_setup_pressure_matrix(_desc(domain), int(nonOrthoMode), int(bool(useFaceTransform))),
          "pressure matrix")
 
@@ -203,11 +202,10 @@ def SetupPressureRHSdiv(domain, timeStep, nonOrthoMode=0, useFaceTransform=False
     """Divergence of the pressure RHS, then its explicit non-orthogonal terms (PISO_multiblock_cuda_kernel.cu
     :5659-5704)."""
     L = lib(_bits(domain))
-    if useFaceTransform and domain.getSpatialDims() > 1 and int(nonOrthoMode) & (2 | 8):
-        raise NotImplementedError("useFaceTransform with non-orthogonal right-hand-side terms")
     _chk(L, L.oracle_setup_pressure_rhs_div(_desc(domain), _dt(timeStep), int(timeStepNorm),
                                             int(bool(useFaceTransform))), "pressure div")
-    _chk(L, L.oracle_pressure_rhs_add_non_ortho(_desc(domain), int(nonOrthoMode)), "pressure non-ortho rhs")
+    add = L.oracle_pressure_rhs_add_non_ortho_face if useFaceTransform else L.oracle_pressure_rhs_add_non_ortho
+    _chk(L, add(_desc(domain), int(nonOrthoMode)), "pressure non-ortho rhs")
 
 
 def PressureRHSaddNonOrtho(domain, nonOrthoMode):

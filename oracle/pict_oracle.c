@@ -636,6 +636,21 @@ done:;
    face value of alpha_{axis, (axis+i)%D} (times 1/A and the viscosity when asked).  Faces of a
    cell at either end of an axis whose face on that axis is prescribed are left 0, as in the
    reference's condition (0 < x < size-1 || face not prescribed). */
+/* useFaceTransform for the non-orthogonal pressure terms (:1962-1972, :3066-3068, :3128-3146): set only around the
+   pressure stages that take the option, read by interp_nonortho and nonortho_rhs */
+static int g_face_coefficients = 0;
+
+/* getLaplaceCoefficientsSingleFace :1320-1343: det Minv[cc].Minv[axis] of the face's own transform (the unit vector
+   along the face's axis without a transform) */
+static real lap_single_face(const Ctx *c, const pict_block_desc *b, I4 pos, int face, int cc) {
+    const int axis = face >> 1;
+    if (!b->has_transform) return cc == axis ? (real)1 : (real)0;
+    const real *T = face_T(c, b, pos, face);
+    real s = 0;
+    for (int j = 0; j < c->D; ++j) s += T_MINV(c, T, cc, j) * T_MINV(c, T, axis, j);
+    return T_DET(c, T) * s;
+}
+
 static void interp_nonortho(const Ctx *c, int bi, I4 pos, int with_visc, int with_a, real *out) {
     const pict_block_desc *b = blk(c, bi);
     const real *A = (const real *)c->d->A;
@@ -664,7 +679,9 @@ static void interp_nonortho(const Ctx *c, int bi, I4 pos, int with_visc, int wit
             const real raN = (with_a ? one / A[global_index(pn, nb)] : one) * (with_visc ? viscosity(c, nb, pn) : one);
             for (int i = 1; i < c->D; ++i) {
                 const int oa = (axis + i) % c->D;
-                out[face * (c->D - 1) + i - 1] = (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN) * (real)0.5;
+                out[face * (c->D - 1) + i - 1] = g_face_coefficients
+                    ? lap_single_face(c, b, pos, face, oa) * (raP + raN) * (real)0.5
+                    : (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN) * (real)0.5;
             }
         } else {
             for (int i = 1; i < c->D; ++i) out[face * (c->D - 1) + i - 1] = 0;
@@ -721,7 +738,8 @@ static real nonortho_rhs(const Ctx *c, int bi, I4 pos, int flags, int type, int 
         if (at_bound(pos, face, b) && prescribed(face, b)) {
             if (fixed_type(b, face, type, pos.a[3]) == PICT_BC_NEUMANN || gd_is_pressure(type)) continue;
             real bA[3];
-            lap_neighbour_boundary(c, &b->bounds[face], pos, axis, bA);
+            if (g_face_coefficients) for (int cc = 0; cc < D; ++cc) bA[cc] = lap_single_face(c, b, pos, face, cc);
+            else lap_neighbour_boundary(c, &b->bounds[face], pos, axis, bA);
             for (int i = 1; i < D; ++i) {
                 const int ta = (axis + i) % D;
                 const int tlo = pos.a[ta] == 0, thi = pos.a[ta] == b->size[ta] - 1;
@@ -743,7 +761,8 @@ static real nonortho_rhs(const Ctx *c, int bi, I4 pos, int flags, int type, int 
         const real raN = (with_a ? one / A[global_index(n.pos, nb)] : one) * (with_visc ? viscosity(c, nb, n.pos) : one);
         for (int i = 1; i < D; ++i) {
             const int ta = (axis + i) % D;
-            const real fa = (aP[lap_ci(axis, ta)] * raP + aN[ta] * raN) * (real)0.5;
+            const real fa = g_face_coefficients ? lap_single_face(c, b, pos, face, ta) * (raP + raN) * (real)0.5
+                                                : (aP[lap_ci(axis, ta)] * raP + aN[ta] * raN) * (real)0.5;
             real tdg = 0;
             for (int tu = 0; tu < 2; ++tu) {
                 const int tf = (ta << 1) | tu, tfs = face_sign(tf);
@@ -996,6 +1015,13 @@ int oracle_setup_advection_velocity(const pict_domain_desc *d, double time_step,
    orthogonal terms only) */
 int oracle_setup_pressure_matrix(const pict_domain_desc *d, int flags, int use_face) {
     Ctx c; int st = init_ctx(&c, d); if (st) return st;
+    if (use_face && c.D > 1 && (flags & (PICT_NON_ORTHO_DIRECT_MATRIX | PICT_NON_ORTHO_CENTER_MATRIX)))
+        for (int bi = 0; bi < d->num_blocks; ++bi)
+            if (d->blocks[bi].has_transform && !d->blocks[bi].face_transform) {
+                snprintf(g_err, sizeof g_err, "block %d: face coefficients need face transforms", bi);
+                return PICT_ERR_INVALID;
+            }
+    g_face_coefficients = use_face;
     real *Pv = (real *)d->P_value; const real *A = (const real *)d->A;
     for (int bi = 0; bi < d->num_blocks; ++bi) {
         const pict_block_desc *b = blk(&c, bi);
@@ -1031,6 +1057,7 @@ int oracle_setup_pressure_matrix(const pict_domain_desc *d, int flags, int use_f
             for (int i = 0; i < n; ++i) { int s = lowest(idx); Pv[start + i] = rv[s]; idx[s] = -1; }
         }
     }
+    g_face_coefficients = 0;
     return PICT_OK;
 }
 
@@ -1167,6 +1194,20 @@ int oracle_pressure_rhs_add_non_ortho(const pict_domain_desc *d, int flags) {
         }
     }
     return PICT_OK;
+}
+
+/* the same with the face coefficients of useFaceTransform (:3128-3146); every transformed block needs face
+   transforms */
+int oracle_pressure_rhs_add_non_ortho_face(const pict_domain_desc *d, int flags) {
+    for (int bi = 0; bi < d->num_blocks; ++bi)
+        if (d->blocks[bi].has_transform && !d->blocks[bi].face_transform) {
+            snprintf(g_err, sizeof g_err, "block %d: face coefficients need face transforms", bi);
+            return PICT_ERR_INVALID;
+        }
+    g_face_coefficients = 1;
+    const int st = oracle_pressure_rhs_add_non_ortho(d, flags);
+    g_face_coefficients = 0;
+    return st;
 }
 
 /* k_computeVelocityDivergenceFromFlux :6333-6354 */

@@ -199,7 +199,7 @@ __device__ __forceinline__ void p_matrix_cell(const DomainK<R> &dom, int tile, i
         if constexpr (NO && D > 1) {
             if (flags & kNonOrthoMatrix) {
                 R ai[2 * D * (D - 1)];
-                interp_nonortho<R, D>(dom, bi, pos, false, true, ai);
+                interp_nonortho<R, D, FACE>(dom, bi, pos, false, true, ai);
                 nonortho_matrix_face<R, D>(dom, bi, pos, face, ai, flags, R(1), GD_IS_FIXED, true, rv);
             }
         }
@@ -351,14 +351,15 @@ __device__ __forceinline__ void divergence_face_cell(const DomainK<R> &dom, int 
 
 // k_pressureRHSaddNonOrthoComponents :5455-5477: pressure_rhs_div += the explicit non-orthogonal part of the
 // pressure Laplacian from the pressure result
-template <typename R, int D>
+template <typename R, int D, bool FACE = false>
 __device__ __forceinline__ void p_rhs_nonortho_cell(const DomainK<R> &dom, int tile, int flags) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, tile, b, bi, f)) return;
     const int g = f + b->global_offset;
     R S = dom.prhs_div[g];
-    if constexpr (D > 1) S += nonortho_rhs<R, D>(dom, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, false, true);
+    if constexpr (D > 1)
+        S += nonortho_rhs<R, D, FACE>(dom, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, false, true);
     dom.prhs_div[g] = S;
 }
 

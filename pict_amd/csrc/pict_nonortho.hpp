@@ -324,7 +324,22 @@ __device__ __forceinline__ bool interp_face_live(const pict_block_desc *b, const
 }
 // interpolateNonOrthoLaplaceComponents :1911-1986 (cell metrics): out[face*(D-1) + i-1] is the face value of
 // alpha_{axis,(axis+i)%D}, times 1/A and the viscosity when asked.
+// getLaplaceCoefficientsSingleFace :1320-1343: det Minv[c].Minv[axis] of the face's own transform (the unit
+// vector along the face's axis without a transform; a transformed block needs face transforms)
 template <typename R, int D>
+__device__ __forceinline__ R lap_single_face(const pict_block_desc *b, const I4 &pos, int face, int c) {
+    const int axis = face >> 1;
+    if (!b->has_transform) return c == axis ? R(1) : R(0);
+    const R *T = face_T<R, D>(b, pos, face);
+    R s = 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) s += Tminv<R, D>(T, c, j) * Tminv<R, D>(T, axis, j);
+    return Tdet<R, D>(T) * s;
+}
+
+// FACE (useFaceTransform, :1962-1972): each face's cross coefficients from the face's transform times the mean of
+// the two 1/A, instead of the mean of the two cells' products
+template <typename R, int D, bool FACE = false>
 __device__ __forceinline__ void interp_nonortho(const DomainK<R> &dom, int bi, const I4 &pos, bool with_visc,
                                                 bool with_a, R *out) {
     const pict_block_desc *b = dom.blocks + bi;
@@ -345,7 +360,8 @@ __device__ __forceinline__ void interp_nonortho(const DomainK<R> &dom, int bi, c
 #pragma unroll
             for (int i = 1; i < D; ++i) {
                 const int oa = (axis + i) % D;
-                out[face * (D - 1) + i - 1] = (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN) * R(0.5);
+                out[face * (D - 1) + i - 1] = FACE ? lap_single_face<R, D>(b, pos, face, oa) * (raP + raN) * R(0.5)
+                                                   : (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN) * R(0.5);
             }
         } else {
 #pragma unroll
@@ -394,8 +410,9 @@ __device__ __forceinline__ void nonortho_matrix_face(const DomainK<R> &dom, int 
 }
 
 // getNonOrthoLaplaceRHS_v2 :3033-3190 (cell metrics): the explicit non-orthogonal part of the Laplacian of `type`
-// at `pos` (w = component) from the previous iterate.
-template <typename R, int D>
+// at `pos` (w = component) from the previous iterate.  FACE (useFaceTransform, :3066-3068, :3128-3146): the face
+// coefficients from the face's transform, times the mean of the two 1/A.
+template <typename R, int D, bool FACE = false>
 __device__ __forceinline__ R nonortho_rhs(const DomainK<R> &dom, int bi, const I4 &pos, int flags, int type,
                                           bool with_visc, bool with_a) {
     if (!(flags & kNonOrthoRhs)) return R(0);
@@ -411,7 +428,12 @@ __device__ __forceinline__ R nonortho_rhs(const DomainK<R> &dom, int bi, const I
         if (at_bound(pos, face, b) && prescribed(face, b)) {
             if (fixed_type(b, face, type, pos.a[3]) == PICT_BC_NEUMANN || type == GD_PRESSURE_RESULT) continue;
             R bA[D];
-            lap_neighbour_boundary<R, D>(&b->bounds[face], pos, axis, bA);
+            if constexpr (FACE) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) bA[c] = lap_single_face<R, D>(b, pos, face, c);
+            } else {
+                lap_neighbour_boundary<R, D>(&b->bounds[face], pos, axis, bA);
+            }
 #pragma unroll
             for (int i = 1; i < D; ++i) {
                 const int ta = (axis + i) % D;
@@ -434,7 +456,8 @@ __device__ __forceinline__ R nonortho_rhs(const DomainK<R> &dom, int bi, const I
 #pragma unroll
         for (int i = 1; i < D; ++i) {
             const int ta = (axis + i) % D;
-            const R fa = (aP[lap_ci(axis, ta)] * raP + aN[ta] * raN) * R(0.5);
+            const R fa = FACE ? lap_single_face<R, D>(b, pos, face, ta) * (raP + raN) * R(0.5)
+                              : (aP[lap_ci(axis, ta)] * raP + aN[ta] * raN) * R(0.5);
             R tdg = 0;
 #pragma unroll
             for (int tu = 0; tu < 2; ++tu) {

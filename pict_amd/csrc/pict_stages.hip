@@ -160,9 +160,25 @@ __global__ __launch_bounds__(kCellsPerTile) void k_divergence(DomainK<R> dom, co
     divergence_cell<R, D>(dom, (int)blockIdx.x, vg, out, dt, tsn);
 }
 
+template <typename R, int D, bool NO>
+__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_face(DomainK<R> dom, int flags) {
+    p_matrix_cell<R, D, NO, true>(dom, (int)blockIdx.x, flags);
+}
+
 template <typename R, int D>
-__global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_face(DomainK<R> dom) {
-    p_matrix_cell<R, D, false, true>(dom, (int)blockIdx.x, 0);
+__global__ __launch_bounds__(kCellsPerTile) void k_p_rhs_nonortho_face(DomainK<R> dom, int flags) {
+    p_rhs_nonortho_cell<R, D, true>(dom, (int)blockIdx.x, flags);
+}
+
+// the non-orthogonal terms with face transforms read every face's transform of a transformed block
+// (getLaplaceCoefficientsSingleFace :1320-1343)
+static pict_status check_face_transforms(const pict_atlas *a) {
+    for (int i = 0; i < a->desc.num_blocks; ++i) {
+        const pict_block_desc &b = a->host_blocks[i];
+        PICT_CHECK_ARG(!b.has_transform || (b.has_face_transform && b.face_transform),
+                       "useFaceTransform with non-orthogonal terms: block %d has a transform but no face transforms", i);
+    }
+    return PICT_OK;
 }
 
 template <typename R, int D>
@@ -452,12 +468,14 @@ pict_status pict_setup_pressure_matrix(const pict_atlas *a, double dt, int32_t f
     (void)dt;
     PICT_CHECK_ARG(a && a->desc.P_value && a->desc.A, "P/A missing");
     if (use_face_transform) {
-        // useFaceTransform (:4857-4865): the orthogonal coefficients from the faces' transforms
+        // useFaceTransform (:4857-4865, :4882-4883): the coefficients from the faces' transforms
         if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix)) {
-            set_error("useFaceTransform with non-orthogonal matrix terms is not implemented by libpict_hip");
-            return PICT_ERR_UNSUPPORTED;
+            pict_status st = check_face_transforms(a);
+            if (st) return st;
+            PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_face<R, D, true>), a->view<R>(), (int)flags));
+        } else {
+            PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_face<R, D, false>), a->view<R>(), 0));
         }
-        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_face<R, D>), a->view<R>()));
         return check_launch();
     }
     if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
@@ -472,16 +490,17 @@ pict_status pict_setup_pressure_rhs_div(const pict_atlas *a, double dt, int32_t 
     PICT_CHECK_ARG(a && a->desc.pressure_rhs && a->desc.pressure_rhs_div, "pressure rhs missing");
     if (use_face_transform) {
         // computeFluxesWithFaceTransforms :1703-1785 reads every face's transform
-        if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
-            set_error("useFaceTransform with non-orthogonal right-hand-side terms is not implemented by libpict_hip");
-            return PICT_ERR_UNSUPPORTED;
-        }
         for (int i = 0; i < a->desc.num_blocks; ++i)
             PICT_CHECK_ARG(a->host_blocks[i].has_face_transform && a->host_blocks[i].face_transform,
                            "useFaceTransform: block %d has no face transforms", i);
         PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_divergence_face<R, D>), a->view<R>(),
                                            (const R *)a->desc.pressure_rhs, (R *)a->desc.pressure_rhs_div, (R)dt,
                                            (int)tsn));
+        pict_status st = check_launch();
+        if (st || a->desc.num_dims < 2 || !(flags & kNonOrthoRhs)) return st;
+        // k_pressureRHSaddNonOrthoComponents with face coefficients (:5455-5477, :3128-3146)
+        PICT_CHECK_ARG(a->desc.pressure_result && a->desc.A, "pressure_result missing (non-orthogonal RHS terms)");
+        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs_nonortho_face<R, D>), a->view<R>(), (int)flags));
         return check_launch();
     }
     PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_divergence<R, D>), a->view<R>(),

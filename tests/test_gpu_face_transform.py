@@ -1,9 +1,9 @@
 """useFaceTransform in the pressure setup (computeFluxesWithFaceTransforms PISO_multiblock_cuda_kernel.cu:1703-1785
 in the pressure-RHS divergence :5374-5419; getLaplaceCoefficientOrthogonalFace :1226-1241 in the pressure matrix
 :4857-4865) on the GPU against the FP64 oracle (tests/test_oracle_face_transform.py pins it), on every scenario with
-vertex coordinates: FP64 1e-11, FP32 1e-5 relative (the divergence, a difference of fluxes, 10x that).  Blocks
-without face transforms: the matrix takes the coefficient 1 as in the reference, the divergence is refused;
-non-orthogonal terms with face transforms are refused."""
+vertex coordinates, and the non-orthogonal terms with face coefficients on the sheared scenarios: FP64 1e-11, FP32
+1e-5 relative (the divergence, a difference of fluxes, 10x that).  Blocks without face transforms: the matrix takes
+the coefficient 1 as in the reference, the divergence is refused."""
 import pytest
 import torch
 
@@ -54,14 +54,38 @@ def test_face_transform_pressure_setup_matches_the_oracle(name, dtype):
                 O.SetupPressureRHSdiv(c, dt, 0, True, tsn)
 
 
-def test_face_transform_with_non_orthogonal_terms_is_refused():
-    g = scenarios.NON_ORTHO["sheared_walls"](dtype=torch.float32, device="cuda")
-    tdt = torch.tensor([0.05], dtype=torch.float32)
+@pytest.mark.parametrize("name", list(scenarios.NON_ORTHO))
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_non_orthogonal_face_terms_match_the_oracle(name, dtype):
+    """The non-orthogonal pressure terms with face coefficients (:1962-1972, :3128-3146; flags 25) against the
+    oracle; their VJP is refused."""
+    print("case", name, dtype, flush=True)
+    g = scenarios.NON_ORTHO[name](dtype=dtype, device="cuda")
+    c = O.cpu_twin(g)
+    tol = tol_for(dtype)
+    dt = 0.05
+    tdt = torch.tensor([dt], dtype=dtype)
     P.SetupAdvectionMatrix(g, tdt, 0)
+    O.SetupAdvectionMatrix(c, dt)
+    P.SetupPressureMatrix(g, tdt, 25, True)
+    O.SetupPressureMatrix(c, dt, 25, True)
+    assert_close(g.P.value, c.P.value, tol, "non-orthogonal pressure matrix with face coefficients")
+    H = _rand(g.pressureRHS.shape, 28, 0.5)
+    pr = _rand(g.pressureRHSdiv.shape, 29)
+    g.setPressureRHS(H.to(dtype=dtype, device="cuda"))
+    c.setPressureRHS(H.to(dtype=dtype).double())
+    g.setPressureResult(pr.to(dtype=dtype, device="cuda"))
+    c.setPressureResult(pr.to(dtype=dtype).double())
+    g.UpdateDomainData()
+    c.UpdateDomainData()
+    P.SetupPressureRHSdiv(g, tdt, 25, True)
+    O.SetupPressureRHSdiv(c, dt, 25, True)
+    assert_close(g.pressureRHSdiv, c.pressureRHSdiv, 10 * tol, "divergence with non-orthogonal face terms")
+    from grad_util import create_grads
+    create_grads(g)
+    g.UpdateDomainData()
     with pytest.raises(P.PictUnsupportedError):
-        P.SetupPressureMatrix(g, tdt, 25, True)
-    with pytest.raises(P.PictUnsupportedError):
-        P.SetupPressureRHSdiv(g, tdt, 25, True)
+        P.SetupPressureMatrixGrad(g, tdt, 25, True)
 
 
 @pytest.mark.parametrize("name", ["rotated_connection", "box3d", "sheared_two_block_rotated"])

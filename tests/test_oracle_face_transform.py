@@ -6,7 +6,9 @@ pressure-matrix coefficient of getLaplaceCoefficientOrthogonalFace (:1226-1241, 
 - u = (c x, 0) on a uniform orthogonal grid away from x = 0: interior cells c hx hy; a cell next to an x wall
   0.75 c hx hy, because the reference blends the wall's velocity with the cell's (the mean of collinear vectors);
 - the matrix: on an affine grid the face coefficients are the cell coefficients; on a grid stretched along x the
-  coefficient between two cells is hy over the distance of their centres (face area over centre distance).
+  coefficient between two cells is hy over the distance of their centres (face area over centre distance);
+- the non-orthogonal terms (:1962-1972, :3128-3146): on an affine grid the face cross coefficients equal the cells'
+  for any A, in the matrix and in the explicit RHS terms; on a warped grid they differ.
 """
 import numpy as np
 import pytest
@@ -117,13 +119,53 @@ def test_face_matrix_coefficient_is_area_over_centre_distance():
     assert np.abs(cell - Pv).max() > 1e-3
 
 
-def test_face_options_with_non_orthogonal_terms_are_refused():
+def _non_ortho_parts(d, face):
+    """(pressure matrix values, non-orthogonal part of the pressure-RHS divergence) with flags 25"""
+    O.SetupPressureMatrix(d, 0.1, 25, face)
+    Pv = d.P.value.clone()
+    O.SetupPressureRHSdiv(d, 0.1, 0, face)
+    d0 = d.pressureRHSdiv.clone()
+    O.SetupPressureRHSdiv(d, 0.1, 25, face)
+    return Pv, d.pressureRHSdiv - d0
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_non_orthogonal_face_coefficients(dims):
+    """On an affine grid a face's transform is its cells' transform, so the face coefficient times the mean of the
+    two 1/A equals the mean of the two cells' products for any A: the matrix and the explicit non-orthogonal RHS
+    terms agree with and without useFaceTransform.  On a warped grid they differ."""
+    size = (6, 5, 4)[:dims]
+    shear = (0.4,) if dims == 2 else (0.3, -0.2, 0.25)
+    for coords, same in ((scenarios.sheared_coords(size, shear, h=0.1), True), (_warped(size, shear), False)):
+        d = _domain(dims)
+        b = scenarios._coords_block(d, coords)
+        b.CloseAllBoundaries()
+        b.MakePeriodic("x")
+        _prep(d)
+        g = torch.Generator().manual_seed(12)
+        N = d.getTotalSize()
+        d.setA(torch.rand(N, generator=g, dtype=torch.float64) + 0.5)
+        d.setPressureRHS(torch.randn(dims * N, generator=g, dtype=torch.float64))
+        d.setPressureResult(torch.randn(N, generator=g, dtype=torch.float64))
+        d.UpdateDomainData()
+        Pc, Sc = _non_ortho_parts(d, False)
+        Pf, Sf = _non_ortho_parts(d, True)
+        if same:
+            np.testing.assert_allclose(Pf.numpy(), Pc.numpy(), rtol=1e-12, atol=1e-12)
+            np.testing.assert_allclose(Sf.numpy(), Sc.numpy(), rtol=1e-11, atol=1e-12)
+        else:
+            assert float((Pf - Pc).abs().max()) > 1e-4 and float((Sf - Sc).abs().max()) > 1e-6
+
+
+def test_face_coefficients_without_face_transforms_are_refused():
+    """A transformed block without face transforms: the non-orthogonal face coefficients cannot be read."""
     d = _domain(2)
-    b = scenarios._coords_block(d, scenarios.sheared_coords((5, 4), (0.3,), h=0.1))
+    b = d.CreateBlock(velocity=torch.zeros(1, 2, 4, 5, dtype=torch.float64))
+    c = scenarios.sheared_coords((5, 4), (0.3,), h=0.1)
+    b.setTransform(O.CoordsToTransforms(c))
     b.CloseAllBoundaries()
     _prep(d)
     d.setA(torch.ones(d.getTotalSize(), dtype=torch.float64))
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(RuntimeError):
         O.SetupPressureMatrix(d, 0.1, 25, True)
-    with pytest.raises(NotImplementedError):
-        O.SetupPressureRHSdiv(d, 0.1, 25, True)
+    O.SetupPressureMatrix(d, 0.1, 0, True)  # orthogonal terms: coefficient 1 without face transforms

@@ -79,8 +79,8 @@ def test_planted_mistake_fails_projection_pins(name, edits, tmp_path):
 # matrix part is pinned by the split check, whose bent grid varies the cross coefficients.
 NON_ORTHO_MUTATIONS = [
     ("interpolated cross coefficient without the 1/2",
-     [("out[face * (c->D - 1) + i - 1] = (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN) * (real)0.5;",
-       "out[face * (c->D - 1) + i - 1] = (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN);"),
+     [(": (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN) * (real)0.5;",
+       ": (aP[lap_ci(axis, oa)] * raP + aN[oa] * raN);"),
       ("raP_grad += aP[lap_ci(axis, oa)] * (real)0.5 * g;", "raP_grad += aP[lap_ci(axis, oa)] * g;"),
       ("raN_grad += aN[oa] * (real)0.5 * g;", "raN_grad += aN[oa] * g;")]),
     ("matrix corner terms without the tangential face sign",
@@ -92,8 +92,8 @@ NON_ORTHO_MUTATIONS = [
     ("explicit non-orthogonal terms with the sign flipped",
      [("            S -= (real)fs * fa * tdg;\n", "            S += (real)fs * fa * tdg;\n")]),
     ("centre cross coefficient of the wrong axis pair",
-     [("const real fa = (aP[lap_ci(axis, ta)] * raP + aN[ta] * raN) * (real)0.5;",
-       "const real fa = (aP[lap_ci(axis, axis)] * raP + aN[ta] * raN) * (real)0.5;")]),
+     [(": (aP[lap_ci(axis, ta)] * raP + aN[ta] * raN) * (real)0.5;",
+       ": (aP[lap_ci(axis, axis)] * raP + aN[ta] * raN) * (real)0.5;")]),
 ]
 
 
