@@ -60,6 +60,7 @@ def declare(L):
     L.oracle_sparse_outer_product.argtypes = [ctypes.c_void_p] * 4
     L.oracle_ilu0.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     L.oracle_pressure_rhs_add_non_ortho_face.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.oracle_pressure_rhs_add_non_ortho_grad_face.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.oracle_coords_to_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     L.oracle_coords_to_face_transforms.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                    ctypes.c_void_p]
@@ -317,21 +318,18 @@ def SetupAdvectionVelocityGrad(domain, timeStep, nonOrthoFlags=0, applyPressureG
 
 
 def SetupPressureMatrixGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False):
-    """PISO_build_pressure_matrix_GRAD (PISO_multiblock_cuda_kernel.cu:4966-5100): useFaceTransform reaches only its
-    non-orthogonal terms (not restated with face transforms); the orthogonal VJP is the cell-metric one."""
-    if useFaceTransform and domain.getSpatialDims() > 1 and int(nonOrthoMode) & (1 | 16):
-        raise NotImplementedError("useFaceTransform with non-orthogonal matrix terms")
+    """PISO_build_pressure_matrix_GRAD (PISO_multiblock_cuda_kernel.cu:4966-5115): it passes false for the face flag to
+    its interpolation and scatter of the non-orthogonal coefficients, so the VJP is the cell-metric one either way."""
     L = lib(_bits(domain))
     _chk(L, L.oracle_setup_pressure_matrix_grad(_desc(domain), int(nonOrthoMode)), "pressure matrix grad")
 
 
 def SetupPressureRHSdivGrad(domain, timeStep, nonOrthoMode=0, useFaceTransform=False, timeStepNorm=False):
     """k_computePressureRHSdivergenceFromFlux_GRAD (:5422-5452) scatters through the cell-metric fluxes whatever
-    useFaceTransform says; the flag reaches only the non-orthogonal terms (not restated with face transforms)."""
-    if useFaceTransform and domain.getSpatialDims() > 1 and int(nonOrthoMode) & (2 | 8):
-        raise NotImplementedError("useFaceTransform with non-orthogonal right-hand-side terms")
+    useFaceTransform says; the non-orthogonal terms' VJP uses the face coefficients (:5491)."""
     L = lib(_bits(domain))
-    _chk(L, L.oracle_pressure_rhs_add_non_ortho_grad(_desc(domain), int(nonOrthoMode)), "non-ortho rhs grad")
+    add = L.oracle_pressure_rhs_add_non_ortho_grad_face if useFaceTransform else L.oracle_pressure_rhs_add_non_ortho_grad
+    _chk(L, add(_desc(domain), int(nonOrthoMode)), "non-ortho rhs grad")
     _chk(L, L.oracle_setup_pressure_rhs_div_grad(_desc(domain), _dt(timeStep), int(timeStepNorm)), "div grad")
 
 

@@ -1910,7 +1910,8 @@ static void nonortho_matrix_face_grad(const Ctx *c, int bi, I4 pos, int face, co
     }
 }
 
-/* scatterNonOrthoLaplaceRHS_v2_GRAD :3210-3380 (no face transform): the VJP of nonortho_rhs for the
+/* scatterNonOrthoLaplaceRHS_v2_GRAD :3210-3380 (face coefficients under g_face_coefficients, :3294-3312,
+   :3364-3368): the VJP of nonortho_rhs for the
    output gradient s_grad; `type` is the forward GridDataType (gradients go to the matching
    result-gradient vector, FixedBoundary velocity gradients, viscosities and A). */
 static void nonortho_rhs_grad(const Ctx *c, int bi, I4 pos, int flags, int type, int with_visc, int with_a,
@@ -1935,7 +1936,8 @@ static void nonortho_rhs_grad(const Ctx *c, int bi, I4 pos, int flags, int type,
         if (at_bound(pos, face, b) && prescribed(face, b)) {
             if (fixed_type(b, face, type, pos.a[3]) == PICT_BC_NEUMANN || gd_is_pressure(type)) continue;
             real bA[3];
-            lap_neighbour_boundary(c, &b->bounds[face], pos, axis, bA);
+            if (g_face_coefficients) for (int cc = 0; cc < D; ++cc) bA[cc] = lap_single_face(c, b, pos, face, cc);
+            else lap_neighbour_boundary(c, &b->bounds[face], pos, axis, bA);
             for (int i = 1; i < D; ++i) {
                 const int ta = (axis + i) % D;
                 const int tlo = pos.a[ta] == 0, thi = pos.a[ta] == b->size[ta] - 1;
@@ -1966,7 +1968,9 @@ static void nonortho_rhs_grad(const Ctx *c, int bi, I4 pos, int flags, int type,
         real raN_grad = 0;
         for (int i = 1; i < D; ++i) {
             const int ta = (axis + i) % D;
-            const real fa = (aP[lap_ci(axis, ta)] * rvP + aN[ta] * rvN) * (real)0.5;
+            const real cP = g_face_coefficients ? lap_single_face(c, b, pos, face, ta) : aP[lap_ci(axis, ta)];
+            const real cN = g_face_coefficients ? cP : aN[ta];
+            const real fa = (cP * rvP + cN * rvN) * (real)0.5;
             const real tgg = -s_grad * (real)fs * fa;
             real tdg = 0;
             for (int tu = 0; tu < 2; ++tu) {
@@ -1991,8 +1995,8 @@ static void nonortho_rhs_grad(const Ctx *c, int bi, I4 pos, int flags, int type,
             }
             if (diffable) {
                 const real fag = -s_grad * (real)fs * tdg;
-                raP_grad += aP[lap_ci(axis, ta)] * (real)0.5 * fag;
-                raN_grad += aN[ta] * (real)0.5 * fag;
+                raP_grad += cP * (real)0.5 * fag;
+                raN_grad += cN * (real)0.5 * fag;
             }
         }
         if (with_visc) scatter_viscosity(c, nb, n.pos, raN_grad * raN);
@@ -2214,6 +2218,19 @@ int oracle_pressure_rhs_add_non_ortho_grad(const pict_domain_desc *d, int flags)
             nonortho_rhs_grad(&c, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, 0, 1, dg[f + b->global_offset]);
     }
     return PICT_OK;
+}
+
+/* the same with the face coefficients of useFaceTransform, which this VJP uses (:5491) */
+int oracle_pressure_rhs_add_non_ortho_grad_face(const pict_domain_desc *d, int flags) {
+    for (int bi = 0; bi < d->num_blocks; ++bi)
+        if (d->blocks[bi].has_transform && !d->blocks[bi].face_transform) {
+            snprintf(g_err, sizeof g_err, "block %d: face coefficients need face transforms", bi);
+            return PICT_ERR_INVALID;
+        }
+    g_face_coefficients = 1;
+    const int st = oracle_pressure_rhs_add_non_ortho_grad(d, flags);
+    g_face_coefficients = 0;
+    return st;
 }
 
 /* k_computePressureRHSdivergenceFromFlux_GRAD :5423-5452 */

@@ -431,7 +431,7 @@ __device__ __forceinline__ void nonortho_matrix_face_grad(const DomainK<R> &dom,
 
 // scatterNonOrthoLaplaceRHS_v2_GRAD :3210-3380 (cell metrics): VJP of nonortho_rhs for the output gradient s_grad
 // into the result gradient of `type`, FixedBoundary velocity gradients, viscosities and A.
-template <typename R, int D>
+template <typename R, int D, bool FACE = false>
 __device__ __forceinline__ void nonortho_rhs_grad(const DomainK<R> &dom, int bi, const I4 &pos, int flags, int type,
                                                   bool with_visc, bool with_a, R s_grad, UniformAcc<R> &u) {
     if (!(flags & kNonOrthoRhs)) return;
@@ -451,7 +451,12 @@ __device__ __forceinline__ void nonortho_rhs_grad(const DomainK<R> &dom, int bi,
         if (at_bound(pos, face, b) && prescribed(face, b)) {
             if (fixed_type(b, face, type, pos.a[3]) == PICT_BC_NEUMANN || type == GD_PRESSURE_RESULT) continue;
             R bA[D];
-            lap_neighbour_boundary<R, D>(&b->bounds[face], pos, axis, bA);
+            if constexpr (FACE) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) bA[c] = lap_single_face<R, D>(b, pos, face, c);
+            } else {
+                lap_neighbour_boundary<R, D>(&b->bounds[face], pos, axis, bA);
+            }
 #pragma unroll
             for (int i = 1; i < D; ++i) {
                 const int ta = (axis + i) % D;
@@ -483,7 +488,10 @@ __device__ __forceinline__ void nonortho_rhs_grad(const DomainK<R> &dom, int bi,
 #pragma unroll
         for (int i = 1; i < D; ++i) {
             const int ta = (axis + i) % D;
-            const R fa = (aP[lap_ci(axis, ta)] * rvP + aN[ta] * rvN) * R(0.5);
+            // useFaceTransform (:3294-3312, :3364-3368): the face's coefficient for both cells
+            const R fc = FACE ? lap_single_face<R, D>(b, pos, face, ta) : R(0);
+            const R cP = FACE ? fc : aP[lap_ci(axis, ta)], cN = FACE ? fc : aN[ta];
+            const R fa = (cP * rvP + cN * rvN) * R(0.5);
             const R tgg = -s_grad * (R)fs * fa;
             R tdg = 0;
 #pragma unroll
@@ -509,8 +517,8 @@ __device__ __forceinline__ void nonortho_rhs_grad(const DomainK<R> &dom, int bi,
             }
             if (diffable) {
                 const R fag = -s_grad * (R)fs * tdg;
-                raP_grad += aP[lap_ci(axis, ta)] * R(0.5) * fag;
-                raN_grad += aN[ta] * R(0.5) * fag;
+                raP_grad += cP * R(0.5) * fag;
+                raN_grad += cN * R(0.5) * fag;
             }
         }
         if (with_visc) scatter_viscosity<R>(dom, nb, n.pos, raN_grad * raN, b, u);
@@ -762,14 +770,15 @@ __global__ __launch_bounds__(kCellsPerTile) void k_p_matrix_grad(DomainK<R> dom,
 }
 
 // k_pressureRHSaddNonOrthoComponents_GRAD :5479-5494: pressure_rhs_div_grad -> pressure_result_grad (+=), A_grad (+=)
-template <typename R, int D>
+// (FACE: with useFaceTransform's face coefficients, which this VJP does use, :5491)
+template <typename R, int D, bool FACE = false>
 __global__ __launch_bounds__(kCellsPerTile) void k_p_rhs_nonortho_grad(DomainK<R> dom, int flags) {
     const pict_block_desc *b;
     int bi, f;
     if (!tile_cell(dom, b, bi, f)) return;
     UniformAcc<R> u;  // no viscosity targets here
     if constexpr (D > 1)
-        nonortho_rhs_grad<R, D>(dom, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, false, true,
+        nonortho_rhs_grad<R, D, FACE>(dom, bi, unflat(f, b), flags, GD_PRESSURE_RESULT, false, true,
                                 dom.prhs_div_grad[b->global_offset + f], u);
 }
 
@@ -992,12 +1001,9 @@ pict_status pict_setup_pressure_matrix_grad(const pict_atlas *a, double dt, int3
                                             void *stream) {
     (void)dt;
     PICT_CHECK_ARG(a && a->desc.P_grad && a->desc.A_grad && a->desc.A, "P_grad/A_grad missing");
-    // PISO_build_pressure_matrix_GRAD (:4966-5100) takes useFaceTransform but uses it only in the non-orthogonal
-    // terms: the orthogonal VJP is the cell-metric one with or without it
-    if (use_face_transform && a->desc.num_dims > 1 && (flags & kNonOrthoMatrix)) {
-        set_error("useFaceTransform with non-orthogonal matrix terms is not implemented by libpict_hip");
-        return PICT_ERR_UNSUPPORTED;
-    }
+    // PISO_build_pressure_matrix_GRAD (:4966-5115) takes useFaceTransform but passes false to its interpolation and
+    // scatter of the non-orthogonal coefficients (:5043-5044, :5104-5105): the cell-metric VJP either way
+    (void)use_face_transform;
     if (a->desc.num_dims > 1 && (flags & kNonOrthoMatrix))
         PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_matrix_grad<R, D, true>), a->view<R>(), (int)flags));
     else
@@ -1009,16 +1015,21 @@ pict_status pict_setup_pressure_rhs_div_grad(const pict_atlas *a, double dt, int
                                              int32_t tsn, void *stream) {
     PICT_CHECK_ARG(a && a->desc.pressure_rhs_div_grad && a->desc.pressure_rhs_grad, "pressure rhs grads missing");
     // k_computePressureRHSdivergenceFromFlux_GRAD (:5422-5452) scatters through the cell-metric fluxes whatever
-    // useFaceTransform says; only the non-orthogonal terms would use the face transforms
-    if (use_face_transform && a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
-        set_error("useFaceTransform with non-orthogonal right-hand-side terms is not implemented by libpict_hip");
-        return PICT_ERR_UNSUPPORTED;
-    }
+    // useFaceTransform says; the non-orthogonal terms' VJP uses the face coefficients (:5491)
     if (a->desc.num_dims > 1 && (flags & kNonOrthoRhs)) {
         // k_pressureRHSaddNonOrthoComponents_GRAD first (_SetupPressureRHS_GRAD :5792-5803)
         PICT_CHECK_ARG(a->desc.pressure_result && a->desc.pressure_result_grad && a->desc.A && a->desc.A_grad,
                        "pressure_result / its gradient or A_grad missing (non-orthogonal RHS terms)");
-        PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs_nonortho_grad<R, D>), a->view<R>(), (int)flags));
+        if (use_face_transform) {
+            for (int i = 0; i < a->desc.num_blocks; ++i)
+                PICT_CHECK_ARG(!a->host_blocks[i].has_transform ||
+                                   (a->host_blocks[i].has_face_transform && a->host_blocks[i].face_transform),
+                               "useFaceTransform with non-orthogonal terms: block %d has no face transforms", i);
+            PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs_nonortho_grad<R, D, true>), a->view<R>(),
+                                               (int)flags));
+        } else {
+            PICT_DISPATCH(a, PICT_LAUNCH_CELLS(a, stream, (k_p_rhs_nonortho_grad<R, D>), a->view<R>(), (int)flags));
+        }
         pict_status st = check_launch();
         if (st) return st;
     }

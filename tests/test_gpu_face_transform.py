@@ -57,8 +57,8 @@ def test_face_transform_pressure_setup_matches_the_oracle(name, dtype):
 @pytest.mark.parametrize("name", list(scenarios.NON_ORTHO))
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_non_orthogonal_face_terms_match_the_oracle(name, dtype):
-    """The non-orthogonal pressure terms with face coefficients (:1962-1972, :3128-3146; flags 25) against the
-    oracle; their VJP is refused."""
+    """The non-orthogonal pressure terms with face coefficients (:1962-1972, :3128-3146; flags 25) and the VJP of
+    the explicit RHS terms with them (:3294-3312, :3364-3368) against the oracle."""
     print("case", name, dtype, flush=True)
     g = scenarios.NON_ORTHO[name](dtype=dtype, device="cuda")
     c = O.cpu_twin(g)
@@ -81,11 +81,22 @@ def test_non_orthogonal_face_terms_match_the_oracle(name, dtype):
     P.SetupPressureRHSdiv(g, tdt, 25, True)
     O.SetupPressureRHSdiv(c, dt, 25, True)
     assert_close(g.pressureRHSdiv, c.pressureRHSdiv, 10 * tol, "divergence with non-orthogonal face terms")
-    from grad_util import create_grads
+    # the VJP of the non-orthogonal RHS terms with face coefficients, against the oracle's (pinned by finite
+    # differences); the matrix VJP is the cell-metric one, as in the reference
+    from grad_util import create_grads, input_tensors
     create_grads(g)
+    create_grads(c)
+    ybar = _rand(g.pressureRHSdiv.shape, 31)
+    g.setPressureRHSdivGrad(ybar.to(dtype=dtype, device="cuda"))
+    c.setPressureRHSdivGrad(ybar.to(dtype=dtype).double())
     g.UpdateDomainData()
-    with pytest.raises(P.PictUnsupportedError):
-        P.SetupPressureMatrixGrad(g, tdt, 25, True)
+    c._set_changed(False)
+    P.SetupPressureRHSdivGrad(g, tdt, 25, True)
+    O.SetupPressureRHSdivGrad(c, dt, 25, True)
+    torch.cuda.synchronize()
+    for inp in ("PRESSURE_RESULT", "A", "PRESSURE_RHS"):
+        for (_, gg), (_, gc) in zip(input_tensors(g, inp), input_tensors(c, inp)):
+            assert_close(gg(), gc(), tol if dtype == torch.float32 else 10 * tol, "face VJP of %s" % inp)
 
 
 @pytest.mark.parametrize("name", ["rotated_connection", "box3d", "sheared_two_block_rotated"])

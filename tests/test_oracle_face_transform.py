@@ -169,3 +169,49 @@ def test_face_coefficients_without_face_transforms_are_refused():
     with pytest.raises(RuntimeError):
         O.SetupPressureMatrix(d, 0.1, 25, True)
     O.SetupPressureMatrix(d, 0.1, 0, True)  # orthogonal terms: coefficient 1 without face transforms
+
+
+def test_non_orthogonal_face_terms_vjp_matches_finite_differences():
+    """The VJP of the explicit non-orthogonal pressure terms with face coefficients (scatterNonOrthoLaplaceRHS_v2_GRAD
+    with useFaceTransform, :3294-3312, :3364-3368) against central differences of the forward terms, in the pressure
+    result and in A, on a warped grid."""
+    from grad_util import create_grads, input_tensors
+    d = _domain(2)
+    b = scenarios._coords_block(d, _warped((6, 5), (0.3,)))
+    b.CloseAllBoundaries()
+    b.MakePeriodic("x")
+    _prep(d)
+    g = torch.Generator().manual_seed(21)
+    N = d.getTotalSize()
+    A = torch.rand(N, generator=g, dtype=torch.float64) + 0.5
+    pr = torch.randn(N, generator=g, dtype=torch.float64)
+    d.setA(A.clone())
+    d.setPressureRHS(torch.randn(2 * N, generator=g, dtype=torch.float64))
+    d.setPressureResult(pr.clone())
+    d.UpdateDomainData()
+    ybar = torch.randn(N, generator=g, dtype=torch.float64)
+
+    def F(A_, pr_):
+        d.setA(A_.clone())
+        d.setPressureResult(pr_.clone())
+        d.UpdateDomainData()
+        O.SetupPressureRHSdiv(d, 0.1, 25, True)
+        return float((d.pressureRHSdiv * ybar).sum())
+
+    F(A, pr)
+    create_grads(d)
+    d.setPressureRHSdivGrad(ybar.clone())
+    d.UpdateDomainData()
+    O.SetupPressureRHSdivGrad(d, 0.1, 25, True)
+    (_, ga), = input_tensors(d, "A")
+    (_, gp), = input_tensors(d, "PRESSURE_RESULT")
+    ga, gp = ga().clone(), gp().clone()
+    assert float(gp.abs().max()) > 0 and float(ga.abs().max()) > 0
+    eps = 1e-6
+    for k in range(0, N, 3):
+        e = torch.zeros(N, dtype=torch.float64)
+        e[k] = eps
+        fd_p = (F(A, pr + e) - F(A, pr - e)) / (2 * eps)
+        fd_a = (F(A + e, pr) - F(A - e, pr)) / (2 * eps)
+        np.testing.assert_allclose(float(gp[k]), fd_p, rtol=1e-6, atol=1e-8)
+        np.testing.assert_allclose(float(ga[k]), fd_a, rtol=1e-6, atol=1e-8)
