@@ -200,7 +200,9 @@ pict_status pict_setup_advection_matrix(const pict_atlas *atlas, double time_ste
 pict_status pict_setup_advection_velocity(const pict_atlas *atlas, double time_step,
                                           int32_t non_ortho_flags,
                                           int32_t apply_pressure_gradient, void *stream);
-/* SetupPressureMatrix (PISOtorch.cpp:516-518; kernel :4797-4963): P values. */
+/* SetupPressureMatrix (PISOtorch.cpp:516-518; kernel :4797-4963): P values.  use_face_transform: the coefficients
+   from the faces' transforms (:4857-4865, non-orthogonal :1962-1972; a transformed block needs face transforms for
+   the non-orthogonal terms, else PICT_ERR_INVALID). */
 pict_status pict_setup_pressure_matrix(const pict_atlas *atlas, double time_step,
                                        int32_t non_ortho_flags, int32_t use_face_transform,
                                        void *stream);
@@ -209,7 +211,9 @@ pict_status pict_setup_pressure_matrix(const pict_atlas *atlas, double time_step
 pict_status pict_setup_pressure_rhs(const pict_atlas *atlas, double time_step,
                                     int32_t non_ortho_flags, int32_t use_face_transform,
                                     int32_t time_step_norm, void *stream);
-/* SetupPressureRHSdiv (PISOtorch.cpp:522-524; kernel :5374-5419): pressure_rhs_div. */
+/* SetupPressureRHSdiv (PISOtorch.cpp:522-524; kernel :5374-5419): pressure_rhs_div.  use_face_transform: fluxes
+   from the faces' transforms (computeFluxesWithFaceTransforms :1703-1785; every block needs face transforms, else
+   PICT_ERR_INVALID) and face coefficients in the non-orthogonal terms (:3128-3146). */
 pict_status pict_setup_pressure_rhs_div(const pict_atlas *atlas, double time_step,
                                         int32_t non_ortho_flags, int32_t use_face_transform,
                                         int32_t time_step_norm, void *stream);
